@@ -1,0 +1,187 @@
+/*
+ * seidart_hip.h -- C-ABI of libseidart_hip.so, the gfx950 HIP implementation of
+ * SeidarT-Backend's anisotropic CPML finite-difference time-domain solvers.
+ *
+ * The reference has no C-ABI: each solver is a Fortran subroutine taking
+ * derived types and reading its inputs from files in the working directory
+ * (src/fortran/cpmlfdtd.f08:57 seismic2, :1341 seismic3, :2317 electromag2,
+ * :3353 electromag3).  The entry points below replace that contract one for
+ * one:
+ *
+ *   Domain_Type / Source_Type (src/fortran/seidart_types.f08:13-33)
+ *       -> sdt_params, passed to sdt_create
+ *   material_rw2 / material_rw3 reads of c11.dat ... bcoef_half_cpml.dat,
+ *   initialcondition*.dat (src/fortran/seidartio.f08:225-292, used at
+ *   cpmlfdtd.f08:159-174,253-266,1501-1645,2413-2506,3481-3618)
+ *       -> sdt_set_array
+ *   loadsource of seismicsource*.dat / electromagneticsource*.dat
+ *   (src/fortran/seidartio.f08:161-183, used at cpmlfdtd.f08:230-237,
+ *   1606-1617, 2476-2479, 3581-3585)
+ *       -> sdt_set_source
+ *   the `do it = 1, time_steps` loops (cpmlfdtd.f08:311, 1747, 2566, 3689)
+ *       -> sdt_run
+ *   write_image2 / write_image3 snapshots (src/fortran/seidartio.f08:366-426)
+ *       -> sdt_get_field (one call per step and component)
+ *   write_array of velocity_norm.dat / stress_norm.dat (cpmlfdtd.f08:2282-2283)
+ *       -> sdt_get_norms
+ *   `stop 'code became unstable and blew up'` (cpmlfdtd.f08:567, 2263, 2740,
+ *   3972) -> SDT_ERR_UNSTABLE returned with the step index
+ *
+ * Receiver recording (sdt_set_receivers / sdt_get_receivers) has no reference
+ * counterpart: the SeidarT frontend samples the snapshot files instead; the
+ * values recorded equal the snapshot values at the receiver cells.
+ *
+ * Conventions: arrays are Fortran order (x fastest, then y, then z), padded
+ * sizes (nx = n_json + 2*cpml, src/fortran/main.f08:69-71), FP64 on the host
+ * side.  2D solvers use the (x, z) plane with ny = 1.  Source and receiver
+ * indices are 1-based and unpadded, as in the JSON (the solver adds cpml,
+ * cpmlfdtd.f08:178-179, 1533-1535).  Every pointer is borrowed for the
+ * duration of the call.  No entry point exits the process.  One HIP stream per
+ * handle; handles are independent, a single handle is not thread-safe.
+ */
+#ifndef SEIDART_HIP_H
+#define SEIDART_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDT_ABI_VERSION 1
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    SDT_OK = 0,
+    SDT_ERR_UNSTABLE = 1, /* max velocity/E norm exceeded 1e25
+                             (constants.f08:12, cpmlfdtd.f08:567)       */
+    SDT_ERR_ARG = 2,      /* bad argument                                */
+    SDT_ERR_HIP = 3,      /* HIP runtime error                           */
+    SDT_ERR_STATE = 4     /* call out of order (e.g. run before inputs)  */
+};
+
+/* ---- solver kinds (src/fortran/main.f08:90-119 dispatch) ---------------- */
+enum {
+    SDT_ELASTIC2 = 0, /* seismic2     cpmlfdtd.f08:57   */
+    SDT_ELASTIC3 = 2, /* seismic3     cpmlfdtd.f08:1341 */
+    SDT_EM2 = 3,      /* electromag2  cpmlfdtd.f08:2317 */
+    SDT_EM3 = 5       /* electromag3  cpmlfdtd.f08:3353 */
+};
+
+enum { SDT_FP64 = 0, SDT_FP32 = 1 };
+
+/* density averaging codes, identical to scalar_mean (averaging.f08:5-25) */
+enum {
+    SDT_DENSITY_HARMONIC = 1,
+    SDT_DENSITY_GEOMETRIC = 2,
+    SDT_DENSITY_ARITHMETIC = 3,
+    SDT_DENSITY_NONE = 4
+};
+
+/* velocity-from-stress stencil: the reference's 3-term form
+ * (cpmlfdtd.f08:405-406, 434-435, 2022-2024, 2063-2066, 2107-2111), or the
+ * consistent 4-term staggered form (opt-in, not the reference's output). */
+enum { SDT_STENCIL_REFERENCE = 0, SDT_STENCIL_CORRECTED = 1 };
+
+/* ---- input array identifiers: one per reference input file ------------- */
+enum {
+    SDT_C11 = 0, SDT_C12, SDT_C13, SDT_C14, SDT_C15, SDT_C16,
+    SDT_C22, SDT_C23, SDT_C24, SDT_C25, SDT_C26,
+    SDT_C33, SDT_C34, SDT_C35, SDT_C36,
+    SDT_C44, SDT_C45, SDT_C46,
+    SDT_C55, SDT_C56,
+    SDT_C66,                                   /* 0..20  c??.dat        */
+    SDT_RHO = 21,                              /* rho.dat               */
+    SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z,
+    SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ,  /* 22..27 gamma_*.dat    */
+    SDT_KAPPA = 28, SDT_ALPHA, SDT_ACOEF, SDT_BCOEF,             /* *_cpml.dat      */
+    SDT_KAPPA_HALF, SDT_ALPHA_HALF, SDT_ACOEF_HALF, SDT_BCOEF_HALF, /* *_half_cpml.dat */
+    SDT_E11 = 36, SDT_E12, SDT_E13, SDT_E22, SDT_E23, SDT_E33,   /* e??.dat (relative) */
+    SDT_S11 = 42, SDT_S12, SDT_S13, SDT_S22, SDT_S23, SDT_S33,   /* s??.dat (S/m)      */
+    SDT_IC_X = 48, SDT_IC_Y, SDT_IC_Z,  /* initialconditionV?.dat / initialconditionE?.dat */
+    SDT_NUM_ARRAYS = 51
+};
+
+/* ---- source series identifiers ------------------------------------------ */
+enum {
+    SDT_SRC_X = 0, SDT_SRC_Y, SDT_SRC_Z,             /* seismicsource?.dat, electromagneticsource?.dat */
+    SDT_SRC_XX, SDT_SRC_XY, SDT_SRC_XZ,
+    SDT_SRC_YY, SDT_SRC_YZ, SDT_SRC_ZZ,              /* seismicsource??.dat                            */
+    SDT_NUM_SRC = 9
+};
+
+/* ---- field identifiers (state arrays) ------------------------------------ */
+enum {
+    SDT_F_VX = 0, SDT_F_VY, SDT_F_VZ,
+    SDT_F_SXX, SDT_F_SYY, SDT_F_SZZ, SDT_F_SXY, SDT_F_SXZ, SDT_F_SYZ,
+    SDT_F_EX = 0, SDT_F_EY = 1, SDT_F_EZ = 2,
+    SDT_F_HX = 3, SDT_F_HY = 4, SDT_F_HZ = 5,
+    SDT_NUM_FIELDS = 9
+};
+
+typedef struct sdt_params {
+    int32_t abi_version;   /* must be SDT_ABI_VERSION                       */
+    int32_t physics;       /* SDT_ELASTIC2 / SDT_ELASTIC3 / SDT_EM2 / SDT_EM3 */
+    int32_t precision;     /* SDT_FP64 or SDT_FP32                          */
+    int32_t nx, ny, nz;    /* padded sizes; ny = 1 for 2D                   */
+    double dx, dy, dz;     /* Domain_Type%dx..dz                            */
+    int32_t cpml;          /* Domain_Type%cpml                              */
+    double dt;             /* Source_Type%dt                                */
+    int32_t time_steps;    /* Source_Type%time_steps (length of the series) */
+    int32_t xind, yind, zind; /* Source_Type%xind.. (1-based, unpadded)     */
+    int32_t density_code;  /* SDT_DENSITY_*, elastic only                   */
+    int32_t stencil;       /* SDT_STENCIL_*                                 */
+    int32_t device;        /* HIP device ordinal                            */
+    int32_t reserved[8];   /* zero                                          */
+} sdt_params;
+
+typedef struct sdt_handle sdt_handle;
+
+/* library-level queries */
+int32_t sdt_version(void);
+int32_t sdt_device_count(int32_t *count);
+/* last error message of a handle, or of the calling thread when h is NULL */
+const char *sdt_last_error(const sdt_handle *h);
+
+/* lifetime */
+int32_t sdt_create(const sdt_params *p, sdt_handle **out);
+int32_t sdt_destroy(sdt_handle *h);
+
+/* inputs: count must equal nx*ny*nz (arrays) or time_steps (series).
+ * Arrays not set take the reference's defaults (kappa = 1, everything else 0,
+ * cpmlfdtd.f08:240-247). */
+int32_t sdt_set_array(sdt_handle *h, int32_t array_id, const double *host, int64_t count);
+int32_t sdt_set_source(sdt_handle *h, int32_t component, const double *series, int64_t count);
+/* n receivers, ijk = n triples of 1-based unpadded (x, y, z) indices (y ignored in 2D);
+ * each receiver records the three velocity / E components every step */
+int32_t sdt_set_receivers(sdt_handle *h, int32_t n, const int32_t *ijk);
+
+/* advance nsteps time steps (continues from the current step).  On
+ * SDT_ERR_UNSTABLE, *failed_step (if non-NULL) receives the 1-based step. */
+int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step);
+/* steps completed so far */
+int32_t sdt_steps_done(const sdt_handle *h, int32_t *steps);
+
+/* outputs (FP64 host copies; blocking) */
+int32_t sdt_get_field(sdt_handle *h, int32_t field_id, double *out, int64_t count);
+int32_t sdt_set_field(sdt_handle *h, int32_t field_id, const double *host, int64_t count);
+/* out = steps_done * n_receivers * 3 doubles, step-major */
+int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count);
+/* per-step max norms: velocity (or E) and, for 3D elastic, stress; NULL skips */
+int32_t sdt_get_norms(sdt_handle *h, double *vel, double *stress, int64_t count);
+
+/* kernel timing with HIP events on the handle's stream.  When enabled, each
+ * step records events around the stress (H) and velocity (E) kernels;
+ * sdt_get_kernel_times returns the summed milliseconds and the number of
+ * launches timed since enabling. */
+int32_t sdt_set_timing(sdt_handle *h, int32_t enable);
+int32_t sdt_get_kernel_times(sdt_handle *h, double *ms_stress, double *ms_velocity,
+                             int64_t *launches);
+/* block until all work queued on the handle's stream has finished */
+int32_t sdt_synchronize(sdt_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SEIDART_HIP_H */

@@ -1,0 +1,160 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker used by tests/, __graft_entry__.smoke()
+and bench.py's CPU-baseline leg.  It mirrors seidart_hip.solver.run_config so
+a configuration dict runs unchanged through either.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+NUM_ARRAYS, NUM_SRC, NUM_FIELDS = 51, 9, 9
+
+_C_NAMES = ["c11", "c12", "c13", "c14", "c15", "c16", "c22", "c23", "c24", "c25", "c26",
+            "c33", "c34", "c35", "c36", "c44", "c45", "c46", "c55", "c56", "c66"]
+ARRAY_IDS = {n: q for q, n in enumerate(_C_NAMES)}
+ARRAY_IDS.update({
+    "rho": 21, "gamma_x": 22, "gamma_y": 23, "gamma_z": 24, "gamma_xy": 25, "gamma_xz": 26,
+    "gamma_yz": 27, "kappa_cpml": 28, "alpha_cpml": 29, "acoef_cpml": 30, "bcoef_cpml": 31,
+    "kappa_half_cpml": 32, "alpha_half_cpml": 33, "acoef_half_cpml": 34, "bcoef_half_cpml": 35,
+    "e11": 36, "e12": 37, "e13": 38, "e22": 39, "e23": 40, "e33": 41,
+    "s11": 42, "s12": 43, "s13": 44, "s22": 45, "s23": 46, "s33": 47,
+    "ic_x": 48, "ic_y": 49, "ic_z": 50,
+})
+SOURCE_IDS = {"x": 0, "y": 1, "z": 2, "xx": 3, "xy": 4, "xz": 5, "yy": 6, "yz": 7, "zz": 8}
+FIELD_IDS_ELASTIC = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7,
+                     "syz": 8}
+FIELD_IDS_EM = {"ex": 0, "ey": 1, "ez": 2, "hx": 3, "hy": 4, "hz": 5}
+DENSITY_CODES = {"harmonic": 1, "geometric": 2, "arithmetic": 3, "none": 4}
+
+_D = C.POINTER(C.c_double)
+
+
+class OracleRun(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("nsteps", C.c_int32), ("isrc", C.c_int32), ("jsrc", C.c_int32), ("ksrc", C.c_int32),
+        ("density_code", C.c_int32), ("stencil", C.c_int32),
+        ("arr", _D * NUM_ARRAYS), ("src", _D * NUM_SRC),
+        ("nrec", C.c_int32), ("rec", C.POINTER(C.c_int32)), ("rec_out", _D),
+        ("field_in", _D * NUM_FIELDS), ("field_out", _D * NUM_FIELDS),
+        ("velnorm", _D), ("stressnorm", _D), ("unstable_step", C.c_int32), ("nthreads", C.c_int32),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    for name in ("oracle_seismic2", "oracle_seismic3", "oracle_electromag2", "oracle_electromag3"):
+        fn = getattr(lib, name)
+        fn.argtypes = [C.POINTER(OracleRun)]
+        fn.restype = C.c_int32
+    lib.oracle_scalar_mean.argtypes = [C.c_double, C.c_double, C.c_int32]
+    lib.oracle_scalar_mean.restype = C.c_double
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def scalar_mean(a, b, code):
+    return lib().oracle_scalar_mean(float(a), float(b), int(code))
+
+
+_ENTRY = {"elastic2": "oracle_seismic2", "elastic3": "oracle_seismic3", "em2": "oracle_electromag2",
+          "em3": "oracle_electromag3"}
+
+
+def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
+    """Run a configuration dict through the oracle.
+
+    Returns (receivers[steps, nrec, 3], (velnorm, stressnorm), {field: array},
+    unstable_step)."""
+    phys = cfg["physics"]
+    is3d = phys.endswith("3")
+    shape = tuple(cfg["shape"])
+    nsteps = int(steps if steps is not None else cfg["time_steps"])
+    r = OracleRun()
+    if is3d:
+        r.nx, r.ny, r.nz = shape
+        r.dx, r.dy, r.dz = cfg["spacing"]
+    else:
+        r.nx, r.nz = shape
+        r.ny = 1
+        r.dx, r.dz = cfg["spacing"]
+        r.dy = 1.0
+    r.dt = cfg["dt"]
+    r.nsteps = nsteps
+    cp = cfg["cpml"]
+    src = cfg["src"]
+    if is3d:
+        r.isrc, r.jsrc, r.ksrc = src[0] + cp, src[1] + cp, src[2] + cp
+    else:
+        r.isrc, r.jsrc, r.ksrc = src[0] + cp, 1, src[1] + cp
+    r.density_code = DENSITY_CODES[cfg.get("density_method", "arithmetic")]
+    r.stencil = 1 if cfg.get("stencil", "reference") == "corrected" else 0
+    r.nthreads = nthreads
+    keep = []
+
+    def ptr(a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+        keep.append(a)
+        return a.ctypes.data_as(_D)
+
+    for name, a in cfg["arrays"].items():
+        r.arr[ARRAY_IDS[name]] = ptr(np.asarray(a).reshape(shape).ravel(order="F"))
+    for comp, s in cfg["sources"].items():
+        s = np.asarray(s, dtype=np.float64)
+        if s.size < nsteps:
+            raise ValueError("source series shorter than the run")
+        r.src[SOURCE_IDS[comp]] = ptr(s[:nsteps])
+    recs = np.asarray(cfg.get("receivers", np.zeros((0, 3))), dtype=np.int32).reshape(-1, 3)
+    padded = recs.copy()
+    padded[:, 0] += cp
+    padded[:, 2] += cp
+    if is3d:
+        padded[:, 1] += cp
+    else:
+        padded[:, 1] = 1
+    padded = np.ascontiguousarray(padded)
+    keep.append(padded)
+    r.nrec = padded.shape[0]
+    r.rec = padded.ctypes.data_as(C.POINTER(C.c_int32))
+    rec_out = np.zeros((nsteps, padded.shape[0], 3))
+    r.rec_out = rec_out.ctypes.data_as(_D)
+    fid = FIELD_IDS_ELASTIC if phys.startswith("elastic") else FIELD_IDS_EM
+    if field_in:
+        for name, a in field_in.items():
+            r.field_in[fid[name]] = ptr(np.asarray(a).reshape(shape).ravel(order="F"))
+    outs = {}
+    for name in fields:
+        o = np.zeros(int(np.prod(shape)))
+        outs[name] = o
+        r.field_out[fid[name]] = o.ctypes.data_as(_D)
+    vn = np.zeros(nsteps)
+    sn = np.zeros(nsteps)
+    r.velnorm = vn.ctypes.data_as(_D)
+    r.stressnorm = sn.ctypes.data_as(_D)
+    getattr(lib(), _ENTRY[phys])(C.byref(r))
+    fields_out = {k: v.reshape(shape, order="F") for k, v in outs.items()}
+    return rec_out, (vn, sn), fields_out, r.unstable_step

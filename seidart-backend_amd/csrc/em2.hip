@@ -1,0 +1,226 @@
+// em2.hip -- 2D anisotropic electromagnetic CPML-FDTD (electromag2,
+// cpmlfdtd.f08:2317-2760) on gfx950: Hy from (Ex, Ez), then (Ex, Ez) from Hy,
+// 2nd-order Yee differences, full 2x2 permittivity and conductivity tensors.
+//
+// E_old == E at the start of every step (the reference copies E into E_old
+// after the Dirichlet step, 2716-2721), and each E cell reads only its own
+// E_old, so the E update runs in place without the E_old arrays.
+//
+// Dirichlet faces of Hy (2692-2711) are applied lazily: the H kernel treats a
+// face cell's previous Hy as 0 and stores the fresh value, which the E kernel
+// of the same step reads exactly as the reference does (its zeroing follows
+// the E update); field read-back masks the faces.
+#include <math.h>
+
+#include <vector>
+
+#include "sdt_device.h"
+#include "sdt_internal.h"
+
+namespace sdt {
+
+template <typename T>
+struct Em2Args {
+    T *Ex, *Ez, *Hy;
+    const T *aEx, *bEx, *bEz, *dEx, *eEx, *dEz, *eEz;   // aEz == bEx
+    const T *a, *b, *ik, *ah, *bh, *ikh;
+    T *m_dEx_dz, *m_dEz_dx, *m_dHy_dx, *m_dHy_dz;
+    const uint8_t *flags;
+    int nx, nz, tiles_x;
+    T rdx, rdz, dt, dbHy;
+    const int *d_it;
+    unsigned long long *vnorm;
+    int64_t src_cell;
+};
+
+template <typename T, bool PML>
+__device__ __forceinline__ void em2_h_cell(const Em2Args<T> &A, int i, int j)
+{
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = (int64_t)(i - 1) + (int64_t)nx * (j - 1);
+    if (i == nx || j == nz) { A.Hy[c] = T(0); return; }
+    // cpmlfdtd.f08:2576-2593
+    T dEx_dz = (A.Ex[c + nx] - A.Ex[c]) * A.rdz;
+    T dEz_dx = (A.Ez[c + 1] - A.Ez[c]) * A.rdx;
+    if (PML) {
+        A.m_dEx_dz[c] = A.b[c] * A.m_dEx_dz[c] + A.a[c] * dEx_dz;
+        A.m_dEz_dx[c] = A.bh[c] * A.m_dEz_dx[c] + A.ah[c] * dEz_dx;
+        dEz_dx = dEz_dx * A.ikh[c] + A.m_dEz_dx[c];
+        dEx_dz = dEx_dz * A.ik[c] + A.m_dEx_dz[c];
+    }
+    const T hold = (i == 1 || j == 1) ? T(0) : A.Hy[c];
+    A.Hy[c] = hold + A.dbHy * (dEz_dx - dEx_dz);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) em2_h_kernel(Em2Args<T> A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    if (i > A.nx || j > A.nz) return;
+    if (A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) em2_h_cell<T, true>(A, i, j);
+    else em2_h_cell<T, false>(A, i, j);
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
+{
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = (int64_t)(i - 1) + (int64_t)nx * (j - 1);
+    T ex, ez;
+    if (i == 1 || i == nx || j == 1 || j == nz) {
+        ex = ez = T(0);
+    } else {
+        // cpmlfdtd.f08:2605-2620
+        T dHy_dz = (A.Hy[c] - A.Hy[c - nx]) * A.rdz;
+        T dHy_dx = (A.Hy[c] - A.Hy[c - 1]) * A.rdx;
+        if (PML) {
+            A.m_dHy_dx[c] = A.bh[c] * A.m_dHy_dx[c] + A.ah[c] * dHy_dx;
+            A.m_dHy_dz[c] = A.b[c] * A.m_dHy_dz[c] + A.a[c] * dHy_dz;
+            dHy_dz = dHy_dz * A.ik[c] + A.m_dHy_dz[c];
+            dHy_dx = dHy_dx * A.ikh[c] + A.m_dHy_dx[c];
+        }
+        const T exo = A.Ex[c], ezo = A.Ez[c];
+        const T bEx = A.bEx[c];
+        ez = ezo + (-bEx * dHy_dz + A.bEz[c] * dHy_dx - A.dEz[c] * exo - A.eEz[c] * ezo) * A.dt;
+        ex = exo + (-A.aEx[c] * dHy_dz + bEx * dHy_dx - A.dEx[c] * exo - A.eEx[c] * ezo) * A.dt;
+    }
+    A.Ex[c] = ex;
+    A.Ez[c] = ez;
+    if (c == A.src_cell) return 0.0;
+    const double x = ex, z = ez;
+    return sqrt(x * x + z * z);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) em2_e_kernel(Em2Args<T> A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    double nrm = 0.0;
+    if (i <= A.nx && j <= A.nz) {
+        if (A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = em2_e_cell<T, true>(A, i, j);
+        else nrm = em2_e_cell<T, false>(A, i, j);
+    }
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T>
+static Em2Args<T> em2_args(Device &d)
+{
+    Em2Args<T> A;
+    A.Ex = (T *)d.field[SDT_F_EX];
+    A.Ez = (T *)d.field[SDT_F_EZ];
+    A.Hy = (T *)d.field[SDT_F_HY];
+    const T **mp[] = {&A.aEx, &A.bEx, &A.bEz, &A.dEx, &A.eEx, &A.dEz, &A.eEz};
+    for (int q = 0; q < 7; ++q) *mp[q] = (const T *)d.mat[q];
+    A.a = (const T *)d.cpml[0];
+    A.b = (const T *)d.cpml[1];
+    A.ik = (const T *)d.cpml[2];
+    A.ah = (const T *)d.cpml[3];
+    A.bh = (const T *)d.cpml[4];
+    A.ikh = (const T *)d.cpml[5];
+    A.m_dEx_dz = (T *)d.mem[0];
+    A.m_dEz_dx = (T *)d.mem[1];
+    A.m_dHy_dx = (T *)d.mem[2];
+    A.m_dHy_dz = (T *)d.mem[3];
+    A.flags = d.tile_flags;
+    A.nx = d.nx;
+    A.nz = d.nz;
+    A.tiles_x = d.tiles_x;
+    A.rdx = (T)(1.0 / d.dx);
+    A.rdz = (T)(1.0 / d.dz);
+    A.dt = (T)d.dt;
+    A.dbHy = (T)(d.dt / kMu0);   // cpmlfdtd.f08:2516
+    A.d_it = d.d_it;
+    A.vnorm = d.norms;
+    A.src_cell = d.tail.vfix[0];
+    return A;
+}
+
+template <typename T>
+static void em2_step_t(Device &d, bool timed)
+{
+    const Em2Args<T> A = em2_args<T>(d);
+    const dim3 grid(d.tiles_x, d.tiles_z), block(64, 4);
+    if (timed) (void)hipEventRecord(d.ev[0], d.stream);
+    hipLaunchKernelGGL(em2_h_kernel<T>, grid, block, 0, d.stream, A);
+    if (timed) (void)hipEventRecord(d.ev[1], d.stream);
+    hipLaunchKernelGGL(em2_e_kernel<T>, grid, block, 0, d.stream, A);
+    if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_tail(d);
+}
+
+void em2_step(Device &d, bool timed)
+{
+    if (d.precision == SDT_FP64) em2_step_t<double>(d, timed);
+    else em2_step_t<float>(d, timed);
+}
+
+// Coefficients of cpmlfdtd.f08:2519-2540 (permittivity scaled by eps0,
+// determinant clamped at 1e-18*eps0^2), computed in FP64 on the host and
+// stored as aEx, bEx (= aEz), bEz, dEx, eEx, dEz, eEz.
+int em2_setup(Device &d, const double *const *in)
+{
+    const int64_t n = d.ncell;
+    std::vector<double> co[7];
+    for (auto &v : co) v.resize((size_t)n);
+    auto val = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
+    for (int64_t c = 0; c < n; ++c) {
+        const double e11 = val(SDT_E11, c) * kEps0, e13 = val(SDT_E13, c) * kEps0, e33 = val(SDT_E33, c) * kEps0;
+        const double s11 = val(SDT_S11, c), s13 = val(SDT_S13, c), s33 = val(SDT_S33, c);
+        double det = e11 * e33 - e13 * e13;
+        if (det <= 1.0e-18 * kEps0 * kEps0) det = 1.0e-18 * kEps0 * kEps0;
+        co[0][(size_t)c] = e33 / det;
+        co[1][(size_t)c] = -e13 / det;
+        co[2][(size_t)c] = e11 / det;
+        co[3][(size_t)c] = (e33 * s11 - e13 * s13) / det;
+        co[4][(size_t)c] = (e33 * s13 - e13 * s33) / det;
+        co[5][(size_t)c] = (e11 * s13 - e13 * s11) / det;
+        co[6][(size_t)c] = (e11 * s33 - e13 * s13) / det;
+    }
+    d.nmat = 0;
+    for (int q = 0; q < 7; ++q) {
+        int rc = upload_converted(d, co[q].data(), &d.mat[d.nmat++], n);
+        if (rc) return rc;
+    }
+    d.nmem = 4;
+    for (int q = 0; q < d.nmem; ++q) {
+        int rc = alloc_zeroed(d, &d.mem[q], n);
+        if (rc) return rc;
+    }
+    const int fids[] = {SDT_F_EX, SDT_F_EZ, SDT_F_HY};
+    for (int f : fids) {
+        int rc;
+        if (f == SDT_F_EX && in[SDT_IC_X]) rc = upload_converted(d, in[SDT_IC_X], &d.field[f], n);
+        else if (f == SDT_F_EZ && in[SDT_IC_Z]) rc = upload_converted(d, in[SDT_IC_Z], &d.field[f], n);
+        else rc = alloc_zeroed(d, &d.field[f], n);
+        if (rc) return rc;
+    }
+
+    // point source, cpmlfdtd.f08:2678-2681
+    const int is = d.isrc, js = d.ksrc, nx = d.nx, nz = d.nz;
+    const int64_t c = (int64_t)(is - 1) + (int64_t)nx * (js - 1);
+    const int face = (is == 1 || is == nx || js == 1 || js == nz);
+    TailArgs &t = d.tail;
+    t.nterms = 0;
+    t.terms[t.nterms++] = {c, SDT_F_EX, SDT_SRC_X, d.dt, val(SDT_E11, c) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EZ, SDT_SRC_Z, d.dt, val(SDT_E33, c) * kEps0, face};
+    t.nvfix = 1;
+    t.vfix[0] = c;
+    t.nvcomp = 2;
+    t.vcomp[0] = SDT_F_EX;
+    t.vcomp[1] = SDT_F_EZ;
+    t.stress_norm = 0;
+    t.nsfix = 0;
+    t.rec_field[0] = SDT_F_EX;
+    t.rec_field[1] = -1;
+    t.rec_field[2] = SDT_F_EZ;
+
+    d.tile_x = 64;
+    d.tile_y = 1;
+    d.tile_z = 4;
+    return compute_tile_flags(d, in);
+}
+
+}  // namespace sdt

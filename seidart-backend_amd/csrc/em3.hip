@@ -1,0 +1,289 @@
+// em3.hip -- 3D anisotropic electromagnetic CPML-FDTD (electromag3,
+// cpmlfdtd.f08:3353-4001) on gfx950: (Hx, Hy, Hz) from E, then E from H with
+// the full symmetric inverse permittivity and the conductivity tensor applied
+// to E_old.  2nd-order Yee differences.
+//
+// As in em2.hip: E_old == E at each step start, so the E update runs in
+// place; the H faces (3921-3961) are zeroed lazily (previous H taken as 0 on a
+// face, fresh value kept for this step's E update, masked on read-back), and
+// H cells outside every H nest lie on faces and are written 0 directly.
+#include <math.h>
+
+#include <vector>
+
+#include "sdt_device.h"
+#include "sdt_internal.h"
+
+namespace sdt {
+
+template <typename T>
+struct Em3Args {
+    T *E[3], *H[3];
+    const T *aEx, *bEx, *cEx, *bEy, *cEy, *cEz;            // aEy=bEx, aEz=cEx, bEz=cEy
+    const T *s11, *s12, *s13, *s22, *s23, *s33;
+    const T *a, *b, *ik, *ah, *bh, *ikh;
+    // E-side (H update) and H-side (E update) memory variables
+    T *m_dEy_dz, *m_dEz_dy, *m_dEx_dz, *m_dEz_dx, *m_dEx_dy, *m_dEy_dx;
+    T *m_dHz_dy, *m_dHy_dz, *m_dHz_dx, *m_dHx_dz, *m_dHx_dy, *m_dHy_dx;
+    const uint8_t *flags;
+    int nx, ny, nz, tiles_x, tiles_y;
+    T rdx, rdy, rdz, dt, dbH;
+    const int *d_it;
+    unsigned long long *vnorm;
+    int64_t src_cell;
+};
+
+template <typename T, bool PML>
+__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
+{
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const bool face = i == 1 || j == 1 || k == 1;
+    const T *Ex = A.E[0], *Ey = A.E[1], *Ez = A.E[2];
+    if (i == nx || j == ny || k == nz) {
+        A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0);
+        return;
+    }
+    T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
+    if (PML) {
+        a = A.a[c]; b = A.b[c]; ik = A.ik[c];
+        ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+    }
+    {   // Hx: cpmlfdtd.f08:3700-3718
+        T dEz_dy = (Ez[c + sy] - Ez[c]) * A.rdy;
+        T dEy_dz = (Ey[c + sz] - Ey[c]) * A.rdz;
+        if (PML) {
+            A.m_dEy_dz[c] = bh * A.m_dEy_dz[c] + ah * dEy_dz;
+            A.m_dEz_dy[c] = bh * A.m_dEz_dy[c] + ah * dEz_dy;
+            dEz_dy = dEz_dy * ikh + A.m_dEz_dy[c];
+            dEy_dz = dEy_dz * ikh + A.m_dEy_dz[c];
+        }
+        const T hold = face ? T(0) : A.H[0][c];
+        A.H[0][c] = hold + A.dbH * (dEy_dz - dEz_dy);
+    }
+    {   // Hy: cpmlfdtd.f08:3730-3749
+        T dEx_dz = (Ex[c + sz] - Ex[c]) * A.rdz;
+        T dEz_dx = (Ez[c + 1] - Ez[c]) * A.rdx;
+        if (PML) {
+            A.m_dEx_dz[c] = b * A.m_dEx_dz[c] + a * dEx_dz;
+            A.m_dEz_dx[c] = b * A.m_dEz_dx[c] + a * dEz_dx;
+            dEx_dz = dEx_dz * ik + A.m_dEx_dz[c];
+            dEz_dx = dEz_dx * ik + A.m_dEz_dx[c];
+        }
+        const T hold = face ? T(0) : A.H[1][c];
+        A.H[1][c] = hold + A.dbH * (dEz_dx - dEx_dz);
+    }
+    if (k >= 2) {   // Hz: cpmlfdtd.f08:3761-3778 (k from 2)
+        T dEx_dy = (Ex[c + sy] - Ex[c]) * A.rdy;
+        T dEy_dx = (Ey[c + 1] - Ey[c]) * A.rdx;
+        if (PML) {
+            A.m_dEx_dy[c] = b * A.m_dEx_dy[c] + a * dEx_dy;
+            A.m_dEy_dx[c] = b * A.m_dEy_dx[c] + a * dEy_dx;
+            dEx_dy = dEx_dy * ik + A.m_dEx_dy[c];
+            dEy_dx = dEy_dx * ik + A.m_dEy_dx[c];
+        }
+        const T hold = face ? T(0) : A.H[2][c];
+        A.H[2][c] = hold + A.dbH * (dEx_dy - dEy_dx);
+    } else {
+        A.H[2][c] = T(0);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    if (i > A.nx || j > A.ny) return;
+    const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
+    if (A.flags[t]) em3_h_cell<T, true>(A, i, j, k);
+    else em3_h_cell<T, false>(A, i, j, k);
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
+{
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    T ex, ey, ez;
+    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
+        ex = ey = ez = T(0);
+    } else {
+        // cpmlfdtd.f08:3795-3828
+        const T *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
+        T dHz_dy = (Hz[c] - Hz[c - sy]) * A.rdy;
+        T dHy_dz = (Hy[c] - Hy[c - sz]) * A.rdz;
+        T dHz_dx = (Hz[c] - Hz[c - 1]) * A.rdx;
+        T dHx_dz = (Hx[c] - Hx[c - sz]) * A.rdz;
+        T dHy_dx = (Hy[c] - Hy[c - 1]) * A.rdx;
+        T dHx_dy = (Hx[c] - Hx[c - sy]) * A.rdy;
+        if (PML) {
+            const T a = A.a[c], b = A.b[c], ik = A.ik[c];
+            const T ah = A.ah[c], bh = A.bh[c], ikh = A.ikh[c];
+            A.m_dHz_dy[c] = bh * A.m_dHz_dy[c] + ah * dHz_dy;
+            A.m_dHy_dz[c] = b * A.m_dHy_dz[c] + a * dHy_dz;
+            A.m_dHz_dx[c] = bh * A.m_dHz_dx[c] + ah * dHz_dx;
+            A.m_dHx_dz[c] = bh * A.m_dHx_dz[c] + ah * dHx_dz;
+            A.m_dHx_dy[c] = bh * A.m_dHx_dy[c] + ah * dHx_dy;
+            A.m_dHy_dx[c] = bh * A.m_dHy_dx[c] + ah * dHy_dx;
+            dHz_dy = dHz_dy * ikh + A.m_dHz_dy[c];
+            dHy_dz = dHy_dz * ik + A.m_dHy_dz[c];
+            dHz_dx = dHz_dx * ikh + A.m_dHz_dx[c];
+            dHx_dz = dHx_dz * ikh + A.m_dHx_dz[c];
+            dHx_dy = dHx_dy * ikh + A.m_dHx_dy[c];
+            dHy_dx = dHy_dx * ikh + A.m_dHy_dx[c];
+        }
+        const T exo = A.E[0][c], eyo = A.E[1][c], ezo = A.E[2][c];
+        const T s11 = A.s11[c], s12 = A.s12[c], s13 = A.s13[c], s22 = A.s22[c], s23 = A.s23[c], s33 = A.s33[c];
+        const T rx = dHz_dy - dHy_dz - (s11 * exo + s12 * eyo + s13 * ezo);
+        const T ry = dHx_dz - dHz_dx - (s12 * exo + s22 * eyo + s23 * ezo);
+        const T rz = dHy_dx - dHx_dy - (s13 * exo + s23 * eyo + s33 * ezo);
+        const T aEx = A.aEx[c], bEx = A.bEx[c], cEx = A.cEx[c], bEy = A.bEy[c], cEy = A.cEy[c], cEz = A.cEz[c];
+        ex = exo + (aEx * rx + bEx * ry + cEx * rz) * A.dt;
+        ey = eyo + (bEx * rx + bEy * ry + cEy * rz) * A.dt;
+        ez = ezo + (cEx * rx + cEy * ry + cEz * rz) * A.dt;
+    }
+    A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
+    if (c == A.src_cell) return 0.0;
+    const double x = ex, y = ey, z = ez;
+    return sqrt(x * x + y * y + z * z);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) em3_e_kernel(Em3Args<T> A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k = blockIdx.z + 1;
+    double nrm = 0.0;
+    if (i <= A.nx && j <= A.ny) {
+        const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
+        if (A.flags[t]) nrm = em3_e_cell<T, true>(A, i, j, k);
+        else nrm = em3_e_cell<T, false>(A, i, j, k);
+    }
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T>
+static Em3Args<T> em3_args(Device &d)
+{
+    Em3Args<T> A;
+    for (int q = 0; q < 3; ++q) {
+        A.E[q] = (T *)d.field[SDT_F_EX + q];
+        A.H[q] = (T *)d.field[SDT_F_HX + q];
+    }
+    const T **mp[] = {&A.aEx, &A.bEx, &A.cEx, &A.bEy, &A.cEy, &A.cEz,
+                      &A.s11, &A.s12, &A.s13, &A.s22, &A.s23, &A.s33};
+    for (int q = 0; q < 12; ++q) *mp[q] = (const T *)d.mat[q];
+    A.a = (const T *)d.cpml[0];
+    A.b = (const T *)d.cpml[1];
+    A.ik = (const T *)d.cpml[2];
+    A.ah = (const T *)d.cpml[3];
+    A.bh = (const T *)d.cpml[4];
+    A.ikh = (const T *)d.cpml[5];
+    T **mm[] = {&A.m_dEy_dz, &A.m_dEz_dy, &A.m_dEx_dz, &A.m_dEz_dx, &A.m_dEx_dy, &A.m_dEy_dx,
+                &A.m_dHz_dy, &A.m_dHy_dz, &A.m_dHz_dx, &A.m_dHx_dz, &A.m_dHx_dy, &A.m_dHy_dx};
+    for (int q = 0; q < 12; ++q) *mm[q] = (T *)d.mem[q];
+    A.flags = d.tile_flags;
+    A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
+    A.tiles_x = d.tiles_x; A.tiles_y = d.tiles_y;
+    A.rdx = (T)(1.0 / d.dx);
+    A.rdy = (T)(1.0 / d.dy);
+    A.rdz = (T)(1.0 / d.dz);
+    A.dt = (T)d.dt;
+    A.dbH = (T)(d.dt / kMu0);   // cpmlfdtd.f08:3510-3518
+    A.d_it = d.d_it;
+    A.vnorm = d.norms;
+    A.src_cell = d.tail.vfix[0];
+    return A;
+}
+
+template <typename T>
+static void em3_step_t(Device &d, bool timed)
+{
+    const Em3Args<T> A = em3_args<T>(d);
+    const dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z), block(64, 4, 1);
+    if (timed) (void)hipEventRecord(d.ev[0], d.stream);
+    hipLaunchKernelGGL(em3_h_kernel<T>, grid, block, 0, d.stream, A);
+    if (timed) (void)hipEventRecord(d.ev[1], d.stream);
+    hipLaunchKernelGGL(em3_e_kernel<T>, grid, block, 0, d.stream, A);
+    if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_tail(d);
+}
+
+void em3_step(Device &d, bool timed)
+{
+    if (d.precision == SDT_FP64) em3_step_t<double>(d, timed);
+    else em3_step_t<float>(d, timed);
+}
+
+// Inverse permittivity by adjugate/determinant, cpmlfdtd.f08:3640-3661, in
+// FP64 on the host; stored as the six unique entries.
+int em3_setup(Device &d, const double *const *in)
+{
+    const int64_t n = d.ncell;
+    std::vector<double> co[6];
+    for (auto &v : co) v.resize((size_t)n);
+    auto val = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
+    for (int64_t c = 0; c < n; ++c) {
+        const double e11 = val(SDT_E11, c) * kEps0, e12 = val(SDT_E12, c) * kEps0, e13 = val(SDT_E13, c) * kEps0;
+        const double e22 = val(SDT_E22, c) * kEps0, e23 = val(SDT_E23, c) * kEps0, e33 = val(SDT_E33, c) * kEps0;
+        const double det = e11 * (e22 * e33 - e23 * e23) - e12 * (e12 * e33 - e23 * e13) +
+                           e13 * (e12 * e23 - e22 * e13);
+        co[0][(size_t)c] = (e22 * e33 - e23 * e23) / det;
+        co[1][(size_t)c] = -(e12 * e33 - e23 * e13) / det;
+        co[2][(size_t)c] = (e12 * e23 - e22 * e13) / det;
+        co[3][(size_t)c] = (e11 * e33 - e13 * e13) / det;
+        co[4][(size_t)c] = -(e11 * e23 - e12 * e13) / det;
+        co[5][(size_t)c] = (e11 * e22 - e12 * e12) / det;
+    }
+    d.nmat = 0;
+    for (int q = 0; q < 6; ++q) {
+        int rc = upload_converted(d, co[q].data(), &d.mat[d.nmat++], n);
+        if (rc) return rc;
+    }
+    std::vector<double> zero;
+    for (int id = SDT_S11; id <= SDT_S33; ++id) {
+        if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
+        int rc = upload_converted(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++], n);
+        if (rc) return rc;
+    }
+    d.nmem = 12;
+    for (int q = 0; q < d.nmem; ++q) {
+        int rc = alloc_zeroed(d, &d.mem[q], n);
+        if (rc) return rc;
+    }
+    for (int f = SDT_F_EX; f <= SDT_F_HZ; ++f) {
+        int rc;
+        if (f <= SDT_F_EZ && in[SDT_IC_X + f]) rc = upload_converted(d, in[SDT_IC_X + f], &d.field[f], n);
+        else rc = alloc_zeroed(d, &d.field[f], n);
+        if (rc) return rc;
+    }
+
+    // point source, cpmlfdtd.f08:3909-3914
+    const int is = d.isrc, js = d.jsrc, ks = d.ksrc, nx = d.nx, ny = d.ny, nz = d.nz;
+    const int64_t c = (int64_t)(is - 1) + (int64_t)nx * ((int64_t)(js - 1) + (int64_t)ny * (ks - 1));
+    const int face = (is == 1 || is == nx || js == 1 || js == ny || ks == 1 || ks == nz);
+    TailArgs &t = d.tail;
+    t.nterms = 0;
+    t.terms[t.nterms++] = {c, SDT_F_EX, SDT_SRC_X, d.dt, val(SDT_E11, c) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EY, SDT_SRC_Y, d.dt, val(SDT_E22, c) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EZ, SDT_SRC_Z, d.dt, val(SDT_E33, c) * kEps0, face};
+    t.nvfix = 1;
+    t.vfix[0] = c;
+    t.nvcomp = 3;
+    t.vcomp[0] = SDT_F_EX; t.vcomp[1] = SDT_F_EY; t.vcomp[2] = SDT_F_EZ;
+    t.stress_norm = 0;
+    t.nsfix = 0;
+    t.rec_field[0] = SDT_F_EX; t.rec_field[1] = SDT_F_EY; t.rec_field[2] = SDT_F_EZ;
+
+    d.tile_x = 64;
+    d.tile_y = 4;
+    d.tile_z = 1;
+    return compute_tile_flags(d, in);
+}
+
+}  // namespace sdt

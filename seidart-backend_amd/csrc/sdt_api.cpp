@@ -1,0 +1,491 @@
+// sdt_api.cpp -- the extern "C" boundary of libseidart_hip.so (declared in
+// include/seidart_hip.h).  Host staging of the reference's input arrays,
+// device setup, the time loop (direct launches or captured hipGraphs of
+// several steps for small, launch-bound grids), the blow-up check and the
+// read-back of fields, receivers and norms.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "sdt_internal.h"
+
+using namespace sdt;
+
+struct sdt_handle {
+    Device d;
+    sdt_params p{};
+    std::vector<double> staged[SDT_NUM_ARRAYS];
+    std::vector<double> series;       // [SDT_NUM_SRC][time_steps]
+    std::vector<int64_t> rec_cells;
+    bool prepared = false;
+    bool failed = false;
+    int steps_done = 0;
+    std::string err;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    std::vector<hipEvent_t> tev;       // timing events, 3 per timed step
+};
+
+static thread_local std::string g_err;
+
+static int fail(sdt_handle *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    g_err = msg;
+    return code;
+}
+
+static int hip_fail(sdt_handle *h, hipError_t e, const char *where)
+{
+    return fail(h, SDT_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(h, call)                                     \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #call); \
+    } while (0)
+
+static bool is3d(int physics) { return physics == SDT_ELASTIC3 || physics == SDT_EM3; }
+static bool elastic(int physics) { return physics == SDT_ELASTIC2 || physics == SDT_ELASTIC3; }
+
+static int nfields_of(int physics, int f)
+{
+    // fields present per solver
+    switch (physics) {
+    case SDT_ELASTIC2: return f == SDT_F_VX || f == SDT_F_VZ || f == SDT_F_SXX || f == SDT_F_SZZ || f == SDT_F_SXZ;
+    case SDT_ELASTIC3: return f >= 0 && f < 9;
+    case SDT_EM2: return f == SDT_F_EX || f == SDT_F_EZ || f == SDT_F_HY;
+    case SDT_EM3: return f >= 0 && f < 6;
+    }
+    return 0;
+}
+
+extern "C" {
+
+int32_t sdt_version(void) { return SDT_ABI_VERSION; }
+
+int32_t sdt_device_count(int32_t *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (count) *count = (e == hipSuccess) ? n : 0;
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDeviceCount");
+    return SDT_OK;
+}
+
+const char *sdt_last_error(const sdt_handle *h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+int32_t sdt_create(const sdt_params *p, sdt_handle **out)
+{
+    if (!p || !out) return fail(nullptr, SDT_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (p->abi_version != SDT_ABI_VERSION) return fail(nullptr, SDT_ERR_ARG, "abi_version mismatch");
+    if (p->physics != SDT_ELASTIC2 && p->physics != SDT_ELASTIC3 && p->physics != SDT_EM2 && p->physics != SDT_EM3)
+        return fail(nullptr, SDT_ERR_ARG, "unknown physics");
+    if (p->precision != SDT_FP64 && p->precision != SDT_FP32) return fail(nullptr, SDT_ERR_ARG, "unknown precision");
+    const bool d3 = is3d(p->physics);
+    if (p->nx < 5 || p->nz < 5 || (d3 ? p->ny < 5 : p->ny != 1))
+        return fail(nullptr, SDT_ERR_ARG, "grid too small (need >= 5 cells per axis; ny = 1 in 2D)");
+    if (!(p->dx > 0) || !(p->dz > 0) || (d3 && !(p->dy > 0)) || !(p->dt > 0))
+        return fail(nullptr, SDT_ERR_ARG, "dx, dy, dz and dt must be positive");
+    if (p->time_steps < 1) return fail(nullptr, SDT_ERR_ARG, "time_steps must be >= 1");
+    if (p->cpml < 0) return fail(nullptr, SDT_ERR_ARG, "cpml must be >= 0");
+    if (elastic(p->physics) && (p->density_code < 1 || p->density_code > 4))
+        return fail(nullptr, SDT_ERR_ARG, "density_code must be 1..4");
+    if (p->stencil != SDT_STENCIL_REFERENCE && p->stencil != SDT_STENCIL_CORRECTED)
+        return fail(nullptr, SDT_ERR_ARG, "unknown stencil");
+    // source cells: padded 1-based, plus the +1 offsets of the elastic stress
+    // and velocity sources (cpmlfdtd.f08:519-523, 2208-2219)
+    const int is = p->xind + p->cpml, js = (d3 ? p->yind : 1) + (d3 ? p->cpml : 0), ks = p->zind + p->cpml;
+    const int off = elastic(p->physics) ? 1 : 0;
+    if (is < 1 || ks < 1 || js < 1 || is + off > p->nx || ks + off > p->nz || (d3 && js + off > p->ny))
+        return fail(nullptr, SDT_ERR_ARG, "source index outside the padded grid");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) return fail(nullptr, SDT_ERR_HIP, "no HIP device available");
+    if (p->device < 0 || p->device >= ndev) return fail(nullptr, SDT_ERR_ARG, "device ordinal out of range");
+
+    sdt_handle *h = new sdt_handle();
+    h->p = *p;
+    Device &d = h->d;
+    d.physics = p->physics;
+    d.precision = p->precision;
+    d.esize = p->precision == SDT_FP64 ? 8 : 4;
+    d.nx = p->nx; d.ny = p->ny; d.nz = p->nz;
+    d.dx = p->dx; d.dy = d3 ? p->dy : 1.0; d.dz = p->dz; d.dt = p->dt;
+    d.time_steps = p->time_steps;
+    d.isrc = is; d.jsrc = js; d.ksrc = ks;
+    d.density_code = p->density_code;
+    d.stencil = p->stencil;
+    d.ncell = (int64_t)p->nx * p->ny * p->nz;
+    h->series.assign((size_t)SDT_NUM_SRC * p->time_steps, 0.0);
+    e = hipSetDevice(p->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        int rc = hip_fail(nullptr, e, "stream creation");
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return SDT_OK;
+}
+
+static void release(sdt_handle *h)
+{
+    Device &d = h->d;
+    (void)hipSetDevice(h->p.device);
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    auto drop = [](void *p) { if (p) (void)hipFree(p); };
+    for (auto &p : d.field) drop(p);
+    for (auto &p : d.mem) drop(p);
+    for (auto &p : d.mat) drop(p);
+    for (auto &p : d.cpml) drop(p);
+    drop(d.tile_flags);
+    drop(d.series);
+    drop(d.norms);
+    drop(d.d_it);
+    drop(d.rec_cell);
+    drop(d.rec_out);
+    for (auto ev : h->tev) (void)hipEventDestroy(ev);
+    if (d.stream) (void)hipStreamDestroy(d.stream);
+}
+
+int32_t sdt_destroy(sdt_handle *h)
+{
+    if (!h) return SDT_OK;
+    release(h);
+    delete h;
+    return SDT_OK;
+}
+
+int32_t sdt_set_array(sdt_handle *h, int32_t id, const double *host, int64_t count)
+{
+    if (!h || !host) return fail(h, SDT_ERR_ARG, "null argument");
+    if (h->prepared) return fail(h, SDT_ERR_STATE, "inputs are fixed once the first step has run");
+    if (id < 0 || id >= SDT_NUM_ARRAYS) return fail(h, SDT_ERR_ARG, "unknown array id");
+    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "array length must equal nx*ny*nz");
+    h->staged[id].assign(host, host + count);
+    return SDT_OK;
+}
+
+int32_t sdt_set_source(sdt_handle *h, int32_t comp, const double *series, int64_t count)
+{
+    if (!h || !series) return fail(h, SDT_ERR_ARG, "null argument");
+    if (h->prepared) return fail(h, SDT_ERR_STATE, "inputs are fixed once the first step has run");
+    if (comp < 0 || comp >= SDT_NUM_SRC) return fail(h, SDT_ERR_ARG, "unknown source component");
+    if (count != h->d.time_steps) return fail(h, SDT_ERR_ARG, "series length must equal time_steps");
+    memcpy(h->series.data() + (size_t)comp * h->d.time_steps, series, (size_t)count * sizeof(double));
+    return SDT_OK;
+}
+
+int32_t sdt_set_receivers(sdt_handle *h, int32_t n, const int32_t *ijk)
+{
+    if (!h || (n > 0 && !ijk) || n < 0) return fail(h, SDT_ERR_ARG, "bad receiver list");
+    if (h->prepared) return fail(h, SDT_ERR_STATE, "receivers are fixed once the first step has run");
+    const Device &d = h->d;
+    const bool d3 = is3d(d.physics);
+    std::vector<int64_t> cells((size_t)n);
+    for (int q = 0; q < n; ++q) {
+        const int i = ijk[3 * q] + h->p.cpml;
+        const int j = d3 ? ijk[3 * q + 1] + h->p.cpml : 1;
+        const int k = ijk[3 * q + 2] + h->p.cpml;
+        if (i < 1 || i > d.nx || j < 1 || j > d.ny || k < 1 || k > d.nz)
+            return fail(h, SDT_ERR_ARG, "receiver " + std::to_string(q) + " outside the padded grid");
+        cells[(size_t)q] = (int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1));
+    }
+    h->rec_cells = cells;
+    return SDT_OK;
+}
+
+static int prepare(sdt_handle *h)
+{
+    if (h->prepared) return SDT_OK;
+    Device &d = h->d;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    const double *in[SDT_NUM_ARRAYS];
+    for (int q = 0; q < SDT_NUM_ARRAYS; ++q) in[q] = h->staged[q].empty() ? nullptr : h->staged[q].data();
+    // the reference fills kappa/kappa_half with 1 before reading them
+    // (cpmlfdtd.f08:240-247); arrays never set keep that default
+    int rc = SDT_OK;
+    switch (d.physics) {
+    case SDT_ELASTIC2: rc = el2_setup(d, in); break;
+    case SDT_ELASTIC3: rc = el3_setup(d, in); break;
+    case SDT_EM2: rc = em2_setup(d, in); break;
+    case SDT_EM3: rc = em3_setup(d, in); break;
+    }
+    if (rc) return fail(h, rc, "device setup failed (out of device memory?)");
+    // CPML coefficients: a, b and 1/kappa for the full and half sets
+    const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
+    std::vector<double> tmp((size_t)d.ncell);
+    for (int q = 0; q < 6; ++q) {
+        const double *src = in[cids[q]];
+        const bool inverse = (q == 2 || q == 5);
+        for (int64_t c = 0; c < d.ncell; ++c) {
+            const double v = src ? src[c] : (inverse ? 1.0 : 0.0);
+            tmp[(size_t)c] = inverse ? 1.0 / v : v;
+        }
+        if (upload_converted(d, tmp.data(), &d.cpml[q], d.ncell)) return fail(h, SDT_ERR_HIP, "cpml upload failed");
+    }
+    tmp.clear();
+    tmp.shrink_to_fit();
+    // source series, norms, step counter, receivers
+    const size_t ns = (size_t)SDT_NUM_SRC * d.time_steps;
+    HIPCHK(h, hipMalloc((void **)&d.series, ns * sizeof(double)));
+    HIPCHK(h, hipMemcpy(d.series, h->series.data(), ns * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMalloc((void **)&d.norms, 2 * (size_t)d.time_steps * sizeof(unsigned long long)));
+    HIPCHK(h, hipMemset(d.norms, 0, 2 * (size_t)d.time_steps * sizeof(unsigned long long)));
+    HIPCHK(h, hipMalloc((void **)&d.d_it, sizeof(int)));
+    const int one = 1;
+    HIPCHK(h, hipMemcpy(d.d_it, &one, sizeof(int), hipMemcpyHostToDevice));
+    d.nrec = (int)h->rec_cells.size();
+    if (d.nrec > 0) {
+        HIPCHK(h, hipMalloc((void **)&d.rec_cell, (size_t)d.nrec * sizeof(int64_t)));
+        HIPCHK(h, hipMemcpy(d.rec_cell, h->rec_cells.data(), (size_t)d.nrec * sizeof(int64_t), hipMemcpyHostToDevice));
+        const size_t nr = (size_t)d.time_steps * d.nrec * 3;
+        HIPCHK(h, hipMalloc((void **)&d.rec_out, nr * sizeof(double)));
+        HIPCHK(h, hipMemset(d.rec_out, 0, nr * sizeof(double)));
+    }
+    TailArgs &t = d.tail;
+    t.d_it = d.d_it;
+    t.series = d.series;
+    t.time_steps = d.time_steps;
+    t.vnorm = d.norms;
+    t.snorm = d.norms + d.time_steps;
+    t.nrec = d.nrec;
+    t.rec_cell = d.rec_cell;
+    t.rec_out = d.rec_out;
+    HIPCHK(h, hipDeviceSynchronize());
+    for (auto &s : h->staged) { std::vector<double>().swap(s); }
+    h->prepared = true;
+    return SDT_OK;
+}
+
+static void enqueue_step(sdt_handle *h, bool timed)
+{
+    Device &d = h->d;
+    switch (d.physics) {
+    case SDT_ELASTIC2: el2_step(d, timed); break;
+    case SDT_ELASTIC3: el3_step(d, timed); break;
+    case SDT_EM2: em2_step(d, timed); break;
+    case SDT_EM3: em3_step(d, timed); break;
+    }
+}
+
+// Small grids are launch-bound: capture kGraphSteps steps once and replay.
+// The step index lives in device memory, so one graph serves every replay.
+static constexpr int kGraphSteps = 32;
+static constexpr int64_t kGraphMaxCells = 4 * 1024 * 1024;
+
+static int ensure_graph(sdt_handle *h)
+{
+    if (h->graph) return SDT_OK;
+    Device &d = h->d;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < kGraphSteps; ++s) enqueue_step(h, false);
+    HIPCHK(h, hipStreamEndCapture(d.stream, &g));
+    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+    h->graph_steps = kGraphSteps;
+    return SDT_OK;
+}
+
+// Check the velocity norms of steps [first, first+n) (1-based first).
+static int check_norms(sdt_handle *h, int first, int n, int32_t *failed_step)
+{
+    Device &d = h->d;
+    std::vector<unsigned long long> bits((size_t)n);
+    HIPCHK(h, hipMemcpyAsync(bits.data(), d.norms + (first - 1), (size_t)n * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, d.stream));
+    HIPCHK(h, hipStreamSynchronize(d.stream));
+    for (int s = 0; s < n; ++s) {
+        double v;
+        memcpy(&v, &bits[(size_t)s], sizeof(double));
+        if (v > kStability || v != v) {
+            if (failed_step) *failed_step = first + s;
+            h->failed = true;
+            h->steps_done = first + s;
+            return fail(h, SDT_ERR_UNSTABLE, "code became unstable and blew up at step " + std::to_string(first + s));
+        }
+    }
+    return SDT_OK;
+}
+
+int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    if (failed_step) *failed_step = 0;
+    if (h->failed) return fail(h, SDT_ERR_STATE, "the run has already blown up");
+    if (nsteps < 0) return fail(h, SDT_ERR_ARG, "nsteps must be >= 0");
+    if (h->steps_done + nsteps > h->d.time_steps)
+        return fail(h, SDT_ERR_ARG, "run would exceed time_steps");
+    int rc = prepare(h);
+    if (rc) return rc;
+    Device &d = h->d;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    const bool use_graph = !d.timing && d.ncell <= kGraphMaxCells;
+    const int chunk = 256;   // steps between blow-up checks
+    int left = nsteps;
+    while (left > 0) {
+        const int n = left < chunk ? left : chunk;
+        const int first = h->steps_done + 1;
+        if (d.timing) {
+            if ((int)h->tev.size() < 3 * n) {
+                size_t old = h->tev.size();
+                h->tev.resize((size_t)3 * n);
+                for (size_t q = old; q < h->tev.size(); ++q) HIPCHK(h, hipEventCreate(&h->tev[q]));
+            }
+            for (int s = 0; s < n; ++s) {
+                d.ev[0] = h->tev[3 * s];
+                d.ev[1] = h->tev[3 * s + 1];
+                d.ev[2] = h->tev[3 * s + 2];
+                enqueue_step(h, true);
+            }
+            HIPCHK(h, hipStreamSynchronize(d.stream));
+            for (int s = 0; s < n; ++s) {
+                float a = 0, b = 0;
+                HIPCHK(h, hipEventElapsedTime(&a, h->tev[3 * s], h->tev[3 * s + 1]));
+                HIPCHK(h, hipEventElapsedTime(&b, h->tev[3 * s + 1], h->tev[3 * s + 2]));
+                d.ms_stress += a;
+                d.ms_vel += b;
+                d.timed_launches += 1;
+            }
+        } else {
+            int s = 0;
+            if (use_graph && n >= kGraphSteps) {
+                rc = ensure_graph(h);
+                if (rc) return rc;
+                for (; s + kGraphSteps <= n; s += kGraphSteps) HIPCHK(h, hipGraphLaunch(h->graph, d.stream));
+            }
+            for (; s < n; ++s) enqueue_step(h, false);
+        }
+        HIPCHK(h, hipGetLastError());
+        rc = check_norms(h, first, n, failed_step);
+        if (rc) return rc;
+        h->steps_done += n;
+        left -= n;
+    }
+    return SDT_OK;
+}
+
+int32_t sdt_steps_done(const sdt_handle *h, int32_t *steps)
+{
+    if (!h || !steps) return fail(nullptr, SDT_ERR_ARG, "null argument");
+    *steps = h->steps_done;
+    return SDT_OK;
+}
+
+int32_t sdt_get_field(sdt_handle *h, int32_t f, double *out, int64_t count)
+{
+    if (!h || !out) return fail(h, SDT_ERR_ARG, "null argument");
+    if (f < 0 || f >= SDT_NUM_FIELDS || !nfields_of(h->d.physics, f)) return fail(h, SDT_ERR_ARG, "field not present in this solver");
+    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "count must equal nx*ny*nz");
+    int rc = prepare(h);
+    if (rc) return rc;
+    Device &d = h->d;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamSynchronize(d.stream));
+    if (d.precision == SDT_FP64) {
+        HIPCHK(h, hipMemcpy(out, d.field[f], (size_t)count * 8, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp((size_t)count);
+        HIPCHK(h, hipMemcpy(tmp.data(), d.field[f], (size_t)count * 4, hipMemcpyDeviceToHost));
+        for (int64_t c = 0; c < count; ++c) out[c] = tmp[(size_t)c];
+    }
+    // lazily applied Dirichlet faces of H (em2.hip / em3.hip)
+    const bool hfield = (d.physics == SDT_EM2 && f == SDT_F_HY) || (d.physics == SDT_EM3 && f >= SDT_F_HX);
+    if (hfield && h->steps_done > 0) {
+        for (int k = 1; k <= d.nz; ++k)
+            for (int j = 1; j <= d.ny; ++j)
+                for (int i = 1; i <= d.nx; ++i) {
+                    const bool face = i == 1 || i == d.nx || k == 1 || k == d.nz ||
+                                      (d.physics == SDT_EM3 && (j == 1 || j == d.ny));
+                    if (face) out[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))] = 0.0;
+                }
+    }
+    return SDT_OK;
+}
+
+int32_t sdt_set_field(sdt_handle *h, int32_t f, const double *host, int64_t count)
+{
+    if (!h || !host) return fail(h, SDT_ERR_ARG, "null argument");
+    if (f < 0 || f >= SDT_NUM_FIELDS || !nfields_of(h->d.physics, f)) return fail(h, SDT_ERR_ARG, "field not present in this solver");
+    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "count must equal nx*ny*nz");
+    int rc = prepare(h);
+    if (rc) return rc;
+    Device &d = h->d;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamSynchronize(d.stream));
+    if (d.precision == SDT_FP64) {
+        HIPCHK(h, hipMemcpy(d.field[f], host, (size_t)count * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tmp((size_t)count);
+        for (int64_t c = 0; c < count; ++c) tmp[(size_t)c] = (float)host[c];
+        HIPCHK(h, hipMemcpy(d.field[f], tmp.data(), (size_t)count * 4, hipMemcpyHostToDevice));
+    }
+    return SDT_OK;
+}
+
+int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count)
+{
+    if (!h || (!out && count)) return fail(h, SDT_ERR_ARG, "null argument");
+    const int64_t need = (int64_t)h->steps_done * (int64_t)h->rec_cells.size() * 3;
+    if (count != need) return fail(h, SDT_ERR_ARG, "count must equal steps_done * n_receivers * 3");
+    if (need == 0) return SDT_OK;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamSynchronize(h->d.stream));
+    HIPCHK(h, hipMemcpy(out, h->d.rec_out, (size_t)need * sizeof(double), hipMemcpyDeviceToHost));
+    return SDT_OK;
+}
+
+int32_t sdt_get_norms(sdt_handle *h, double *vel, double *stress, int64_t count)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    if (count != h->steps_done) return fail(h, SDT_ERR_ARG, "count must equal steps_done");
+    if (count == 0) return SDT_OK;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamSynchronize(h->d.stream));
+    std::vector<unsigned long long> bits((size_t)count);
+    double *dst[2] = {vel, stress};
+    for (int q = 0; q < 2; ++q) {
+        if (!dst[q]) continue;
+        HIPCHK(h, hipMemcpy(bits.data(), h->d.norms + (size_t)q * h->d.time_steps, (size_t)count * 8, hipMemcpyDeviceToHost));
+        memcpy(dst[q], bits.data(), (size_t)count * 8);
+    }
+    return SDT_OK;
+}
+
+int32_t sdt_set_timing(sdt_handle *h, int32_t enable)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    h->d.timing = enable != 0;
+    h->d.ms_stress = h->d.ms_vel = 0;
+    h->d.timed_launches = 0;
+    return SDT_OK;
+}
+
+int32_t sdt_get_kernel_times(sdt_handle *h, double *ms_stress, double *ms_velocity, int64_t *launches)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    if (ms_stress) *ms_stress = h->d.ms_stress;
+    if (ms_velocity) *ms_velocity = h->d.ms_vel;
+    if (launches) *launches = h->d.timed_launches;
+    return SDT_OK;
+}
+
+int32_t sdt_synchronize(sdt_handle *h)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    HIPCHK(h, hipSetDevice(h->p.device));
+    HIPCHK(h, hipStreamSynchronize(h->d.stream));
+    return SDT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,138 @@
+// sdt_common.hip -- per-step tail kernel (point sources, norm fix-up,
+// receivers, step counter) and setup helpers shared by all solvers.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "sdt_device.h"
+#include "sdt_internal.h"
+
+namespace sdt {
+
+template <typename T>
+__device__ __forceinline__ T *fptr(void *const *field, int f)
+{
+    return reinterpret_cast<T *>(field[f]);
+}
+
+struct FieldPtrs {
+    void *f[SDT_NUM_FIELDS];
+};
+
+// One workgroup.  Thread 0 injects the point sources in the reference's order
+// (e.g. cpmlfdtd.f08:519-523, 2208-2219, 2678-2681, 3909-3914) and folds the
+// source cells into the step's norms; then every thread records receivers;
+// finally thread 0 advances the device step counter.
+template <typename T>
+__global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
+{
+    const int it = *a.d_it;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < a.nterms; ++q) {
+            const SrcTerm &s = a.terms[q];
+            if (s.cell < 0 || s.on_face) continue;
+            const double v = a.series[(int64_t)s.component * a.time_steps + (it - 1)];
+            T *f = reinterpret_cast<T *>(fp.f[s.field]);
+            f[s.cell] = (T)((double)f[s.cell] + (T)((v * s.mul) / s.div));
+        }
+        for (int q = 0; q < a.nvfix; ++q) {
+            double ss = 0.0;
+            for (int c = 0; c < a.nvcomp; ++c) {
+                const double v = (double)reinterpret_cast<T *>(fp.f[a.vcomp[c]])[a.vfix[q]];
+                ss += v * v;
+            }
+            atomic_max_nonneg(&a.vnorm[it - 1], sqrt(ss));
+        }
+        if (a.stress_norm) {
+            for (int q = 0; q < a.nsfix; ++q) {
+                const int64_t c = a.sfix[q];
+                const double sxx = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXX])[c];
+                const double syy = (double)reinterpret_cast<T *>(fp.f[SDT_F_SYY])[c];
+                const double szz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SZZ])[c];
+                const double sxy = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXY])[c];
+                const double sxz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXZ])[c];
+                const double syz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SYZ])[c];
+                const double s = sqrt(sxx * sxx + syy * syy + szz * szz +
+                                      2.0 * (sxy * sxy + syz * syz + sxz * sxz));
+                atomic_max_nonneg(&a.snorm[it - 1], s);
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < a.nrec; q += blockDim.x) {
+        const int64_t c = a.rec_cell[q];
+        double *o = a.rec_out + ((int64_t)(it - 1) * a.nrec + q) * 3;
+        for (int m = 0; m < 3; ++m)
+            o[m] = a.rec_field[m] >= 0 ? (double)reinterpret_cast<T *>(fp.f[a.rec_field[m]])[c] : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *a.d_it = it + 1;
+}
+
+void launch_tail(Device &d)
+{
+    FieldPtrs fp;
+    for (int f = 0; f < SDT_NUM_FIELDS; ++f) fp.f[f] = d.field[f];
+    if (d.precision == SDT_FP64)
+        hipLaunchKernelGGL(tail_kernel<double>, dim3(1), dim3(256), 0, d.stream, fp, d.tail);
+    else
+        hipLaunchKernelGGL(tail_kernel<float>, dim3(1), dim3(256), 0, d.stream, fp, d.tail);
+}
+
+int upload_converted(Device &d, const double *host, void **dev, int64_t n)
+{
+    const size_t bytes = (size_t)n * d.esize;
+    if (hipMalloc(dev, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (d.precision == SDT_FP64) {
+        if (hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice) != hipSuccess) return SDT_ERR_HIP;
+    } else {
+        std::vector<float> tmp((size_t)n);
+        for (int64_t i = 0; i < n; ++i) tmp[(size_t)i] = (float)host[i];
+        if (hipMemcpy(*dev, tmp.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return SDT_ERR_HIP;
+    }
+    return SDT_OK;
+}
+
+int alloc_zeroed(Device &d, void **dev, int64_t n)
+{
+    const size_t bytes = (size_t)n * d.esize;
+    if (hipMalloc(dev, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+static inline bool trivial(double a, double b, double kappa)
+{
+    return a == 0.0 && kappa == 1.0 && isfinite(b);
+}
+
+int compute_tile_flags(Device &d, const double *const *host_in)
+{
+    d.tiles_x = (d.nx + d.tile_x - 1) / d.tile_x;
+    d.tiles_y = (d.ny + d.tile_y - 1) / d.tile_y;
+    d.tiles_z = (d.nz + d.tile_z - 1) / d.tile_z;
+    const size_t ntiles = (size_t)d.tiles_x * d.tiles_y * d.tiles_z;
+    std::vector<uint8_t> flags(ntiles, 0);
+    const double *a = host_in[SDT_ACOEF], *b = host_in[SDT_BCOEF], *k = host_in[SDT_KAPPA];
+    const double *ah = host_in[SDT_ACOEF_HALF], *bh = host_in[SDT_BCOEF_HALF],
+                 *kh = host_in[SDT_KAPPA_HALF];
+    for (int kz = 0; kz < d.nz; ++kz)
+        for (int jy = 0; jy < d.ny; ++jy) {
+            const int64_t row = (int64_t)d.nx * (jy + (int64_t)d.ny * kz);
+            const size_t trow = (size_t)d.tiles_x * (jy / d.tile_y + (size_t)d.tiles_y * (kz / d.tile_z));
+            for (int ix = 0; ix < d.nx; ++ix) {
+                const int64_t c = row + ix;
+                const bool t = trivial(a ? a[c] : 0.0, b ? b[c] : 0.0, k ? k[c] : 1.0) &&
+                               trivial(ah ? ah[c] : 0.0, bh ? bh[c] : 0.0, kh ? kh[c] : 1.0);
+                if (!t) flags[trow + ix / d.tile_x] = 1;
+            }
+        }
+    if (hipMalloc((void **)&d.tile_flags, ntiles) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(d.tile_flags, flags.data(), ntiles, hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+}  // namespace sdt

@@ -1,0 +1,62 @@
+// sdt_device.h -- device-side helpers shared by the gfx950 CPML-FDTD kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdt {
+
+// Fortran-order linear index of 1-based (i, j, k) on an nx*ny*nz grid.
+__device__ __forceinline__ int64_t lin3(int i, int j, int k, int nx, int ny)
+{
+    return (int64_t)(i - 1) + (int64_t)nx * ((int64_t)(j - 1) + (int64_t)ny * (int64_t)(k - 1));
+}
+
+// scalar_mean(a, b, code) of averaging.f08:5-25 (1 harmonic, 2 geometric,
+// 3 arithmetic, 4 none).
+template <typename T>
+__device__ __forceinline__ T scalar_mean(T a, T b, int code)
+{
+    switch (code) {
+    case 1: return T(2) * a * b / (a + b);
+    case 2: return sqrt(a * b);
+    case 3: return T(0.5) * (a + b);
+    default: return a;
+    }
+}
+
+// Max of non-negative doubles through their IEEE bit patterns (monotone for
+// x >= 0; NaN patterns compare above +inf, so a NaN norm reads as a blow-up).
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, double v)
+{
+    atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// Workgroup max-reduction of a non-negative value followed by one atomic per
+// workgroup.  blockDim.x*blockDim.y*blockDim.z must be a multiple of 64.
+__device__ __forceinline__ void block_max_to(unsigned long long *slot, double v)
+{
+    __shared__ double wave_max[16];
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_down(v, off, 64);
+        v = (o > v || o != o) ? o : v;
+    }
+    if (lane == 0) wave_max[wave] = v;
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+        double m = wave_max[0];
+        for (int w = 1; w < nw; ++w) m = (wave_max[w] > m || wave_max[w] != wave_max[w]) ? wave_max[w] : m;
+        if (m > 0.0 || m != m) atomic_max_nonneg(slot, m);
+    }
+}
+
+// Per-launch constants shared by every solver kernel.
+struct GridInfo {
+    int nx, ny, nz;
+    int tiles_x, tiles_y;   // tile counts of the launch grid (flags indexing)
+};
+
+}  // namespace sdt

@@ -1,0 +1,113 @@
+// sdt_internal.h -- host-side state shared between the C-ABI (sdt_api.cpp)
+// and the per-solver launchers (*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/seidart_hip.h"
+
+namespace sdt {
+
+constexpr int kMaxMem = 32;     // memory-variable arrays per solver (3D elastic: 29)
+constexpr int kMaxMat = 32;     // device material arrays per solver (3D elastic: 28)
+constexpr double kEps0 = 8.85418782e-12;                  // constants.f08:11
+constexpr double kMu0 = 4.0 * 3.141592653589793238462643 * 1.0e-7;  // constants.f08:10
+constexpr double kStability = 1.0e25;                    // constants.f08:12
+
+// One point-source injection: field[cell] += (series[it] * mul) / div.
+struct SrcTerm {
+    int64_t cell;     // linear index, -1 = inactive
+    int field;        // SDT_F_*
+    int component;    // SDT_SRC_*
+    double mul, div;  // reference evaluation order: (src * mul) / div
+    int on_face;      // velocity/E injection on a Dirichlet face: dropped
+};
+
+// Cells whose norm contribution the tail kernel recomputes after injection.
+constexpr int kMaxNormFix = 9;
+
+struct TailArgs {
+    int *d_it;                       // device step counter (1-based step being run)
+    const double *series;            // [SDT_NUM_SRC][time_steps]
+    int time_steps;
+    SrcTerm terms[SDT_NUM_SRC];
+    int nterms;
+    int64_t vfix[kMaxNormFix];       // cells excluded from the fused velocity norm
+    int nvfix;
+    int64_t sfix[kMaxNormFix];       // cells excluded from the fused stress norm
+    int nsfix;
+    int nvcomp;                      // components in the velocity norm (2 or 3)
+    int vcomp[3];                    // their field ids
+    int stress_norm;                 // 1 for 3D elastic
+    unsigned long long *vnorm;       // [time_steps] bit patterns
+    unsigned long long *snorm;       // [time_steps]
+    int nrec;
+    const int64_t *rec_cell;         // [nrec]
+    double *rec_out;                 // [time_steps][nrec][3]
+    int rec_field[3];                // field ids recorded (-1 -> 0.0)
+};
+
+struct Device {
+    int physics = 0, precision = 0;
+    int nx = 0, ny = 0, nz = 0;
+    double dx = 0, dy = 0, dz = 0, dt = 0;
+    int time_steps = 0;
+    int isrc = 0, jsrc = 0, ksrc = 0;   // padded 1-based
+    int density_code = 3, stencil = 0;
+    int64_t ncell = 0;
+    size_t esize = 8;                  // sizeof(T)
+
+    void *field[SDT_NUM_FIELDS] = {};
+    void *mem[kMaxMem] = {};
+    int nmem = 0;
+    void *mat[kMaxMat] = {};
+    int nmat = 0;
+    void *cpml[6] = {};                // a, b, 1/kappa (full), a, b, 1/kappa (half)
+    uint8_t *tile_flags = nullptr;     // 1 = tile holds CPML cells
+    int tiles_x = 0, tiles_y = 0, tiles_z = 0;
+    int tile_x = 64, tile_y = 4, tile_z = 1;
+
+    double *series = nullptr;          // [SDT_NUM_SRC][time_steps]
+    unsigned long long *norms = nullptr;  // [2][time_steps]
+    int *d_it = nullptr;
+    int64_t *rec_cell = nullptr;
+    double *rec_out = nullptr;
+    int nrec = 0;
+
+    TailArgs tail{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    bool timing = false;
+    double ms_stress = 0, ms_vel = 0;
+    int64_t timed_launches = 0;
+};
+
+// Per-solver hooks, defined in the solver's .hip file.
+//   setup: convert the staged FP64 inputs (host_in) into the solver's device
+//          layout; called once before the first step.
+//   step:  enqueue one time step (stress/H kernel, velocity/E kernel, tail).
+typedef int (*setup_fn)(Device &d, const double *const *host_in);
+typedef void (*step_fn)(Device &d, bool timed);
+
+int el2_setup(Device &d, const double *const *host_in);
+void el2_step(Device &d, bool timed);
+int el3_setup(Device &d, const double *const *host_in);
+void el3_step(Device &d, bool timed);
+int em2_setup(Device &d, const double *const *host_in);
+void em2_step(Device &d, bool timed);
+int em3_setup(Device &d, const double *const *host_in);
+void em3_step(Device &d, bool timed);
+
+// common pieces (sdt_common.hip)
+void launch_tail(Device &d);
+// Tile flags: 1 where any cell of the (tile_x, tile_y, tile_z) tile has a
+// non-identity CPML coefficient (a != 0, kappa != 1 or non-finite b, in
+// either the full or the half set).
+int compute_tile_flags(Device &d, const double *const *host_in);
+// copy n FP64 host values into a freshly allocated device array of the
+// handle's precision
+int upload_converted(Device &d, const double *host, void **dev, int64_t n);
+int alloc_zeroed(Device &d, void **dev, int64_t n);
+
+}  // namespace sdt

@@ -1,0 +1,184 @@
+"""In-memory solver handle over the C-ABI.
+
+A `Solver` owns one device handle: inputs are numpy arrays of the padded grid
+(Fortran order, shape (nx, nz) in 2D or (nx, ny, nz) in 3D), outputs are
+numpy arrays of the same layout.  The file-based drop-in of the reference
+(`seidart_hip.cpmlfdtd`) and the CLI are built on top of it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+PHYSICS = {"elastic2": L.ELASTIC2, "elastic3": L.ELASTIC3, "em2": L.EM2, "em3": L.EM3}
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """One simulation on one GPU.
+
+    physics: "elastic2" | "elastic3" | "em2" | "em3"
+    shape: padded (nx, nz) or (nx, ny, nz)
+    spacing: (dx, dz) or (dx, dy, dz)
+    src: 1-based unpadded source indices (xind, zind) or (xind, yind, zind),
+         as Source_Type%xind.. (src/fortran/seidart_types.f08:23-33)
+    """
+
+    def __init__(self, physics, shape, spacing, cpml, dt, time_steps, src, *,
+                 precision="fp64", density_method="arithmetic", stencil="reference", device=0):
+        if physics not in PHYSICS:
+            raise ValueError(f"unknown physics {physics!r}")
+        self.physics = physics
+        self.is3d = physics.endswith("3")
+        self.shape = tuple(int(s) for s in shape)
+        if len(self.shape) != (3 if self.is3d else 2):
+            raise ValueError("shape rank does not match physics")
+        p = L.Params()
+        p.abi_version = L.ABI_VERSION
+        p.physics = PHYSICS[physics]
+        p.precision = {"fp64": L.FP64, "fp32": L.FP32}[precision]
+        if self.is3d:
+            p.nx, p.ny, p.nz = self.shape
+            p.dx, p.dy, p.dz = (float(s) for s in spacing)
+            p.xind, p.yind, p.zind = (int(s) for s in src)
+        else:
+            p.nx, p.nz = self.shape
+            p.ny = 1
+            p.dx, p.dz = (float(s) for s in spacing)
+            p.dy = 1.0
+            p.xind, p.zind = (int(s) for s in src)
+            p.yind = 1
+        p.cpml = int(cpml)
+        p.dt = float(dt)
+        p.time_steps = int(time_steps)
+        p.density_code = L.DENSITY_CODES[density_method]
+        p.stencil = {"reference": L.STENCIL_REFERENCE, "corrected": L.STENCIL_CORRECTED}[stencil]
+        p.device = int(device)
+        self.params = p
+        self.ncell = int(np.prod(self.shape))
+        self.time_steps = int(time_steps)
+        self.fields = L.FIELD_IDS_ELASTIC if physics.startswith("elastic") else L.FIELD_IDS_EM
+        self._h = C.c_void_p()
+        L.check(L.lib.sdt_create(C.byref(p), C.byref(self._h)))
+        self.nrec = 0
+
+    # ---- lifetime ---------------------------------------------------------
+    def close(self):
+        if self._h:
+            L.lib.sdt_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- inputs -----------------------------------------------------------
+    def _grid_array(self, a):
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"array shape {a.shape} != grid {self.shape}")
+        return np.asfortranarray(a).ravel(order="F")
+
+    def set_array(self, name, a):
+        """name: reference input file stem, e.g. 'c11', 'rho', 'kappa_cpml', 'ic_x'."""
+        flat = np.ascontiguousarray(self._grid_array(a))
+        L.check(L.lib.sdt_set_array(self._h, L.ARRAY_IDS[name], _dptr(flat), flat.size), self._h)
+
+    def set_arrays(self, arrays):
+        for k, v in arrays.items():
+            self.set_array(k, v)
+
+    def set_source(self, component, series):
+        s = np.ascontiguousarray(np.asarray(series, dtype=np.float64).ravel())
+        L.check(L.lib.sdt_set_source(self._h, L.SOURCE_IDS[component], _dptr(s), s.size), self._h)
+
+    def set_receivers(self, idx):
+        """idx: (n, 3) 1-based unpadded (x, y, z) indices; y ignored in 2D."""
+        r = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 3))
+        L.check(L.lib.sdt_set_receivers(self._h, r.shape[0], r.ctypes.data_as(C.POINTER(C.c_int32))),
+                self._h)
+        self.nrec = r.shape[0]
+
+    # ---- time loop ----------------------------------------------------------
+    def run(self, nsteps=None):
+        if nsteps is None:
+            nsteps = self.time_steps - self.steps_done
+        failed = C.c_int32(0)
+        rc = L.lib.sdt_run(self._h, int(nsteps), C.byref(failed))
+        L.check(rc, self._h, failed.value)
+
+    @property
+    def steps_done(self):
+        s = C.c_int32(0)
+        L.check(L.lib.sdt_steps_done(self._h, C.byref(s)), self._h)
+        return s.value
+
+    def synchronize(self):
+        L.check(L.lib.sdt_synchronize(self._h), self._h)
+
+    # ---- outputs ------------------------------------------------------------
+    def field(self, name):
+        out = np.empty(self.ncell, dtype=np.float64)
+        L.check(L.lib.sdt_get_field(self._h, self.fields[name], _dptr(out), out.size), self._h)
+        return out.reshape(self.shape, order="F")
+
+    def set_field(self, name, a):
+        flat = np.ascontiguousarray(self._grid_array(a))
+        L.check(L.lib.sdt_set_field(self._h, self.fields[name], _dptr(flat), flat.size), self._h)
+
+    def receivers(self):
+        """(steps_done, n_receivers, 3) array of the velocity / E components."""
+        n = self.steps_done
+        out = np.empty((n, self.nrec, 3), dtype=np.float64)
+        L.check(L.lib.sdt_get_receivers(self._h, _dptr(out), out.size), self._h)
+        return out
+
+    def norms(self):
+        n = self.steps_done
+        v = np.empty(n, dtype=np.float64)
+        s = np.empty(n, dtype=np.float64)
+        L.check(L.lib.sdt_get_norms(self._h, _dptr(v), _dptr(s), n), self._h)
+        return v, s
+
+    # ---- kernel timing --------------------------------------------------------
+    def set_timing(self, on=True):
+        L.check(L.lib.sdt_set_timing(self._h, 1 if on else 0), self._h)
+
+    def kernel_times(self):
+        a, b, n = C.c_double(0), C.c_double(0), C.c_int64(0)
+        L.check(L.lib.sdt_get_kernel_times(self._h, C.byref(a), C.byref(b), C.byref(n)), self._h)
+        return a.value, b.value, n.value
+
+
+def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=()):
+    """Run a synthetic configuration dict (see seidart_hip.models) and return
+    (receivers, norms, {field: array})."""
+    s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"],
+               cfg["time_steps"], cfg["src"], precision=precision,
+               density_method=cfg.get("density_method", "arithmetic"),
+               stencil=cfg.get("stencil", "reference"), device=device)
+    try:
+        s.set_arrays(cfg["arrays"])
+        for comp, series in cfg["sources"].items():
+            s.set_source(comp, series)
+        if len(cfg.get("receivers", ())):
+            s.set_receivers(cfg["receivers"])
+        s.run(steps if steps is not None else cfg["time_steps"])
+        out = {f: s.field(f) for f in fields}
+        return s.receivers(), s.norms(), out
+    finally:
+        s.close()

@@ -1,0 +1,114 @@
+"""GPU parity: libseidart_hip.so (through its C-ABI) against the CPU oracle on
+the same seeded inputs.  FP64 bar: relative L2 <= 1e-10 on receiver traces,
+per-step norms and final fields."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def _run_both(cfg, fields, steps=None, precision="fp64"):
+    from seidart_hip.solver import run_config
+    from oracle.oracle_ctypes import run_oracle
+    print(f"[parity] {cfg['physics']} shape={cfg['shape']} steps={steps or cfg['time_steps']} "
+          f"{precision}", flush=True)
+    g_rec, (g_vn, g_sn), g_f = run_config(cfg, precision=precision, steps=steps, fields=fields)
+    o_rec, (o_vn, o_sn), o_f, us = run_oracle(cfg, steps=steps, fields=fields)
+    assert us == 0
+    return (g_rec, g_vn, g_sn, g_f), (o_rec, o_vn, o_sn, o_f)
+
+
+def _check(g, o, tol, stress_norm=False):
+    g_rec, g_vn, g_sn, g_f = g
+    o_rec, o_vn, o_sn, o_f = o
+    errs = {"receivers": rel_l2(g_rec, o_rec), "velnorm": rel_l2(g_vn, o_vn)}
+    if stress_norm:
+        errs["stressnorm"] = rel_l2(g_sn, o_sn)
+    for k in o_f:
+        errs[k] = rel_l2(g_f[k], o_f[k])
+    print({k: f"{v:.3e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= tol, f"{k}: rel L2 {v:.3e} > {tol:.0e}"
+    return errs
+
+
+def test_elastic2_config0_fp64():
+    """2D elastic, 200x200 (+10 CPML), 1000 steps, two tilted ice layers."""
+    from seidart_hip import models
+    cfg = models.elastic2()
+    g, o = _run_both(cfg, ("vx", "vz", "sxx", "szz", "sxz"))
+    _check(g, o, TOL64)
+
+
+@pytest.mark.parametrize("density", ["harmonic", "geometric", "none"])
+def test_elastic2_density_methods(density):
+    from seidart_hip import models
+    cfg = models.elastic2(nx=80, nz=70, cpml=8, steps=300, src=(40, 20), rec_z=20, nrec=8,
+                          density_method=density)
+    g, o = _run_both(cfg, ("vx", "vz", "sxz"))
+    _check(g, o, TOL64)
+
+
+def test_elastic2_stress_source_and_attenuation():
+    from seidart_hip import models
+    cfg = models.elastic2(nx=80, nz=70, cpml=8, steps=300, src=(40, 20), rec_z=20, nrec=8,
+                          gamma=50.0)
+    s = cfg["sources"]["z"]
+    cfg["sources"] = {"xx": s, "xz": 0.5 * s, "zz": -0.7 * s}
+    g, o = _run_both(cfg, ("vx", "vz", "sxx", "szz", "sxz"))
+    _check(g, o, TOL64)
+
+
+def test_elastic3_small_fp64():
+    from seidart_hip import models
+    cfg = models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16)
+    fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    g, o = _run_both(cfg, fields)
+    _check(g, o, TOL64, stress_norm=True)
+
+
+def test_elastic3_stress_source_harmonic():
+    from seidart_hip import models
+    cfg = models.elastic3(nx=30, ny=28, nz=26, cpml=5, steps=150, src=(15, 14, 9), nrec=8,
+                          density_method="harmonic")
+    s = cfg["sources"]["z"]
+    cfg["sources"] = {"xx": s, "xy": 0.3 * s, "xz": -0.2 * s, "yy": 0.9 * s, "yz": 0.4 * s,
+                      "zz": -s, "x": 0.1 * s, "y": -0.2 * s, "z": 0.3 * s}
+    fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    g, o = _run_both(cfg, fields)
+    _check(g, o, TOL64, stress_norm=True)
+
+
+def test_em2_fp64():
+    from seidart_hip import models
+    cfg = models.em2(nx=120, nz=80, steps=400, src=(60, 20), nrec=16)
+    g, o = _run_both(cfg, ("ex", "ez", "hy"))
+    _check(g, o, TOL64)
+
+
+def test_em3_fp64():
+    from seidart_hip import models
+    cfg = models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)
+    cfg["sources"]["y"] = 0.5 * cfg["sources"]["x"]
+    cfg["sources"]["z"] = -0.25 * cfg["sources"]["x"]
+    g, o = _run_both(cfg, ("ex", "ey", "ez", "hx", "hy", "hz"))
+    _check(g, o, TOL64)
+
+
+@pytest.mark.parametrize("physics", ["elastic2", "elastic3", "em2", "em3"])
+def test_fp32_close_to_fp64(physics):
+    """FP32 wavefields with FP64-precomputed coefficients: trace error vs the
+    FP64 oracle stated per solver (bar 1e-3 relative L2 on these short runs)."""
+    from seidart_hip import models
+    cfg = {"elastic2": lambda: models.elastic2(nx=80, nz=70, cpml=8, steps=300, src=(40, 20), rec_z=20, nrec=8),
+           "elastic3": lambda: models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16),
+           "em2": lambda: models.em2(nx=120, nz=80, steps=400, src=(60, 20), nrec=16),
+           "em3": lambda: models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)}[physics]()
+    g, o = _run_both(cfg, (), precision="fp32")
+    err = rel_l2(g[0], o[0])
+    print(f"[fp32] {physics} receiver rel L2 vs FP64 oracle: {err:.3e}", flush=True)
+    assert err <= 1e-3
