@@ -1,0 +1,158 @@
+"""Throughput of the 3D anisotropic elastic CPML-FDTD solver (seismic3) on one
+MI355X: 256^3 cells + 12-cell CPML (280^3 padded), dx = 0.5 m, eight seeded
+rotated-ice layers with attenuation, Ricker 100 Hz vertical force, 256
+receivers, FP64 (the reference's arithmetic).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32]
+
+One step = one time step over the whole padded grid (stress kernel, velocity
+kernel, source/receiver tail).  Prints one JSON line.  `value` is
+GCell-updates/s over the K timed steps with every input resident in HBM.
+`roofline` prices the dominant kernel (the fused stress kernel) by its
+algorithmic bytes per launch against the 8 TB/s HBM peak, timed with HIP
+events on the solver's stream during the timed region.  `cpu_baseline` runs
+the CPU oracle (a restatement of cpmlfdtd.f08's seismic3 loops, OpenMP) on
+the same model for a few steps on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "seidart-backend_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+# per-cell counts of the algorithmic byte model (fields read+written once,
+# material arrays read once; CPML cells add their live memory recursions read
+# + written and the six coefficient arrays)
+EL3 = dict(F=9, M=28, PSI=29, C=6)
+EL3_STRESS = dict(reads_v=3, rw_sigma=6, mat=27, psi=20, C=6)
+
+
+def algorithmic_bytes(shape, cpml, w):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    n_int = max(nx - 2 * cpml, 0) * max(ny - 2 * cpml, 0) * max(nz - 2 * cpml, 0)
+    n_pml = n - n_int
+    step = w * (n * (2 * EL3["F"] + EL3["M"]) + n_pml * (2 * EL3["PSI"] + EL3["C"]))
+    s = EL3_STRESS
+    stress = w * (n * (s["reads_v"] + 2 * s["rw_sigma"] + s["mat"]) + n_pml * (2 * s["psi"] + s["C"]))
+    return step, stress
+
+
+def measured_traffic():
+    """HBM bytes per stress-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "el3_stress_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--n", type=int, default=256, help="unpadded cells per axis")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps for the CPU baseline (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (one time-stepped grid per GPU); "
+              "--gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from seidart_hip import models
+    from seidart_hip.solver import Solver
+
+    K, W = args.steps, args.warmup
+    n = args.n
+    t_gen = time.perf_counter()
+    cfg = models.elastic3(nx=n, ny=n, nz=n, steps=W + K, src=(n // 2, n // 2, 20), nrec=min(256, n))
+    t_gen = time.perf_counter() - t_gen
+    shape = cfg["shape"]
+    ncell = int(np.prod(shape))
+    w = 8 if args.precision == "fp64" else 4
+
+    s = Solver("elastic3", shape, cfg["spacing"], cfg["cpml"], cfg["dt"], W + K, cfg["src"],
+               precision=args.precision, density_method=cfg["density_method"])
+    s.set_arrays(cfg["arrays"])
+    for comp, series in cfg["sources"].items():
+        s.set_source(comp, series)
+    s.set_receivers(cfg["receivers"])
+    s.run(W)           # first call uploads every input to HBM
+    s.synchronize()
+
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.run(K)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_stress, ms_vel, launches = s.kernel_times()
+    rec = s.receivers()
+    s.close()
+    if not np.all(np.isfinite(rec)):
+        raise RuntimeError("non-finite receiver traces")
+
+    value = ncell * K / elapsed / 1e9
+    step_bytes, stress_bytes = algorithmic_bytes(shape, cfg["cpml"], w)
+    avg_stress_s = ms_stress / max(launches, 1) / 1e3
+    avg_vel_s = ms_vel / max(launches, 1) / 1e3
+    achieved = stress_bytes / avg_stress_s / 1e9
+    traffic = measured_traffic() if args.precision == "fp64" and n == 256 else None
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle.oracle_ctypes import run_oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        t1 = time.perf_counter()
+        run_oracle(cfg, steps=args.cpu_steps, nthreads=threads)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": round(ncell * args.cpu_steps / cpu_s / 1e9, 5), "unit": "GCell-updates/s",
+               "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} time steps of the same {shape[0]}x{shape[1]}x{shape[2]} model "
+                         f"through the C oracle (FP64, OpenMP, {threads} threads), {cpu_s:.1f} s"}
+
+    line = {
+        "metric": "GCell-updates/s (3D elastic CPML-FDTD, FP64)" if args.precision == "fp64"
+                  else "GCell-updates/s (3D elastic CPML-FDTD, FP32)",
+        "value": round(value, 4),
+        "unit": "GCell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(elapsed / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded 8-layer rotated-ice model, Ricker source)",
+        "config": {"workload": f"seismic3 {n}^3 + {cfg['cpml']}-cell CPML ({shape[0]}^3 padded), "
+                               f"dx=0.5 m, 8 layers, attenuation, 256 receivers",
+                   "cells": ncell, "precision": args.precision},
+        "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": "el3_stress_kernel",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": stress_bytes, "avg_launch_ms": round(avg_stress_s * 1e3, 4),
+                     "velocity_kernel_avg_ms": round(avg_vel_s * 1e3, 4)},
+        "cpu_baseline": cpu,
+        "setup_s": round(t_gen, 2),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
