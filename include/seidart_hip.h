@@ -180,6 +180,15 @@ int32_t sdt_get_kernel_times(sdt_handle *h, double *ms_stress, double *ms_veloci
 /* block until all work queued on the handle's stream has finished */
 int32_t sdt_synchronize(sdt_handle *h);
 
+/* tuning options (no reference counterpart); set before the first step */
+enum {
+    SDT_OPT_KERNEL_VARIANT = 0, /* 0 auto, 1 one thread per cell, 2 z-marching columns */
+    SDT_OPT_ZCHUNK = 1,         /* planes per z-marching workgroup (0 = auto)            */
+    SDT_OPT_GRAPH = 2           /* 1 = replay captured hipGraphs of several steps when the
+                                   grid is small (default), 0 = direct launches          */
+};
+int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
+
 #ifdef __cplusplus
 }
 #endif

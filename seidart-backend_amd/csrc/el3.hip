@@ -76,8 +76,10 @@ __device__ __forceinline__ T stress_update(T s, const T *const *C, int64_t c, in
                  C[i5][c] * xy) * dt) * ig;
 }
 
+// Velocity values of one column at planes k-2..k+2 (zq[comp][0..4]); the
+// x and y neighbours are read from memory (L1/L2 hits within a workgroup).
 template <typename T, bool PML>
-__device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, int j, int k)
+__device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, int j, int k, const T (&zq)[3][5])
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
@@ -96,39 +98,40 @@ __device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, in
     for (int q = 0; q < 3; ++q) {
         const T *v = A.v[q];
         T *const *m = A.m + q * K_NKIND;
-        if (L1 || L4) {
-            T x = dplus(v, c, sx) * A.rd24[0];
+        const T v0 = zq[q][2];
+        if (L1 || L4) {   // D+x, half set
+            T x = (T(27) * (v[c + sx] - v0) - (v[c + 2 * sx] - v[c - sx])) * A.rd24[0];
             if (PML) x = cpml_fix3(x, m[K_XPH][c], ah, bh, ikh);
             d[q][K_XPH] = x;
         }
-        if (L2 || L3) {
-            T x = dminus(v, c, sx) * A.rd24[0];
+        if (L2 || L3) {   // D-x, full set
+            T x = (T(27) * (v0 - v[c - sx]) - (v[c + sx] - v[c - 2 * sx])) * A.rd24[0];
             if (PML) x = cpml_fix3(x, m[K_XMF][c], a, b, ik);
             d[q][K_XMF] = x;
         }
-        if (L1 || L3) {
-            T x = dminus(v, c, sy) * A.rd24[1];
+        if (L1 || L3) {   // D-y, full set
+            T x = (T(27) * (v0 - v[c - sy]) - (v[c + sy] - v[c - 2 * sy])) * A.rd24[1];
             if (PML) x = cpml_fix3(x, m[K_YMF][c], a, b, ik);
             d[q][K_YMF] = x;
         }
-        if (L2 || L4) {
-            T x = dplus(v, c, sy) * A.rd24[1];
+        if (L2 || L4) {   // D+y, half set
+            T x = (T(27) * (v[c + sy] - v0) - (v[c + 2 * sy] - v[c - sy])) * A.rd24[1];
             if (PML) x = cpml_fix3(x, m[K_YPH][c], ah, bh, ikh);
             d[q][K_YPH] = x;
         }
-        if (L1) {
-            T x = dminus(v, c, sz) * A.rd24[2];
+        const T zm = (T(27) * (zq[q][2] - zq[q][1]) - (zq[q][3] - zq[q][0])) * A.rd24[2];
+        if (L1) {         // D-z, full set
+            T x = zm;
             if (PML) x = cpml_fix3(x, m[K_ZMF][c], a, b, ik);
             d[q][K_ZMF] = x;
         }
-        if (L2) {
-            // vx's z derivative enters sigma_xy uncorrected (cpmlfdtd.f08:1858-1865)
-            T x = dminus(v, c, sz) * A.rd24[2];
+        if (L2) {         // D-z, half set; vx's enters sigma_xy uncorrected (1858-1865)
+            T x = zm;
             if (PML && q != 0) x = cpml_fix3(x, m[K_ZMH][c], ah, bh, ikh);
             d[q][K_ZMH] = x;
         }
-        if (L3 || L4) {
-            T x = dplus(v, c, sz) * A.rd24[2];
+        if (L3 || L4) {   // D+z, half set
+            T x = (T(27) * (zq[q][3] - zq[q][2]) - (zq[q][4] - zq[q][1])) * A.rd24[2];
             if (PML) x = cpml_fix3(x, m[K_ZPH][c], ah, bh, ikh);
             d[q][K_ZPH] = x;
         }
@@ -168,6 +171,13 @@ __device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, in
 }
 
 template <typename T>
+__device__ __forceinline__ T zload(const T *f, int64_t col, int64_t sz, int k, int nz)
+{
+    return (k >= 1 && k <= nz) ? f[col + sz * (k - 1)] : T(0);
+}
+
+// One thread per cell (64 x 4 x 1 workgroups): every z neighbour read from memory.
+template <typename T>
 __global__ void __launch_bounds__(256) el3_stress_kernel(El3Args<T> A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
@@ -175,28 +185,76 @@ __global__ void __launch_bounds__(256) el3_stress_kernel(El3Args<T> A)
     const int k = blockIdx.z + 1;
     double nrm = 0.0;
     if (i <= A.nx && j <= A.ny) {
+        const int64_t sz = (int64_t)A.nx * A.ny, col = (int64_t)(i - 1) + (int64_t)A.nx * (j - 1);
+        T zq[3][5];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int s = 0; s < 5; ++s) zq[q][s] = zload(A.v[q], col, sz, k - 2 + s, A.nz);
         const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (A.flags[t]) nrm = el3_stress_cell<T, true>(A, i, j, k);
-        else nrm = el3_stress_cell<T, false>(A, i, j, k);
+        if (A.flags[t]) nrm = el3_stress_cell<T, true>(A, i, j, k, zq);
+        else nrm = el3_stress_cell<T, false>(A, i, j, k, zq);
     }
     block_max_to(&A.snorm[*A.d_it - 1], nrm);
 }
 
+// z-marching columns: a 64 x 4 workgroup walks `kc` planes; each thread keeps
+// its column's five velocity planes in registers, so every velocity value is
+// fetched from memory once per sweep (plus the chunk's 4 halo planes).  The
+// CPML choice is made per plane and is uniform across the workgroup.
 template <typename T>
-__device__ __forceinline__ T tminus3(const T *f, int64_t c, int64_t s, int corr)
+__global__ void __launch_bounds__(256) el3_stress_zm_kernel(El3Args<T> A, int kc)
 {
-    return corr ? T(27) * (f[c] - f[c - s]) - (f[c + s] - f[c - 2 * s])
-                : T(27) * f[c] - T(27) * f[c - s] + f[c - 2 * s];
-}
-template <typename T>
-__device__ __forceinline__ T tplus3(const T *f, int64_t c, int64_t s, int corr)
-{
-    return corr ? T(27) * (f[c + s] - f[c]) - (f[c + 2 * s] - f[c - s])
-                : -T(27) * f[c] + T(27) * f[c + s] - f[c + 2 * s];
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k0 = blockIdx.z * kc + 1;
+    const int k1 = min(k0 + kc - 1, A.nz);
+    const bool in = i <= A.nx && j <= A.ny;
+    const int64_t sz = (int64_t)A.nx * A.ny;
+    const int64_t col = in ? (int64_t)(i - 1) + (int64_t)A.nx * (j - 1) : 0;
+    T zq[3][5];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) zq[q][s] = in ? zload(A.v[q], col, sz, k0 - 2 + s, A.nz) : T(0);
+    double nrm = 0.0;
+    const size_t trow = blockIdx.x + (size_t)A.tiles_x * blockIdx.y;
+    const size_t tplane = (size_t)A.tiles_x * A.tiles_y;
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) zq[q][4] = in ? zload(A.v[q], col, sz, k + 2, A.nz) : T(0);
+        if (in) {
+            double r;
+            if (A.flags[trow + tplane * (k - 1)]) r = el3_stress_cell<T, true>(A, i, j, k, zq);
+            else r = el3_stress_cell<T, false>(A, i, j, k, zq);
+            nrm = (r > nrm || r != r) ? r : nrm;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            zq[q][0] = zq[q][1]; zq[q][1] = zq[q][2]; zq[q][2] = zq[q][3]; zq[q][3] = zq[q][4];
+        }
+    }
+    block_max_to(&A.snorm[*A.d_it - 1], nrm);
 }
 
+// 3-term (reference) / 4-term (corrected) velocity differences from five
+// consecutive samples f(-2..+2) (cpmlfdtd.f08:2022-2024, 2063-2066, 2107-2111)
+template <typename T>
+__device__ __forceinline__ T tminus5(T fm2, T fm1, T f0, T fp1, int corr)
+{
+    return corr ? T(27) * (f0 - fm1) - (fp1 - fm2) : T(27) * f0 - T(27) * fm1 + fm2;
+}
+template <typename T>
+__device__ __forceinline__ T tplus5(T fm1, T f0, T fp1, T fp2, int corr)
+{
+    return corr ? T(27) * (fp1 - f0) - (fp2 - fm1) : -T(27) * f0 + T(27) * fp1 - fp2;
+}
+
+// zs[0..2] = sigma_xz, sigma_yz, sigma_zz at planes k-2..k+2 of the column;
+// zr = rho at k-1, k, k+1.
 template <typename T, bool PML>
-__device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, int j, int k)
+__device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, int j, int k, const T (&zs)[3][5],
+                                                    const T (&zr)[3])
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
@@ -207,20 +265,22 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
         A.v[0][c] = vx; A.v[1][c] = vy; A.v[2][c] = vz;
     } else {
         const int dc = A.density_code, corr = A.corrected;
-        const T r = A.rho[c];
+        const T r = zr[1];
         const T ryy = scalar_mean(r, r, dc);
         T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
         if (PML) {
             a = A.a[c]; b = A.b[c]; ik = A.ik[c];
             ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
         }
-        const T *sxx = A.s[0], *syy = A.s[1], *szz = A.s[2], *sxy = A.s[3], *sxz = A.s[4], *syz = A.s[5];
+        const T *sxx = A.s[0], *syy = A.s[1], *sxy = A.s[3], *sxz = A.s[4], *syz = A.s[5];
+        const T sxz0 = zs[0][2], syz0 = zs[1][2], szz0 = zs[2][2];
         if (i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1) {
             // cpmlfdtd.f08:2014-2038
-            const T rx = scalar_mean(r, A.rho[c - sx], dc), rz = scalar_mean(r, A.rho[c - sz], dc);
-            T d1 = tminus3(sxx, c, sx, corr) * A.rd24[0];
-            T d2 = tminus3(sxy, c, sy, corr) * A.rd24[1];
-            T d3 = tminus3(sxz, c, sz, corr) * A.rd24[2];
+            const T rx = scalar_mean(r, A.rho[c - sx], dc), rz = scalar_mean(r, zr[0], dc);
+            const T sxy0 = sxy[c];
+            T d1 = tminus5(sxx[c - 2 * sx], sxx[c - sx], sxx[c], corr ? sxx[c + sx] : T(0), corr) * A.rd24[0];
+            T d2 = tminus5(sxy[c - 2 * sy], sxy[c - sy], sxy0, corr ? sxy[c + sy] : T(0), corr) * A.rd24[1];
+            T d3 = tminus5(zs[0][0], zs[0][1], sxz0, zs[0][3], corr) * A.rd24[2];
             if (PML) {
                 d1 = cpml_fix3(d1, A.m[M_SXX_X][c], ah, bh, ikh);
                 d2 = cpml_fix3(d2, A.m[M_SXY_Y][c], ah, bh, ikh);
@@ -231,10 +291,10 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
         }
         if (i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1) {
             // cpmlfdtd.f08:2055-2080
-            const T rx = scalar_mean(r, A.rho[c + sx], dc), rz = scalar_mean(r, A.rho[c - sz], dc);
-            T d1 = tplus3(sxy, c, sx, corr) * A.rd24[0];
-            T d2 = tplus3(syy, c, sy, corr) * A.rd24[1];
-            T d3 = tminus3(syz, c, sz, corr) * A.rd24[2];
+            const T rx = scalar_mean(r, A.rho[c + sx], dc), rz = scalar_mean(r, zr[0], dc);
+            T d1 = tplus5(corr ? sxy[c - sx] : T(0), sxy[c], sxy[c + sx], sxy[c + 2 * sx], corr) * A.rd24[0];
+            T d2 = tplus5(corr ? syy[c - sy] : T(0), syy[c], syy[c + sy], syy[c + 2 * sy], corr) * A.rd24[1];
+            T d3 = tminus5(zs[1][0], zs[1][1], syz0, zs[1][3], corr) * A.rd24[2];
             if (PML) {
                 d1 = cpml_fix3(d1, A.m[M_SXY_X][c], ah, bh, ikh);
                 d2 = cpml_fix3(d2, A.m[M_SYY_Y][c], ah, bh, ikh);
@@ -245,10 +305,10 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
         }
         if (i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2) {
             // cpmlfdtd.f08:2098-2127
-            const T rx = scalar_mean(r, A.rho[c + sx], dc), rz = scalar_mean(r, A.rho[c + sz], dc);
-            T d1 = tplus3(sxz, c, sx, corr) * A.rd24[0];
-            T d2 = tminus3(syz, c, sy, corr) * A.rd24[1];
-            T d3 = tplus3(szz, c, sz, corr) * A.rd24[2];
+            const T rx = scalar_mean(r, A.rho[c + sx], dc), rz = scalar_mean(r, zr[2], dc);
+            T d1 = tplus5(corr ? sxz[c - sx] : T(0), sxz0, sxz[c + sx], sxz[c + 2 * sx], corr) * A.rd24[0];
+            T d2 = tminus5(syz[c - 2 * sy], syz[c - sy], syz0, corr ? syz[c + sy] : T(0), corr) * A.rd24[1];
+            T d3 = tplus5(zs[2][1], szz0, zs[2][3], zs[2][4], corr) * A.rd24[2];
             if (PML) {
                 d1 = cpml_fix3(d1, A.m[M_SXZ_X][c], ah, bh, ikh);
                 d2 = cpml_fix3(d2, A.m[M_SYZ_Y][c], a, b, ik);
@@ -271,9 +331,59 @@ __global__ void __launch_bounds__(256) el3_velocity_kernel(El3Args<T> A)
     const int k = blockIdx.z + 1;
     double nrm = 0.0;
     if (i <= A.nx && j <= A.ny) {
+        const int64_t sz = (int64_t)A.nx * A.ny, col = (int64_t)(i - 1) + (int64_t)A.nx * (j - 1);
+        T zs[3][5], zr[3];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            zs[0][s] = zload(A.s[4], col, sz, k - 2 + s, A.nz);
+            zs[1][s] = zload(A.s[5], col, sz, k - 2 + s, A.nz);
+            zs[2][s] = zload(A.s[2], col, sz, k - 2 + s, A.nz);
+        }
+#pragma unroll
+        for (int s = 0; s < 3; ++s) zr[s] = zload(A.rho, col, sz, k - 1 + s, A.nz);
         const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (A.flags[t]) nrm = el3_velocity_cell<T, true>(A, i, j, k);
-        else nrm = el3_velocity_cell<T, false>(A, i, j, k);
+        if (A.flags[t]) nrm = el3_velocity_cell<T, true>(A, i, j, k, zs, zr);
+        else nrm = el3_velocity_cell<T, false>(A, i, j, k, zs, zr);
+    }
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) el3_velocity_zm_kernel(El3Args<T> A, int kc)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int j = blockIdx.y * 4 + threadIdx.y + 1;
+    const int k0 = blockIdx.z * kc + 1;
+    const int k1 = min(k0 + kc - 1, A.nz);
+    const bool in = i <= A.nx && j <= A.ny;
+    const int64_t sz = (int64_t)A.nx * A.ny;
+    const int64_t col = in ? (int64_t)(i - 1) + (int64_t)A.nx * (j - 1) : 0;
+    const T *zsrc[3] = {A.s[4], A.s[5], A.s[2]};
+    T zs[3][5], zr[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) zs[q][s] = in ? zload(zsrc[q], col, sz, k0 - 2 + s, A.nz) : T(0);
+    zr[0] = in ? zload(A.rho, col, sz, k0 - 1, A.nz) : T(0);
+    zr[1] = in ? zload(A.rho, col, sz, k0, A.nz) : T(0);
+    double nrm = 0.0;
+    const size_t trow = blockIdx.x + (size_t)A.tiles_x * blockIdx.y;
+    const size_t tplane = (size_t)A.tiles_x * A.tiles_y;
+    for (int k = k0; k <= k1; ++k) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) zs[q][4] = in ? zload(zsrc[q], col, sz, k + 2, A.nz) : T(0);
+        zr[2] = in ? zload(A.rho, col, sz, k + 1, A.nz) : T(0);
+        if (in) {
+            double r;
+            if (A.flags[trow + tplane * (k - 1)]) r = el3_velocity_cell<T, true>(A, i, j, k, zs, zr);
+            else r = el3_velocity_cell<T, false>(A, i, j, k, zs, zr);
+            nrm = (r > nrm || r != r) ? r : nrm;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            zs[q][0] = zs[q][1]; zs[q][1] = zs[q][2]; zs[q][2] = zs[q][3]; zs[q][3] = zs[q][4];
+        }
+        zr[0] = zr[1]; zr[1] = zr[2];
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
@@ -315,11 +425,16 @@ template <typename T>
 static void el3_step_t(Device &d, bool timed)
 {
     const El3Args<T> A = el3_args<T>(d);
-    const dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z), block(64, 4, 1);
+    const dim3 block(64, 4, 1);
+    const bool zm = d.variant != 1;
+    const int kc = d.zchunk > 0 ? d.zchunk : 32;
+    const dim3 grid(d.tiles_x, d.tiles_y, zm ? (d.nz + kc - 1) / kc : d.tiles_z);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    hipLaunchKernelGGL(el3_stress_kernel<T>, grid, block, 0, d.stream, A);
+    if (zm) hipLaunchKernelGGL(el3_stress_zm_kernel<T>, grid, block, 0, d.stream, A, kc);
+    else hipLaunchKernelGGL(el3_stress_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL(el3_velocity_kernel<T>, grid, block, 0, d.stream, A);
+    if (zm) hipLaunchKernelGGL(el3_velocity_zm_kernel<T>, grid, block, 0, d.stream, A, kc);
+    else hipLaunchKernelGGL(el3_velocity_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }

@@ -330,7 +330,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
     if (rc) return rc;
     Device &d = h->d;
     HIPCHK(h, hipSetDevice(h->p.device));
-    const bool use_graph = !d.timing && d.ncell <= kGraphMaxCells;
+    const bool use_graph = d.graphs && !d.timing && d.ncell <= kGraphMaxCells;
     const int chunk = 256;   // steps between blow-up checks
     int left = nsteps;
     while (left > 0) {
@@ -478,6 +478,27 @@ int32_t sdt_get_kernel_times(sdt_handle *h, double *ms_stress, double *ms_veloci
     if (ms_velocity) *ms_velocity = h->d.ms_vel;
     if (launches) *launches = h->d.timed_launches;
     return SDT_OK;
+}
+
+int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    switch (key) {
+    case SDT_OPT_KERNEL_VARIANT:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "kernel variant is fixed once the first step has run");
+        if (value < 0 || value > 2) return fail(h, SDT_ERR_ARG, "kernel variant must be 0, 1 or 2");
+        h->d.variant = (int)value;
+        return SDT_OK;
+    case SDT_OPT_ZCHUNK:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "z chunk is fixed once the first step has run");
+        if (value < 0 || value > 1 << 20) return fail(h, SDT_ERR_ARG, "z chunk out of range");
+        h->d.zchunk = (int)value;
+        return SDT_OK;
+    case SDT_OPT_GRAPH:
+        h->d.graphs = value != 0;
+        return SDT_OK;
+    }
+    return fail(h, SDT_ERR_ARG, "unknown option");
 }
 
 int32_t sdt_synchronize(sdt_handle *h)

@@ -76,6 +76,9 @@ struct Device {
     int nrec = 0;
 
     TailArgs tail{};
+    int variant = 0;                   // SDT_OPT_KERNEL_VARIANT
+    int zchunk = 0;                    // SDT_OPT_ZCHUNK
+    bool graphs = true;                // SDT_OPT_GRAPH
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     bool timing = false;

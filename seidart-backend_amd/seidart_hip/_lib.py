@@ -63,8 +63,10 @@ EXPORTS = [
     "sdt_version", "sdt_device_count", "sdt_last_error", "sdt_create", "sdt_destroy",
     "sdt_set_array", "sdt_set_source", "sdt_set_receivers", "sdt_run", "sdt_steps_done",
     "sdt_get_field", "sdt_set_field", "sdt_get_receivers", "sdt_get_norms", "sdt_set_timing",
-    "sdt_get_kernel_times", "sdt_synchronize",
+    "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option",
 ]
+
+OPT_KERNEL_VARIANT, OPT_ZCHUNK, OPT_GRAPH = 0, 1, 2
 
 
 class SdtError(RuntimeError):
@@ -107,6 +109,7 @@ def _load():
         "sdt_set_timing": ([P, I32], I32),
         "sdt_get_kernel_times": ([P, D, D, C.POINTER(I64)], I32),
         "sdt_synchronize": ([P], I32),
+        "sdt_set_option": ([P, I32, I64], I32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

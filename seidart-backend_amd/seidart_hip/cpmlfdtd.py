@@ -1,0 +1,191 @@
+"""File-based drop-in of the reference solvers.
+
+    seismic2(domain, source, density_method, SINGLE_OUTPUT)     cpmlfdtd.f08:57
+    seismic3(domain, source, density_method, SINGLE_OUTPUT)     cpmlfdtd.f08:1341
+    electromag2(domain, source, SINGLE_OUTPUT)                  cpmlfdtd.f08:2317
+    electromag3(domain, source, SINGLE_OUTPUT)                  cpmlfdtd.f08:3353
+
+Same argument order and meaning as the Fortran subroutines: `domain` holds
+padded sizes (src/fortran/main.f08:69-71 pads before the call), every input
+is read from the working directory under the reference's file names, and a
+snapshot of every velocity / E component is written each step under the
+reference's names and formats.  3D elastic also writes velocity_norm.dat and
+stress_norm.dat (cpmlfdtd.f08:2282-2283).
+
+Differences from the reference, by design:
+* blow-up raises seidart_hip.UnstableError (the reference prints and
+  `stop`s with exit status 0, cpmlfdtd.f08:567);
+* electromag3 writes its 3D snapshots with the write_image3 format; the
+  reference's call to write_image stops at the end of step 1 for 3D grids
+  (seidartio.f08:315-324);
+* plane-wave sources ('pw', 'pws', 'pwp', ...) are not supported yet and raise
+  NotImplementedError.
+"""
+from __future__ import annotations
+
+import dataclasses
+import os
+
+import numpy as np
+
+from . import fortran_io as fio
+from .solver import Solver
+
+PLANE_WAVE_TYPES = {"pw", "pws", "pw_s", "pw-s", "pwp", "pw_p", "pw-p"}   # plane_wave_source.f08:20-27
+
+
+@dataclasses.dataclass
+class Domain:
+    """Domain_Type, src/fortran/seidart_types.f08:13-21."""
+    dim: float
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    cpml: int
+    nmats: int = 0
+    image_file: str = ""
+
+
+@dataclasses.dataclass
+class Source:
+    """Source_Type, src/fortran/seidart_types.f08:23-33."""
+    dt: float
+    time_steps: int
+    x: float = 0.0
+    y: float = 0.0
+    z: float = 0.0
+    xind: int = 1
+    yind: int = 1
+    zind: int = 1
+    source_frequency: float = 0.0
+    x_z_rotation: float = 0.0
+    x_y_rotation: float = 0.0
+    y_z_rotation: float = 0.0
+    amplitude: float = 1.0
+    source_type: str = ""
+    source_wavelet: str = ""
+
+
+def _read(workdir, name, shape):
+    return fio.read_array(os.path.join(workdir, name + ".dat"), shape)
+
+
+def _series(workdir, name, n):
+    return fio.read_record(os.path.join(workdir, name + ".dat"), np.float64, n)
+
+
+def _check_source_type(source):
+    st = (source.source_type or "").strip()
+    if st in PLANE_WAVE_TYPES:
+        raise NotImplementedError("plane-wave sources are not implemented in seidart_hip yet")
+    return st
+
+
+def _cpml_inputs(s, workdir, shape):
+    for stem in ("kappa", "alpha", "acoef", "bcoef"):
+        for sfx in ("_cpml", "_half_cpml"):
+            s.set_array(stem + sfx, _read(workdir, stem + sfx, shape))
+
+
+def _snapshots(s, workdir, source, it, names, single, is3d):
+    for fname, channel in names:
+        a = s.field(fname)
+        path = os.path.join(workdir, fio.snapshot_name(channel, it, source.xind, source.zind,
+                                                       source.yind if is3d else None))
+        fio.write_record(path, a, np.float32 if single else np.float64)
+
+
+def _time_loop(s, workdir, source, names, single, is3d, snapshot_every):
+    for it in range(1, source.time_steps + 1):
+        s.run(1)
+        if snapshot_every and it % snapshot_every == 0:
+            _snapshots(s, workdir, source, it, names, single, is3d)
+
+
+def seismic2(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *, workdir=".",
+             precision="fp64", device=0, snapshot_every=1):
+    """2D elastic; inputs c11 c13 c15 c33 c35 c55 rho gamma_{x,z,xz} CPML
+    initialconditionV{x,z} and seismicsource{x,z} ('ac') or
+    seismicsource{xx,xz,zz} (cpmlfdtd.f08:159-266); writes Vx/Vz snapshots."""
+    st = _check_source_type(source)
+    shape = (domain.nx, domain.nz)
+    with Solver("elastic2", shape, (domain.dx, domain.dz), domain.cpml, source.dt, source.time_steps,
+                (source.xind, source.zind), precision=precision, density_method=density_method.strip(),
+                device=device) as s:
+        for name in ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "gamma_x", "gamma_z", "gamma_xz"):
+            s.set_array(name, _read(workdir, name, shape))
+        _cpml_inputs(s, workdir, shape)
+        s.set_array("ic_x", _read(workdir, "initialconditionVx", shape))
+        s.set_array("ic_z", _read(workdir, "initialconditionVz", shape))
+        comps = ("x", "z") if st == "ac" else ("xx", "xz", "zz")
+        for c in comps:
+            s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
+        _time_loop(s, workdir, source, (("vx", "Vx"), ("vz", "Vz")), SINGLE_OUTPUT, False, snapshot_every)
+
+
+def seismic3(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *, workdir=".",
+             precision="fp64", device=0, snapshot_every=1):
+    """3D elastic; inputs c11..c66 rho gamma_* CPML initialconditionV{x,y,z}
+    and seismicsource{x,y,z} ('ac') or seismicsource{xx,xy,xz,yy,yz,zz}
+    (cpmlfdtd.f08:1501-1645); writes Vx/Vy/Vz snapshots and the norm files."""
+    st = _check_source_type(source)
+    shape = (domain.nx, domain.ny, domain.nz)
+    with Solver("elastic3", shape, (domain.dx, domain.dy, domain.dz), domain.cpml, source.dt,
+                source.time_steps, (source.xind, source.yind, source.zind), precision=precision,
+                density_method=density_method.strip(), device=device) as s:
+        for name in ("c11", "c12", "c13", "c14", "c15", "c16", "c22", "c23", "c24", "c25", "c26",
+                     "c33", "c34", "c35", "c36", "c44", "c45", "c46", "c55", "c56", "c66", "rho",
+                     "gamma_x", "gamma_z", "gamma_y", "gamma_xz", "gamma_yz", "gamma_xy"):
+            s.set_array(name, _read(workdir, name, shape))
+        _cpml_inputs(s, workdir, shape)
+        for c in "xyz":
+            s.set_array("ic_" + c, _read(workdir, "initialconditionV" + c, shape))
+        comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
+        for c in comps:
+            s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
+        _time_loop(s, workdir, source, (("vx", "Vx"), ("vy", "Vy"), ("vz", "Vz")), SINGLE_OUTPUT, True,
+                   snapshot_every)
+        vn, sn = s.norms()
+        fio.write_record(os.path.join(workdir, "velocity_norm.dat"), vn)
+        fio.write_record(os.path.join(workdir, "stress_norm.dat"), sn)
+
+
+def electromag2(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="fp64", device=0,
+                snapshot_every=1):
+    """2D EM; inputs e11 e13 e33 s11 s13 s33 CPML initialconditionE{x,z} and
+    electromagneticsource{x,z} (cpmlfdtd.f08:2413-2506); writes Ex/Ez."""
+    _check_source_type(source)
+    shape = (domain.nx, domain.nz)
+    with Solver("em2", shape, (domain.dx, domain.dz), domain.cpml, source.dt, source.time_steps,
+                (source.xind, source.zind), precision=precision, device=device) as s:
+        for name in ("e11", "e13", "e33", "s11", "s13", "s33"):
+            s.set_array(name, _read(workdir, name, shape))
+        _cpml_inputs(s, workdir, shape)
+        s.set_array("ic_x", _read(workdir, "initialconditionEx", shape))
+        s.set_array("ic_z", _read(workdir, "initialconditionEz", shape))
+        for c in ("x", "z"):
+            s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
+        _time_loop(s, workdir, source, (("ex", "Ex"), ("ez", "Ez")), SINGLE_OUTPUT, False, snapshot_every)
+
+
+def electromag3(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="fp64", device=0,
+                snapshot_every=1):
+    """3D EM; inputs e11..e33 s11..s33 CPML initialconditionE{x,y,z} and
+    electromagneticsource{x,y,z} (cpmlfdtd.f08:3481-3618); writes Ex/Ey/Ez."""
+    _check_source_type(source)
+    shape = (domain.nx, domain.ny, domain.nz)
+    with Solver("em3", shape, (domain.dx, domain.dy, domain.dz), domain.cpml, source.dt,
+                source.time_steps, (source.xind, source.yind, source.zind), precision=precision,
+                device=device) as s:
+        for name in ("e11", "e12", "e13", "e22", "e23", "e33", "s11", "s12", "s13", "s22", "s23", "s33"):
+            s.set_array(name, _read(workdir, name, shape))
+        _cpml_inputs(s, workdir, shape)
+        for c in "xyz":
+            s.set_array("ic_" + c, _read(workdir, "initialconditionE" + c, shape))
+        for c in "xyz":
+            s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
+        _time_loop(s, workdir, source, (("ex", "Ex"), ("ey", "Ey"), ("ez", "Ez")), SINGLE_OUTPUT, True,
+                   snapshot_every)

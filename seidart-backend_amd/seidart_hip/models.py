@@ -329,3 +329,64 @@ def em2(nx=600, nz=300, d=0.05, cpml=10, steps=3000, seed=4, nlayers=6, f0=100e6
     return dict(physics="em2", shape=shape, spacing=(d, d), cpml=cpml, dt=dt, time_steps=steps,
                 src=src, arrays=arrays, sources={"x": ricker(f0, dt, steps), "z": np.zeros(steps)},
                 receivers=recs, vmax=vmax)
+
+
+# ------------------------------------------------------ reference file set ----
+_IC_PREFIX = {"elastic2": "initialconditionV", "elastic3": "initialconditionV",
+              "em2": "initialconditionE", "em3": "initialconditionE"}
+
+
+def write_inputs(cfg, workdir, source_type="ac"):
+    """Write a configuration as the reference's input files (one Fortran
+    record each, padded, column-major) plus input.json, as the SeidarT
+    frontend does before running seidartfdtd."""
+    import json
+    import os
+
+    from . import fortran_io as fio
+
+    phys = cfg["physics"]
+    is3d = phys.endswith("3")
+    os.makedirs(workdir, exist_ok=True)
+    shape = cfg["shape"]
+    defaults = no_cpml(shape)
+    for name, a in {**defaults, **cfg["arrays"]}.items():
+        if name.startswith("ic_"):
+            fname = _IC_PREFIX[phys] + name[-1]
+        else:
+            fname = name
+        fio.write_record(os.path.join(workdir, fname + ".dat"), np.asarray(a, dtype=np.float64).reshape(shape))
+    comps = "xyz" if is3d else "xz"
+    for c in comps:
+        if "ic_" + c not in cfg["arrays"]:
+            fio.write_record(os.path.join(workdir, _IC_PREFIX[phys] + c + ".dat"), np.zeros(shape))
+    n = cfg["time_steps"]
+    stem = "seismicsource" if phys.startswith("elastic") else "electromagneticsource"
+    names = (list(comps) if (source_type == "ac" or not phys.startswith("elastic"))
+             else (["xx", "xy", "xz", "yy", "yz", "zz"] if is3d else ["xx", "xz", "zz"]))
+    for c in names:
+        s = np.asarray(cfg["sources"].get(c, np.zeros(n)), dtype=np.float64)
+        fio.write_record(os.path.join(workdir, stem + c + ".dat"), s)
+    cp = cfg["cpml"]
+    sp = cfg["spacing"]
+    if is3d:
+        nx, ny, nz = (s_ - 2 * cp for s_ in shape)
+        dx, dy, dz = sp
+        xind, yind, zind = cfg["src"]
+    else:
+        nx, nz = (s_ - 2 * cp for s_ in shape)
+        ny, dy = 1, 1.0
+        dx, dz = sp
+        xind, zind = cfg["src"]
+        yind = 1
+    src = {"x": 0.0, "y": 0.0, "z": 0.0, "xind": int(xind), "yind": int(yind), "zind": int(zind),
+           "source_frequency": 0.0, "x-z_rotation": 0.0, "x-y_rotation": 0.0, "y-z_rotation": 0.0,
+           "amplitude": 1.0, "source_type": source_type, "source_wavelet": "gaus2",
+           "dt": float(cfg["dt"]), "time_steps": int(n)}
+    doc = {"Domain": {"dim": 3.0 if is3d else 2.0, "nx": int(nx), "ny": int(ny), "nz": int(nz),
+                      "dx": float(dx), "dy": float(dy), "dz": float(dz), "cpml": int(cp), "nmats": 1,
+                      "image_file": "model.png"},
+           "Seismic": {"Source": dict(src)}, "Electromagnetic": {"Source": dict(src)}}
+    with open(os.path.join(workdir, "input.json"), "w") as f:
+        json.dump(doc, f, indent=2)
+    return os.path.join(workdir, "input.json")

@@ -154,6 +154,13 @@ class Solver:
         L.check(L.lib.sdt_get_norms(self._h, _dptr(v), _dptr(s), n), self._h)
         return v, s
 
+    # ---- tuning -----------------------------------------------------------------
+    def set_option(self, key, value):
+        """key: 'variant' (0 auto, 1 one thread per cell, 2 z-marching),
+        'zchunk' (planes per workgroup, 0 auto) or 'graph' (0/1)."""
+        k = {"variant": L.OPT_KERNEL_VARIANT, "zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH}[key]
+        L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
+
     # ---- kernel timing --------------------------------------------------------
     def set_timing(self, on=True):
         L.check(L.lib.sdt_set_timing(self._h, 1 if on else 0), self._h)
@@ -164,7 +171,7 @@ class Solver:
         return a.value, b.value, n.value
 
 
-def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=()):
+def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=(), options=None):
     """Run a synthetic configuration dict (see seidart_hip.models) and return
     (receivers, norms, {field: array})."""
     s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"],
@@ -172,6 +179,8 @@ def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=()):
                density_method=cfg.get("density_method", "arithmetic"),
                stencil=cfg.get("stencil", "reference"), device=device)
     try:
+        for k, v in (options or {}).items():
+            s.set_option(k, v)
         s.set_arrays(cfg["arrays"])
         for comp, series in cfg["sources"].items():
             s.set_source(comp, series)

@@ -11,12 +11,13 @@ pytestmark = pytest.mark.gpu
 TOL64 = 1e-10
 
 
-def _run_both(cfg, fields, steps=None, precision="fp64"):
+def _run_both(cfg, fields, steps=None, precision="fp64", options=None):
     from seidart_hip.solver import run_config
     from oracle.oracle_ctypes import run_oracle
     print(f"[parity] {cfg['physics']} shape={cfg['shape']} steps={steps or cfg['time_steps']} "
-          f"{precision}", flush=True)
-    g_rec, (g_vn, g_sn), g_f = run_config(cfg, precision=precision, steps=steps, fields=fields)
+          f"{precision} {options or ''}", flush=True)
+    g_rec, (g_vn, g_sn), g_f = run_config(cfg, precision=precision, steps=steps, fields=fields,
+                                          options=options)
     o_rec, (o_vn, o_sn), o_f, us = run_oracle(cfg, steps=steps, fields=fields)
     assert us == 0
     return (g_rec, g_vn, g_sn, g_f), (o_rec, o_vn, o_sn, o_f)
@@ -63,12 +64,39 @@ def test_elastic2_stress_source_and_attenuation():
     _check(g, o, TOL64)
 
 
-def test_elastic3_small_fp64():
+@pytest.mark.parametrize("options", [{"variant": 1}, {"variant": 2}, {"variant": 2, "zchunk": 5},
+                                     {"variant": 2, "zchunk": 1}])
+def test_elastic3_small_fp64(options):
+    """Both kernel variants (one thread per cell; z-marching columns with
+    several chunk heights, so chunk seams fall inside and outside the CPML)."""
     from seidart_hip import models
     cfg = models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16)
     fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    g, o = _run_both(cfg, fields, options=options)
+    _check(g, o, TOL64, stress_norm=True)
+
+
+def test_elastic3_corrected_stencil():
+    from seidart_hip import models
+    cfg = models.elastic3(nx=30, ny=26, nz=28, cpml=5, steps=120, src=(15, 13, 9), nrec=8)
+    cfg["stencil"] = "corrected"
+    fields = ("vx", "vy", "vz", "sxx", "syz")
     g, o = _run_both(cfg, fields)
     _check(g, o, TOL64, stress_norm=True)
+
+
+def test_elastic3_interior_and_cpml_paths_agree():
+    """A vanishing but non-zero CPML coefficient everywhere forces every tile
+    through the CPML body; results must equal the face-specialised run."""
+    from seidart_hip import models
+    from seidart_hip.solver import run_config
+    cfg = models.elastic3(nx=30, ny=28, nz=26, cpml=5, steps=80, src=(15, 14, 9), nrec=8)
+    r1, _, f1 = run_config(cfg, fields=("vx", "szz"))
+    a = cfg["arrays"]["acoef_cpml"].copy()
+    a[a == 0] = 1e-300
+    cfg["arrays"]["acoef_cpml"] = a
+    r2, _, f2 = run_config(cfg, fields=("vx", "szz"))
+    assert rel_l2(r2, r1) < 1e-14 and rel_l2(f2["vx"], f1["vx"]) < 1e-14
 
 
 def test_elastic3_stress_source_harmonic():
@@ -81,6 +109,26 @@ def test_elastic3_stress_source_harmonic():
     fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
     g, o = _run_both(cfg, fields)
     _check(g, o, TOL64, stress_norm=True)
+
+
+def test_elastic2_corrected_stencil():
+    from seidart_hip import models
+    cfg = models.elastic2(nx=80, nz=70, cpml=8, steps=300, src=(40, 20), rec_z=20, nrec=8)
+    cfg["stencil"] = "corrected"
+    g, o = _run_both(cfg, ("vx", "vz", "sxz"))
+    _check(g, o, TOL64)
+
+
+def test_unstable_run_reports_step():
+    from seidart_hip import models, UnstableError
+    from seidart_hip.solver import run_config
+    cfg = models.elastic3(nx=20, ny=20, nz=20, cpml=4, steps=10, src=(10, 10, 8), nrec=2)
+    ic = np.zeros(cfg["shape"], order="F")
+    ic[12, 12, 12] = 1e26
+    cfg["arrays"]["ic_y"] = ic
+    with pytest.raises(UnstableError) as e:
+        run_config(cfg)
+    assert e.value.step == 1
 
 
 def test_em2_fp64():
