@@ -69,6 +69,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps for the CPU baseline (0 = skip)")
     ap.add_argument("--variant", type=int, default=0, help="0 auto, 1 one thread per cell, 2 z-marching")
     ap.add_argument("--zchunk", type=int, default=0, help="planes per z-marching workgroup (0 = auto)")
+    ap.add_argument("--stagger", type=int, default=1, help="staggered array starts (1) or plain (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one time-stepped grid per GPU); "
@@ -92,6 +93,7 @@ def main():
                precision=args.precision, density_method=cfg["density_method"])
     s.set_option("variant", args.variant)
     s.set_option("zchunk", args.zchunk)
+    s.set_option("stagger", args.stagger)
     s.set_arrays(cfg["arrays"])
     for comp, series in cfg["sources"].items():
         s.set_source(comp, series)
@@ -146,7 +148,7 @@ def main():
         "config": {"workload": f"seismic3 {n}^3 + {cfg['cpml']}-cell CPML ({shape[0]}^3 padded), "
                                f"dx=0.5 m, 8 layers, attenuation, 256 receivers",
                    "cells": ncell, "precision": args.precision, "variant": args.variant,
-                   "zchunk": args.zchunk},
+                   "zchunk": args.zchunk, "stagger": args.stagger},
         "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": "el3_stress_kernel",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -22,6 +22,7 @@ struct El2Args {
     const T *a, *b, *ik, *ah, *bh, *ikh;
     T *m[12];
     const uint8_t *flags;
+    int box[6];
     int nx, nz, tiles_x;
     T rdx24, rdz24, dt;
     int density_code, corrected;
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(256) el2_stress_kernel(El2Args<T> A)
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     if (i > A.nx || j > A.nz) return;
-    if (A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) el2_stress_cell<T, true>(A, i, j);
+    if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) el2_stress_cell<T, true>(A, i, j);
     else el2_stress_cell<T, false>(A, i, j);
 }
 
@@ -159,7 +160,7 @@ __global__ void __launch_bounds__(256) el2_velocity_kernel(El2Args<T> A)
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     double nrm = 0.0;
     if (i <= A.nx && j <= A.nz) {
-        if (A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = el2_velocity_cell<T, true>(A, i, j);
+        if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = el2_velocity_cell<T, true>(A, i, j);
         else nrm = el2_velocity_cell<T, false>(A, i, j);
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
@@ -184,6 +185,7 @@ static El2Args<T> el2_args(Device &d)
     A.ikh = (const T *)d.cpml[5];
     for (int q = 0; q < 12; ++q) A.m[q] = (T *)d.mem[q];
     A.flags = d.tile_flags;
+    for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx;
     A.nz = d.nz;
     A.tiles_x = d.tiles_x;

@@ -19,62 +19,9 @@
 
 #include <vector>
 
-#include "sdt_device.h"
-#include "sdt_internal.h"
+#include "el3_common.h"
 
 namespace sdt {
-
-// stress-side recursion slots: m[comp * 7 + kind]
-enum { K_XPH = 0, K_XMF, K_YMF, K_YPH, K_ZMF, K_ZMH, K_ZPH, K_NKIND };
-// velocity-side slots follow: 21 + q
-enum { M_SXX_X = 21, M_SXY_Y, M_SXZ_Z, M_SXY_X, M_SYY_Y, M_SYZ_Z, M_SXZ_X, M_SYZ_Y, M_SZZ_Z, M_TOTAL };
-
-template <typename T>
-struct El3Args {
-    T *v[3];        // vx vy vz
-    T *s[6];        // sxx syy szz sxy sxz syz
-    const T *C[21]; // c11 c12 c13 c14 c15 c16 c22 c23 c24 c25 c26 c33 c34 c35 c36 c44 c45 c46 c55 c56 c66
-    const T *rho;
-    const T *ig[6]; // 1/(1+gamma*dt): x y z xy xz yz
-    const T *a, *b, *ik, *ah, *bh, *ikh;
-    T *m[M_TOTAL];
-    const uint8_t *flags;
-    int nx, ny, nz, tiles_x, tiles_y;
-    T rd24[3];      // 1/(24 dx), 1/(24 dy), 1/(24 dz)
-    T dt;
-    int density_code, corrected;
-    const int *d_it;
-    unsigned long long *vnorm, *snorm;
-    int64_t vfix[3], sfix[4];
-};
-
-template <typename T>
-__device__ __forceinline__ T cpml_fix3(T d, T &psi, T a, T b, T ik)
-{
-    psi = b * psi + a * d;
-    return d * ik + psi;
-}
-
-// forward / backward 4th-order differences (cpmlfdtd.f08:1765-1775)
-template <typename T>
-__device__ __forceinline__ T dplus(const T *f, int64_t c, int64_t s)
-{
-    return T(27) * (f[c + s] - f[c]) - (f[c + 2 * s] - f[c - s]);
-}
-template <typename T>
-__device__ __forceinline__ T dminus(const T *f, int64_t c, int64_t s)
-{
-    return T(27) * (f[c] - f[c - s]) - (f[c + s] - f[c - 2 * s]);
-}
-
-template <typename T>
-__device__ __forceinline__ T stress_update(T s, const T *const *C, int64_t c, int i0, int i1, int i2, int i3,
-                                           int i4, int i5, T dx_, T dy_, T dz_, T yz, T xz, T xy, T dt,
-                                           T ig)
-{
-    return (s + (C[i0][c] * dx_ + C[i1][c] * dy_ + C[i2][c] * dz_ + C[i3][c] * yz + C[i4][c] * xz +
-                 C[i5][c] * xy) * dt) * ig;
-}
 
 // Velocity values of one column at planes k-2..k+2 (zq[comp][0..4]); the
 // x and y neighbours are read from memory (L1/L2 hits within a workgroup).
@@ -91,8 +38,8 @@ __device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, in
     T d[3][K_NKIND];
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
     if (PML && (L1 || L2 || L3 || L4)) {
-        a = A.a[c]; b = A.b[c]; ik = A.ik[c];
-        ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+        a = ldnt(&A.a[c]); b = ldnt(&A.b[c]); ik = ldnt(&A.ik[c]);
+        ah = ldnt(&A.ah[c]); bh = ldnt(&A.bh[c]); ikh = ldnt(&A.ikh[c]);
     }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -101,69 +48,70 @@ __device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, in
         const T v0 = zq[q][2];
         if (L1 || L4) {   // D+x, half set
             T x = (T(27) * (v[c + sx] - v0) - (v[c + 2 * sx] - v[c - sx])) * A.rd24[0];
-            if (PML) x = cpml_fix3(x, m[K_XPH][c], ah, bh, ikh);
+            if (PML) x = cpml_fix3(x, &m[K_XPH][c], ah, bh, ikh);
             d[q][K_XPH] = x;
         }
         if (L2 || L3) {   // D-x, full set
             T x = (T(27) * (v0 - v[c - sx]) - (v[c + sx] - v[c - 2 * sx])) * A.rd24[0];
-            if (PML) x = cpml_fix3(x, m[K_XMF][c], a, b, ik);
+            if (PML) x = cpml_fix3(x, &m[K_XMF][c], a, b, ik);
             d[q][K_XMF] = x;
         }
         if (L1 || L3) {   // D-y, full set
             T x = (T(27) * (v0 - v[c - sy]) - (v[c + sy] - v[c - 2 * sy])) * A.rd24[1];
-            if (PML) x = cpml_fix3(x, m[K_YMF][c], a, b, ik);
+            if (PML) x = cpml_fix3(x, &m[K_YMF][c], a, b, ik);
             d[q][K_YMF] = x;
         }
         if (L2 || L4) {   // D+y, half set
             T x = (T(27) * (v[c + sy] - v0) - (v[c + 2 * sy] - v[c - sy])) * A.rd24[1];
-            if (PML) x = cpml_fix3(x, m[K_YPH][c], ah, bh, ikh);
+            if (PML) x = cpml_fix3(x, &m[K_YPH][c], ah, bh, ikh);
             d[q][K_YPH] = x;
         }
         const T zm = (T(27) * (zq[q][2] - zq[q][1]) - (zq[q][3] - zq[q][0])) * A.rd24[2];
         if (L1) {         // D-z, full set
             T x = zm;
-            if (PML) x = cpml_fix3(x, m[K_ZMF][c], a, b, ik);
+            if (PML) x = cpml_fix3(x, &m[K_ZMF][c], a, b, ik);
             d[q][K_ZMF] = x;
         }
         if (L2) {         // D-z, half set; vx's enters sigma_xy uncorrected (1858-1865)
             T x = zm;
-            if (PML && q != 0) x = cpml_fix3(x, m[K_ZMH][c], ah, bh, ikh);
+            if (PML && q != 0) x = cpml_fix3(x, &m[K_ZMH][c], ah, bh, ikh);
             d[q][K_ZMH] = x;
         }
         if (L3 || L4) {   // D+z, half set
             T x = (T(27) * (zq[q][3] - zq[q][2]) - (zq[q][4] - zq[q][1])) * A.rd24[2];
-            if (PML) x = cpml_fix3(x, m[K_ZPH][c], ah, bh, ikh);
+            if (PML) x = cpml_fix3(x, &m[K_ZPH][c], ah, bh, ikh);
             d[q][K_ZPH] = x;
         }
     }
     const T dt = A.dt;
     const T *const *C = A.C;
-    T sxx = A.s[0][c], syy = A.s[1][c], szz = A.s[2][c], sxy = A.s[3][c], sxz = A.s[4][c], syz = A.s[5][c];
+    T sxx = ldnt(&A.s[0][c]), syy = ldnt(&A.s[1][c]), szz = ldnt(&A.s[2][c]), sxy = ldnt(&A.s[3][c]),
+      sxz = ldnt(&A.s[4][c]), syz = ldnt(&A.s[5][c]);
     if (L1) {
         const T dxx = d[0][K_XPH], dyy = d[1][K_YMF], dzz = d[2][K_ZMF];
         const T yz = d[1][K_ZMF] + d[2][K_YMF], xz = d[0][K_ZMF] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YMF];
-        sxx = stress_update(sxx, C, c, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, A.ig[0][c]);
-        syy = stress_update(syy, C, c, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, A.ig[1][c]);
-        szz = stress_update(szz, C, c, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, A.ig[2][c]);
-        A.s[0][c] = sxx; A.s[1][c] = syy; A.s[2][c] = szz;
+        sxx = stress_update(sxx, C, c, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[0][c]));
+        syy = stress_update(syy, C, c, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[1][c]));
+        szz = stress_update(szz, C, c, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[2][c]));
+        stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); stnt(&A.s[2][c], szz);
     }
     if (L2) {
         const T dxx = d[0][K_XMF], dyy = d[1][K_YPH], dzz = d[2][K_ZMH];
         const T yz = d[1][K_ZMH] + d[2][K_YPH], xz = d[0][K_ZMH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YPH];
-        sxy = stress_update(sxy, C, c, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, A.ig[3][c]);
-        A.s[3][c] = sxy;
+        sxy = stress_update(sxy, C, c, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[3][c]));
+        stnt(&A.s[3][c], sxy);
     }
     if (L3) {
         const T dxx = d[0][K_XMF], dyy = d[1][K_YMF], dzz = d[2][K_ZPH];
         const T yz = d[1][K_ZPH] + d[2][K_YMF], xz = d[0][K_ZPH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YMF];
-        sxz = stress_update(sxz, C, c, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, A.ig[4][c]);
-        A.s[4][c] = sxz;
+        sxz = stress_update(sxz, C, c, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[4][c]));
+        stnt(&A.s[4][c], sxz);
     }
     if (L4) {
         const T dxx = d[0][K_XPH], dyy = d[1][K_YPH], dzz = d[2][K_ZPH];
         const T yz = d[1][K_ZPH] + d[2][K_YPH], xz = d[0][K_ZPH] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YPH];
-        syz = stress_update(syz, C, c, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, A.ig[5][c]);
-        A.s[5][c] = syz;
+        syz = stress_update(syz, C, c, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[5][c]));
+        stnt(&A.s[5][c], syz);
     }
     if (c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3]) return 0.0;
     const double a0 = sxx, a1 = syy, a2 = szz, a3 = sxy, a4 = syz, a5 = sxz;
@@ -192,62 +140,10 @@ __global__ void __launch_bounds__(256) el3_stress_kernel(El3Args<T> A)
 #pragma unroll
             for (int s = 0; s < 5; ++s) zq[q][s] = zload(A.v[q], col, sz, k - 2 + s, A.nz);
         const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (A.flags[t]) nrm = el3_stress_cell<T, true>(A, i, j, k, zq);
+        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = el3_stress_cell<T, true>(A, i, j, k, zq);
         else nrm = el3_stress_cell<T, false>(A, i, j, k, zq);
     }
     block_max_to(&A.snorm[*A.d_it - 1], nrm);
-}
-
-// z-marching columns: a 64 x 4 workgroup walks `kc` planes; each thread keeps
-// its column's five velocity planes in registers, so every velocity value is
-// fetched from memory once per sweep (plus the chunk's 4 halo planes).  The
-// CPML choice is made per plane and is uniform across the workgroup.
-template <typename T>
-__global__ void __launch_bounds__(256) el3_stress_zm_kernel(El3Args<T> A, int kc)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kc + 1;
-    const int k1 = min(k0 + kc - 1, A.nz);
-    const bool in = i <= A.nx && j <= A.ny;
-    const int64_t sz = (int64_t)A.nx * A.ny;
-    const int64_t col = in ? (int64_t)(i - 1) + (int64_t)A.nx * (j - 1) : 0;
-    T zq[3][5];
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) zq[q][s] = in ? zload(A.v[q], col, sz, k0 - 2 + s, A.nz) : T(0);
-    double nrm = 0.0;
-    const size_t trow = blockIdx.x + (size_t)A.tiles_x * blockIdx.y;
-    const size_t tplane = (size_t)A.tiles_x * A.tiles_y;
-    for (int k = k0; k <= k1; ++k) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) zq[q][4] = in ? zload(A.v[q], col, sz, k + 2, A.nz) : T(0);
-        if (in) {
-            double r;
-            if (A.flags[trow + tplane * (k - 1)]) r = el3_stress_cell<T, true>(A, i, j, k, zq);
-            else r = el3_stress_cell<T, false>(A, i, j, k, zq);
-            nrm = (r > nrm || r != r) ? r : nrm;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            zq[q][0] = zq[q][1]; zq[q][1] = zq[q][2]; zq[q][2] = zq[q][3]; zq[q][3] = zq[q][4];
-        }
-    }
-    block_max_to(&A.snorm[*A.d_it - 1], nrm);
-}
-
-// 3-term (reference) / 4-term (corrected) velocity differences from five
-// consecutive samples f(-2..+2) (cpmlfdtd.f08:2022-2024, 2063-2066, 2107-2111)
-template <typename T>
-__device__ __forceinline__ T tminus5(T fm2, T fm1, T f0, T fp1, int corr)
-{
-    return corr ? T(27) * (f0 - fm1) - (fp1 - fm2) : T(27) * f0 - T(27) * fm1 + fm2;
-}
-template <typename T>
-__device__ __forceinline__ T tplus5(T fm1, T f0, T fp1, T fp2, int corr)
-{
-    return corr ? T(27) * (fp1 - f0) - (fp2 - fm1) : -T(27) * f0 + T(27) * fp1 - fp2;
 }
 
 // zs[0..2] = sigma_xz, sigma_yz, sigma_zz at planes k-2..k+2 of the column;
@@ -259,18 +155,18 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
     const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    T vx = A.v[0][c], vy = A.v[1][c], vz = A.v[2][c];
+    T vx = ldnt(&A.v[0][c]), vy = ldnt(&A.v[1][c]), vz = ldnt(&A.v[2][c]);
     if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
         vx = vy = vz = T(0);
-        A.v[0][c] = vx; A.v[1][c] = vy; A.v[2][c] = vz;
+        stnt(&A.v[0][c], vx); stnt(&A.v[1][c], vy); stnt(&A.v[2][c], vz);
     } else {
         const int dc = A.density_code, corr = A.corrected;
         const T r = zr[1];
         const T ryy = scalar_mean(r, r, dc);
         T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
         if (PML) {
-            a = A.a[c]; b = A.b[c]; ik = A.ik[c];
-            ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+            a = ldnt(&A.a[c]); b = ldnt(&A.b[c]); ik = ldnt(&A.ik[c]);
+            ah = ldnt(&A.ah[c]); bh = ldnt(&A.bh[c]); ikh = ldnt(&A.ikh[c]);
         }
         const T *sxx = A.s[0], *syy = A.s[1], *sxy = A.s[3], *sxz = A.s[4], *syz = A.s[5];
         const T sxz0 = zs[0][2], syz0 = zs[1][2], szz0 = zs[2][2];
@@ -282,12 +178,12 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
             T d2 = tminus5(sxy[c - 2 * sy], sxy[c - sy], sxy0, corr ? sxy[c + sy] : T(0), corr) * A.rd24[1];
             T d3 = tminus5(zs[0][0], zs[0][1], sxz0, zs[0][3], corr) * A.rd24[2];
             if (PML) {
-                d1 = cpml_fix3(d1, A.m[M_SXX_X][c], ah, bh, ikh);
-                d2 = cpml_fix3(d2, A.m[M_SXY_Y][c], ah, bh, ikh);
-                d3 = cpml_fix3(d3, A.m[M_SXZ_Z][c], ah, bh, ikh);
+                d1 = cpml_fix3(d1, &A.m[M_SXX_X][c], ah, bh, ikh);
+                d2 = cpml_fix3(d2, &A.m[M_SXY_Y][c], ah, bh, ikh);
+                d3 = cpml_fix3(d3, &A.m[M_SXZ_Z][c], ah, bh, ikh);
             }
             vx = vx + (d1 / rx + d2 / ryy + d3 / rz) * A.dt;
-            A.v[0][c] = vx;
+            stnt(&A.v[0][c], vx);
         }
         if (i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1) {
             // cpmlfdtd.f08:2055-2080
@@ -296,12 +192,12 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
             T d2 = tplus5(corr ? syy[c - sy] : T(0), syy[c], syy[c + sy], syy[c + 2 * sy], corr) * A.rd24[1];
             T d3 = tminus5(zs[1][0], zs[1][1], syz0, zs[1][3], corr) * A.rd24[2];
             if (PML) {
-                d1 = cpml_fix3(d1, A.m[M_SXY_X][c], ah, bh, ikh);
-                d2 = cpml_fix3(d2, A.m[M_SYY_Y][c], ah, bh, ikh);
-                d3 = cpml_fix3(d3, A.m[M_SYZ_Z][c], a, b, ik);
+                d1 = cpml_fix3(d1, &A.m[M_SXY_X][c], ah, bh, ikh);
+                d2 = cpml_fix3(d2, &A.m[M_SYY_Y][c], ah, bh, ikh);
+                d3 = cpml_fix3(d3, &A.m[M_SYZ_Z][c], a, b, ik);
             }
             vy = vy + (d1 / rx + d2 / ryy + d3 / rz) * A.dt;
-            A.v[1][c] = vy;
+            stnt(&A.v[1][c], vy);
         }
         if (i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2) {
             // cpmlfdtd.f08:2098-2127
@@ -310,12 +206,12 @@ __device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, 
             T d2 = tminus5(syz[c - 2 * sy], syz[c - sy], syz0, corr ? syz[c + sy] : T(0), corr) * A.rd24[1];
             T d3 = tplus5(zs[2][1], szz0, zs[2][3], zs[2][4], corr) * A.rd24[2];
             if (PML) {
-                d1 = cpml_fix3(d1, A.m[M_SXZ_X][c], ah, bh, ikh);
-                d2 = cpml_fix3(d2, A.m[M_SYZ_Y][c], a, b, ik);
-                d3 = cpml_fix3(d3, A.m[M_SZZ_Z][c], ah, bh, ikh);
+                d1 = cpml_fix3(d1, &A.m[M_SXZ_X][c], ah, bh, ikh);
+                d2 = cpml_fix3(d2, &A.m[M_SYZ_Y][c], a, b, ik);
+                d3 = cpml_fix3(d3, &A.m[M_SZZ_Z][c], ah, bh, ikh);
             }
             vz = vz + (d1 / rx + d2 / ryy + d3 / rz) * A.dt;
-            A.v[2][c] = vz;
+            stnt(&A.v[2][c], vz);
         }
     }
     if (c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2]) return 0.0;
@@ -342,48 +238,8 @@ __global__ void __launch_bounds__(256) el3_velocity_kernel(El3Args<T> A)
 #pragma unroll
         for (int s = 0; s < 3; ++s) zr[s] = zload(A.rho, col, sz, k - 1 + s, A.nz);
         const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (A.flags[t]) nrm = el3_velocity_cell<T, true>(A, i, j, k, zs, zr);
+        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = el3_velocity_cell<T, true>(A, i, j, k, zs, zr);
         else nrm = el3_velocity_cell<T, false>(A, i, j, k, zs, zr);
-    }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) el3_velocity_zm_kernel(El3Args<T> A, int kc)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k0 = blockIdx.z * kc + 1;
-    const int k1 = min(k0 + kc - 1, A.nz);
-    const bool in = i <= A.nx && j <= A.ny;
-    const int64_t sz = (int64_t)A.nx * A.ny;
-    const int64_t col = in ? (int64_t)(i - 1) + (int64_t)A.nx * (j - 1) : 0;
-    const T *zsrc[3] = {A.s[4], A.s[5], A.s[2]};
-    T zs[3][5], zr[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) zs[q][s] = in ? zload(zsrc[q], col, sz, k0 - 2 + s, A.nz) : T(0);
-    zr[0] = in ? zload(A.rho, col, sz, k0 - 1, A.nz) : T(0);
-    zr[1] = in ? zload(A.rho, col, sz, k0, A.nz) : T(0);
-    double nrm = 0.0;
-    const size_t trow = blockIdx.x + (size_t)A.tiles_x * blockIdx.y;
-    const size_t tplane = (size_t)A.tiles_x * A.tiles_y;
-    for (int k = k0; k <= k1; ++k) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) zs[q][4] = in ? zload(zsrc[q], col, sz, k + 2, A.nz) : T(0);
-        zr[2] = in ? zload(A.rho, col, sz, k + 1, A.nz) : T(0);
-        if (in) {
-            double r;
-            if (A.flags[trow + tplane * (k - 1)]) r = el3_velocity_cell<T, true>(A, i, j, k, zs, zr);
-            else r = el3_velocity_cell<T, false>(A, i, j, k, zs, zr);
-            nrm = (r > nrm || r != r) ? r : nrm;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            zs[q][0] = zs[q][1]; zs[q][1] = zs[q][2]; zs[q][2] = zs[q][3]; zs[q][3] = zs[q][4];
-        }
-        zr[0] = zr[1]; zr[1] = zr[2];
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
@@ -405,6 +261,7 @@ static El3Args<T> el3_args(Device &d)
     A.ikh = (const T *)d.cpml[5];
     for (int q = 0; q < M_TOTAL; ++q) A.m[q] = (T *)d.mem[q];
     A.flags = d.tile_flags;
+    for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
     A.tiles_x = d.tiles_x; A.tiles_y = d.tiles_y;
     A.rd24[0] = (T)(1.0 / (24.0 * d.dx));
@@ -425,15 +282,13 @@ template <typename T>
 static void el3_step_t(Device &d, bool timed)
 {
     const El3Args<T> A = el3_args<T>(d);
-    const dim3 block(64, 4, 1);
-    const bool zm = d.variant != 1;
-    const int kc = d.zchunk > 0 ? d.zchunk : 32;
-    const dim3 grid(d.tiles_x, d.tiles_y, zm ? (d.nz + kc - 1) / kc : d.tiles_z);
+    const bool fast = d.variant != 1;
+    const dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z), block(64, 4, 1);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    if (zm) hipLaunchKernelGGL(el3_stress_zm_kernel<T>, grid, block, 0, d.stream, A, kc);
+    if (fast) el3_fast_stress<T>(d, A);
     else hipLaunchKernelGGL(el3_stress_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    if (zm) hipLaunchKernelGGL(el3_velocity_zm_kernel<T>, grid, block, 0, d.stream, A, kc);
+    if (fast) el3_fast_velocity<T>(d, A);
     else hipLaunchKernelGGL(el3_velocity_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
@@ -513,7 +368,20 @@ int el3_setup(Device &d, const double *const *in)
     d.tile_x = 64;
     d.tile_y = 4;
     d.tile_z = 1;
-    return compute_tile_flags(d, in);
+    int rc = compute_tile_flags(d, in);
+    if (rc) return rc;
+    // The branch-free interior kernels need every nest active on the box
+    // (cpmlfdtd.f08:1761-2127 ranges intersect to [3, n-2] per axis) and no
+    // CPML cell inside it; otherwise every cell takes the shell kernels.
+    const int lim[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; ++a) {
+        d.box[2 * a] = d.box[2 * a] < 3 ? 3 : d.box[2 * a];
+        d.box[2 * a + 1] = d.box[2 * a + 1] > lim[a] - 2 ? lim[a] - 2 : d.box[2 * a + 1];
+    }
+    if (d.nflagged > 0) {
+        for (int a = 0; a < 3; ++a) { d.box[2 * a] = 1; d.box[2 * a + 1] = 0; }
+    }
+    return SDT_OK;
 }
 
 }  // namespace sdt

@@ -26,6 +26,7 @@ struct Em2Args {
     const T *a, *b, *ik, *ah, *bh, *ikh;
     T *m_dEx_dz, *m_dEz_dx, *m_dHy_dx, *m_dHy_dz;
     const uint8_t *flags;
+    int box[6];
     int nx, nz, tiles_x;
     T rdx, rdz, dt, dbHy;
     const int *d_it;
@@ -58,7 +59,7 @@ __global__ void __launch_bounds__(256) em2_h_kernel(Em2Args<T> A)
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     if (i > A.nx || j > A.nz) return;
-    if (A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) em2_h_cell<T, true>(A, i, j);
+    if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) em2_h_cell<T, true>(A, i, j);
     else em2_h_cell<T, false>(A, i, j);
 }
 
@@ -99,7 +100,7 @@ __global__ void __launch_bounds__(256) em2_e_kernel(Em2Args<T> A)
     const int j = blockIdx.y * 4 + threadIdx.y + 1;
     double nrm = 0.0;
     if (i <= A.nx && j <= A.nz) {
-        if (A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = em2_e_cell<T, true>(A, i, j);
+        if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = em2_e_cell<T, true>(A, i, j);
         else nrm = em2_e_cell<T, false>(A, i, j);
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
@@ -125,6 +126,7 @@ static Em2Args<T> em2_args(Device &d)
     A.m_dHy_dx = (T *)d.mem[2];
     A.m_dHy_dz = (T *)d.mem[3];
     A.flags = d.tile_flags;
+    for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx;
     A.nz = d.nz;
     A.tiles_x = d.tiles_x;

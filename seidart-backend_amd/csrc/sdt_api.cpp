@@ -142,10 +142,7 @@ static void release(sdt_handle *h)
     if (d.stream) (void)hipStreamSynchronize(d.stream);
     if (h->graph) (void)hipGraphExecDestroy(h->graph);
     auto drop = [](void *p) { if (p) (void)hipFree(p); };
-    for (auto &p : d.field) drop(p);
-    for (auto &p : d.mem) drop(p);
-    for (auto &p : d.mat) drop(p);
-    for (auto &p : d.cpml) drop(p);
+    free_grids(d);   // fields, memory variables, materials, CPML coefficients
     drop(d.tile_flags);
     drop(d.series);
     drop(d.norms);
@@ -496,6 +493,10 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
         return SDT_OK;
     case SDT_OPT_GRAPH:
         h->d.graphs = value != 0;
+        return SDT_OK;
+    case SDT_OPT_STAGGER:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "allocation layout is fixed once the first step has run");
+        h->d.stagger = value != 0;
         return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown option");

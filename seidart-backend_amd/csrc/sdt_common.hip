@@ -81,10 +81,29 @@ void launch_tail(Device &d)
         hipLaunchKernelGGL(tail_kernel<float>, dim3(1), dim3(256), 0, d.stream, fp, d.tail);
 }
 
+int alloc_grid(Device &d, void **dev, size_t bytes)
+{
+    const size_t shift = d.stagger ? (size_t)(d.nalloc % 61) * 4352u : 0;
+    d.nalloc++;
+    if (d.nbases >= (int)(sizeof(d.bases) / sizeof(d.bases[0]))) return SDT_ERR_HIP;
+    void *base = nullptr;
+    if (hipMalloc(&base, bytes + shift) != hipSuccess) return SDT_ERR_HIP;
+    d.bases[d.nbases++] = base;
+    *dev = (char *)base + shift;
+    return SDT_OK;
+}
+
+void free_grids(Device &d)
+{
+    for (int q = 0; q < d.nbases; ++q)
+        if (d.bases[q]) (void)hipFree(d.bases[q]);
+    d.nbases = 0;
+}
+
 int upload_converted(Device &d, const double *host, void **dev, int64_t n)
 {
     const size_t bytes = (size_t)n * d.esize;
-    if (hipMalloc(dev, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
     if (d.precision == SDT_FP64) {
         if (hipMemcpy(*dev, host, bytes, hipMemcpyHostToDevice) != hipSuccess) return SDT_ERR_HIP;
     } else {
@@ -98,7 +117,7 @@ int upload_converted(Device &d, const double *host, void **dev, int64_t n)
 int alloc_zeroed(Device &d, void **dev, int64_t n)
 {
     const size_t bytes = (size_t)n * d.esize;
-    if (hipMalloc(dev, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
     if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
     return SDT_OK;
 }
@@ -118,17 +137,34 @@ int compute_tile_flags(Device &d, const double *const *host_in)
     const double *a = host_in[SDT_ACOEF], *b = host_in[SDT_BCOEF], *k = host_in[SDT_KAPPA];
     const double *ah = host_in[SDT_ACOEF_HALF], *bh = host_in[SDT_BCOEF_HALF],
                  *kh = host_in[SDT_KAPPA_HALF];
-    for (int kz = 0; kz < d.nz; ++kz)
-        for (int jy = 0; jy < d.ny; ++jy) {
-            const int64_t row = (int64_t)d.nx * (jy + (int64_t)d.ny * kz);
-            const size_t trow = (size_t)d.tiles_x * (jy / d.tile_y + (size_t)d.tiles_y * (kz / d.tile_z));
-            for (int ix = 0; ix < d.nx; ++ix) {
-                const int64_t c = row + ix;
-                const bool t = trivial(a ? a[c] : 0.0, b ? b[c] : 0.0, k ? k[c] : 1.0) &&
-                               trivial(ah ? ah[c] : 0.0, bh ? bh[c] : 0.0, kh ? kh[c] : 1.0);
-                if (!t) flags[trow + ix / d.tile_x] = 1;
-            }
-        }
+    auto ident = [&](int i, int j, int kk) {   // 1-based
+        const int64_t c = (int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (kk - 1));
+        return trivial(a ? a[c] : 0.0, b ? b[c] : 0.0, k ? k[c] : 1.0) &&
+               trivial(ah ? ah[c] : 0.0, bh ? bh[c] : 0.0, kh ? kh[c] : 1.0);
+    };
+    const int ic = (d.nx + 1) / 2, jc = (d.ny + 1) / 2, kc = (d.nz + 1) / 2;
+    int *bx = d.box;
+    if (!ident(ic, jc, kc)) {
+        bx[0] = 1; bx[1] = 0; bx[2] = 1; bx[3] = 0; bx[4] = 1; bx[5] = 0;   // empty
+    } else {
+        bx[0] = ic; while (bx[0] > 1 && ident(bx[0] - 1, jc, kc)) --bx[0];
+        bx[1] = ic; while (bx[1] < d.nx && ident(bx[1] + 1, jc, kc)) ++bx[1];
+        bx[2] = jc; while (bx[2] > 1 && ident(ic, bx[2] - 1, kc)) --bx[2];
+        bx[3] = jc; while (bx[3] < d.ny && ident(ic, bx[3] + 1, kc)) ++bx[3];
+        bx[4] = kc; while (bx[4] > 1 && ident(ic, jc, bx[4] - 1)) --bx[4];
+        bx[5] = kc; while (bx[5] < d.nz && ident(ic, jc, bx[5] + 1)) ++bx[5];
+    }
+    // cells inside the box that are not identity cells flag their tile
+    for (int kk = bx[4]; kk <= bx[5]; ++kk)
+        for (int j = bx[2]; j <= bx[3]; ++j)
+            for (int i = bx[0]; i <= bx[1]; ++i)
+                if (!ident(i, j, kk)) {
+                    const size_t t = (size_t)((i - 1) / d.tile_x) +
+                                     (size_t)d.tiles_x * ((size_t)((j - 1) / d.tile_y) +
+                                                          (size_t)d.tiles_y * ((kk - 1) / d.tile_z));
+                    d.nflagged += flags[t] ? 0 : 1;
+                    flags[t] = 1;
+                }
     if (hipMalloc((void **)&d.tile_flags, ntiles) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(d.tile_flags, flags.data(), ntiles, hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;

@@ -53,6 +53,37 @@ __device__ __forceinline__ void block_max_to(unsigned long long *slot, double v)
     }
 }
 
+// Streaming loads/stores for data touched once per step (materials, memory
+// variables, CPML coefficients): marked non-temporal so they do not evict
+// the stencil operands that neighbouring cells re-read from L2.
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void stnt(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+__device__ __forceinline__ bool in_box(const int (&b)[6], int i, int j, int k)
+{
+    return i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && k >= b[4] && k <= b[5];
+}
+
+// gfx950 (MI355X) has 8 XCDs, each with its own L2; the dispatcher hands
+// workgroup b to XCD b % 8.  xcd_remap turns the hardware id into a logical
+// id such that every XCD owns one contiguous range of logical ids, so tiles
+// that share stencil halos run on the same XCD and hit the same L2.
+constexpr int kNumXcd = 8;
+__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nb)
+{
+    const unsigned x = bid % kNumXcd, s = bid / kNumXcd;
+    const unsigned per = nb / kNumXcd, rem = nb % kNumXcd;
+    return x * per + (x < rem ? x : rem) + s;
+}
+
 // Per-launch constants shared by every solver kernel.
 struct GridInfo {
     int nx, ny, nz;

@@ -64,7 +64,10 @@ struct Device {
     void *mat[kMaxMat] = {};
     int nmat = 0;
     void *cpml[6] = {};                // a, b, 1/kappa (full), a, b, 1/kappa (half)
-    uint8_t *tile_flags = nullptr;     // 1 = tile holds CPML cells
+    uint8_t *tile_flags = nullptr;     // 1 = tile holds a CPML cell inside the interior box
+    int64_t nflagged = 0;              // tiles flagged inside the box
+    int box[6] = {1, 0, 1, 0, 1, 0};   // interior box ilo,ihi,jlo,jhi,klo,khi (1-based, inclusive):
+                                       // every cell inside has identity CPML coefficients
     int tiles_x = 0, tiles_y = 0, tiles_z = 0;
     int tile_x = 64, tile_y = 4, tile_z = 1;
 
@@ -79,6 +82,10 @@ struct Device {
     int variant = 0;                   // SDT_OPT_KERNEL_VARIANT
     int zchunk = 0;                    // SDT_OPT_ZCHUNK
     bool graphs = true;                // SDT_OPT_GRAPH
+    bool stagger = true;               // SDT_OPT_STAGGER
+    int nalloc = 0;                    // allocations made (stagger index)
+    void *bases[256] = {};             // hipMalloc'd base of every array
+    int nbases = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     bool timing = false;
@@ -104,13 +111,23 @@ void em3_step(Device &d, bool timed);
 
 // common pieces (sdt_common.hip)
 void launch_tail(Device &d);
-// Tile flags: 1 where any cell of the (tile_x, tile_y, tile_z) tile has a
-// non-identity CPML coefficient (a != 0, kappa != 1 or non-finite b, in
-// either the full or the half set).
+// CPML classification.  A cell is an identity cell when a == 0, kappa == 1
+// and b is finite in both the full and the half coefficient set (its memory
+// variables then stay exactly zero and its derivatives pass unchanged).
+// d.box is the box grown from the grid centre along the three centre lines
+// while cells stay identity cells; cells outside it take the CPML body.
+// tile_flags marks the (tile_x, tile_y, tile_z) tiles that nevertheless hold
+// a non-identity cell inside the box, whose cells then take the CPML body
+// too.  Kernels test: cpml(cell) = !inside(box) || tile_flags[tile].
 int compute_tile_flags(Device &d, const double *const *host_in);
 // copy n FP64 host values into a freshly allocated device array of the
 // handle's precision
 int upload_converted(Device &d, const double *host, void **dev, int64_t n);
 int alloc_zeroed(Device &d, void **dev, int64_t n);
+// hipMalloc of a grid array whose start is offset by a different multiple of
+// 4352 bytes per allocation (when d.stagger), so that the dozens of equally
+// sized arrays streamed together do not begin on the same HBM channel.
+int alloc_grid(Device &d, void **dev, size_t bytes);
+void free_grids(Device &d);
 
 }  // namespace sdt
