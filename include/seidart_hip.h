@@ -182,12 +182,12 @@ int32_t sdt_synchronize(sdt_handle *h);
 
 /* tuning options (no reference counterpart); set before the first step */
 enum {
-    SDT_OPT_KERNEL_VARIANT = 0, /* 0 auto, 1 one thread per cell, 2 z-marching columns */
     SDT_OPT_ZCHUNK = 1,         /* planes per z-marching workgroup (0 = auto)            */
     SDT_OPT_GRAPH = 2,          /* 1 = replay captured hipGraphs of several steps when the
                                    grid is small (default), 0 = direct launches          */
-    SDT_OPT_STAGGER = 3         /* 1 = offset each grid array's start by a different
+    SDT_OPT_STAGGER = 3,        /* 1 = offset each grid array's start by a different
                                    multiple of 4352 B (default), 0 = plain hipMalloc     */
+    SDT_OPT_TILE_ROWS = 4       /* rows of the 3D interior workgroup tile: 0 auto, 4, 8, 16 */
 };
 int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
 

@@ -23,7 +23,8 @@ struct El2Args {
     T *m[12];
     const uint8_t *flags;
     int box[6];
-    int nx, nz, tiles_x;
+    int nx, nz, tiles_x, pitch;
+    int64_t xoff;
     T rdx24, rdz24, dt;
     int density_code, corrected;
     const int *d_it;
@@ -44,9 +45,9 @@ template <typename T, bool PML>
 __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
-    const int64_t c = (int64_t)(i - 1) + (int64_t)nx * (j - 1);
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     const T *vx = A.vx, *vz = A.vz;
-    const int64_t sx = 1, sz = nx;
+    const int64_t sx = 1, sz = A.pitch;
     if (i >= 2 && i <= nx - 2 && j >= 3 && j <= nz - 1) {
         T dvx_dx = (T(27) * (vx[c + sx] - vx[c]) - (vx[c + 2 * sx] - vx[c - sx])) * A.rdx24;
         T dvz_dz = (T(27) * (vz[c] - vz[c - sz]) - (vz[c + sz] - vz[c - 2 * sz])) * A.rdz24;
@@ -111,8 +112,8 @@ template <typename T, bool PML>
 __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
-    const int64_t c = (int64_t)(i - 1) + (int64_t)nx * (j - 1);
-    const int64_t sx = 1, sz = nx;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+    const int64_t sx = 1, sz = A.pitch;
     T vx = A.vx[c], vz = A.vz[c];
     if (i == 1 || i == nx || j == 1 || j == nz) {
         vx = T(0);
@@ -188,6 +189,8 @@ static El2Args<T> el2_args(Device &d)
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx;
     A.nz = d.nz;
+    A.pitch = d.pitch;
+    A.xoff = d.xoff;
     A.tiles_x = d.tiles_x;
     A.rdx24 = (T)(1.0 / (24.0 * d.dx));
     A.rdz24 = (T)(1.0 / (24.0 * d.dz));
@@ -223,40 +226,45 @@ void el2_step(Device &d, bool timed)
 // attenuation divisors precomputed as 1/(1 + gamma*dt) in FP64.
 int el2_setup(Device &d, const double *const *in)
 {
+    d.tile_x = 64;
+    d.tile_y = 1;
+    d.tile_z = 4;
+    int rc = compute_tile_flags(d, in);
+    if (rc) return rc;
+    set_layout(d);
     const int64_t n = d.ncell;
     const int ids[] = {SDT_C11, SDT_C13, SDT_C15, SDT_C33, SDT_C35, SDT_C55, SDT_RHO};
     d.nmat = 0;
     std::vector<double> zero((size_t)n, 0.0);
     for (int id : ids) {
-        int rc = upload_converted(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++], n);
+        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Z, SDT_GAMMA_XZ};
     std::vector<double> inv((size_t)n);
     for (int id : gids) {
         for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
-        int rc = upload_converted(d, inv.data(), &d.mat[d.nmat++], n);
+        rc = upload_grid(d, inv.data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     d.nmem = 12;
     for (int q = 0; q < d.nmem; ++q) {
-        int rc = alloc_zeroed(d, &d.mem[q], n);
+        rc = alloc_grid_zeroed(d, &d.mem[q]);
         if (rc) return rc;
     }
     const int fids[] = {SDT_F_VX, SDT_F_VZ, SDT_F_SXX, SDT_F_SZZ, SDT_F_SXZ};
     for (int f : fids) {
-        int rc;
-        if (f == SDT_F_VX && in[SDT_IC_X]) rc = upload_converted(d, in[SDT_IC_X], &d.field[f], n);
-        else if (f == SDT_F_VZ && in[SDT_IC_Z]) rc = upload_converted(d, in[SDT_IC_Z], &d.field[f], n);
-        else rc = alloc_zeroed(d, &d.field[f], n);
+        if (f == SDT_F_VX && in[SDT_IC_X]) rc = upload_grid(d, in[SDT_IC_X], &d.field[f]);
+        else if (f == SDT_F_VZ && in[SDT_IC_Z]) rc = upload_grid(d, in[SDT_IC_Z], &d.field[f]);
+        else rc = alloc_grid_zeroed(d, &d.field[f]);
         if (rc) return rc;
     }
 
     // point sources, cpmlfdtd.f08:519-523; indices (is, js) are padded 1-based
     const int is = d.isrc, js = d.ksrc, nx = d.nx, nz = d.nz;
-    auto cell = [&](int i, int j) { return (int64_t)(i - 1) + (int64_t)nx * (j - 1); };
+    auto cell = [&](int i, int j) { return cell_index(d, i, 1, j); };
     const double *rho = in[SDT_RHO];
-    auto rho_at = [&](int i, int j) { return rho ? rho[cell(i, j)] : 0.0; };
+    auto rho_at = [&](int i, int j) { return rho ? rho[(int64_t)(i - 1) + (int64_t)nx * (j - 1)] : 0.0; };
     const int face = (is == 1 || is == nx || js == 1 || js == nz);
     TailArgs &t = d.tail;
     t.nterms = 0;
@@ -275,11 +283,7 @@ int el2_setup(Device &d, const double *const *in)
     t.rec_field[0] = SDT_F_VX;
     t.rec_field[1] = -1;
     t.rec_field[2] = SDT_F_VZ;
-
-    d.tile_x = 64;
-    d.tile_y = 1;
-    d.tile_z = 4;
-    return compute_tile_flags(d, in);
+    return upload_cpml_full(d, in);
 }
 
 }  // namespace sdt

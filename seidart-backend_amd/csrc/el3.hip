@@ -1,11 +1,12 @@
 // el3.hip -- 3D anisotropic elastic CPML-FDTD (seismic3,
 // cpmlfdtd.f08:1341-2313) on gfx950.
 //
-// The reference's four stress nests (L1 sigma_xx/yy/zz, L2 sigma_xy, L3
-// sigma_xz, L4 sigma_yz; cpmlfdtd.f08:1761-1992) run as one fused stress
-// kernel; its three velocity nests (2014-2127) as one fused velocity kernel
-// that also applies the Dirichlet faces (2226-2248) and the velocity norm
-// (2255).  Each cell evaluates only the nests whose index range contains it.
+// Setup, device layout and the step sequence.  The reference's four stress
+// nests (L1 sigma_xx/yy/zz, L2 sigma_xy, L3 sigma_xz, L4 sigma_yz;
+// cpmlfdtd.f08:1761-1992) run fused in one stress pass and its three velocity
+// nests (2014-2127) in one velocity pass that also applies the Dirichlet
+// faces (2226-2248) and the velocity norm (2255); each pass is an interior
+// kernel plus a CPML-shell kernel (el3_fast.hip).
 //
 // Memory variables are deduplicated: a recursion psi <- b psi + a d depends
 // only on its own cell's history of d, so the reference arrays that share
@@ -22,227 +23,6 @@
 #include "el3_common.h"
 
 namespace sdt {
-
-// Velocity values of one column at planes k-2..k+2 (zq[comp][0..4]); the
-// x and y neighbours are read from memory (L1/L2 hits within a workgroup).
-template <typename T, bool PML>
-__device__ __forceinline__ double el3_stress_cell(const El3Args<T> &A, int i, int j, int k, const T (&zq)[3][5])
-{
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
-    const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    const bool L1 = i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1;
-    const bool L2 = i >= 3 && i <= nx - 1 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1;
-    const bool L3 = i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2;
-    const bool L4 = i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 2 && k <= nz - 2;
-    T d[3][K_NKIND];
-    T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
-    if (PML && (L1 || L2 || L3 || L4)) {
-        a = ldnt(&A.a[c]); b = ldnt(&A.b[c]); ik = ldnt(&A.ik[c]);
-        ah = ldnt(&A.ah[c]); bh = ldnt(&A.bh[c]); ikh = ldnt(&A.ikh[c]);
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        const T *v = A.v[q];
-        T *const *m = A.m + q * K_NKIND;
-        const T v0 = zq[q][2];
-        if (L1 || L4) {   // D+x, half set
-            T x = (T(27) * (v[c + sx] - v0) - (v[c + 2 * sx] - v[c - sx])) * A.rd24[0];
-            if (PML) x = cpml_fix3(x, &m[K_XPH][c], ah, bh, ikh);
-            d[q][K_XPH] = x;
-        }
-        if (L2 || L3) {   // D-x, full set
-            T x = (T(27) * (v0 - v[c - sx]) - (v[c + sx] - v[c - 2 * sx])) * A.rd24[0];
-            if (PML) x = cpml_fix3(x, &m[K_XMF][c], a, b, ik);
-            d[q][K_XMF] = x;
-        }
-        if (L1 || L3) {   // D-y, full set
-            T x = (T(27) * (v0 - v[c - sy]) - (v[c + sy] - v[c - 2 * sy])) * A.rd24[1];
-            if (PML) x = cpml_fix3(x, &m[K_YMF][c], a, b, ik);
-            d[q][K_YMF] = x;
-        }
-        if (L2 || L4) {   // D+y, half set
-            T x = (T(27) * (v[c + sy] - v0) - (v[c + 2 * sy] - v[c - sy])) * A.rd24[1];
-            if (PML) x = cpml_fix3(x, &m[K_YPH][c], ah, bh, ikh);
-            d[q][K_YPH] = x;
-        }
-        const T zm = (T(27) * (zq[q][2] - zq[q][1]) - (zq[q][3] - zq[q][0])) * A.rd24[2];
-        if (L1) {         // D-z, full set
-            T x = zm;
-            if (PML) x = cpml_fix3(x, &m[K_ZMF][c], a, b, ik);
-            d[q][K_ZMF] = x;
-        }
-        if (L2) {         // D-z, half set; vx's enters sigma_xy uncorrected (1858-1865)
-            T x = zm;
-            if (PML && q != 0) x = cpml_fix3(x, &m[K_ZMH][c], ah, bh, ikh);
-            d[q][K_ZMH] = x;
-        }
-        if (L3 || L4) {   // D+z, half set
-            T x = (T(27) * (zq[q][3] - zq[q][2]) - (zq[q][4] - zq[q][1])) * A.rd24[2];
-            if (PML) x = cpml_fix3(x, &m[K_ZPH][c], ah, bh, ikh);
-            d[q][K_ZPH] = x;
-        }
-    }
-    const T dt = A.dt;
-    const T *const *C = A.C;
-    T sxx = ldnt(&A.s[0][c]), syy = ldnt(&A.s[1][c]), szz = ldnt(&A.s[2][c]), sxy = ldnt(&A.s[3][c]),
-      sxz = ldnt(&A.s[4][c]), syz = ldnt(&A.s[5][c]);
-    if (L1) {
-        const T dxx = d[0][K_XPH], dyy = d[1][K_YMF], dzz = d[2][K_ZMF];
-        const T yz = d[1][K_ZMF] + d[2][K_YMF], xz = d[0][K_ZMF] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YMF];
-        sxx = stress_update(sxx, C, c, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[0][c]));
-        syy = stress_update(syy, C, c, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[1][c]));
-        szz = stress_update(szz, C, c, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[2][c]));
-        stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); stnt(&A.s[2][c], szz);
-    }
-    if (L2) {
-        const T dxx = d[0][K_XMF], dyy = d[1][K_YPH], dzz = d[2][K_ZMH];
-        const T yz = d[1][K_ZMH] + d[2][K_YPH], xz = d[0][K_ZMH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YPH];
-        sxy = stress_update(sxy, C, c, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[3][c]));
-        stnt(&A.s[3][c], sxy);
-    }
-    if (L3) {
-        const T dxx = d[0][K_XMF], dyy = d[1][K_YMF], dzz = d[2][K_ZPH];
-        const T yz = d[1][K_ZPH] + d[2][K_YMF], xz = d[0][K_ZPH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YMF];
-        sxz = stress_update(sxz, C, c, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[4][c]));
-        stnt(&A.s[4][c], sxz);
-    }
-    if (L4) {
-        const T dxx = d[0][K_XPH], dyy = d[1][K_YPH], dzz = d[2][K_ZPH];
-        const T yz = d[1][K_ZPH] + d[2][K_YPH], xz = d[0][K_ZPH] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YPH];
-        syz = stress_update(syz, C, c, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[5][c]));
-        stnt(&A.s[5][c], syz);
-    }
-    if (c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3]) return 0.0;
-    const double a0 = sxx, a1 = syy, a2 = szz, a3 = sxy, a4 = syz, a5 = sxz;
-    return sqrt(a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5));
-}
-
-template <typename T>
-__device__ __forceinline__ T zload(const T *f, int64_t col, int64_t sz, int k, int nz)
-{
-    return (k >= 1 && k <= nz) ? f[col + sz * (k - 1)] : T(0);
-}
-
-// One thread per cell (64 x 4 x 1 workgroups): every z neighbour read from memory.
-template <typename T>
-__global__ void __launch_bounds__(256) el3_stress_kernel(El3Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.ny) {
-        const int64_t sz = (int64_t)A.nx * A.ny, col = (int64_t)(i - 1) + (int64_t)A.nx * (j - 1);
-        T zq[3][5];
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-#pragma unroll
-            for (int s = 0; s < 5; ++s) zq[q][s] = zload(A.v[q], col, sz, k - 2 + s, A.nz);
-        const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = el3_stress_cell<T, true>(A, i, j, k, zq);
-        else nrm = el3_stress_cell<T, false>(A, i, j, k, zq);
-    }
-    block_max_to(&A.snorm[*A.d_it - 1], nrm);
-}
-
-// zs[0..2] = sigma_xz, sigma_yz, sigma_zz at planes k-2..k+2 of the column;
-// zr = rho at k-1, k, k+1.
-template <typename T, bool PML>
-__device__ __forceinline__ double el3_velocity_cell(const El3Args<T> &A, int i, int j, int k, const T (&zs)[3][5],
-                                                    const T (&zr)[3])
-{
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sx = 1, sy = nx, sz = (int64_t)nx * ny;
-    const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    T vx = ldnt(&A.v[0][c]), vy = ldnt(&A.v[1][c]), vz = ldnt(&A.v[2][c]);
-    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
-        vx = vy = vz = T(0);
-        stnt(&A.v[0][c], vx); stnt(&A.v[1][c], vy); stnt(&A.v[2][c], vz);
-    } else {
-        const int dc = A.density_code, corr = A.corrected;
-        const T r = zr[1];
-        const T ryy = scalar_mean(r, r, dc);
-        T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
-        if (PML) {
-            a = ldnt(&A.a[c]); b = ldnt(&A.b[c]); ik = ldnt(&A.ik[c]);
-            ah = ldnt(&A.ah[c]); bh = ldnt(&A.bh[c]); ikh = ldnt(&A.ikh[c]);
-        }
-        const T *sxx = A.s[0], *syy = A.s[1], *sxy = A.s[3], *sxz = A.s[4], *syz = A.s[5];
-        const T sxz0 = zs[0][2], syz0 = zs[1][2], szz0 = zs[2][2];
-        if (i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1) {
-            // cpmlfdtd.f08:2014-2038
-            const T rx = scalar_mean(r, A.rho[c - sx], dc), rz = scalar_mean(r, zr[0], dc);
-            const T sxy0 = sxy[c];
-            T d1 = tminus5(sxx[c - 2 * sx], sxx[c - sx], sxx[c], corr ? sxx[c + sx] : T(0), corr) * A.rd24[0];
-            T d2 = tminus5(sxy[c - 2 * sy], sxy[c - sy], sxy0, corr ? sxy[c + sy] : T(0), corr) * A.rd24[1];
-            T d3 = tminus5(zs[0][0], zs[0][1], sxz0, zs[0][3], corr) * A.rd24[2];
-            if (PML) {
-                d1 = cpml_fix3(d1, &A.m[M_SXX_X][c], ah, bh, ikh);
-                d2 = cpml_fix3(d2, &A.m[M_SXY_Y][c], ah, bh, ikh);
-                d3 = cpml_fix3(d3, &A.m[M_SXZ_Z][c], ah, bh, ikh);
-            }
-            vx = vx + (d1 / rx + d2 / ryy + d3 / rz) * A.dt;
-            stnt(&A.v[0][c], vx);
-        }
-        if (i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1) {
-            // cpmlfdtd.f08:2055-2080
-            const T rx = scalar_mean(r, A.rho[c + sx], dc), rz = scalar_mean(r, zr[0], dc);
-            T d1 = tplus5(corr ? sxy[c - sx] : T(0), sxy[c], sxy[c + sx], sxy[c + 2 * sx], corr) * A.rd24[0];
-            T d2 = tplus5(corr ? syy[c - sy] : T(0), syy[c], syy[c + sy], syy[c + 2 * sy], corr) * A.rd24[1];
-            T d3 = tminus5(zs[1][0], zs[1][1], syz0, zs[1][3], corr) * A.rd24[2];
-            if (PML) {
-                d1 = cpml_fix3(d1, &A.m[M_SXY_X][c], ah, bh, ikh);
-                d2 = cpml_fix3(d2, &A.m[M_SYY_Y][c], ah, bh, ikh);
-                d3 = cpml_fix3(d3, &A.m[M_SYZ_Z][c], a, b, ik);
-            }
-            vy = vy + (d1 / rx + d2 / ryy + d3 / rz) * A.dt;
-            stnt(&A.v[1][c], vy);
-        }
-        if (i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2) {
-            // cpmlfdtd.f08:2098-2127
-            const T rx = scalar_mean(r, A.rho[c + sx], dc), rz = scalar_mean(r, zr[2], dc);
-            T d1 = tplus5(corr ? sxz[c - sx] : T(0), sxz0, sxz[c + sx], sxz[c + 2 * sx], corr) * A.rd24[0];
-            T d2 = tminus5(syz[c - 2 * sy], syz[c - sy], syz0, corr ? syz[c + sy] : T(0), corr) * A.rd24[1];
-            T d3 = tplus5(zs[2][1], szz0, zs[2][3], zs[2][4], corr) * A.rd24[2];
-            if (PML) {
-                d1 = cpml_fix3(d1, &A.m[M_SXZ_X][c], ah, bh, ikh);
-                d2 = cpml_fix3(d2, &A.m[M_SYZ_Y][c], a, b, ik);
-                d3 = cpml_fix3(d3, &A.m[M_SZZ_Z][c], ah, bh, ikh);
-            }
-            vz = vz + (d1 / rx + d2 / ryy + d3 / rz) * A.dt;
-            stnt(&A.v[2][c], vz);
-        }
-    }
-    if (c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2]) return 0.0;
-    const double a0 = vx, a1 = vy, a2 = vz;
-    return sqrt(a0 * a0 + a1 * a1 + a2 * a2);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) el3_velocity_kernel(El3Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.ny) {
-        const int64_t sz = (int64_t)A.nx * A.ny, col = (int64_t)(i - 1) + (int64_t)A.nx * (j - 1);
-        T zs[3][5], zr[3];
-#pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            zs[0][s] = zload(A.s[4], col, sz, k - 2 + s, A.nz);
-            zs[1][s] = zload(A.s[5], col, sz, k - 2 + s, A.nz);
-            zs[2][s] = zload(A.s[2], col, sz, k - 2 + s, A.nz);
-        }
-#pragma unroll
-        for (int s = 0; s < 3; ++s) zr[s] = zload(A.rho, col, sz, k - 1 + s, A.nz);
-        const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = el3_velocity_cell<T, true>(A, i, j, k, zs, zr);
-        else nrm = el3_velocity_cell<T, false>(A, i, j, k, zs, zr);
-    }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
-}
 
 template <typename T>
 static El3Args<T> el3_args(Device &d)
@@ -263,6 +43,7 @@ static El3Args<T> el3_args(Device &d)
     A.flags = d.tile_flags;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
+    A.pitch = d.pitch; A.xoff = d.xoff;
     A.tiles_x = d.tiles_x; A.tiles_y = d.tiles_y;
     A.rd24[0] = (T)(1.0 / (24.0 * d.dx));
     A.rd24[1] = (T)(1.0 / (24.0 * d.dy));
@@ -282,14 +63,10 @@ template <typename T>
 static void el3_step_t(Device &d, bool timed)
 {
     const El3Args<T> A = el3_args<T>(d);
-    const bool fast = d.variant != 1;
-    const dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z), block(64, 4, 1);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    if (fast) el3_fast_stress<T>(d, A);
-    else hipLaunchKernelGGL(el3_stress_kernel<T>, grid, block, 0, d.stream, A);
+    el3_fast_stress<T>(d, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    if (fast) el3_fast_velocity<T>(d, A);
-    else hipLaunchKernelGGL(el3_velocity_kernel<T>, grid, block, 0, d.stream, A);
+    el3_fast_velocity<T>(d, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }
@@ -300,46 +77,59 @@ void el3_step(Device &d, bool timed)
     else el3_step_t<float>(d, timed);
 }
 
+int el3_build_shell(Device &d, const double *const *in, int kc);
+
 int el3_setup(Device &d, const double *const *in)
 {
+    d.tile_x = 64;
+    d.tile_y = 4;
+    d.tile_z = 1;
+    int rc = compute_tile_flags(d, in);
+    if (rc) return rc;
+    // The branch-free interior kernels need every nest active on the box
+    // (cpmlfdtd.f08:1761-2127 ranges intersect to [3, n-2] per axis) and no
+    // CPML cell inside it; otherwise every cell takes the shell kernels.
+    const int lim[3] = {d.nx, d.ny, d.nz};
+    for (int a = 0; a < 3; ++a) {
+        d.box[2 * a] = d.box[2 * a] < 3 ? 3 : d.box[2 * a];
+        d.box[2 * a + 1] = d.box[2 * a + 1] > lim[a] - 2 ? lim[a] - 2 : d.box[2 * a + 1];
+    }
+    const bool empty = d.nflagged > 0 || d.box[1] < d.box[0] || d.box[3] < d.box[2] || d.box[5] < d.box[4];
+    if (empty)
+        for (int a = 0; a < 3; ++a) { d.box[2 * a] = 1; d.box[2 * a + 1] = 0; }
+    set_layout(d);
+
     const int64_t n = d.ncell;
     std::vector<double> zero;
     d.nmat = 0;
     for (int id = SDT_C11; id <= SDT_RHO; ++id) {
         if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
-        int rc = upload_converted(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++], n);
+        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     std::vector<double> inv((size_t)n);
     const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};
     for (int id : gids) {
         for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
-        int rc = upload_converted(d, inv.data(), &d.mat[d.nmat++], n);
-        if (rc) return rc;
-    }
-    d.nmem = M_TOTAL;
-    for (int q = 0; q < d.nmem; ++q) {
-        if (q == 0 * K_NKIND + K_ZMH) continue;   // dead recursion, never touched
-        int rc = alloc_zeroed(d, &d.mem[q], n);
+        rc = upload_grid(d, inv.data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     for (int f = SDT_F_VX; f <= SDT_F_SYZ; ++f) {
-        int rc;
-        if (f <= SDT_F_VZ && in[SDT_IC_X + f]) rc = upload_converted(d, in[SDT_IC_X + f], &d.field[f], n);
-        else rc = alloc_zeroed(d, &d.field[f], n);
+        if (f <= SDT_F_VZ && in[SDT_IC_X + f]) rc = upload_grid(d, in[SDT_IC_X + f], &d.field[f]);
+        else rc = alloc_grid_zeroed(d, &d.field[f]);
         if (rc) return rc;
     }
 
     // point sources, cpmlfdtd.f08:2208-2219
     const int is = d.isrc, js = d.jsrc, ks = d.ksrc, nx = d.nx, ny = d.ny, nz = d.nz;
-    auto cell = [&](int i, int j, int k) {
-        return (int64_t)(i - 1) + (int64_t)nx * ((int64_t)(j - 1) + (int64_t)ny * (k - 1));
-    };
+    auto cell = [&](int i, int j, int k) { return cell_index(d, i, j, k); };
     auto face = [&](int i, int j, int k) {
         return (int)(i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz);
     };
     const double *rho = in[SDT_RHO];
-    auto R = [&](int i, int j, int k) { return rho ? rho[cell(i, j, k)] : 0.0; };
+    auto R = [&](int i, int j, int k) {
+        return rho ? rho[(int64_t)(i - 1) + (int64_t)nx * ((int64_t)(j - 1) + (int64_t)ny * (k - 1))] : 0.0;
+    };
     TailArgs &t = d.tail;
     t.nterms = 0;
     t.terms[t.nterms++] = {cell(is, js, ks), SDT_F_SXX, SDT_SRC_XX, 1.0, R(is, js, ks), 0};
@@ -365,21 +155,103 @@ int el3_setup(Device &d, const double *const *in)
     t.sfix[3] = cell(is, js + 1, ks + 1);
     t.rec_field[0] = SDT_F_VX; t.rec_field[1] = SDT_F_VY; t.rec_field[2] = SDT_F_VZ;
 
-    d.tile_x = 64;
-    d.tile_y = 4;
-    d.tile_z = 1;
-    int rc = compute_tile_flags(d, in);
-    if (rc) return rc;
-    // The branch-free interior kernels need every nest active on the box
-    // (cpmlfdtd.f08:1761-2127 ranges intersect to [3, n-2] per axis) and no
-    // CPML cell inside it; otherwise every cell takes the shell kernels.
-    const int lim[3] = {d.nx, d.ny, d.nz};
-    for (int a = 0; a < 3; ++a) {
-        d.box[2 * a] = d.box[2 * a] < 3 ? 3 : d.box[2 * a];
-        d.box[2 * a + 1] = d.box[2 * a + 1] > lim[a] - 2 ? lim[a] - 2 : d.box[2 * a + 1];
+    return el3_build_shell(d, in, d.zchunk > 0 ? d.zchunk : 16);
+}
+
+// CPML-shell tiles (see El3Tile) and their compact memory-variable and CPML
+// coefficient storage.  Tiles are listed plane-chunk by plane-chunk, rows
+// then columns, so that consecutive tiles (which the XCD remapping keeps on
+// one XCD) share stencil halos.
+int el3_build_shell(Device &d, const double *const *in, int kc)
+{
+    std::vector<El3Tile> tiles;
+    const int *b = d.box;
+    const bool empty = b[1] < b[0];
+    const int line = (int)(128 / d.esize);
+    // x origin of line-aligned tiles: cell i sits at row element xoff + i - 1
+    const int xa = 1 - (int)d.xoff;
+    int64_t base = 0;
+    auto region = [&](int kind, int i, int j) {
+        if (i < 1 || i > d.nx || j < 1 || j > d.ny) return false;
+        if (kind == 0) return true;
+        if (kind == 1) return j < b[2] || j > b[3];
+        return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
+    };
+    auto add = [&](int x0, int y0, int k0, int k1, int kind, int txl) {
+        const int tw = 1 << txl, th = 256 >> txl;
+        for (int lane = 0; lane < 256; ++lane) {
+            const int i = x0 + (lane & (tw - 1)), j = y0 + (lane >> txl);
+            if (region(kind, i, j)) {
+                tiles.push_back(El3Tile{x0, y0, k0, k1, kind, txl, base, i, j, lane});
+                base += (int64_t)(k1 - k0 + 1) * 256;
+                return;
+            }
+        }
+        (void)th;
+    };
+    auto zslab = [&](int ka, int kb) {
+        for (int k0 = ka; k0 <= kb; k0 += kc) {
+            const int k1 = k0 + kc - 1 < kb ? k0 + kc - 1 : kb;
+            for (int y0 = 1; y0 <= d.ny; y0 += 4)
+                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 0, 6);
+        }
+    };
+    // 16-wide tiles aligned to the line grid that cover columns [lo, hi]
+    auto xstrip = [&](int lo, int hi, int y0, int k0, int k1) {
+        if (lo > hi) return;
+        int x0 = xa + ((lo - xa) / line) * line;
+        for (; x0 <= hi; x0 += 16) add(x0, y0, k0, k1, 2, 4);
+    };
+    if (empty) {
+        zslab(1, d.nz);
+    } else {
+        zslab(1, b[4] - 1);
+        for (int k0 = b[4]; k0 <= b[5]; k0 += kc) {
+            const int k1 = k0 + kc - 1 < b[5] ? k0 + kc - 1 : b[5];
+            for (int y0 = 1; y0 < b[2]; y0 += 4)
+                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 6);
+            for (int y0 = b[2]; y0 <= b[3]; y0 += 16) {
+                xstrip(1, b[0] - 1, y0, k0, k1);
+                xstrip(b[1] + 1, d.nx, y0, k0, k1);
+            }
+            for (int y0 = b[3] + 1; y0 <= d.ny; y0 += 4)
+                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 6);
+        }
+        zslab(b[5] + 1, d.nz);
     }
-    if (d.nflagged > 0) {
-        for (int a = 0; a < 3; ++a) { d.box[2 * a] = 1; d.box[2 * a + 1] = 0; }
+    d.ntiles = (int)tiles.size();
+    d.nslots = base;
+    if (d.ntiles == 0) return SDT_OK;
+    if (hipMalloc(&d.tiles, tiles.size() * sizeof(El3Tile)) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(d.tiles, tiles.data(), tiles.size() * sizeof(El3Tile), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    // compact memory variables (zero) and CPML coefficients (gathered)
+    d.nmem = M_TOTAL;
+    for (int q = 0; q < d.nmem; ++q) {
+        if (q == 0 * K_NKIND + K_ZMH) continue;   // dead recursion, never touched
+        int rc = alloc_zeroed(d, &d.mem[q], d.nslots);
+        if (rc) return rc;
+    }
+    const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
+    std::vector<double> tmp((size_t)d.nslots);
+    for (int q = 0; q < 6; ++q) {
+        const double *src = in[cids[q]];
+        const bool inverse = (q == 2 || q == 5);
+        const double dflt = inverse ? 1.0 : 0.0;
+        for (const El3Tile &t : tiles) {
+            const int tw = 1 << t.txl, th = 256 >> t.txl;
+            for (int k = t.k0; k <= t.k1; ++k)
+                for (int ty = 0; ty < th; ++ty)
+                    for (int tx = 0; tx < tw; ++tx) {
+                        const int i = t.x0 + tx, j = t.y0 + ty;
+                        double v = dflt;
+                        if (src && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny)
+                            v = src[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))];
+                        tmp[(size_t)(t.base + (int64_t)(k - t.k0) * 256 + ty * tw + tx)] = inverse ? 1.0 / v : v;
+                    }
+        }
+        int rc = upload_converted(d, tmp.data(), &d.cpml[q], d.nslots);
+        if (rc) return rc;
     }
     return SDT_OK;
 }

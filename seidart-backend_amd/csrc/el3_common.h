@@ -23,7 +23,8 @@ struct El3Args {
     T *m[M_TOTAL];
     const uint8_t *flags;
     int box[6];
-    int nx, ny, nz, tiles_x, tiles_y;
+    int nx, ny, nz, tiles_x, tiles_y, pitch;
+    int64_t xoff;
     T rd24[3];      // 1/(24 dx), 1/(24 dy), 1/(24 dz)
     T dt;
     int density_code, corrected;
@@ -74,6 +75,17 @@ __device__ __forceinline__ T tplus5(T fm1, T f0, T fp1, T fp2, int corr)
 {
     return corr ? T(27) * (fp1 - f0) - (fp2 - fm1) : -T(27) * f0 + T(27) * fp1 - fp2;
 }
+
+// One z-marching tile of the CPML shell: x-y footprint (1 << txl) x
+// (256 >> txl) cells from (x0, y0), planes k0..k1, region kind (0 z slab,
+// 1 y strip, 2 x strip) and the first compact slot of its memory variables.
+struct El3Tile {
+    int x0, y0, k0, k1;
+    int kind, txl;
+    int64_t base;
+    int ax, ay, alane;   // an active cell of the tile and its lane: inactive lanes
+                         // load from it, so they add no memory traffic
+};
 
 // Interior (branch-free, z-marching) and CPML-shell kernels, el3_fast.hip.
 template <typename T>

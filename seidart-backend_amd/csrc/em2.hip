@@ -27,7 +27,8 @@ struct Em2Args {
     T *m_dEx_dz, *m_dEz_dx, *m_dHy_dx, *m_dHy_dz;
     const uint8_t *flags;
     int box[6];
-    int nx, nz, tiles_x;
+    int nx, nz, tiles_x, pitch;
+    int64_t xoff;
     T rdx, rdz, dt, dbHy;
     const int *d_it;
     unsigned long long *vnorm;
@@ -38,10 +39,10 @@ template <typename T, bool PML>
 __device__ __forceinline__ void em2_h_cell(const Em2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
-    const int64_t c = (int64_t)(i - 1) + (int64_t)nx * (j - 1);
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     if (i == nx || j == nz) { A.Hy[c] = T(0); return; }
     // cpmlfdtd.f08:2576-2593
-    T dEx_dz = (A.Ex[c + nx] - A.Ex[c]) * A.rdz;
+    T dEx_dz = (A.Ex[c + A.pitch] - A.Ex[c]) * A.rdz;
     T dEz_dx = (A.Ez[c + 1] - A.Ez[c]) * A.rdx;
     if (PML) {
         A.m_dEx_dz[c] = A.b[c] * A.m_dEx_dz[c] + A.a[c] * dEx_dz;
@@ -67,13 +68,13 @@ template <typename T, bool PML>
 __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
-    const int64_t c = (int64_t)(i - 1) + (int64_t)nx * (j - 1);
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     T ex, ez;
     if (i == 1 || i == nx || j == 1 || j == nz) {
         ex = ez = T(0);
     } else {
         // cpmlfdtd.f08:2605-2620
-        T dHy_dz = (A.Hy[c] - A.Hy[c - nx]) * A.rdz;
+        T dHy_dz = (A.Hy[c] - A.Hy[c - A.pitch]) * A.rdz;
         T dHy_dx = (A.Hy[c] - A.Hy[c - 1]) * A.rdx;
         if (PML) {
             A.m_dHy_dx[c] = A.bh[c] * A.m_dHy_dx[c] + A.ah[c] * dHy_dx;
@@ -129,6 +130,8 @@ static Em2Args<T> em2_args(Device &d)
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx;
     A.nz = d.nz;
+    A.pitch = d.pitch;
+    A.xoff = d.xoff;
     A.tiles_x = d.tiles_x;
     A.rdx = (T)(1.0 / d.dx);
     A.rdz = (T)(1.0 / d.dz);
@@ -164,6 +167,12 @@ void em2_step(Device &d, bool timed)
 // stored as aEx, bEx (= aEz), bEz, dEx, eEx, dEz, eEz.
 int em2_setup(Device &d, const double *const *in)
 {
+    d.tile_x = 64;
+    d.tile_y = 1;
+    d.tile_z = 4;
+    int rc = compute_tile_flags(d, in);
+    if (rc) return rc;
+    set_layout(d);
     const int64_t n = d.ncell;
     std::vector<double> co[7];
     for (auto &v : co) v.resize((size_t)n);
@@ -183,31 +192,31 @@ int em2_setup(Device &d, const double *const *in)
     }
     d.nmat = 0;
     for (int q = 0; q < 7; ++q) {
-        int rc = upload_converted(d, co[q].data(), &d.mat[d.nmat++], n);
+        rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     d.nmem = 4;
     for (int q = 0; q < d.nmem; ++q) {
-        int rc = alloc_zeroed(d, &d.mem[q], n);
+        rc = alloc_grid_zeroed(d, &d.mem[q]);
         if (rc) return rc;
     }
     const int fids[] = {SDT_F_EX, SDT_F_EZ, SDT_F_HY};
     for (int f : fids) {
-        int rc;
-        if (f == SDT_F_EX && in[SDT_IC_X]) rc = upload_converted(d, in[SDT_IC_X], &d.field[f], n);
-        else if (f == SDT_F_EZ && in[SDT_IC_Z]) rc = upload_converted(d, in[SDT_IC_Z], &d.field[f], n);
-        else rc = alloc_zeroed(d, &d.field[f], n);
+        if (f == SDT_F_EX && in[SDT_IC_X]) rc = upload_grid(d, in[SDT_IC_X], &d.field[f]);
+        else if (f == SDT_F_EZ && in[SDT_IC_Z]) rc = upload_grid(d, in[SDT_IC_Z], &d.field[f]);
+        else rc = alloc_grid_zeroed(d, &d.field[f]);
         if (rc) return rc;
     }
 
     // point source, cpmlfdtd.f08:2678-2681
     const int is = d.isrc, js = d.ksrc, nx = d.nx, nz = d.nz;
-    const int64_t c = (int64_t)(is - 1) + (int64_t)nx * (js - 1);
+    const int64_t hc = (int64_t)(is - 1) + (int64_t)nx * (js - 1);
+    const int64_t c = cell_index(d, is, 1, js);
     const int face = (is == 1 || is == nx || js == 1 || js == nz);
     TailArgs &t = d.tail;
     t.nterms = 0;
-    t.terms[t.nterms++] = {c, SDT_F_EX, SDT_SRC_X, d.dt, val(SDT_E11, c) * kEps0, face};
-    t.terms[t.nterms++] = {c, SDT_F_EZ, SDT_SRC_Z, d.dt, val(SDT_E33, c) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EX, SDT_SRC_X, d.dt, val(SDT_E11, hc) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EZ, SDT_SRC_Z, d.dt, val(SDT_E33, hc) * kEps0, face};
     t.nvfix = 1;
     t.vfix[0] = c;
     t.nvcomp = 2;
@@ -219,10 +228,7 @@ int em2_setup(Device &d, const double *const *in)
     t.rec_field[1] = -1;
     t.rec_field[2] = SDT_F_EZ;
 
-    d.tile_x = 64;
-    d.tile_y = 1;
-    d.tile_z = 4;
-    return compute_tile_flags(d, in);
+    return upload_cpml_full(d, in);
 }
 
 }  // namespace sdt

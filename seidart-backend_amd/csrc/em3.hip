@@ -27,7 +27,8 @@ struct Em3Args {
     T *m_dHz_dy, *m_dHy_dz, *m_dHz_dx, *m_dHx_dz, *m_dHx_dy, *m_dHy_dx;
     const uint8_t *flags;
     int box[6];
-    int nx, ny, nz, tiles_x, tiles_y;
+    int nx, ny, nz, tiles_x, tiles_y, pitch;
+    int64_t xoff;
     T rdx, rdy, rdz, dt, dbH;
     const int *d_it;
     unsigned long long *vnorm;
@@ -38,8 +39,8 @@ template <typename T, bool PML>
 __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
     const bool face = i == 1 || j == 1 || k == 1;
     const T *Ex = A.E[0], *Ey = A.E[1], *Ez = A.E[2];
     if (i == nx || j == ny || k == nz) {
@@ -107,8 +108,8 @@ template <typename T, bool PML>
 __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    const int64_t c = (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
     T ex, ey, ez;
     if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
         ex = ey = ez = T(0);
@@ -191,6 +192,7 @@ static Em3Args<T> em3_args(Device &d)
     A.flags = d.tile_flags;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
+    A.pitch = d.pitch; A.xoff = d.xoff;
     A.tiles_x = d.tiles_x; A.tiles_y = d.tiles_y;
     A.rdx = (T)(1.0 / d.dx);
     A.rdy = (T)(1.0 / d.dy);
@@ -226,6 +228,12 @@ void em3_step(Device &d, bool timed)
 // FP64 on the host; stored as the six unique entries.
 int em3_setup(Device &d, const double *const *in)
 {
+    d.tile_x = 64;
+    d.tile_y = 4;
+    d.tile_z = 1;
+    int rc = compute_tile_flags(d, in);
+    if (rc) return rc;
+    set_layout(d);
     const int64_t n = d.ncell;
     std::vector<double> co[6];
     for (auto &v : co) v.resize((size_t)n);
@@ -244,36 +252,36 @@ int em3_setup(Device &d, const double *const *in)
     }
     d.nmat = 0;
     for (int q = 0; q < 6; ++q) {
-        int rc = upload_converted(d, co[q].data(), &d.mat[d.nmat++], n);
+        rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     std::vector<double> zero;
     for (int id = SDT_S11; id <= SDT_S33; ++id) {
         if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
-        int rc = upload_converted(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++], n);
+        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
         if (rc) return rc;
     }
     d.nmem = 12;
     for (int q = 0; q < d.nmem; ++q) {
-        int rc = alloc_zeroed(d, &d.mem[q], n);
+        rc = alloc_grid_zeroed(d, &d.mem[q]);
         if (rc) return rc;
     }
     for (int f = SDT_F_EX; f <= SDT_F_HZ; ++f) {
-        int rc;
-        if (f <= SDT_F_EZ && in[SDT_IC_X + f]) rc = upload_converted(d, in[SDT_IC_X + f], &d.field[f], n);
-        else rc = alloc_zeroed(d, &d.field[f], n);
+        if (f <= SDT_F_EZ && in[SDT_IC_X + f]) rc = upload_grid(d, in[SDT_IC_X + f], &d.field[f]);
+        else rc = alloc_grid_zeroed(d, &d.field[f]);
         if (rc) return rc;
     }
 
     // point source, cpmlfdtd.f08:3909-3914
     const int is = d.isrc, js = d.jsrc, ks = d.ksrc, nx = d.nx, ny = d.ny, nz = d.nz;
-    const int64_t c = (int64_t)(is - 1) + (int64_t)nx * ((int64_t)(js - 1) + (int64_t)ny * (ks - 1));
+    const int64_t hc = (int64_t)(is - 1) + (int64_t)nx * ((int64_t)(js - 1) + (int64_t)ny * (ks - 1));
+    const int64_t c = cell_index(d, is, js, ks);
     const int face = (is == 1 || is == nx || js == 1 || js == ny || ks == 1 || ks == nz);
     TailArgs &t = d.tail;
     t.nterms = 0;
-    t.terms[t.nterms++] = {c, SDT_F_EX, SDT_SRC_X, d.dt, val(SDT_E11, c) * kEps0, face};
-    t.terms[t.nterms++] = {c, SDT_F_EY, SDT_SRC_Y, d.dt, val(SDT_E22, c) * kEps0, face};
-    t.terms[t.nterms++] = {c, SDT_F_EZ, SDT_SRC_Z, d.dt, val(SDT_E33, c) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EX, SDT_SRC_X, d.dt, val(SDT_E11, hc) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EY, SDT_SRC_Y, d.dt, val(SDT_E22, hc) * kEps0, face};
+    t.terms[t.nterms++] = {c, SDT_F_EZ, SDT_SRC_Z, d.dt, val(SDT_E33, hc) * kEps0, face};
     t.nvfix = 1;
     t.vfix[0] = c;
     t.nvcomp = 3;
@@ -282,10 +290,7 @@ int em3_setup(Device &d, const double *const *in)
     t.nsfix = 0;
     t.rec_field[0] = SDT_F_EX; t.rec_field[1] = SDT_F_EY; t.rec_field[2] = SDT_F_EZ;
 
-    d.tile_x = 64;
-    d.tile_y = 4;
-    d.tile_z = 1;
-    return compute_tile_flags(d, in);
+    return upload_cpml_full(d, in);
 }
 
 }  // namespace sdt

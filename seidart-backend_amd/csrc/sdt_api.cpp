@@ -20,7 +20,7 @@ struct sdt_handle {
     sdt_params p{};
     std::vector<double> staged[SDT_NUM_ARRAYS];
     std::vector<double> series;       // [SDT_NUM_SRC][time_steps]
-    std::vector<int64_t> rec_cells;
+    std::vector<int32_t> rec_ijk;     // padded 1-based (i, j, k) per receiver
     bool prepared = false;
     bool failed = false;
     int steps_done = 0;
@@ -144,6 +144,7 @@ static void release(sdt_handle *h)
     auto drop = [](void *p) { if (p) (void)hipFree(p); };
     free_grids(d);   // fields, memory variables, materials, CPML coefficients
     drop(d.tile_flags);
+    drop(d.tiles);
     drop(d.series);
     drop(d.norms);
     drop(d.d_it);
@@ -187,16 +188,18 @@ int32_t sdt_set_receivers(sdt_handle *h, int32_t n, const int32_t *ijk)
     if (h->prepared) return fail(h, SDT_ERR_STATE, "receivers are fixed once the first step has run");
     const Device &d = h->d;
     const bool d3 = is3d(d.physics);
-    std::vector<int64_t> cells((size_t)n);
+    std::vector<int32_t> cells((size_t)n * 3);
     for (int q = 0; q < n; ++q) {
         const int i = ijk[3 * q] + h->p.cpml;
         const int j = d3 ? ijk[3 * q + 1] + h->p.cpml : 1;
         const int k = ijk[3 * q + 2] + h->p.cpml;
         if (i < 1 || i > d.nx || j < 1 || j > d.ny || k < 1 || k > d.nz)
             return fail(h, SDT_ERR_ARG, "receiver " + std::to_string(q) + " outside the padded grid");
-        cells[(size_t)q] = (int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1));
+        cells[3 * (size_t)q] = i;
+        cells[3 * (size_t)q + 1] = j;
+        cells[3 * (size_t)q + 2] = k;
     }
-    h->rec_cells = cells;
+    h->rec_ijk = cells;
     return SDT_OK;
 }
 
@@ -217,20 +220,6 @@ static int prepare(sdt_handle *h)
     case SDT_EM3: rc = em3_setup(d, in); break;
     }
     if (rc) return fail(h, rc, "device setup failed (out of device memory?)");
-    // CPML coefficients: a, b and 1/kappa for the full and half sets
-    const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
-    std::vector<double> tmp((size_t)d.ncell);
-    for (int q = 0; q < 6; ++q) {
-        const double *src = in[cids[q]];
-        const bool inverse = (q == 2 || q == 5);
-        for (int64_t c = 0; c < d.ncell; ++c) {
-            const double v = src ? src[c] : (inverse ? 1.0 : 0.0);
-            tmp[(size_t)c] = inverse ? 1.0 / v : v;
-        }
-        if (upload_converted(d, tmp.data(), &d.cpml[q], d.ncell)) return fail(h, SDT_ERR_HIP, "cpml upload failed");
-    }
-    tmp.clear();
-    tmp.shrink_to_fit();
     // source series, norms, step counter, receivers
     const size_t ns = (size_t)SDT_NUM_SRC * d.time_steps;
     HIPCHK(h, hipMalloc((void **)&d.series, ns * sizeof(double)));
@@ -240,10 +229,14 @@ static int prepare(sdt_handle *h)
     HIPCHK(h, hipMalloc((void **)&d.d_it, sizeof(int)));
     const int one = 1;
     HIPCHK(h, hipMemcpy(d.d_it, &one, sizeof(int), hipMemcpyHostToDevice));
-    d.nrec = (int)h->rec_cells.size();
+    d.nrec = (int)(h->rec_ijk.size() / 3);
     if (d.nrec > 0) {
+        std::vector<int64_t> cells((size_t)d.nrec);
+        for (int q = 0; q < d.nrec; ++q)
+            cells[(size_t)q] = cell_index(d, h->rec_ijk[3 * (size_t)q], h->rec_ijk[3 * (size_t)q + 1],
+                                          h->rec_ijk[3 * (size_t)q + 2]);
         HIPCHK(h, hipMalloc((void **)&d.rec_cell, (size_t)d.nrec * sizeof(int64_t)));
-        HIPCHK(h, hipMemcpy(d.rec_cell, h->rec_cells.data(), (size_t)d.nrec * sizeof(int64_t), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(d.rec_cell, cells.data(), (size_t)d.nrec * sizeof(int64_t), hipMemcpyHostToDevice));
         const size_t nr = (size_t)d.time_steps * d.nrec * 3;
         HIPCHK(h, hipMalloc((void **)&d.rec_out, nr * sizeof(double)));
         HIPCHK(h, hipMemset(d.rec_out, 0, nr * sizeof(double)));
@@ -389,13 +382,7 @@ int32_t sdt_get_field(sdt_handle *h, int32_t f, double *out, int64_t count)
     Device &d = h->d;
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(d.stream));
-    if (d.precision == SDT_FP64) {
-        HIPCHK(h, hipMemcpy(out, d.field[f], (size_t)count * 8, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> tmp((size_t)count);
-        HIPCHK(h, hipMemcpy(tmp.data(), d.field[f], (size_t)count * 4, hipMemcpyDeviceToHost));
-        for (int64_t c = 0; c < count; ++c) out[c] = tmp[(size_t)c];
-    }
+    if (download_grid(d, d.field[f], out) != SDT_OK) return fail(h, SDT_ERR_HIP, "field read-back failed");
     // lazily applied Dirichlet faces of H (em2.hip / em3.hip)
     const bool hfield = (d.physics == SDT_EM2 && f == SDT_F_HY) || (d.physics == SDT_EM3 && f >= SDT_F_HX);
     if (hfield && h->steps_done > 0) {
@@ -420,20 +407,14 @@ int32_t sdt_set_field(sdt_handle *h, int32_t f, const double *host, int64_t coun
     Device &d = h->d;
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(d.stream));
-    if (d.precision == SDT_FP64) {
-        HIPCHK(h, hipMemcpy(d.field[f], host, (size_t)count * 8, hipMemcpyHostToDevice));
-    } else {
-        std::vector<float> tmp((size_t)count);
-        for (int64_t c = 0; c < count; ++c) tmp[(size_t)c] = (float)host[c];
-        HIPCHK(h, hipMemcpy(d.field[f], tmp.data(), (size_t)count * 4, hipMemcpyHostToDevice));
-    }
+    if (store_grid(d, d.field[f], host) != SDT_OK) return fail(h, SDT_ERR_HIP, "field upload failed");
     return SDT_OK;
 }
 
 int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count)
 {
     if (!h || (!out && count)) return fail(h, SDT_ERR_ARG, "null argument");
-    const int64_t need = (int64_t)h->steps_done * (int64_t)h->rec_cells.size() * 3;
+    const int64_t need = (int64_t)h->steps_done * (int64_t)(h->rec_ijk.size() / 3) * 3;
     if (count != need) return fail(h, SDT_ERR_ARG, "count must equal steps_done * n_receivers * 3");
     if (need == 0) return SDT_OK;
     HIPCHK(h, hipSetDevice(h->p.device));
@@ -481,15 +462,15 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
 {
     if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
     switch (key) {
-    case SDT_OPT_KERNEL_VARIANT:
-        if (h->prepared) return fail(h, SDT_ERR_STATE, "kernel variant is fixed once the first step has run");
-        if (value < 0 || value > 2) return fail(h, SDT_ERR_ARG, "kernel variant must be 0, 1 or 2");
-        h->d.variant = (int)value;
-        return SDT_OK;
     case SDT_OPT_ZCHUNK:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "z chunk is fixed once the first step has run");
         if (value < 0 || value > 1 << 20) return fail(h, SDT_ERR_ARG, "z chunk out of range");
         h->d.zchunk = (int)value;
+        return SDT_OK;
+    case SDT_OPT_TILE_ROWS:
+        if (value != 0 && value != 4 && value != 8 && value != 16)
+            return fail(h, SDT_ERR_ARG, "tile rows must be 0 (auto), 4, 8 or 16");
+        h->d.tile_rows = (int)value;
         return SDT_OK;
     case SDT_OPT_GRAPH:
         h->d.graphs = value != 0;

@@ -122,6 +122,86 @@ int alloc_zeroed(Device &d, void **dev, int64_t n)
     return SDT_OK;
 }
 
+int upload_cpml_full(Device &d, const double *const *in)
+{
+    const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
+    std::vector<double> tmp((size_t)d.ncell);
+    for (int q = 0; q < 6; ++q) {
+        const double *src = in[cids[q]];
+        const bool inverse = (q == 2 || q == 5);
+        for (int64_t c = 0; c < d.ncell; ++c) {
+            const double v = src ? src[c] : (inverse ? 1.0 : 0.0);
+            tmp[(size_t)c] = inverse ? 1.0 / v : v;
+        }
+        int rc = upload_grid(d, tmp.data(), &d.cpml[q]);
+        if (rc) return rc;
+    }
+    return SDT_OK;
+}
+
+void set_layout(Device &d)
+{
+    const int line = (int)(128 / d.esize);
+    const bool empty = d.box[1] < d.box[0];
+    const int first = empty ? 1 : d.box[0];
+    d.xoff = (line - (first - 1) % line) % line;
+    d.pitch = (int)((d.xoff + d.nx + line - 1) / line * line);
+    d.ncell_dev = (int64_t)d.pitch * d.ny * d.nz;
+}
+
+// rows of the grid = ny * nz, each nx elements on the host, pitch on the device
+int upload_grid(Device &d, const double *host, void **dev)
+{
+    const size_t bytes = (size_t)d.ncell_dev * d.esize;
+    if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
+    if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
+    return store_grid(d, *dev, host);
+}
+
+int alloc_grid_zeroed(Device &d, void **dev)
+{
+    const size_t bytes = (size_t)d.ncell_dev * d.esize;
+    if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
+    if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+int store_grid(Device &d, void *dev, const double *host)
+{
+    const size_t rows = (size_t)d.ny * d.nz, es = d.esize;
+    char *dst = (char *)dev + d.xoff * es;
+    if (d.precision == SDT_FP64) {
+        if (hipMemcpy2D(dst, (size_t)d.pitch * es, host, (size_t)d.nx * es, (size_t)d.nx * es, rows,
+                        hipMemcpyHostToDevice) != hipSuccess)
+            return SDT_ERR_HIP;
+    } else {
+        std::vector<float> tmp((size_t)d.ncell);
+        for (int64_t i = 0; i < d.ncell; ++i) tmp[(size_t)i] = (float)host[i];
+        if (hipMemcpy2D(dst, (size_t)d.pitch * es, tmp.data(), (size_t)d.nx * es, (size_t)d.nx * es, rows,
+                        hipMemcpyHostToDevice) != hipSuccess)
+            return SDT_ERR_HIP;
+    }
+    return SDT_OK;
+}
+
+int download_grid(Device &d, const void *dev, double *host)
+{
+    const size_t rows = (size_t)d.ny * d.nz, es = d.esize;
+    const char *src = (const char *)dev + d.xoff * es;
+    if (d.precision == SDT_FP64) {
+        if (hipMemcpy2D(host, (size_t)d.nx * es, src, (size_t)d.pitch * es, (size_t)d.nx * es, rows,
+                        hipMemcpyDeviceToHost) != hipSuccess)
+            return SDT_ERR_HIP;
+    } else {
+        std::vector<float> tmp((size_t)d.ncell);
+        if (hipMemcpy2D(tmp.data(), (size_t)d.nx * es, src, (size_t)d.pitch * es, (size_t)d.nx * es, rows,
+                        hipMemcpyDeviceToHost) != hipSuccess)
+            return SDT_ERR_HIP;
+        for (int64_t i = 0; i < d.ncell; ++i) host[i] = tmp[(size_t)i];
+    }
+    return SDT_OK;
+}
+
 static inline bool trivial(double a, double b, double kappa)
 {
     return a == 0.0 && kappa == 1.0 && isfinite(b);

@@ -57,6 +57,14 @@ struct Device {
     int density_code = 3, stencil = 0;
     int64_t ncell = 0;
     size_t esize = 8;                  // sizeof(T)
+    // Device layout of grid arrays: row (j, k) holds x cells 1..nx at
+    // elements xoff .. xoff+nx-1 of a row of `pitch` elements; pitch is a
+    // multiple of a 128-byte cache line and xoff puts the first interior
+    // column on a line boundary.  Cell (i, j, k) -> xoff + (i-1) +
+    // pitch * ((j-1) + ny * (k-1)).
+    int pitch = 0;
+    int64_t xoff = 0;
+    int64_t ncell_dev = 0;
 
     void *field[SDT_NUM_FIELDS] = {};
     void *mem[kMaxMem] = {};
@@ -66,6 +74,9 @@ struct Device {
     void *cpml[6] = {};                // a, b, 1/kappa (full), a, b, 1/kappa (half)
     uint8_t *tile_flags = nullptr;     // 1 = tile holds a CPML cell inside the interior box
     int64_t nflagged = 0;              // tiles flagged inside the box
+    void *tiles = nullptr;             // 3D elastic CPML-shell tile list (device)
+    int ntiles = 0;
+    int64_t nslots = 0;                // compact CPML storage length
     int box[6] = {1, 0, 1, 0, 1, 0};   // interior box ilo,ihi,jlo,jhi,klo,khi (1-based, inclusive):
                                        // every cell inside has identity CPML coefficients
     int tiles_x = 0, tiles_y = 0, tiles_z = 0;
@@ -79,8 +90,8 @@ struct Device {
     int nrec = 0;
 
     TailArgs tail{};
-    int variant = 0;                   // SDT_OPT_KERNEL_VARIANT
     int zchunk = 0;                    // SDT_OPT_ZCHUNK
+    int tile_rows = 0;                 // SDT_OPT_TILE_ROWS
     bool graphs = true;                // SDT_OPT_GRAPH
     bool stagger = true;               // SDT_OPT_STAGGER
     int nalloc = 0;                    // allocations made (stagger index)
@@ -120,6 +131,22 @@ void launch_tail(Device &d);
 // a non-identity cell inside the box, whose cells then take the CPML body
 // too.  Kernels test: cpml(cell) = !inside(box) || tile_flags[tile].
 int compute_tile_flags(Device &d, const double *const *host_in);
+// full-grid CPML coefficient arrays: a, b, 1/kappa for the full and half
+// sets (identity defaults where not supplied)
+int upload_cpml_full(Device &d, const double *const *host_in);
+// choose pitch / xoff from d.box (call after compute_tile_flags)
+void set_layout(Device &d);
+inline int64_t cell_index(const Device &d, int i, int j, int k)
+{
+    return d.xoff + (int64_t)(i - 1) + (int64_t)d.pitch * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1));
+}
+// grid arrays in the padded layout: upload an FP64 host array given in
+// Fortran order (nx, ny, nz), or allocate a zeroed one
+int upload_grid(Device &d, const double *host, void **dev);
+int alloc_grid_zeroed(Device &d, void **dev);
+// copy a device grid array to / from an FP64 host array in Fortran order
+int download_grid(Device &d, const void *dev, double *host);
+int store_grid(Device &d, void *dev, const double *host);
 // copy n FP64 host values into a freshly allocated device array of the
 // handle's precision
 int upload_converted(Device &d, const double *host, void **dev, int64_t n);

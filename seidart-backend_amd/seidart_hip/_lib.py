@@ -66,7 +66,7 @@ EXPORTS = [
     "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option",
 ]
 
-OPT_KERNEL_VARIANT, OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER = 0, 1, 2, 3
+OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS = 1, 2, 3, 4
 
 
 class SdtError(RuntimeError):

@@ -156,10 +156,10 @@ class Solver:
 
     # ---- tuning -----------------------------------------------------------------
     def set_option(self, key, value):
-        """key: 'variant' (0 auto, 1 one thread per cell, 2 z-marching),
-        'zchunk' (planes per workgroup, 0 auto), 'graph' (0/1) or 'stagger' (0/1)."""
-        k = {"variant": L.OPT_KERNEL_VARIANT, "zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH,
-             "stagger": L.OPT_STAGGER}[key]
+        """key: 'zchunk' (planes per z-marching workgroup, 0 auto), 'graph'
+        (0/1), 'stagger' (0/1) or 'tile_rows' (0 auto, 4, 8, 16)."""
+        k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
+             "tile_rows": L.OPT_TILE_ROWS}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     # ---- kernel timing --------------------------------------------------------

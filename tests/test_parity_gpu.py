@@ -64,11 +64,10 @@ def test_elastic2_stress_source_and_attenuation():
     _check(g, o, TOL64)
 
 
-@pytest.mark.parametrize("options", [{"variant": 1}, {"variant": 2}, {"variant": 2, "zchunk": 5},
-                                     {"variant": 2, "zchunk": 1}])
+@pytest.mark.parametrize("options", [{}, {"zchunk": 5}, {"zchunk": 1}, {"zchunk": 7, "stagger": 0}])
 def test_elastic3_small_fp64(options):
-    """Both kernel variants (one thread per cell; z-marching columns with
-    several chunk heights, so chunk seams fall inside and outside the CPML)."""
+    """z-marching chunk heights that put chunk seams inside and outside the
+    CPML slabs and the interior box."""
     from seidart_hip import models
     cfg = models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16)
     fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
