@@ -52,11 +52,33 @@ def measured_traffic():
     """HBM bytes per stress-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
     or None when no summary is present."""
-    path = os.path.join(ROOT, "profiles", "el3_stress_hbm_traffic.txt")
+    path = os.path.join(ROOT, "profiles", "el3_seismic3_280cube_fp64_hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read(), re.M)
     return float(m.group(1)) if m else None
+
+
+def calibrate_copy(nbytes=2 << 30, reps=20):
+    """Plain element-wise copy of a known size (one dispatch per copy) whose
+    rocprofv3 FETCH_SIZE / WRITE_SIZE readings scale the counters of the
+    solver kernels to bytes."""
+    import torch
+    n = nbytes // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.neg(x, out=y)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "ms": round(ms, 4), "GB/s": round(2 * n * 8 / ms / 1e6, 1)}), flush=True)
 
 
 def main():
@@ -70,7 +92,12 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0, help="planes per z-marching workgroup (0 = auto)")
     ap.add_argument("--stagger", type=int, default=1, help="staggered array starts (1) or plain (0)")
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per interior workgroup tile (0 = auto)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (one time-stepped grid per GPU); "
               "--gpus must be 1", file=sys.stderr)
