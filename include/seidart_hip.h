@@ -4,28 +4,31 @@
  *
  * The reference has no C-ABI: each solver is a Fortran subroutine taking
  * derived types and reading its inputs from files in the working directory
- * (src/fortran/cpmlfdtd.f08:57 seismic2, :1341 seismic3, :2317 electromag2,
- * :3353 electromag3).  The entry points below replace that contract one for
+ * (src/fortran/cpmlfdtd.f08:57 seismic2, :600 seismic25, :1341 seismic3,
+ * :2317 electromag2, :3353 electromag3).  The entry points below replace that contract one for
  * one:
  *
  *   Domain_Type / Source_Type (src/fortran/seidart_types.f08:13-33)
  *       -> sdt_params, passed to sdt_create
- *   material_rw2 / material_rw3 reads of c11.dat ... bcoef_half_cpml.dat,
- *   initialcondition*.dat (src/fortran/seidartio.f08:225-292, used at
- *   cpmlfdtd.f08:159-174,253-266,1501-1645,2413-2506,3481-3618)
+ *   material_rw2 / material_rw2c / material_rw3 reads of c11.dat ...
+ *   bcoef_half_cpml.dat, initialcondition*.dat (src/fortran/seidartio.f08:
+ *   225-292, used at cpmlfdtd.f08:159-174,253-266,763-872,1501-1645,
+ *   2413-2506,3481-3618)
  *       -> sdt_set_array
  *   loadsource of seismicsource*.dat / electromagneticsource*.dat
  *   (src/fortran/seidartio.f08:161-183, used at cpmlfdtd.f08:230-237,
  *   1606-1617, 2476-2479, 3581-3585)
  *       -> sdt_set_source
- *   the `do it = 1, time_steps` loops (cpmlfdtd.f08:311, 1747, 2566, 3689)
- *       -> sdt_run
+ *   the environment variable KY read by seismic25 (cpmlfdtd.f08:697-703)
+ *       -> sdt_params.ky
+ *   the `do it = 1, time_steps` loops (cpmlfdtd.f08:311, 931, 1747, 2566,
+ *   3689) -> sdt_run
  *   write_image2 / write_image3 snapshots (src/fortran/seidartio.f08:366-426)
  *       -> sdt_get_field (one call per step and component)
  *   write_array of velocity_norm.dat / stress_norm.dat (cpmlfdtd.f08:2282-2283)
  *       -> sdt_get_norms
- *   `stop 'code became unstable and blew up'` (cpmlfdtd.f08:567, 2263, 2740,
- *   3972) -> SDT_ERR_UNSTABLE returned with the step index
+ *   `stop 'code became unstable and blew up'` (cpmlfdtd.f08:567, 1292, 2263,
+ *   2740, 3972) -> SDT_ERR_UNSTABLE returned with the step index
  *
  * Receiver recording (sdt_set_receivers / sdt_get_receivers) has no reference
  * counterpart: the SeidarT frontend samples the snapshot files instead; the
@@ -38,6 +41,11 @@
  * cpmlfdtd.f08:178-179, 1533-1535).  Every pointer is borrowed for the
  * duration of the call.  No entry point exits the process.  One HIP stream per
  * handle; handles are independent, a single handle is not thread-safe.
+ *
+ * Complex values (2.5D elastic, SDT_ELASTIC25): fields, the initial-condition
+ * arrays and receiver samples are complex128 stored as interleaved (re, im)
+ * doubles, exactly numpy's complex128 memory layout; their counts are twice
+ * the real ones.  Material, CPML and source arrays stay real.
  */
 #ifndef SEIDART_HIP_H
 #define SEIDART_HIP_H
@@ -48,7 +56,7 @@
 extern "C" {
 #endif
 
-#define SDT_ABI_VERSION 1
+#define SDT_ABI_VERSION 2
 
 /* ---- status codes -------------------------------------------------------- */
 enum {
@@ -62,10 +70,11 @@ enum {
 
 /* ---- solver kinds (src/fortran/main.f08:90-119 dispatch) ---------------- */
 enum {
-    SDT_ELASTIC2 = 0, /* seismic2     cpmlfdtd.f08:57   */
-    SDT_ELASTIC3 = 2, /* seismic3     cpmlfdtd.f08:1341 */
-    SDT_EM2 = 3,      /* electromag2  cpmlfdtd.f08:2317 */
-    SDT_EM3 = 5       /* electromag3  cpmlfdtd.f08:3353 */
+    SDT_ELASTIC2 = 0,  /* seismic2     cpmlfdtd.f08:57   */
+    SDT_ELASTIC25 = 1, /* seismic25    cpmlfdtd.f08:600  */
+    SDT_ELASTIC3 = 2,  /* seismic3     cpmlfdtd.f08:1341 */
+    SDT_EM2 = 3,       /* electromag2  cpmlfdtd.f08:2317 */
+    SDT_EM3 = 5        /* electromag3  cpmlfdtd.f08:3353 */
 };
 
 enum { SDT_FP64 = 0, SDT_FP32 = 1 };
@@ -121,7 +130,7 @@ enum {
 
 typedef struct sdt_params {
     int32_t abi_version;   /* must be SDT_ABI_VERSION                       */
-    int32_t physics;       /* SDT_ELASTIC2 / SDT_ELASTIC3 / SDT_EM2 / SDT_EM3 */
+    int32_t physics;       /* SDT_ELASTIC2 / 25 / 3, SDT_EM2 / SDT_EM3      */
     int32_t precision;     /* SDT_FP64 or SDT_FP32                          */
     int32_t nx, ny, nz;    /* padded sizes; ny = 1 for 2D                   */
     double dx, dy, dz;     /* Domain_Type%dx..dz                            */
@@ -132,7 +141,10 @@ typedef struct sdt_params {
     int32_t density_code;  /* SDT_DENSITY_*, elastic only                   */
     int32_t stencil;       /* SDT_STENCIL_*                                 */
     int32_t device;        /* HIP device ordinal                            */
-    int32_t reserved[8];   /* zero                                          */
+    int32_t reserved0;     /* zero                                          */
+    double ky;             /* 2.5D elastic: y wavenumber (env KY,
+                              cpmlfdtd.f08:697-703); 0 otherwise             */
+    int32_t reserved[6];   /* zero                                          */
 } sdt_params;
 
 typedef struct sdt_handle sdt_handle;
@@ -147,7 +159,9 @@ const char *sdt_last_error(const sdt_handle *h);
 int32_t sdt_create(const sdt_params *p, sdt_handle **out);
 int32_t sdt_destroy(sdt_handle *h);
 
-/* inputs: count must equal nx*ny*nz (arrays) or time_steps (series).
+/* inputs: count must equal nx*ny*nz (arrays) or time_steps (series); for
+ * SDT_ELASTIC25 the SDT_IC_* arrays take either 2*nx*nz doubles (complex128,
+ * initialconditionV?.dat read by material_rw2c) or nx*nz (real part only).
  * Arrays not set take the reference's defaults (kappa = 1, everything else 0,
  * cpmlfdtd.f08:240-247). */
 int32_t sdt_set_array(sdt_handle *h, int32_t array_id, const double *host, int64_t count);
@@ -162,10 +176,12 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step);
 /* steps completed so far */
 int32_t sdt_steps_done(const sdt_handle *h, int32_t *steps);
 
-/* outputs (FP64 host copies; blocking) */
+/* outputs (FP64 host copies; blocking); count = nx*ny*nz, or 2*nx*nz for the
+ * complex fields of SDT_ELASTIC25 */
 int32_t sdt_get_field(sdt_handle *h, int32_t field_id, double *out, int64_t count);
 int32_t sdt_set_field(sdt_handle *h, int32_t field_id, const double *host, int64_t count);
-/* out = steps_done * n_receivers * 3 doubles, step-major */
+/* out = steps_done * n_receivers * 3 values, step-major (complex for
+ * SDT_ELASTIC25, i.e. 6 doubles per receiver and step) */
 int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count);
 /* per-step max norms: velocity (or E) and, for 3D elastic, stress; NULL skips */
 int32_t sdt_get_norms(sdt_handle *h, double *vel, double *stress, int64_t count);

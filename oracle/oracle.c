@@ -872,3 +872,296 @@ int32_t oracle_electromag3(oracle_run *r)
     for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
     return r->unstable_step ? SDT_ERR_UNSTABLE : SDT_OK;
 }
+
+/* ========================================================================
+ * array_averaging2d: averaging.f08:27-80.  The sweeps run in place in the
+ * reference's loop order, so '+i' and '+j' average with the neighbour that was
+ * already overwritten (a recursive smear) while '-i' and '-j' average with the
+ * neighbour that is still original.
+ * ======================================================================== */
+void oracle_array_averaging2d(double *a, int32_t ni, int32_t nj, int32_t code, int32_t direction)
+{
+#define A2(i, j) a[(size_t)((i) - 1) + (size_t)ni * (size_t)((j) - 1)]
+    switch (direction) {
+    case 0:
+        for (int j = 1; j <= nj; ++j)
+            for (int i = 2; i <= ni; ++i) A2(i, j) = oracle_scalar_mean(A2(i, j), A2(i - 1, j), code);
+        break;
+    case 1:
+        for (int j = 1; j <= nj; ++j)
+            for (int i = 1; i <= ni - 1; ++i) A2(i, j) = oracle_scalar_mean(A2(i, j), A2(i + 1, j), code);
+        break;
+    case 2:
+        for (int j = 2; j <= nj; ++j)
+            for (int i = 1; i <= ni; ++i) A2(i, j) = oracle_scalar_mean(A2(i, j), A2(i, j - 1), code);
+        break;
+    case 3:
+        for (int j = 1; j <= nj - 1; ++j)
+            for (int i = 1; i <= ni; ++i) A2(i, j) = oracle_scalar_mean(A2(i, j), A2(i, j + 1), code);
+        break;
+    }
+#undef A2
+}
+
+/* ========================================================================
+ * seismic25: cpmlfdtd.f08:600-1337 (time loop 931-1305).  Complex fields as
+ * (re, im) pairs; the y derivative is i*ky*f, i.e. (-ky*im, ky*re), which is
+ * what (0,1)*ky*f evaluates to (cpmlfdtd.f08:946-948).  Real coefficients
+ * multiply and divide both parts.  Memory variables are kept per nest as in
+ * the reference (24 stress-side + 6 velocity-side complex arrays).
+ * ======================================================================== */
+typedef struct { double re, im; } cplx_t;
+
+static inline cplx_t cx(double re, double im) { cplx_t z = {re, im}; return z; }
+static inline cplx_t cadd(cplx_t a, cplx_t b) { return cx(a.re + b.re, a.im + b.im); }
+static inline cplx_t crmul(double s, cplx_t a) { return cx(s * a.re, s * a.im); }
+static inline cplx_t cdivr(cplx_t a, double s) { return cx(a.re / s, a.im / s); }
+static inline cplx_t cmulr(cplx_t a, double s) { return cx(a.re * s, a.im * s); }
+static inline cplx_t ciky(double ky, cplx_t a) { return cx(-ky * a.im, ky * a.re); }
+/* 4th-order staggered difference (27(f1-f0) - (f2-fm1)) on both parts */
+static inline cplx_t cd4(cplx_t fm1, cplx_t f0, cplx_t f1, cplx_t f2, double h24)
+{
+    return cx((27.0 * (f1.re - f0.re) - (f2.re - fm1.re)) / h24, (27.0 * (f1.im - f0.im) - (f2.im - fm1.im)) / h24);
+}
+/* psi = b*psi + a*d ; d = d/kappa + psi */
+static inline cplx_t ccpml(cplx_t d, cplx_t *m, double a, double b, double kap)
+{
+    m->re = b * m->re + a * d.re;
+    m->im = b * m->im + a * d.im;
+    return cx(d.re / kap + m->re, d.im / kap + m->im);
+}
+static inline cplx_t ctminus(cplx_t sm2, cplx_t sm1, cplx_t s0, cplx_t sp1, int corr, double h24)
+{
+    return cx(tminus(sm2.re, sm1.re, s0.re, sp1.re, corr) / h24, tminus(sm2.im, sm1.im, s0.im, sp1.im, corr) / h24);
+}
+static inline cplx_t ctplus(cplx_t sm1, cplx_t s0, cplx_t sp1, cplx_t sp2, int corr, double h24)
+{
+    return cx(tplus(sm1.re, s0.re, sp1.re, sp2.re, corr) / h24, tplus(sm1.im, s0.im, sp1.im, sp2.im, corr) / h24);
+}
+/* (s + (C(1)*d1 + C(2)*d2 + C(3)*d3 + C(4)*(yz) + C(5)*(xz) + C(6)*(xy))*dt)/(1+gamma*dt),
+ * cpmlfdtd.f08:974-987 */
+static inline cplx_t cstress(cplx_t s, double k1, double k2, double k3, double k4, double k5, double k6,
+                             cplx_t dxx, cplx_t dyy, cplx_t dzz, cplx_t yz, cplx_t xz, cplx_t xy, double dt,
+                             double gam)
+{
+    cplx_t t = crmul(k1, dxx);
+    t = cadd(t, crmul(k2, dyy));
+    t = cadd(t, crmul(k3, dzz));
+    t = cadd(t, crmul(k4, yz));
+    t = cadd(t, crmul(k5, xz));
+    t = cadd(t, crmul(k6, xy));
+    return cdivr(cadd(s, cmulr(t, dt)), 1 + gam * dt);
+}
+
+int32_t oracle_seismic25(oracle_run *r)
+{
+    set_threads(r);
+    grid_t g = {r->nx, 1, r->nz, (size_t)r->nx * (size_t)r->nz};
+    const int nx = g.nx, nz = g.nz;
+    const double dx = r->dx, dz = r->dz, dt = r->dt, ky = r->ky;
+    const double hx = 24.0 * dx, hz = 24.0 * dz;
+    const int dc = r->density_code, corr = r->stencil == SDT_STENCIL_CORRECTED;
+    const size_t n = g.n;
+
+    double *C[21];
+    for (int q = 0; q < 21; ++q) C[q] = input_copy(r, SDT_C11 + q, n);
+    double *c11 = C[0], *c12 = C[1], *c13 = C[2], *c14 = C[3], *c15 = C[4], *c16 = C[5], *c22 = C[6],
+           *c23 = C[7], *c24 = C[8], *c25 = C[9], *c26 = C[10], *c33 = C[11], *c34 = C[12], *c35 = C[13],
+           *c36 = C[14], *c44 = C[15], *c45 = C[16], *c46 = C[17], *c55 = C[18], *c56 = C[19], *c66 = C[20];
+    double *rho = input_copy(r, SDT_RHO, n);
+    double *gamma_x = input_copy(r, SDT_GAMMA_X, n), *gamma_y = input_copy(r, SDT_GAMMA_Y, n),
+           *gamma_z = input_copy(r, SDT_GAMMA_Z, n), *gamma_xy = input_copy(r, SDT_GAMMA_XY, n),
+           *gamma_xz = input_copy(r, SDT_GAMMA_XZ, n), *gamma_yz = input_copy(r, SDT_GAMMA_YZ, n);
+    double *kappa = input_copy(r, SDT_KAPPA, n), *acoef = input_copy(r, SDT_ACOEF, n),
+           *bcoef = input_copy(r, SDT_BCOEF, n);
+    double *kappa_half = input_copy(r, SDT_KAPPA_HALF, n), *acoef_half = input_copy(r, SDT_ACOEF_HALF, n),
+           *bcoef_half = input_copy(r, SDT_BCOEF_HALF, n);
+
+    /* density averaging, cpmlfdtd.f08:889-898 */
+    double *rhoxx = input_copy(r, SDT_RHO, n), *rhozx = input_copy(r, SDT_RHO, n),
+           *rhoxy = input_copy(r, SDT_RHO, n), *rhozy = input_copy(r, SDT_RHO, n),
+           *rhoxz = input_copy(r, SDT_RHO, n), *rhozz = input_copy(r, SDT_RHO, n);
+    const double *rhoyx = rho, *rhoyy = rho, *rhoyz = rho;
+    oracle_array_averaging2d(rhoxx, nx, nz, dc, 0);
+    oracle_array_averaging2d(rhozx, nx, nz, dc, 2);
+    oracle_array_averaging2d(rhoxy, nx, nz, dc, 1);
+    oracle_array_averaging2d(rhozy, nx, nz, dc, 2);
+    oracle_array_averaging2d(rhoxz, nx, nz, dc, 1);
+    oracle_array_averaging2d(rhozz, nx, nz, dc, 3);
+
+    cplx_t *F[9];
+    for (int f = 0; f < 9; ++f) F[f] = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t));
+    cplx_t *vx = F[SDT_F_VX], *vy = F[SDT_F_VY], *vz = F[SDT_F_VZ], *sxx = F[SDT_F_SXX],
+           *syy = F[SDT_F_SYY], *szz = F[SDT_F_SZZ], *sxy = F[SDT_F_SXY], *sxz = F[SDT_F_SXZ],
+           *syz = F[SDT_F_SYZ];
+    /* initialconditionV{x,y,z} (complex128, cpmlfdtd.f08:870-872) */
+    for (int f = 0; f < 9; ++f) {
+        const double *in = r->field_in[f] ? r->field_in[f] : (f < 3 ? r->arr[SDT_IC_X + f] : NULL);
+        if (in) memcpy(F[f], in, n * sizeof(cplx_t));
+    }
+    /* memory arrays: [nest 0..3][x: vx,vy,vz | z: vx,vy,vz] and 6 velocity-side */
+    cplx_t *M[4][6], *MV[6];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 6; ++b) M[a][b] = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t));
+    for (int b = 0; b < 6; ++b) MV[b] = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t));
+
+    const int is = r->isrc, ks = r->ksrc;
+    r->unstable_step = 0;
+    cplx_t *V[3] = {vx, vy, vz};
+
+    for (int it = 1; it <= r->nsteps; ++it) {
+        /* L1 sigma_xx, sigma_yy, sigma_zz: x forward half, z backward full (cpmlfdtd.f08:941-984) */
+#pragma omp parallel for schedule(static)
+        for (int k = 3; k <= nz - 1; ++k)
+            for (int i = 2; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t ddx[3], ddy[3], ddz[3];
+                for (int m = 0; m < 3; ++m) {
+                    const cplx_t *v = V[m];
+                    ddx[m] = cd4(v[I2(i - 1, k)], v[c], v[I2(i + 1, k)], v[I2(i + 2, k)], hx);
+                    ddy[m] = ciky(ky, v[c]);
+                    ddz[m] = cd4(v[I2(i, k - 2)], v[I2(i, k - 1)], v[c], v[I2(i, k + 1)], hz);
+                }
+                for (int m = 0; m < 3; ++m) ddx[m] = ccpml(ddx[m], &M[0][m][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                for (int m = 0; m < 3; ++m) ddz[m] = ccpml(ddz[m], &M[0][3 + m][c], acoef[c], bcoef[c], kappa[c]);
+                const cplx_t yz = cadd(ddz[1], ddy[2]), xz = cadd(ddz[0], ddx[2]), xy = cadd(ddx[1], ddy[0]);
+                sxx[c] = cstress(sxx[c], c11[c], c12[c], c13[c], c14[c], c15[c], c16[c], ddx[0], ddy[1], ddz[2], yz, xz, xy, dt, gamma_x[c]);
+                syy[c] = cstress(syy[c], c12[c], c22[c], c23[c], c24[c], c25[c], c26[c], ddx[0], ddy[1], ddz[2], yz, xz, xy, dt, gamma_y[c]);
+                szz[c] = cstress(szz[c], c13[c], c23[c], c33[c], c34[c], c35[c], c36[c], ddx[0], ddy[1], ddz[2], yz, xz, xy, dt, gamma_z[c]);
+            }
+        /* L2 sigma_xy: x backward full, z backward half (cpmlfdtd.f08:998-1027) */
+#pragma omp parallel for schedule(static)
+        for (int k = 3; k <= nz - 1; ++k)
+            for (int i = 3; i <= nx - 1; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t ddx[3], ddy[3], ddz[3];
+                for (int m = 0; m < 3; ++m) {
+                    const cplx_t *v = V[m];
+                    ddx[m] = cd4(v[I2(i - 2, k)], v[I2(i - 1, k)], v[c], v[I2(i + 1, k)], hx);
+                    ddy[m] = ciky(ky, v[c]);
+                    ddz[m] = cd4(v[I2(i, k - 2)], v[I2(i, k - 1)], v[c], v[I2(i, k + 1)], hz);
+                }
+                for (int m = 0; m < 3; ++m) ddx[m] = ccpml(ddx[m], &M[1][m][c], acoef[c], bcoef[c], kappa[c]);
+                for (int m = 0; m < 3; ++m) ddz[m] = ccpml(ddz[m], &M[1][3 + m][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                const cplx_t yz = cadd(ddz[1], ddy[2]), xz = cadd(ddz[0], ddx[2]), xy = cadd(ddx[1], ddy[0]);
+                sxy[c] = cstress(sxy[c], c16[c], c26[c], c36[c], c46[c], c56[c], c66[c], ddx[0], ddy[1], ddz[2], yz, xz, xy, dt, gamma_xy[c]);
+            }
+        /* L3 sigma_xz: x backward full, z forward half (cpmlfdtd.f08:1041-1070) */
+#pragma omp parallel for schedule(static)
+        for (int k = 2; k <= nz - 2; ++k)
+            for (int i = 3; i <= nx - 1; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t ddx[3], ddy[3], ddz[3];
+                for (int m = 0; m < 3; ++m) {
+                    const cplx_t *v = V[m];
+                    ddx[m] = cd4(v[I2(i - 2, k)], v[I2(i - 1, k)], v[c], v[I2(i + 1, k)], hx);
+                    ddy[m] = ciky(ky, v[c]);
+                    ddz[m] = cd4(v[I2(i, k - 1)], v[c], v[I2(i, k + 1)], v[I2(i, k + 2)], hz);
+                }
+                for (int m = 0; m < 3; ++m) ddx[m] = ccpml(ddx[m], &M[2][m][c], acoef[c], bcoef[c], kappa[c]);
+                for (int m = 0; m < 3; ++m) ddz[m] = ccpml(ddz[m], &M[2][3 + m][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                const cplx_t yz = cadd(ddz[1], ddy[2]), xz = cadd(ddz[0], ddx[2]), xy = cadd(ddx[1], ddy[0]);
+                sxz[c] = cstress(sxz[c], c15[c], c25[c], c35[c], c45[c], c55[c], c56[c], ddx[0], ddy[1], ddz[2], yz, xz, xy, dt, gamma_xz[c]);
+            }
+        /* L4 sigma_yz: x forward half, z forward half (cpmlfdtd.f08:1084-1113) */
+#pragma omp parallel for schedule(static)
+        for (int k = 2; k <= nz - 2; ++k)
+            for (int i = 2; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t ddx[3], ddy[3], ddz[3];
+                for (int m = 0; m < 3; ++m) {
+                    const cplx_t *v = V[m];
+                    ddx[m] = cd4(v[I2(i - 1, k)], v[c], v[I2(i + 1, k)], v[I2(i + 2, k)], hx);
+                    ddy[m] = ciky(ky, v[c]);
+                    ddz[m] = cd4(v[I2(i, k - 1)], v[c], v[I2(i, k + 1)], v[I2(i, k + 2)], hz);
+                }
+                for (int m = 0; m < 3; ++m) ddx[m] = ccpml(ddx[m], &M[3][m][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                for (int m = 0; m < 3; ++m) ddz[m] = ccpml(ddz[m], &M[3][3 + m][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                const cplx_t yz = cadd(ddz[1], ddy[2]), xz = cadd(ddz[0], ddx[2]), xy = cadd(ddx[1], ddy[0]);
+                syz[c] = cstress(syz[c], c14[c], c24[c], c34[c], c44[c], c45[c], c46[c], ddx[0], ddy[1], ddz[2], yz, xz, xy, dt, gamma_yz[c]);
+            }
+        /* vx: T- x sigma_xx (half), i ky sigma_xy, T- z sigma_xz (half) (cpmlfdtd.f08:1130-1143) */
+#pragma omp parallel for schedule(static)
+        for (int k = 3; k <= nz - 1; ++k)
+            for (int i = 3; i <= nx - 1; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t d1 = ctminus(sxx[I2(i - 2, k)], sxx[I2(i - 1, k)], sxx[c], sxx[I2(i + 1, k)], corr, hx);
+                const cplx_t d2 = ciky(ky, sxy[c]);
+                cplx_t d3 = ctminus(sxz[I2(i, k - 2)], sxz[I2(i, k - 1)], sxz[c], sxz[I2(i, k + 1)], corr, hz);
+                d1 = ccpml(d1, &MV[0][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                d3 = ccpml(d3, &MV[1][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                vx[c] = cadd(vx[c], cmulr(cadd(cadd(cdivr(d1, rhoxx[c]), cdivr(d2, rhoyx[c])), cdivr(d3, rhozx[c])), dt));
+            }
+        /* vy: T+ x sigma_xy (half), i ky sigma_yy, T- z sigma_yz (full) (cpmlfdtd.f08:1159-1172) */
+#pragma omp parallel for schedule(static)
+        for (int k = 3; k <= nz - 1; ++k)
+            for (int i = 2; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t d1 = ctplus(sxy[I2(i - 1, k)], sxy[c], sxy[I2(i + 1, k)], sxy[I2(i + 2, k)], corr, hx);
+                const cplx_t d2 = ciky(ky, syy[c]);
+                cplx_t d3 = ctminus(syz[I2(i, k - 2)], syz[I2(i, k - 1)], syz[c], syz[I2(i, k + 1)], corr, hz);
+                d1 = ccpml(d1, &MV[2][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                d3 = ccpml(d3, &MV[3][c], acoef[c], bcoef[c], kappa[c]);
+                vy[c] = cadd(vy[c], cmulr(cadd(cadd(cdivr(d1, rhoxy[c]), cdivr(d2, rhoyy[c])), cdivr(d3, rhozy[c])), dt));
+            }
+        /* vz: T+ x sigma_xz (half), i ky sigma_yz, T+ z sigma_zz (half) (cpmlfdtd.f08:1187-1200) */
+#pragma omp parallel for schedule(static)
+        for (int k = 2; k <= nz - 2; ++k)
+            for (int i = 2; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t d1 = ctplus(sxz[I2(i - 1, k)], sxz[c], sxz[I2(i + 1, k)], sxz[I2(i + 2, k)], corr, hx);
+                const cplx_t d2 = ciky(ky, syz[c]);
+                cplx_t d3 = ctplus(szz[I2(i, k - 1)], szz[c], szz[I2(i, k + 1)], szz[I2(i, k + 2)], corr, hz);
+                d1 = ccpml(d1, &MV[4][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                d3 = ccpml(d3, &MV[5][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                vz[c] = cadd(vz[c], cmulr(cadd(cadd(cdivr(d1, rhoxz[c]), cdivr(d2, rhoyz[c])), cdivr(d3, rhozz[c])), dt));
+            }
+        /* point source, cpmlfdtd.f08:1268-1276 (real series add to the real part) */
+        sxx[I2(is, ks)].re += src_at(r, SDT_SRC_XX, it) / rho[I2(is, ks)];
+        sxy[I2(is + 1, ks)].re += src_at(r, SDT_SRC_XY, it) / rho[I2(is + 1, ks)];
+        sxz[I2(is + 1, ks + 1)].re += src_at(r, SDT_SRC_XZ, it) / rho[I2(is + 1, ks)];
+        syy[I2(is, ks)].re += src_at(r, SDT_SRC_YY, it) / rho[I2(is, ks)];
+        syz[I2(is, ks + 1)].re += src_at(r, SDT_SRC_YZ, it) / rho[I2(is, ks + 1)];
+        szz[I2(is, ks)].re += src_at(r, SDT_SRC_ZZ, it) / rho[I2(is, ks)];
+        vx[I2(is + 1, ks)].re += src_at(r, SDT_SRC_X, it) * dt / rho[I2(is + 1, ks)];
+        vy[I2(is, ks)].re += src_at(r, SDT_SRC_Y, it) * dt / rho[I2(is, ks)];
+        vz[I2(is, ks + 1)].re += src_at(r, SDT_SRC_Z, it) * dt / rho[I2(is, ks + 1)];
+        /* Dirichlet edges, cpmlfdtd.f08:1280-1285 */
+        for (int m = 0; m < 3; ++m) {
+            for (int k = 1; k <= nz; ++k) { V[m][I2(1, k)] = cx(0, 0); V[m][I2(nx, k)] = cx(0, 0); }
+            for (int i = 1; i <= nx; ++i) { V[m][I2(i, 1)] = cx(0, 0); V[m][I2(i, nz)] = cx(0, 0); }
+        }
+        /* velocity norm, cpmlfdtd.f08:1291-1292 */
+        double velocnorm = 0.0;
+#pragma omp parallel for reduction(max : velocnorm) schedule(static)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t c = I2(i, k);
+                const double s = (vx[c].re * vx[c].re + vx[c].im * vx[c].im) + (vy[c].re * vy[c].re + vy[c].im * vy[c].im) +
+                                 (vz[c].re * vz[c].re + vz[c].im * vz[c].im);
+                const double v = sqrt(s);
+                if (v > velocnorm) velocnorm = v;
+            }
+        if (r->velnorm) r->velnorm[it - 1] = velocnorm;
+        if (r->rec_out)
+            for (int q = 0; q < r->nrec; ++q) {
+                const size_t c = I2(r->rec[3 * q], r->rec[3 * q + 2]);
+                double *o = r->rec_out + ((size_t)(it - 1) * r->nrec + q) * 6;
+                for (int m = 0; m < 3; ++m) { o[2 * m] = V[m][c].re; o[2 * m + 1] = V[m][c].im; }
+            }
+        if (velocnorm > O_STABILITY) { r->unstable_step = it; break; }
+    }
+
+    for (int f = 0; f < 9; ++f)
+        if (r->field_out[f]) memcpy(r->field_out[f], F[f], n * sizeof(cplx_t));
+
+    for (int q = 0; q < 21; ++q) free(C[q]);
+    double *all[] = {rho, gamma_x, gamma_y, gamma_z, gamma_xy, gamma_xz, gamma_yz, kappa, acoef, bcoef,
+                     kappa_half, acoef_half, bcoef_half, rhoxx, rhozx, rhoxy, rhozy, rhoxz, rhozz};
+    for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
+    for (int f = 0; f < 9; ++f) free(F[f]);
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 6; ++b) free(M[a][b]);
+    for (int b = 0; b < 6; ++b) free(MV[b]);
+    return r->unstable_step ? SDT_ERR_UNSTABLE : SDT_OK;
+}

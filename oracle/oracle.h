@@ -37,15 +37,24 @@ typedef struct oracle_run {
     double *stressnorm;          /* [nsteps] or NULL (3D elastic only)            */
     int32_t unstable_step;       /* out: 0, or the 1-based step that blew up      */
     int32_t nthreads;            /* OpenMP threads, 0 = runtime default           */
+    double ky;                   /* 2.5D elastic: y wavenumber (env KY)           */
 } oracle_run;
 
+/* 2.5D elastic (seismic25): fields, field_in/field_out and the ic_x..ic_z
+ * arrays are complex, stored as interleaved (re, im) pairs of 2*nx*nz
+ * doubles; rec_out is [nsteps][nrec][6] = (re, im) of vx, vy, vz. */
+
 int32_t oracle_seismic2(oracle_run *r);
+int32_t oracle_seismic25(oracle_run *r);
 int32_t oracle_seismic3(oracle_run *r);
 int32_t oracle_electromag2(oracle_run *r);
 int32_t oracle_electromag3(oracle_run *r);
 
 /* scalar_mean(a, b, code), averaging.f08:5-25 */
 double oracle_scalar_mean(double a, double b, int32_t code);
+/* array_averaging2d(array, code, direction) in place, averaging.f08:27-80;
+ * direction 0 '+i', 1 '-i', 2 '+j', 3 '-j' */
+void oracle_array_averaging2d(double *a, int32_t ni, int32_t nj, int32_t code, int32_t direction);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,7 @@ class OracleRun(C.Structure):
         ("nrec", C.c_int32), ("rec", C.POINTER(C.c_int32)), ("rec_out", _D),
         ("field_in", _D * NUM_FIELDS), ("field_out", _D * NUM_FIELDS),
         ("velnorm", _D), ("stressnorm", _D), ("unstable_step", C.c_int32), ("nthreads", C.c_int32),
+        ("ky", C.c_double),
     ]
 
 
@@ -58,12 +59,15 @@ def _load():
     if not os.path.exists(LIB_PATH):
         build()
     lib = C.CDLL(LIB_PATH)
-    for name in ("oracle_seismic2", "oracle_seismic3", "oracle_electromag2", "oracle_electromag3"):
+    for name in ("oracle_seismic2", "oracle_seismic25", "oracle_seismic3", "oracle_electromag2",
+                 "oracle_electromag3"):
         fn = getattr(lib, name)
         fn.argtypes = [C.POINTER(OracleRun)]
         fn.restype = C.c_int32
     lib.oracle_scalar_mean.argtypes = [C.c_double, C.c_double, C.c_int32]
     lib.oracle_scalar_mean.restype = C.c_double
+    lib.oracle_array_averaging2d.argtypes = [_D, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    lib.oracle_array_averaging2d.restype = None
     return lib
 
 
@@ -81,7 +85,17 @@ def scalar_mean(a, b, code):
     return lib().oracle_scalar_mean(float(a), float(b), int(code))
 
 
-_ENTRY = {"elastic2": "oracle_seismic2", "elastic3": "oracle_seismic3", "em2": "oracle_electromag2",
+def array_averaging2d(a, code, direction):
+    """In-place sweep of averaging.f08:27-80 on a copy of the (ni, nj) array;
+    direction one of '+i', '-i', '+j', '-j'."""
+    out = np.asfortranarray(np.array(a, dtype=np.float64))
+    ni, nj = out.shape
+    d = {"+i": 0, "-i": 1, "+j": 2, "-j": 3}[direction]
+    lib().oracle_array_averaging2d(out.ctypes.data_as(_D), ni, nj, int(code), d)
+    return out
+
+
+_ENTRY = {"elastic2": "oracle_seismic2", "elastic25": "oracle_seismic25", "elastic3": "oracle_seismic3", "em2": "oracle_electromag2",
           "em3": "oracle_electromag3"}
 
 
@@ -89,7 +103,8 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
     """Run a configuration dict through the oracle.
 
     Returns (receivers[steps, nrec, 3], (velnorm, stressnorm), {field: array},
-    unstable_step)."""
+    unstable_step).  For "elastic25" the receivers, fields and ic_* arrays are
+    complex128 and cfg["ky"] is the y wavenumber."""
     phys = cfg["physics"]
     is3d = phys.endswith("3")
     shape = tuple(cfg["shape"])
@@ -104,6 +119,8 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
         r.dx, r.dz = cfg["spacing"]
         r.dy = 1.0
     r.dt = cfg["dt"]
+    r.ky = float(cfg.get("ky", 0.0))
+    cplx = phys == "elastic25"
     r.nsteps = nsteps
     cp = cfg["cpml"]
     src = cfg["src"]
@@ -121,8 +138,16 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
         keep.append(a)
         return a.ctypes.data_as(_D)
 
+    def cptr(a):   # complex grid array -> interleaved (re, im) doubles
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128).reshape(shape).ravel(order="F"))
+        keep.append(a)
+        return a.ctypes.data_as(_D)
+
     for name, a in cfg["arrays"].items():
-        r.arr[ARRAY_IDS[name]] = ptr(np.asarray(a).reshape(shape).ravel(order="F"))
+        if cplx and name.startswith("ic_"):
+            r.arr[ARRAY_IDS[name]] = cptr(a)
+        else:
+            r.arr[ARRAY_IDS[name]] = ptr(np.asarray(a).reshape(shape).ravel(order="F"))
     for comp, s in cfg["sources"].items():
         s = np.asarray(s, dtype=np.float64)
         if s.size < nsteps:
@@ -140,15 +165,15 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
     keep.append(padded)
     r.nrec = padded.shape[0]
     r.rec = padded.ctypes.data_as(C.POINTER(C.c_int32))
-    rec_out = np.zeros((nsteps, padded.shape[0], 3))
+    rec_out = np.zeros((nsteps, padded.shape[0], 6 if cplx else 3))
     r.rec_out = rec_out.ctypes.data_as(_D)
     fid = FIELD_IDS_ELASTIC if phys.startswith("elastic") else FIELD_IDS_EM
     if field_in:
         for name, a in field_in.items():
-            r.field_in[fid[name]] = ptr(np.asarray(a).reshape(shape).ravel(order="F"))
+            r.field_in[fid[name]] = cptr(a) if cplx else ptr(np.asarray(a).reshape(shape).ravel(order="F"))
     outs = {}
     for name in fields:
-        o = np.zeros(int(np.prod(shape)))
+        o = np.zeros(int(np.prod(shape)), dtype=np.complex128 if cplx else np.float64)
         outs[name] = o
         r.field_out[fid[name]] = o.ctypes.data_as(_D)
     vn = np.zeros(nsteps)
@@ -157,4 +182,6 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
     r.stressnorm = sn.ctypes.data_as(_D)
     getattr(lib(), _ENTRY[phys])(C.byref(r))
     fields_out = {k: v.reshape(shape, order="F") for k, v in outs.items()}
+    if cplx:
+        rec_out = rec_out.view(np.complex128)
     return rec_out, (vn, sn), fields_out, r.unstable_step

@@ -1,0 +1,462 @@
+// el25.hip -- 2.5D anisotropic elastic CPML-FDTD (seismic25,
+// cpmlfdtd.f08:600-1337) on gfx950.
+//
+// The grid is the (x, z) plane; the y dependence is exp(i ky y), so every
+// field is complex and every y derivative is i*ky*f = (-ky*im, ky*re)
+// (cpmlfdtd.f08:946-948).  Complex arrays are interleaved (re, im) pairs, one
+// 16-byte (FP64) or 8-byte (FP32) load per cell.  Material arrays stay real.
+//
+// Layout and tiling as the 2D elastic solver: padded rows, one thread per cell,
+// 64 x 4-cell tiles; tiles entirely inside the identity-CPML box run the body
+// without memory variables.  The reference keeps one memory array per nest
+// and derivative (30); recursions that share field, direction, stencil and
+// coefficient set hold identical values, so 21 arrays are kept, each updated
+// once per step on the union of the nests' ranges:
+//   stress side  X+H (L1, L4), X-F (L2, L3), Z-F (L1), Z-H (L2), Z+H (L3, L4),
+//                three velocity components each                    (15)
+//   velocity side dsxx/dx H, dsxz/dz H, dsxy/dx H, dsyz/dz F, dsxz/dx H,
+//                dszz/dz H                                          (6)
+#include <math.h>
+
+#include <vector>
+
+#include "sdt_device.h"
+#include "sdt_internal.h"
+
+namespace sdt {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) Cx {
+    T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cmk(T re, T im)
+{
+    Cx<T> z;
+    z.re = re;
+    z.im = im;
+    return z;
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> operator+(Cx<T> a, Cx<T> b) { return cmk(a.re + b.re, a.im + b.im); }
+template <typename T>
+__device__ __forceinline__ Cx<T> operator*(T s, Cx<T> a) { return cmk(s * a.re, s * a.im); }
+// i*ky*f
+template <typename T>
+__device__ __forceinline__ Cx<T> iky(T ky, Cx<T> f) { return cmk(-ky * f.im, ky * f.re); }
+// psi = b*psi + a*d ; d = d/kappa + psi (1/kappa precomputed)
+template <typename T>
+__device__ __forceinline__ Cx<T> cfix(Cx<T> d, Cx<T> *psi, int64_t c, T a, T b, T ik)
+{
+    Cx<T> m = psi[c];
+    m.re = b * m.re + a * d.re;
+    m.im = b * m.im + a * d.im;
+    psi[c] = m;
+    return cmk(d.re * ik + m.re, d.im * ik + m.im);
+}
+// 4th-order staggered difference 27(f1 - f0) - (f2 - fm1), scaled
+template <typename T>
+__device__ __forceinline__ Cx<T> cd4(Cx<T> fm1, Cx<T> f0, Cx<T> f1, Cx<T> f2, T r)
+{
+    return cmk((T(27) * (f1.re - f0.re) - (f2.re - fm1.re)) * r, (T(27) * (f1.im - f0.im) - (f2.im - fm1.im)) * r);
+}
+// velocity-from-stress stencils: the reference's 3-term forms
+// (cpmlfdtd.f08:1132-1134, 1161-1163, 1189-1191) or the 4-term ones
+template <typename T>
+__device__ __forceinline__ T tm1(T sm2, T sm1, T s0, T sp1, int corr)
+{
+    return corr ? T(27) * (s0 - sm1) - (sp1 - sm2) : T(27) * s0 - T(27) * sm1 + sm2;
+}
+template <typename T>
+__device__ __forceinline__ T tp1(T sm1, T s0, T sp1, T sp2, int corr)
+{
+    return corr ? T(27) * (sp1 - s0) - (sp2 - sm1) : -T(27) * s0 + T(27) * sp1 - sp2;
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> ctm(const Cx<T> *f, int64_t c, int64_t s, int corr, T r)
+{
+    const Cx<T> a = f[c - 2 * s], b = f[c - s], z = f[c], d = f[c + s];
+    return cmk(tm1(a.re, b.re, z.re, d.re, corr) * r, tm1(a.im, b.im, z.im, d.im, corr) * r);
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> ctp(const Cx<T> *f, int64_t c, int64_t s, int corr, T r)
+{
+    const Cx<T> a = f[c - s], b = f[c], z = f[c + s], d = f[c + 2 * s];
+    return cmk(tp1(a.re, b.re, z.re, d.re, corr) * r, tp1(a.im, b.im, z.im, d.im, corr) * r);
+}
+
+// memory-variable slots
+enum {
+    P_XPH = 0,    // + m: D+x, half set        (L1, L4)
+    P_XMF = 3,    // D-x, full set              (L2, L3)
+    P_ZMF = 6,    // D-z, full set              (L1)
+    P_ZMH = 9,    // D-z, half set              (L2)
+    P_ZPH = 12,   // D+z, half set              (L3, L4)
+    P_VX_X = 15, P_VX_Z, P_VY_X, P_VY_Z, P_VZ_X, P_VZ_Z,
+    P_TOTAL = 21
+};
+// material slots: 21 stiffness, 6 attenuation divisors, 5 densities
+enum { M_IG = 21, M_RHO = 27, M_RHO_PI, M_RHO_PJ, M_RHO_MI, M_RHO_MJ, M_TOTAL25 = 32 };
+
+template <typename T>
+struct El25Args {
+    Cx<T> *v[3];          // vx, vy, vz
+    Cx<T> *s[6];          // sxx, syy, szz, sxy, sxz, syz
+    const T *mat[M_TOTAL25];
+    const T *a, *b, *ik, *ah, *bh, *ikh;
+    Cx<T> *m[P_TOTAL];
+    const uint8_t *flags;
+    int box[6];
+    int nx, nz, tiles_x, pitch;
+    int64_t xoff;
+    T rdx24, rdz24, dt, ky;
+    int corrected;
+    const int *d_it;
+    unsigned long long *vnorm;
+    int64_t vfix[3];      // source cells excluded from the fused norm
+};
+
+// sigma = (sigma + (K1 d1 + K2 d2 + K3 d3 + K4 yz + K5 xz + K6 xy) dt) * ig,
+// cpmlfdtd.f08:974-987
+template <typename T>
+__device__ __forceinline__ Cx<T> cstress(Cx<T> s, T k1, T k2, T k3, T k4, T k5, T k6, Cx<T> d1, Cx<T> d2,
+                                         Cx<T> d3, Cx<T> yz, Cx<T> xz, Cx<T> xy, T dt, T ig)
+{
+    const T re = k1 * d1.re + k2 * d2.re + k3 * d3.re + k4 * yz.re + k5 * xz.re + k6 * xy.re;
+    const T im = k1 * d1.im + k2 * d2.im + k3 * d3.im + k4 * yz.im + k5 * xz.im + k6 * xy.im;
+    return cmk((s.re + re * dt) * ig, (s.im + im * dt) * ig);
+}
+
+// the four stress nests, cpmlfdtd.f08:941-1113
+template <typename T, bool PML>
+__device__ __forceinline__ void el25_stress_cell(const El25Args<T> &A, int i, int k)
+{
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (k - 1);
+    const int64_t sz = A.pitch;
+    const bool L1 = k >= 3 && k <= nz - 1 && i >= 2 && i <= nx - 2;
+    const bool L2 = k >= 3 && k <= nz - 1 && i >= 3 && i <= nx - 1;
+    const bool L3 = k >= 2 && k <= nz - 2 && i >= 3 && i <= nx - 1;
+    const bool L4 = k >= 2 && k <= nz - 2 && i >= 2 && i <= nx - 2;
+    if (!(L1 || L2 || L3 || L4)) return;
+    const T rx = A.rdx24, rz = A.rdz24, ky = A.ky, dt = A.dt;
+    T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
+    if (PML) {
+        a = A.a[c]; b = A.b[c]; ik = A.ik[c];
+        ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+    }
+    Cx<T> dy[3], dxp[3], dxm[3], dzmF[3], dzmH[3], dzp[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const Cx<T> *v = A.v[q];
+        const Cx<T> v0 = v[c];
+        dy[q] = iky(ky, v0);
+        if (L1 || L4) {
+            dxp[q] = cd4(v[c - 1], v0, v[c + 1], v[c + 2], rx);
+            if (PML) dxp[q] = cfix(dxp[q], A.m[P_XPH + q], c, ah, bh, ikh);
+        }
+        if (L2 || L3) {
+            dxm[q] = cd4(v[c - 2], v[c - 1], v0, v[c + 1], rx);
+            if (PML) dxm[q] = cfix(dxm[q], A.m[P_XMF + q], c, a, b, ik);
+        }
+        if (L1 || L2) {
+            const Cx<T> d = cd4(v[c - 2 * sz], v[c - sz], v0, v[c + sz], rz);
+            dzmF[q] = d;
+            dzmH[q] = d;
+            if (PML && L1) dzmF[q] = cfix(d, A.m[P_ZMF + q], c, a, b, ik);
+            if (PML && L2) dzmH[q] = cfix(d, A.m[P_ZMH + q], c, ah, bh, ikh);
+        }
+        if (L3 || L4) {
+            dzp[q] = cd4(v[c - sz], v0, v[c + sz], v[c + 2 * sz], rz);
+            if (PML) dzp[q] = cfix(dzp[q], A.m[P_ZPH + q], c, ah, bh, ikh);
+        }
+    }
+    const T *const *M = A.mat;
+    // Voigt slots of the 21 stiffness arrays: c11 0, c12 1, c13 2, c14 3, c15 4,
+    // c16 5, c22 6, c23 7, c24 8, c25 9, c26 10, c33 11, c34 12, c35 13, c36 14,
+    // c44 15, c45 16, c46 17, c55 18, c56 19, c66 20
+    if (L1) {
+        const Cx<T> yz = dzmF[1] + dy[2], xz = dzmF[0] + dxp[2], xy = dxp[1] + dy[0];
+        A.s[0][c] = cstress(A.s[0][c], M[0][c], M[1][c], M[2][c], M[3][c], M[4][c], M[5][c], dxp[0], dy[1], dzmF[2],
+                            yz, xz, xy, dt, M[M_IG + 0][c]);
+        A.s[1][c] = cstress(A.s[1][c], M[1][c], M[6][c], M[7][c], M[8][c], M[9][c], M[10][c], dxp[0], dy[1], dzmF[2],
+                            yz, xz, xy, dt, M[M_IG + 1][c]);
+        A.s[2][c] = cstress(A.s[2][c], M[2][c], M[7][c], M[11][c], M[12][c], M[13][c], M[14][c], dxp[0], dy[1],
+                            dzmF[2], yz, xz, xy, dt, M[M_IG + 2][c]);
+    }
+    if (L2) {
+        const Cx<T> yz = dzmH[1] + dy[2], xz = dzmH[0] + dxm[2], xy = dxm[1] + dy[0];
+        A.s[3][c] = cstress(A.s[3][c], M[5][c], M[10][c], M[14][c], M[17][c], M[19][c], M[20][c], dxm[0], dy[1],
+                            dzmH[2], yz, xz, xy, dt, M[M_IG + 3][c]);
+    }
+    if (L3) {
+        const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxm[2], xy = dxm[1] + dy[0];
+        A.s[4][c] = cstress(A.s[4][c], M[4][c], M[9][c], M[13][c], M[16][c], M[18][c], M[19][c], dxm[0], dy[1],
+                            dzp[2], yz, xz, xy, dt, M[M_IG + 4][c]);
+    }
+    if (L4) {
+        const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxp[2], xy = dxp[1] + dy[0];
+        A.s[5][c] = cstress(A.s[5][c], M[3][c], M[8][c], M[12][c], M[15][c], M[16][c], M[17][c], dxp[0], dy[1],
+                            dzp[2], yz, xz, xy, dt, M[M_IG + 5][c]);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) el25_stress_kernel(El25Args<T> A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int k = blockIdx.y * 4 + threadIdx.y + 1;
+    if (i > A.nx || k > A.nz) return;
+    if (!in_box(A.box, i, 1, k) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y])
+        el25_stress_cell<T, true>(A, i, k);
+    else
+        el25_stress_cell<T, false>(A, i, k);
+}
+
+// v += (d1 / rho_x + d2 / rho_y + d3 / rho_z) * dt
+template <typename T>
+__device__ __forceinline__ Cx<T> vupd(Cx<T> v, Cx<T> d1, Cx<T> d2, Cx<T> d3, T r1, T r2, T r3, T dt)
+{
+    return cmk(v.re + (d1.re / r1 + d2.re / r2 + d3.re / r3) * dt, v.im + (d1.im / r1 + d2.im / r2 + d3.im / r3) * dt);
+}
+
+// vx, vy, vz (cpmlfdtd.f08:1130-1200), Dirichlet edges (1280-1285) and the
+// per-step velocity norm (1291) without the source cells
+template <typename T, bool PML>
+__device__ __forceinline__ double el25_velocity_cell(const El25Args<T> &A, int i, int k)
+{
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (k - 1);
+    const int64_t sz = A.pitch;
+    Cx<T> vx = A.v[0][c], vy = A.v[1][c], vz = A.v[2][c];
+    if (i == 1 || i == nx || k == 1 || k == nz) {
+        vx = vy = vz = cmk(T(0), T(0));
+        A.v[0][c] = vx;
+        A.v[1][c] = vy;
+        A.v[2][c] = vz;
+    } else {
+        const T rx = A.rdx24, rz = A.rdz24, ky = A.ky, dt = A.dt;
+        const int corr = A.corrected;
+        const T *const *M = A.mat;
+        const T rho = M[M_RHO][c];
+        T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
+        if (PML) {
+            a = A.a[c]; b = A.b[c]; ik = A.ik[c];
+            ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+        }
+        if (k >= 3 && k <= nz - 1 && i >= 3 && i <= nx - 1) {
+            Cx<T> d1 = ctm(A.s[0], c, (int64_t)1, corr, rx);
+            const Cx<T> d2 = iky(ky, A.s[3][c]);
+            Cx<T> d3 = ctm(A.s[4], c, sz, corr, rz);
+            if (PML) {
+                d1 = cfix(d1, A.m[P_VX_X], c, ah, bh, ikh);
+                d3 = cfix(d3, A.m[P_VX_Z], c, ah, bh, ikh);
+            }
+            vx = vupd(vx, d1, d2, d3, M[M_RHO_PI][c], rho, M[M_RHO_PJ][c], dt);
+            A.v[0][c] = vx;
+        }
+        if (k >= 3 && k <= nz - 1 && i >= 2 && i <= nx - 2) {
+            Cx<T> d1 = ctp(A.s[3], c, (int64_t)1, corr, rx);
+            const Cx<T> d2 = iky(ky, A.s[1][c]);
+            Cx<T> d3 = ctm(A.s[5], c, sz, corr, rz);
+            if (PML) {
+                d1 = cfix(d1, A.m[P_VY_X], c, ah, bh, ikh);
+                d3 = cfix(d3, A.m[P_VY_Z], c, a, b, ik);
+            }
+            vy = vupd(vy, d1, d2, d3, M[M_RHO_MI][c], rho, M[M_RHO_PJ][c], dt);
+            A.v[1][c] = vy;
+        }
+        if (k >= 2 && k <= nz - 2 && i >= 2 && i <= nx - 2) {
+            Cx<T> d1 = ctp(A.s[4], c, (int64_t)1, corr, rx);
+            const Cx<T> d2 = iky(ky, A.s[5][c]);
+            Cx<T> d3 = ctp(A.s[2], c, sz, corr, rz);
+            if (PML) {
+                d1 = cfix(d1, A.m[P_VZ_X], c, ah, bh, ikh);
+                d3 = cfix(d3, A.m[P_VZ_Z], c, ah, bh, ikh);
+            }
+            vz = vupd(vz, d1, d2, d3, M[M_RHO_MI][c], rho, M[M_RHO_MJ][c], dt);
+            A.v[2][c] = vz;
+        }
+    }
+    if (c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2]) return 0.0;
+    const double s = ((double)vx.re * vx.re + (double)vx.im * vx.im) + ((double)vy.re * vy.re + (double)vy.im * vy.im) +
+                     ((double)vz.re * vz.re + (double)vz.im * vz.im);
+    return sqrt(s);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) el25_velocity_kernel(El25Args<T> A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x + 1;
+    const int k = blockIdx.y * 4 + threadIdx.y + 1;
+    double nrm = 0.0;
+    if (i <= A.nx && k <= A.nz) {
+        if (!in_box(A.box, i, 1, k) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y])
+            nrm = el25_velocity_cell<T, true>(A, i, k);
+        else
+            nrm = el25_velocity_cell<T, false>(A, i, k);
+    }
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T>
+static El25Args<T> el25_args(Device &d)
+{
+    El25Args<T> A;
+    for (int q = 0; q < 3; ++q) A.v[q] = (Cx<T> *)d.field[SDT_F_VX + q];
+    for (int q = 0; q < 6; ++q) A.s[q] = (Cx<T> *)d.field[SDT_F_SXX + q];
+    for (int q = 0; q < M_TOTAL25; ++q) A.mat[q] = (const T *)d.mat[q];
+    A.a = (const T *)d.cpml[0];
+    A.b = (const T *)d.cpml[1];
+    A.ik = (const T *)d.cpml[2];
+    A.ah = (const T *)d.cpml[3];
+    A.bh = (const T *)d.cpml[4];
+    A.ikh = (const T *)d.cpml[5];
+    for (int q = 0; q < P_TOTAL; ++q) A.m[q] = (Cx<T> *)d.mem[q];
+    A.flags = d.tile_flags;
+    for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
+    A.nx = d.nx;
+    A.nz = d.nz;
+    A.pitch = d.pitch;
+    A.xoff = d.xoff;
+    A.tiles_x = d.tiles_x;
+    A.rdx24 = (T)(1.0 / (24.0 * d.dx));
+    A.rdz24 = (T)(1.0 / (24.0 * d.dz));
+    A.dt = (T)d.dt;
+    A.ky = (T)d.ky;
+    A.corrected = d.stencil == SDT_STENCIL_CORRECTED;
+    A.d_it = d.d_it;
+    A.vnorm = d.norms;
+    for (int q = 0; q < 3; ++q) A.vfix[q] = d.tail.vfix[q];
+    return A;
+}
+
+template <typename T>
+static void el25_step_t(Device &d, bool timed)
+{
+    const El25Args<T> A = el25_args<T>(d);
+    const dim3 grid(d.tiles_x, d.tiles_z), block(64, 4);
+    if (timed) (void)hipEventRecord(d.ev[0], d.stream);
+    hipLaunchKernelGGL(el25_stress_kernel<T>, grid, block, 0, d.stream, A);
+    if (timed) (void)hipEventRecord(d.ev[1], d.stream);
+    hipLaunchKernelGGL(el25_velocity_kernel<T>, grid, block, 0, d.stream, A);
+    if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_tail(d);
+}
+
+void el25_step(Device &d, bool timed)
+{
+    if (d.precision == SDT_FP64) el25_step_t<double>(d, timed);
+    else el25_step_t<float>(d, timed);
+}
+
+// In-place density sweep of array_averaging2d (averaging.f08:27-80), run once
+// at setup: the '+' directions average with the neighbour already overwritten
+// in the same sweep, the '-' directions with the still-original neighbour.
+static void smear(std::vector<double> &a, int nx, int nz, int code, int dir)
+{
+    auto A = [&](int i, int k) -> double & { return a[(size_t)(i - 1) + (size_t)nx * (size_t)(k - 1)]; };
+    auto mean = [&](double x, double y) {
+        switch (code) {
+        case 1: return 2.0 * x * y / (x + y);
+        case 2: return sqrt(x * y);
+        case 3: return 0.5 * (x + y);
+        default: return x;
+        }
+    };
+    if (dir == 0)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 2; i <= nx; ++i) A(i, k) = mean(A(i, k), A(i - 1, k));
+    else if (dir == 1)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx - 1; ++i) A(i, k) = mean(A(i, k), A(i + 1, k));
+    else if (dir == 2)
+        for (int k = 2; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) A(i, k) = mean(A(i, k), A(i, k - 1));
+    else
+        for (int k = 1; k <= nz - 1; ++k)
+            for (int i = 1; i <= nx; ++i) A(i, k) = mean(A(i, k), A(i, k + 1));
+}
+
+// Device material slots: c11..c66 (21), 1/(1 + gamma*dt) for x, y, z, xy,
+// xz, yz, then rho and its four averaged copies ('+i' rhoxx, '+j' rhozx =
+// rhozy, '-i' rhoxy = rhoxz, '-j' rhozz; cpmlfdtd.f08:889-898).
+int el25_setup(Device &d, const double *const *in)
+{
+    d.tile_x = 64;
+    d.tile_y = 1;
+    d.tile_z = 4;
+    d.ncomp = 2;
+    int rc = compute_tile_flags(d, in);
+    if (rc) return rc;
+    set_layout(d);
+    const int64_t n = d.ncell;
+    std::vector<double> zero((size_t)n, 0.0);
+    d.nmat = 0;
+    for (int id = SDT_C11; id <= SDT_C66; ++id) {
+        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
+        if (rc) return rc;
+    }
+    const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};
+    std::vector<double> tmp((size_t)n);
+    for (int id : gids) {
+        for (int64_t c = 0; c < n; ++c) tmp[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
+        rc = upload_grid(d, tmp.data(), &d.mat[d.nmat++]);
+        if (rc) return rc;
+    }
+    const double *rho = in[SDT_RHO] ? in[SDT_RHO] : zero.data();
+    rc = upload_grid(d, rho, &d.mat[d.nmat++]);
+    if (rc) return rc;
+    for (int dir : {0, 2, 1, 3}) {   // M_RHO_PI, M_RHO_PJ, M_RHO_MI, M_RHO_MJ
+        tmp.assign(rho, rho + n);
+        smear(tmp, d.nx, d.nz, d.density_code, dir);
+        rc = upload_grid(d, tmp.data(), &d.mat[d.nmat++]);
+        if (rc) return rc;
+    }
+    d.nmem = P_TOTAL;
+    for (int q = 0; q < d.nmem; ++q) {
+        rc = alloc_grid_zeroed(d, &d.mem[q], 2);
+        if (rc) return rc;
+    }
+    for (int f = SDT_F_VX; f <= SDT_F_SYZ; ++f) {
+        if (f <= SDT_F_VZ && in[SDT_IC_X + f]) rc = upload_grid(d, in[SDT_IC_X + f], &d.field[f], 2);
+        else rc = alloc_grid_zeroed(d, &d.field[f], 2);
+        if (rc) return rc;
+    }
+
+    // point sources, cpmlfdtd.f08:1268-1276 (isource = xind + cpml, ksource =
+    // zind + cpml; the series are real and add to the real part)
+    const int is = d.isrc, ks = d.ksrc, nx = d.nx, nz = d.nz;
+    auto cell = [&](int i, int k) { return cell_index(d, i, 1, k); };
+    auto R = [&](int i, int k) { return rho[(int64_t)(i - 1) + (int64_t)nx * (k - 1)]; };
+    auto face = [&](int i, int k) { return (int)(i == 1 || i == nx || k == 1 || k == nz); };
+    TailArgs &t = d.tail;
+    t.nterms = 0;
+    t.terms[t.nterms++] = {cell(is, ks), SDT_F_SXX, SDT_SRC_XX, 1.0, R(is, ks), 0};
+    t.terms[t.nterms++] = {cell(is + 1, ks), SDT_F_SXY, SDT_SRC_XY, 1.0, R(is + 1, ks), 0};
+    t.terms[t.nterms++] = {cell(is + 1, ks + 1), SDT_F_SXZ, SDT_SRC_XZ, 1.0, R(is + 1, ks), 0};
+    t.terms[t.nterms++] = {cell(is, ks), SDT_F_SYY, SDT_SRC_YY, 1.0, R(is, ks), 0};
+    t.terms[t.nterms++] = {cell(is, ks + 1), SDT_F_SYZ, SDT_SRC_YZ, 1.0, R(is, ks + 1), 0};
+    t.terms[t.nterms++] = {cell(is, ks), SDT_F_SZZ, SDT_SRC_ZZ, 1.0, R(is, ks), 0};
+    t.terms[t.nterms++] = {cell(is + 1, ks), SDT_F_VX, SDT_SRC_X, d.dt, R(is + 1, ks), face(is + 1, ks)};
+    t.terms[t.nterms++] = {cell(is, ks), SDT_F_VY, SDT_SRC_Y, d.dt, R(is, ks), face(is, ks)};
+    t.terms[t.nterms++] = {cell(is, ks + 1), SDT_F_VZ, SDT_SRC_Z, d.dt, R(is, ks + 1), face(is, ks + 1)};
+    t.nvfix = 3;
+    t.vfix[0] = cell(is + 1, ks);
+    t.vfix[1] = cell(is, ks);
+    t.vfix[2] = cell(is, ks + 1);
+    t.nvcomp = 3;
+    t.vcomp[0] = SDT_F_VX;
+    t.vcomp[1] = SDT_F_VY;
+    t.vcomp[2] = SDT_F_VZ;
+    t.stress_norm = 0;
+    t.nsfix = 0;
+    t.rec_field[0] = SDT_F_VX;
+    t.rec_field[1] = SDT_F_VY;
+    t.rec_field[2] = SDT_F_VZ;
+    t.cplx = 1;
+    return upload_cpml_full(d, in);
+}
+
+}  // namespace sdt

@@ -51,13 +51,19 @@ static int hip_fail(sdt_handle *h, hipError_t e, const char *where)
     } while (0)
 
 static bool is3d(int physics) { return physics == SDT_ELASTIC3 || physics == SDT_EM3; }
-static bool elastic(int physics) { return physics == SDT_ELASTIC2 || physics == SDT_ELASTIC3; }
+static bool elastic(int physics)
+{
+    return physics == SDT_ELASTIC2 || physics == SDT_ELASTIC25 || physics == SDT_ELASTIC3;
+}
+// values per field element: 2 for the complex fields of the 2.5D solver
+static int ncomp_of(int physics) { return physics == SDT_ELASTIC25 ? 2 : 1; }
 
 static int nfields_of(int physics, int f)
 {
     // fields present per solver
     switch (physics) {
     case SDT_ELASTIC2: return f == SDT_F_VX || f == SDT_F_VZ || f == SDT_F_SXX || f == SDT_F_SZZ || f == SDT_F_SXZ;
+    case SDT_ELASTIC25:
     case SDT_ELASTIC3: return f >= 0 && f < 9;
     case SDT_EM2: return f == SDT_F_EX || f == SDT_F_EZ || f == SDT_F_HY;
     case SDT_EM3: return f >= 0 && f < 6;
@@ -85,8 +91,10 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out)
     if (!p || !out) return fail(nullptr, SDT_ERR_ARG, "null argument");
     *out = nullptr;
     if (p->abi_version != SDT_ABI_VERSION) return fail(nullptr, SDT_ERR_ARG, "abi_version mismatch");
-    if (p->physics != SDT_ELASTIC2 && p->physics != SDT_ELASTIC3 && p->physics != SDT_EM2 && p->physics != SDT_EM3)
+    if (p->physics != SDT_ELASTIC2 && p->physics != SDT_ELASTIC25 && p->physics != SDT_ELASTIC3 &&
+        p->physics != SDT_EM2 && p->physics != SDT_EM3)
         return fail(nullptr, SDT_ERR_ARG, "unknown physics");
+    if (!isfinite(p->ky)) return fail(nullptr, SDT_ERR_ARG, "ky must be finite");
     if (p->precision != SDT_FP64 && p->precision != SDT_FP32) return fail(nullptr, SDT_ERR_ARG, "unknown precision");
     const bool d3 = is3d(p->physics);
     if (p->nx < 5 || p->nz < 5 || (d3 ? p->ny < 5 : p->ny != 1))
@@ -122,6 +130,8 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out)
     d.isrc = is; d.jsrc = js; d.ksrc = ks;
     d.density_code = p->density_code;
     d.stencil = p->stencil;
+    d.ky = p->physics == SDT_ELASTIC25 ? p->ky : 0.0;
+    d.ncomp = ncomp_of(p->physics);
     d.ncell = (int64_t)p->nx * p->ny * p->nz;
     h->series.assign((size_t)SDT_NUM_SRC * p->time_steps, 0.0);
     e = hipSetDevice(p->device);
@@ -167,7 +177,20 @@ int32_t sdt_set_array(sdt_handle *h, int32_t id, const double *host, int64_t cou
     if (!h || !host) return fail(h, SDT_ERR_ARG, "null argument");
     if (h->prepared) return fail(h, SDT_ERR_STATE, "inputs are fixed once the first step has run");
     if (id < 0 || id >= SDT_NUM_ARRAYS) return fail(h, SDT_ERR_ARG, "unknown array id");
-    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "array length must equal nx*ny*nz");
+    const int64_t n = h->d.ncell;
+    if (h->d.ncomp == 2 && id >= SDT_IC_X && id <= SDT_IC_Z) {
+        // complex initial condition (material_rw2c); a real array sets the real part
+        if (count == 2 * n) {
+            h->staged[id].assign(host, host + count);
+        } else if (count == n) {
+            h->staged[id].assign((size_t)(2 * n), 0.0);
+            for (int64_t c = 0; c < n; ++c) h->staged[id][(size_t)(2 * c)] = host[c];
+        } else {
+            return fail(h, SDT_ERR_ARG, "initial-condition length must equal 2*nx*nz (complex) or nx*nz");
+        }
+        return SDT_OK;
+    }
+    if (count != n) return fail(h, SDT_ERR_ARG, "array length must equal nx*ny*nz");
     h->staged[id].assign(host, host + count);
     return SDT_OK;
 }
@@ -215,6 +238,7 @@ static int prepare(sdt_handle *h)
     int rc = SDT_OK;
     switch (d.physics) {
     case SDT_ELASTIC2: rc = el2_setup(d, in); break;
+    case SDT_ELASTIC25: rc = el25_setup(d, in); break;
     case SDT_ELASTIC3: rc = el3_setup(d, in); break;
     case SDT_EM2: rc = em2_setup(d, in); break;
     case SDT_EM3: rc = em3_setup(d, in); break;
@@ -237,7 +261,7 @@ static int prepare(sdt_handle *h)
                                           h->rec_ijk[3 * (size_t)q + 2]);
         HIPCHK(h, hipMalloc((void **)&d.rec_cell, (size_t)d.nrec * sizeof(int64_t)));
         HIPCHK(h, hipMemcpy(d.rec_cell, cells.data(), (size_t)d.nrec * sizeof(int64_t), hipMemcpyHostToDevice));
-        const size_t nr = (size_t)d.time_steps * d.nrec * 3;
+        const size_t nr = (size_t)d.time_steps * d.nrec * 3 * d.ncomp;
         HIPCHK(h, hipMalloc((void **)&d.rec_out, nr * sizeof(double)));
         HIPCHK(h, hipMemset(d.rec_out, 0, nr * sizeof(double)));
     }
@@ -261,6 +285,7 @@ static void enqueue_step(sdt_handle *h, bool timed)
     Device &d = h->d;
     switch (d.physics) {
     case SDT_ELASTIC2: el2_step(d, timed); break;
+    case SDT_ELASTIC25: el25_step(d, timed); break;
     case SDT_ELASTIC3: el3_step(d, timed); break;
     case SDT_EM2: em2_step(d, timed); break;
     case SDT_EM3: em3_step(d, timed); break;
@@ -376,13 +401,14 @@ int32_t sdt_get_field(sdt_handle *h, int32_t f, double *out, int64_t count)
 {
     if (!h || !out) return fail(h, SDT_ERR_ARG, "null argument");
     if (f < 0 || f >= SDT_NUM_FIELDS || !nfields_of(h->d.physics, f)) return fail(h, SDT_ERR_ARG, "field not present in this solver");
-    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "count must equal nx*ny*nz");
+    if (count != h->d.ncell * h->d.ncomp)
+        return fail(h, SDT_ERR_ARG, h->d.ncomp == 2 ? "count must equal 2*nx*nz (complex)" : "count must equal nx*ny*nz");
     int rc = prepare(h);
     if (rc) return rc;
     Device &d = h->d;
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(d.stream));
-    if (download_grid(d, d.field[f], out) != SDT_OK) return fail(h, SDT_ERR_HIP, "field read-back failed");
+    if (download_grid(d, d.field[f], out, d.ncomp) != SDT_OK) return fail(h, SDT_ERR_HIP, "field read-back failed");
     // lazily applied Dirichlet faces of H (em2.hip / em3.hip)
     const bool hfield = (d.physics == SDT_EM2 && f == SDT_F_HY) || (d.physics == SDT_EM3 && f >= SDT_F_HX);
     if (hfield && h->steps_done > 0) {
@@ -401,21 +427,24 @@ int32_t sdt_set_field(sdt_handle *h, int32_t f, const double *host, int64_t coun
 {
     if (!h || !host) return fail(h, SDT_ERR_ARG, "null argument");
     if (f < 0 || f >= SDT_NUM_FIELDS || !nfields_of(h->d.physics, f)) return fail(h, SDT_ERR_ARG, "field not present in this solver");
-    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "count must equal nx*ny*nz");
+    if (count != h->d.ncell * h->d.ncomp)
+        return fail(h, SDT_ERR_ARG, h->d.ncomp == 2 ? "count must equal 2*nx*nz (complex)" : "count must equal nx*ny*nz");
     int rc = prepare(h);
     if (rc) return rc;
     Device &d = h->d;
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(d.stream));
-    if (store_grid(d, d.field[f], host) != SDT_OK) return fail(h, SDT_ERR_HIP, "field upload failed");
+    if (store_grid(d, d.field[f], host, d.ncomp) != SDT_OK) return fail(h, SDT_ERR_HIP, "field upload failed");
     return SDT_OK;
 }
 
 int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count)
 {
     if (!h || (!out && count)) return fail(h, SDT_ERR_ARG, "null argument");
-    const int64_t need = (int64_t)h->steps_done * (int64_t)(h->rec_ijk.size() / 3) * 3;
-    if (count != need) return fail(h, SDT_ERR_ARG, "count must equal steps_done * n_receivers * 3");
+    const int64_t need = (int64_t)h->steps_done * (int64_t)(h->rec_ijk.size() / 3) * 3 * h->d.ncomp;
+    if (count != need)
+        return fail(h, SDT_ERR_ARG, h->d.ncomp == 2 ? "count must equal steps_done * n_receivers * 6 (complex)"
+                                                    : "count must equal steps_done * n_receivers * 3");
     if (need == 0) return SDT_OK;
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(h->d.stream));

@@ -35,13 +35,20 @@ __global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
             if (s.cell < 0 || s.on_face) continue;
             const double v = a.series[(int64_t)s.component * a.time_steps + (it - 1)];
             T *f = reinterpret_cast<T *>(fp.f[s.field]);
-            f[s.cell] = (T)((double)f[s.cell] + (T)((v * s.mul) / s.div));
+            const int64_t e = s.cell << a.cplx;   // complex: the real part
+            f[e] = (T)((double)f[e] + (T)((v * s.mul) / s.div));
         }
         for (int q = 0; q < a.nvfix; ++q) {
             double ss = 0.0;
             for (int c = 0; c < a.nvcomp; ++c) {
-                const double v = (double)reinterpret_cast<T *>(fp.f[a.vcomp[c]])[a.vfix[q]];
-                ss += v * v;
+                const T *f = reinterpret_cast<const T *>(fp.f[a.vcomp[c]]);
+                if (a.cplx) {
+                    const double re = (double)f[2 * a.vfix[q]], im = (double)f[2 * a.vfix[q] + 1];
+                    ss += re * re + im * im;
+                } else {
+                    const double v = (double)f[a.vfix[q]];
+                    ss += v * v;
+                }
             }
             atomic_max_nonneg(&a.vnorm[it - 1], sqrt(ss));
         }
@@ -63,9 +70,13 @@ __global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
     __syncthreads();
     for (int q = threadIdx.x; q < a.nrec; q += blockDim.x) {
         const int64_t c = a.rec_cell[q];
-        double *o = a.rec_out + ((int64_t)(it - 1) * a.nrec + q) * 3;
+        const int nv = 1 << a.cplx;
+        double *o = a.rec_out + ((int64_t)(it - 1) * a.nrec + q) * 3 * nv;
         for (int m = 0; m < 3; ++m)
-            o[m] = a.rec_field[m] >= 0 ? (double)reinterpret_cast<T *>(fp.f[a.rec_field[m]])[c] : 0.0;
+            for (int p = 0; p < nv; ++p)
+                o[m * nv + p] = a.rec_field[m] >= 0
+                                    ? (double)reinterpret_cast<T *>(fp.f[a.rec_field[m]])[(c << a.cplx) + p]
+                                    : 0.0;
     }
     __syncthreads();
     if (threadIdx.x == 0) *a.d_it = it + 1;
@@ -149,34 +160,36 @@ void set_layout(Device &d)
     d.ncell_dev = (int64_t)d.pitch * d.ny * d.nz;
 }
 
-// rows of the grid = ny * nz, each nx elements on the host, pitch on the device
-int upload_grid(Device &d, const double *host, void **dev)
+// rows of the grid = ny * nz, each nx elements on the host, pitch on the
+// device; an element is ncomp values (2 = complex, interleaved re/im)
+int upload_grid(Device &d, const double *host, void **dev, int ncomp)
 {
-    const size_t bytes = (size_t)d.ncell_dev * d.esize;
+    const size_t bytes = (size_t)d.ncell_dev * d.esize * ncomp;
     if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
     if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
-    return store_grid(d, *dev, host);
+    return store_grid(d, *dev, host, ncomp);
 }
 
-int alloc_grid_zeroed(Device &d, void **dev)
+int alloc_grid_zeroed(Device &d, void **dev, int ncomp)
 {
-    const size_t bytes = (size_t)d.ncell_dev * d.esize;
+    const size_t bytes = (size_t)d.ncell_dev * d.esize * ncomp;
     if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
     if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
     return SDT_OK;
 }
 
-int store_grid(Device &d, void *dev, const double *host)
+int store_grid(Device &d, void *dev, const double *host, int ncomp)
 {
-    const size_t rows = (size_t)d.ny * d.nz, es = d.esize;
+    const size_t rows = (size_t)d.ny * d.nz, es = d.esize * ncomp;
     char *dst = (char *)dev + d.xoff * es;
     if (d.precision == SDT_FP64) {
         if (hipMemcpy2D(dst, (size_t)d.pitch * es, host, (size_t)d.nx * es, (size_t)d.nx * es, rows,
                         hipMemcpyHostToDevice) != hipSuccess)
             return SDT_ERR_HIP;
     } else {
-        std::vector<float> tmp((size_t)d.ncell);
-        for (int64_t i = 0; i < d.ncell; ++i) tmp[(size_t)i] = (float)host[i];
+        const size_t nv = (size_t)d.ncell * ncomp;
+        std::vector<float> tmp(nv);
+        for (size_t i = 0; i < nv; ++i) tmp[i] = (float)host[i];
         if (hipMemcpy2D(dst, (size_t)d.pitch * es, tmp.data(), (size_t)d.nx * es, (size_t)d.nx * es, rows,
                         hipMemcpyHostToDevice) != hipSuccess)
             return SDT_ERR_HIP;
@@ -184,20 +197,21 @@ int store_grid(Device &d, void *dev, const double *host)
     return SDT_OK;
 }
 
-int download_grid(Device &d, const void *dev, double *host)
+int download_grid(Device &d, const void *dev, double *host, int ncomp)
 {
-    const size_t rows = (size_t)d.ny * d.nz, es = d.esize;
+    const size_t rows = (size_t)d.ny * d.nz, es = d.esize * ncomp;
     const char *src = (const char *)dev + d.xoff * es;
     if (d.precision == SDT_FP64) {
         if (hipMemcpy2D(host, (size_t)d.nx * es, src, (size_t)d.pitch * es, (size_t)d.nx * es, rows,
                         hipMemcpyDeviceToHost) != hipSuccess)
             return SDT_ERR_HIP;
     } else {
-        std::vector<float> tmp((size_t)d.ncell);
+        const size_t nv = (size_t)d.ncell * ncomp;
+        std::vector<float> tmp(nv);
         if (hipMemcpy2D(tmp.data(), (size_t)d.nx * es, src, (size_t)d.pitch * es, (size_t)d.nx * es, rows,
                         hipMemcpyDeviceToHost) != hipSuccess)
             return SDT_ERR_HIP;
-        for (int64_t i = 0; i < d.ncell; ++i) host[i] = tmp[(size_t)i];
+        for (size_t i = 0; i < nv; ++i) host[i] = tmp[i];
     }
     return SDT_OK;
 }

@@ -46,6 +46,7 @@ struct TailArgs {
     const int64_t *rec_cell;         // [nrec]
     double *rec_out;                 // [time_steps][nrec][3]
     int rec_field[3];                // field ids recorded (-1 -> 0.0)
+    int cplx;                        // 1: complex fields (interleaved re, im)
 };
 
 struct Device {
@@ -55,6 +56,8 @@ struct Device {
     int time_steps = 0;
     int isrc = 0, jsrc = 0, ksrc = 0;   // padded 1-based
     int density_code = 3, stencil = 0;
+    double ky = 0.0;                   // 2.5D elastic y wavenumber
+    int ncomp = 1;                     // values per field element (2 = complex)
     int64_t ncell = 0;
     size_t esize = 8;                  // sizeof(T)
     // Device layout of grid arrays: row (j, k) holds x cells 1..nx at
@@ -113,6 +116,8 @@ typedef void (*step_fn)(Device &d, bool timed);
 
 int el2_setup(Device &d, const double *const *host_in);
 void el2_step(Device &d, bool timed);
+int el25_setup(Device &d, const double *const *host_in);
+void el25_step(Device &d, bool timed);
 int el3_setup(Device &d, const double *const *host_in);
 void el3_step(Device &d, bool timed);
 int em2_setup(Device &d, const double *const *host_in);
@@ -141,12 +146,13 @@ inline int64_t cell_index(const Device &d, int i, int j, int k)
     return d.xoff + (int64_t)(i - 1) + (int64_t)d.pitch * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1));
 }
 // grid arrays in the padded layout: upload an FP64 host array given in
-// Fortran order (nx, ny, nz), or allocate a zeroed one
-int upload_grid(Device &d, const double *host, void **dev);
-int alloc_grid_zeroed(Device &d, void **dev);
+// Fortran order (nx, ny, nz), or allocate a zeroed one; ncomp = 2 for complex
+// elements (interleaved re, im on host and device)
+int upload_grid(Device &d, const double *host, void **dev, int ncomp = 1);
+int alloc_grid_zeroed(Device &d, void **dev, int ncomp = 1);
 // copy a device grid array to / from an FP64 host array in Fortran order
-int download_grid(Device &d, const void *dev, double *host);
-int store_grid(Device &d, void *dev, const double *host);
+int download_grid(Device &d, const void *dev, double *host, int ncomp = 1);
+int store_grid(Device &d, void *dev, const double *host, int ncomp = 1);
 // copy n FP64 host values into a freshly allocated device array of the
 // handle's precision
 int upload_converted(Device &d, const double *host, void **dev, int64_t n);

@@ -11,12 +11,12 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libseidart_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 # status codes
 OK, ERR_UNSTABLE, ERR_ARG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4
 # solvers
-ELASTIC2, ELASTIC3, EM2, EM3 = 0, 2, 3, 5
+ELASTIC2, ELASTIC25, ELASTIC3, EM2, EM3 = 0, 1, 2, 3, 5
 # precision
 FP64, FP32 = 0, 1
 # density codes, averaging.f08:5-25
@@ -55,7 +55,7 @@ class Params(C.Structure):
         ("cpml", C.c_int32), ("dt", C.c_double), ("time_steps", C.c_int32),
         ("xind", C.c_int32), ("yind", C.c_int32), ("zind", C.c_int32),
         ("density_code", C.c_int32), ("stencil", C.c_int32), ("device", C.c_int32),
-        ("reserved", C.c_int32 * 8),
+        ("reserved0", C.c_int32), ("ky", C.c_double), ("reserved", C.c_int32 * 6),
     ]
 
 

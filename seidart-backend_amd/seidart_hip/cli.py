@@ -87,7 +87,11 @@ def main(argv=None):
                 print("Running 3D electromagnetic model with", em.time_steps, "time steps")
                 cpmlfdtd.electromag3(domain, em, True)
         elif domain.dim == 2.5:
-            raise NotImplementedError("2.5D solvers are not implemented in seidart_hip yet")
+            if seismic:
+                print("Running 2.5D seismic model with", seis.time_steps, "time steps")
+                cpmlfdtd.seismic25(domain, seis)
+            else:
+                raise NotImplementedError("electromag25 (2.5D EM) is not implemented in seidart_hip yet")
         else:
             # the 2D solvers work on the (x, z) plane
             domain.ny = 1

@@ -1,6 +1,7 @@
 """File-based drop-in of the reference solvers.
 
     seismic2(domain, source, density_method, SINGLE_OUTPUT)     cpmlfdtd.f08:57
+    seismic25(domain, source)                                   cpmlfdtd.f08:600
     seismic3(domain, source, density_method, SINGLE_OUTPUT)     cpmlfdtd.f08:1341
     electromag2(domain, source, SINGLE_OUTPUT)                  cpmlfdtd.f08:2317
     electromag3(domain, source, SINGLE_OUTPUT)                  cpmlfdtd.f08:3353
@@ -10,7 +11,9 @@ padded sizes (src/fortran/main.f08:69-71 pads before the call), every input
 is read from the working directory under the reference's file names, and a
 snapshot of every velocity / E component is written each step under the
 reference's names and formats.  3D elastic also writes velocity_norm.dat and
-stress_norm.dat (cpmlfdtd.f08:2282-2283).
+stress_norm.dat (cpmlfdtd.f08:2282-2283).  The 2.5D solver reads KY,
+DENSITY_METHOD, SINGLE_OUTPUT, FDTD_LEGACY_OUTPUT and FDTD_STEPS_PER_BLOCK from
+the environment as the reference does and writes zstd blocks by default.
 
 Differences from the reference, by design:
 * blow-up raises seidart_hip.UnstableError (the reference prints and
@@ -189,3 +192,81 @@ def electromag3(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
             s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
         _time_loop(s, workdir, source, (("ex", "Ex"), ("ey", "Ey"), ("ez", "Ez")), SINGLE_OUTPUT, True,
                    snapshot_every)
+
+
+def _env_logical(name, default=True):
+    """list-directed read of a LOGICAL (T/F, .TRUE./.FALSE.); unreadable or
+    absent -> default (cpmlfdtd.f08:688-695)."""
+    v = os.environ.get(name)
+    if v is None:
+        return default
+    t = v.strip().lstrip(".").upper()
+    if t.startswith("T"):
+        return True
+    if t.startswith("F"):
+        return False
+    return default
+
+
+def _env_ky():
+    """KY, 0 when absent or unreadable (cpmlfdtd.f08:697-703)."""
+    try:
+        return float(os.environ.get("KY", "0").strip().replace("d", "e").replace("D", "e"))
+    except ValueError:
+        return 0.0
+
+
+def _env_density():
+    """DENSITY_METHOD, harmonic by default and for unknown values
+    (cpmlfdtd.f08:705-716)."""
+    v = os.environ.get("DENSITY_METHOD", "").strip()
+    return v if v in ("harmonic", "geometric", "arithmetic", "none") else "harmonic"
+
+
+def seismic25(domain, source, *, workdir=".", precision="fp64", device=0, ky=None, density_method=None,
+              SINGLE_OUTPUT=None, legacy_output=None, steps_per_block=None):
+    """2.5D elastic; inputs c11..c66 rho gamma_* CPML, complex128
+    initialconditionV{x,y,z} and seismicsource{x,y,z} ('ac') or
+    seismicsource{xx,xy,xz,yy,yz,zz} (cpmlfdtd.f08:763-872).  Keyword
+    arguments left as None take the reference's environment variables.
+    Output: S25.<first>-<last>.<xind>.<yind>.<zind>.blk.zst blocks of the
+    real parts of vx, vy, vz (default), or with FDTD_LEGACY_OUTPUT=TRUE
+    per-step Vx/Vy/Vz images of the real parts (cpmlfdtd.f08:1295-1305)."""
+    st = _check_source_type(source)
+    ky = _env_ky() if ky is None else float(ky)
+    density_method = _env_density() if density_method is None else density_method
+    single = _env_logical("SINGLE_OUTPUT") if SINGLE_OUTPUT is None else bool(SINGLE_OUTPUT)
+    if legacy_output is None:
+        legacy_output = os.environ.get("FDTD_LEGACY_OUTPUT", "").strip() == "TRUE"
+    shape = (domain.nx, domain.nz)
+    writer = None
+    if not legacy_output:
+        spb = steps_per_block or fio.steps_per_block_2d(domain.nx, domain.nz, domain.cpml)
+        writer = fio.BlockWriter2D(domain.nx, domain.nz, domain.cpml, spb, source.time_steps, "S25",
+                                   source.xind, source.yind, source.zind, workdir)
+    with Solver("elastic25", shape, (domain.dx, domain.dz), domain.cpml, source.dt, source.time_steps,
+                (source.xind, source.zind), precision=precision, density_method=density_method,
+                device=device, ky=ky) as s:
+        for name in ("c11", "c12", "c13", "c14", "c15", "c16", "c22", "c23", "c24", "c25", "c26",
+                     "c33", "c34", "c35", "c36", "c44", "c45", "c46", "c55", "c56", "c66", "rho",
+                     "gamma_x", "gamma_z", "gamma_y", "gamma_xz", "gamma_yz", "gamma_xy"):
+            s.set_array(name, _read(workdir, name, shape))
+        _cpml_inputs(s, workdir, shape)
+        for c in "xyz":
+            s.set_array("ic_" + c, fio.read_array(os.path.join(workdir, "initialconditionV" + c + ".dat"),
+                                                  shape, np.complex128))
+        comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
+        for c in comps:
+            s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
+        for it in range(1, source.time_steps + 1):
+            s.run(1)
+            vx, vy, vz = s.field("vx"), s.field("vy"), s.field("vz")
+            if writer is not None:
+                writer.add(vx, vy, vz)
+            if legacy_output:
+                for a, ch in ((vx, "Vx"), (vy, "Vy"), (vz, "Vz")):
+                    path = os.path.join(workdir, fio.snapshot_name(ch, it, source.xind, source.zind))
+                    fio.write_record(path, a.real, np.float32 if single else np.float64)
+        if writer is not None:
+            writer.flush()
+    return writer.written if writer is not None else []

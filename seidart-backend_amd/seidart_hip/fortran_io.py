@@ -41,9 +41,11 @@ def read_record(path, dtype=np.float64, count=None):
 
 
 def read_array(path, shape, dtype=np.float64):
-    """Column-major (x fastest) array of `shape` from one record."""
+    """Column-major (x fastest) array of `shape` from one record (complex128
+    records, as read by material_rw2c at seidartio.f08:249-270, stay complex)."""
     n = int(np.prod(shape))
-    return read_record(path, dtype, n).reshape(shape, order="F").astype(np.float64)
+    out = np.dtype(np.complex128) if np.dtype(dtype).kind == "c" else np.dtype(np.float64)
+    return read_record(path, dtype, n).reshape(shape, order="F").astype(out)
 
 
 def write_record(path, array, dtype=None):
@@ -74,3 +76,106 @@ def snapshot_name(channel, it, xind, zind, yind=None):
     if yind is None:
         return f"{channel}.{it:06d}.{xind}.{zind}.dat"
     return f"{channel}.{it:06d}.{xind}.{yind}.{zind}.dat"
+
+
+# --------------------------------------------------------------- zstd blocks ----
+_ZSTD_MAGIC = 0xFD2FB528
+_ZSTD_RAW_BLOCK = 128 * 1024
+
+
+def zstd_raw_frame(payload):
+    """A zstd frame (RFC 8878) of raw (stored) blocks: single-segment header
+    with the 8-byte content size, blocks of at most 128 KiB, no checksum.
+    Every zstd decoder reads it; it is used when no zstd compressor is
+    importable."""
+    payload = bytes(payload)
+    out = bytearray(struct.pack("<I", _ZSTD_MAGIC))
+    out.append((3 << 6) | (1 << 5))          # FCS 8 bytes, single segment
+    out += struct.pack("<Q", len(payload))
+    pos, n = 0, len(payload)
+    while True:
+        size = min(n - pos, _ZSTD_RAW_BLOCK)
+        last = pos + size >= n
+        header = (size << 3) | (0 << 1) | (1 if last else 0)   # block type 0 = raw
+        out += struct.pack("<I", header)[:3]
+        out += payload[pos:pos + size]
+        pos += size
+        if last:
+            break
+    return bytes(out)
+
+
+def zstd_compress(payload, level=3):
+    """zstd frame of `payload` at `level` (seidartio.f08:559-560 uses 3) via
+    pyarrow's bundled libzstd, else a frame of raw blocks."""
+    try:
+        import pyarrow as pa
+        return pa.Codec("zstd", compression_level=level).compress(payload, asbytes=True)
+    except ImportError:
+        return zstd_raw_frame(payload)
+
+
+def zstd_decompress(frame, size):
+    """Inverse of zstd_compress for a frame of known decompressed size."""
+    import pyarrow as pa
+    return pa.Codec("zstd").decompress(frame, decompressed_size=size, asbytes=True)
+
+
+def block_name(prefix, first, last, xind, yind, zind):
+    """`(a, i6.6, '-', i6.6, '.', i0, '.', i0, '.', i0, '.blk.zst')` of the
+    2.5D block files (cpmlfdtd.f08:1299-1302 'S25.', 3317-3320 'EM25.')."""
+    return f"{prefix}.{first:06d}-{last:06d}.{xind}.{yind}.{zind}.blk.zst"
+
+
+class BlockWriter2D:
+    """Block output of the 2.5D solvers (seidartio.f08:614-734): each step's
+    real parts of three components, CPML stripped, as FP32 in a Fortran-order
+    buffer (nx - 2c, 1, nz - 2c, 3, steps); a full (or the final, partial)
+    block is written as one zstd frame of its populated leading part."""
+
+    def __init__(self, nx, nz, cpml, steps_per_block, time_steps, prefix, xind, yind, zind, workdir="."):
+        if steps_per_block < 1:
+            raise ValueError("steps per zstd block must be at least 1")
+        self.onx, self.onz = nx - 2 * cpml, nz - 2 * cpml
+        if self.onx < 1 or self.onz < 1:
+            raise ValueError("cpml removes the entire zstd block output domain")
+        self.cpml, self.spb, self.nsteps = cpml, steps_per_block, time_steps
+        self.buf = np.zeros((self.onx, 1, self.onz, 3, steps_per_block), dtype=np.float32, order="F")
+        self.cur, self.block = 0, 0
+        self.prefix, self.ind, self.workdir = prefix, (xind, yind, zind), workdir
+        self.written = []
+
+    def add(self, a, b, c):
+        if self.cur == 0:
+            self.block += 1
+            first = (self.block - 1) * self.spb + 1
+            last = min(self.block * self.spb, self.nsteps)
+            self.name = block_name(self.prefix, first, last, *self.ind)
+        cp = self.cpml
+        for q, f in enumerate((a, b, c)):
+            f = np.asarray(f)
+            self.buf[:, 0, :, q, self.cur] = np.real(f[cp:f.shape[0] - cp, cp:f.shape[1] - cp]).astype(np.float32)
+        self.cur += 1
+        if self.cur == self.spb:
+            self.flush()
+
+    def flush(self):
+        if self.cur < 1:
+            return
+        n = self.onx * self.onz * 3 * self.cur
+        payload = self.buf.ravel(order="F")[:n].astype("<f4").tobytes()
+        path = os.path.join(self.workdir, self.name)
+        with open(path, "wb") as f:
+            f.write(zstd_compress(payload))
+        self.written.append((path, len(payload)))
+        self.cur = 0
+
+
+def steps_per_block_2d(nx, nz, cpml, env=None):
+    """FDTD_STEPS_PER_BLOCK, else about 2 GiB of FP32 per block capped at 500
+    steps (seidartio.f08:685-734)."""
+    env = os.environ if env is None else env
+    if "FDTD_STEPS_PER_BLOCK" in env:
+        return int(env["FDTD_STEPS_PER_BLOCK"].strip())
+    bytes_per_step = float(nx - 2 * cpml) * (nz - 2 * cpml) * 3 * 4
+    return min(500, max(1, int(2147483648.0 / bytes_per_step)))

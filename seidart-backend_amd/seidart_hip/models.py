@@ -271,6 +271,47 @@ def elastic3(nx=256, ny=256, nz=256, d=0.5, cpml=12, steps=2000, seed=1, nlayers
                 receivers=recs, density_method=density_method, vmax=vmax)
 
 
+def elastic25(nx=120, nz=100, dx=1.0, dz=1.0, cpml=10, steps=400, seed=5, nlayers=3, f0=60.0,
+              src=(60, 30), cfl=0.4, nrec=24, ky=0.05, density_method="harmonic", anomaly=True,
+              with_cpml=True):
+    """2.5D elastic (x-z grid, y handled as i*ky): seeded layers of Euler-rotated
+    ice tensors with attenuation and, optionally, a rectangular anomaly of a
+    different material so density varies along x as well as z."""
+    rng = np.random.default_rng(seed)
+    NX, NZ = nx + 2 * cpml, nz + 2 * cpml
+    shape = (NX, NZ)
+    lid = layer_index(NZ, cpml, nlayers, rng, min_thick=min(4, max(1, nz // (2 * nlayers))))
+    nmat = nlayers + (1 if anomaly else 0)
+    Cs, rhos, gams = [], [], []
+    for _ in range(nmat):
+        a, b, g = rng.uniform(0, 2 * math.pi), math.acos(rng.uniform(-1, 1)), rng.uniform(0, 2 * math.pi)
+        Cs.append(rotate_voigt(ice_voigt(rng), euler_zyz(a, b, g)))
+        rhos.append(rng.uniform(900.0, 2600.0))
+        gams.append(rng.uniform(0.001, 0.01, 6))
+    mid = np.broadcast_to(lid[None, :], shape).copy()
+    if anomaly:
+        i0, i1 = cpml + nx // 3, cpml + (2 * nx) // 3
+        k0, k1 = cpml + nz // 2, cpml + (3 * nz) // 4
+        mid[i0:i1, k0:k1] = nlayers
+    arrays = {}
+    for name, (i, j) in zip(VOIGT_NAMES, VOIGT_IJ):
+        arrays[name] = np.asfortranarray(np.array([C[i, j] for C in Cs])[mid])
+    arrays["rho"] = np.asfortranarray(np.array(rhos)[mid])
+    for q, gname in enumerate(["gamma_x", "gamma_y", "gamma_z", "gamma_xy", "gamma_xz", "gamma_yz"]):
+        arrays[gname] = np.asfortranarray(np.array([g[q] for g in gams])[mid])
+    vmax = max_speed(Cs, min(rhos))
+    dt = cfl * min(dx, dz) / (math.sqrt(2.0) * vmax)
+    arrays.update(cpml_arrays(shape, (dx, dz), cpml, vmax, dt, f0) if with_cpml else no_cpml(shape))
+    for c in ("ic_x", "ic_y", "ic_z"):
+        arrays[c] = np.zeros(shape, dtype=np.complex128, order="F")
+    nrec = min(nrec, nx)
+    recs = np.array([[1 + (nx - 1) * k // max(nrec - 1, 1), 1, src[1]] for k in range(nrec)], dtype=np.int32)
+    return dict(physics="elastic25", shape=shape, spacing=(dx, dz), cpml=cpml, dt=dt, time_steps=steps,
+                src=src, arrays=arrays, ky=float(ky),
+                sources={"x": np.zeros(steps), "y": np.zeros(steps), "z": ricker(f0, dt, steps)},
+                receivers=recs, density_method=density_method, vmax=vmax)
+
+
 def em3(nx=384, ny=384, nz=192, d=0.05, cpml=10, steps=2000, seed=2, nlayers=5, f0=100e6,
         src=(192, 192, 20), cfl=0.5, nrec=128, with_cpml=True):
     """3D EM, five seeded SPD permittivity/conductivity layers (the
@@ -332,7 +373,7 @@ def em2(nx=600, nz=300, d=0.05, cpml=10, steps=3000, seed=4, nlayers=6, f0=100e6
 
 
 # ------------------------------------------------------ reference file set ----
-_IC_PREFIX = {"elastic2": "initialconditionV", "elastic3": "initialconditionV",
+_IC_PREFIX = {"elastic2": "initialconditionV", "elastic25": "initialconditionV", "elastic3": "initialconditionV",
               "em2": "initialconditionE", "em3": "initialconditionE"}
 
 
@@ -347,23 +388,27 @@ def write_inputs(cfg, workdir, source_type="ac"):
 
     phys = cfg["physics"]
     is3d = phys.endswith("3")
+    is25 = phys == "elastic25"
     os.makedirs(workdir, exist_ok=True)
     shape = cfg["shape"]
     defaults = no_cpml(shape)
+    ic_dtype = np.complex128 if is25 else np.float64   # material_rw2c for 2.5D
     for name, a in {**defaults, **cfg["arrays"]}.items():
         if name.startswith("ic_"):
             fname = _IC_PREFIX[phys] + name[-1]
+            dt_ = ic_dtype
         else:
             fname = name
-        fio.write_record(os.path.join(workdir, fname + ".dat"), np.asarray(a, dtype=np.float64).reshape(shape))
-    comps = "xyz" if is3d else "xz"
+            dt_ = np.float64
+        fio.write_record(os.path.join(workdir, fname + ".dat"), np.asarray(a, dtype=dt_).reshape(shape))
+    comps = "xyz" if (is3d or is25) else "xz"
     for c in comps:
         if "ic_" + c not in cfg["arrays"]:
-            fio.write_record(os.path.join(workdir, _IC_PREFIX[phys] + c + ".dat"), np.zeros(shape))
+            fio.write_record(os.path.join(workdir, _IC_PREFIX[phys] + c + ".dat"), np.zeros(shape, dtype=ic_dtype))
     n = cfg["time_steps"]
     stem = "seismicsource" if phys.startswith("elastic") else "electromagneticsource"
     names = (list(comps) if (source_type == "ac" or not phys.startswith("elastic"))
-             else (["xx", "xy", "xz", "yy", "yz", "zz"] if is3d else ["xx", "xz", "zz"]))
+             else (["xx", "xy", "xz", "yy", "yz", "zz"] if (is3d or is25) else ["xx", "xz", "zz"]))
     for c in names:
         s = np.asarray(cfg["sources"].get(c, np.zeros(n)), dtype=np.float64)
         fio.write_record(os.path.join(workdir, stem + c + ".dat"), s)
@@ -383,7 +428,7 @@ def write_inputs(cfg, workdir, source_type="ac"):
            "source_frequency": 0.0, "x-z_rotation": 0.0, "x-y_rotation": 0.0, "y-z_rotation": 0.0,
            "amplitude": 1.0, "source_type": source_type, "source_wavelet": "gaus2",
            "dt": float(cfg["dt"]), "time_steps": int(n)}
-    doc = {"Domain": {"dim": 3.0 if is3d else 2.0, "nx": int(nx), "ny": int(ny), "nz": int(nz),
+    doc = {"Domain": {"dim": 3.0 if is3d else (2.5 if is25 else 2.0), "nx": int(nx), "ny": int(ny), "nz": int(nz),
                       "dx": float(dx), "dy": float(dy), "dz": float(dz), "cpml": int(cp), "nmats": 1,
                       "image_file": "model.png"},
            "Seismic": {"Source": dict(src)}, "Electromagnetic": {"Source": dict(src)}}

@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _lib as L
 
-PHYSICS = {"elastic2": L.ELASTIC2, "elastic3": L.ELASTIC3, "em2": L.EM2, "em3": L.EM3}
+PHYSICS = {"elastic2": L.ELASTIC2, "elastic25": L.ELASTIC25, "elastic3": L.ELASTIC3, "em2": L.EM2,
+           "em3": L.EM3}
 
 
 def _dptr(a):
@@ -23,15 +24,19 @@ def _dptr(a):
 class Solver:
     """One simulation on one GPU.
 
-    physics: "elastic2" | "elastic3" | "em2" | "em3"
+    physics: "elastic2" | "elastic25" | "elastic3" | "em2" | "em3"
     shape: padded (nx, nz) or (nx, ny, nz)
     spacing: (dx, dz) or (dx, dy, dz)
     src: 1-based unpadded source indices (xind, zind) or (xind, yind, zind),
          as Source_Type%xind.. (src/fortran/seidart_types.f08:23-33)
+    ky: y wavenumber of the 2.5D solver (the reference's KY environment
+        variable, src/fortran/cpmlfdtd.f08:697-703); its fields, initial
+        conditions and receiver samples are complex128
     """
 
     def __init__(self, physics, shape, spacing, cpml, dt, time_steps, src, *,
-                 precision="fp64", density_method="arithmetic", stencil="reference", device=0):
+                 precision="fp64", density_method="arithmetic", stencil="reference", device=0,
+                 ky=0.0):
         if physics not in PHYSICS:
             raise ValueError(f"unknown physics {physics!r}")
         self.physics = physics
@@ -60,7 +65,9 @@ class Solver:
         p.density_code = L.DENSITY_CODES[density_method]
         p.stencil = {"reference": L.STENCIL_REFERENCE, "corrected": L.STENCIL_CORRECTED}[stencil]
         p.device = int(device)
+        p.ky = float(ky)
         self.params = p
+        self.complex = physics == "elastic25"
         self.ncell = int(np.prod(self.shape))
         self.time_steps = int(time_steps)
         self.fields = L.FIELD_IDS_ELASTIC if physics.startswith("elastic") else L.FIELD_IDS_EM
@@ -93,9 +100,18 @@ class Solver:
             raise ValueError(f"array shape {a.shape} != grid {self.shape}")
         return np.asfortranarray(a).ravel(order="F")
 
+    def _complex_grid(self, a):
+        a = np.asarray(a, dtype=np.complex128)
+        if a.shape != self.shape:
+            raise ValueError(f"array shape {a.shape} != grid {self.shape}")
+        return np.ascontiguousarray(a.ravel(order="F")).view(np.float64)
+
     def set_array(self, name, a):
         """name: reference input file stem, e.g. 'c11', 'rho', 'kappa_cpml', 'ic_x'."""
-        flat = np.ascontiguousarray(self._grid_array(a))
+        if self.complex and name.startswith("ic_"):
+            flat = self._complex_grid(a)
+        else:
+            flat = np.ascontiguousarray(self._grid_array(a))
         L.check(L.lib.sdt_set_array(self._h, L.ARRAY_IDS[name], _dptr(flat), flat.size), self._h)
 
     def set_arrays(self, arrays):
@@ -132,19 +148,25 @@ class Solver:
 
     # ---- outputs ------------------------------------------------------------
     def field(self, name):
-        out = np.empty(self.ncell, dtype=np.float64)
-        L.check(L.lib.sdt_get_field(self._h, self.fields[name], _dptr(out), out.size), self._h)
+        if self.complex:
+            out = np.empty(self.ncell, dtype=np.complex128)
+            flat = out.view(np.float64)
+        else:
+            out = flat = np.empty(self.ncell, dtype=np.float64)
+        L.check(L.lib.sdt_get_field(self._h, self.fields[name], _dptr(flat), flat.size), self._h)
         return out.reshape(self.shape, order="F")
 
     def set_field(self, name, a):
-        flat = np.ascontiguousarray(self._grid_array(a))
+        flat = self._complex_grid(a) if self.complex else np.ascontiguousarray(self._grid_array(a))
         L.check(L.lib.sdt_set_field(self._h, self.fields[name], _dptr(flat), flat.size), self._h)
 
     def receivers(self):
-        """(steps_done, n_receivers, 3) array of the velocity / E components."""
+        """(steps_done, n_receivers, 3) array of the velocity / E components
+        (complex128 for the 2.5D solver)."""
         n = self.steps_done
-        out = np.empty((n, self.nrec, 3), dtype=np.float64)
-        L.check(L.lib.sdt_get_receivers(self._h, _dptr(out), out.size), self._h)
+        out = np.empty((n, self.nrec, 3), dtype=np.complex128 if self.complex else np.float64)
+        flat = out.view(np.float64)
+        L.check(L.lib.sdt_get_receivers(self._h, _dptr(flat), flat.size), self._h)
         return out
 
     def norms(self):
@@ -178,7 +200,7 @@ def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=(), option
     s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"],
                cfg["time_steps"], cfg["src"], precision=precision,
                density_method=cfg.get("density_method", "arithmetic"),
-               stencil=cfg.get("stencil", "reference"), device=device)
+               stencil=cfg.get("stencil", "reference"), device=device, ky=cfg.get("ky", 0.0))
     try:
         for k, v in (options or {}).items():
             s.set_option(k, v)

@@ -15,8 +15,9 @@ def pytest_configure(config):
 
 def rel_l2(a, b):
     import numpy as np
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    dt = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = np.asarray(a, dtype=dt)
+    b = np.asarray(b, dtype=dt)
     nb = np.linalg.norm(b)
     if nb == 0.0:
         return float(np.linalg.norm(a))

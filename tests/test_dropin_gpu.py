@@ -77,3 +77,53 @@ def test_cli_reports_blowup(tmp_path, monkeypatch):
     print("[dropin] forced blow-up", flush=True)
     assert cli.main(["input.json"]) == 1
     assert not os.path.exists(tmp_path / fio.snapshot_name("Vx", 1, 20, 10))
+
+
+def test_cli_seismic25_zstd_blocks(tmp_path, monkeypatch):
+    """2.5D through the CLI with the reference's environment (KY,
+    DENSITY_METHOD, FDTD_STEPS_PER_BLOCK): zstd blocks of the CPML-stripped
+    real parts, named S25.<first>-<last>.<xind>.<yind>.<zind>.blk.zst."""
+    from seidart_hip import cli, fortran_io as fio, models
+    cfg = models.elastic25(nx=48, nz=40, cpml=6, steps=40, src=(24, 12), nrec=4, ky=0.15,
+                           density_method="geometric")
+    models.write_inputs(cfg, str(tmp_path))
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("KY", "0.15")
+    monkeypatch.setenv("DENSITY_METHOD", "geometric")
+    monkeypatch.setenv("FDTD_STEPS_PER_BLOCK", "16")
+    monkeypatch.delenv("FDTD_LEGACY_OUTPUT", raising=False)
+    print("[dropin] cli seismic25 blocks", flush=True)
+    assert cli.main(["input.json", "seismic=true"]) == 0
+    names = [fio.block_name("S25", a, b, 24, 1, 12) for a, b in ((1, 16), (17, 32), (33, 40))]
+    for n in names:
+        assert (tmp_path / n).exists(), n
+    _, _, f = _oracle(cfg, ("vx", "vy", "vz"))
+    onx, onz = 48, 40
+    raw = fio.zstd_decompress((tmp_path / names[2]).read_bytes(), onx * onz * 3 * 8 * 4)
+    blk = np.frombuffer(raw, dtype="<f4").reshape((onx, 1, onz, 3, 8), order="F")
+    for q, key in enumerate(("vx", "vy", "vz")):
+        want = f[key].real[6:-6, 6:-6].astype(np.float32)
+        assert rel_l2(blk[:, 0, :, q, 7], want) < 1e-6, key
+
+
+def test_dropin_seismic25_legacy_images(tmp_path, monkeypatch):
+    from seidart_hip import cpmlfdtd, fortran_io as fio, models
+    cfg = models.elastic25(nx=40, nz=36, cpml=6, steps=20, src=(20, 10), nrec=4, ky=0.0,
+                           density_method="harmonic")
+    s = cfg["sources"]["z"]
+    cfg["sources"] = {"xx": s, "xy": 0.4 * s, "xz": -0.3 * s, "yy": 0.8 * s, "yz": 0.2 * s, "zz": -s}
+    models.write_inputs(cfg, str(tmp_path), source_type="tensor")
+    sh = cfg["shape"]
+    dom = cpmlfdtd.Domain(dim=2.5, nx=sh[0], ny=1, nz=sh[1], dx=1.0, dy=1.0, dz=1.0, cpml=6)
+    src = cpmlfdtd.Source(dt=cfg["dt"], time_steps=20, xind=20, yind=1, zind=10, source_type="tensor")
+    monkeypatch.delenv("KY", raising=False)
+    monkeypatch.delenv("DENSITY_METHOD", raising=False)
+    monkeypatch.setenv("FDTD_LEGACY_OUTPUT", "TRUE")
+    monkeypatch.setenv("SINGLE_OUTPUT", ".FALSE.")
+    print("[dropin] seismic25 legacy images", flush=True)
+    cpmlfdtd.seismic25(dom, src, workdir=str(tmp_path))
+    _, _, f = _oracle(cfg, ("vx", "vz"))
+    assert np.abs(f["vz"]).max() > 0
+    for ch, key in (("Vx", "vx"), ("Vz", "vz")):
+        got = fio.read_array(str(tmp_path / fio.snapshot_name(ch, 20, 20, 10)), sh)
+        assert rel_l2(got, f[key].real) < 1e-10

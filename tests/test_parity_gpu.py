@@ -146,12 +146,13 @@ def test_em3_fp64():
     _check(g, o, TOL64)
 
 
-@pytest.mark.parametrize("physics", ["elastic2", "elastic3", "em2", "em3"])
+@pytest.mark.parametrize("physics", ["elastic2", "elastic25", "elastic3", "em2", "em3"])
 def test_fp32_close_to_fp64(physics):
     """FP32 wavefields with FP64-precomputed coefficients: trace error vs the
     FP64 oracle stated per solver (bar 1e-3 relative L2 on these short runs)."""
     from seidart_hip import models
     cfg = {"elastic2": lambda: models.elastic2(nx=80, nz=70, cpml=8, steps=300, src=(40, 20), rec_z=20, nrec=8),
+           "elastic25": lambda: models.elastic25(nx=70, nz=60, cpml=8, steps=250, src=(35, 20), nrec=8),
            "elastic3": lambda: models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16),
            "em2": lambda: models.em2(nx=120, nz=80, steps=400, src=(60, 20), nrec=16),
            "em3": lambda: models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)}[physics]()
@@ -159,3 +160,40 @@ def test_fp32_close_to_fp64(physics):
     err = rel_l2(g[0], o[0])
     print(f"[fp32] {physics} receiver rel L2 vs FP64 oracle: {err:.3e}", flush=True)
     assert err <= 1e-3
+
+
+EL25_FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def test_elastic25_fp64():
+    """2.5D elastic: layered rotated-ice model with a lateral anomaly,
+    attenuation, harmonic density smear, ky = 0.05, 400 steps."""
+    from seidart_hip import models
+    cfg = models.elastic25()
+    g, o = _run_both(cfg, EL25_FIELDS)
+    _check(g, o, TOL64)
+
+
+@pytest.mark.parametrize("density,ky", [("arithmetic", 0.0), ("geometric", 0.3), ("none", 0.12)])
+def test_elastic25_sources_and_density(density, ky):
+    from seidart_hip import models
+    cfg = models.elastic25(nx=70, nz=60, cpml=8, steps=250, src=(35, 20), nrec=8, ky=ky,
+                           density_method=density)
+    s = cfg["sources"]["z"]
+    cfg["sources"] = {"xx": s, "xy": 0.3 * s, "xz": -0.2 * s, "yy": 0.9 * s, "yz": 0.4 * s,
+                      "zz": -s, "x": 0.1 * s, "y": -0.2 * s, "z": 0.3 * s}
+    g, o = _run_both(cfg, EL25_FIELDS)
+    _check(g, o, TOL64)
+
+
+def test_elastic25_complex_ic_corrected_stencil():
+    from seidart_hip import models
+    cfg = models.elastic25(nx=64, nz=56, cpml=8, steps=200, src=(32, 20), nrec=8, ky=0.2)
+    cfg["stencil"] = "corrected"
+    shape = cfg["shape"]
+    x, z = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    g0 = np.exp(-((x - 40) ** 2 + (z - 30) ** 2) / 20.0)
+    cfg["arrays"]["ic_x"] = (0.3 + 0.7j) * g0
+    cfg["arrays"]["ic_z"] = (-0.5 + 0.2j) * np.roll(g0, 5, axis=1)
+    g, o = _run_both(cfg, EL25_FIELDS)
+    _check(g, o, TOL64)
