@@ -37,15 +37,52 @@ EL3 = dict(F=9, M=28, PSI=29, C=6)
 EL3_STRESS = dict(reads_v=3, rw_sigma=6, mat=27, psi=20, C=6)
 
 
-def algorithmic_bytes(shape, cpml, w):
+def algorithmic_bytes(shape, cpml, w, table=False):
+    """Bytes per step and per stress pass.  table=False is the model with one
+    array per material parameter (M = 28 per cell); table=True replaces those
+    by the one-byte material id read once per pass."""
     nx, ny, nz = shape
     n = nx * ny * nz
     n_int = max(nx - 2 * cpml, 0) * max(ny - 2 * cpml, 0) * max(nz - 2 * cpml, 0)
     n_pml = n - n_int
-    step = w * (n * (2 * EL3["F"] + EL3["M"]) + n_pml * (2 * EL3["PSI"] + EL3["C"]))
+    M, Ms = (0, 0) if table else (EL3["M"], EL3_STRESS["mat"])
+    step = w * (n * (2 * EL3["F"] + M) + n_pml * (2 * EL3["PSI"] + EL3["C"])) + (2 * n if table else 0)
     s = EL3_STRESS
-    stress = w * (n * (s["reads_v"] + 2 * s["rw_sigma"] + s["mat"]) + n_pml * (2 * s["psi"] + s["C"]))
+    stress = w * (n * (s["reads_v"] + 2 * s["rw_sigma"] + Ms) + n_pml * (2 * s["psi"] + s["C"])) + (n if table else 0)
     return step, stress
+
+
+def timed_run(cfg, args, material_table):
+    """W untimed steps (the first uploads every input), then K steps timed by
+    the wall clock around synchronisations and, per kernel pass, by HIP events
+    on the solver's stream."""
+    import numpy as np
+    from seidart_hip.solver import Solver
+    K, W = args.steps, args.warmup
+    s = Solver("elastic3", cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K, cfg["src"],
+               precision=args.precision, density_method=cfg["density_method"])
+    s.set_option("zchunk", args.zchunk)
+    s.set_option("stagger", args.stagger)
+    s.set_option("tile_rows", args.tile_rows)
+    s.set_option("material_table", material_table)
+    s.set_arrays(cfg["arrays"])
+    for comp, series in cfg["sources"].items():
+        s.set_source(comp, series)
+    s.set_receivers(cfg["receivers"])
+    s.run(W)
+    s.synchronize()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.run(K)
+    s.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_stress, ms_vel, launches = s.kernel_times()
+    nmat = s.materials()
+    rec = s.receivers()
+    s.close()
+    if not np.all(np.isfinite(rec)):
+        raise RuntimeError("non-finite receiver traces")
+    return elapsed, ms_stress, ms_vel, launches, nmat
 
 
 def measured_traffic():
@@ -92,6 +129,10 @@ def main():
     ap.add_argument("--zchunk", type=int, default=0, help="planes per z-marching workgroup (0 = auto)")
     ap.add_argument("--stagger", type=int, default=1, help="staggered array starts (1) or plain (0)")
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per interior workgroup tile (0 = auto)")
+    ap.add_argument("--material-table", type=int, default=1,
+                    help="1: material ids + table when the model allows (default), 0: per-cell arrays")
+    ap.add_argument("--compare-layouts", type=int, default=1,
+                    help="also time the per-cell-array layout and report it beside the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -105,7 +146,6 @@ def main():
 
     import numpy as np
     from seidart_hip import models
-    from seidart_hip.solver import Solver
 
     K, W = args.steps, args.warmup
     n = args.n
@@ -116,34 +156,18 @@ def main():
     ncell = int(np.prod(shape))
     w = 8 if args.precision == "fp64" else 4
 
-    s = Solver("elastic3", shape, cfg["spacing"], cfg["cpml"], cfg["dt"], W + K, cfg["src"],
-               precision=args.precision, density_method=cfg["density_method"])
-    s.set_option("zchunk", args.zchunk)
-    s.set_option("stagger", args.stagger)
-    s.set_option("tile_rows", args.tile_rows)
-    s.set_arrays(cfg["arrays"])
-    for comp, series in cfg["sources"].items():
-        s.set_source(comp, series)
-    s.set_receivers(cfg["receivers"])
-    s.run(W)           # first call uploads every input to HBM
-    s.synchronize()
-
-    s.set_timing(True)
-    t0 = time.perf_counter()
-    s.run(K)
-    s.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_stress, ms_vel, launches = s.kernel_times()
-    rec = s.receivers()
-    s.close()
-    if not np.all(np.isfinite(rec)):
-        raise RuntimeError("non-finite receiver traces")
+    main_run = timed_run(cfg, args, args.material_table)
+    dense_run = timed_run(cfg, args, 0) if (args.material_table and args.compare_layouts) else None
+    elapsed, ms_stress, ms_vel, launches, nmat = main_run
 
     value = ncell * K / elapsed / 1e9
     step_bytes, stress_bytes = algorithmic_bytes(shape, cfg["cpml"], w)
+    table = bool(args.material_table) and nmat > 0
+    step_bytes_path, stress_bytes_path = algorithmic_bytes(shape, cfg["cpml"], w, table=table)
     avg_stress_s = ms_stress / max(launches, 1) / 1e3
     avg_vel_s = ms_vel / max(launches, 1) / 1e3
     achieved = stress_bytes / avg_stress_s / 1e9
+    achieved_path = stress_bytes_path / avg_stress_s / 1e9
     traffic = measured_traffic() if args.precision == "fp64" and n == 256 else None
 
     cpu = None
@@ -175,16 +199,29 @@ def main():
         "config": {"workload": f"seismic3 {n}^3 + {cfg['cpml']}-cell CPML ({shape[0]}^3 padded), "
                                f"dx=0.5 m, 8 layers, attenuation, 256 receivers",
                    "cells": ncell, "precision": args.precision,
+                   "material_layout": f"table ({nmat} materials, 1-byte id per cell)" if table
+                                      else "per-cell arrays",
                    "zchunk": args.zchunk, "stagger": args.stagger, "tile_rows": args.tile_rows},
         "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "hbm_fraction_step_path_bytes": round(step_bytes_path * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": "el3 stress pass (interior + CPML-shell kernels)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes": stress_bytes, "avg_launch_ms": round(avg_stress_s * 1e3, 4),
+                     "algorithmic_bytes": stress_bytes,
+                     "path_algorithmic_bytes": stress_bytes_path,
+                     "path_achieved": round(achieved_path, 1),
+                     "path_frac": round(achieved_path / HBM_PEAK_GBS, 4),
+                     "avg_launch_ms": round(avg_stress_s * 1e3, 4),
                      "velocity_kernel_avg_ms": round(avg_vel_s * 1e3, 4)},
         "cpu_baseline": cpu,
         "setup_s": round(t_gen, 2),
     }
+    if dense_run is not None:
+        e2, s2, v2, l2, _ = dense_run
+        line["per_cell_arrays_layout"] = {
+            "value": round(ncell * K / e2 / 1e9, 4), "ms_per_step": round(e2 / K * 1e3, 4),
+            "hbm_fraction_step": round(step_bytes * K / e2 / 1e9 / HBM_PEAK_GBS, 4),
+            "stress_pass_ms": round(s2 / max(l2, 1), 4), "velocity_pass_ms": round(v2 / max(l2, 1), 4)}
     print(json.dumps(line), flush=True)
 
 

@@ -203,9 +203,23 @@ enum {
                                    grid is small (default), 0 = direct launches          */
     SDT_OPT_STAGGER = 3,        /* 1 = offset each grid array's start by a different
                                    multiple of 4352 B (default), 0 = plain hipMalloc     */
-    SDT_OPT_TILE_ROWS = 4       /* rows of the 3D interior workgroup tile: 0 auto, 4, 8, 16 */
+    SDT_OPT_TILE_ROWS = 4,      /* rows of the 3D interior workgroup tile: 0 auto, 4, 8, 16 */
+    SDT_OPT_MATERIAL_TABLE = 5  /* 3D elastic: 1 = when the model holds at most 256 distinct
+                                   (c11..c66, gamma_*, rho) tuples, keep one byte of material id
+                                   per cell and a table of the tuples instead of 28 per-cell
+                                   arrays (default; same values, results equal to rounding),
+                                   0 = always the per-cell arrays                          */
 };
 int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
+
+/* read-only facts about the device layout chosen at setup (no reference
+ * counterpart); valid after the first sdt_run */
+enum {
+    SDT_INFO_MATERIALS = 1,     /* distinct materials in the material table, 0 = per-cell arrays */
+    SDT_INFO_SHELL_TILES = 2,   /* 3D elastic CPML-shell tiles                                    */
+    SDT_INFO_DEVICE_BYTES = 3   /* device memory held by the handle's grid arrays                 */
+};
+int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,11 @@ struct El3Args {
     const T *C[21]; // c11 c12 c13 c14 c15 c16 c22 c23 c24 c25 c26 c33 c34 c35 c36 c44 c45 c46 c55 c56 c66
     const T *rho;
     const T *ig[6]; // 1/(1+gamma*dt): x y z xy xz yz
+    // material table (when the model has at most 256 distinct materials):
+    // mid[c] indexes rows of tab, row = c11..c66 (21), 1/(1+gamma*dt) x y z
+    // xy xz yz (6), rho, 4 unused
+    const uint8_t *mid;
+    const T *tab;
     const T *a, *b, *ik, *ah, *bh, *ikh;
     T *m[M_TOTAL];
     const uint8_t *flags;
@@ -86,6 +91,10 @@ struct El3Tile {
     int ax, ay, alane;   // an active cell of the tile and its lane: inactive lanes
                          // load from it, so they add no memory traffic
 };
+
+constexpr int kTabRow = 32;     // material-table row length (elements)
+constexpr int kTabIg = 21, kTabRho = 27;
+constexpr int kMaxMaterials = 256;
 
 // Interior (branch-free, z-marching) and CPML-shell kernels, el3_fast.hip.
 template <typename T>

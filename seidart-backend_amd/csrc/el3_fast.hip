@@ -57,6 +57,49 @@ __device__ __forceinline__ void load_stiff(Stiff<T> &K, const T *const *C, int64
 #pragma unroll
     for (int q = 0; q < 21; ++q) K.v[q] = ldnt(&C[q][c]);
 }
+// Material of cell c: 21 stiffness entries and the six attenuation
+// divisors, from the dense per-cell arrays or from the material table.  When
+// every lane of the wavefront holds the same material (the common case: a
+// wavefront spans one x row) the row is read with scalar loads through the
+// constant address space; otherwise each lane gathers its own row.
+template <typename T>
+using const_row = __attribute__((address_space(4))) const T *;
+
+template <typename T, bool TAB>
+__device__ __forceinline__ void load_material(Stiff<T> &K, T (&ig)[6], const El3Args<T> &A, int64_t c)
+{
+    if (TAB) {
+        const int m = A.mid[c];
+        const int m0 = __builtin_amdgcn_readfirstlane(m);
+        if (__all(m == m0)) {
+            const_row<T> row = (const_row<T>)(A.tab) + m0 * kTabRow;
+#pragma unroll
+            for (int q = 0; q < 21; ++q) K.v[q] = row[q];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ig[q] = row[kTabIg + q];
+        } else {
+            const T *row = (const T *)__builtin_assume_aligned(A.tab + m * kTabRow, kTabRow * sizeof(T));
+#pragma unroll
+            for (int q = 0; q < 21; ++q) K.v[q] = row[q];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ig[q] = row[kTabIg + q];
+        }
+    } else {
+        load_stiff(K, A.C, c);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ig[q] = ldnt(&A.ig[q][c]);
+    }
+}
+template <typename T, bool TAB>
+__device__ __forceinline__ T rho_at(const El3Args<T> &A, int64_t c)
+{
+    if (!TAB) return A.rho[c];
+    const int m = A.mid[c];
+    const int m0 = __builtin_amdgcn_readfirstlane(m);
+    if (__all(m == m0)) return ((const_row<T>)(A.tab))[m0 * kTabRow + kTabRho];
+    return A.tab[m * kTabRow + kTabRho];
+}
+
 template <typename T>
 __device__ __forceinline__ T supk(T s, const Stiff<T> &K, int i0, int i1, int i2, int i3, int i4, int i5, T dxx, T dyy,
                                   T dzz, T yz, T xz, T xy, T dt, T ig)
@@ -65,7 +108,7 @@ __device__ __forceinline__ T supk(T s, const Stiff<T> &K, int i0, int i1, int i2
 }
 
 // ------------------------------------------------------------------ interior
-template <typename T, int TY>
+template <typename T, int TY, bool TAB>
 __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, int kc)
 {
     const int i0 = A.box[0], j0 = A.box[2];
@@ -80,7 +123,6 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * A.ny;
     int64_t c = A.xoff + (int64_t)((act ? i : i0) - 1) + sy * ((act ? j : j0) - 1) + sz * (kb - 1);
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
-    const T *const *C = A.C;
     T q[3][5];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
@@ -106,25 +148,26 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
         // nests (cpmlfdtd.f08:1761-1992); without CPML the full and half
         // coefficient sets leave a derivative unchanged
         Stiff<T> K;
-        load_stiff(K, C, c);
+        T ig[6];
+        load_material<T, TAB>(K, ig, A, c);
         T sxx, syy, szz, sxy, sxz, syz;
         {   // L1: D+x, D-y, D-z
             const T yz = dym[2] + dzm[1], xz = dzm[0] + dxp[2], xy = dxp[1] + dym[0];
-            sxx = supk(ldnt(&A.s[0][c]), K, 0, 1, 2, 3, 4, 5, dxp[0], dym[1], dzm[2], yz, xz, xy, dt, ldnt(&A.ig[0][c]));
-            syy = supk(ldnt(&A.s[1][c]), K, 1, 6, 7, 8, 9, 10, dxp[0], dym[1], dzm[2], yz, xz, xy, dt, ldnt(&A.ig[1][c]));
-            szz = supk(ldnt(&A.s[2][c]), K, 2, 7, 11, 12, 13, 14, dxp[0], dym[1], dzm[2], yz, xz, xy, dt, ldnt(&A.ig[2][c]));
+            sxx = supk(ldnt(&A.s[0][c]), K, 0, 1, 2, 3, 4, 5, dxp[0], dym[1], dzm[2], yz, xz, xy, dt, ig[0]);
+            syy = supk(ldnt(&A.s[1][c]), K, 1, 6, 7, 8, 9, 10, dxp[0], dym[1], dzm[2], yz, xz, xy, dt, ig[1]);
+            szz = supk(ldnt(&A.s[2][c]), K, 2, 7, 11, 12, 13, 14, dxp[0], dym[1], dzm[2], yz, xz, xy, dt, ig[2]);
         }
         {   // L2: D-x, D+y, D-z
             const T yz = dzm[1] + dyp[2], xz = dzm[0] + dxm[2], xy = dxm[1] + dyp[0];
-            sxy = supk(ldnt(&A.s[3][c]), K, 5, 10, 14, 17, 19, 20, dxm[0], dyp[1], dzm[2], yz, xz, xy, dt, ldnt(&A.ig[3][c]));
+            sxy = supk(ldnt(&A.s[3][c]), K, 5, 10, 14, 17, 19, 20, dxm[0], dyp[1], dzm[2], yz, xz, xy, dt, ig[3]);
         }
         {   // L3: D-x, D-y, D+z
             const T yz = dzp[1] + dym[2], xz = dzp[0] + dxm[2], xy = dxm[1] + dym[0];
-            sxz = supk(ldnt(&A.s[4][c]), K, 4, 9, 13, 16, 18, 19, dxm[0], dym[1], dzp[2], yz, xz, xy, dt, ldnt(&A.ig[4][c]));
+            sxz = supk(ldnt(&A.s[4][c]), K, 4, 9, 13, 16, 18, 19, dxm[0], dym[1], dzp[2], yz, xz, xy, dt, ig[4]);
         }
         {   // L4: D+x, D+y, D+z
             const T yz = dzp[1] + dyp[2], xz = dzp[0] + dxp[2], xy = dxp[1] + dyp[0];
-            syz = supk(ldnt(&A.s[5][c]), K, 3, 8, 12, 15, 16, 17, dxp[0], dyp[1], dzp[2], yz, xz, xy, dt, ldnt(&A.ig[5][c]));
+            syz = supk(ldnt(&A.s[5][c]), K, 3, 8, 12, 15, 16, 17, dxp[0], dyp[1], dzp[2], yz, xz, xy, dt, ig[5]);
         }
         if (act) {
             stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); stnt(&A.s[2][c], szz);
@@ -142,7 +185,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
     block_max_to(&A.snorm[*A.d_it - 1], nrm);
 }
 
-template <typename T, bool CORR, int TY>
+template <typename T, bool CORR, int TY, bool TAB>
 __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A, int kc)
 {
     const int i0 = A.box[0], j0 = A.box[2];
@@ -166,15 +209,15 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
         qyz[s] = syz[c + (s - 2) * sz];
         qzz[s] = szz[c + (s - 2) * sz];
     }
-    qr[0] = A.rho[c - sz];
-    qr[1] = A.rho[c];
+    qr[0] = rho_at<T, TAB>(A, c - sz);
+    qr[1] = rho_at<T, TAB>(A, c);
     double nrm = 0.0;
     for (int k = kb; k <= ke; ++k, c += sz) {
         qxz[4] = sxz[c + 2 * sz];
         qyz[4] = syz[c + 2 * sz];
         qzz[4] = szz[c + 2 * sz];
-        qr[2] = A.rho[c + sz];
-        const T r = qr[1], rxm = A.rho[c - 1], rxp = A.rho[c + 1];
+        qr[2] = rho_at<T, TAB>(A, c + sz);
+        const T r = qr[1], rxm = rho_at<T, TAB>(A, c - 1), rxp = rho_at<T, TAB>(A, c + 1);
         const T sxx_m2 = sxx[c - 2], sxx_m1 = sxx[c - 1], sxx_0 = sxx[c], sxx_p1 = CORR ? sxx[c + 1] : T(0);
         const T sxy_ym2 = sxy[c - 2 * sy], sxy_ym1 = sxy[c - sy], sxy_0 = sxy[c], sxy_yp1 = CORR ? sxy[c + sy] : T(0);
         const T sxy_xm1 = CORR ? sxy[c - 1] : T(0), sxy_xp1 = sxy[c + 1], sxy_xp2 = sxy[c + 2];
@@ -237,7 +280,7 @@ __device__ __forceinline__ T cpml_upd(T d, const T *psi_p, T a, T b, T ik, T &ps
 
 // Memory variables and CPML coefficients of shell cells live in compact,
 // tile-major storage: slot = tile.base + (k - tile.k0) * 256 + lane.
-template <typename T>
+template <typename T, bool TAB>
 __global__ void __launch_bounds__(256) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
 {
     const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
@@ -253,7 +296,6 @@ __global__ void __launch_bounds__(256) el3_shell_stress_kernel(El3Args<T> A, con
     const int64_t yp1 = (min(jj + 1, ny) - jj) * sy, yp2 = (min(jj + 2, ny) - jj) * sy;
     const int64_t col = A.xoff + (int64_t)(i - 1) + sy * (jj - 1);
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
-    const T *const *C = A.C;
     T q[3][5];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
@@ -304,31 +346,32 @@ __global__ void __launch_bounds__(256) el3_shell_stress_kernel(El3Args<T> A, con
         T sxx = ldnt(&A.s[0][c]), syy = ldnt(&A.s[1][c]), szz = ldnt(&A.s[2][c]);
         T sxy = ldnt(&A.s[3][c]), sxz = ldnt(&A.s[4][c]), syz = ldnt(&A.s[5][c]);
         Stiff<T> K;
-        load_stiff(K, C, c);
+        T ig[6];
+        load_material<T, TAB>(K, ig, A, c);
         {
             const T dxx = d[0][K_XPH], dyy = d[1][K_YMF], dzz = d[2][K_ZMF];
             const T yz = d[1][K_ZMF] + d[2][K_YMF], xz = d[0][K_ZMF] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YMF];
-            const T n0 = supk(sxx, K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[0][c]));
-            const T n1 = supk(syy, K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[1][c]));
-            const T n2 = supk(szz, K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[2][c]));
+            const T n0 = supk(sxx, K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, ig[0]);
+            const T n1 = supk(syy, K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, ig[1]);
+            const T n2 = supk(szz, K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, ig[2]);
             if (L1) { sxx = n0; syy = n1; szz = n2; stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); stnt(&A.s[2][c], szz); }
         }
         {
             const T dxx = d[0][K_XMF], dyy = d[1][K_YPH], dzz = d[2][K_ZMH];
             const T yz = d[1][K_ZMH] + d[2][K_YPH], xz = d[0][K_ZMH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YPH];
-            const T n = supk(sxy, K, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[3][c]));
+            const T n = supk(sxy, K, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, ig[3]);
             if (L2) { sxy = n; stnt(&A.s[3][c], sxy); }
         }
         {
             const T dxx = d[0][K_XMF], dyy = d[1][K_YMF], dzz = d[2][K_ZPH];
             const T yz = d[1][K_ZPH] + d[2][K_YMF], xz = d[0][K_ZPH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YMF];
-            const T n = supk(sxz, K, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[4][c]));
+            const T n = supk(sxz, K, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, ig[4]);
             if (L3) { sxz = n; stnt(&A.s[4][c], sxz); }
         }
         {
             const T dxx = d[0][K_XPH], dyy = d[1][K_YPH], dzz = d[2][K_ZPH];
             const T yz = d[1][K_ZPH] + d[2][K_YPH], xz = d[0][K_ZPH] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YPH];
-            const T n = supk(syz, K, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, ldnt(&A.ig[5][c]));
+            const T n = supk(syz, K, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, ig[5]);
             if (L4) { syz = n; stnt(&A.s[5][c], syz); }
         }
         if (on) {
@@ -345,7 +388,7 @@ __global__ void __launch_bounds__(256) el3_shell_stress_kernel(El3Args<T> A, con
     block_max_to(&A.snorm[*A.d_it - 1], nrm);
 }
 
-template <typename T, bool CORR>
+template <typename T, bool CORR, bool TAB>
 __global__ void __launch_bounds__(256) el3_shell_velocity_kernel(El3Args<T> A, const El3Tile *tiles)
 {
     const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
@@ -370,15 +413,15 @@ __global__ void __launch_bounds__(256) el3_shell_velocity_kernel(El3Args<T> A, c
         const int64_t o = col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1);
         qxz[s] = sxz[o]; qyz[s] = syz[o]; qzz[s] = szz[o];
     }
-    qr[0] = A.rho[col + sz * (max(tl.k0 - 1, 1) - 1)];
-    qr[1] = A.rho[col + sz * (tl.k0 - 1)];
+    qr[0] = rho_at<T, TAB>(A, col + sz * (max(tl.k0 - 1, 1) - 1));
+    qr[1] = rho_at<T, TAB>(A, col + sz * (tl.k0 - 1));
     double nrm = 0.0;
     int64_t slot = tl.base + (on ? lane : tl.alane);
     for (int k = tl.k0; k <= tl.k1; ++k, slot += 256) {
         const int64_t c = col + sz * (k - 1);
         const int64_t zp1 = sz * (min(k + 1, nz) - k), zp2 = sz * (min(k + 2, nz) - k);
         qxz[4] = sxz[c + zp2]; qyz[4] = syz[c + zp2]; qzz[4] = szz[c + zp2];
-        qr[2] = A.rho[c + zp1];
+        qr[2] = rho_at<T, TAB>(A, c + zp1);
         const bool face = xface || k == 1 || k == nz;
         const bool Lx = on && !face && i >= 3 && j >= 3 && k >= 3;
         const bool Ly = on && !face && i <= nx - 2 && j <= ny - 2 && k >= 3;
@@ -396,7 +439,7 @@ __global__ void __launch_bounds__(256) el3_shell_velocity_kernel(El3Args<T> A, c
             d1 = cpml_upd(d1, &A.m[M_SXX_X][slot], ah, bh, ikh, p0);
             d2 = cpml_upd(d2, &A.m[M_SXY_Y][slot], ah, bh, ikh, p1);
             d3 = cpml_upd(d3, &A.m[M_SXZ_Z][slot], ah, bh, ikh, p2);
-            const T n = vx + (d1 / scalar_mean(r, A.rho[c + xm1], dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
+            const T n = vx + (d1 / scalar_mean(r, rho_at<T, TAB>(A, c + xm1), dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
             if (Lx) {
                 stnt(&A.m[M_SXX_X][slot], p0); stnt(&A.m[M_SXY_Y][slot], p1); stnt(&A.m[M_SXZ_Z][slot], p2);
                 vx = n;
@@ -409,7 +452,7 @@ __global__ void __launch_bounds__(256) el3_shell_velocity_kernel(El3Args<T> A, c
             d1 = cpml_upd(d1, &A.m[M_SXY_X][slot], ah, bh, ikh, p0);
             d2 = cpml_upd(d2, &A.m[M_SYY_Y][slot], ah, bh, ikh, p1);
             d3 = cpml_upd(d3, &A.m[M_SYZ_Z][slot], a, b, ik, p2);
-            const T n = vy + (d1 / scalar_mean(r, A.rho[c + xp1], dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
+            const T n = vy + (d1 / scalar_mean(r, rho_at<T, TAB>(A, c + xp1), dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
             if (Ly) {
                 stnt(&A.m[M_SXY_X][slot], p0); stnt(&A.m[M_SYY_Y][slot], p1); stnt(&A.m[M_SYZ_Z][slot], p2);
                 vy = n;
@@ -422,7 +465,7 @@ __global__ void __launch_bounds__(256) el3_shell_velocity_kernel(El3Args<T> A, c
             d1 = cpml_upd(d1, &A.m[M_SXZ_X][slot], ah, bh, ikh, p0);
             d2 = cpml_upd(d2, &A.m[M_SYZ_Y][slot], a, b, ik, p1);
             d3 = cpml_upd(d3, &A.m[M_SZZ_Z][slot], ah, bh, ikh, p2);
-            const T n = vz + (d1 / scalar_mean(r, A.rho[c + xp1], dc) + d2 / ryy + d3 / scalar_mean(r, qr[2], dc)) * dt;
+            const T n = vz + (d1 / scalar_mean(r, rho_at<T, TAB>(A, c + xp1), dc) + d2 / ryy + d3 / scalar_mean(r, qr[2], dc)) * dt;
             if (Lz) {
                 stnt(&A.m[M_SXZ_X][slot], p0); stnt(&A.m[M_SYZ_Y][slot], p1); stnt(&A.m[M_SZZ_Z][slot], p2);
                 vz = n;
@@ -460,53 +503,65 @@ static bool box_empty(const Device &d)
     return d.box[1] < d.box[0] || d.box[3] < d.box[2] || d.box[5] < d.box[4];
 }
 
-template <typename T, int TY>
+template <typename T, int TY, bool TAB>
 static void int_stress(Device &d, const El3Args<T> &A, int kc)
 {
-    hipLaunchKernelGGL((el3_int_stress_kernel<T, TY>), interior_grid(d, kc, TY), dim3(64, TY, 1), 0, d.stream, A, kc);
+    hipLaunchKernelGGL((el3_int_stress_kernel<T, TY, TAB>), interior_grid(d, kc, TY), dim3(64, TY, 1), 0, d.stream,
+                       A, kc);
 }
 
-template <typename T, bool CORR, int TY>
+template <typename T, bool CORR, int TY, bool TAB>
 static void int_velocity(Device &d, const El3Args<T> &A, int kc)
 {
-    hipLaunchKernelGGL((el3_int_velocity_kernel<T, CORR, TY>), interior_grid(d, kc, TY), dim3(64, TY, 1), 0,
+    hipLaunchKernelGGL((el3_int_velocity_kernel<T, CORR, TY, TAB>), interior_grid(d, kc, TY), dim3(64, TY, 1), 0,
                        d.stream, A, kc);
+}
+
+template <typename T, bool TAB>
+static void fast_stress(Device &d, const El3Args<T> &A)
+{
+    const int kc = el3_kc(d), ty = el3_ty(d);
+    if (!box_empty(d)) {
+        if (ty == 16) int_stress<T, 16, TAB>(d, A, kc);
+        else if (ty == 8) int_stress<T, 8, TAB>(d, A, kc);
+        else int_stress<T, 4, TAB>(d, A, kc);
+    }
+    if (d.ntiles > 0)
+        hipLaunchKernelGGL((el3_shell_stress_kernel<T, TAB>), dim3(d.ntiles), dim3(256), 0, d.stream, A,
+                           (const El3Tile *)d.tiles);
+}
+
+template <typename T, bool CORR, bool TAB>
+static void fast_velocity(Device &d, const El3Args<T> &A)
+{
+    const int kc = el3_kc(d), ty = el3_ty(d);
+    if (!box_empty(d)) {
+        if (ty == 16) int_velocity<T, CORR, 16, TAB>(d, A, kc);
+        else if (ty == 8) int_velocity<T, CORR, 8, TAB>(d, A, kc);
+        else int_velocity<T, CORR, 4, TAB>(d, A, kc);
+    }
+    if (d.ntiles > 0)
+        hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB>), dim3(d.ntiles), dim3(256), 0, d.stream, A,
+                           (const El3Tile *)d.tiles);
 }
 
 template <typename T>
 void el3_fast_stress(Device &d, const El3Args<T> &A)
 {
-    const int kc = el3_kc(d), ty = el3_ty(d);
-    if (!box_empty(d)) {
-        if (ty == 16) int_stress<T, 16>(d, A, kc);
-        else if (ty == 8) int_stress<T, 8>(d, A, kc);
-        else int_stress<T, 4>(d, A, kc);
-    }
-    if (d.ntiles > 0)
-        hipLaunchKernelGGL(el3_shell_stress_kernel<T>, dim3(d.ntiles), dim3(256), 0, d.stream, A,
-                           (const El3Tile *)d.tiles);
+    if (d.mat_table) fast_stress<T, true>(d, A);
+    else fast_stress<T, false>(d, A);
 }
 
 template <typename T>
 void el3_fast_velocity(Device &d, const El3Args<T> &A)
 {
-    const int kc = el3_kc(d), ty = el3_ty(d);
     const bool corr = d.stencil == SDT_STENCIL_CORRECTED;
-    if (!box_empty(d)) {
-        if (corr) {
-            if (ty == 16) int_velocity<T, true, 16>(d, A, kc);
-            else if (ty == 8) int_velocity<T, true, 8>(d, A, kc);
-            else int_velocity<T, true, 4>(d, A, kc);
-        } else {
-            if (ty == 16) int_velocity<T, false, 16>(d, A, kc);
-            else if (ty == 8) int_velocity<T, false, 8>(d, A, kc);
-            else int_velocity<T, false, 4>(d, A, kc);
-        }
-    }
-    if (d.ntiles > 0) {
-        const El3Tile *t = (const El3Tile *)d.tiles;
-        if (corr) hipLaunchKernelGGL((el3_shell_velocity_kernel<T, true>), dim3(d.ntiles), dim3(256), 0, d.stream, A, t);
-        else hipLaunchKernelGGL((el3_shell_velocity_kernel<T, false>), dim3(d.ntiles), dim3(256), 0, d.stream, A, t);
+    if (d.mat_table) {
+        if (corr) fast_velocity<T, true, true>(d, A);
+        else fast_velocity<T, false, true>(d, A);
+    } else {
+        if (corr) fast_velocity<T, true, false>(d, A);
+        else fast_velocity<T, false, false>(d, A);
     }
 }
 

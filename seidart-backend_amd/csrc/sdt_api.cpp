@@ -155,6 +155,7 @@ static void release(sdt_handle *h)
     free_grids(d);   // fields, memory variables, materials, CPML coefficients
     drop(d.tile_flags);
     drop(d.tiles);
+    drop(d.mid);   // d.tab is one of the grid allocations freed above
     drop(d.series);
     drop(d.norms);
     drop(d.d_it);
@@ -504,12 +505,29 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
     case SDT_OPT_GRAPH:
         h->d.graphs = value != 0;
         return SDT_OK;
+    case SDT_OPT_MATERIAL_TABLE:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "material layout is fixed once the first step has run");
+        if (value != 0 && value != 1) return fail(h, SDT_ERR_ARG, "material table option must be 0 or 1");
+        h->d.mat_table_opt = (int)value;
+        return SDT_OK;
     case SDT_OPT_STAGGER:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "allocation layout is fixed once the first step has run");
         h->d.stagger = value != 0;
         return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown option");
+}
+
+int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
+{
+    if (!h || !value) return fail(h, SDT_ERR_ARG, "null argument");
+    if (!h->prepared) return fail(h, SDT_ERR_STATE, "layout is chosen at the first step");
+    switch (key) {
+    case SDT_INFO_MATERIALS: *value = h->d.mat_table ? h->d.nmaterials : 0; return SDT_OK;
+    case SDT_INFO_SHELL_TILES: *value = h->d.ntiles; return SDT_OK;
+    case SDT_INFO_DEVICE_BYTES: *value = (int64_t)h->d.grid_bytes; return SDT_OK;
+    }
+    return fail(h, SDT_ERR_ARG, "unknown info key");
 }
 
 int32_t sdt_synchronize(sdt_handle *h)

@@ -100,6 +100,7 @@ int alloc_grid(Device &d, void **dev, size_t bytes)
     void *base = nullptr;
     if (hipMalloc(&base, bytes + shift) != hipSuccess) return SDT_ERR_HIP;
     d.bases[d.nbases++] = base;
+    d.grid_bytes += bytes + shift;
     *dev = (char *)base + shift;
     return SDT_OK;
 }

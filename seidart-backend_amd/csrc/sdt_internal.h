@@ -94,12 +94,18 @@ struct Device {
 
     TailArgs tail{};
     int zchunk = 0;                    // SDT_OPT_ZCHUNK
+    int mat_table_opt = 1;             // SDT_OPT_MATERIAL_TABLE
+    bool mat_table = false;            // per-cell material ids + table in use
+    int nmaterials = 0;                // distinct materials found at setup
+    uint8_t *mid = nullptr;            // material id per (padded) cell
+    void *tab = nullptr;               // material rows
     int tile_rows = 0;                 // SDT_OPT_TILE_ROWS
     bool graphs = true;                // SDT_OPT_GRAPH
     bool stagger = true;               // SDT_OPT_STAGGER
     int nalloc = 0;                    // allocations made (stagger index)
     void *bases[256] = {};             // hipMalloc'd base of every array
     int nbases = 0;
+    size_t grid_bytes = 0;             // bytes allocated through alloc_grid
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     bool timing = false;

@@ -63,10 +63,11 @@ EXPORTS = [
     "sdt_version", "sdt_device_count", "sdt_last_error", "sdt_create", "sdt_destroy",
     "sdt_set_array", "sdt_set_source", "sdt_set_receivers", "sdt_run", "sdt_steps_done",
     "sdt_get_field", "sdt_set_field", "sdt_get_receivers", "sdt_get_norms", "sdt_set_timing",
-    "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option",
+    "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option", "sdt_get_info",
 ]
 
-OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS = 1, 2, 3, 4
+OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE = 1, 2, 3, 4, 5
+INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES = 1, 2, 3
 
 
 class SdtError(RuntimeError):
@@ -110,6 +111,7 @@ def _load():
         "sdt_get_kernel_times": ([P, D, D, C.POINTER(I64)], I32),
         "sdt_synchronize": ([P], I32),
         "sdt_set_option": ([P, I32, I64], I32),
+        "sdt_get_info": ([P, I32, C.POINTER(I64)], I32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

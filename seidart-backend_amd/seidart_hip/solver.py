@@ -179,10 +179,24 @@ class Solver:
     # ---- tuning -----------------------------------------------------------------
     def set_option(self, key, value):
         """key: 'zchunk' (planes per z-marching workgroup, 0 auto), 'graph'
-        (0/1), 'stagger' (0/1) or 'tile_rows' (0 auto, 4, 8, 16)."""
+        (0/1), 'stagger' (0/1), 'tile_rows' (0 auto, 4, 8, 16) or
+        'material_table' (1 = one-byte material ids + table when the 3D
+        elastic model has <= 256 distinct materials, 0 = per-cell arrays)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
-             "tile_rows": L.OPT_TILE_ROWS}[key]
+             "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
+
+    def info(self, key):
+        """'materials' (distinct materials in the table, 0 = per-cell arrays),
+        'shell_tiles' or 'device_bytes'; valid after the first run()."""
+        k = {"materials": L.INFO_MATERIALS, "shell_tiles": L.INFO_SHELL_TILES,
+             "device_bytes": L.INFO_DEVICE_BYTES}[key]
+        v = C.c_int64(0)
+        L.check(L.lib.sdt_get_info(self._h, k, C.byref(v)), self._h)
+        return v.value
+
+    def materials(self):
+        return self.info("materials")
 
     # ---- kernel timing --------------------------------------------------------
     def set_timing(self, on=True):

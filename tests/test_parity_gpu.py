@@ -64,7 +64,8 @@ def test_elastic2_stress_source_and_attenuation():
     _check(g, o, TOL64)
 
 
-@pytest.mark.parametrize("options", [{}, {"zchunk": 5}, {"zchunk": 1}, {"zchunk": 7, "stagger": 0}])
+@pytest.mark.parametrize("options", [{}, {"zchunk": 5}, {"zchunk": 1}, {"zchunk": 7, "stagger": 0},
+                                     {"material_table": 0}, {"material_table": 0, "zchunk": 3}])
 def test_elastic3_small_fp64(options):
     """z-marching chunk heights that put chunk seams inside and outside the
     CPML slabs and the interior box."""
@@ -96,6 +97,48 @@ def test_elastic3_interior_and_cpml_paths_agree():
     cfg["arrays"]["acoef_cpml"] = a
     r2, _, f2 = run_config(cfg, fields=("vx", "szz"))
     assert rel_l2(r2, r1) < 1e-14 and rel_l2(f2["vx"], f1["vx"]) < 1e-14
+
+
+def test_elastic3_material_table_equals_per_cell_arrays():
+    """The material-table layout reads the same FP64 values as the per-cell
+    arrays; the two runs agree to rounding (the compiler may contract the
+    stiffness products into FMAs differently when the row comes from scalar
+    registers)."""
+    from seidart_hip import models
+    from seidart_hip.solver import run_config
+    cfg = models.elastic3(nx=36, ny=30, nz=34, cpml=6, steps=120, src=(18, 15, 10), nrec=8,
+                          density_method="geometric")
+    fields = ("vx", "vy", "vz", "sxx", "syz")
+    r1, n1, f1 = run_config(cfg, fields=fields, options={"material_table": 1})
+    r0, n0, f0 = run_config(cfg, fields=fields, options={"material_table": 0})
+    assert rel_l2(r1, r0) < 1e-13 and rel_l2(n1[0], n0[0]) < 1e-13 and rel_l2(n1[1], n0[1]) < 1e-13
+    for k in fields:
+        assert rel_l2(f1[k], f0[k]) < 1e-13, k
+
+
+def test_elastic3_material_count_reported():
+    from seidart_hip import models
+    from seidart_hip.solver import Solver
+    cfg = models.elastic3(nx=24, ny=22, nz=20, cpml=4, steps=4, src=(12, 11, 6), nrec=2, nlayers=5)
+    counts = []
+    for opt in (1, 0):
+        with Solver("elastic3", cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], 4, cfg["src"]) as s:
+            s.set_option("material_table", opt)
+            s.set_arrays(cfg["arrays"])
+            s.run(1)
+            counts.append(s.materials())
+    assert counts == [5, 0]
+
+
+def test_elastic3_heterogeneous_model_uses_per_cell_arrays():
+    """More than 256 distinct materials (per-cell density jitter): the
+    per-cell layout is chosen and parity holds."""
+    from seidart_hip import models
+    cfg = models.elastic3(nx=30, ny=28, nz=26, cpml=5, steps=100, src=(15, 14, 9), nrec=8)
+    rng = np.random.default_rng(7)
+    cfg["arrays"]["rho"] = cfg["arrays"]["rho"] * (1.0 + 0.05 * rng.random(cfg["shape"]))
+    g, o = _run_both(cfg, ("vx", "vz", "sxy"))
+    _check(g, o, TOL64, stress_norm=True)
 
 
 def test_elastic3_stress_source_harmonic():
