@@ -37,13 +37,7 @@ static El3Args<T> el3_args(Device &d)
     for (int q = 0; q < 6; ++q) A.ig[q] = (const T *)d.mat[22 + q];
     A.mid = d.mid;
     A.tab = (const T *)d.tab;
-    A.a = (const T *)d.cpml[0];
-    A.b = (const T *)d.cpml[1];
-    A.ik = (const T *)d.cpml[2];
-    A.ah = (const T *)d.cpml[3];
-    A.bh = (const T *)d.cpml[4];
-    A.ikh = (const T *)d.cpml[5];
-    for (int q = 0; q < M_TOTAL; ++q) A.m[q] = (T *)d.mem[q];
+    A.shell = (T *)d.shell;
     A.flags = d.tile_flags;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
@@ -256,22 +250,23 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         if (kind == 1) return j < b[2] || j > b[3];
         return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
     };
+    // a tile covers kShellLanes cells of a plane: (1 << txl) x (kShellLanes >> txl)
     auto add = [&](int x0, int y0, int k0, int k1, int kind, int txl) {
-        const int tw = 1 << txl, th = 256 >> txl;
-        for (int lane = 0; lane < 256; ++lane) {
+        const int tw = 1 << txl;
+        for (int lane = 0; lane < kShellLanes; ++lane) {
             const int i = x0 + (lane & (tw - 1)), j = y0 + (lane >> txl);
             if (region(kind, i, j)) {
                 tiles.push_back(El3Tile{x0, y0, k0, k1, kind, txl, base, i, j, lane});
-                base += (int64_t)(k1 - k0 + 1) * 256;
+                base += (int64_t)(k1 - k0 + 1) * kShellLanes;
                 return;
             }
         }
-        (void)th;
     };
+    const int rows64 = kShellLanes >> 6, rows16 = kShellLanes >> 4;   // tile heights of 64- and 16-wide tiles
     auto zslab = [&](int ka, int kb) {
         for (int k0 = ka; k0 <= kb; k0 += kc) {
             const int k1 = k0 + kc - 1 < kb ? k0 + kc - 1 : kb;
-            for (int y0 = 1; y0 <= d.ny; y0 += 4)
+            for (int y0 = 1; y0 <= d.ny; y0 += rows64)
                 for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 0, 6);
         }
     };
@@ -287,13 +282,13 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         zslab(1, b[4] - 1);
         for (int k0 = b[4]; k0 <= b[5]; k0 += kc) {
             const int k1 = k0 + kc - 1 < b[5] ? k0 + kc - 1 : b[5];
-            for (int y0 = 1; y0 < b[2]; y0 += 4)
+            for (int y0 = 1; y0 < b[2]; y0 += rows64)
                 for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 6);
-            for (int y0 = b[2]; y0 <= b[3]; y0 += 16) {
+            for (int y0 = b[2]; y0 <= b[3]; y0 += rows16) {
                 xstrip(1, b[0] - 1, y0, k0, k1);
                 xstrip(b[1] + 1, d.nx, y0, k0, k1);
             }
-            for (int y0 = b[3] + 1; y0 <= d.ny; y0 += 4)
+            for (int y0 = b[3] + 1; y0 <= d.ny; y0 += rows64)
                 for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 6);
         }
         zslab(b[5] + 1, d.nz);
@@ -304,34 +299,34 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
     if (hipMalloc(&d.tiles, tiles.size() * sizeof(El3Tile)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(d.tiles, tiles.data(), tiles.size() * sizeof(El3Tile), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
-    // compact memory variables (zero) and CPML coefficients (gathered)
-    d.nmem = M_TOTAL;
-    for (int q = 0; q < d.nmem; ++q) {
-        if (q == 0 * K_NKIND + K_ZMH) continue;   // dead recursion, never touched
-        int rc = alloc_zeroed(d, &d.mem[q], d.nslots);
-        if (rc) return rc;
-    }
+    // Shell storage, blocked by tile-plane: the S_NQ streams of one tile-plane
+    // (memory variables, then a, b, 1/kappa, a_half, b_half, 1/kappa_half)
+    // are kShellLanes-element runs of one contiguous block, so a workgroup's
+    // accesses of a plane fall in one block (one TLB entry, open DRAM rows).
+    const int64_t nblk = d.nslots / kShellLanes;
+    std::vector<double> blk((size_t)(nblk * S_NQ * kShellLanes), 0.0);
     const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
-    std::vector<double> tmp((size_t)d.nslots);
     for (int q = 0; q < 6; ++q) {
         const double *src = in[cids[q]];
         const bool inverse = (q == 2 || q == 5);
         const double dflt = inverse ? 1.0 : 0.0;
         for (const El3Tile &t : tiles) {
-            const int tw = 1 << t.txl, th = 256 >> t.txl;
-            for (int k = t.k0; k <= t.k1; ++k)
+            const int tw = 1 << t.txl, th = kShellLanes >> t.txl;
+            for (int k = t.k0; k <= t.k1; ++k) {
+                const int64_t tp = t.base / kShellLanes + (k - t.k0);
+                double *row = &blk[(size_t)((tp * S_NQ + S_CF + q) * kShellLanes)];
                 for (int ty = 0; ty < th; ++ty)
                     for (int tx = 0; tx < tw; ++tx) {
                         const int i = t.x0 + tx, j = t.y0 + ty;
                         double v = dflt;
                         if (src && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny)
                             v = src[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))];
-                        tmp[(size_t)(t.base + (int64_t)(k - t.k0) * 256 + ty * tw + tx)] = inverse ? 1.0 / v : v;
+                        row[ty * tw + tx] = inverse ? 1.0 / v : v;
                     }
+            }
         }
-        int rc = upload_converted(d, tmp.data(), &d.cpml[q], d.nslots);
-        if (rc) return rc;
     }
+    if (upload_converted(d, blk.data(), &d.shell, (int64_t)blk.size()) != SDT_OK) return SDT_ERR_HIP;
     return SDT_OK;
 }
 

@@ -11,6 +11,9 @@ namespace sdt {
 enum { K_XPH = 0, K_XMF, K_YMF, K_YPH, K_ZMF, K_ZMH, K_ZPH, K_NKIND };
 // velocity-side slots follow: 21 + q
 enum { M_SXX_X = 21, M_SXY_Y, M_SXZ_Z, M_SXY_X, M_SYY_Y, M_SYZ_Z, M_SXZ_X, M_SYZ_Y, M_SZZ_Z, M_TOTAL };
+// streams of a shell tile-plane block: the M_TOTAL memory variables, then the
+// CPML coefficients a, b, 1/kappa, a_half, b_half, 1/kappa_half
+enum { S_CF = M_TOTAL, S_NQ = M_TOTAL + 6 };
 
 template <typename T>
 struct El3Args {
@@ -24,8 +27,7 @@ struct El3Args {
     // xy xz yz (6), rho, 4 unused
     const uint8_t *mid;
     const T *tab;
-    const T *a, *b, *ik, *ah, *bh, *ikh;
-    T *m[M_TOTAL];
+    T *shell;       // CPML-shell storage: [tile-plane][S_NQ][kShellLanes]
     const uint8_t *flags;
     int box[6];
     int nx, ny, nz, tiles_x, tiles_y, pitch;
@@ -82,8 +84,10 @@ __device__ __forceinline__ T tplus5(T fm1, T f0, T fp1, T fp2, int corr)
 }
 
 // One z-marching tile of the CPML shell: x-y footprint (1 << txl) x
-// (256 >> txl) cells from (x0, y0), planes k0..k1, region kind (0 z slab,
-// 1 y strip, 2 x strip) and the first compact slot of its memory variables.
+// (kShellLanes >> txl) cells from (x0, y0), planes k0..k1, region kind (0 z
+// slab, 1 y strip, 2 x strip) and the first compact slot of its memory
+// variables (slot = base + (k - k0) * kShellLanes + lane).
+constexpr int kShellLanes = 64;
 struct El3Tile {
     int x0, y0, k0, k1;
     int kind, txl;

@@ -9,12 +9,13 @@
 //    load of a plane is unconditional so the compiler issues them together.
 //
 //  * shell kernels over the cells outside the box (the CPML slabs and the
-//    Dirichlet faces), as a list of z-marching tiles: 64 x 4 tiles over the
-//    z slabs and the y strips, 16 x 16 tiles over the (12-cell) x strips.
-//    Their memory variables and CPML coefficients are stored compactly in
-//    tile order, so those streams are contiguous per wavefront.  Neighbour
-//    indices are clamped into the grid so every load is unconditional; nest
-//    ranges only mask the stores.
+//    Dirichlet faces), as a list of z-marching tiles of 64 cells per plane:
+//    64 x 1 over the z slabs and the y strips, 16 x 4 over the (12-cell) x
+//    strips, three cooperating wavefronts per tile.  Their memory variables
+//    and CPML coefficients are stored compactly in tile order, so those
+//    streams are contiguous per wavefront.  Neighbour indices are clamped
+//    into the grid so every load is unconditional; nest ranges only mask the
+//    stores.
 //
 // Both write disjoint cells of the same arrays and fold into the same per-step
 // norm slots, so their order within a step does not matter.
@@ -260,15 +261,55 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 }
 
 // --------------------------------------------------------------------- shell
-// Lane of a shell tile: active when inside the grid and in the tile's region
-// (z slab: planes outside the box; y strip: rows outside the box, planes
-// inside; x strip: columns outside, rows and planes inside).
+// A shell tile is kShellLanes (64) cells of a plane (64 x 1 over the z slabs
+// and y strips, 16 x 4 over the x strips) marched over planes k0..k1.  Its
+// workgroup is three wavefronts over the same 64 cells, so that no lane has to
+// hold a whole cell's state:
+//   stress pass: wavefront m computes the seven CPML-corrected derivatives of
+//     velocity component m (updating its memory variables) into LDS; after a
+//     barrier each wavefront updates two of the six stresses from them;
+//   velocity pass: wavefront m updates velocity component m (its three
+//     memory variables); the three meet in LDS only for the norm.
+// Lanes outside the grid or the tile's region load from an active cell of the
+// tile (no extra traffic) and store nothing.
 __device__ __forceinline__ bool shell_lane(const El3Tile &t, const int (&b)[6], int i, int j, int nx, int ny)
 {
     if (i < 1 || i > nx || j > ny) return false;
     if (t.kind == 0) return true;
     if (t.kind == 1) return j < b[2] || j > b[3];
     return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
+}
+
+struct ShellCol {
+    bool on;
+    int i, j;                 // cell (i, j) of the lane (the anchor cell when !on)
+    int64_t col;              // element of (i, j, 1)
+    int64_t xm2, xm1, xp1, xp2, ym2, ym1, yp1, yp2;   // neighbour offsets clamped into the grid
+};
+
+__device__ __forceinline__ ShellCol shell_col(const El3Tile &tl, const int (&box)[6], int lane, int nx, int ny,
+                                              int64_t sy, int64_t xoff)
+{
+    ShellCol L;
+    const int tx = lane & ((1 << tl.txl) - 1), ty = lane >> tl.txl;
+    const int i0 = tl.x0 + tx, j0 = tl.y0 + ty;
+    L.on = shell_lane(tl, box, i0, j0, nx, ny);
+    L.i = L.on ? i0 : tl.ax;
+    L.j = L.on ? j0 : tl.ay;
+    const int i = L.i, j = L.j;
+    L.xm2 = max(i - 2, 1) - i; L.xm1 = max(i - 1, 1) - i; L.xp1 = min(i + 1, nx) - i; L.xp2 = min(i + 2, nx) - i;
+    L.ym2 = (max(j - 2, 1) - j) * sy; L.ym1 = (max(j - 1, 1) - j) * sy;
+    L.yp1 = (min(j + 1, ny) - j) * sy; L.yp2 = (min(j + 2, ny) - j) * sy;
+    L.col = xoff + (int64_t)(i - 1) + sy * (j - 1);
+    return L;
+}
+
+// First element of a lane's streams in the tile's first plane; the next
+// plane's block follows after S_NQ * kShellLanes elements.
+template <typename T>
+__device__ __forceinline__ T *shell_block(const El3Args<T> &A, const El3Tile &tl, int lane)
+{
+    return A.shell + (tl.base / kShellLanes) * (int64_t)(S_NQ * kShellLanes) + lane;
 }
 
 template <typename T>
@@ -278,210 +319,322 @@ __device__ __forceinline__ T cpml_upd(T d, const T *psi_p, T a, T b, T ik, T &ps
     return d * ik + psi_out;
 }
 
-// Memory variables and CPML coefficients of shell cells live in compact,
-// tile-major storage: slot = tile.base + (k - tile.k0) * 256 + lane.
-template <typename T, bool TAB>
-__global__ void __launch_bounds__(256) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
+// Stress pass, phase 1 for velocity component M: the four stress nests
+// (cpmlfdtd.f08:1761-1992) use seven (direction, stencil, coefficient set)
+// derivatives per component; each memory variable is updated once, on the
+// union of the ranges of the nests that read it.
+template <typename T, int M>
+__device__ __forceinline__ void shell_derivs(const El3Args<T> &A, const ShellCol &L, T (&q)[5], int64_t c,
+                                             int64_t zp2, T *blk, bool L1, bool L2, bool L3, bool L4,
+                                             const T (&cf)[6], T *D, int lane)
 {
+    const T *v = A.v[M];
+    q[4] = v[c + zp2];
+    const T v0 = q[2];
+    const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2];
+    const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
+    const T dxp = dp4(v[c + L.xm1], v0, v[c + L.xp1], v[c + L.xp2], rx);
+    const T dxm = dm4(v[c + L.xm2], v[c + L.xm1], v0, v[c + L.xp1], rx);
+    const T dyp = dp4(v[c + L.ym1], v0, v[c + L.yp1], v[c + L.yp2], ry);
+    const T dym = dm4(v[c + L.ym2], v[c + L.ym1], v0, v[c + L.yp1], ry);
+    const T dzp = dp4(q[1], v0, q[3], q[4], rz);
+    const T dzm = dm4(q[0], q[1], v0, q[3], rz);
+    T *mm = blk + M * K_NKIND * kShellLanes;   // this component's seven streams
+    T *Dm = D + M * K_NKIND * kShellLanes + lane;
+    T p0, p1, p2, p3, p4, p5 = T(0), p6;
+    Dm[K_XPH * kShellLanes] = cpml_upd(dxp, &mm[K_XPH * kShellLanes], ah, bh, ikh, p0);
+    Dm[K_XMF * kShellLanes] = cpml_upd(dxm, &mm[K_XMF * kShellLanes], a, b, ik, p1);
+    Dm[K_YMF * kShellLanes] = cpml_upd(dym, &mm[K_YMF * kShellLanes], a, b, ik, p2);
+    Dm[K_YPH * kShellLanes] = cpml_upd(dyp, &mm[K_YPH * kShellLanes], ah, bh, ikh, p3);
+    Dm[K_ZMF * kShellLanes] = cpml_upd(dzm, &mm[K_ZMF * kShellLanes], a, b, ik, p4);
+    // vx's z derivative enters sigma_xy uncorrected (cpmlfdtd.f08:1858-1865)
+    Dm[K_ZMH * kShellLanes] = M == 0 ? dzm : cpml_upd(dzm, &mm[K_ZMH * kShellLanes], ah, bh, ikh, p5);
+    Dm[K_ZPH * kShellLanes] = cpml_upd(dzp, &mm[K_ZPH * kShellLanes], ah, bh, ikh, p6);
+    if (L1 || L4) stnt(&mm[K_XPH * kShellLanes], p0);
+    if (L2 || L3) stnt(&mm[K_XMF * kShellLanes], p1);
+    if (L1 || L3) stnt(&mm[K_YMF * kShellLanes], p2);
+    if (L2 || L4) stnt(&mm[K_YPH * kShellLanes], p3);
+    if (L1) stnt(&mm[K_ZMF * kShellLanes], p4);
+    if (M != 0 && L2) stnt(&mm[K_ZMH * kShellLanes], p5);
+    if (L3 || L4) stnt(&mm[K_ZPH * kShellLanes], p6);
+}
+
+// Row accessors of the stiffness entries (0..20) and attenuation divisors
+// (21..26) of one cell.
+template <typename T>
+struct ScalarRow {   // wave-uniform table row, scalar loads
+    const_row<T> p;
+    __device__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct LaneRow {     // per-lane table row
+    const T *p;
+    __device__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct DenseRow {    // per-cell arrays (cached loads: the three wavefronts share entries)
+    const El3Args<T> *A;
+    int64_t c;
+    __device__ T operator[](int q) const { return q < 21 ? A->C[q][c] : A->ig[q - 21][c]; }
+};
+
+template <typename T, typename R>
+__device__ __forceinline__ T supr(T s, const R &K, int i0, int i1, int i2, int i3, int i4, int i5, T dxx, T dyy,
+                                  T dzz, T yz, T xz, T xy, T dt, int ig)
+{
+    return (s + (K[i0] * dxx + K[i1] * dyy + K[i2] * dzz + K[i3] * yz + K[i4] * xz + K[i5] * xy) * dt) * K[ig];
+}
+
+// Stress pass, phase 2 for wavefront W: W = 0 sigma_xx, sigma_yy (L1);
+// W = 1 sigma_zz (L1), sigma_xy (L2); W = 2 sigma_xz (L3), sigma_yz (L4).
+// Returns this wavefront's share of the stress norm's sum of squares.
+template <typename T, int W, typename R>
+__device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K, int64_t c, const T *D, int lane,
+                                                 bool L1, bool L2, bool L3, bool L4)
+{
+    auto d = [&](int m, int kind) { return D[(m * K_NKIND + kind) * kShellLanes + lane]; };
+    const T dt = A.dt;
+    if (W == 0 || W == 1) {
+        // L1 derivatives: D+x half, D-y full, D-z full
+        const T dxx = d(0, K_XPH), dyy = d(1, K_YMF), dzz = d(2, K_ZMF);
+        const T yz = d(1, K_ZMF) + d(2, K_YMF), xz = d(0, K_ZMF) + d(2, K_XPH), xy = d(1, K_XPH) + d(0, K_YMF);
+        if (W == 0) {
+            T sxx = ldnt(&A.s[0][c]), syy = ldnt(&A.s[1][c]);
+            const T n0 = supr(sxx, K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 0);
+            const T n1 = supr(syy, K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 1);
+            if (L1) { sxx = n0; syy = n1; stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); }
+            const double a0 = sxx, a1 = syy;
+            return a0 * a0 + a1 * a1;
+        }
+        T szz = ldnt(&A.s[2][c]), sxy = ldnt(&A.s[3][c]);
+        const T n2 = supr(szz, K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 2);
+        if (L1) { szz = n2; stnt(&A.s[2][c], szz); }
+        {   // L2 derivatives: D-x full, D+y half, D-z half (vx's uncorrected)
+            const T ex = d(0, K_XMF), ey = d(1, K_YPH), ez = d(2, K_ZMH);
+            const T eyz = d(1, K_ZMH) + d(2, K_YPH), exz = d(0, K_ZMH) + d(2, K_XMF), exy = d(1, K_XMF) + d(0, K_YPH);
+            const T n3 = supr(sxy, K, 5, 10, 14, 17, 19, 20, ex, ey, ez, eyz, exz, exy, dt, kTabIg + 3);
+            if (L2) { sxy = n3; stnt(&A.s[3][c], sxy); }
+        }
+        const double a2 = szz, a3 = sxy;
+        return a2 * a2 + 2.0 * (a3 * a3);
+    }
+    T sxz = ldnt(&A.s[4][c]), syz = ldnt(&A.s[5][c]);
+    {   // L3 derivatives: D-x full, D-y full, D+z half
+        const T dxx = d(0, K_XMF), dyy = d(1, K_YMF), dzz = d(2, K_ZPH);
+        const T yz = d(1, K_ZPH) + d(2, K_YMF), xz = d(0, K_ZPH) + d(2, K_XMF), xy = d(1, K_XMF) + d(0, K_YMF);
+        const T n4 = supr(sxz, K, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 4);
+        if (L3) { sxz = n4; stnt(&A.s[4][c], sxz); }
+    }
+    {   // L4 derivatives: D+x half, D+y half, D+z half
+        const T dxx = d(0, K_XPH), dyy = d(1, K_YPH), dzz = d(2, K_ZPH);
+        const T yz = d(1, K_ZPH) + d(2, K_YPH), xz = d(0, K_ZPH) + d(2, K_XPH), xy = d(1, K_XPH) + d(0, K_YPH);
+        const T n5 = supr(syz, K, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 5);
+        if (L4) { syz = n5; stnt(&A.s[5][c], syz); }
+    }
+    const double a4 = syz, a5 = sxz;
+    return 2.0 * (a4 * a4 + a5 * a5);
+}
+
+template <typename T, bool TAB, int W>
+__device__ __forceinline__ double shell_stresses_at(const El3Args<T> &A, int64_t c, const T *D, int lane, bool L1,
+                                                    bool L2, bool L3, bool L4)
+{
+    if (TAB) {
+        const int m = A.mid[c];
+        const int m0 = __builtin_amdgcn_readfirstlane(m);
+        if (__all(m == m0)) {
+            const ScalarRow<T> K{(const_row<T>)(A.tab) + m0 * kTabRow};
+            return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+        }
+        const LaneRow<T> K{A.tab + m * kTabRow};
+        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+    }
+    const DenseRow<T> K{&A, c};
+    return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+}
+
+// The stress kernel runs its two phases as a pipeline over the planes: in one
+// barrier interval a wavefront computes the derivatives of plane k+1 (into
+// one LDS buffer) and two stresses of plane k (from the other), so the loads
+// of both phases are in flight together.  Buffers alternate per plane; every
+// buffer is rewritten only after the barrier that follows its last read.
+template <typename T, bool TAB>
+__global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
+{
+    __shared__ T D[2][3 * K_NKIND * kShellLanes];
+    __shared__ double N[2][3 * kShellLanes];
     const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int lane = threadIdx.x;
-    const int tx = lane & ((1 << tl.txl) - 1), ty = lane >> tl.txl;
-    const int i0 = tl.x0 + tx, j = tl.y0 + ty;
-    const bool on = shell_lane(tl, A.box, i0, j, nx, ny);
-    const int i = on ? i0 : tl.ax, jj = on ? j : tl.ay;
+    const int lane = threadIdx.x & (kShellLanes - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kShellLanes);
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
-    const int64_t xm2 = max(i - 2, 1) - i, xm1 = max(i - 1, 1) - i, xp1 = min(i + 1, nx) - i, xp2 = min(i + 2, nx) - i;
-    const int64_t ym2 = (max(jj - 2, 1) - jj) * sy, ym1 = (max(jj - 1, 1) - jj) * sy;
-    const int64_t yp1 = (min(jj + 1, ny) - jj) * sy, yp2 = (min(jj + 2, ny) - jj) * sy;
-    const int64_t col = A.xoff + (int64_t)(i - 1) + sy * (jj - 1);
-    const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
-    T q[3][5];
+    const ShellCol L = shell_col(tl, A.box, lane, nx, ny, sy, A.xoff);
+    const int i = L.i, j = L.j;
+    auto nests = [&](int k, bool &L1, bool &L2, bool &L3, bool &L4) {
+        L1 = L.on && i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1;
+        L2 = L.on && i >= 3 && i <= nx - 1 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1;
+        L3 = L.on && i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2;
+        L4 = L.on && i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 2 && k <= nz - 2;
+    };
+    // z queue of this wavefront's velocity component
+    const T *vq = w == 0 ? A.v[0] : (w == 1 ? A.v[1] : A.v[2]);
+    T q[5];
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) q[m][s] = A.v[m][col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
-    double nrm = 0.0;
-    int64_t slot = tl.base + (on ? lane : tl.alane);
-    for (int k = tl.k0; k <= tl.k1; ++k, slot += 256) {
-        const int64_t c = col + sz * (k - 1);
+    for (int s = 0; s < 4; ++s) q[s] = vq[L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
+    T *const blk0 = shell_block(A, tl, L.on ? lane : tl.alane);
+    // phase 1 of plane k into buffer b
+    auto phase1 = [&](int k, int b) {
+        const int64_t c = L.col + sz * (k - 1);
         const int64_t zp2 = sz * (min(k + 2, nz) - k);
+        T *const blk = blk0 + (int64_t)(k - tl.k0) * (S_NQ * kShellLanes);
+        bool L1, L2, L3, L4;
+        nests(k, L1, L2, L3, L4);
+        // read by all three wavefronts: cached loads
+        T cf[6];
 #pragma unroll
-        for (int m = 0; m < 3; ++m) q[m][4] = A.v[m][c + zp2];
-        const bool L1 = on && i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1;
-        const bool L2 = on && i >= 3 && i <= nx - 1 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1;
-        const bool L3 = on && i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2;
-        const bool L4 = on && i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 2 && k <= nz - 2;
-        const T a = ldnt(&A.a[slot]), b = ldnt(&A.b[slot]), ik = ldnt(&A.ik[slot]);
-        const T ah = ldnt(&A.ah[slot]), bh = ldnt(&A.bh[slot]), ikh = ldnt(&A.ikh[slot]);
-        T d[3][K_NKIND];
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            const T *v = A.v[m];
-            T *const *mm = A.m + m * K_NKIND;
-            const T v0 = q[m][2];
-            const T dxp = dp4(v[c + xm1], v0, v[c + xp1], v[c + xp2], rx);
-            const T dxm = dm4(v[c + xm2], v[c + xm1], v0, v[c + xp1], rx);
-            const T dyp = dp4(v[c + ym1], v0, v[c + yp1], v[c + yp2], ry);
-            const T dym = dm4(v[c + ym2], v[c + ym1], v0, v[c + yp1], ry);
-            const T dzp = dp4(q[m][1], v0, q[m][3], q[m][4], rz);
-            const T dzm = dm4(q[m][0], q[m][1], v0, q[m][3], rz);
-            T p0, p1, p2, p3, p4, p5 = T(0), p6;
-            d[m][K_XPH] = cpml_upd(dxp, &mm[K_XPH][slot], ah, bh, ikh, p0);
-            d[m][K_XMF] = cpml_upd(dxm, &mm[K_XMF][slot], a, b, ik, p1);
-            d[m][K_YMF] = cpml_upd(dym, &mm[K_YMF][slot], a, b, ik, p2);
-            d[m][K_YPH] = cpml_upd(dyp, &mm[K_YPH][slot], ah, bh, ikh, p3);
-            d[m][K_ZMF] = cpml_upd(dzm, &mm[K_ZMF][slot], a, b, ik, p4);
-            // vx's z derivative enters sigma_xy uncorrected (cpmlfdtd.f08:1858-1865)
-            d[m][K_ZMH] = m == 0 ? dzm : cpml_upd(dzm, &mm[K_ZMH][slot], ah, bh, ikh, p5);
-            d[m][K_ZPH] = cpml_upd(dzp, &mm[K_ZPH][slot], ah, bh, ikh, p6);
-            if (L1 || L4) stnt(&mm[K_XPH][slot], p0);
-            if (L2 || L3) stnt(&mm[K_XMF][slot], p1);
-            if (L1 || L3) stnt(&mm[K_YMF][slot], p2);
-            if (L2 || L4) stnt(&mm[K_YPH][slot], p3);
-            if (L1) stnt(&mm[K_ZMF][slot], p4);
-            if (L2 && m != 0) stnt(&mm[K_ZMH][slot], p5);
-            if (L3 || L4) stnt(&mm[K_ZPH][slot], p6);
-        }
-        T sxx = ldnt(&A.s[0][c]), syy = ldnt(&A.s[1][c]), szz = ldnt(&A.s[2][c]);
-        T sxy = ldnt(&A.s[3][c]), sxz = ldnt(&A.s[4][c]), syz = ldnt(&A.s[5][c]);
-        Stiff<T> K;
-        T ig[6];
-        load_material<T, TAB>(K, ig, A, c);
-        {
-            const T dxx = d[0][K_XPH], dyy = d[1][K_YMF], dzz = d[2][K_ZMF];
-            const T yz = d[1][K_ZMF] + d[2][K_YMF], xz = d[0][K_ZMF] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YMF];
-            const T n0 = supk(sxx, K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, ig[0]);
-            const T n1 = supk(syy, K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, ig[1]);
-            const T n2 = supk(szz, K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, ig[2]);
-            if (L1) { sxx = n0; syy = n1; szz = n2; stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); stnt(&A.s[2][c], szz); }
-        }
-        {
-            const T dxx = d[0][K_XMF], dyy = d[1][K_YPH], dzz = d[2][K_ZMH];
-            const T yz = d[1][K_ZMH] + d[2][K_YPH], xz = d[0][K_ZMH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YPH];
-            const T n = supk(sxy, K, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, ig[3]);
-            if (L2) { sxy = n; stnt(&A.s[3][c], sxy); }
-        }
-        {
-            const T dxx = d[0][K_XMF], dyy = d[1][K_YMF], dzz = d[2][K_ZPH];
-            const T yz = d[1][K_ZPH] + d[2][K_YMF], xz = d[0][K_ZPH] + d[2][K_XMF], xy = d[1][K_XMF] + d[0][K_YMF];
-            const T n = supk(sxz, K, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, ig[4]);
-            if (L3) { sxz = n; stnt(&A.s[4][c], sxz); }
-        }
-        {
-            const T dxx = d[0][K_XPH], dyy = d[1][K_YPH], dzz = d[2][K_ZPH];
-            const T yz = d[1][K_ZPH] + d[2][K_YPH], xz = d[0][K_ZPH] + d[2][K_XPH], xy = d[1][K_XPH] + d[0][K_YPH];
-            const T n = supk(syz, K, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, ig[5]);
-            if (L4) { syz = n; stnt(&A.s[5][c], syz); }
-        }
-        if (on) {
+        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+        if (w == 0) shell_derivs<T, 0>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
+        else if (w == 1) shell_derivs<T, 1>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
+        else shell_derivs<T, 2>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
+        q[0] = q[1]; q[1] = q[2]; q[2] = q[3]; q[3] = q[4];
+    };
+    phase1(tl.k0, 0);
+    __syncthreads();
+    double nrm = 0.0;
+    for (int k = tl.k0; k <= tl.k1; ++k) {
+        const int b = (k - tl.k0) & 1;
+        if (k < tl.k1) phase1(k + 1, b ^ 1);
+        const int64_t c = L.col + sz * (k - 1);
+        bool L1, L2, L3, L4;
+        nests(k, L1, L2, L3, L4);
+        double part;
+        if (w == 0) part = shell_stresses_at<T, TAB, 0>(A, c, D[b], lane, L1, L2, L3, L4);
+        else if (w == 1) part = shell_stresses_at<T, TAB, 1>(A, c, D[b], lane, L1, L2, L3, L4);
+        else part = shell_stresses_at<T, TAB, 2>(A, c, D[b], lane, L1, L2, L3, L4);
+        N[b][w * kShellLanes + lane] = part;
+        __syncthreads();
+        if (w == 0 && L.on) {
             const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3];
-            const double a0 = sxx, a1 = syy, a2 = szz, a3 = sxy, a4 = syz, a5 = sxz;
-            const double n = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5));
+            const double n = sqrt(N[b][lane] + N[b][kShellLanes + lane] + N[b][2 * kShellLanes + lane]);
             nrm = fix ? nrm : fmax_nan(nrm, n);
-        }
-#pragma unroll
-        for (int m = 0; m < 3; ++m) {
-            q[m][0] = q[m][1]; q[m][1] = q[m][2]; q[m][2] = q[m][3]; q[m][3] = q[m][4];
         }
     }
     block_max_to(&A.snorm[*A.d_it - 1], nrm);
 }
 
-template <typename T, bool CORR, bool TAB>
-__global__ void __launch_bounds__(256) el3_shell_velocity_kernel(El3Args<T> A, const El3Tile *tiles)
+// Velocity pass for component M (cpmlfdtd.f08:2014-2038 vx, 2055-2080 vy,
+// 2098-2127 vz): three stress derivatives, their memory variables, the
+// averaged densities; Dirichlet faces (2226-2248) give 0.  Returns the new
+// value (the old one outside the nest's range).
+template <typename T, bool CORR, bool TAB, int M>
+__device__ __forceinline__ T shell_velocity(const El3Args<T> &A, const ShellCol &L, T (&qs)[5], T (&qr)[3], int64_t c,
+                                            T *blk, int k, const T (&cf)[6])
 {
-    const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int lane = threadIdx.x;
-    const int tx = lane & ((1 << tl.txl) - 1), ty = lane >> tl.txl;
-    const int i0 = tl.x0 + tx, j = tl.y0 + ty;
-    const bool on = shell_lane(tl, A.box, i0, j, nx, ny);
-    const int i = on ? i0 : tl.ax, jj = on ? j : tl.ay;
-    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
-    const int64_t xm2 = max(i - 2, 1) - i, xm1 = max(i - 1, 1) - i, xp1 = min(i + 1, nx) - i, xp2 = min(i + 2, nx) - i;
-    const int64_t ym2 = (max(jj - 2, 1) - jj) * sy, ym1 = (max(jj - 1, 1) - jj) * sy;
-    const int64_t yp1 = (min(jj + 1, ny) - jj) * sy, yp2 = (min(jj + 2, ny) - jj) * sy;
-    const int64_t col = A.xoff + (int64_t)(i - 1) + sy * (jj - 1);
+    const int nx = A.nx, ny = A.ny, nz = A.nz, i = L.i, j = L.j;
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
     const int dc = A.density_code;
-    const T *sxx = A.s[0], *syy = A.s[1], *szz = A.s[2], *sxy = A.s[3], *sxz = A.s[4], *syz = A.s[5];
-    const bool xface = i == 1 || i == nx || jj == 1 || jj == ny;
-    T qxz[5], qyz[5], qzz[5], qr[3];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int64_t o = col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1);
-        qxz[s] = sxz[o]; qyz[s] = syz[o]; qzz[s] = szz[o];
+    const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
+    const bool face = i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz;
+    const T r = qr[1];
+    const T ryy = scalar_mean(r, r, dc);
+    T v = ldnt(&A.v[M][c]);
+    T p0, p1, p2;
+    T d1, d2, d3;
+    T *m1, *m2, *m3;
+    T r1, r3;
+    bool in;
+    if (M == 0) {
+        const T *sxx = A.s[0], *sxy = A.s[3];
+        d1 = tminus5(sxx[c + L.xm2], sxx[c + L.xm1], sxx[c], CORR ? sxx[c + L.xp1] : T(0), CORR) * rx;
+        d2 = tminus5(sxy[c + L.ym2], sxy[c + L.ym1], sxy[c], CORR ? sxy[c + L.yp1] : T(0), CORR) * ry;
+        d3 = tminus5(qs[0], qs[1], qs[2], qs[3], CORR) * rz;
+        m1 = blk + M_SXX_X * kShellLanes; m2 = blk + M_SXY_Y * kShellLanes; m3 = blk + M_SXZ_Z * kShellLanes;
+        d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
+        d2 = cpml_upd(d2, m2, ah, bh, ikh, p1);
+        d3 = cpml_upd(d3, m3, ah, bh, ikh, p2);
+        r1 = scalar_mean(r, rho_at<T, TAB>(A, c + L.xm1), dc);
+        r3 = scalar_mean(r, qr[0], dc);
+        in = i >= 3 && j >= 3 && k >= 3;
+    } else if (M == 1) {
+        const T *sxy = A.s[3], *syy = A.s[1];
+        d1 = tplus5(CORR ? sxy[c + L.xm1] : T(0), sxy[c], sxy[c + L.xp1], sxy[c + L.xp2], CORR) * rx;
+        d2 = tplus5(CORR ? syy[c + L.ym1] : T(0), syy[c], syy[c + L.yp1], syy[c + L.yp2], CORR) * ry;
+        d3 = tminus5(qs[0], qs[1], qs[2], qs[3], CORR) * rz;
+        m1 = blk + M_SXY_X * kShellLanes; m2 = blk + M_SYY_Y * kShellLanes; m3 = blk + M_SYZ_Z * kShellLanes;
+        d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
+        d2 = cpml_upd(d2, m2, ah, bh, ikh, p1);
+        d3 = cpml_upd(d3, m3, a, b, ik, p2);
+        r1 = scalar_mean(r, rho_at<T, TAB>(A, c + L.xp1), dc);
+        r3 = scalar_mean(r, qr[0], dc);
+        in = i <= nx - 2 && j <= ny - 2 && k >= 3;
+    } else {
+        const T *sxz = A.s[4], *syz = A.s[5];
+        d1 = tplus5(CORR ? sxz[c + L.xm1] : T(0), sxz[c], sxz[c + L.xp1], sxz[c + L.xp2], CORR) * rx;
+        d2 = tminus5(syz[c + L.ym2], syz[c + L.ym1], syz[c], CORR ? syz[c + L.yp1] : T(0), CORR) * ry;
+        d3 = tplus5(qs[1], qs[2], qs[3], qs[4], CORR) * rz;
+        m1 = blk + M_SXZ_X * kShellLanes; m2 = blk + M_SYZ_Y * kShellLanes; m3 = blk + M_SZZ_Z * kShellLanes;
+        d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
+        d2 = cpml_upd(d2, m2, a, b, ik, p1);
+        d3 = cpml_upd(d3, m3, ah, bh, ikh, p2);
+        r1 = scalar_mean(r, rho_at<T, TAB>(A, c + L.xp1), dc);
+        r3 = scalar_mean(r, qr[2], dc);
+        in = i <= nx - 2 && j >= 3 && k <= nz - 2;
     }
-    qr[0] = rho_at<T, TAB>(A, col + sz * (max(tl.k0 - 1, 1) - 1));
-    qr[1] = rho_at<T, TAB>(A, col + sz * (tl.k0 - 1));
+    const T n = v + (d1 / r1 + d2 / ryy + d3 / r3) * dt;
+    if (L.on && !face && in) {
+        stnt(m1, p0); stnt(m2, p1); stnt(m3, p2);
+        v = n;
+    }
+    if (face) v = T(0);
+    if (L.on) stnt(&A.v[M][c], v);
+    return v;
+}
+
+// One barrier per plane: the norm of a plane needs all three components, so
+// each wavefront leaves v^2 in LDS (double-buffered) for wavefront 0.
+template <typename T, bool CORR, bool TAB>
+__global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3Args<T> A, const El3Tile *tiles)
+{
+    __shared__ double V2[2][3 * kShellLanes];
+    const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    const int nz = A.nz;
+    const int lane = threadIdx.x & (kShellLanes - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kShellLanes);
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * A.ny;
+    const ShellCol L = shell_col(tl, A.box, lane, A.nx, A.ny, sy, A.xoff);
+    // z queue: sigma_xz (vx), sigma_yz (vy), sigma_zz (vz), planes k-2..k+2
+    const T *sq = w == 0 ? A.s[4] : (w == 1 ? A.s[5] : A.s[2]);
+    T qs[5], qr[3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) qs[s] = sq[L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
+    qr[0] = rho_at<T, TAB>(A, L.col + sz * (max(tl.k0 - 1, 1) - 1));
+    qr[1] = rho_at<T, TAB>(A, L.col + sz * (tl.k0 - 1));
     double nrm = 0.0;
-    int64_t slot = tl.base + (on ? lane : tl.alane);
-    for (int k = tl.k0; k <= tl.k1; ++k, slot += 256) {
-        const int64_t c = col + sz * (k - 1);
+    T *blk = shell_block(A, tl, L.on ? lane : tl.alane);
+    for (int k = tl.k0; k <= tl.k1; ++k, blk += S_NQ * kShellLanes) {
+        const int b = (k - tl.k0) & 1;
+        const int64_t c = L.col + sz * (k - 1);
         const int64_t zp1 = sz * (min(k + 1, nz) - k), zp2 = sz * (min(k + 2, nz) - k);
-        qxz[4] = sxz[c + zp2]; qyz[4] = syz[c + zp2]; qzz[4] = szz[c + zp2];
+        qs[4] = sq[c + zp2];
         qr[2] = rho_at<T, TAB>(A, c + zp1);
-        const bool face = xface || k == 1 || k == nz;
-        const bool Lx = on && !face && i >= 3 && j >= 3 && k >= 3;
-        const bool Ly = on && !face && i <= nx - 2 && j <= ny - 2 && k >= 3;
-        const bool Lz = on && !face && i <= nx - 2 && j >= 3 && k <= nz - 2;
-        const T a = ldnt(&A.a[slot]), b = ldnt(&A.b[slot]), ik = ldnt(&A.ik[slot]);
-        const T ah = ldnt(&A.ah[slot]), bh = ldnt(&A.bh[slot]), ikh = ldnt(&A.ikh[slot]);
-        const T r = qr[1];
-        const T ryy = scalar_mean(r, r, dc);
-        T vx = ldnt(&A.v[0][c]), vy = ldnt(&A.v[1][c]), vz = ldnt(&A.v[2][c]);
-        T p0, p1, p2;
-        {   // vx: cpmlfdtd.f08:2014-2038
-            T d1 = tminus5(sxx[c + xm2], sxx[c + xm1], sxx[c], CORR ? sxx[c + xp1] : T(0), CORR) * rx;
-            T d2 = tminus5(sxy[c + ym2], sxy[c + ym1], sxy[c], CORR ? sxy[c + yp1] : T(0), CORR) * ry;
-            T d3 = tminus5(qxz[0], qxz[1], qxz[2], qxz[3], CORR) * rz;
-            d1 = cpml_upd(d1, &A.m[M_SXX_X][slot], ah, bh, ikh, p0);
-            d2 = cpml_upd(d2, &A.m[M_SXY_Y][slot], ah, bh, ikh, p1);
-            d3 = cpml_upd(d3, &A.m[M_SXZ_Z][slot], ah, bh, ikh, p2);
-            const T n = vx + (d1 / scalar_mean(r, rho_at<T, TAB>(A, c + xm1), dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
-            if (Lx) {
-                stnt(&A.m[M_SXX_X][slot], p0); stnt(&A.m[M_SXY_Y][slot], p1); stnt(&A.m[M_SXZ_Z][slot], p2);
-                vx = n;
-            }
-        }
-        {   // vy: cpmlfdtd.f08:2055-2080
-            T d1 = tplus5(CORR ? sxy[c + xm1] : T(0), sxy[c], sxy[c + xp1], sxy[c + xp2], CORR) * rx;
-            T d2 = tplus5(CORR ? syy[c + ym1] : T(0), syy[c], syy[c + yp1], syy[c + yp2], CORR) * ry;
-            T d3 = tminus5(qyz[0], qyz[1], qyz[2], qyz[3], CORR) * rz;
-            d1 = cpml_upd(d1, &A.m[M_SXY_X][slot], ah, bh, ikh, p0);
-            d2 = cpml_upd(d2, &A.m[M_SYY_Y][slot], ah, bh, ikh, p1);
-            d3 = cpml_upd(d3, &A.m[M_SYZ_Z][slot], a, b, ik, p2);
-            const T n = vy + (d1 / scalar_mean(r, rho_at<T, TAB>(A, c + xp1), dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
-            if (Ly) {
-                stnt(&A.m[M_SXY_X][slot], p0); stnt(&A.m[M_SYY_Y][slot], p1); stnt(&A.m[M_SYZ_Z][slot], p2);
-                vy = n;
-            }
-        }
-        {   // vz: cpmlfdtd.f08:2098-2127
-            T d1 = tplus5(CORR ? sxz[c + xm1] : T(0), qxz[2], sxz[c + xp1], sxz[c + xp2], CORR) * rx;
-            T d2 = tminus5(syz[c + ym2], syz[c + ym1], qyz[2], CORR ? syz[c + yp1] : T(0), CORR) * ry;
-            T d3 = tplus5(qzz[1], qzz[2], qzz[3], qzz[4], CORR) * rz;
-            d1 = cpml_upd(d1, &A.m[M_SXZ_X][slot], ah, bh, ikh, p0);
-            d2 = cpml_upd(d2, &A.m[M_SYZ_Y][slot], a, b, ik, p1);
-            d3 = cpml_upd(d3, &A.m[M_SZZ_Z][slot], ah, bh, ikh, p2);
-            const T n = vz + (d1 / scalar_mean(r, rho_at<T, TAB>(A, c + xp1), dc) + d2 / ryy + d3 / scalar_mean(r, qr[2], dc)) * dt;
-            if (Lz) {
-                stnt(&A.m[M_SXZ_X][slot], p0); stnt(&A.m[M_SYZ_Y][slot], p1); stnt(&A.m[M_SZZ_Z][slot], p2);
-                vz = n;
-            }
-        }
-        if (face) vx = vy = vz = T(0);   // Dirichlet faces, cpmlfdtd.f08:2226-2248
-        if (on) {
-            stnt(&A.v[0][c], vx); stnt(&A.v[1][c], vy); stnt(&A.v[2][c], vz);
+        // read by all three wavefronts: cached loads
+        T cf[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+        T v;
+        if (w == 0) v = shell_velocity<T, CORR, TAB, 0>(A, L, qs, qr, c, blk, k, cf);
+        else if (w == 1) v = shell_velocity<T, CORR, TAB, 1>(A, L, qs, qr, c, blk, k, cf);
+        else v = shell_velocity<T, CORR, TAB, 2>(A, L, qs, qr, c, blk, k, cf);
+        const double dv = v;
+        V2[b][w * kShellLanes + lane] = dv * dv;
+        __syncthreads();
+        if (w == 0 && L.on) {
             const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2];
-            const double a0 = vx, a1 = vy, a2 = vz;
-            nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
+            const double n = sqrt(V2[b][lane] + V2[b][kShellLanes + lane] + V2[b][2 * kShellLanes + lane]);
+            nrm = fix ? nrm : fmax_nan(nrm, n);
         }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            qxz[s] = qxz[s + 1]; qyz[s] = qyz[s + 1]; qzz[s] = qzz[s + 1];
-        }
+        for (int s = 0; s < 4; ++s) qs[s] = qs[s + 1];
         qr[0] = qr[1]; qr[1] = qr[2];
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
@@ -527,7 +680,7 @@ static void fast_stress(Device &d, const El3Args<T> &A)
         else int_stress<T, 4, TAB>(d, A, kc);
     }
     if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_stress_kernel<T, TAB>), dim3(d.ntiles), dim3(256), 0, d.stream, A,
+        hipLaunchKernelGGL((el3_shell_stress_kernel<T, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, d.stream, A,
                            (const El3Tile *)d.tiles);
 }
 
@@ -541,8 +694,8 @@ static void fast_velocity(Device &d, const El3Args<T> &A)
         else int_velocity<T, CORR, 4, TAB>(d, A, kc);
     }
     if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB>), dim3(d.ntiles), dim3(256), 0, d.stream, A,
-                           (const El3Tile *)d.tiles);
+        hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0,
+                           d.stream, A, (const El3Tile *)d.tiles);
 }
 
 template <typename T>

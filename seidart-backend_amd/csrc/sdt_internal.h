@@ -79,7 +79,8 @@ struct Device {
     int64_t nflagged = 0;              // tiles flagged inside the box
     void *tiles = nullptr;             // 3D elastic CPML-shell tile list (device)
     int ntiles = 0;
-    int64_t nslots = 0;                // compact CPML storage length
+    int64_t nslots = 0;                // compact CPML storage length (cells x planes)
+    void *shell = nullptr;             // 3D elastic blocked shell storage (a grid allocation)
     int box[6] = {1, 0, 1, 0, 1, 0};   // interior box ilo,ihi,jlo,jhi,klo,khi (1-based, inclusive):
                                        // every cell inside has identity CPML coefficients
     int tiles_x = 0, tiles_y = 0, tiles_z = 0;
