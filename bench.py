@@ -65,6 +65,7 @@ def timed_run(cfg, args, material_table):
     s.set_option("stagger", args.stagger)
     s.set_option("tile_rows", args.tile_rows)
     s.set_option("material_table", material_table)
+    s.set_option("concurrent", args.concurrent)
     s.set_arrays(cfg["arrays"])
     for comp, series in cfg["sources"].items():
         s.set_source(comp, series)
@@ -131,6 +132,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per interior workgroup tile (0 = auto)")
     ap.add_argument("--material-table", type=int, default=1,
                     help="1: material ids + table when the model allows (default), 0: per-cell arrays")
+    ap.add_argument("--concurrent", type=int, default=0,
+                    help="1: CPML-shell kernels on a second stream beside the interior kernels")
     ap.add_argument("--compare-layouts", type=int, default=1,
                     help="also time the per-cell-array layout and report it beside the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -201,7 +204,8 @@ def main():
                    "cells": ncell, "precision": args.precision,
                    "material_layout": f"table ({nmat} materials, 1-byte id per cell)" if table
                                       else "per-cell arrays",
-                   "zchunk": args.zchunk, "stagger": args.stagger, "tile_rows": args.tile_rows},
+                   "zchunk": args.zchunk, "stagger": args.stagger, "tile_rows": args.tile_rows,
+                   "concurrent_shell": bool(args.concurrent)},
         "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "hbm_fraction_step_path_bytes": round(step_bytes_path * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": "el3 stress pass (interior + CPML-shell kernels)",

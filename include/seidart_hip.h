@@ -204,6 +204,10 @@ enum {
     SDT_OPT_STAGGER = 3,        /* 1 = offset each grid array's start by a different
                                    multiple of 4352 B (default), 0 = plain hipMalloc     */
     SDT_OPT_TILE_ROWS = 4,      /* rows of the 3D interior workgroup tile: 0 auto, 4, 8, 16 */
+    SDT_OPT_CONCURRENT = 6,     /* 3D elastic: 1 = run the CPML-shell kernels on a second stream
+                                   concurrently with the interior kernels of the same half step,
+                                   0 = one after the other on one stream (default; the two are
+                                   equally fast at 280^3, where HBM is the shared limit)      */
     SDT_OPT_MATERIAL_TABLE = 5  /* 3D elastic: 1 = when the model holds at most 256 distinct
                                    (c11..c66, gamma_*, rho) tuples, keep one byte of material id
                                    per cell and a table of the tuples instead of 28 per-cell

@@ -38,6 +38,9 @@ static El3Args<T> el3_args(Device &d)
     A.mid = d.mid;
     A.tab = (const T *)d.tab;
     A.shell = (T *)d.shell;
+    A.snq = d.shell_snq;
+    A.cid = d.shell_cid;
+    A.ctab = (const T *)d.shell_ctab;
     A.flags = d.tile_flags;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
@@ -299,33 +302,85 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
     if (hipMalloc(&d.tiles, tiles.size() * sizeof(El3Tile)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(d.tiles, tiles.data(), tiles.size() * sizeof(El3Tile), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
-    // Shell storage, blocked by tile-plane: the S_NQ streams of one tile-plane
-    // (memory variables, then a, b, 1/kappa, a_half, b_half, 1/kappa_half)
-    // are kShellLanes-element runs of one contiguous block, so a workgroup's
-    // accesses of a plane fall in one block (one TLB entry, open DRAM rows).
-    const int64_t nblk = d.nslots / kShellLanes;
-    std::vector<double> blk((size_t)(nblk * S_NQ * kShellLanes), 0.0);
+    // CPML coefficients of every slot: a, b, 1/kappa, a_half, b_half, 1/kappa_half
     const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
-    for (int q = 0; q < 6; ++q) {
+    auto coef = [&](int q, int i, int j, int k) {
         const double *src = in[cids[q]];
         const bool inverse = (q == 2 || q == 5);
-        const double dflt = inverse ? 1.0 : 0.0;
+        double v = inverse ? 1.0 : 0.0;
+        if (src && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny)
+            v = src[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))];
+        return inverse ? 1.0 / v : v;
+    };
+    // Coefficient dictionary: CPML profiles give few distinct tuples (tens
+    // for the usual per-axis profiles), so the shell keeps a one-byte id per
+    // slot and a small table when there are at most 256 of them.
+    struct Tup {
+        double v[8];
+        bool operator==(const Tup &o) const { return memcmp(v, o.v, sizeof(v)) == 0; }
+    };
+    std::vector<Tup> tups;
+    std::vector<uint8_t> ids((size_t)d.nslots, 0);
+    bool dict = true;
+    {
+        std::unordered_multimap<uint64_t, int> index;
         for (const El3Tile &t : tiles) {
             const int tw = 1 << t.txl, th = kShellLanes >> t.txl;
-            for (int k = t.k0; k <= t.k1; ++k) {
-                const int64_t tp = t.base / kShellLanes + (k - t.k0);
-                double *row = &blk[(size_t)((tp * S_NQ + S_CF + q) * kShellLanes)];
-                for (int ty = 0; ty < th; ++ty)
+            for (int k = t.k0; k <= t.k1 && dict; ++k)
+                for (int ty = 0; ty < th && dict; ++ty)
                     for (int tx = 0; tx < tw; ++tx) {
-                        const int i = t.x0 + tx, j = t.y0 + ty;
-                        double v = dflt;
-                        if (src && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny)
-                            v = src[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))];
-                        row[ty * tw + tx] = inverse ? 1.0 / v : v;
+                        Tup u;
+                        memset(&u, 0, sizeof(u));
+                        for (int q = 0; q < 6; ++q) u.v[q] = coef(q, t.x0 + tx, t.y0 + ty, k);
+                        uint64_t h = 1469598103934665603ull;
+                        for (int q = 0; q < 6; ++q) {
+                            uint64_t w;
+                            memcpy(&w, &u.v[q], 8);
+                            h = (h ^ w) * 1099511628211ull;
+                        }
+                        int id = -1;
+                        auto range = index.equal_range(h);
+                        for (auto it = range.first; it != range.second; ++it)
+                            if (tups[(size_t)it->second] == u) { id = it->second; break; }
+                        if (id < 0) {
+                            if ((int)tups.size() == 256) { dict = false; break; }
+                            id = (int)tups.size();
+                            tups.push_back(u);
+                            index.emplace(h, id);
+                        }
+                        ids[(size_t)(t.base + (int64_t)(k - t.k0) * kShellLanes + ty * tw + tx)] = (uint8_t)id;
                     }
-            }
         }
     }
+    d.shell_snq = dict ? M_TOTAL : S_NQ;
+    if (dict) {
+        if (hipMalloc((void **)&d.shell_cid, (size_t)d.nslots) != hipSuccess) return SDT_ERR_HIP;
+        if (hipMemcpy(d.shell_cid, ids.data(), (size_t)d.nslots, hipMemcpyHostToDevice) != hipSuccess)
+            return SDT_ERR_HIP;
+        std::vector<double> flat(tups.size() * 8);
+        for (size_t r = 0; r < tups.size(); ++r) memcpy(&flat[r * 8], tups[r].v, sizeof(tups[r].v));
+        if (upload_converted(d, flat.data(), &d.shell_ctab, (int64_t)flat.size()) != SDT_OK) return SDT_ERR_HIP;
+        d.shell_ncoef = (int)tups.size();
+    }
+    // Shell storage, blocked by tile-plane: the snq streams of one tile-plane
+    // (memory variables, then without the dictionary a, b, 1/kappa, a_half,
+    // b_half, 1/kappa_half) are kShellLanes-element runs of one contiguous
+    // block, so a workgroup's accesses of a plane fall in one block (one TLB
+    // entry, open DRAM rows).
+    const int snq = d.shell_snq;
+    const int64_t nblk = d.nslots / kShellLanes;
+    std::vector<double> blk((size_t)(nblk * snq * kShellLanes), 0.0);
+    if (!dict)
+        for (int q = 0; q < 6; ++q)
+            for (const El3Tile &t : tiles) {
+                const int tw = 1 << t.txl, th = kShellLanes >> t.txl;
+                for (int k = t.k0; k <= t.k1; ++k) {
+                    const int64_t tp = t.base / kShellLanes + (k - t.k0);
+                    double *row = &blk[(size_t)((tp * snq + S_CF + q) * kShellLanes)];
+                    for (int ty = 0; ty < th; ++ty)
+                        for (int tx = 0; tx < tw; ++tx) row[ty * tw + tx] = coef(q, t.x0 + tx, t.y0 + ty, k);
+                }
+            }
     if (upload_converted(d, blk.data(), &d.shell, (int64_t)blk.size()) != SDT_OK) return SDT_ERR_HIP;
     return SDT_OK;
 }

@@ -27,7 +27,13 @@ struct El3Args {
     // xy xz yz (6), rho, 4 unused
     const uint8_t *mid;
     const T *tab;
-    T *shell;       // CPML-shell storage: [tile-plane][S_NQ][kShellLanes]
+    T *shell;       // CPML-shell storage: [tile-plane][snq][kShellLanes]
+    int snq;        // streams per tile-plane block: M_TOTAL (+ 6 coefficient streams without ctab)
+    // CPML coefficient dictionary of the shell (at most 256 distinct
+    // (a, b, 1/kappa, a_half, b_half, 1/kappa_half) tuples): id per slot and
+    // rows of 8 (6 used); ctab == nullptr keeps the coefficients in the block
+    const uint8_t *cid;
+    const T *ctab;
     const uint8_t *flags;
     int box[6];
     int nx, ny, nz, tiles_x, tiles_y, pitch;

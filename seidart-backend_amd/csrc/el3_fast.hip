@@ -309,7 +309,22 @@ __device__ __forceinline__ ShellCol shell_col(const El3Tile &tl, const int (&box
 template <typename T>
 __device__ __forceinline__ T *shell_block(const El3Args<T> &A, const El3Tile &tl, int lane)
 {
-    return A.shell + (tl.base / kShellLanes) * (int64_t)(S_NQ * kShellLanes) + lane;
+    return A.shell + (tl.base / kShellLanes) * (int64_t)(A.snq * kShellLanes) + lane;
+}
+
+// CPML coefficients of a shell slot: from the dictionary (one-byte id, cached
+// table) or from the block's coefficient streams.
+template <typename T>
+__device__ __forceinline__ void shell_coefs(const El3Args<T> &A, const T *blk, int64_t slot, T (&cf)[6])
+{
+    if (A.ctab) {
+        const T *row = A.ctab + 8 * (int)A.cid[slot];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = row[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+    }
 }
 
 template <typename T>
@@ -481,18 +496,19 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
     T q[5];
 #pragma unroll
     for (int s = 0; s < 4; ++s) q[s] = vq[L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
-    T *const blk0 = shell_block(A, tl, L.on ? lane : tl.alane);
+    const int lsel = L.on ? lane : tl.alane;
+    T *const blk0 = shell_block(A, tl, lsel);
+    const int64_t bstride = (int64_t)A.snq * kShellLanes;
     // phase 1 of plane k into buffer b
     auto phase1 = [&](int k, int b) {
         const int64_t c = L.col + sz * (k - 1);
         const int64_t zp2 = sz * (min(k + 2, nz) - k);
-        T *const blk = blk0 + (int64_t)(k - tl.k0) * (S_NQ * kShellLanes);
+        T *const blk = blk0 + (int64_t)(k - tl.k0) * bstride;
         bool L1, L2, L3, L4;
         nests(k, L1, L2, L3, L4);
         // read by all three wavefronts: cached loads
         T cf[6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+        shell_coefs(A, blk, tl.base + (int64_t)(k - tl.k0) * kShellLanes + lsel, cf);
         if (w == 0) shell_derivs<T, 0>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
         else if (w == 1) shell_derivs<T, 1>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
         else shell_derivs<T, 2>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
@@ -610,8 +626,10 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3
     qr[0] = rho_at<T, TAB>(A, L.col + sz * (max(tl.k0 - 1, 1) - 1));
     qr[1] = rho_at<T, TAB>(A, L.col + sz * (tl.k0 - 1));
     double nrm = 0.0;
-    T *blk = shell_block(A, tl, L.on ? lane : tl.alane);
-    for (int k = tl.k0; k <= tl.k1; ++k, blk += S_NQ * kShellLanes) {
+    const int lsel = L.on ? lane : tl.alane;
+    T *blk = shell_block(A, tl, lsel);
+    int64_t slot = tl.base + lsel;
+    for (int k = tl.k0; k <= tl.k1; ++k, blk += (int64_t)A.snq * kShellLanes, slot += kShellLanes) {
         const int b = (k - tl.k0) & 1;
         const int64_t c = L.col + sz * (k - 1);
         const int64_t zp1 = sz * (min(k + 1, nz) - k), zp2 = sz * (min(k + 2, nz) - k);
@@ -619,8 +637,7 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3
         qr[2] = rho_at<T, TAB>(A, c + zp1);
         // read by all three wavefronts: cached loads
         T cf[6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+        shell_coefs(A, blk, slot, cf);
         T v;
         if (w == 0) v = shell_velocity<T, CORR, TAB, 0>(A, L, qs, qr, c, blk, k, cf);
         else if (w == 1) v = shell_velocity<T, CORR, TAB, 1>(A, L, qs, qr, c, blk, k, cf);
@@ -670,32 +687,54 @@ static void int_velocity(Device &d, const El3Args<T> &A, int kc)
                        d.stream, A, kc);
 }
 
+// The interior and shell kernels of a pass write disjoint cells and read only
+// the other half step's fields, so they may run concurrently: the shell kernel
+// goes to the side stream between a fork and a join event.
+static hipStream_t shell_stream_begin(Device &d, int pass)
+{
+    if (!d.concurrent || !d.side) return d.stream;
+    (void)hipEventRecord(d.fork[pass], d.stream);
+    (void)hipStreamWaitEvent(d.side, d.fork[pass], 0);
+    return d.side;
+}
+
+static void shell_stream_end(Device &d, int pass)
+{
+    if (!d.concurrent || !d.side) return;
+    (void)hipEventRecord(d.join[pass], d.side);
+    (void)hipStreamWaitEvent(d.stream, d.join[pass], 0);
+}
+
 template <typename T, bool TAB>
 static void fast_stress(Device &d, const El3Args<T> &A)
 {
     const int kc = el3_kc(d), ty = el3_ty(d);
+    const hipStream_t ss = shell_stream_begin(d, 0);
+    if (d.ntiles > 0)
+        hipLaunchKernelGGL((el3_shell_stress_kernel<T, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss, A,
+                           (const El3Tile *)d.tiles);
     if (!box_empty(d)) {
         if (ty == 16) int_stress<T, 16, TAB>(d, A, kc);
         else if (ty == 8) int_stress<T, 8, TAB>(d, A, kc);
         else int_stress<T, 4, TAB>(d, A, kc);
     }
-    if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_stress_kernel<T, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, d.stream, A,
-                           (const El3Tile *)d.tiles);
+    shell_stream_end(d, 0);
 }
 
 template <typename T, bool CORR, bool TAB>
 static void fast_velocity(Device &d, const El3Args<T> &A)
 {
     const int kc = el3_kc(d), ty = el3_ty(d);
+    const hipStream_t ss = shell_stream_begin(d, 1);
+    if (d.ntiles > 0)
+        hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss,
+                           A, (const El3Tile *)d.tiles);
     if (!box_empty(d)) {
         if (ty == 16) int_velocity<T, CORR, 16, TAB>(d, A, kc);
         else if (ty == 8) int_velocity<T, CORR, 8, TAB>(d, A, kc);
         else int_velocity<T, CORR, 4, TAB>(d, A, kc);
     }
-    if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0,
-                           d.stream, A, (const El3Tile *)d.tiles);
+    shell_stream_end(d, 1);
 }
 
 template <typename T>

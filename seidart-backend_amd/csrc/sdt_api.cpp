@@ -136,6 +136,13 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out)
     h->series.assign((size_t)SDT_NUM_SRC * p->time_steps, 0.0);
     e = hipSetDevice(p->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+    if (e == hipSuccess && p->physics == SDT_ELASTIC3) {
+        e = hipStreamCreateWithFlags(&d.side, hipStreamNonBlocking);
+        for (int q = 0; q < 2 && e == hipSuccess; ++q) {
+            e = hipEventCreateWithFlags(&d.fork[q], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&d.join[q], hipEventDisableTiming);
+        }
+    }
     if (e != hipSuccess) {
         int rc = hip_fail(nullptr, e, "stream creation");
         delete h;
@@ -156,12 +163,18 @@ static void release(sdt_handle *h)
     drop(d.tile_flags);
     drop(d.tiles);
     drop(d.mid);   // d.tab is one of the grid allocations freed above
+    drop(d.shell_cid);
     drop(d.series);
     drop(d.norms);
     drop(d.d_it);
     drop(d.rec_cell);
     drop(d.rec_out);
     for (auto ev : h->tev) (void)hipEventDestroy(ev);
+    for (int q = 0; q < 2; ++q) {
+        if (d.fork[q]) (void)hipEventDestroy(d.fork[q]);
+        if (d.join[q]) (void)hipEventDestroy(d.join[q]);
+    }
+    if (d.side) (void)hipStreamDestroy(d.side);
     if (d.stream) (void)hipStreamDestroy(d.stream);
 }
 
@@ -504,6 +517,13 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
         return SDT_OK;
     case SDT_OPT_GRAPH:
         h->d.graphs = value != 0;
+        return SDT_OK;
+    case SDT_OPT_CONCURRENT:
+        h->d.concurrent = value != 0;
+        if (h->graph) {   // captured steps follow the old setting: capture again
+            (void)hipGraphExecDestroy(h->graph);
+            h->graph = nullptr;
+        }
         return SDT_OK;
     case SDT_OPT_MATERIAL_TABLE:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "material layout is fixed once the first step has run");

@@ -81,6 +81,10 @@ struct Device {
     int ntiles = 0;
     int64_t nslots = 0;                // compact CPML storage length (cells x planes)
     void *shell = nullptr;             // 3D elastic blocked shell storage (a grid allocation)
+    int shell_snq = 0;                 // streams per shell tile-plane block
+    uint8_t *shell_cid = nullptr;      // CPML coefficient id per shell slot (or null)
+    void *shell_ctab = nullptr;        // CPML coefficient rows (a grid allocation)
+    int shell_ncoef = 0;               // distinct coefficient tuples (0 = in the block)
     int box[6] = {1, 0, 1, 0, 1, 0};   // interior box ilo,ihi,jlo,jhi,klo,khi (1-based, inclusive):
                                        // every cell inside has identity CPML coefficients
     int tiles_x = 0, tiles_y = 0, tiles_z = 0;
@@ -108,6 +112,12 @@ struct Device {
     int nbases = 0;
     size_t grid_bytes = 0;             // bytes allocated through alloc_grid
     hipStream_t stream = nullptr;
+    // second stream for the 3D elastic CPML-shell kernels, which run
+    // concurrently with the interior kernels of the same pass (fork/join by
+    // events); both streams have the default priority
+    hipStream_t side = nullptr;
+    hipEvent_t fork[2] = {}, join[2] = {};
+    bool concurrent = false;           // SDT_OPT_CONCURRENT
     hipEvent_t ev[4] = {};
     bool timing = false;
     double ms_stress = 0, ms_vel = 0;

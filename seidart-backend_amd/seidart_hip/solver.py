@@ -181,9 +181,12 @@ class Solver:
         """key: 'zchunk' (planes per z-marching workgroup, 0 auto), 'graph'
         (0/1), 'stagger' (0/1), 'tile_rows' (0 auto, 4, 8, 16) or
         'material_table' (1 = one-byte material ids + table when the 3D
-        elastic model has <= 256 distinct materials, 0 = per-cell arrays)."""
+        elastic model has <= 256 distinct materials, 0 = per-cell arrays) or
+        'concurrent' (1 = 3D elastic shell kernels on a second stream beside
+        the interior kernels, 0 = sequential)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
-             "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE}[key]
+             "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
+             "concurrent": L.OPT_CONCURRENT}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     def info(self, key):

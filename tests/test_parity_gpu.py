@@ -65,7 +65,8 @@ def test_elastic2_stress_source_and_attenuation():
 
 
 @pytest.mark.parametrize("options", [{}, {"zchunk": 5}, {"zchunk": 1}, {"zchunk": 7, "stagger": 0},
-                                     {"material_table": 0}, {"material_table": 0, "zchunk": 3}])
+                                     {"material_table": 0}, {"material_table": 0, "zchunk": 3},
+                                     {"concurrent": 0}, {"concurrent": 0, "graph": 0}])
 def test_elastic3_small_fp64(options):
     """z-marching chunk heights that put chunk seams inside and outside the
     CPML slabs and the interior box."""
