@@ -59,8 +59,8 @@ def timed_run(cfg, args, material_table):
     import numpy as np
     from seidart_hip.solver import Solver
     K, W = args.steps, args.warmup
-    s = Solver("elastic3", cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K, cfg["src"],
-               precision=args.precision, density_method=cfg["density_method"])
+    s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K, cfg["src"],
+               precision=args.precision, density_method=cfg.get("density_method", "arithmetic"))
     s.set_option("zchunk", args.zchunk)
     s.set_option("stagger", args.stagger)
     s.set_option("tile_rows", args.tile_rows)
@@ -119,6 +119,48 @@ def calibrate_copy(nbytes=2 << 30, reps=20):
                       "ms": round(ms, 4), "GB/s": round(2 * n * 8 / ms / 1e6, 1)}), flush=True)
 
 
+EM3 = dict(F=6, M=12, PSI=12, C=6)
+
+
+def bench_em3(args):
+    """3D anisotropic EM (electromag3) on 384 x 384 x 192 + 10-cell CPML
+    (404 x 404 x 212 padded), dx = 0.05 m, five seeded SPD permittivity /
+    conductivity layers, 100 MHz Ricker Ex dipole, 128 receivers."""
+    import numpy as np
+    from seidart_hip import models
+    K, W = args.steps, args.warmup
+    cfg = models.em3(steps=W + K)
+    shape = cfg["shape"]
+    ncell = int(np.prod(shape))
+    w = 8 if args.precision == "fp64" else 4
+    elapsed, ms_h, ms_e, launches, nmat = timed_run(cfg, args, args.material_table)
+    nx, ny, nz = shape
+    cp = cfg["cpml"]
+    n_pml = ncell - (nx - 2 * cp) * (ny - 2 * cp) * (nz - 2 * cp)
+    step_bytes = w * (ncell * (2 * EM3["F"] + EM3["M"]) + n_pml * (2 * EM3["PSI"] + EM3["C"]))
+    # E pass: E read+written, H read, inverse permittivity and conductivity
+    # (12), CPML cells 6 memory variables + 6 coefficients
+    e_bytes = w * (ncell * (6 + 3 + 12) + n_pml * (12 + 6))
+    avg_e = ms_e / max(launches, 1) / 1e3
+    line = {
+        "metric": f"GCell-updates/s (3D EM CPML-FDTD, {args.precision.upper()})",
+        "value": round(ncell * K / elapsed / 1e9, 4), "unit": "GCell-updates/s", "n_gpus": 1,
+        "steps": K, "warmup": W, "ms_per_step": round(elapsed / K * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded 5-layer SPD permittivity/conductivity, Ricker Ex dipole)",
+        "config": {"workload": f"electromag3 384x384x192 + {cp}-cell CPML ({nx}x{ny}x{nz} padded), dx=0.05 m",
+                   "cells": ncell, "precision": args.precision,
+                   "material_layout": f"table ({nmat} materials, 1-byte id per cell)" if nmat else "per-cell arrays"},
+        "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": "em3 E update", "achieved": round(e_bytes / avg_e / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(e_bytes / avg_e / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes": e_bytes, "avg_launch_ms": round(avg_e * 1e3, 4),
+                     "h_kernel_avg_ms": round(ms_h / max(launches, 1), 4)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +180,8 @@ def main():
                     help="also time the per-cell-array layout and report it beside the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
+    ap.add_argument("--workload", default="seismic3", choices=["seismic3", "electromag3"],
+                    help="seismic3: the 3D elastic 256^3 case (default); electromag3: the 3D EM 384x384x192 case")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -146,6 +190,10 @@ def main():
         print("bench.py: this build runs on one GPU only (one time-stepped grid per GPU); "
               "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "electromag3":
+        bench_em3(args)
+        return
 
     import numpy as np
     from seidart_hip import models

@@ -208,11 +208,12 @@ enum {
                                    concurrently with the interior kernels of the same half step,
                                    0 = one after the other on one stream (default; the two are
                                    equally fast at 280^3, where HBM is the shared limit)      */
-    SDT_OPT_MATERIAL_TABLE = 5  /* 3D elastic: 1 = when the model holds at most 256 distinct
-                                   (c11..c66, gamma_*, rho) tuples, keep one byte of material id
-                                   per cell and a table of the tuples instead of 28 per-cell
-                                   arrays (default; same values, results equal to rounding),
-                                   0 = always the per-cell arrays                          */
+    SDT_OPT_MATERIAL_TABLE = 5  /* 3D elastic and 3D EM: 1 = when the model holds at most 256
+                                   distinct material tuples ((c11..c66, gamma_*, rho), or
+                                   (inverse permittivity, conductivity)), keep one byte of
+                                   material id per cell and a table of the tuples instead of
+                                   the per-cell arrays (default; same values, results equal to
+                                   rounding), 0 = always the per-cell arrays                */
 };
 int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
 

@@ -89,66 +89,12 @@ int el3_build_shell(Device &d, const double *const *in, int kc);
 // Otherwise d.mat_table stays false and the per-cell arrays are used.
 static int el3_material_table(Device &d, const double *const *in)
 {
-    const int64_t n = d.ncell;
     const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};
-    struct Row {
-        double v[kTabRow];
-    };
-    auto fill = [&](int64_t c, Row &r) {
-        memset(&r, 0, sizeof(r));
-        for (int q = 0; q < 21; ++q) r.v[q] = in[SDT_C11 + q] ? in[SDT_C11 + q][c] : 0.0;
-        for (int q = 0; q < 6; ++q) r.v[kTabIg + q] = 1.0 / (1.0 + (in[gids[q]] ? in[gids[q]][c] : 0.0) * d.dt);
-        r.v[kTabRho] = in[SDT_RHO] ? in[SDT_RHO][c] : 0.0;
-    };
-    auto hash = [](const Row &r) {
-        uint64_t h = 1469598103934665603ull;
-        for (int q = 0; q < kTabRow; ++q) {
-            uint64_t w;
-            memcpy(&w, &r.v[q], 8);
-            h = (h ^ w) * 1099511628211ull;
-            h ^= h >> 29;
-        }
-        return h;
-    };
-    std::vector<Row> rows;
-    std::unordered_multimap<uint64_t, int> index;
-    std::vector<uint8_t> ids((size_t)n);
-    Row cur, prev;
-    int prev_id = -1;
-    for (int64_t c = 0; c < n; ++c) {
-        fill(c, cur);
-        if (prev_id >= 0 && memcmp(&cur, &prev, sizeof(Row)) == 0) {
-            ids[(size_t)c] = (uint8_t)prev_id;
-            continue;
-        }
-        const uint64_t h = hash(cur);
-        int id = -1;
-        auto range = index.equal_range(h);
-        for (auto it = range.first; it != range.second; ++it)
-            if (memcmp(&rows[(size_t)it->second], &cur, sizeof(Row)) == 0) { id = it->second; break; }
-        if (id < 0) {
-            if ((int)rows.size() == kMaxMaterials) return SDT_OK;   // too many: per-cell arrays
-            id = (int)rows.size();
-            rows.push_back(cur);
-            index.emplace(h, id);
-        }
-        ids[(size_t)c] = (uint8_t)id;
-        prev = cur;
-        prev_id = id;
-    }
-    // ids in the padded layout (1 byte per element)
-    const size_t bytes = (size_t)d.ncell_dev;
-    if (hipMalloc((void **)&d.mid, bytes) != hipSuccess) return SDT_ERR_HIP;
-    if (hipMemset(d.mid, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
-    if (hipMemcpy2D(d.mid + d.xoff, (size_t)d.pitch, ids.data(), (size_t)d.nx, (size_t)d.nx,
-                    (size_t)d.ny * d.nz, hipMemcpyHostToDevice) != hipSuccess)
-        return SDT_ERR_HIP;
-    std::vector<double> flat(rows.size() * kTabRow);
-    for (size_t r = 0; r < rows.size(); ++r) memcpy(&flat[r * kTabRow], rows[r].v, sizeof(Row));
-    if (upload_converted(d, flat.data(), &d.tab, (int64_t)flat.size()) != SDT_OK) return SDT_ERR_HIP;
-    d.nmaterials = (int)rows.size();
-    d.mat_table = true;
-    return SDT_OK;
+    return build_material_table(d, kTabRow, [&](int64_t c, double *r) {
+        for (int q = 0; q < 21; ++q) r[q] = in[SDT_C11 + q] ? in[SDT_C11 + q][c] : 0.0;
+        for (int q = 0; q < 6; ++q) r[kTabIg + q] = 1.0 / (1.0 + (in[gids[q]] ? in[gids[q]][c] : 0.0) * d.dt);
+        r[kTabRho] = in[SDT_RHO] ? in[SDT_RHO][c] : 0.0;
+    });
 }
 
 int el3_setup(Device &d, const double *const *in)

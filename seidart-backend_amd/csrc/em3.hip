@@ -33,7 +33,52 @@ struct Em3Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t src_cell;
+    // material table (at most 256 distinct materials): id per padded cell,
+    // rows of kEmRow = aEx bEx cEx bEy cEy cEz s11 s12 s13 s22 s23 s33
+    const uint8_t *mid;
+    const T *tab;
 };
+
+constexpr int kEmRow = 16;
+
+template <typename T>
+using em_const_row = __attribute__((address_space(4))) const T *;
+
+// Material accessors for the E update: entry q of the row above.
+template <typename T>
+struct EmScalarRow {   // wave-uniform row: scalar loads through the constant address space
+    em_const_row<T> p;
+    __device__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct EmLaneRow {     // per-lane row from the (cached) table
+    const T *p;
+    __device__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct EmDenseRow {    // per-cell arrays
+    const Em3Args<T> *A;
+    int64_t c;
+    __device__ T operator[](int q) const
+    {
+        const T *const arr[12] = {A->aEx, A->bEx, A->cEx, A->bEy, A->cEy, A->cEz,
+                                  A->s11, A->s12, A->s13, A->s22, A->s23, A->s33};
+        return arr[q][c];
+    }
+};
+
+// E += dt * inv(eps) * (curl H - sigma E_old), cpmlfdtd.f08:3819-3828
+template <typename T, typename R>
+__device__ __forceinline__ void em3_e_update(const R &M, T exo, T eyo, T ezo, T cx, T cy, T cz, T dt, T &ex, T &ey,
+                                             T &ez)
+{
+    const T rx = cx - (M[6] * exo + M[7] * eyo + M[8] * ezo);
+    const T ry = cy - (M[7] * exo + M[9] * eyo + M[10] * ezo);
+    const T rz = cz - (M[8] * exo + M[10] * eyo + M[11] * ezo);
+    ex = exo + (M[0] * rx + M[1] * ry + M[2] * rz) * dt;
+    ey = eyo + (M[1] * rx + M[3] * ry + M[4] * rz) * dt;
+    ez = ezo + (M[2] * rx + M[4] * ry + M[5] * rz) * dt;
+}
 
 template <typename T, bool PML>
 __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
@@ -104,7 +149,7 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
     else em3_h_cell<T, false>(A, i, j, k);
 }
 
-template <typename T, bool PML>
+template <typename T, bool PML, bool TAB>
 __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
@@ -139,14 +184,21 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
             dHy_dx = dHy_dx * ikh + A.m_dHy_dx[c];
         }
         const T exo = A.E[0][c], eyo = A.E[1][c], ezo = A.E[2][c];
-        const T s11 = A.s11[c], s12 = A.s12[c], s13 = A.s13[c], s22 = A.s22[c], s23 = A.s23[c], s33 = A.s33[c];
-        const T rx = dHz_dy - dHy_dz - (s11 * exo + s12 * eyo + s13 * ezo);
-        const T ry = dHx_dz - dHz_dx - (s12 * exo + s22 * eyo + s23 * ezo);
-        const T rz = dHy_dx - dHx_dy - (s13 * exo + s23 * eyo + s33 * ezo);
-        const T aEx = A.aEx[c], bEx = A.bEx[c], cEx = A.cEx[c], bEy = A.bEy[c], cEy = A.cEy[c], cEz = A.cEz[c];
-        ex = exo + (aEx * rx + bEx * ry + cEx * rz) * A.dt;
-        ey = eyo + (bEx * rx + bEy * ry + cEy * rz) * A.dt;
-        ez = ezo + (cEx * rx + cEy * ry + cEz * rz) * A.dt;
+        const T cx = dHz_dy - dHy_dz, cy = dHx_dz - dHz_dx, cz = dHy_dx - dHx_dy;
+        if (TAB) {
+            const int m = A.mid[c];
+            const int m0 = __builtin_amdgcn_readfirstlane(m);
+            if (__all(m == m0)) {
+                const EmScalarRow<T> M{(em_const_row<T>)(A.tab) + m0 * kEmRow};
+                em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
+            } else {
+                const EmLaneRow<T> M{A.tab + m * kEmRow};
+                em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
+            }
+        } else {
+            const EmDenseRow<T> M{&A, c};
+            em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
+        }
     }
     A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
     if (c == A.src_cell) return 0.0;
@@ -154,7 +206,7 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
     return sqrt(x * x + y * y + z * z);
 }
 
-template <typename T>
+template <typename T, bool TAB>
 __global__ void __launch_bounds__(256) em3_e_kernel(Em3Args<T> A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
@@ -163,8 +215,8 @@ __global__ void __launch_bounds__(256) em3_e_kernel(Em3Args<T> A)
     double nrm = 0.0;
     if (i <= A.nx && j <= A.ny) {
         const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = em3_e_cell<T, true>(A, i, j, k);
-        else nrm = em3_e_cell<T, false>(A, i, j, k);
+        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = em3_e_cell<T, true, TAB>(A, i, j, k);
+        else nrm = em3_e_cell<T, false, TAB>(A, i, j, k);
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
@@ -180,6 +232,8 @@ static Em3Args<T> em3_args(Device &d)
     const T **mp[] = {&A.aEx, &A.bEx, &A.cEx, &A.bEy, &A.cEy, &A.cEz,
                       &A.s11, &A.s12, &A.s13, &A.s22, &A.s23, &A.s33};
     for (int q = 0; q < 12; ++q) *mp[q] = (const T *)d.mat[q];
+    A.mid = d.mid;
+    A.tab = (const T *)d.tab;
     A.a = (const T *)d.cpml[0];
     A.b = (const T *)d.cpml[1];
     A.ik = (const T *)d.cpml[2];
@@ -213,7 +267,8 @@ static void em3_step_t(Device &d, bool timed)
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
     hipLaunchKernelGGL(em3_h_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL(em3_e_kernel<T>, grid, block, 0, d.stream, A);
+    if (d.mat_table) hipLaunchKernelGGL((em3_e_kernel<T, true>), grid, block, 0, d.stream, A);
+    else hipLaunchKernelGGL((em3_e_kernel<T, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }
@@ -235,31 +290,45 @@ int em3_setup(Device &d, const double *const *in)
     if (rc) return rc;
     set_layout(d);
     const int64_t n = d.ncell;
-    std::vector<double> co[6];
-    for (auto &v : co) v.resize((size_t)n);
     auto val = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
-    for (int64_t c = 0; c < n; ++c) {
+    auto inv_eps = [&](int64_t c, double *co) {
         const double e11 = val(SDT_E11, c) * kEps0, e12 = val(SDT_E12, c) * kEps0, e13 = val(SDT_E13, c) * kEps0;
         const double e22 = val(SDT_E22, c) * kEps0, e23 = val(SDT_E23, c) * kEps0, e33 = val(SDT_E33, c) * kEps0;
         const double det = e11 * (e22 * e33 - e23 * e23) - e12 * (e12 * e33 - e23 * e13) +
                            e13 * (e12 * e23 - e22 * e13);
-        co[0][(size_t)c] = (e22 * e33 - e23 * e23) / det;
-        co[1][(size_t)c] = -(e12 * e33 - e23 * e13) / det;
-        co[2][(size_t)c] = (e12 * e23 - e22 * e13) / det;
-        co[3][(size_t)c] = (e11 * e33 - e13 * e13) / det;
-        co[4][(size_t)c] = -(e11 * e23 - e12 * e13) / det;
-        co[5][(size_t)c] = (e11 * e22 - e12 * e12) / det;
-    }
+        co[0] = (e22 * e33 - e23 * e23) / det;
+        co[1] = -(e12 * e33 - e23 * e13) / det;
+        co[2] = (e12 * e23 - e22 * e13) / det;
+        co[3] = (e11 * e33 - e13 * e13) / det;
+        co[4] = -(e11 * e23 - e12 * e13) / det;
+        co[5] = (e11 * e22 - e12 * e12) / det;
+    };
     d.nmat = 0;
-    for (int q = 0; q < 6; ++q) {
-        rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
-        if (rc) return rc;
-    }
-    std::vector<double> zero;
-    for (int id = SDT_S11; id <= SDT_S33; ++id) {
-        if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
-        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
-        if (rc) return rc;
+    // few distinct materials (layered / blocky models): one-byte ids and a
+    // table of (inverse permittivity, conductivity) rows
+    rc = d.mat_table_opt ? build_material_table(d, kEmRow, [&](int64_t c, double *r) {
+        inv_eps(c, r);
+        for (int q = 0; q < 6; ++q) r[6 + q] = val(SDT_S11 + q, c);
+    }) : SDT_OK;
+    if (rc) return rc;
+    if (!d.mat_table) {
+        std::vector<double> co[6];
+        for (auto &v : co) v.resize((size_t)n);
+        double r[6];
+        for (int64_t c = 0; c < n; ++c) {
+            inv_eps(c, r);
+            for (int q = 0; q < 6; ++q) co[q][(size_t)c] = r[q];
+        }
+        for (int q = 0; q < 6; ++q) {
+            rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
+        std::vector<double> zero;
+        for (int id = SDT_S11; id <= SDT_S33; ++id) {
+            if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
+            rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
     }
     d.nmem = 12;
     for (int q = 0; q < d.nmem; ++q) {

@@ -4,6 +4,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <unordered_map>
 #include <vector>
 
 #include "sdt_device.h"
@@ -214,6 +216,60 @@ int download_grid(Device &d, const void *dev, double *host, int ncomp)
             return SDT_ERR_HIP;
         for (size_t i = 0; i < nv; ++i) host[i] = tmp[i];
     }
+    return SDT_OK;
+}
+
+int build_material_table(Device &d, int row_len, const std::function<void(int64_t, double *)> &fill)
+{
+    const int64_t n = d.ncell;
+    const size_t rb = (size_t)row_len * sizeof(double);
+    auto hash = [&](const double *r) {
+        uint64_t h = 1469598103934665603ull;
+        for (int q = 0; q < row_len; ++q) {
+            uint64_t w;
+            memcpy(&w, &r[q], 8);
+            h = (h ^ w) * 1099511628211ull;
+            h ^= h >> 29;
+        }
+        return h;
+    };
+    std::vector<double> rows;                       // distinct rows, row_len each
+    std::unordered_multimap<uint64_t, int> index;
+    std::vector<uint8_t> ids((size_t)n);
+    std::vector<double> cur((size_t)row_len), prev((size_t)row_len);
+    int prev_id = -1;
+    for (int64_t c = 0; c < n; ++c) {
+        std::fill(cur.begin(), cur.end(), 0.0);
+        fill(c, cur.data());
+        if (prev_id >= 0 && memcmp(cur.data(), prev.data(), rb) == 0) {   // runs along x
+            ids[(size_t)c] = (uint8_t)prev_id;
+            continue;
+        }
+        const uint64_t h = hash(cur.data());
+        int id = -1;
+        auto range = index.equal_range(h);
+        for (auto it = range.first; it != range.second; ++it)
+            if (memcmp(&rows[(size_t)it->second * row_len], cur.data(), rb) == 0) { id = it->second; break; }
+        if (id < 0) {
+            if ((int)(rows.size() / row_len) == 256) return SDT_OK;   // too many: per-cell arrays
+            id = (int)(rows.size() / row_len);
+            rows.insert(rows.end(), cur.begin(), cur.end());
+            index.emplace(h, id);
+        }
+        ids[(size_t)c] = (uint8_t)id;
+        prev.swap(cur);
+        prev_id = id;
+    }
+    // ids in the padded layout (1 byte per element)
+    const size_t bytes = (size_t)d.ncell_dev;
+    if (hipMalloc((void **)&d.mid, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemset(d.mid, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy2D(d.mid + d.xoff, (size_t)d.pitch, ids.data(), (size_t)d.nx, (size_t)d.nx,
+                    (size_t)d.ny * d.nz, hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (upload_converted(d, rows.data(), &d.tab, (int64_t)rows.size()) != SDT_OK) return SDT_ERR_HIP;
+    d.nmaterials = (int)(rows.size() / row_len);
+    d.mat_table = true;
     return SDT_OK;
 }
 

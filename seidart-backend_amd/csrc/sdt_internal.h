@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "../../include/seidart_hip.h"
 
 namespace sdt {
@@ -179,5 +181,11 @@ int alloc_zeroed(Device &d, void **dev, int64_t n);
 // sized arrays streamed together do not begin on the same HBM channel.
 int alloc_grid(Device &d, void **dev, size_t bytes);
 void free_grids(Device &d);
+// Material dictionary: fill(c, row) writes the row_len material values of
+// host cell c (unused entries stay 0).  When the grid holds at most 256
+// distinct rows, d.mid (one byte per padded cell), d.tab (rows of row_len in
+// the handle's precision), d.nmaterials and d.mat_table are set; otherwise
+// nothing is allocated and d.mat_table stays false.
+int build_material_table(Device &d, int row_len, const std::function<void(int64_t, double *)> &fill);
 
 }  // namespace sdt

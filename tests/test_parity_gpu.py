@@ -181,12 +181,23 @@ def test_em2_fp64():
     _check(g, o, TOL64)
 
 
-def test_em3_fp64():
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+def test_em3_fp64(options):
     from seidart_hip import models
     cfg = models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)
     cfg["sources"]["y"] = 0.5 * cfg["sources"]["x"]
     cfg["sources"]["z"] = -0.25 * cfg["sources"]["x"]
-    g, o = _run_both(cfg, ("ex", "ey", "ez", "hx", "hy", "hz"))
+    g, o = _run_both(cfg, ("ex", "ey", "ez", "hx", "hy", "hz"), options=options)
+    _check(g, o, TOL64)
+
+
+def test_em3_heterogeneous_permittivity():
+    """More than 256 distinct materials: the per-cell arrays are used."""
+    from seidart_hip import models
+    cfg = models.em3(nx=30, ny=28, nz=26, cpml=5, steps=120, src=(15, 14, 9), nrec=8)
+    rng = np.random.default_rng(11)
+    cfg["arrays"]["e11"] = cfg["arrays"]["e11"] * (1.0 + 0.05 * rng.random(cfg["shape"]))
+    g, o = _run_both(cfg, ("ex", "hz"))
     _check(g, o, TOL64)
 
 
