@@ -181,7 +181,7 @@ def test_em2_fp64():
     _check(g, o, TOL64)
 
 
-@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"zchunk": 5}, {"zchunk": 1, "material_table": 0}])
 def test_em3_fp64(options):
     from seidart_hip import models
     cfg = models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)
