@@ -66,6 +66,7 @@ def timed_run(cfg, args, material_table):
     s.set_option("tile_rows", args.tile_rows)
     s.set_option("material_table", material_table)
     s.set_option("concurrent", args.concurrent)
+    s.set_option("xcd_remap", args.xcd_remap)
     s.set_arrays(cfg["arrays"])
     for comp, series in cfg["sources"].items():
         s.set_source(comp, series)
@@ -176,6 +177,8 @@ def main():
                     help="1: material ids + table when the model allows (default), 0: per-cell arrays")
     ap.add_argument("--concurrent", type=int, default=0,
                     help="1: CPML-shell kernels on a second stream beside the interior kernels")
+    ap.add_argument("--xcd-remap", type=int, default=0,
+                    help="3D elastic block order: bit 0 interior, bit 1 shell (XCD-contiguous tile ranges)")
     ap.add_argument("--compare-layouts", type=int, default=1,
                     help="also time the per-cell-array layout and report it beside the main line")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -208,6 +208,10 @@ enum {
                                    concurrently with the interior kernels of the same half step,
                                    0 = one after the other on one stream (default; the two are
                                    equally fast at 280^3, where HBM is the shared limit)      */
+    SDT_OPT_XCD_REMAP = 7,      /* 3D elastic block order: bit 0 interior kernels, bit 1 shell
+                                   kernels; set = each XCD takes a contiguous range of tiles,
+                                   clear = the dispatcher's round-robin order (default 0: the
+                                   round-robin order measured faster at 280^3)               */
     SDT_OPT_MATERIAL_TABLE = 5  /* 3D elastic and 3D EM: 1 = when the model holds at most 256
                                    distinct material tuples ((c11..c66, gamma_*, rho), or
                                    (inverse permittivity, conductivity)), keep one byte of

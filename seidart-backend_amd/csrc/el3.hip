@@ -38,6 +38,7 @@ static El3Args<T> el3_args(Device &d)
     A.mid = d.mid;
     A.tab = (const T *)d.tab;
     A.shell = (T *)d.shell;
+    A.remap = d.xcd_remap;
     A.snq = d.shell_snq;
     A.cid = d.shell_cid;
     A.ctab = (const T *)d.shell_ctab;

@@ -41,6 +41,7 @@ struct El3Args {
     T rd24[3];      // 1/(24 dx), 1/(24 dy), 1/(24 dz)
     T dt;
     int density_code, corrected;
+    int remap;      // bit 0: XCD-contiguous block order of the interior kernels, bit 1: of the shell kernels
     const int *d_it;
     unsigned long long *vnorm, *snorm;
     int64_t vfix[3], sfix[4];

@@ -114,7 +114,8 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
 {
     const int i0 = A.box[0], j0 = A.box[2];
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned L = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    const unsigned b0 = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned L = (A.remap & 1) ? xcd_remap(b0, nb) : b0;
     const unsigned bx = L % gridDim.x, by = (L / gridDim.x) % gridDim.y, bz = L / (gridDim.x * gridDim.y);
     const int i = i0 + bx * 64 + threadIdx.x;
     const int j = j0 + by * TY + threadIdx.y;
@@ -191,7 +192,8 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 {
     const int i0 = A.box[0], j0 = A.box[2];
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned L = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    const unsigned b0 = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const unsigned L = (A.remap & 1) ? xcd_remap(b0, nb) : b0;
     const unsigned bx = L % gridDim.x, by = (L / gridDim.x) % gridDim.y, bz = L / (gridDim.x * gridDim.y);
     const int i = i0 + bx * 64 + threadIdx.x;
     const int j = j0 + by * TY + threadIdx.y;
@@ -478,7 +480,7 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
 {
     __shared__ T D[2][3 * K_NKIND * kShellLanes];
     __shared__ double N[2][3 * kShellLanes];
-    const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    const El3Tile tl = tiles[(A.remap & 2) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x];
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int lane = threadIdx.x & (kShellLanes - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kShellLanes);
@@ -612,7 +614,7 @@ template <typename T, bool CORR, bool TAB>
 __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3Args<T> A, const El3Tile *tiles)
 {
     __shared__ double V2[2][3 * kShellLanes];
-    const El3Tile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    const El3Tile tl = tiles[(A.remap & 2) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x];
     const int nz = A.nz;
     const int lane = threadIdx.x & (kShellLanes - 1);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kShellLanes);

@@ -525,6 +525,14 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
             h->graph = nullptr;
         }
         return SDT_OK;
+    case SDT_OPT_XCD_REMAP:
+        if (value < 0 || value > 3) return fail(h, SDT_ERR_ARG, "XCD remap option must be 0..3");
+        h->d.xcd_remap = (int)value;
+        if (h->graph) {
+            (void)hipGraphExecDestroy(h->graph);
+            h->graph = nullptr;
+        }
+        return SDT_OK;
     case SDT_OPT_MATERIAL_TABLE:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "material layout is fixed once the first step has run");
         if (value != 0 && value != 1) return fail(h, SDT_ERR_ARG, "material table option must be 0 or 1");

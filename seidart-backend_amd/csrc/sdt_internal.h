@@ -120,6 +120,7 @@ struct Device {
     hipStream_t side = nullptr;
     hipEvent_t fork[2] = {}, join[2] = {};
     bool concurrent = false;           // SDT_OPT_CONCURRENT
+    int xcd_remap = 0;                 // SDT_OPT_XCD_REMAP
     hipEvent_t ev[4] = {};
     bool timing = false;
     double ms_stress = 0, ms_vel = 0;

@@ -66,7 +66,8 @@ EXPORTS = [
     "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option", "sdt_get_info",
 ]
 
-OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT = 1, 2, 3, 4, 5, 6
+OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
+    1, 2, 3, 4, 5, 6, 7)
 INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES = 1, 2, 3
 
 

@@ -183,10 +183,11 @@ class Solver:
         'material_table' (1 = one-byte material ids + table when the 3D
         elastic model has <= 256 distinct materials, 0 = per-cell arrays) or
         'concurrent' (1 = 3D elastic shell kernels on a second stream beside
-        the interior kernels, 0 = sequential)."""
+        the interior kernels, 0 = sequential) or 'xcd_remap' (3D elastic block
+        order, bit 0 interior, bit 1 shell: XCD-contiguous tile ranges)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
              "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
-             "concurrent": L.OPT_CONCURRENT}[key]
+             "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     def info(self, key):
