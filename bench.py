@@ -54,12 +54,14 @@ def algorithmic_bytes(shape, cpml, w, table=False):
 
 def timed_run(cfg, args, material_table):
     """W untimed steps (the first uploads every input), then K steps timed by
-    the wall clock around synchronisations and, per kernel pass, by HIP events
-    on the solver's stream."""
+    the wall clock between synchronisations (captured step graphs, no per-
+    kernel events), then KT more steps with HIP events around each kernel pass
+    on the solver's stream for the per-pass durations."""
     import numpy as np
     from seidart_hip.solver import Solver
     K, W = args.steps, args.warmup
-    s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K, cfg["src"],
+    KT = max(1, min(K, 100))
+    s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K + KT, cfg["src"],
                precision=args.precision, density_method=cfg.get("density_method", "arithmetic"))
     s.set_option("zchunk", args.zchunk)
     s.set_option("stagger", args.stagger)
@@ -69,15 +71,18 @@ def timed_run(cfg, args, material_table):
     s.set_option("xcd_remap", args.xcd_remap)
     s.set_arrays(cfg["arrays"])
     for comp, series in cfg["sources"].items():
+        series = np.concatenate([np.asarray(series, dtype=np.float64), np.zeros(KT)])
         s.set_source(comp, series)
     s.set_receivers(cfg["receivers"])
     s.run(W)
     s.synchronize()
-    s.set_timing(True)
     t0 = time.perf_counter()
     s.run(K)
     s.synchronize()
     elapsed = time.perf_counter() - t0
+    s.set_timing(True)
+    s.run(KT)
+    s.synchronize()
     ms_stress, ms_vel, launches = s.kernel_times()
     nmat = s.materials()
     rec = s.receivers()
@@ -88,7 +93,7 @@ def timed_run(cfg, args, material_table):
 
 
 def measured_traffic():
-    """HBM bytes per stress-kernel launch from the committed rocprofv3 --pmc
+    """HBM bytes per stress-pass launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
     or None when no summary is present."""
     path = os.path.join(ROOT, "profiles", "el3_seismic3_280cube_fp64_hbm_traffic.txt")

@@ -5,7 +5,7 @@
  * The reference has no C-ABI: each solver is a Fortran subroutine taking
  * derived types and reading its inputs from files in the working directory
  * (src/fortran/cpmlfdtd.f08:57 seismic2, :600 seismic25, :1341 seismic3,
- * :2317 electromag2, :3353 electromag3).  The entry points below replace that contract one for
+ * :2317 electromag2, :2764 electromag25, :3353 electromag3).  The entry points below replace that contract one for
  * one:
  *
  *   Domain_Type / Source_Type (src/fortran/seidart_types.f08:13-33)
@@ -19,7 +19,8 @@
  *   (src/fortran/seidartio.f08:161-183, used at cpmlfdtd.f08:230-237,
  *   1606-1617, 2476-2479, 3581-3585)
  *       -> sdt_set_source
- *   the environment variable KY read by seismic25 (cpmlfdtd.f08:697-703)
+ *   the environment variable KY read by seismic25 / electromag25
+ *   (cpmlfdtd.f08:697-703, 2861-2867)
  *       -> sdt_params.ky
  *   the `do it = 1, time_steps` loops (cpmlfdtd.f08:311, 931, 1747, 2566,
  *   3689) -> sdt_run
@@ -42,7 +43,7 @@
  * duration of the call.  No entry point exits the process.  One HIP stream per
  * handle; handles are independent, a single handle is not thread-safe.
  *
- * Complex values (2.5D elastic, SDT_ELASTIC25): fields, the initial-condition
+ * Complex values (2.5D solvers, SDT_ELASTIC25 and SDT_EM25): fields, the initial-condition
  * arrays and receiver samples are complex128 stored as interleaved (re, im)
  * doubles, exactly numpy's complex128 memory layout; their counts are twice
  * the real ones.  Material, CPML and source arrays stay real.
@@ -74,6 +75,7 @@ enum {
     SDT_ELASTIC25 = 1, /* seismic25    cpmlfdtd.f08:600  */
     SDT_ELASTIC3 = 2,  /* seismic3     cpmlfdtd.f08:1341 */
     SDT_EM2 = 3,       /* electromag2  cpmlfdtd.f08:2317 */
+    SDT_EM25 = 4,      /* electromag25 cpmlfdtd.f08:2764 */
     SDT_EM3 = 5        /* electromag3  cpmlfdtd.f08:3353 */
 };
 
@@ -142,7 +144,7 @@ typedef struct sdt_params {
     int32_t stencil;       /* SDT_STENCIL_*                                 */
     int32_t device;        /* HIP device ordinal                            */
     int32_t reserved0;     /* zero                                          */
-    double ky;             /* 2.5D elastic: y wavenumber (env KY,
+    double ky;             /* 2.5D solvers: y wavenumber (env KY,
                               cpmlfdtd.f08:697-703); 0 otherwise             */
     int32_t reserved[6];   /* zero                                          */
 } sdt_params;
@@ -160,8 +162,8 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out);
 int32_t sdt_destroy(sdt_handle *h);
 
 /* inputs: count must equal nx*ny*nz (arrays) or time_steps (series); for
- * SDT_ELASTIC25 the SDT_IC_* arrays take either 2*nx*nz doubles (complex128,
- * initialconditionV?.dat read by material_rw2c) or nx*nz (real part only).
+ * the 2.5D solvers the SDT_IC_* arrays take either 2*nx*nz doubles (complex128,
+ * initialcondition[VE]?.dat read by material_rw2c) or nx*nz (real part only).
  * Arrays not set take the reference's defaults (kappa = 1, everything else 0,
  * cpmlfdtd.f08:240-247). */
 int32_t sdt_set_array(sdt_handle *h, int32_t array_id, const double *host, int64_t count);
@@ -177,11 +179,11 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step);
 int32_t sdt_steps_done(const sdt_handle *h, int32_t *steps);
 
 /* outputs (FP64 host copies; blocking); count = nx*ny*nz, or 2*nx*nz for the
- * complex fields of SDT_ELASTIC25 */
+ * complex fields of the 2.5D solvers */
 int32_t sdt_get_field(sdt_handle *h, int32_t field_id, double *out, int64_t count);
 int32_t sdt_set_field(sdt_handle *h, int32_t field_id, const double *host, int64_t count);
-/* out = steps_done * n_receivers * 3 values, step-major (complex for
- * SDT_ELASTIC25, i.e. 6 doubles per receiver and step) */
+/* out = steps_done * n_receivers * 3 values, step-major (complex for the
+ * 2.5D solvers, i.e. 6 doubles per receiver and step) */
 int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count);
 /* per-step max norms: velocity (or E) and, for 3D elastic, stress; NULL skips */
 int32_t sdt_get_norms(sdt_handle *h, double *vel, double *stress, int64_t count);

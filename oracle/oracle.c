@@ -1165,3 +1165,173 @@ int32_t oracle_seismic25(oracle_run *r)
     for (int b = 0; b < 6; ++b) free(MV[b]);
     return r->unstable_step ? SDT_ERR_UNSTABLE : SDT_OK;
 }
+
+/* ========================================================================
+ * electromag25: cpmlfdtd.f08:2764-3350 (time loop 3095-3330).  Complex E and
+ * H on the (x, z) plane, y derivatives i*ky*f; 4th-order staggered
+ * differences; H-side memory variables with the half coefficient set, E-side
+ * with the full set (3108-3228); sigma applied to E_old (3209-3225).
+ * ======================================================================== */
+int32_t oracle_electromag25(oracle_run *r)
+{
+    set_threads(r);
+    grid_t g = {r->nx, 1, r->nz, (size_t)r->nx * (size_t)r->nz};
+    const int nx = g.nx, nz = g.nz;
+    const double dx = r->dx, dz = r->dz, dt = r->dt, ky = r->ky;
+    const double hx = 24.0 * dx, hz = 24.0 * dz;
+    const size_t n = g.n;
+    const double dbH = dt / O_MU0;
+
+    double *eps[6], *sig[6];
+    for (int q = 0; q < 6; ++q) {
+        eps[q] = input_copy(r, SDT_E11 + q, n);
+        sig[q] = input_copy(r, SDT_S11 + q, n);
+    }
+    double *eps11 = eps[0], *eps12 = eps[1], *eps13 = eps[2], *eps22 = eps[3], *eps23 = eps[4], *eps33 = eps[5];
+    double *sig11 = sig[0], *sig12 = sig[1], *sig13 = sig[2], *sig22 = sig[3], *sig23 = sig[4], *sig33 = sig[5];
+    double *kappa = input_copy(r, SDT_KAPPA, n), *acoef = input_copy(r, SDT_ACOEF, n),
+           *bcoef = input_copy(r, SDT_BCOEF, n);
+    double *kappa_half = input_copy(r, SDT_KAPPA_HALF, n), *acoef_half = input_copy(r, SDT_ACOEF_HALF, n),
+           *bcoef_half = input_copy(r, SDT_BCOEF_HALF, n);
+    /* eps scaled to absolute values, adjugate/determinant (cpmlfdtd.f08:3040-3066) */
+    double *aEx = alloc_zero(n), *bEx = alloc_zero(n), *cEx = alloc_zero(n), *bEy = alloc_zero(n),
+           *cEy = alloc_zero(n), *cEz = alloc_zero(n);
+    for (size_t c = 0; c < n; ++c) {
+        for (int q = 0; q < 6; ++q) eps[q][c] *= O_EPS0;
+        const double det = eps11[c] * (eps22[c] * eps33[c] - eps23[c] * eps23[c]) -
+                           eps12[c] * (eps12[c] * eps33[c] - eps23[c] * eps13[c]) +
+                           eps13[c] * (eps12[c] * eps23[c] - eps22[c] * eps13[c]);
+        aEx[c] = (eps22[c] * eps33[c] - eps23[c] * eps23[c]) / det;
+        bEx[c] = -(eps12[c] * eps33[c] - eps23[c] * eps13[c]) / det;
+        cEx[c] = (eps12[c] * eps23[c] - eps22[c] * eps13[c]) / det;
+        bEy[c] = (eps11[c] * eps33[c] - eps13[c] * eps13[c]) / det;
+        cEy[c] = -(eps11[c] * eps23[c] - eps12[c] * eps13[c]) / det;
+        cEz[c] = (eps11[c] * eps22[c] - eps12[c] * eps12[c]) / det;
+    }
+    const double *aEy = bEx, *aEz = cEx, *bEz = cEy;
+
+    cplx_t *F[6];
+    for (int f = 0; f < 6; ++f) F[f] = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t));
+    cplx_t *Ex = F[SDT_F_EX], *Ey = F[SDT_F_EY], *Ez = F[SDT_F_EZ], *Hx = F[SDT_F_HX], *Hy = F[SDT_F_HY],
+           *Hz = F[SDT_F_HZ];
+    for (int f = 0; f < 6; ++f) {
+        const double *in = r->field_in[f] ? r->field_in[f] : (f < 3 ? r->arr[SDT_IC_X + f] : NULL);
+        if (in) memcpy(F[f], in, n * sizeof(cplx_t));
+    }
+    cplx_t *Exo = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)), *Eyo = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)),
+           *Ezo = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t));
+    memcpy(Exo, Ex, n * sizeof(cplx_t));
+    memcpy(Eyo, Ey, n * sizeof(cplx_t));
+    memcpy(Ezo, Ez, n * sizeof(cplx_t));
+    cplx_t *m_dEy_dx = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)), *m_dEy_dz = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)),
+           *m_dEx_dz = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)), *m_dEz_dx = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)),
+           *m_dHz_dx = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)), *m_dHx_dz = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)),
+           *m_dHy_dz = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t)), *m_dHy_dx = (cplx_t *)calloc(n ? n : 1, sizeof(cplx_t));
+
+    const int is = r->isrc, ks = r->ksrc;
+    r->unstable_step = 0;
+    for (int it = 1; it <= r->nsteps; ++it) {
+        /* Hx, cpmlfdtd.f08:3106-3116 */
+#pragma omp parallel for schedule(static)
+        for (int k = 2; k <= nz - 2; ++k)
+            for (int i = 2; i <= nx - 1; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t d = cd4(Ey[I2(i, k - 1)], Ey[c], Ey[I2(i, k + 1)], Ey[I2(i, k + 2)], hz);
+                d = ccpml(d, &m_dEy_dz[c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                const cplx_t t = ciky(ky, Ez[c]);
+                Hx[c] = cx(Hx[c].re - dbH * (t.re - d.re), Hx[c].im - dbH * (t.im - d.im));
+            }
+        /* Hy, cpmlfdtd.f08:3131-3146 */
+#pragma omp parallel for schedule(static)
+        for (int k = 2; k <= nz - 2; ++k)
+            for (int i = 2; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t dxz = cd4(Ex[I2(i, k - 1)], Ex[c], Ex[I2(i, k + 1)], Ex[I2(i, k + 2)], hz);
+                cplx_t dzx = cd4(Ez[I2(i - 1, k)], Ez[c], Ez[I2(i + 1, k)], Ez[I2(i + 2, k)], hx);
+                dxz = ccpml(dxz, &m_dEx_dz[c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                dzx = ccpml(dzx, &m_dEz_dx[c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                Hy[c] = cx(Hy[c].re - dbH * (dxz.re - dzx.re), Hy[c].im - dbH * (dxz.im - dzx.im));
+            }
+        /* Hz, cpmlfdtd.f08:3160-3170 */
+#pragma omp parallel for schedule(static)
+        for (int k = 2; k <= nz - 1; ++k)
+            for (int i = 2; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                cplx_t d = cd4(Ey[I2(i - 1, k)], Ey[c], Ey[I2(i + 1, k)], Ey[I2(i + 2, k)], hx);
+                d = ccpml(d, &m_dEy_dx[c], acoef_half[c], bcoef_half[c], kappa_half[c]);
+                const cplx_t t = ciky(ky, Ex[c]);
+                Hz[c] = cx(Hz[c].re - dbH * (d.re - t.re), Hz[c].im - dbH * (d.im - t.im));
+            }
+        /* E, cpmlfdtd.f08:3190-3228 */
+#pragma omp parallel for schedule(static)
+        for (int k = 3; k <= nz - 2; ++k)
+            for (int i = 3; i <= nx - 2; ++i) {
+                const size_t c = I2(i, k);
+                const cplx_t dHz_dy = ciky(ky, Hz[c]);
+                cplx_t dHy_dz = cd4(Hy[I2(i, k - 2)], Hy[I2(i, k - 1)], Hy[c], Hy[I2(i, k + 1)], hz);
+                cplx_t dHx_dz = cd4(Hx[I2(i, k - 2)], Hx[I2(i, k - 1)], Hx[c], Hx[I2(i, k + 1)], hz);
+                cplx_t dHz_dx = cd4(Hz[I2(i - 2, k)], Hz[I2(i - 1, k)], Hz[c], Hz[I2(i + 1, k)], hx);
+                cplx_t dHy_dx = cd4(Hy[I2(i - 2, k)], Hy[I2(i - 1, k)], Hy[c], Hy[I2(i + 1, k)], hx);
+                const cplx_t dHx_dy = ciky(ky, Hx[c]);
+                dHy_dz = ccpml(dHy_dz, &m_dHy_dz[c], acoef[c], bcoef[c], kappa[c]);
+                dHx_dz = ccpml(dHx_dz, &m_dHx_dz[c], acoef[c], bcoef[c], kappa[c]);
+                dHz_dx = ccpml(dHz_dx, &m_dHz_dx[c], acoef[c], bcoef[c], kappa[c]);
+                dHy_dx = ccpml(dHy_dx, &m_dHy_dx[c], acoef[c], bcoef[c], kappa[c]);
+                const cplx_t eo[3] = {Exo[c], Eyo[c], Ezo[c]};
+                cplx_t rx, ry, rz;
+                rx.re = dHz_dy.re - dHy_dz.re - (sig11[c] * eo[0].re + sig12[c] * eo[1].re + sig13[c] * eo[2].re);
+                rx.im = dHz_dy.im - dHy_dz.im - (sig11[c] * eo[0].im + sig12[c] * eo[1].im + sig13[c] * eo[2].im);
+                ry.re = dHx_dz.re - dHz_dx.re - (sig12[c] * eo[0].re + sig22[c] * eo[1].re + sig23[c] * eo[2].re);
+                ry.im = dHx_dz.im - dHz_dx.im - (sig12[c] * eo[0].im + sig22[c] * eo[1].im + sig23[c] * eo[2].im);
+                rz.re = dHy_dx.re - dHx_dy.re - (sig13[c] * eo[0].re + sig23[c] * eo[1].re + sig33[c] * eo[2].re);
+                rz.im = dHy_dx.im - dHx_dy.im - (sig13[c] * eo[0].im + sig23[c] * eo[1].im + sig33[c] * eo[2].im);
+                Ex[c] = cx(eo[0].re + (aEx[c] * rx.re + bEx[c] * ry.re + cEx[c] * rz.re) * dt,
+                           eo[0].im + (aEx[c] * rx.im + bEx[c] * ry.im + cEx[c] * rz.im) * dt);
+                Ey[c] = cx(eo[1].re + (aEy[c] * rx.re + bEy[c] * ry.re + cEy[c] * rz.re) * dt,
+                           eo[1].im + (aEy[c] * rx.im + bEy[c] * ry.im + cEy[c] * rz.im) * dt);
+                Ez[c] = cx(eo[2].re + (aEz[c] * rx.re + bEz[c] * ry.re + cEz[c] * rz.re) * dt,
+                           eo[2].im + (aEz[c] * rx.im + bEz[c] * ry.im + cEz[c] * rz.im) * dt);
+            }
+        /* point source, cpmlfdtd.f08:3236-3241 (real part) */
+        Ex[I2(is, ks)].re += src_at(r, SDT_SRC_X, it) * dt / eps11[I2(is, ks)];
+        Ey[I2(is, ks)].re += src_at(r, SDT_SRC_Y, it) * dt / eps22[I2(is, ks)];
+        Ez[I2(is, ks)].re += src_at(r, SDT_SRC_Z, it) * dt / eps33[I2(is, ks)];
+        /* Dirichlet edges on E and H, cpmlfdtd.f08:3246-3274 */
+        for (int f = 0; f < 6; ++f) {
+            for (int k = 1; k <= nz; ++k) { F[f][I2(1, k)] = cx(0, 0); F[f][I2(nx, k)] = cx(0, 0); }
+            for (int i = 1; i <= nx; ++i) { F[f][I2(i, 1)] = cx(0, 0); F[f][I2(i, nz)] = cx(0, 0); }
+        }
+        memcpy(Exo, Ex, n * sizeof(cplx_t));
+        memcpy(Eyo, Ey, n * sizeof(cplx_t));
+        memcpy(Ezo, Ez, n * sizeof(cplx_t));
+        /* norm, cpmlfdtd.f08:3309-3310 */
+        double velocnorm = 0.0;
+#pragma omp parallel for reduction(max : velocnorm) schedule(static)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                const size_t c = I2(i, k);
+                const double s = (Ex[c].re * Ex[c].re + Ex[c].im * Ex[c].im) + (Ey[c].re * Ey[c].re + Ey[c].im * Ey[c].im) +
+                                 (Ez[c].re * Ez[c].re + Ez[c].im * Ez[c].im);
+                const double v = sqrt(s);
+                if (v > velocnorm) velocnorm = v;
+            }
+        if (r->velnorm) r->velnorm[it - 1] = velocnorm;
+        if (r->rec_out)
+            for (int q = 0; q < r->nrec; ++q) {
+                const size_t c = I2(r->rec[3 * q], r->rec[3 * q + 2]);
+                double *o = r->rec_out + ((size_t)(it - 1) * r->nrec + q) * 6;
+                for (int m = 0; m < 3; ++m) { o[2 * m] = F[m][c].re; o[2 * m + 1] = F[m][c].im; }
+            }
+        if (velocnorm > O_STABILITY) { r->unstable_step = it; break; }
+    }
+    for (int f = 0; f < 6; ++f)
+        if (r->field_out[f]) memcpy(r->field_out[f], F[f], n * sizeof(cplx_t));
+
+    for (int q = 0; q < 6; ++q) { free(eps[q]); free(sig[q]); }
+    double *all[] = {kappa, acoef, bcoef, kappa_half, acoef_half, bcoef_half, aEx, bEx, cEx, bEy, cEy, cEz};
+    for (size_t q = 0; q < sizeof(all) / sizeof(all[0]); ++q) free(all[q]);
+    for (int f = 0; f < 6; ++f) free(F[f]);
+    cplx_t *cm[] = {Exo, Eyo, Ezo, m_dEy_dx, m_dEy_dz, m_dEx_dz, m_dEz_dx, m_dHz_dx, m_dHx_dz, m_dHy_dz, m_dHy_dx};
+    for (size_t q = 0; q < sizeof(cm) / sizeof(cm[0]); ++q) free(cm[q]);
+    return r->unstable_step ? SDT_ERR_UNSTABLE : SDT_OK;
+}

@@ -46,6 +46,9 @@ typedef struct oracle_run {
 
 int32_t oracle_seismic2(oracle_run *r);
 int32_t oracle_seismic25(oracle_run *r);
+/* 2.5D EM (electromag25): E and H complex, same conventions as seismic25;
+ * rec_out = (re, im) of Ex, Ey, Ez */
+int32_t oracle_electromag25(oracle_run *r);
 int32_t oracle_seismic3(oracle_run *r);
 int32_t oracle_electromag2(oracle_run *r);
 int32_t oracle_electromag3(oracle_run *r);

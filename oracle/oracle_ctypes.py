@@ -60,7 +60,7 @@ def _load():
         build()
     lib = C.CDLL(LIB_PATH)
     for name in ("oracle_seismic2", "oracle_seismic25", "oracle_seismic3", "oracle_electromag2",
-                 "oracle_electromag3"):
+                 "oracle_electromag25", "oracle_electromag3"):
         fn = getattr(lib, name)
         fn.argtypes = [C.POINTER(OracleRun)]
         fn.restype = C.c_int32
@@ -95,7 +95,8 @@ def array_averaging2d(a, code, direction):
     return out
 
 
-_ENTRY = {"elastic2": "oracle_seismic2", "elastic25": "oracle_seismic25", "elastic3": "oracle_seismic3", "em2": "oracle_electromag2",
+_ENTRY = {"elastic2": "oracle_seismic2", "elastic25": "oracle_seismic25", "em25": "oracle_electromag25",
+          "elastic3": "oracle_seismic3", "em2": "oracle_electromag2",
           "em3": "oracle_electromag3"}
 
 
@@ -104,7 +105,7 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
 
     Returns (receivers[steps, nrec, 3], (velnorm, stressnorm), {field: array},
     unstable_step).  For "elastic25" the receivers, fields and ic_* arrays are
-    complex128 and cfg["ky"] is the y wavenumber."""
+    complex128 and cfg["ky"] is the y wavenumber (likewise "em25")."""
     phys = cfg["physics"]
     is3d = phys.endswith("3")
     shape = tuple(cfg["shape"])
@@ -120,7 +121,7 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
         r.dy = 1.0
     r.dt = cfg["dt"]
     r.ky = float(cfg.get("ky", 0.0))
-    cplx = phys == "elastic25"
+    cplx = phys in ("elastic25", "em25")
     r.nsteps = nsteps
     cp = cfg["cpml"]
     src = cfg["src"]

@@ -178,7 +178,7 @@ int el3_setup(Device &d, const double *const *in)
     t.sfix[3] = cell(is, js + 1, ks + 1);
     t.rec_field[0] = SDT_F_VX; t.rec_field[1] = SDT_F_VY; t.rec_field[2] = SDT_F_VZ;
 
-    return el3_build_shell(d, in, d.zchunk > 0 ? d.zchunk : 16);
+    return el3_build_shell(d, in, d.zchunk > 0 ? d.zchunk : 8);
 }
 
 // CPML-shell tiles (see El3Tile) and their compact memory-variable and CPML

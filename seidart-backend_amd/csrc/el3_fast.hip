@@ -660,7 +660,7 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3
 }
 
 // ----------------------------------------------------------------- launchers
-static int el3_kc(const Device &d) { return d.zchunk > 0 ? d.zchunk : 16; }
+static int el3_kc(const Device &d) { return d.zchunk > 0 ? d.zchunk : 8; }
 
 static int el3_ty(const Device &d) { return d.tile_rows == 8 || d.tile_rows == 16 ? d.tile_rows : 4; }
 

@@ -56,7 +56,7 @@ static bool elastic(int physics)
     return physics == SDT_ELASTIC2 || physics == SDT_ELASTIC25 || physics == SDT_ELASTIC3;
 }
 // values per field element: 2 for the complex fields of the 2.5D solver
-static int ncomp_of(int physics) { return physics == SDT_ELASTIC25 ? 2 : 1; }
+static int ncomp_of(int physics) { return physics == SDT_ELASTIC25 || physics == SDT_EM25 ? 2 : 1; }
 
 static int nfields_of(int physics, int f)
 {
@@ -66,6 +66,7 @@ static int nfields_of(int physics, int f)
     case SDT_ELASTIC25:
     case SDT_ELASTIC3: return f >= 0 && f < 9;
     case SDT_EM2: return f == SDT_F_EX || f == SDT_F_EZ || f == SDT_F_HY;
+    case SDT_EM25: return f >= 0 && f < 6;
     case SDT_EM3: return f >= 0 && f < 6;
     }
     return 0;
@@ -92,7 +93,7 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out)
     *out = nullptr;
     if (p->abi_version != SDT_ABI_VERSION) return fail(nullptr, SDT_ERR_ARG, "abi_version mismatch");
     if (p->physics != SDT_ELASTIC2 && p->physics != SDT_ELASTIC25 && p->physics != SDT_ELASTIC3 &&
-        p->physics != SDT_EM2 && p->physics != SDT_EM3)
+        p->physics != SDT_EM2 && p->physics != SDT_EM25 && p->physics != SDT_EM3)
         return fail(nullptr, SDT_ERR_ARG, "unknown physics");
     if (!isfinite(p->ky)) return fail(nullptr, SDT_ERR_ARG, "ky must be finite");
     if (p->precision != SDT_FP64 && p->precision != SDT_FP32) return fail(nullptr, SDT_ERR_ARG, "unknown precision");
@@ -130,7 +131,7 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out)
     d.isrc = is; d.jsrc = js; d.ksrc = ks;
     d.density_code = p->density_code;
     d.stencil = p->stencil;
-    d.ky = p->physics == SDT_ELASTIC25 ? p->ky : 0.0;
+    d.ky = (p->physics == SDT_ELASTIC25 || p->physics == SDT_EM25) ? p->ky : 0.0;
     d.ncomp = ncomp_of(p->physics);
     d.ncell = (int64_t)p->nx * p->ny * p->nz;
     h->series.assign((size_t)SDT_NUM_SRC * p->time_steps, 0.0);
@@ -255,6 +256,7 @@ static int prepare(sdt_handle *h)
     case SDT_ELASTIC25: rc = el25_setup(d, in); break;
     case SDT_ELASTIC3: rc = el3_setup(d, in); break;
     case SDT_EM2: rc = em2_setup(d, in); break;
+    case SDT_EM25: rc = em25_setup(d, in); break;
     case SDT_EM3: rc = em3_setup(d, in); break;
     }
     if (rc) return fail(h, rc, "device setup failed (out of device memory?)");
@@ -302,14 +304,16 @@ static void enqueue_step(sdt_handle *h, bool timed)
     case SDT_ELASTIC25: el25_step(d, timed); break;
     case SDT_ELASTIC3: el3_step(d, timed); break;
     case SDT_EM2: em2_step(d, timed); break;
+    case SDT_EM25: em25_step(d, timed); break;
     case SDT_EM3: em3_step(d, timed); break;
     }
 }
 
-// Small grids are launch-bound: capture kGraphSteps steps once and replay.
-// The step index lives in device memory, so one graph serves every replay.
+// Capture kGraphSteps steps once and replay them: no per-kernel launch gaps
+// (small grids are launch-bound; large ones still lose a few microseconds per
+// launch).  The step index lives in device memory, so one graph serves every
+// replay.
 static constexpr int kGraphSteps = 32;
-static constexpr int64_t kGraphMaxCells = 4 * 1024 * 1024;
 
 static int ensure_graph(sdt_handle *h)
 {
@@ -359,7 +363,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
     if (rc) return rc;
     Device &d = h->d;
     HIPCHK(h, hipSetDevice(h->p.device));
-    const bool use_graph = d.graphs && !d.timing && d.ncell <= kGraphMaxCells;
+    const bool use_graph = d.graphs && !d.timing;
     const int chunk = 256;   // steps between blow-up checks
     int left = nsteps;
     while (left > 0) {

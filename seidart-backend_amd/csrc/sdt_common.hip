@@ -23,54 +23,52 @@ struct FieldPtrs {
     void *f[SDT_NUM_FIELDS];
 };
 
-// One workgroup.  Thread 0 injects the point sources in the reference's order
-// (e.g. cpmlfdtd.f08:519-523, 2208-2219, 2678-2681, 3909-3914) and folds the
-// source cells into the step's norms; then every thread records receivers;
-// finally thread 0 advances the device step counter.
+// One workgroup.  The point sources are injected in parallel, one thread per
+// term: every term of a solver adds to a different (field, cell) pair, so the
+// reference's sequential order (e.g. cpmlfdtd.f08:519-523, 2208-2219,
+// 2678-2681, 3909-3914) gives the same values.  After a barrier the source
+// cells' norms are folded in and the receivers recorded; then thread 0
+// advances the device step counter.
 template <typename T>
 __global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
 {
     const int it = *a.d_it;
-    if (threadIdx.x == 0) {
-        for (int q = 0; q < a.nterms; ++q) {
-            const SrcTerm &s = a.terms[q];
-            if (s.cell < 0 || s.on_face) continue;
+    const int t = threadIdx.x;
+    if (t < a.nterms) {
+        const SrcTerm &s = a.terms[t];
+        if (s.cell >= 0 && !s.on_face) {
             const double v = a.series[(int64_t)s.component * a.time_steps + (it - 1)];
             T *f = reinterpret_cast<T *>(fp.f[s.field]);
             const int64_t e = s.cell << a.cplx;   // complex: the real part
             f[e] = (T)((double)f[e] + (T)((v * s.mul) / s.div));
         }
-        for (int q = 0; q < a.nvfix; ++q) {
-            double ss = 0.0;
-            for (int c = 0; c < a.nvcomp; ++c) {
-                const T *f = reinterpret_cast<const T *>(fp.f[a.vcomp[c]]);
-                if (a.cplx) {
-                    const double re = (double)f[2 * a.vfix[q]], im = (double)f[2 * a.vfix[q] + 1];
-                    ss += re * re + im * im;
-                } else {
-                    const double v = (double)f[a.vfix[q]];
-                    ss += v * v;
-                }
-            }
-            atomic_max_nonneg(&a.vnorm[it - 1], sqrt(ss));
-        }
-        if (a.stress_norm) {
-            for (int q = 0; q < a.nsfix; ++q) {
-                const int64_t c = a.sfix[q];
-                const double sxx = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXX])[c];
-                const double syy = (double)reinterpret_cast<T *>(fp.f[SDT_F_SYY])[c];
-                const double szz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SZZ])[c];
-                const double sxy = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXY])[c];
-                const double sxz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXZ])[c];
-                const double syz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SYZ])[c];
-                const double s = sqrt(sxx * sxx + syy * syy + szz * szz +
-                                      2.0 * (sxy * sxy + syz * syz + sxz * sxz));
-                atomic_max_nonneg(&a.snorm[it - 1], s);
-            }
-        }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < a.nrec; q += blockDim.x) {
+    if (t < a.nvfix) {
+        double ss = 0.0;
+        for (int c = 0; c < a.nvcomp; ++c) {
+            const T *f = reinterpret_cast<const T *>(fp.f[a.vcomp[c]]);
+            if (a.cplx) {
+                const double re = (double)f[2 * a.vfix[t]], im = (double)f[2 * a.vfix[t] + 1];
+                ss += re * re + im * im;
+            } else {
+                const double v = (double)f[a.vfix[t]];
+                ss += v * v;
+            }
+        }
+        atomic_max_nonneg(&a.vnorm[it - 1], sqrt(ss));
+    } else if (a.stress_norm && t >= 64 && t < 64 + a.nsfix) {
+        const int64_t c = a.sfix[t - 64];
+        const double sxx = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXX])[c];
+        const double syy = (double)reinterpret_cast<T *>(fp.f[SDT_F_SYY])[c];
+        const double szz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SZZ])[c];
+        const double sxy = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXY])[c];
+        const double sxz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SXZ])[c];
+        const double syz = (double)reinterpret_cast<T *>(fp.f[SDT_F_SYZ])[c];
+        const double n = sqrt(sxx * sxx + syy * syy + szz * szz + 2.0 * (sxy * sxy + syz * syz + sxz * sxz));
+        atomic_max_nonneg(&a.snorm[it - 1], n);
+    }
+    for (int q = t; q < a.nrec; q += blockDim.x) {
         const int64_t c = a.rec_cell[q];
         const int nv = 1 << a.cplx;
         double *o = a.rec_out + ((int64_t)(it - 1) * a.nrec + q) * 3 * nv;
@@ -81,7 +79,7 @@ __global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
                                     : 0.0;
     }
     __syncthreads();
-    if (threadIdx.x == 0) *a.d_it = it + 1;
+    if (t == 0) *a.d_it = it + 1;
 }
 
 void launch_tail(Device &d)

@@ -140,6 +140,8 @@ int el25_setup(Device &d, const double *const *host_in);
 void el25_step(Device &d, bool timed);
 int el3_setup(Device &d, const double *const *host_in);
 void el3_step(Device &d, bool timed);
+int em25_setup(Device &d, const double *const *host_in);
+void em25_step(Device &d, bool timed);
 int em2_setup(Device &d, const double *const *host_in);
 void em2_step(Device &d, bool timed);
 int em3_setup(Device &d, const double *const *host_in);

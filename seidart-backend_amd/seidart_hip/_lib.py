@@ -16,7 +16,7 @@ ABI_VERSION = 2
 # status codes
 OK, ERR_UNSTABLE, ERR_ARG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4
 # solvers
-ELASTIC2, ELASTIC25, ELASTIC3, EM2, EM3 = 0, 1, 2, 3, 5
+ELASTIC2, ELASTIC25, ELASTIC3, EM2, EM25, EM3 = 0, 1, 2, 3, 4, 5
 # precision
 FP64, FP32 = 0, 1
 # density codes, averaging.f08:5-25

@@ -91,7 +91,8 @@ def main(argv=None):
                 print("Running 2.5D seismic model with", seis.time_steps, "time steps")
                 cpmlfdtd.seismic25(domain, seis)
             else:
-                raise NotImplementedError("electromag25 (2.5D EM) is not implemented in seidart_hip yet")
+                print("Running 2.5D electromagnetic model with", em.time_steps, "time steps")
+                cpmlfdtd.electromag25(domain, em)
         else:
             # the 2D solvers work on the (x, z) plane
             domain.ny = 1

@@ -4,6 +4,7 @@
     seismic25(domain, source)                                   cpmlfdtd.f08:600
     seismic3(domain, source, density_method, SINGLE_OUTPUT)     cpmlfdtd.f08:1341
     electromag2(domain, source, SINGLE_OUTPUT)                  cpmlfdtd.f08:2317
+    electromag25(domain, source)                                cpmlfdtd.f08:2764
     electromag3(domain, source, SINGLE_OUTPUT)                  cpmlfdtd.f08:3353
 
 Same argument order and meaning as the Fortran subroutines: `domain` holds
@@ -11,9 +12,10 @@ padded sizes (src/fortran/main.f08:69-71 pads before the call), every input
 is read from the working directory under the reference's file names, and a
 snapshot of every velocity / E component is written each step under the
 reference's names and formats.  3D elastic also writes velocity_norm.dat and
-stress_norm.dat (cpmlfdtd.f08:2282-2283).  The 2.5D solver reads KY,
-DENSITY_METHOD, SINGLE_OUTPUT, FDTD_LEGACY_OUTPUT and FDTD_STEPS_PER_BLOCK from
-the environment as the reference does and writes zstd blocks by default.
+stress_norm.dat (cpmlfdtd.f08:2282-2283).  The 2.5D solvers read KY,
+SINGLE_OUTPUT, FDTD_LEGACY_OUTPUT, FDTD_STEPS_PER_BLOCK (and seismic25
+DENSITY_METHOD) from the environment as the reference does and write zstd
+blocks by default.
 
 Differences from the reference, by design:
 * blow-up raises seidart_hip.UnstableError (the reference prints and
@@ -22,7 +24,8 @@ Differences from the reference, by design:
   reference's call to write_image stops at the end of step 1 for 3D grids
   (seidartio.f08:315-324);
 * plane-wave sources ('pw', 'pws', 'pwp', ...) are not supported yet and raise
-  NotImplementedError.
+  NotImplementedError (electromag25's 'pw' branch leaves its point-source
+  series uninitialised in the reference, cpmlfdtd.f08:2992-2996).
 """
 from __future__ import annotations
 
@@ -265,6 +268,48 @@ def seismic25(domain, source, *, workdir=".", precision="fp64", device=0, ky=Non
                 writer.add(vx, vy, vz)
             if legacy_output:
                 for a, ch in ((vx, "Vx"), (vy, "Vy"), (vz, "Vz")):
+                    path = os.path.join(workdir, fio.snapshot_name(ch, it, source.xind, source.zind))
+                    fio.write_record(path, a.real, np.float32 if single else np.float64)
+        if writer is not None:
+            writer.flush()
+    return writer.written if writer is not None else []
+
+
+def electromag25(domain, source, *, workdir=".", precision="fp64", device=0, ky=None, SINGLE_OUTPUT=None,
+                 legacy_output=None, steps_per_block=None):
+    """2.5D EM; inputs e11..e33 s11..s33 CPML, complex128
+    initialconditionE{x,y,z} and electromagneticsource{x,y,z}
+    (cpmlfdtd.f08:2896-3036).  Output: EM25.<first>-<last>.<xind>.<yind>.
+    <zind>.blk.zst blocks of the real parts of Ex, Ey, Ez, or with
+    FDTD_LEGACY_OUTPUT=TRUE per-step Ex/Ey/Ez images (cpmlfdtd.f08:3313-3327)."""
+    _check_source_type(source)
+    ky = _env_ky() if ky is None else float(ky)
+    single = _env_logical("SINGLE_OUTPUT") if SINGLE_OUTPUT is None else bool(SINGLE_OUTPUT)
+    if legacy_output is None:
+        legacy_output = os.environ.get("FDTD_LEGACY_OUTPUT", "").strip() == "TRUE"
+    shape = (domain.nx, domain.nz)
+    writer = None
+    if not legacy_output:
+        spb = steps_per_block or fio.steps_per_block_2d(domain.nx, domain.nz, domain.cpml)
+        writer = fio.BlockWriter2D(domain.nx, domain.nz, domain.cpml, spb, source.time_steps, "EM25",
+                                   source.xind, source.yind, source.zind, workdir)
+    with Solver("em25", shape, (domain.dx, domain.dz), domain.cpml, source.dt, source.time_steps,
+                (source.xind, source.zind), precision=precision, device=device, ky=ky) as s:
+        for name in ("e11", "e12", "e13", "e22", "e23", "e33", "s11", "s12", "s13", "s22", "s23", "s33"):
+            s.set_array(name, _read(workdir, name, shape))
+        _cpml_inputs(s, workdir, shape)
+        for c in "xyz":
+            s.set_array("ic_" + c, fio.read_array(os.path.join(workdir, "initialconditionE" + c + ".dat"),
+                                                  shape, np.complex128))
+        for c in "xyz":
+            s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
+        for it in range(1, source.time_steps + 1):
+            s.run(1)
+            ex, ey, ez = s.field("ex"), s.field("ey"), s.field("ez")
+            if writer is not None:
+                writer.add(ex, ey, ez)
+            if legacy_output:
+                for a, ch in ((ex, "Ex"), (ey, "Ey"), (ez, "Ez")):
                     path = os.path.join(workdir, fio.snapshot_name(ch, it, source.xind, source.zind))
                     fio.write_record(path, a.real, np.float32 if single else np.float64)
         if writer is not None:

@@ -346,6 +346,35 @@ def em3(nx=384, ny=384, nz=192, d=0.05, cpml=10, steps=2000, seed=2, nlayers=5, 
                 receivers=recs, vmax=vmax)
 
 
+def em25(nx=120, nz=90, d=0.05, cpml=10, steps=400, seed=6, nlayers=4, f0=100e6, src=(60, 25), cfl=0.4,
+         nrec=24, ky=5.0, with_cpml=True):
+    """2.5D EM (x-z grid, y handled as i*ky): seeded layers of SPD
+    permittivity and conductivity tensors, Ricker Ex/Ez dipole."""
+    rng = np.random.default_rng(seed)
+    NX, NZ = nx + 2 * cpml, nz + 2 * cpml
+    shape = (NX, NZ)
+    lid = layer_index(NZ, cpml, nlayers, rng, min_thick=min(4, max(1, nz // (2 * nlayers))))
+    eps = [spd_tensor(rng, 3.0, 12.0, 0.3) for _ in range(nlayers)]
+    sig = [spd_tensor(rng, 1e-5, 1e-2, 0.1, log=True) for _ in range(nlayers)]
+    arrays = {}
+    for name, (i, j) in zip(["11", "12", "13", "22", "23", "33"],
+                            [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+        arrays["e" + name] = np.asfortranarray(np.broadcast_to(np.array([eps[l][i, j] for l in lid])[None, :], shape))
+        arrays["s" + name] = np.asfortranarray(np.broadcast_to(np.array([sig[l][i, j] for l in lid])[None, :], shape))
+    lam_min = min(np.linalg.eigvalsh(e).min() for e in eps)
+    vmax = CLIGHT / math.sqrt(lam_min)
+    dt = cfl * d / (math.sqrt(2.0) * vmax)
+    arrays.update(cpml_arrays(shape, (d, d), cpml, vmax, dt, f0) if with_cpml else no_cpml(shape))
+    for c in ("ic_x", "ic_y", "ic_z"):
+        arrays[c] = np.zeros(shape, dtype=np.complex128, order="F")
+    nrec = min(nrec, nx)
+    recs = np.array([[1 + (nx - 1) * k // max(nrec - 1, 1), 1, src[1]] for k in range(nrec)], dtype=np.int32)
+    w = ricker(f0, dt, steps)
+    return dict(physics="em25", shape=shape, spacing=(d, d), cpml=cpml, dt=dt, time_steps=steps, src=src,
+                arrays=arrays, ky=float(ky), sources={"x": w, "y": 0.3 * w, "z": -0.5 * w},
+                receivers=recs, vmax=vmax)
+
+
 def em2(nx=600, nz=300, d=0.05, cpml=10, steps=3000, seed=4, nlayers=6, f0=100e6,
         src=(300, 20), cfl=0.5, nrec=64, with_cpml=True):
     """2D EM, seeded layered anisotropic permittivity and conductivity."""
@@ -374,6 +403,7 @@ def em2(nx=600, nz=300, d=0.05, cpml=10, steps=3000, seed=4, nlayers=6, f0=100e6
 
 # ------------------------------------------------------ reference file set ----
 _IC_PREFIX = {"elastic2": "initialconditionV", "elastic25": "initialconditionV", "elastic3": "initialconditionV",
+              "em25": "initialconditionE",
               "em2": "initialconditionE", "em3": "initialconditionE"}
 
 
@@ -388,7 +418,7 @@ def write_inputs(cfg, workdir, source_type="ac"):
 
     phys = cfg["physics"]
     is3d = phys.endswith("3")
-    is25 = phys == "elastic25"
+    is25 = phys in ("elastic25", "em25")
     os.makedirs(workdir, exist_ok=True)
     shape = cfg["shape"]
     defaults = no_cpml(shape)

@@ -14,7 +14,7 @@ import numpy as np
 from . import _lib as L
 
 PHYSICS = {"elastic2": L.ELASTIC2, "elastic25": L.ELASTIC25, "elastic3": L.ELASTIC3, "em2": L.EM2,
-           "em3": L.EM3}
+           "em25": L.EM25, "em3": L.EM3}
 
 
 def _dptr(a):
@@ -24,14 +24,14 @@ def _dptr(a):
 class Solver:
     """One simulation on one GPU.
 
-    physics: "elastic2" | "elastic25" | "elastic3" | "em2" | "em3"
+    physics: "elastic2" | "elastic25" | "elastic3" | "em2" | "em25" | "em3"
     shape: padded (nx, nz) or (nx, ny, nz)
     spacing: (dx, dz) or (dx, dy, dz)
     src: 1-based unpadded source indices (xind, zind) or (xind, yind, zind),
          as Source_Type%xind.. (src/fortran/seidart_types.f08:23-33)
-    ky: y wavenumber of the 2.5D solver (the reference's KY environment
-        variable, src/fortran/cpmlfdtd.f08:697-703); its fields, initial
-        conditions and receiver samples are complex128
+    ky: y wavenumber of the 2.5D solvers (the reference's KY environment
+        variable, src/fortran/cpmlfdtd.f08:697-703, 2861-2867); their fields,
+        initial conditions and receiver samples are complex128
     """
 
     def __init__(self, physics, shape, spacing, cpml, dt, time_steps, src, *,
@@ -67,7 +67,7 @@ class Solver:
         p.device = int(device)
         p.ky = float(ky)
         self.params = p
-        self.complex = physics == "elastic25"
+        self.complex = physics in ("elastic25", "em25")
         self.ncell = int(np.prod(self.shape))
         self.time_steps = int(time_steps)
         self.fields = L.FIELD_IDS_ELASTIC if physics.startswith("elastic") else L.FIELD_IDS_EM

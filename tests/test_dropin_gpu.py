@@ -127,3 +127,25 @@ def test_dropin_seismic25_legacy_images(tmp_path, monkeypatch):
     for ch, key in (("Vx", "vx"), ("Vz", "vz")):
         got = fio.read_array(str(tmp_path / fio.snapshot_name(ch, 20, 20, 10)), sh)
         assert rel_l2(got, f[key].real) < 1e-10
+
+
+def test_cli_electromag25_blocks(tmp_path, monkeypatch):
+    """2.5D EM through the CLI: EM25.<first>-<last>.<x>.<y>.<z>.blk.zst blocks."""
+    from seidart_hip import cli, fortran_io as fio, models
+    cfg = models.em25(nx=40, nz=36, cpml=6, steps=24, src=(20, 12), nrec=4, ky=8.0)
+    models.write_inputs(cfg, str(tmp_path))
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setenv("KY", "8.0")
+    monkeypatch.setenv("FDTD_STEPS_PER_BLOCK", "10")
+    monkeypatch.delenv("FDTD_LEGACY_OUTPUT", raising=False)
+    print("[dropin] cli electromag25 blocks", flush=True)
+    assert cli.main(["input.json", "seismic=false"]) == 0
+    names = [fio.block_name("EM25", a, b, 20, 1, 12) for a, b in ((1, 10), (11, 20), (21, 24))]
+    for n in names:
+        assert (tmp_path / n).exists(), n
+    _, _, f = _oracle(cfg, ("ex", "ey", "ez"))
+    raw = fio.zstd_decompress((tmp_path / names[2]).read_bytes(), 40 * 36 * 3 * 4 * 4)
+    blk = np.frombuffer(raw, dtype="<f4").reshape((40, 1, 36, 3, 4), order="F")
+    for q, key in enumerate(("ex", "ey", "ez")):
+        want = f[key].real[6:-6, 6:-6].astype(np.float32)
+        assert rel_l2(blk[:, 0, :, q, 3], want) < 1e-6, key

@@ -154,3 +154,87 @@ def test_array_averaging_recursion():
     u = np.full((5, 5), 4.0)
     assert np.array_equal(array_averaging2d(u, 2, "+j"), u)
     assert np.array_equal(array_averaging2d(u, 1, "-i"), u)
+
+
+# ------------------------------------------------------------- electromag25
+EPS0 = 8.85418782e-12
+MU0 = 4.0 * math.pi * 1e-7
+
+
+def _em25_uniform(nsteps, ky, eps_r, sig, ic):
+    nx = nz = 100
+    cp = 10
+    shape = (nx + 2 * cp, nz + 2 * cp)
+    arrays = {}
+    for k, v in (("e11", eps_r), ("e22", eps_r), ("e33", eps_r), ("s11", sig), ("s22", sig), ("s33", sig)):
+        arrays[k] = np.full(shape, v, order="F")
+    for k in ("e12", "e13", "e23", "s12", "s13", "s23"):
+        arrays[k] = np.zeros(shape, order="F")
+    arrays.update(models.no_cpml(shape))
+    for k, v in ic.items():
+        arrays[k] = np.full(shape, v + 0j, order="F")
+    d = 0.05
+    dt = 0.2 * d / 3e8
+    cfg = dict(physics="em25", shape=shape, spacing=(d, d), cpml=cp, dt=dt, time_steps=nsteps, src=(50, 50),
+               arrays=arrays, ky=ky, sources={}, receivers=np.zeros((0, 3), np.int32))
+    return cfg, dt
+
+
+def test_em25_y_mode_closed_form():
+    """Uniform fields: Ez<->Hx and Ex<->Hz are leapfrog oscillators with
+    w = ky*dt/sqrt(mu0*eps), E_n = cos(n t) - w^2/(2 sin t) sin(n t); Hx_1 =
+    -i*ky*dt/mu0*Ez_0 fixes the sign of the y derivative; Ey stays put."""
+    ky, eps_r = 40.0, 4.0
+    for n in (1, 4, 10):
+        cfg, dt = _em25_uniform(n, ky, eps_r, 0.0, {"ic_x": 1.0, "ic_y": 1.0, "ic_z": 1.0})
+        _, _, f, _ = run_oracle(cfg, fields=("ex", "ey", "ez", "hx", "hz"))
+        c = (60, 60)
+        w = ky * dt / math.sqrt(MU0 * EPS0 * eps_r)
+        th = 2.0 * math.asin(w / 2.0)
+        expect = math.cos(n * th) - w * w / (2.0 * math.sin(th)) * math.sin(n * th)
+        for k in ("ex", "ez"):
+            assert abs(f[k][c].real - expect) < 1e-12 and abs(f[k][c].imag) < 1e-15, (k, n, f[k][c], expect)
+        assert abs(f["ey"][c] - 1.0) < 1e-15
+        if n == 1:
+            assert abs(f["hx"][c].imag + ky * dt / MU0) < 1e-12 * ky * dt / MU0
+            assert abs(f["hz"][c].imag - ky * dt / MU0) < 1e-12 * ky * dt / MU0
+
+
+def test_em25_conductive_decay():
+    """ky = 0, uniform E: E_n = (1 - dt*sigma/eps)^n E_0 (sigma acts on E_old)."""
+    eps_r, sig, n = 6.0, 2e-3, 12
+    cfg, dt = _em25_uniform(n, 0.0, eps_r, sig, {"ic_x": 0.7, "ic_y": -0.3, "ic_z": 0.2})
+    _, _, f, _ = run_oracle(cfg, fields=("ex", "ey", "ez"))
+    g = (1.0 - dt * sig / (EPS0 * eps_r)) ** n
+    for k, v in (("ex", 0.7), ("ey", -0.3), ("ez", 0.2)):
+        got = f[k][60, 60]
+        assert abs(got.real - v * g) < 1e-13 and got.imag == 0.0, (k, got, v * g)
+
+
+def test_em25_complex_linearity():
+    shape = (70, 60)
+    rng = np.random.default_rng(5)
+    cfg, _ = _em25_uniform(1, 25.0, 5.0, 1e-3, {})
+    cfg["arrays"] = {k: (v[:70, :60] if hasattr(v, "shape") else v) for k, v in cfg["arrays"].items()}
+    cfg["shape"] = shape
+    cfg["arrays"].update(models.cpml_arrays(shape, (0.05, 0.05), 8, 1.5e8, cfg["dt"], 1e8))
+    cfg["cpml"] = 8
+    cfg["time_steps"] = 80
+    cfg["src"] = (20, 20)
+    x, z = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    ics = {}
+    for c in ("ic_x", "ic_y", "ic_z"):
+        x0, z0 = rng.uniform(25, 45), rng.uniform(20, 40)
+        gss = np.exp(-((x - x0) ** 2 + (z - z0) ** 2) / 15.0)
+        ics[c] = (rng.normal() * gss, rng.normal() * gss)
+    outs = []
+    for part in ("re", "im", "both"):
+        c = dict(cfg)
+        c["arrays"] = dict(cfg["arrays"])
+        for k, (a, b) in ics.items():
+            c["arrays"][k] = {"re": a + 0j, "im": b + 0j, "both": a + 1j * b}[part]
+        outs.append(run_oracle(c, fields=("ex", "ey", "ez", "hx", "hy", "hz"))[2])
+    for k in ("ex", "ey", "ez", "hx", "hy", "hz"):
+        ref = outs[0][k] + 1j * outs[1][k]
+        err = np.linalg.norm(outs[2][k] - ref) / np.linalg.norm(ref)
+        assert err < 1e-13, (k, err)

@@ -201,13 +201,14 @@ def test_em3_heterogeneous_permittivity():
     _check(g, o, TOL64)
 
 
-@pytest.mark.parametrize("physics", ["elastic2", "elastic25", "elastic3", "em2", "em3"])
+@pytest.mark.parametrize("physics", ["elastic2", "elastic25", "elastic3", "em2", "em25", "em3"])
 def test_fp32_close_to_fp64(physics):
     """FP32 wavefields with FP64-precomputed coefficients: trace error vs the
     FP64 oracle stated per solver (bar 1e-3 relative L2 on these short runs)."""
     from seidart_hip import models
     cfg = {"elastic2": lambda: models.elastic2(nx=80, nz=70, cpml=8, steps=300, src=(40, 20), rec_z=20, nrec=8),
            "elastic25": lambda: models.elastic25(nx=70, nz=60, cpml=8, steps=250, src=(35, 20), nrec=8),
+           "em25": lambda: models.em25(nx=70, nz=60, cpml=8, steps=250, src=(35, 20), nrec=8),
            "elastic3": lambda: models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16),
            "em2": lambda: models.em2(nx=120, nz=80, steps=400, src=(60, 20), nrec=16),
            "em3": lambda: models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)}[physics]()
@@ -251,4 +252,29 @@ def test_elastic25_complex_ic_corrected_stencil():
     cfg["arrays"]["ic_x"] = (0.3 + 0.7j) * g0
     cfg["arrays"]["ic_z"] = (-0.5 + 0.2j) * np.roll(g0, 5, axis=1)
     g, o = _run_both(cfg, EL25_FIELDS)
+    _check(g, o, TOL64)
+
+
+EM25_FIELDS = ("ex", "ey", "ez", "hx", "hy", "hz")
+
+
+@pytest.mark.parametrize("ky", [0.0, 5.0, 30.0])
+def test_em25_fp64(ky):
+    """2.5D EM: layered SPD permittivity / conductivity, 4th-order
+    differences, i*ky y derivatives, three-component dipole."""
+    from seidart_hip import models
+    cfg = models.em25(ky=ky)
+    g, o = _run_both(cfg, EM25_FIELDS)
+    _check(g, o, TOL64)
+
+
+def test_em25_complex_ic():
+    from seidart_hip import models
+    cfg = models.em25(nx=70, nz=60, cpml=8, steps=200, src=(35, 20), nrec=8, ky=12.0)
+    shape = cfg["shape"]
+    x, z = np.meshgrid(np.arange(shape[0]), np.arange(shape[1]), indexing="ij")
+    g0 = np.exp(-((x - 40) ** 2 + (z - 30) ** 2) / 20.0)
+    cfg["arrays"]["ic_y"] = (0.2 - 0.9j) * g0
+    cfg["arrays"]["ic_z"] = (0.6 + 0.1j) * np.roll(g0, 4, axis=0)
+    g, o = _run_both(cfg, EM25_FIELDS)
     _check(g, o, TOL64)
