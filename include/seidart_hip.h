@@ -232,6 +232,25 @@ enum {
 };
 int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 
+/* ---- batched shots (2D elastic, 2D EM) ---------------------------------------
+ * No reference counterpart: the reference runs one shot per process.  A batch
+ * advances many prepared handles of one solver kind and precision (each with
+ * its own grid size, model, source and number of time steps) together: one
+ * kernel launch per half step covers the tiles of every shot, one more
+ * handles every shot's sources and receivers, and the step loop is replayed
+ * from captured hipGraphs.  The tile height is chosen by timing each
+ * candidate on the first steps of the run.  Handles stay owned by the caller
+ * and must outlive the batch; their fields, receivers and norms are read with
+ * the usual calls afterwards. */
+typedef struct sdt_batch sdt_batch;
+int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out);
+/* advance every shot by min(nsteps, its remaining steps); on a blow-up
+ * returns SDT_ERR_UNSTABLE with the shot index and its 1-based step */
+int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_t *failed_step);
+enum { SDT_BATCH_INFO_TILE_ROWS = 1, SDT_BATCH_INFO_TILES = 2 };
+int32_t sdt_batch_info(const sdt_batch *b, int32_t key, int64_t *value);
+int32_t sdt_batch_destroy(sdt_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,7 @@ struct El2Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t src_cell;   // excluded from the fused norm (the tail adds it)
+    int nsteps;         // time_steps of the run (batched shots stop there)
 };
 
 // memory recursion psi = b*psi + a*d ; d = d/kappa + psi (cpmlfdtd.f08:332-340)
@@ -200,7 +201,85 @@ static El2Args<T> el2_args(Device &d)
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
+    A.nsteps = d.time_steps;
     return A;
+}
+
+// ---- batched shots: one launch covers the tiles of every shot --------------
+// CPML classification per cell from the shot's 64 x 4 tile flags, so any
+// tile height works.
+template <typename T>
+__device__ __forceinline__ bool el2_pml(const El2Args<T> &A, int i, int j)
+{
+    return !in_box(A.box, i, 1, j) || A.flags[((i - 1) >> 6) + (size_t)A.tiles_x * ((j - 1) >> 2)];
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY) el2_stress_batch_kernel(const El2Args<T> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const El2Args<T> &A = args[t.x];
+    if (*A.d_it > A.nsteps) return;
+    const int i = t.y * 64 + threadIdx.x + 1;
+    const int j = t.z * TY + threadIdx.y + 1;
+    if (i > A.nx || j > A.nz) return;
+    if (el2_pml(A, i, j)) el2_stress_cell<T, true>(A, i, j);
+    else el2_stress_cell<T, false>(A, i, j);
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY) el2_velocity_batch_kernel(const El2Args<T> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const El2Args<T> &A = args[t.x];
+    if (*A.d_it > A.nsteps) return;   // uniform per workgroup
+    const int i = t.y * 64 + threadIdx.x + 1;
+    const int j = t.z * TY + threadIdx.y + 1;
+    double nrm = 0.0;
+    if (i <= A.nx && j <= A.nz)
+        nrm = el2_pml(A, i, j) ? el2_velocity_cell<T, true>(A, i, j) : el2_velocity_cell<T, false>(A, i, j);
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T>
+static int el2_batch_args_t(Batch &b, Device *const *shots, int n)
+{
+    std::vector<El2Args<T>> a((size_t)n);
+    for (int s = 0; s < n; ++s) a[(size_t)s] = el2_args<T>(*shots[s]);
+    if (hipMalloc(&b.args, a.size() * sizeof(El2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(b.args, a.data(), a.size() * sizeof(El2Args<T>), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+int el2_batch_args(Batch &b, Device *const *shots, int n)
+{
+    return b.precision == SDT_FP64 ? el2_batch_args_t<double>(b, shots, n) : el2_batch_args_t<float>(b, shots, n);
+}
+
+template <typename T, int TY>
+static void el2_batch_launch_t(Batch &b, int which)
+{
+    const El2Args<T> *a = (const El2Args<T> *)b.args;
+    const dim3 grid(b.ntiles[which]), block(64, TY);
+    hipLaunchKernelGGL((el2_stress_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
+    hipLaunchKernelGGL((el2_velocity_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
+}
+
+template <typename T>
+static void el2_batch_launch_p(Batch &b, int which)
+{
+    switch (kBatchTileRows[which]) {
+    case 2: el2_batch_launch_t<T, 2>(b, which); break;
+    case 8: el2_batch_launch_t<T, 8>(b, which); break;
+    default: el2_batch_launch_t<T, 4>(b, which); break;
+    }
+}
+
+void el2_batch_launch(Batch &b, int which)
+{
+    if (b.precision == SDT_FP64) el2_batch_launch_p<double>(b, which);
+    else el2_batch_launch_p<float>(b, which);
 }
 
 template <typename T>

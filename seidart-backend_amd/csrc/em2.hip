@@ -33,6 +33,7 @@ struct Em2Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t src_cell;
+    int nsteps;         // time_steps of the run (batched shots stop there)
 };
 
 template <typename T, bool PML>
@@ -140,7 +141,82 @@ static Em2Args<T> em2_args(Device &d)
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
+    A.nsteps = d.time_steps;
     return A;
+}
+
+// ---- batched shots: one launch covers the tiles of every shot --------------
+template <typename T>
+__device__ __forceinline__ bool em2_pml(const Em2Args<T> &A, int i, int j)
+{
+    return !in_box(A.box, i, 1, j) || A.flags[((i - 1) >> 6) + (size_t)A.tiles_x * ((j - 1) >> 2)];
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY) em2_h_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const Em2Args<T> &A = args[t.x];
+    if (*A.d_it > A.nsteps) return;
+    const int i = t.y * 64 + threadIdx.x + 1;
+    const int j = t.z * TY + threadIdx.y + 1;
+    if (i > A.nx || j > A.nz) return;
+    if (em2_pml(A, i, j)) em2_h_cell<T, true>(A, i, j);
+    else em2_h_cell<T, false>(A, i, j);
+}
+
+template <typename T, int TY>
+__global__ void __launch_bounds__(64 * TY) em2_e_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    const Em2Args<T> &A = args[t.x];
+    if (*A.d_it > A.nsteps) return;   // uniform per workgroup
+    const int i = t.y * 64 + threadIdx.x + 1;
+    const int j = t.z * TY + threadIdx.y + 1;
+    double nrm = 0.0;
+    if (i <= A.nx && j <= A.nz) nrm = em2_pml(A, i, j) ? em2_e_cell<T, true>(A, i, j) : em2_e_cell<T, false>(A, i, j);
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T>
+static int em2_batch_args_t(Batch &b, Device *const *shots, int n)
+{
+    std::vector<Em2Args<T>> a((size_t)n);
+    for (int s = 0; s < n; ++s) a[(size_t)s] = em2_args<T>(*shots[s]);
+    if (hipMalloc(&b.args, a.size() * sizeof(Em2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(b.args, a.data(), a.size() * sizeof(Em2Args<T>), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+int em2_batch_args(Batch &b, Device *const *shots, int n)
+{
+    return b.precision == SDT_FP64 ? em2_batch_args_t<double>(b, shots, n) : em2_batch_args_t<float>(b, shots, n);
+}
+
+template <typename T, int TY>
+static void em2_batch_launch_t(Batch &b, int which)
+{
+    const Em2Args<T> *a = (const Em2Args<T> *)b.args;
+    const dim3 grid(b.ntiles[which]), block(64, TY);
+    hipLaunchKernelGGL((em2_h_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
+    hipLaunchKernelGGL((em2_e_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
+}
+
+template <typename T>
+static void em2_batch_launch_p(Batch &b, int which)
+{
+    switch (kBatchTileRows[which]) {
+    case 2: em2_batch_launch_t<T, 2>(b, which); break;
+    case 8: em2_batch_launch_t<T, 8>(b, which); break;
+    default: em2_batch_launch_t<T, 4>(b, which); break;
+    }
+}
+
+void em2_batch_launch(Batch &b, int which)
+{
+    if (b.precision == SDT_FP64) em2_batch_launch_p<double>(b, which);
+    else em2_batch_launch_p<float>(b, which);
 }
 
 template <typename T>

@@ -562,6 +562,158 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
     return fail(h, SDT_ERR_ARG, "unknown info key");
 }
 
+// ---- batched shots ----------------------------------------------------------
+struct sdt_batch {
+    std::vector<sdt_handle *> shots;
+    Batch b;
+    hipGraphExec_t graph = nullptr;
+    hipEvent_t ev[2] = {};
+    std::string err;
+};
+
+static int bfail(sdt_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg;
+    g_err = msg;
+    return code;
+}
+
+int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
+{
+    if (!handles || !out || n < 1) return fail(nullptr, SDT_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    const int phys = handles[0] ? handles[0]->p.physics : -1;
+    if (phys != SDT_ELASTIC2 && phys != SDT_EM2)
+        return fail(nullptr, SDT_ERR_ARG, "batches hold 2D elastic or 2D EM handles");
+    for (int q = 0; q < n; ++q) {
+        sdt_handle *h = handles[q];
+        if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle in batch");
+        if (h->p.physics != phys || h->p.precision != handles[0]->p.precision ||
+            h->p.device != handles[0]->p.device)
+            return fail(nullptr, SDT_ERR_ARG, "batch handles must share solver kind, precision and device");
+        if (h->failed) return fail(nullptr, SDT_ERR_STATE, "a batch handle has already blown up");
+        int rc = prepare(h);
+        if (rc) return rc;
+    }
+    sdt_batch *b = new sdt_batch();
+    b->shots.assign(handles, handles + n);
+    std::vector<Device *> devs((size_t)n);
+    for (int q = 0; q < n; ++q) devs[(size_t)q] = &handles[q]->d;
+    hipError_t e = hipSetDevice(handles[0]->p.device);
+    int rc = e == hipSuccess ? batch_setup(b->b, devs.data(), n) : SDT_ERR_HIP;
+    if (rc == SDT_OK) {
+        for (int q = 0; q < 2 && rc == SDT_OK; ++q)
+            if (hipEventCreate(&b->ev[q]) != hipSuccess) rc = SDT_ERR_HIP;
+    }
+    if (rc != SDT_OK) {
+        batch_free(b->b);
+        delete b;
+        return fail(nullptr, rc, "batch setup failed");
+    }
+    *out = b;
+    return SDT_OK;
+}
+
+int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_t *failed_step)
+{
+    if (!b) return fail(nullptr, SDT_ERR_ARG, "null batch");
+    if (failed_shot) *failed_shot = -1;
+    if (failed_step) *failed_step = 0;
+    if (nsteps < 0) return bfail(b, SDT_ERR_ARG, "nsteps must be >= 0");
+    const int n = (int)b->shots.size();
+    std::vector<int> adv((size_t)n);
+    int todo = 0;
+    for (int q = 0; q < n; ++q) {
+        sdt_handle *h = b->shots[(size_t)q];
+        if (h->failed) return bfail(b, SDT_ERR_STATE, "a batch handle has already blown up");
+        const int left = h->d.time_steps - h->steps_done;
+        adv[(size_t)q] = nsteps < left ? nsteps : left;
+        todo = adv[(size_t)q] > todo ? adv[(size_t)q] : todo;
+    }
+    hipError_t e = hipSetDevice(b->shots[0]->p.device);
+    if (e != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    Batch &B = b->b;
+    int done = 0;
+    // autotune: each candidate tile height runs one untimed and kTune timed
+    // real steps (every height gives the same results), the fastest is kept
+    constexpr int kTune = 4;
+    if (B.choice < 0 && todo >= 3 * (kTune + 1)) {
+        float best = 0;
+        for (int w = 0; w < 3; ++w) {
+            batch_step(B, w);
+            (void)hipEventRecord(b->ev[0], B.stream);
+            for (int s = 0; s < kTune; ++s) batch_step(B, w);
+            (void)hipEventRecord(b->ev[1], B.stream);
+            if ((e = hipEventSynchronize(b->ev[1])) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+            if (B.choice < 0 || ms < best) { best = ms; B.choice = w; }
+        }
+        done += 3 * (kTune + 1);
+    }
+    const int w = B.choice < 0 ? 1 : B.choice;
+    constexpr int kSteps = 32;
+    if (todo - done >= kSteps && !b->graph) {
+        hipGraph_t g = nullptr;
+        if ((e = hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+            return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+        for (int s = 0; s < kSteps; ++s) batch_step(B, w);
+        if ((e = hipStreamEndCapture(B.stream, &g)) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    }
+    for (; b->graph && done + kSteps <= todo; done += kSteps)
+        if ((e = hipGraphLaunch(b->graph, B.stream)) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    for (; done < todo; ++done) batch_step(B, w);
+    if ((e = hipGetLastError()) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(B.stream)) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    // every shot is checked and advanced; the first blow-up is reported
+    int status = SDT_OK;
+    for (int q = 0; q < n; ++q) {
+        sdt_handle *h = b->shots[(size_t)q];
+        if (adv[(size_t)q] == 0) continue;
+        const int first = h->steps_done + 1;
+        int32_t fs = 0;
+        int rc = check_norms(h, first, adv[(size_t)q], &fs);
+        if (rc == SDT_ERR_UNSTABLE) {
+            if (status == SDT_OK) {
+                if (failed_shot) *failed_shot = q;
+                if (failed_step) *failed_step = fs;
+                status = bfail(b, rc, "shot " + std::to_string(q) + ": " + h->err);
+            }
+            continue;
+        }
+        if (rc) return bfail(b, rc, h->err);
+        h->steps_done += adv[(size_t)q];
+    }
+    if (status != SDT_OK) g_err = b->err;   // later shots' checks overwrote it
+    return status;
+}
+
+int32_t sdt_batch_info(const sdt_batch *b, int32_t key, int64_t *value)
+{
+    if (!b || !value) return fail(nullptr, SDT_ERR_ARG, "null argument");
+    switch (key) {
+    case SDT_BATCH_INFO_TILE_ROWS: *value = b->b.choice < 0 ? 0 : kBatchTileRows[b->b.choice]; return SDT_OK;
+    case SDT_BATCH_INFO_TILES: *value = b->b.ntiles[b->b.choice < 0 ? 1 : b->b.choice]; return SDT_OK;
+    }
+    return fail(nullptr, SDT_ERR_ARG, "unknown batch info key");
+}
+
+int32_t sdt_batch_destroy(sdt_batch *b)
+{
+    if (!b) return SDT_OK;
+    (void)hipSetDevice(b->shots[0]->p.device);
+    (void)hipStreamSynchronize(b->b.stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto ev : b->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    batch_free(b->b);
+    delete b;
+    return SDT_OK;
+}
+
 int32_t sdt_synchronize(sdt_handle *h)
 {
     if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");

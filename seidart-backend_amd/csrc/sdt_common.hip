@@ -30,7 +30,7 @@ struct FieldPtrs {
 // cells' norms are folded in and the receivers recorded; then thread 0
 // advances the device step counter.
 template <typename T>
-__global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
+__device__ __forceinline__ void tail_body(const FieldPtrs &fp, const TailArgs &a)
 {
     const int it = *a.d_it;
     const int t = threadIdx.x;
@@ -80,6 +80,85 @@ __global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
     }
     __syncthreads();
     if (t == 0) *a.d_it = it + 1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) tail_kernel(FieldPtrs fp, TailArgs a)
+{
+    tail_body<T>(fp, a);
+}
+
+// one workgroup per shot of a batch; finished shots do nothing
+template <typename T>
+__global__ void __launch_bounds__(256) tail_batch_kernel(const FieldPtrs *fps, const TailArgs *tails)
+{
+    const TailArgs &a = tails[blockIdx.x];
+    if (*a.d_it > a.time_steps) return;
+    tail_body<T>(fps[blockIdx.x], a);
+}
+
+void launch_tail_batch(Batch &b)
+{
+    if (b.precision == SDT_FP64)
+        hipLaunchKernelGGL(tail_batch_kernel<double>, dim3(b.nshots), dim3(256), 0, b.stream,
+                           (const FieldPtrs *)b.fps, (const TailArgs *)b.tails);
+    else
+        hipLaunchKernelGGL(tail_batch_kernel<float>, dim3(b.nshots), dim3(256), 0, b.stream,
+                           (const FieldPtrs *)b.fps, (const TailArgs *)b.tails);
+}
+
+int batch_setup(Batch &b, Device *const *shots, int n)
+{
+    b.nshots = n;
+    b.physics = shots[0]->physics;
+    b.precision = shots[0]->precision;
+    b.stream = shots[0]->stream;
+    // tile lists for every candidate tile height
+    for (int w = 0; w < 3; ++w) {
+        const int ty = kBatchTileRows[w];
+        std::vector<int4> t;
+        for (int s = 0; s < n; ++s) {
+            const int tx = (shots[s]->nx + 63) / 64, tz = (shots[s]->nz + ty - 1) / ty;
+            for (int z = 0; z < tz; ++z)
+                for (int x = 0; x < tx; ++x) t.push_back(make_int4(s, x, z, 0));
+        }
+        b.ntiles[w] = (int)t.size();
+        if (hipMalloc((void **)&b.tiles[w], t.size() * sizeof(int4)) != hipSuccess) return SDT_ERR_HIP;
+        if (hipMemcpy(b.tiles[w], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+            return SDT_ERR_HIP;
+    }
+    std::vector<FieldPtrs> fps((size_t)n);
+    std::vector<TailArgs> tails((size_t)n);
+    for (int s = 0; s < n; ++s) {
+        for (int f = 0; f < SDT_NUM_FIELDS; ++f) fps[(size_t)s].f[f] = shots[s]->field[f];
+        tails[(size_t)s] = shots[s]->tail;
+    }
+    if (hipMalloc(&b.fps, fps.size() * sizeof(FieldPtrs)) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(b.fps, fps.data(), fps.size() * sizeof(FieldPtrs), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (hipMalloc((void **)&b.tails, tails.size() * sizeof(TailArgs)) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(b.tails, tails.data(), tails.size() * sizeof(TailArgs), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (b.physics == SDT_ELASTIC2) return el2_batch_args(b, shots, n);
+    if (b.physics == SDT_EM2) return em2_batch_args(b, shots, n);
+    return SDT_ERR_ARG;
+}
+
+void batch_free(Batch &b)
+{
+    for (int w = 0; w < 3; ++w)
+        if (b.tiles[w]) (void)hipFree(b.tiles[w]);
+    if (b.fps) (void)hipFree(b.fps);
+    if (b.tails) (void)hipFree(b.tails);
+    if (b.args) (void)hipFree(b.args);
+    b = Batch();
+}
+
+void batch_step(Batch &b, int which)
+{
+    if (b.physics == SDT_ELASTIC2) el2_batch_launch(b, which);
+    else em2_batch_launch(b, which);
+    launch_tail_batch(b);
 }
 
 void launch_tail(Device &d)

@@ -147,6 +147,31 @@ void em2_step(Device &d, bool timed);
 int em3_setup(Device &d, const double *const *host_in);
 void em3_step(Device &d, bool timed);
 
+// Many 2D shots advanced together: one launch per pass covers the tiles of
+// every shot (tile list entries name the shot), one tail launch covers every
+// shot's sources and receivers.  A shot whose step counter passed its
+// time_steps is skipped, so shots of different lengths share the launches.
+constexpr int kBatchTileRows[3] = {2, 4, 8};   // tile heights the autotuner tries
+struct Batch {
+    int physics = 0, precision = 0, nshots = 0;
+    void *args = nullptr;          // per-shot kernel arguments (device)
+    int4 *tiles[3] = {};           // per tile height: (shot, tile x, tile z, 0)
+    int ntiles[3] = {};
+    void *fps = nullptr;           // per-shot field pointers (device)
+    TailArgs *tails = nullptr;     // per-shot tail arguments (device)
+    int choice = -1;               // autotuned tile height index, -1 = not yet
+    hipStream_t stream = nullptr;
+};
+int batch_setup(Batch &b, Device *const *shots, int n);
+void batch_free(Batch &b);
+// one step of every unfinished shot with tile height kBatchTileRows[which]
+void batch_step(Batch &b, int which);
+int el2_batch_args(Batch &b, Device *const *shots, int n);
+int em2_batch_args(Batch &b, Device *const *shots, int n);
+void el2_batch_launch(Batch &b, int which);
+void em2_batch_launch(Batch &b, int which);
+void launch_tail_batch(Batch &b);
+
 // common pieces (sdt_common.hip)
 void launch_tail(Device &d);
 // CPML classification.  A cell is an identity cell when a == 0, kappa == 1

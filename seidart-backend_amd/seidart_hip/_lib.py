@@ -64,11 +64,13 @@ EXPORTS = [
     "sdt_set_array", "sdt_set_source", "sdt_set_receivers", "sdt_run", "sdt_steps_done",
     "sdt_get_field", "sdt_set_field", "sdt_get_receivers", "sdt_get_norms", "sdt_set_timing",
     "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option", "sdt_get_info",
+    "sdt_batch_create", "sdt_batch_run", "sdt_batch_info", "sdt_batch_destroy",
 ]
 
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
     1, 2, 3, 4, 5, 6, 7)
 INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES = 1, 2, 3
+BATCH_INFO_TILE_ROWS, BATCH_INFO_TILES = 1, 2
 
 
 class SdtError(RuntimeError):
@@ -113,6 +115,10 @@ def _load():
         "sdt_synchronize": ([P], I32),
         "sdt_set_option": ([P, I32, I64], I32),
         "sdt_get_info": ([P, I32, C.POINTER(I64)], I32),
+        "sdt_batch_create": ([C.POINTER(P), I32, C.POINTER(P)], I32),
+        "sdt_batch_run": ([P, I32, C.POINTER(I32), C.POINTER(I32)], I32),
+        "sdt_batch_info": ([P, I32, C.POINTER(I64)], I32),
+        "sdt_batch_destroy": ([P], I32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

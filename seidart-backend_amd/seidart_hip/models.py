@@ -401,6 +401,33 @@ def em2(nx=600, nz=300, d=0.05, cpml=10, steps=3000, seed=4, nlayers=6, f0=100e6
                 receivers=recs, vmax=vmax)
 
 
+def batch2d(physics="elastic2", nshots=64, seed=3, nx=(300, 800), nz=(150, 400), steps=(1500, 3000),
+            nlayers=4, stencil="reference"):
+    """Uneven multi-shot 2D set: per shot a seeded grid size (nx, nz uniform in
+    the given ranges), its own `nlayers`-layer anisotropic model (as
+    `elastic2` / `em2`), its own source position and step count.
+
+    stencil (elastic only): the reference's 3-term velocity differences
+    (cpmlfdtd.f08:405-446) carry a zero-order term that makes long runs grow
+    without bound in many of these models; "corrected" selects the 4-term
+    differences, which cost the same memory traffic."""
+    rng = np.random.default_rng(seed)
+    shots = []
+    for q in range(nshots):
+        sx = int(rng.integers(nx[0], nx[1] + 1))
+        sz = int(rng.integers(nz[0], nz[1] + 1))
+        n = int(rng.integers(steps[0], steps[1] + 1))
+        src = (int(rng.integers(sx // 4, 3 * sx // 4)), int(rng.integers(10, sz // 3)))
+        sd = int(rng.integers(0, 2**31 - 1))
+        if physics == "elastic2":
+            shots.append(elastic2(nx=sx, nz=sz, steps=n, seed=sd, nlayers=nlayers, src=src,
+                                  rec_z=src[1], nrec=20))
+            shots[-1]["stencil"] = stencil
+        else:
+            shots.append(em2(nx=sx, nz=sz, steps=n, seed=sd, nlayers=nlayers, src=src, nrec=20))
+    return shots
+
+
 # ------------------------------------------------------ reference file set ----
 _IC_PREFIX = {"elastic2": "initialconditionV", "elastic25": "initialconditionV", "elastic3": "initialconditionV",
               "em25": "initialconditionE",

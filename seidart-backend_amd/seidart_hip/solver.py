@@ -212,9 +212,67 @@ class Solver:
         return a.value, b.value, n.value
 
 
-def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=(), options=None):
-    """Run a synthetic configuration dict (see seidart_hip.models) and return
-    (receivers, norms, {field: array})."""
+class Batch:
+    """Many 2D shots advanced together on one GPU.
+
+    `solvers`: prepared-or-not `Solver`s of one kind ("elastic2" or "em2")
+    and one precision, each with its own grid, model, source, receivers and
+    time_steps.  Every shot's inputs are fixed when the batch is made.  One
+    launch per half step covers the tiles of all shots; the tile height is
+    picked by timing the candidates on the first steps.  Results equal those
+    of running each `Solver` alone; read them from the solvers afterwards.
+    """
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        if not self.solvers:
+            raise ValueError("empty batch")
+        hs = (C.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        self._b = C.c_void_p()
+        L.check(L.lib.sdt_batch_create(hs, len(self.solvers), C.byref(self._b)))
+
+    def run(self, nsteps=None):
+        """Advance every shot by min(nsteps, its remaining steps); default:
+        run all shots to their ends.  Raises UnstableError (with .shot and
+        .step) when a shot blows up."""
+        if nsteps is None:
+            nsteps = max(s.time_steps - s.steps_done for s in self.solvers)
+        shot, step = C.c_int32(-1), C.c_int32(0)
+        rc = L.lib.sdt_batch_run(self._b, int(nsteps), C.byref(shot), C.byref(step))
+        try:
+            L.check(rc, None, step.value)
+        except L.UnstableError as e:
+            e.shot = shot.value
+            raise
+
+    def info(self, key):
+        """'tile_rows' (the chosen tile height, 0 before tuning) or 'tiles'."""
+        k = {"tile_rows": L.BATCH_INFO_TILE_ROWS, "tiles": L.BATCH_INFO_TILES}[key]
+        v = C.c_int64(0)
+        L.check(L.lib.sdt_batch_info(self._b, k, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._b:
+            L.lib.sdt_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def make_solver(cfg, *, precision="fp64", device=0, options=None):
+    """A `Solver` loaded with a configuration dict's model, sources and
+    receivers (see seidart_hip.models), not yet run."""
     s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"],
                cfg["time_steps"], cfg["src"], precision=precision,
                density_method=cfg.get("density_method", "arithmetic"),
@@ -227,6 +285,17 @@ def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=(), option
             s.set_source(comp, series)
         if len(cfg.get("receivers", ())):
             s.set_receivers(cfg["receivers"])
+    except Exception:
+        s.close()
+        raise
+    return s
+
+
+def run_config(cfg, *, precision="fp64", steps=None, device=0, fields=(), options=None):
+    """Run a synthetic configuration dict (see seidart_hip.models) and return
+    (receivers, norms, {field: array})."""
+    s = make_solver(cfg, precision=precision, device=device, options=options)
+    try:
         s.run(steps if steps is not None else cfg["time_steps"])
         out = {f: s.field(f) for f in fields}
         return s.receivers(), s.norms(), out
