@@ -200,7 +200,8 @@ def bench_batch2d(args):
     out = {}
     for physics in ("elastic2", "em2") if args.batch_physics == "both" else (args.batch_physics,):
         t_gen = time.perf_counter()
-        cfgs = models.batch2d(physics, nshots=args.shots, stencil=args.batch_stencil)
+        cfgs = models.batch2d(physics, nshots=args.shots, stencil=args.batch_stencil,
+                              tilt=(0.0, args.batch_tilt))
         t_gen = time.perf_counter() - t_gen
         cell_steps = sum(_cells_2d(c)[0] * c["time_steps"] for c in cfgs)
         P = EL2 if physics == "elastic2" else EM2
@@ -214,16 +215,17 @@ def bench_batch2d(args):
         t0 = time.perf_counter()
         b.run()
         t_batch = time.perf_counter() - t0
-        rows, tiles = b.info("tile_rows"), b.info("tiles")
+        rows, tiles, xcd = b.info("tile_rows"), b.info("tiles"), b.info("xcd_order")
         b.close()
         rec32 = [s.receivers() for s in shots]
         for s in shots:
             s.close()
-        print(f"[batch2d] {physics}: batched fp32 {t_batch:.3f} s (tile rows {rows})", flush=True)
+        print(f"[batch2d] {physics}: batched fp32 {t_batch:.3f} s (tile rows {rows}, xcd order {xcd})",
+              flush=True)
 
         serial = {}
         rec64 = []
-        for prec in ("fp32", "fp64"):
+        for prec in ("fp32", "fp64") if args.batch_serial else ():
             total = 0.0
             for c in cfgs:
                 s = make_solver(c, precision=prec)
@@ -238,15 +240,21 @@ def bench_batch2d(args):
                 s.close()
             serial[prec] = total
             print(f"[batch2d] {physics}: serial {prec} {total:.3f} s", flush=True)
-        errs = [rel_l2(a, b_) for a, b_ in zip(rec32, rec64)]
         if not all(np.isfinite(r).all() for r in rec32):
             raise RuntimeError("non-finite receiver traces in the batch")
+        if not args.batch_serial:
+            out[physics] = {"shots": len(cfgs), "cell_updates": cell_steps,
+                            "batched_fp32": {"s": round(t_batch, 4),
+                                             "GCell_per_s": round(cell_steps / t_batch / 1e9, 3)}}
+            print(json.dumps(out[physics]), flush=True)
+            continue
+        errs = [rel_l2(a, b_) for a, b_ in zip(rec32, rec64)]
         out[physics] = {
             "shots": len(cfgs), "cell_updates": cell_steps,
             "batched_fp32": {"s": round(t_batch, 4), "GCell_per_s": round(cell_steps / t_batch / 1e9, 3),
                              "shots_per_s": round(len(cfgs) / t_batch, 3),
                              "hbm_fraction": round(bytes32 / t_batch / 1e9 / HBM_PEAK_GBS, 4),
-                             "tile_rows": rows, "tiles_per_launch": tiles},
+                             "tile_rows": rows, "xcd_order": xcd, "tiles_per_launch": tiles},
             "serial_fp32": {"s": round(serial["fp32"], 4),
                             "GCell_per_s": round(cell_steps / serial["fp32"] / 1e9, 3)},
             "serial_fp64": {"s": round(serial["fp64"], 4),
@@ -256,6 +264,8 @@ def bench_batch2d(args):
             "fp32_vs_fp64_trace_rel_l2": {"max": float(f"{max(errs):.3e}"),
                                           "median": float(f"{float(np.median(errs)):.3e}")},
         }
+    if not args.batch_serial:
+        return
     main_p = next(iter(out))
     m = out[main_p]
     line = {
@@ -266,7 +276,7 @@ def bench_batch2d(args):
         "data": "synthetic (seeded 4-layer anisotropic models per shot, Ricker sources)",
         "config": {"workload": f"batch2d: {args.shots} shots per solver kind, nx 300-800, nz 150-400 "
                                "(+10-cell CPML), 1500-3000 steps each", "kinds": list(out),
-                   "elastic_stencil": args.batch_stencil},
+                   "elastic_stencil": args.batch_stencil, "elastic_tilt_max_deg": args.batch_tilt},
         "roofline": {"bound": "hbm", "kernel": "whole batched step (byte model over every shot)",
                      "achieved": round(m["batched_fp32"]["hbm_fraction"] * HBM_PEAK_GBS, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": m["batched_fp32"]["hbm_fraction"],
@@ -303,9 +313,15 @@ def main():
                          "384x384x192 case; batch2d: uneven multi-shot 2D batches against serial runs")
     ap.add_argument("--shots", type=int, default=64, help="batch2d: shots per solver kind")
     ap.add_argument("--batch-physics", default="both", choices=["both", "elastic2", "em2"])
-    ap.add_argument("--batch-stencil", default="corrected", choices=["corrected", "reference"],
-                    help="batch2d elastic velocity differences: the reference's 3-term ones grow "
-                         "without bound over 1500-3000 steps in many of the seeded models")
+    ap.add_argument("--batch-serial", type=int, default=1,
+                    help="batch2d: also time the serial single-handle runs (0 = batched run only, "
+                         "for profiling; prints the batched figures only)")
+    ap.add_argument("--batch-stencil", default="reference", choices=["reference", "corrected"],
+                    help="batch2d elastic velocity differences")
+    ap.add_argument("--batch-tilt", type=float, default=0.0,
+                    help="batch2d elastic: largest symmetry-axis tilt in degrees; the reference's "
+                         "uncentred c15/c35 coupling grows without bound over 1500-3000 steps for "
+                         "tilted axes, so the default keeps the axes vertical")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()

@@ -247,7 +247,10 @@ int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
 /* advance every shot by min(nsteps, its remaining steps); on a blow-up
  * returns SDT_ERR_UNSTABLE with the shot index and its 1-based step */
 int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_t *failed_step);
-enum { SDT_BATCH_INFO_TILE_ROWS = 1, SDT_BATCH_INFO_TILES = 2 };
+/* TILE_ROWS: rows per tile of the chosen shape (0 before tuning); TILES:
+ * workgroups per launch; XCD_ORDER: 1 when the chosen tile list is ordered
+ * for XCD locality (-1 before tuning) */
+enum { SDT_BATCH_INFO_TILE_ROWS = 1, SDT_BATCH_INFO_TILES = 2, SDT_BATCH_INFO_XCD_ORDER = 3 };
 int32_t sdt_batch_info(const sdt_batch *b, int32_t key, int64_t *value);
 int32_t sdt_batch_destroy(sdt_batch *b);
 

@@ -1,11 +1,12 @@
 // el2.hip -- 2D anisotropic elastic CPML-FDTD (seismic2, cpmlfdtd.f08:57-597)
 // on gfx950.
 //
-// Layout: every array is (nx, nz) Fortran order, x fastest, in HBM.  One
-// thread per cell, 64x4-cell tiles (one 64-wide wavefront per x row).  A tile
-// whose cells all have identity CPML coefficients runs the branch-free
-// interior body (no memory variables, no a/b/kappa reads); other tiles run the
-// CPML body.  The choice is uniform per workgroup.
+// Layout: every array is (nx, nz) Fortran order, x fastest, in HBM, rows
+// padded to whole 128-byte lines.  One thread per cell; a workgroup of four
+// wavefronts covers 64 line-aligned columns and a band of rows, each
+// wavefront marching its own rows.  A wavefront row whose cells all have
+// identity CPML coefficients runs the interior body (no memory variables, no
+// a/b/kappa reads); other rows run the CPML body.
 #include <math.h>
 
 #include <vector>
@@ -41,135 +42,222 @@ __device__ __forceinline__ T cpml_fix(T d, T &psi, T a, T b, T ik)
     return d * ik + psi;
 }
 
-// sigma_xx, sigma_zz (cpmlfdtd.f08:324-353) and sigma_xz (362-386)
+// sigma_xx, sigma_zz (cpmlfdtd.f08:324-353) and sigma_xz (362-386).  Every
+// operand is read before the first store: the compiler cannot tell the
+// arrays apart, so a store ahead of a load would serialise the memory
+// round trips.
 template <typename T, bool PML>
 __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
+    const bool r1 = i >= 2 && i <= nx - 2 && j >= 3 && j <= nz - 1;   // sigma_xx, sigma_zz
+    const bool r2 = i >= 3 && i <= nx - 1 && j >= 2 && j <= nz - 2;   // sigma_xz
+    if (!r1 && !r2) return;
     const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     const T *vx = A.vx, *vz = A.vz;
     const int64_t sx = 1, sz = A.pitch;
-    if (i >= 2 && i <= nx - 2 && j >= 3 && j <= nz - 1) {
-        T dvx_dx = (T(27) * (vx[c + sx] - vx[c]) - (vx[c + 2 * sx] - vx[c - sx])) * A.rdx24;
-        T dvz_dz = (T(27) * (vz[c] - vz[c - sz]) - (vz[c + sz] - vz[c - 2 * sz])) * A.rdz24;
-        T dvz_dx = (T(27) * (vz[c + sx] - vz[c]) - (vz[c + 2 * sx] - vz[c - sx])) * A.rdx24;
-        T dvx_dz = (T(27) * (vx[c] - vx[c - sz]) - (vx[c + sz] - vx[c - 2 * sz])) * A.rdz24;
+    // stencil operands: x0 = vx(i,j), xl/xr = vx(i-+1,j), xd/xu = vx(i,j-+1), ...
+    const T x0 = vx[c], xl = vx[c - sx], xr = vx[c + sx], xd = vx[c - sz], xu = vx[c + sz];
+    const T z0 = vz[c], zl = vz[c - sx], zr = vz[c + sx], zd = vz[c - sz], zu = vz[c + sz];
+    T xr2 = 0, xd2 = 0, zr2 = 0, zd2 = 0, xl2 = 0, xu2 = 0, zl2 = 0, zu2 = 0;
+    T s_xx = 0, s_zz = 0, s_xz = 0, k11 = 0, k13 = 0, k15 = 0, k33 = 0, k35 = 0, k55 = 0;
+    T gx = 0, gz = 0, gxz = 0;
+    T ah = 0, bh = 0, ikh = 0, a = 0, b = 0, ik = 0, m[8] = {};
+    if (r1) {
+        xr2 = vx[c + 2 * sx];
+        xd2 = vx[c - 2 * sz];
+        zr2 = vz[c + 2 * sx];
+        zd2 = vz[c - 2 * sz];
+        s_xx = A.sxx[c];
+        s_zz = A.szz[c];
+        k11 = ldnt(A.c11 + c);
+        k13 = ldnt(A.c13 + c);
+        k33 = ldnt(A.c33 + c);
+        gx = ldnt(A.igx + c);
+        gz = ldnt(A.igz + c);
+    }
+    if (r2) {
+        xl2 = vx[c - 2 * sx];
+        xu2 = vx[c + 2 * sz];
+        zl2 = vz[c - 2 * sx];
+        zu2 = vz[c + 2 * sz];
+        s_xz = A.sxz[c];
+        k55 = ldnt(A.c55 + c);
+        gxz = ldnt(A.igxz + c);
+    }
+    k15 = ldnt(A.c15 + c);
+    k35 = ldnt(A.c35 + c);
+    if (PML) {
+        ah = ldnt(A.ah + c);
+        bh = ldnt(A.bh + c);
+        ikh = ldnt(A.ikh + c);
+        if (r1) {
+            a = ldnt(A.a + c);
+            b = ldnt(A.b + c);
+            ik = ldnt(A.ik + c);
+            for (int q = 0; q < 4; ++q) m[q] = ldnt(A.m[q] + c);
+        }
+        if (r2)
+            for (int q = 4; q < 8; ++q) m[q] = ldnt(A.m[q] + c);
+    }
+    if (r1) {
+        T dvx_dx = (T(27) * (xr - x0) - (xr2 - xl)) * A.rdx24;
+        T dvz_dz = (T(27) * (z0 - zd) - (zu - zd2)) * A.rdz24;
+        T dvz_dx = (T(27) * (zr - z0) - (zr2 - zl)) * A.rdx24;
+        T dvx_dz = (T(27) * (x0 - xd) - (xu - xd2)) * A.rdz24;
         if (PML) {
-            const T ah = A.ah[c], bh = A.bh[c], ikh = A.ikh[c];
-            const T a = A.a[c], b = A.b[c], ik = A.ik[c];
-            dvx_dx = cpml_fix(dvx_dx, A.m[0][c], ah, bh, ikh);
-            dvx_dz = cpml_fix(dvx_dz, A.m[1][c], ah, bh, ikh);
-            dvz_dz = cpml_fix(dvz_dz, A.m[2][c], a, b, ik);
-            dvz_dx = cpml_fix(dvz_dx, A.m[3][c], a, b, ik);
+            dvx_dx = cpml_fix(dvx_dx, m[0], ah, bh, ikh);
+            dvx_dz = cpml_fix(dvx_dz, m[1], ah, bh, ikh);
+            dvz_dz = cpml_fix(dvz_dz, m[2], a, b, ik);
+            dvz_dx = cpml_fix(dvz_dx, m[3], a, b, ik);
         }
         const T shear = dvz_dx + dvx_dz;
-        A.sxx[c] = (A.sxx[c] + (A.c11[c] * dvx_dx + A.c13[c] * dvz_dz + A.c15[c] * shear) * A.dt) * A.igx[c];
-        A.szz[c] = (A.szz[c] + (A.c13[c] * dvx_dx + A.c33[c] * dvz_dz + A.c35[c] * shear) * A.dt) * A.igz[c];
+        s_xx = (s_xx + (k11 * dvx_dx + k13 * dvz_dz + k15 * shear) * A.dt) * gx;
+        s_zz = (s_zz + (k13 * dvx_dx + k33 * dvz_dz + k35 * shear) * A.dt) * gz;
     }
-    if (i >= 3 && i <= nx - 1 && j >= 2 && j <= nz - 2) {
-        T dvx_dx = (T(27) * (vx[c] - vx[c - sx]) - (vx[c + sx] - vx[c - 2 * sx])) * A.rdx24;
-        T dvz_dz = (T(27) * (vz[c + sz] - vz[c]) - (vz[c + 2 * sz] - vz[c - sz])) * A.rdz24;
-        T dvz_dx = (T(27) * (vz[c] - vz[c - sx]) - (vz[c + sx] - vz[c - 2 * sx])) * A.rdx24;
-        T dvx_dz = (T(27) * (vx[c + sz] - vx[c]) - (vx[c + 2 * sz] - vx[c - sz])) * A.rdz24;
+    if (r2) {
+        T dvx_dx = (T(27) * (x0 - xl) - (xr - xl2)) * A.rdx24;
+        T dvz_dz = (T(27) * (zu - z0) - (zu2 - zd)) * A.rdz24;
+        T dvz_dx = (T(27) * (z0 - zl) - (zr - zl2)) * A.rdx24;
+        T dvx_dz = (T(27) * (xu - x0) - (xu2 - xd)) * A.rdz24;
         if (PML) {
-            const T ah = A.ah[c], bh = A.bh[c], ikh = A.ikh[c];
-            dvx_dx = cpml_fix(dvx_dx, A.m[4][c], ah, bh, ikh);
-            dvx_dz = cpml_fix(dvx_dz, A.m[5][c], ah, bh, ikh);
-            dvz_dz = cpml_fix(dvz_dz, A.m[6][c], ah, bh, ikh);
-            dvz_dx = cpml_fix(dvz_dx, A.m[7][c], ah, bh, ikh);
+            dvx_dx = cpml_fix(dvx_dx, m[4], ah, bh, ikh);
+            dvx_dz = cpml_fix(dvx_dz, m[5], ah, bh, ikh);
+            dvz_dz = cpml_fix(dvz_dz, m[6], ah, bh, ikh);
+            dvz_dx = cpml_fix(dvz_dx, m[7], ah, bh, ikh);
         }
-        A.sxz[c] = (A.sxz[c] + (A.c15[c] * dvx_dx + A.c35[c] * dvz_dz + A.c55[c] * (dvz_dx + dvx_dz)) * A.dt) * A.igxz[c];
+        s_xz = (s_xz + (k15 * dvx_dx + k35 * dvz_dz + k55 * (dvz_dx + dvx_dz)) * A.dt) * gxz;
+    }
+    if (r1) {
+        A.sxx[c] = s_xx;
+        A.szz[c] = s_zz;
+        if (PML)
+            for (int q = 0; q < 4; ++q) A.m[q][c] = m[q];
+    }
+    if (r2) {
+        A.sxz[c] = s_xz;
+        if (PML)
+            for (int q = 4; q < 8; ++q) A.m[q][c] = m[q];
     }
 }
 
+// 3-term (reference) / 4-term (corrected) velocity stencils on operand values,
+// cpmlfdtd.f08:405-406 and 434-435: tminus over f(c), f(c-s), f(c-2s), f(c+s);
+// tplus over f(c), f(c+s), f(c+2s), f(c-s).
 template <typename T>
-__global__ void __launch_bounds__(256) el2_stress_kernel(El2Args<T> A)
+__device__ __forceinline__ T tminus(T f0, T fm1, T fm2, T fp1, int corr)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    if (i > A.nx || j > A.nz) return;
-    if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) el2_stress_cell<T, true>(A, i, j);
-    else el2_stress_cell<T, false>(A, i, j);
-}
-
-// 3-term (reference) / 4-term (corrected) velocity stencils,
-// cpmlfdtd.f08:405-406 and 434-435.
-template <typename T>
-__device__ __forceinline__ T tminus(const T *f, int64_t c, int64_t s, int corr)
-{
-    return corr ? T(27) * (f[c] - f[c - s]) - (f[c + s] - f[c - 2 * s])
-                : T(27) * f[c] - T(27) * f[c - s] + f[c - 2 * s];
+    return corr ? T(27) * (f0 - fm1) - (fp1 - fm2) : T(27) * f0 - T(27) * fm1 + fm2;
 }
 template <typename T>
-__device__ __forceinline__ T tplus(const T *f, int64_t c, int64_t s, int corr)
+__device__ __forceinline__ T tplus(T f0, T fp1, T fp2, T fm1, int corr)
 {
-    return corr ? T(27) * (f[c + s] - f[c]) - (f[c + 2 * s] - f[c - s])
-                : -T(27) * f[c] + T(27) * f[c + s] - f[c + 2 * s];
+    return corr ? T(27) * (fp1 - f0) - (fp2 - fm1) : -T(27) * f0 + T(27) * fp1 - fp2;
 }
 
 // vx (cpmlfdtd.f08:399-416), vz (428-446), Dirichlet edges (534-549) and the
-// per-step velocity norm (552-567) excluding the source cell.
+// squared velocity magnitude for the per-step norm (552-567), 0 at the source
+// cell.  Loads first, as in the stress update.  The norm is the square root
+// of the maximum squared magnitude (sqrt is monotone and correctly rounded,
+// so this equals the maximum of the magnitudes).
 template <typename T, bool PML>
 __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
     const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     const int64_t sx = 1, sz = A.pitch;
+    const int corr = A.corrected;
+    const bool edge = i == 1 || i == nx || j == 1 || j == nz;
+    const bool rx = i >= 3 && i <= nx - 1 && j >= 3 && j <= nz - 1;
+    const bool rz = i >= 2 && i <= nx - 2 && j >= 2 && j <= nz - 2;
     T vx = A.vx[c], vz = A.vz[c];
-    if (i == 1 || i == nx || j == 1 || j == nz) {
+    T r = 0, rxl = 0, rxd = 0, rzr = 0, rzu = 0;
+    T f[8] = {}, g[8] = {};
+    T a = 0, b = 0, ik = 0, ah = 0, bh = 0, ikh = 0, m[4] = {};
+    if (rx || rz) r = ldnt(A.rho + c);
+    if (rx) {
+        rxl = A.rho[c - sx];
+        rxd = A.rho[c - sz];
+        f[0] = A.sxx[c];
+        f[1] = A.sxx[c - sx];
+        f[2] = A.sxx[c - 2 * sx];
+        f[3] = corr ? A.sxx[c + sx] : T(0);
+        f[4] = A.sxz[c];
+        f[5] = A.sxz[c - sz];
+        f[6] = A.sxz[c - 2 * sz];
+        f[7] = corr ? A.sxz[c + sz] : T(0);
+        if (PML) {
+            a = ldnt(A.a + c);
+            b = ldnt(A.b + c);
+            ik = ldnt(A.ik + c);
+            m[0] = ldnt(A.m[8] + c);
+            m[1] = ldnt(A.m[9] + c);
+        }
+    }
+    if (rz) {
+        rzr = A.rho[c + sx];
+        rzu = A.rho[c + sz];
+        g[0] = A.sxz[c];
+        g[1] = A.sxz[c + sx];
+        g[2] = A.sxz[c + 2 * sx];
+        g[3] = corr ? A.sxz[c - sx] : T(0);
+        g[4] = A.szz[c];
+        g[5] = A.szz[c + sz];
+        g[6] = A.szz[c + 2 * sz];
+        g[7] = corr ? A.szz[c - sz] : T(0);
+        if (PML) {
+            ah = ldnt(A.ah + c);
+            bh = ldnt(A.bh + c);
+            ikh = ldnt(A.ikh + c);
+            m[2] = ldnt(A.m[10] + c);
+            m[3] = ldnt(A.m[11] + c);
+        }
+    }
+    if (edge) {
         vx = T(0);
         vz = T(0);
-        A.vx[c] = vx;
-        A.vz[c] = vz;
     } else {
-        const T r = A.rho[c];
-        if (i >= 3 && i <= nx - 1 && j >= 3 && j <= nz - 1) {
-            const T rhoxx = scalar_mean(r, A.rho[c - sx], A.density_code);
-            const T rhozx = scalar_mean(r, A.rho[c - sz], A.density_code);
-            T d1 = tminus(A.sxx, c, sx, A.corrected) * A.rdx24;
-            T d2 = tminus(A.sxz, c, sz, A.corrected) * A.rdz24;
+        if (rx) {
+            const T rhoxx = scalar_mean(r, rxl, A.density_code);
+            const T rhozx = scalar_mean(r, rxd, A.density_code);
+            T d1 = tminus(f[0], f[1], f[2], f[3], corr) * A.rdx24;
+            T d2 = tminus(f[4], f[5], f[6], f[7], corr) * A.rdz24;
             if (PML) {
-                const T a = A.a[c], b = A.b[c], ik = A.ik[c];
-                d1 = cpml_fix(d1, A.m[8][c], a, b, ik);
-                d2 = cpml_fix(d2, A.m[9][c], a, b, ik);
+                d1 = cpml_fix(d1, m[0], a, b, ik);
+                d2 = cpml_fix(d2, m[1], a, b, ik);
             }
             vx = vx + A.dt * (d1 / rhoxx + d2 / rhozx);
-            A.vx[c] = vx;
         }
-        if (i >= 2 && i <= nx - 2 && j >= 2 && j <= nz - 2) {
-            const T rhoxz = scalar_mean(r, A.rho[c + sx], A.density_code);
-            const T rhozz = scalar_mean(r, A.rho[c + sz], A.density_code);
-            T d1 = tplus(A.sxz, c, sx, A.corrected) * A.rdx24;
-            T d2 = tplus(A.szz, c, sz, A.corrected) * A.rdz24;
+        if (rz) {
+            const T rhoxz = scalar_mean(r, rzr, A.density_code);
+            const T rhozz = scalar_mean(r, rzu, A.density_code);
+            T d1 = tplus(g[0], g[1], g[2], g[3], corr) * A.rdx24;
+            T d2 = tplus(g[4], g[5], g[6], g[7], corr) * A.rdz24;
             if (PML) {
-                const T ah = A.ah[c], bh = A.bh[c], ikh = A.ikh[c];
-                d1 = cpml_fix(d1, A.m[10][c], ah, bh, ikh);
-                d2 = cpml_fix(d2, A.m[11][c], ah, bh, ikh);
+                d1 = cpml_fix(d1, m[2], ah, bh, ikh);
+                d2 = cpml_fix(d2, m[3], ah, bh, ikh);
             }
             vz = vz + A.dt * (d1 / rhoxz + d2 / rhozz);
-            A.vz[c] = vz;
         }
+    }
+    if (edge || rx) A.vx[c] = vx;
+    if (edge || rz) A.vz[c] = vz;
+    if (PML && rx) {
+        A.m[8][c] = m[0];
+        A.m[9][c] = m[1];
+    }
+    if (PML && rz) {
+        A.m[10][c] = m[2];
+        A.m[11][c] = m[3];
     }
     if (c == A.src_cell) return 0.0;
     const double dvx = (double)vx, dvz = (double)vz;
-    return sqrt(dvx * dvx + dvz * dvz);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) el2_velocity_kernel(El2Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.nz) {
-        if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = el2_velocity_cell<T, true>(A, i, j);
-        else nrm = el2_velocity_cell<T, false>(A, i, j);
-    }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+    return dvx * dvx + dvz * dvz;
 }
 
 template <typename T>
 static El2Args<T> el2_args(Device &d)
+
 {
     El2Args<T> A;
     A.vx = (T *)d.field[SDT_F_VX];
@@ -205,39 +293,92 @@ static El2Args<T> el2_args(Device &d)
     return A;
 }
 
-// ---- batched shots: one launch covers the tiles of every shot --------------
-// CPML classification per cell from the shot's 64 x 4 tile flags, so any
-// tile height works.
+// ---- tiles ------------------------------------------------------------------
+// A tile is 64 consecutive elements of a padded row (line-aligned: element
+// e = xoff + i - 1 of a row, tile x index e / 64) by kTile2dWaves * R rows;
+// wavefront w marches rows R*w .. R*w + R - 1 of the band.  Single-shot
+// launches index tiles by blockIdx; batched launches read (shot, tile x,
+// band) from a tile list covering every shot.  The CPML body runs for a
+// wavefront row when any of its cells needs it (outside the interior box or
+// in a 64 x 4 block with non-identity coefficients); identity cells give the
+// same values through either body.
 template <typename T>
 __device__ __forceinline__ bool el2_pml(const El2Args<T> &A, int i, int j)
 {
     return !in_box(A.box, i, 1, j) || A.flags[((i - 1) >> 6) + (size_t)A.tiles_x * ((j - 1) >> 2)];
 }
 
-template <typename T, int TY>
-__global__ void __launch_bounds__(64 * TY) el2_stress_batch_kernel(const El2Args<T> *args, const int4 *tiles)
+template <typename T, int R>
+__device__ __forceinline__ void el2_stress_tile(const El2Args<T> &A, int tx, int band)
 {
-    const int4 t = tiles[blockIdx.x];
-    const El2Args<T> &A = args[t.x];
-    if (*A.d_it > A.nsteps) return;
-    const int i = t.y * 64 + threadIdx.x + 1;
-    const int j = t.z * TY + threadIdx.y + 1;
-    if (i > A.nx || j > A.nz) return;
-    if (el2_pml(A, i, j)) el2_stress_cell<T, true>(A, i, j);
-    else el2_stress_cell<T, false>(A, i, j);
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        if (__any(valid && el2_pml(A, i, j))) {
+            if (valid) el2_stress_cell<T, true>(A, i, j);
+        } else if (valid) {
+            el2_stress_cell<T, false>(A, i, j);
+        }
+    }
 }
 
-template <typename T, int TY>
-__global__ void __launch_bounds__(64 * TY) el2_velocity_batch_kernel(const El2Args<T> *args, const int4 *tiles)
+template <typename T, int R>
+__device__ __forceinline__ double el2_velocity_tile(const El2Args<T> &A, int tx, int band)
+{
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    double nrm = 0.0;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        double v = 0.0;
+        if (__any(valid && el2_pml(A, i, j))) {
+            if (valid) v = el2_velocity_cell<T, true>(A, i, j);
+        } else if (valid) {
+            v = el2_velocity_cell<T, false>(A, i, j);
+        }
+        nrm = (v > nrm || v != v) ? v : nrm;
+    }
+    return nrm;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_kernel(El2Args<T> A)
+{
+    el2_stress_tile<T, R>(A, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_velocity_kernel(El2Args<T> A)
+{
+    const double nrm = sqrt(el2_velocity_tile<T, R>(A, blockIdx.x, blockIdx.y));
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_batch_kernel(const El2Args<T> *args,
+                                                                             const int4 *tiles)
 {
     const int4 t = tiles[blockIdx.x];
-    const El2Args<T> &A = args[t.x];
-    if (*A.d_it > A.nsteps) return;   // uniform per workgroup
-    const int i = t.y * 64 + threadIdx.x + 1;
-    const int j = t.z * TY + threadIdx.y + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.nz)
-        nrm = el2_pml(A, i, j) ? el2_velocity_cell<T, true>(A, i, j) : el2_velocity_cell<T, false>(A, i, j);
+    if (t.x < 0) return;   // padding tile
+    const El2Args<T> A = args[t.x];
+    if (*A.d_it > A.nsteps) return;   // finished shot (uniform per workgroup)
+    el2_stress_tile<T, R>(A, t.y, t.z);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_velocity_batch_kernel(const El2Args<T> *args,
+                                                                               const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;   // padding tile
+    const El2Args<T> A = args[t.x];
+    if (*A.d_it > A.nsteps) return;
+    const double nrm = sqrt(el2_velocity_tile<T, R>(A, t.y, t.z));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -257,20 +398,20 @@ int el2_batch_args(Batch &b, Device *const *shots, int n)
     return b.precision == SDT_FP64 ? el2_batch_args_t<double>(b, shots, n) : el2_batch_args_t<float>(b, shots, n);
 }
 
-template <typename T, int TY>
+template <typename T, int R>
 static void el2_batch_launch_t(Batch &b, int which)
 {
     const El2Args<T> *a = (const El2Args<T> *)b.args;
-    const dim3 grid(b.ntiles[which]), block(64, TY);
-    hipLaunchKernelGGL((el2_stress_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
-    hipLaunchKernelGGL((el2_velocity_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
+    const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((el2_stress_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
+    hipLaunchKernelGGL((el2_velocity_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
 }
 
 template <typename T>
 static void el2_batch_launch_p(Batch &b, int which)
 {
-    switch (kBatchTileRows[which]) {
-    case 2: el2_batch_launch_t<T, 2>(b, which); break;
+    switch (kBatchShapes[which].rows) {
+    case 1: el2_batch_launch_t<T, 1>(b, which); break;
     case 8: el2_batch_launch_t<T, 8>(b, which); break;
     default: el2_batch_launch_t<T, 4>(b, which); break;
     }
@@ -285,12 +426,13 @@ void el2_batch_launch(Batch &b, int which)
 template <typename T>
 static void el2_step_t(Device &d, bool timed)
 {
+    constexpr int R = kSingleRows2d;
     const El2Args<T> A = el2_args<T>(d);
-    const dim3 grid(d.tiles_x, d.tiles_z), block(64, 4);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    hipLaunchKernelGGL(el2_stress_kernel<T>, grid, block, 0, d.stream, A);
+    hipLaunchKernelGGL((el2_stress_kernel<T, R>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL(el2_velocity_kernel<T>, grid, block, 0, d.stream, A);
+    hipLaunchKernelGGL((el2_velocity_kernel<T, R>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }

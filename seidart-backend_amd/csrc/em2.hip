@@ -36,35 +36,42 @@ struct Em2Args {
     int nsteps;         // time_steps of the run (batched shots stop there)
 };
 
+// Hy (cpmlfdtd.f08:2576-2593); every operand is read before the first store
 template <typename T, bool PML>
 __device__ __forceinline__ void em2_h_cell(const Em2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
     const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     if (i == nx || j == nz) { A.Hy[c] = T(0); return; }
-    // cpmlfdtd.f08:2576-2593
-    T dEx_dz = (A.Ex[c + A.pitch] - A.Ex[c]) * A.rdz;
-    T dEz_dx = (A.Ez[c + 1] - A.Ez[c]) * A.rdx;
-    if (PML) {
-        A.m_dEx_dz[c] = A.b[c] * A.m_dEx_dz[c] + A.a[c] * dEx_dz;
-        A.m_dEz_dx[c] = A.bh[c] * A.m_dEz_dx[c] + A.ah[c] * dEz_dx;
-        dEz_dx = dEz_dx * A.ikh[c] + A.m_dEz_dx[c];
-        dEx_dz = dEx_dz * A.ik[c] + A.m_dEx_dz[c];
-    }
+    const T ex0 = A.Ex[c], exu = A.Ex[c + A.pitch], ez0 = A.Ez[c], ezr = A.Ez[c + 1];
     const T hold = (i == 1 || j == 1) ? T(0) : A.Hy[c];
+    T mx = 0, mz = 0, a = 0, b = 0, ik = 0, ah = 0, bh = 0, ikh = 0;
+    if (PML) {
+        mx = ldnt(A.m_dEx_dz + c);
+        mz = ldnt(A.m_dEz_dx + c);
+        a = ldnt(A.a + c);
+        b = ldnt(A.b + c);
+        ik = ldnt(A.ik + c);
+        ah = ldnt(A.ah + c);
+        bh = ldnt(A.bh + c);
+        ikh = ldnt(A.ikh + c);
+    }
+    T dEx_dz = (exu - ex0) * A.rdz;
+    T dEz_dx = (ezr - ez0) * A.rdx;
+    if (PML) {
+        mx = b * mx + a * dEx_dz;
+        mz = bh * mz + ah * dEz_dx;
+        dEz_dx = dEz_dx * ikh + mz;
+        dEx_dz = dEx_dz * ik + mx;
+        A.m_dEx_dz[c] = mx;
+        A.m_dEz_dx[c] = mz;
+    }
     A.Hy[c] = hold + A.dbHy * (dEz_dx - dEx_dz);
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) em2_h_kernel(Em2Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    if (i > A.nx || j > A.nz) return;
-    if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) em2_h_cell<T, true>(A, i, j);
-    else em2_h_cell<T, false>(A, i, j);
-}
-
+// Ex, Ez (cpmlfdtd.f08:2605-2620), zero edges, and the squared field
+// magnitude for the per-step norm, 0 at the source cell (the norm is the
+// square root of the maximum, which equals the maximum of the magnitudes)
 template <typename T, bool PML>
 __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
 {
@@ -74,42 +81,44 @@ __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
     if (i == 1 || i == nx || j == 1 || j == nz) {
         ex = ez = T(0);
     } else {
-        // cpmlfdtd.f08:2605-2620
-        T dHy_dz = (A.Hy[c] - A.Hy[c - A.pitch]) * A.rdz;
-        T dHy_dx = (A.Hy[c] - A.Hy[c - 1]) * A.rdx;
-        if (PML) {
-            A.m_dHy_dx[c] = A.bh[c] * A.m_dHy_dx[c] + A.ah[c] * dHy_dx;
-            A.m_dHy_dz[c] = A.b[c] * A.m_dHy_dz[c] + A.a[c] * dHy_dz;
-            dHy_dz = dHy_dz * A.ik[c] + A.m_dHy_dz[c];
-            dHy_dx = dHy_dx * A.ikh[c] + A.m_dHy_dx[c];
-        }
+        const T h0 = A.Hy[c], hd = A.Hy[c - A.pitch], hl = A.Hy[c - 1];
         const T exo = A.Ex[c], ezo = A.Ez[c];
-        const T bEx = A.bEx[c];
-        ez = ezo + (-bEx * dHy_dz + A.bEz[c] * dHy_dx - A.dEz[c] * exo - A.eEz[c] * ezo) * A.dt;
-        ex = exo + (-A.aEx[c] * dHy_dz + bEx * dHy_dx - A.dEx[c] * exo - A.eEx[c] * ezo) * A.dt;
+        const T bEx = ldnt(A.bEx + c), bEz = ldnt(A.bEz + c), dEz = ldnt(A.dEz + c), eEz = ldnt(A.eEz + c);
+        const T aEx = ldnt(A.aEx + c), dEx = ldnt(A.dEx + c), eEx = ldnt(A.eEx + c);
+        T mx = 0, mz = 0, a = 0, b = 0, ik = 0, ah = 0, bh = 0, ikh = 0;
+        if (PML) {
+            mx = ldnt(A.m_dHy_dx + c);
+            mz = ldnt(A.m_dHy_dz + c);
+            a = ldnt(A.a + c);
+            b = ldnt(A.b + c);
+            ik = ldnt(A.ik + c);
+            ah = ldnt(A.ah + c);
+            bh = ldnt(A.bh + c);
+            ikh = ldnt(A.ikh + c);
+        }
+        T dHy_dz = (h0 - hd) * A.rdz;
+        T dHy_dx = (h0 - hl) * A.rdx;
+        if (PML) {
+            mx = bh * mx + ah * dHy_dx;
+            mz = b * mz + a * dHy_dz;
+            dHy_dz = dHy_dz * ik + mz;
+            dHy_dx = dHy_dx * ikh + mx;
+            A.m_dHy_dx[c] = mx;
+            A.m_dHy_dz[c] = mz;
+        }
+        ez = ezo + (-bEx * dHy_dz + bEz * dHy_dx - dEz * exo - eEz * ezo) * A.dt;
+        ex = exo + (-aEx * dHy_dz + bEx * dHy_dx - dEx * exo - eEx * ezo) * A.dt;
     }
     A.Ex[c] = ex;
     A.Ez[c] = ez;
     if (c == A.src_cell) return 0.0;
     const double x = ex, z = ez;
-    return sqrt(x * x + z * z);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) em2_e_kernel(Em2Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.nz) {
-        if (!in_box(A.box, i, 1, j) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y]) nrm = em2_e_cell<T, true>(A, i, j);
-        else nrm = em2_e_cell<T, false>(A, i, j);
-    }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+    return x * x + z * z;
 }
 
 template <typename T>
 static Em2Args<T> em2_args(Device &d)
+
 {
     Em2Args<T> A;
     A.Ex = (T *)d.field[SDT_F_EX];
@@ -145,36 +154,86 @@ static Em2Args<T> em2_args(Device &d)
     return A;
 }
 
-// ---- batched shots: one launch covers the tiles of every shot --------------
+// ---- tiles ------------------------------------------------------------------
+// Same tiling as the 2D elastic solver: 64 line-aligned columns x four
+// wavefronts marching R rows each; the CPML body runs for a wavefront row
+// when any of its cells needs it (identity cells give the same values
+// through either body).
 template <typename T>
 __device__ __forceinline__ bool em2_pml(const Em2Args<T> &A, int i, int j)
 {
     return !in_box(A.box, i, 1, j) || A.flags[((i - 1) >> 6) + (size_t)A.tiles_x * ((j - 1) >> 2)];
 }
 
-template <typename T, int TY>
-__global__ void __launch_bounds__(64 * TY) em2_h_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
+template <typename T, int R>
+__device__ __forceinline__ void em2_h_tile(const Em2Args<T> &A, int tx, int band)
 {
-    const int4 t = tiles[blockIdx.x];
-    const Em2Args<T> &A = args[t.x];
-    if (*A.d_it > A.nsteps) return;
-    const int i = t.y * 64 + threadIdx.x + 1;
-    const int j = t.z * TY + threadIdx.y + 1;
-    if (i > A.nx || j > A.nz) return;
-    if (em2_pml(A, i, j)) em2_h_cell<T, true>(A, i, j);
-    else em2_h_cell<T, false>(A, i, j);
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        if (__any(valid && em2_pml(A, i, j))) {
+            if (valid) em2_h_cell<T, true>(A, i, j);
+        } else if (valid) {
+            em2_h_cell<T, false>(A, i, j);
+        }
+    }
 }
 
-template <typename T, int TY>
-__global__ void __launch_bounds__(64 * TY) em2_e_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
+template <typename T, int R>
+__device__ __forceinline__ double em2_e_tile(const Em2Args<T> &A, int tx, int band)
+{
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    double nrm = 0.0;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        double v = 0.0;
+        if (__any(valid && em2_pml(A, i, j))) {
+            if (valid) v = em2_e_cell<T, true>(A, i, j);
+        } else if (valid) {
+            v = em2_e_cell<T, false>(A, i, j);
+        }
+        nrm = (v > nrm || v != v) ? v : nrm;
+    }
+    return nrm;
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_h_kernel(Em2Args<T> A)
+{
+    em2_h_tile<T, R>(A, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_e_kernel(Em2Args<T> A)
+{
+    const double nrm = sqrt(em2_e_tile<T, R>(A, blockIdx.x, blockIdx.y));
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_h_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
 {
     const int4 t = tiles[blockIdx.x];
-    const Em2Args<T> &A = args[t.x];
-    if (*A.d_it > A.nsteps) return;   // uniform per workgroup
-    const int i = t.y * 64 + threadIdx.x + 1;
-    const int j = t.z * TY + threadIdx.y + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.nz) nrm = em2_pml(A, i, j) ? em2_e_cell<T, true>(A, i, j) : em2_e_cell<T, false>(A, i, j);
+    if (t.x < 0) return;   // padding tile
+    const Em2Args<T> A = args[t.x];
+    if (*A.d_it > A.nsteps) return;   // finished shot (uniform per workgroup)
+    em2_h_tile<T, R>(A, t.y, t.z);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_e_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
+{
+    const int4 t = tiles[blockIdx.x];
+    if (t.x < 0) return;   // padding tile
+    const Em2Args<T> A = args[t.x];
+    if (*A.d_it > A.nsteps) return;
+    const double nrm = sqrt(em2_e_tile<T, R>(A, t.y, t.z));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -194,20 +253,20 @@ int em2_batch_args(Batch &b, Device *const *shots, int n)
     return b.precision == SDT_FP64 ? em2_batch_args_t<double>(b, shots, n) : em2_batch_args_t<float>(b, shots, n);
 }
 
-template <typename T, int TY>
+template <typename T, int R>
 static void em2_batch_launch_t(Batch &b, int which)
 {
     const Em2Args<T> *a = (const Em2Args<T> *)b.args;
-    const dim3 grid(b.ntiles[which]), block(64, TY);
-    hipLaunchKernelGGL((em2_h_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
-    hipLaunchKernelGGL((em2_e_batch_kernel<T, TY>), grid, block, 0, b.stream, a, b.tiles[which]);
+    const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((em2_h_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
+    hipLaunchKernelGGL((em2_e_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
 }
 
 template <typename T>
 static void em2_batch_launch_p(Batch &b, int which)
 {
-    switch (kBatchTileRows[which]) {
-    case 2: em2_batch_launch_t<T, 2>(b, which); break;
+    switch (kBatchShapes[which].rows) {
+    case 1: em2_batch_launch_t<T, 1>(b, which); break;
     case 8: em2_batch_launch_t<T, 8>(b, which); break;
     default: em2_batch_launch_t<T, 4>(b, which); break;
     }
@@ -222,12 +281,13 @@ void em2_batch_launch(Batch &b, int which)
 template <typename T>
 static void em2_step_t(Device &d, bool timed)
 {
+    constexpr int R = kSingleRows2d;
     const Em2Args<T> A = em2_args<T>(d);
-    const dim3 grid(d.tiles_x, d.tiles_z), block(64, 4);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    hipLaunchKernelGGL(em2_h_kernel<T>, grid, block, 0, d.stream, A);
+    hipLaunchKernelGGL((em2_h_kernel<T, R>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL(em2_e_kernel<T>, grid, block, 0, d.stream, A);
+    hipLaunchKernelGGL((em2_e_kernel<T, R>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }

@@ -637,9 +637,9 @@ int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_
     // autotune: each candidate tile height runs one untimed and kTune timed
     // real steps (every height gives the same results), the fastest is kept
     constexpr int kTune = 4;
-    if (B.choice < 0 && todo >= 3 * (kTune + 1)) {
+    if (B.choice < 0 && todo >= kNumBatchShapes * (kTune + 1)) {
         float best = 0;
-        for (int w = 0; w < 3; ++w) {
+        for (int w = 0; w < kNumBatchShapes; ++w) {
             batch_step(B, w);
             (void)hipEventRecord(b->ev[0], B.stream);
             for (int s = 0; s < kTune; ++s) batch_step(B, w);
@@ -649,7 +649,7 @@ int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_
             (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
             if (B.choice < 0 || ms < best) { best = ms; B.choice = w; }
         }
-        done += 3 * (kTune + 1);
+        done += kNumBatchShapes * (kTune + 1);
     }
     const int w = B.choice < 0 ? 1 : B.choice;
     constexpr int kSteps = 32;
@@ -695,8 +695,13 @@ int32_t sdt_batch_info(const sdt_batch *b, int32_t key, int64_t *value)
 {
     if (!b || !value) return fail(nullptr, SDT_ERR_ARG, "null argument");
     switch (key) {
-    case SDT_BATCH_INFO_TILE_ROWS: *value = b->b.choice < 0 ? 0 : kBatchTileRows[b->b.choice]; return SDT_OK;
-    case SDT_BATCH_INFO_TILES: *value = b->b.ntiles[b->b.choice < 0 ? 1 : b->b.choice]; return SDT_OK;
+    case SDT_BATCH_INFO_TILE_ROWS:
+        *value = b->b.choice < 0 ? 0 : (int64_t)kTile2dWaves * kBatchShapes[b->b.choice].rows;
+        return SDT_OK;
+    case SDT_BATCH_INFO_XCD_ORDER:
+        *value = b->b.choice < 0 ? -1 : (int64_t)kBatchShapes[b->b.choice].xcd_order;
+        return SDT_OK;
+    case SDT_BATCH_INFO_TILES: *value = b->b.ntiles[b->b.choice < 0 ? 0 : b->b.choice]; return SDT_OK;
     }
     return fail(nullptr, SDT_ERR_ARG, "unknown batch info key");
 }

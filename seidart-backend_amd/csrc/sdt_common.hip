@@ -114,13 +114,24 @@ int batch_setup(Batch &b, Device *const *shots, int n)
     b.precision = shots[0]->precision;
     b.stream = shots[0]->stream;
     // tile lists for every candidate tile height
-    for (int w = 0; w < 3; ++w) {
-        const int ty = kBatchTileRows[w];
+    for (int w = 0; w < kNumBatchShapes; ++w) {
         std::vector<int4> t;
         for (int s = 0; s < n; ++s) {
-            const int tx = (shots[s]->nx + 63) / 64, tz = (shots[s]->nz + ty - 1) / ty;
+            const int tx = tiles_x_2d(*shots[s]), tz = tiles_z_2d(*shots[s], kBatchShapes[w].rows);
             for (int z = 0; z < tz; ++z)
                 for (int x = 0; x < tx; ++x) t.push_back(make_int4(s, x, z, 0));
+        }
+        if (kBatchShapes[w].xcd_order) {
+            // natural tile q: group g = q / 8 goes to XCD g % 8, slot
+            // (g / 8) * 8 + q % 8; list position = slot * 8 + XCD.  Padding
+            // tiles (shot -1) fill the list to a multiple of 64.
+            t.resize((t.size() + 63) / 64 * 64, make_int4(-1, 0, 0, 0));
+            std::vector<int4> p(t.size());
+            for (size_t q = 0; q < t.size(); ++q) {
+                const size_t g = q / 8, slot = (g / 8) * 8 + q % 8;
+                p[slot * 8 + g % 8] = t[q];
+            }
+            t.swap(p);
         }
         b.ntiles[w] = (int)t.size();
         if (hipMalloc((void **)&b.tiles[w], t.size() * sizeof(int4)) != hipSuccess) return SDT_ERR_HIP;
@@ -146,7 +157,7 @@ int batch_setup(Batch &b, Device *const *shots, int n)
 
 void batch_free(Batch &b)
 {
-    for (int w = 0; w < 3; ++w)
+    for (int w = 0; w < kNumBatchShapes; ++w)
         if (b.tiles[w]) (void)hipFree(b.tiles[w]);
     if (b.fps) (void)hipFree(b.fps);
     if (b.tails) (void)hipFree(b.tails);

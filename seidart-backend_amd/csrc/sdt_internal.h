@@ -151,12 +151,28 @@ void em3_step(Device &d, bool timed);
 // every shot (tile list entries name the shot), one tail launch covers every
 // shot's sources and receivers.  A shot whose step counter passed its
 // time_steps is skipped, so shots of different lengths share the launches.
-constexpr int kBatchTileRows[3] = {2, 4, 8};   // tile heights the autotuner tries
+// 2D tiles: 64 line-aligned columns x kTile2dWaves wavefronts, each marching
+// R consecutive rows; single-shot launches use R = kSingleRows2d.  A batch
+// times each of kBatchShapes (rows per wavefront; tile-list order) on its
+// first steps and keeps the fastest.  In XCD order the tile list is permuted
+// so that runs of 8 x-consecutive tiles land on one XCD (workgroups are
+// dealt to the 8 XCDs round-robin by index), letting the column halos of a
+// tile hit in that XCD's L2.
+constexpr int kTile2dWaves = 4;
+constexpr int kSingleRows2d = 4;
+struct BatchShape {
+    int rows;
+    bool xcd_order;
+};
+constexpr int kNumBatchShapes = 4;
+constexpr BatchShape kBatchShapes[kNumBatchShapes] = {{4, false}, {8, false}, {4, true}, {8, true}};
+inline int tiles_x_2d(const Device &d) { return (int)((d.xoff + d.nx + 63) / 64); }
+inline int tiles_z_2d(const Device &d, int rows) { return (d.nz + kTile2dWaves * rows - 1) / (kTile2dWaves * rows); }
 struct Batch {
     int physics = 0, precision = 0, nshots = 0;
     void *args = nullptr;          // per-shot kernel arguments (device)
-    int4 *tiles[3] = {};           // per tile height: (shot, tile x, tile z, 0)
-    int ntiles[3] = {};
+    int4 *tiles[kNumBatchShapes] = {};   // per shape: (shot or -1, tile x, band, 0)
+    int ntiles[kNumBatchShapes] = {};
     void *fps = nullptr;           // per-shot field pointers (device)
     TailArgs *tails = nullptr;     // per-shot tail arguments (device)
     int choice = -1;               // autotuned tile height index, -1 = not yet
@@ -164,7 +180,7 @@ struct Batch {
 };
 int batch_setup(Batch &b, Device *const *shots, int n);
 void batch_free(Batch &b);
-// one step of every unfinished shot with tile height kBatchTileRows[which]
+// one step of every unfinished shot with tile shape kBatchShapes[which]
 void batch_step(Batch &b, int which);
 int el2_batch_args(Batch &b, Device *const *shots, int n);
 int em2_batch_args(Batch &b, Device *const *shots, int n);

@@ -70,7 +70,7 @@ EXPORTS = [
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
     1, 2, 3, 4, 5, 6, 7)
 INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES = 1, 2, 3
-BATCH_INFO_TILE_ROWS, BATCH_INFO_TILES = 1, 2
+BATCH_INFO_TILE_ROWS, BATCH_INFO_TILES, BATCH_INFO_XCD_ORDER = 1, 2, 3
 
 
 class SdtError(RuntimeError):

@@ -194,8 +194,10 @@ def layer_index(nz_pad, cpml, nlayers, rng, min_thick=4):
 # ------------------------------------------------------------ configurations ----
 def elastic2(nx=200, nz=200, dx=1.0, dz=1.0, cpml=10, steps=1000, seed=0, nlayers=2,
              rho=(917.0, 2000.0), f0=50.0, src=(100, 40), rec_z=40, nrec=20, cfl=0.5,
-             split=None, gamma=0.0, density_method="arithmetic", with_cpml=True):
-    """2D elastic, two seeded ice-like layers (the 200x200, 1000-step case by default)."""
+             split=None, gamma=0.0, density_method="arithmetic", with_cpml=True, tilt=(0.0, 30.0)):
+    """2D elastic, two seeded ice-like layers (the 200x200, 1000-step case by
+    default); each layer's symmetry axis is tilted by a uniform draw from
+    `tilt` degrees."""
     rng = np.random.default_rng(seed)
     NX, NZ = nx + 2 * cpml, nz + 2 * cpml
     if split is None and nlayers == 2:
@@ -205,7 +207,7 @@ def elastic2(nx=200, nz=200, dx=1.0, dz=1.0, cpml=10, steps=1000, seed=0, nlayer
         lid = layer_index(NZ, cpml, nlayers, rng)
     Cs, rhos = [], []
     for q in range(max(nlayers, 2)):
-        C = rotate_voigt(ice_voigt(rng), rot_y(math.radians(rng.uniform(0.0, 30.0))))
+        C = rotate_voigt(ice_voigt(rng), rot_y(math.radians(rng.uniform(*tilt))))
         Cs.append(C)
         rhos.append(rho[q % len(rho)])
     arrays = {}
@@ -402,15 +404,19 @@ def em2(nx=600, nz=300, d=0.05, cpml=10, steps=3000, seed=4, nlayers=6, f0=100e6
 
 
 def batch2d(physics="elastic2", nshots=64, seed=3, nx=(300, 800), nz=(150, 400), steps=(1500, 3000),
-            nlayers=4, stencil="reference"):
+            nlayers=4, stencil="reference", tilt=(0.0, 30.0)):
     """Uneven multi-shot 2D set: per shot a seeded grid size (nx, nz uniform in
     the given ranges), its own `nlayers`-layer anisotropic model (as
     `elastic2` / `em2`), its own source position and step count.
 
-    stencil (elastic only): the reference's 3-term velocity differences
-    (cpmlfdtd.f08:405-446) carry a zero-order term that makes long runs grow
-    without bound in many of these models; "corrected" selects the 4-term
-    differences, which cost the same memory traffic."""
+    Elastic only: `tilt` is the range of symmetry-axis tilts in degrees and
+    `stencil` the velocity differences.  The reference's stress update adds
+    c15 * (dvz_dx + dvx_dz) to sigma_xx with both shear derivatives taken at
+    the sigma_xx point without averaging (cpmlfdtd.f08:325-345); for tilted
+    axes (c15, c35 != 0) that coupling grows exponentially over a few
+    thousand steps, so long runs need tilt (0, 0).  The reference's 3-term
+    velocity differences (cpmlfdtd.f08:405-446) add a slower zero-order
+    growth; "corrected" selects the 4-term differences."""
     rng = np.random.default_rng(seed)
     shots = []
     for q in range(nshots):
@@ -421,7 +427,7 @@ def batch2d(physics="elastic2", nshots=64, seed=3, nx=(300, 800), nz=(150, 400),
         sd = int(rng.integers(0, 2**31 - 1))
         if physics == "elastic2":
             shots.append(elastic2(nx=sx, nz=sz, steps=n, seed=sd, nlayers=nlayers, src=src,
-                                  rec_z=src[1], nrec=20))
+                                  rec_z=src[1], nrec=20, tilt=tilt))
             shots[-1]["stencil"] = stencil
         else:
             shots.append(em2(nx=sx, nz=sz, steps=n, seed=sd, nlayers=nlayers, src=src, nrec=20))

@@ -246,8 +246,11 @@ class Batch:
             raise
 
     def info(self, key):
-        """'tile_rows' (the chosen tile height, 0 before tuning) or 'tiles'."""
-        k = {"tile_rows": L.BATCH_INFO_TILE_ROWS, "tiles": L.BATCH_INFO_TILES}[key]
+        """'tile_rows' (rows per tile of the chosen shape, 0 before tuning),
+        'tiles' (workgroups per launch) or 'xcd_order' (1 when the chosen
+        tile list is ordered for XCD locality, -1 before tuning)."""
+        k = {"tile_rows": L.BATCH_INFO_TILE_ROWS, "tiles": L.BATCH_INFO_TILES,
+             "xcd_order": L.BATCH_INFO_XCD_ORDER}[key]
         v = C.c_int64(0)
         L.check(L.lib.sdt_batch_info(self._b, k, C.byref(v)))
         return v.value

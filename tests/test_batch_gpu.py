@@ -50,7 +50,8 @@ def test_batch_equals_single_runs(physics, precision):
             assert all(s.steps_done == min(50, c["time_steps"]) for s, c in zip(shots, cfgs))
             b.run()         # graph replays, then the remainder; short shots stop early
             rows = b.info("tile_rows")
-            assert rows in (2, 4, 8)
+            assert rows in (4, 16, 32)
+            assert b.info("xcd_order") in (0, 1)
             print(f"[batch] tile rows {rows}, tiles {b.info('tiles')}", flush=True)
         for q, (s, c) in enumerate(zip(shots, cfgs)):
             assert s.steps_done == c["time_steps"]
