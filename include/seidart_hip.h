@@ -228,7 +228,8 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
 enum {
     SDT_INFO_MATERIALS = 1,     /* distinct materials in the material table, 0 = per-cell arrays */
     SDT_INFO_SHELL_TILES = 2,   /* 3D elastic CPML-shell tiles                                    */
-    SDT_INFO_DEVICE_BYTES = 3   /* device memory held by the handle's grid arrays                 */
+    SDT_INFO_DEVICE_BYTES = 3,  /* device memory held by the handle's grid arrays                 */
+    SDT_INFO_COEFFICIENTS = 4   /* update coefficients per cell (2D EM 7, 2D elastic 10, else 0)  */
 };
 int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 
@@ -253,6 +254,48 @@ int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_
 enum { SDT_BATCH_INFO_TILE_ROWS = 1, SDT_BATCH_INFO_TILES = 2, SDT_BATCH_INFO_XCD_ORDER = 3 };
 int32_t sdt_batch_info(const sdt_batch *b, int32_t key, int64_t *value);
 int32_t sdt_batch_destroy(sdt_batch *b);
+
+/* ---- adjoint-state gradients (2D EM, 2D elastic) ------------------------------
+ * No reference counterpart: the reference has no inversion path.  These
+ * entry points give the exact gradient of any function of the receiver
+ * traces (through its gradient with respect to the traces) with respect to
+ * the solver's per-cell update coefficients and to the source series, by the
+ * discrete adjoint of the time stepping.
+ *
+ * Update coefficients (per cell, nx*nz each, in this order):
+ *   2D EM:      aEx, bEx, bEz, dEx, eEx, dEz, eEz -- the 2x2 inverse
+ *               permittivity and conductivity products of cpmlfdtd.f08:2519-2540
+ *               (aEx = e33/det, bEx = -e13/det, bEz = e11/det, dEx = (e33 s11 -
+ *               e13 s13)/det, eEx = (e33 s13 - e13 s33)/det, dEz = (e11 s13 -
+ *               e13 s11)/det, eEz = (e11 s33 - e13 s13)/det, permittivity
+ *               scaled by eps0)
+ *   2D elastic: c11, c13, c15, c33, c35, c55, rho, 1/(1 + gamma_x dt),
+ *               1/(1 + gamma_z dt), 1/(1 + gamma_xz dt)
+ * Chain them to permittivity, conductivity, stiffness or attenuation on the
+ * host (seidart_hip.fwi does so with torch autograd). */
+enum {
+    SDT_OPT_CHECKPOINT = 8,     /* steps between saved forward states for sdt_adjoint
+                                   (0 = none, default); set before the first step      */
+    SDT_OPT_RAW_SOURCE = 9      /* 1 = source series are added to the fields as given
+                                   (no dt / permittivity or density scaling); set before
+                                   the first step                                         */
+};
+/* replace the model inputs by the update coefficients above (count = ncoef *
+ * nx * nz, coefficient-major); needs SDT_OPT_RAW_SOURCE = 1 */
+int32_t sdt_set_coefficients(sdt_handle *h, const double *coef, int64_t count);
+/* After a forward run over all time_steps with checkpoints: trace_grad
+ * (time_steps * n_receivers * 3, the layout of sdt_get_receivers) is the
+ * gradient of the objective with respect to the traces.  Writes the gradient
+ * with respect to the update coefficients (ncoef * nx * nz, coefficient-major)
+ * and, when src_grad is non-null, with respect to the source series
+ * (SDT_NUM_SRC * time_steps, the layout of sdt_set_source).  Fields,
+ * receivers and norms read afterwards are those of the forward run. */
+int32_t sdt_adjoint(sdt_handle *h, const double *trace_grad, int64_t tcount, double *coef_grad,
+                    int64_t gcount, double *src_grad, int64_t scount);
+/* the same for every shot of a batch (all shots run to their ends with
+ * checkpoints, all of one time_steps); arrays per shot */
+int32_t sdt_batch_adjoint(sdt_batch *b, const double *const *trace_grad, double *const *coef_grad,
+                          double *const *src_grad);
 
 #ifdef __cplusplus
 }

@@ -437,6 +437,25 @@ static void el2_step_t(Device &d, bool timed)
     launch_tail(d);
 }
 
+int el2_adj_setup(Device &d)
+{
+    (void)d;
+    return SDT_ERR_ARG;
+}
+
+void el2_adj_step(Device &d, int t)
+{
+    (void)d;
+    (void)t;
+}
+
+void el2_adj_correlate(Device &d, int first, int n)
+{
+    (void)d;
+    (void)first;
+    (void)n;
+}
+
 void el2_step(Device &d, bool timed)
 {
     if (d.precision == SDT_FP64) el2_step_t<double>(d, timed);

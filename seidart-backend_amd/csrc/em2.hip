@@ -34,6 +34,12 @@ struct Em2Args {
     unsigned long long *vnorm;
     int64_t src_cell;
     int nsteps;         // time_steps of the run (batched shots stop there)
+    // segment recomputation for the adjoint: per step (0-based t), slot
+    // t - store_first of `store` holds E_x, E_z before the update and the
+    // CPML-corrected dHy/dz, dHy/dx of every cell ([slot][4][stride])
+    T *store;
+    int64_t stride;
+    int store_first;
 };
 
 // Hy (cpmlfdtd.f08:2576-2593); every operand is read before the first store
@@ -71,15 +77,19 @@ __device__ __forceinline__ void em2_h_cell(const Em2Args<T> &A, int i, int j)
 
 // Ex, Ez (cpmlfdtd.f08:2605-2620), zero edges, and the squared field
 // magnitude for the per-step norm, 0 at the source cell (the norm is the
-// square root of the maximum, which equals the maximum of the magnitudes)
-template <typename T, bool PML>
+// square root of the maximum, which equals the maximum of the magnitudes).
+// STORE: also write the values the adjoint gradient needs (see Em2Args).
+template <typename T, bool PML, bool STORE>
 __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
     const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
     T ex, ez;
+    T *st = nullptr;
+    if (STORE) st = A.store + (int64_t)(*A.d_it - 1 - A.store_first) * 4 * A.stride + c;
     if (i == 1 || i == nx || j == 1 || j == nz) {
         ex = ez = T(0);
+        if (STORE) st[0] = st[A.stride] = st[2 * A.stride] = st[3 * A.stride] = T(0);
     } else {
         const T h0 = A.Hy[c], hd = A.Hy[c - A.pitch], hl = A.Hy[c - 1];
         const T exo = A.Ex[c], ezo = A.Ez[c];
@@ -108,6 +118,12 @@ __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
         }
         ez = ezo + (-bEx * dHy_dz + bEz * dHy_dx - dEz * exo - eEz * ezo) * A.dt;
         ex = exo + (-aEx * dHy_dz + bEx * dHy_dx - dEx * exo - eEx * ezo) * A.dt;
+        if (STORE) {
+            st[0] = exo;
+            st[A.stride] = ezo;
+            st[2 * A.stride] = dHy_dz;
+            st[3 * A.stride] = dHy_dx;
+        }
     }
     A.Ex[c] = ex;
     A.Ez[c] = ez;
@@ -151,6 +167,9 @@ static Em2Args<T> em2_args(Device &d)
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
     A.nsteps = d.time_steps;
+    A.store = (T *)d.adj.seg_fwd;
+    A.stride = d.ncell_dev;
+    A.store_first = d.adj.store_first;
     return A;
 }
 
@@ -182,7 +201,7 @@ __device__ __forceinline__ void em2_h_tile(const Em2Args<T> &A, int tx, int band
     }
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __device__ __forceinline__ double em2_e_tile(const Em2Args<T> &A, int tx, int band)
 {
     const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
@@ -194,9 +213,9 @@ __device__ __forceinline__ double em2_e_tile(const Em2Args<T> &A, int tx, int ba
         if (j > A.nz) break;
         double v = 0.0;
         if (__any(valid && em2_pml(A, i, j))) {
-            if (valid) v = em2_e_cell<T, true>(A, i, j);
+            if (valid) v = em2_e_cell<T, true, STORE>(A, i, j);
         } else if (valid) {
-            v = em2_e_cell<T, false>(A, i, j);
+            v = em2_e_cell<T, false, STORE>(A, i, j);
         }
         nrm = (v > nrm || v != v) ? v : nrm;
     }
@@ -209,10 +228,10 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) em2_h_kernel(Em2Args<T> A)
     em2_h_tile<T, R>(A, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __global__ void __launch_bounds__(64 * kTile2dWaves) em2_e_kernel(Em2Args<T> A)
 {
-    const double nrm = sqrt(em2_e_tile<T, R>(A, blockIdx.x, blockIdx.y));
+    const double nrm = sqrt(em2_e_tile<T, R, STORE>(A, blockIdx.x, blockIdx.y));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -226,14 +245,14 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) em2_h_batch_kernel(const Em
     em2_h_tile<T, R>(A, t.y, t.z);
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __global__ void __launch_bounds__(64 * kTile2dWaves) em2_e_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
 {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;   // padding tile
     const Em2Args<T> A = args[t.x];
     if (*A.d_it > A.nsteps) return;
-    const double nrm = sqrt(em2_e_tile<T, R>(A, t.y, t.z));
+    const double nrm = sqrt(em2_e_tile<T, R, STORE>(A, t.y, t.z));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -259,7 +278,10 @@ static void em2_batch_launch_t(Batch &b, int which)
     const Em2Args<T> *a = (const Em2Args<T> *)b.args;
     const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
     hipLaunchKernelGGL((em2_h_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
-    hipLaunchKernelGGL((em2_e_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
+    if (b.store)
+        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, true>), grid, block, 0, b.stream, a, b.tiles[which]);
+    else
+        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, false>), grid, block, 0, b.stream, a, b.tiles[which]);
 }
 
 template <typename T>
@@ -287,9 +309,258 @@ static void em2_step_t(Device &d, bool timed)
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
     hipLaunchKernelGGL((em2_h_kernel<T, R>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL((em2_e_kernel<T, R>), grid, block, 0, d.stream, A);
+    if (d.adj.store)
+        hipLaunchKernelGGL((em2_e_kernel<T, R, true>), grid, block, 0, d.stream, A);
+    else
+        hipLaunchKernelGGL((em2_e_kernel<T, R, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
+}
+
+// ---- adjoint state (discrete adjoint of the step above) ----------------------
+// Step t maps (E^t, Hy^{t-1/2}, psi^t) to (E^{t+1}, Hy^{t+1/2}, psi^{t+1}):
+// H update (reads E^t), E update (reads E^t, Hy^{t+1/2}), point sources,
+// receivers (read E^{t+1}).  Its transpose runs per step, latest first:
+//   tail   the trace-gradient row t is added to lam(E^{t+1}) at the receivers;
+//   E pass lam(E^{t+1}) is completed with the H-pass transfer terms of step
+//          t+1 (lgz, lgx), recorded at the source cell and in the segment
+//          store, then pushed through the E update: lam(E^t) (own-cell part),
+//          lam(psi_E), and the transfer terms lfz, lfx to lam(Hy^{t+1/2});
+//   H pass lam(Hy^{t+1/2}) is completed with lfz, lfx and pushed through the
+//          H update: lam(Hy^{t-1/2}), lam(psi_H), transfer terms lgz, lgx to
+//          lam(E^t), which the next E pass gathers.
+// Every cell gathers from its neighbours' transfer terms, so no two threads
+// write one element.  Cells outside an update's domain hold zero transfer
+// terms.  Gradients: E^{t+1} = E^t + dt * (-aEx fz + bEx fx - dEx Ex - eEx Ez,
+// -bEx fz + bEz fx - dEz Ex - eEz Ez) gives, per cell,
+//   d/daEx = -dt lx fz, d/dbEx = dt (lx fx - lz fz), d/dbEz = dt lz fx,
+//   d/ddEx = -dt lx Ex, d/deEx = -dt lx Ez, d/ddEz = -dt lz Ex, d/deEz = -dt lz Ez
+// with (lx, lz) = lam(E^{t+1}) and (Ex, Ez, fz, fx) the stored forward values.
+template <typename T>
+struct Em2AdjArgs {
+    Em2Args<T> F;
+    T *lex, *lez, *lhy;        // lam(Ex), lam(Ez) (own-cell part), lam(Hy)
+    T *lm[4];                  // lam of m_dEx_dz, m_dEz_dx, m_dHy_dx, m_dHy_dz
+    T *lfz, *lfx, *lgz, *lgx;  // transfer terms
+    T *adj;                    // segment store: [slot][2][stride] lam(Ex^{t+1}), lam(Ez^{t+1})
+    double *src_lam;           // [time_steps][kAdjSrcSlots]
+};
+
+template <typename T, bool PML>
+__device__ __forceinline__ void em2_adj_e_cell(const Em2AdjArgs<T> &B, int i, int j, int t, int first)
+{
+    const Em2Args<T> &A = B.F;
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+    const T gzd = j >= 2 ? B.lgz[c - A.pitch] : T(0);
+    const T gxl = i >= 2 ? B.lgx[c - 1] : T(0);
+    const T lx = B.lex[c] + (gzd - B.lgz[c]) * A.rdz;
+    const T lz = B.lez[c] + (gxl - B.lgx[c]) * A.rdx;
+    if (c == A.src_cell) {
+        B.src_lam[kAdjSrcSlots * (int64_t)t] = (double)lx;
+        B.src_lam[kAdjSrcSlots * (int64_t)t + 1] = (double)lz;
+    }
+    T *as = B.adj + (int64_t)(t - first) * 2 * A.stride + c;
+    if (i == 1 || i == nx || j == 1 || j == nz) {   // E^{t+1} = 0 there: no dependence on step t
+        B.lex[c] = T(0);
+        B.lez[c] = T(0);
+        B.lfz[c] = T(0);
+        B.lfx[c] = T(0);
+        as[0] = as[A.stride] = T(0);
+        return;
+    }
+    const T aEx = ldnt(A.aEx + c), bEx = ldnt(A.bEx + c), bEz = ldnt(A.bEz + c);
+    const T dEx = ldnt(A.dEx + c), eEx = ldnt(A.eEx + c), dEz = ldnt(A.dEz + c), eEz = ldnt(A.eEz + c);
+    T lfz = A.dt * (-aEx * lx - bEx * lz);
+    T lfx = A.dt * (bEx * lx + bEz * lz);
+    if (PML) {
+        // fz' = fz ik + psi_z', psi_z' = b psi_z + a fz (and the half-cell set for x)
+        const T L4 = B.lm[3][c] + lfz, L3 = B.lm[2][c] + lfx;
+        lfz = lfz * ldnt(A.ik + c) + ldnt(A.a + c) * L4;
+        lfx = lfx * ldnt(A.ikh + c) + ldnt(A.ah + c) * L3;
+        B.lm[3][c] = ldnt(A.b + c) * L4;
+        B.lm[2][c] = ldnt(A.bh + c) * L3;
+    }
+    as[0] = lx;
+    as[A.stride] = lz;
+    B.lex[c] = lx - A.dt * (dEx * lx + dEz * lz);
+    B.lez[c] = lz - A.dt * (eEx * lx + eEz * lz);
+    B.lfz[c] = lfz;
+    B.lfx[c] = lfx;
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ void em2_adj_h_cell(const Em2AdjArgs<T> &B, int i, int j)
+{
+    const Em2Args<T> &A = B.F;
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+    if (i == nx || j == nz) {   // Hy = 0 there: no dependence
+        B.lhy[c] = T(0);
+        B.lgz[c] = T(0);
+        B.lgx[c] = T(0);
+        return;
+    }
+    const T lh = B.lhy[c] + (B.lfz[c] - B.lfz[c + A.pitch]) * A.rdz + (B.lfx[c] - B.lfx[c + 1]) * A.rdx;
+    T lgx = A.dbHy * lh, lgz = -A.dbHy * lh;
+    if (PML) {
+        const T L2 = B.lm[1][c] + lgx, L1 = B.lm[0][c] + lgz;
+        lgx = lgx * ldnt(A.ikh + c) + ldnt(A.ah + c) * L2;
+        lgz = lgz * ldnt(A.ik + c) + ldnt(A.a + c) * L1;
+        B.lm[1][c] = ldnt(A.bh + c) * L2;
+        B.lm[0][c] = ldnt(A.b + c) * L1;
+    }
+    B.lhy[c] = (i == 1 || j == 1) ? T(0) : lh;
+    B.lgz[c] = lgz;
+    B.lgx[c] = lgx;
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void em2_adj_e_tile(const Em2AdjArgs<T> &B, int tx, int band, int t, int first)
+{
+    const Em2Args<T> &A = B.F;
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        if (__any(valid && em2_pml(A, i, j))) {
+            if (valid) em2_adj_e_cell<T, true>(B, i, j, t, first);
+        } else if (valid) {
+            em2_adj_e_cell<T, false>(B, i, j, t, first);
+        }
+    }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void em2_adj_h_tile(const Em2AdjArgs<T> &B, int tx, int band)
+{
+    const Em2Args<T> &A = B.F;
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        if (__any(valid && em2_pml(A, i, j))) {
+            if (valid) em2_adj_h_cell<T, true>(B, i, j);
+        } else if (valid) {
+            em2_adj_h_cell<T, false>(B, i, j);
+        }
+    }
+}
+
+// gradient of segment [first, first + n): per cell, sums over the stored
+// steps in FP64, scaled by dt once
+template <typename T, int R>
+__device__ __forceinline__ void em2_correlate_tile(const Em2AdjArgs<T> &B, double *grad, int tx, int band, int n)
+{
+    const Em2Args<T> &A = B.F;
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    if (i < 1 || i > A.nx) return;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    const int64_t st = A.stride;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+        double g[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < n; ++s) {
+            const T *f = A.store + (int64_t)s * 4 * st + c;
+            const T *l = B.adj + (int64_t)s * 2 * st + c;
+            const double lx = ldnt(l), lz = ldnt(l + st);
+            const double ex = ldnt(f), ez = ldnt(f + st), fz = ldnt(f + 2 * st), fx = ldnt(f + 3 * st);
+            g[0] -= lx * fz;
+            g[1] += lx * fx - lz * fz;
+            g[2] += lz * fx;
+            g[3] -= lx * ex;
+            g[4] -= lx * ez;
+            g[5] -= lz * ex;
+            g[6] -= lz * ez;
+        }
+        const double dt = (double)A.dt;
+        for (int q = 0; q < 7; ++q) grad[q * st + c] += dt * g[q];
+    }
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_adj_e_kernel(Em2AdjArgs<T> B, int t, int first)
+{
+    em2_adj_e_tile<T, R>(B, blockIdx.x, blockIdx.y, t, first);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_adj_h_kernel(Em2AdjArgs<T> B)
+{
+    em2_adj_h_tile<T, R>(B, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_correlate_kernel(Em2AdjArgs<T> B, double *grad, int n)
+{
+    em2_correlate_tile<T, R>(B, grad, blockIdx.x, blockIdx.y, n);
+}
+
+template <typename T>
+static Em2AdjArgs<T> em2_adj_args(Device &d)
+{
+    Em2AdjArgs<T> B;
+    B.F = em2_args<T>(d);
+    T **l = (T **)d.adj.lam;
+    B.lex = l[0];
+    B.lez = l[1];
+    B.lhy = l[2];
+    for (int q = 0; q < 4; ++q) B.lm[q] = l[3 + q];
+    B.lfz = l[7];
+    B.lfx = l[8];
+    B.lgz = l[9];
+    B.lgx = l[10];
+    B.adj = (T *)d.adj.seg_adj;
+    B.src_lam = d.adj.src_lam;
+    return B;
+}
+
+int em2_adj_setup(Device &d)
+{
+    d.adj.nlam = 11;
+    for (int q = 0; q < d.adj.nlam; ++q)
+        if (!d.adj.lam[q] && alloc_grid_zeroed(d, &d.adj.lam[q]) != SDT_OK) return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+template <typename T>
+static void em2_adj_step_t(Device &d, int t)
+{
+    constexpr int R = kSingleRows2d;
+    const Em2AdjArgs<T> B = em2_adj_args<T>(d);
+    void *lam_field[3] = {B.lex, nullptr, B.lez};
+    launch_adj_tail(d, t, lam_field);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((em2_adj_e_kernel<T, R>), grid, block, 0, d.stream, B, t, d.adj.store_first);
+    hipLaunchKernelGGL((em2_adj_h_kernel<T, R>), grid, block, 0, d.stream, B);
+}
+
+void em2_adj_step(Device &d, int t)
+{
+    if (d.precision == SDT_FP64) em2_adj_step_t<double>(d, t);
+    else em2_adj_step_t<float>(d, t);
+}
+
+template <typename T>
+static void em2_adj_correlate_t(Device &d, int n)
+{
+    constexpr int R = kSingleRows2d;
+    const Em2AdjArgs<T> B = em2_adj_args<T>(d);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((em2_correlate_kernel<T, R>), grid, block, 0, d.stream, B, d.adj.grad, n);
+}
+
+void em2_adj_correlate(Device &d, int first, int n)
+{
+    (void)first;   // the store slots are relative to the segment start
+    if (d.precision == SDT_FP64) em2_adj_correlate_t<double>(d, n);
+    else em2_adj_correlate_t<float>(d, n);
 }
 
 void em2_step(Device &d, bool timed)
@@ -313,7 +584,9 @@ int em2_setup(Device &d, const double *const *in)
     std::vector<double> co[7];
     for (auto &v : co) v.resize((size_t)n);
     auto val = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
-    for (int64_t c = 0; c < n; ++c) {
+    for (int64_t c = 0; c < n && d.coef_in; ++c)   // coefficients given directly
+        for (int q = 0; q < 7; ++q) co[q][(size_t)c] = d.coef_in[(size_t)q * n + c];
+    for (int64_t c = 0; c < n && !d.coef_in; ++c) {
         const double e11 = val(SDT_E11, c) * kEps0, e13 = val(SDT_E13, c) * kEps0, e33 = val(SDT_E33, c) * kEps0;
         const double s11 = val(SDT_S11, c), s13 = val(SDT_S13, c), s33 = val(SDT_S33, c);
         double det = e11 * e33 - e13 * e13;
@@ -364,6 +637,15 @@ int em2_setup(Device &d, const double *const *in)
     t.rec_field[1] = -1;
     t.rec_field[2] = SDT_F_EZ;
 
+    // adjoint layout: checkpoint = fields + memory variables; 7 coefficients;
+    // per step 4 forward values (Ex, Ez, fz, fx) and 2 adjoint values
+    Adjoint &a = d.adj;
+    a.nstate = 0;
+    for (int f : fids) a.state[a.nstate++] = d.field[f];
+    for (int q = 0; q < d.nmem; ++q) a.state[a.nstate++] = d.mem[q];
+    a.ncoef = 7;
+    a.nfwd = 4;
+    a.nadj = 2;
     return upload_cpml_full(d, in);
 }
 

@@ -21,6 +21,7 @@ struct sdt_handle {
     std::vector<double> staged[SDT_NUM_ARRAYS];
     std::vector<double> series;       // [SDT_NUM_SRC][time_steps]
     std::vector<int32_t> rec_ijk;     // padded 1-based (i, j, k) per receiver
+    std::vector<double> coef;         // update coefficients (sdt_set_coefficients)
     bool prepared = false;
     bool failed = false;
     int steps_done = 0;
@@ -170,6 +171,8 @@ static void release(sdt_handle *h)
     drop(d.d_it);
     drop(d.rec_cell);
     drop(d.rec_out);
+    drop(d.adj.src_lam);      // the other adjoint arrays are grid allocations
+    drop(d.adj.trace_grad);
     for (auto ev : h->tev) (void)hipEventDestroy(ev);
     for (int q = 0; q < 2; ++q) {
         if (d.fork[q]) (void)hipEventDestroy(d.fork[q]);
@@ -251,6 +254,11 @@ static int prepare(sdt_handle *h)
     // the reference fills kappa/kappa_half with 1 before reading them
     // (cpmlfdtd.f08:240-247); arrays never set keep that default
     int rc = SDT_OK;
+    if (!h->coef.empty() && !d.raw_source)
+        return fail(h, SDT_ERR_STATE, "coefficient input needs SDT_OPT_RAW_SOURCE = 1");
+    if (d.adj.every > 0 && d.physics != SDT_ELASTIC2 && d.physics != SDT_EM2)
+        return fail(h, SDT_ERR_ARG, "checkpointing for the adjoint is available for 2D elastic and 2D EM");
+    d.coef_in = h->coef.empty() ? nullptr : h->coef.data();
     switch (d.physics) {
     case SDT_ELASTIC2: rc = el2_setup(d, in); break;
     case SDT_ELASTIC25: rc = el25_setup(d, in); break;
@@ -259,7 +267,21 @@ static int prepare(sdt_handle *h)
     case SDT_EM25: rc = em25_setup(d, in); break;
     case SDT_EM3: rc = em3_setup(d, in); break;
     }
+    d.coef_in = nullptr;
     if (rc) return fail(h, rc, "device setup failed (out of device memory?)");
+    if (d.raw_source)
+        for (int q = 0; q < d.tail.nterms; ++q) {
+            d.tail.terms[q].mul = 1.0;
+            d.tail.terms[q].div = 1.0;
+        }
+    if (d.adj.every > 0) {
+        Adjoint &a = d.adj;
+        a.nckpt = (d.time_steps + a.every - 1) / a.every + 1;
+        const size_t bytes = (size_t)a.nckpt * a.nstate * d.ncell_dev * d.esize;
+        void *p = nullptr;
+        if (alloc_grid(d, &p, bytes) != SDT_OK) return fail(h, SDT_ERR_HIP, "checkpoint allocation failed");
+        a.ckpt = (char *)p;
+    }
     // source series, norms, step counter, receivers
     const size_t ns = (size_t)SDT_NUM_SRC * d.time_steps;
     HIPCHK(h, hipMalloc((void **)&d.series, ns * sizeof(double)));
@@ -292,6 +314,7 @@ static int prepare(sdt_handle *h)
     t.rec_out = d.rec_out;
     HIPCHK(h, hipDeviceSynchronize());
     for (auto &s : h->staged) { std::vector<double>().swap(s); }
+    std::vector<double>().swap(h->coef);
     h->prepared = true;
     return SDT_OK;
 }
@@ -330,6 +353,18 @@ static int ensure_graph(sdt_handle *h)
     return SDT_OK;
 }
 
+// checkpoint slot k <-> the state arrays (fields + memory variables)
+static void checkpoint_copy(Device &d, int k, bool save)
+{
+    const Adjoint &a = d.adj;
+    const size_t bytes = (size_t)d.ncell_dev * d.esize;
+    for (int q = 0; q < a.nstate; ++q) {
+        char *slot = a.ckpt + ((size_t)k * a.nstate + q) * bytes;
+        if (save) (void)hipMemcpyAsync(slot, a.state[q], bytes, hipMemcpyDeviceToDevice, d.stream);
+        else (void)hipMemcpyAsync(a.state[q], slot, bytes, hipMemcpyDeviceToDevice, d.stream);
+    }
+}
+
 // Check the velocity norms of steps [first, first+n) (1-based first).
 static int check_norms(sdt_handle *h, int first, int n, int32_t *failed_step)
 {
@@ -366,8 +401,14 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
     const bool use_graph = d.graphs && !d.timing;
     const int chunk = 256;   // steps between blow-up checks
     int left = nsteps;
+    const int every = d.adj.every;
     while (left > 0) {
-        const int n = left < chunk ? left : chunk;
+        int n = left < chunk ? left : chunk;
+        if (every > 0) {   // segments end at multiples of `every`; save each segment's first state
+            const int into = h->steps_done % every;
+            if (into == 0) checkpoint_copy(d, h->steps_done / every, true);
+            n = n < every - into ? n : every - into;
+        }
         const int first = h->steps_done + 1;
         if (d.timing) {
             if ((int)h->tev.size() < 3 * n) {
@@ -404,6 +445,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
         if (rc) return rc;
         h->steps_done += n;
         left -= n;
+        if (every > 0 && h->steps_done == d.time_steps) checkpoint_copy(d, d.adj.nckpt - 1, true);
     }
     return SDT_OK;
 }
@@ -546,6 +588,15 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
         if (h->prepared) return fail(h, SDT_ERR_STATE, "allocation layout is fixed once the first step has run");
         h->d.stagger = value != 0;
         return SDT_OK;
+    case SDT_OPT_CHECKPOINT:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "checkpointing is fixed once the first step has run");
+        if (value < 0 || value > h->d.time_steps) return fail(h, SDT_ERR_ARG, "checkpoint interval out of range");
+        h->d.adj.every = (int)value;
+        return SDT_OK;
+    case SDT_OPT_RAW_SOURCE:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "source scaling is fixed once the first step has run");
+        h->d.raw_source = value != 0;
+        return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown option");
 }
@@ -558,8 +609,142 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
     case SDT_INFO_MATERIALS: *value = h->d.mat_table ? h->d.nmaterials : 0; return SDT_OK;
     case SDT_INFO_SHELL_TILES: *value = h->d.ntiles; return SDT_OK;
     case SDT_INFO_DEVICE_BYTES: *value = (int64_t)h->d.grid_bytes; return SDT_OK;
+    case SDT_INFO_COEFFICIENTS: *value = h->d.adj.ncoef; return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown info key");
+}
+
+// ---- adjoint-state gradients ---------------------------------------------------
+static int ncoef_of(int physics)
+{
+    return physics == SDT_EM2 ? 7 : physics == SDT_ELASTIC2 ? 10 : 0;
+}
+
+int32_t sdt_set_coefficients(sdt_handle *h, const double *coef, int64_t count)
+{
+    if (!h || !coef) return fail(h, SDT_ERR_ARG, "null argument");
+    if (h->prepared) return fail(h, SDT_ERR_STATE, "inputs are fixed once the first step has run");
+    const int nc = ncoef_of(h->d.physics);
+    if (!nc) return fail(h, SDT_ERR_ARG, "coefficient input is available for 2D elastic and 2D EM");
+    if (count != (int64_t)nc * h->d.ncell) return fail(h, SDT_ERR_ARG, "count must equal ncoef * nx * nz");
+    h->coef.assign(coef, coef + count);
+    return SDT_OK;
+}
+
+// adjoint arrays: segment stores, gradient, source and trace buffers, the
+// solver's adjoint fields
+static int adjoint_alloc(sdt_handle *h)
+{
+    Device &d = h->d;
+    Adjoint &a = d.adj;
+    if (a.grad) return SDT_OK;
+    const size_t cell = (size_t)d.ncell_dev * d.esize;
+    void *p = nullptr;
+    if (alloc_grid(d, &p, (size_t)a.every * a.nfwd * cell) != SDT_OK) return SDT_ERR_HIP;
+    a.seg_fwd = p;
+    if (alloc_grid(d, &p, (size_t)a.every * a.nadj * cell) != SDT_OK) return SDT_ERR_HIP;
+    a.seg_adj = p;
+    if (alloc_grid(d, &p, (size_t)a.ncoef * d.ncell_dev * sizeof(double)) != SDT_OK) return SDT_ERR_HIP;
+    a.grad = (double *)p;
+    if (hipMalloc((void **)&a.src_lam, (size_t)d.time_steps * kAdjSrcSlots * sizeof(double)) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (hipMalloc((void **)&a.trace_grad, ((size_t)d.time_steps * (d.nrec > 0 ? d.nrec : 1) * 3) * sizeof(double)) !=
+        hipSuccess)
+        return SDT_ERR_HIP;
+    return d.physics == SDT_EM2 ? em2_adj_setup(d) : el2_adj_setup(d);
+}
+
+// zero the adjoint state and upload the trace gradient
+static int adjoint_reset(sdt_handle *h, const double *trace_grad)
+{
+    Device &d = h->d;
+    Adjoint &a = d.adj;
+    const size_t cell = (size_t)d.ncell_dev * d.esize;
+    for (int q = 0; q < a.nlam; ++q) HIPCHK(h, hipMemsetAsync(a.lam[q], 0, cell, d.stream));
+    HIPCHK(h, hipMemsetAsync(a.grad, 0, (size_t)a.ncoef * d.ncell_dev * sizeof(double), d.stream));
+    HIPCHK(h, hipMemsetAsync(a.src_lam, 0, (size_t)d.time_steps * kAdjSrcSlots * sizeof(double), d.stream));
+    if (d.nrec > 0)
+        HIPCHK(h, hipMemcpyAsync(a.trace_grad, trace_grad, (size_t)d.time_steps * d.nrec * 3 * sizeof(double),
+                                 hipMemcpyHostToDevice, d.stream));
+    HIPCHK(h, hipStreamSynchronize(d.stream));
+    return SDT_OK;
+}
+
+static void adjoint_step(Device &d, int t)
+{
+    if (d.physics == SDT_EM2) em2_adj_step(d, t);
+    else el2_adj_step(d, t);
+}
+
+static void adjoint_correlate(Device &d, int first, int n)
+{
+    if (d.physics == SDT_EM2) em2_adj_correlate(d, first, n);
+    else el2_adj_correlate(d, first, n);
+}
+
+// gradient and source gradient back to the host
+static int adjoint_download(sdt_handle *h, double *coef_grad, double *src_grad)
+{
+    Device &d = h->d;
+    Adjoint &a = d.adj;
+    HIPCHK(h, hipStreamSynchronize(d.stream));
+    for (int q = 0; q < a.ncoef; ++q)
+        if (download_grid_f64(d, a.grad + (size_t)q * d.ncell_dev, coef_grad + (size_t)q * d.ncell) != SDT_OK)
+            return fail(h, SDT_ERR_HIP, "gradient read-back failed");
+    if (src_grad) {
+        std::vector<double> lam((size_t)d.time_steps * kAdjSrcSlots);
+        HIPCHK(h, hipMemcpy(lam.data(), a.src_lam, lam.size() * sizeof(double), hipMemcpyDeviceToHost));
+        memset(src_grad, 0, (size_t)SDT_NUM_SRC * d.time_steps * sizeof(double));
+        // term k of the tail records its field's adjoint in slot k; the series
+        // enters as (v * mul) / div
+        for (int k = 0; k < d.tail.nterms && k < kAdjSrcSlots; ++k) {
+            const SrcTerm &s = d.tail.terms[k];
+            if (s.cell < 0 || s.on_face) continue;
+            const double scale = s.mul / s.div;
+            for (int t = 0; t < d.time_steps; ++t)
+                src_grad[(size_t)s.component * d.time_steps + t] += lam[(size_t)t * kAdjSrcSlots + k] * scale;
+        }
+    }
+    return SDT_OK;
+}
+
+int32_t sdt_adjoint(sdt_handle *h, const double *trace_grad, int64_t tcount, double *coef_grad, int64_t gcount,
+                    double *src_grad, int64_t scount)
+{
+    if (!h || !coef_grad || (!trace_grad && tcount)) return fail(h, SDT_ERR_ARG, "null argument");
+    Device &d = h->d;
+    Adjoint &a = d.adj;
+    if (d.physics != SDT_EM2 && d.physics != SDT_ELASTIC2)
+        return fail(h, SDT_ERR_ARG, "the adjoint is available for 2D elastic and 2D EM");
+    if (!h->prepared || a.every <= 0 || h->steps_done != d.time_steps)
+        return fail(h, SDT_ERR_STATE, "run all time_steps with SDT_OPT_CHECKPOINT set first");
+    if (h->failed) return fail(h, SDT_ERR_STATE, "the run has blown up");
+    if (tcount != (int64_t)d.time_steps * d.nrec * 3) return fail(h, SDT_ERR_ARG, "tcount must equal time_steps * n_receivers * 3");
+    if (gcount != (int64_t)a.ncoef * d.ncell) return fail(h, SDT_ERR_ARG, "gcount must equal ncoef * nx * nz");
+    if (src_grad && scount != (int64_t)SDT_NUM_SRC * d.time_steps)
+        return fail(h, SDT_ERR_ARG, "scount must equal SDT_NUM_SRC * time_steps");
+    HIPCHK(h, hipSetDevice(h->p.device));
+    if (adjoint_alloc(h) != SDT_OK) return fail(h, SDT_ERR_HIP, "adjoint allocation failed (out of device memory?)");
+    int rc = adjoint_reset(h, trace_grad);
+    if (rc) return rc;
+    const int nseg = a.nckpt - 1;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int first = seg * a.every;
+        const int n = d.time_steps - first < a.every ? d.time_steps - first : a.every;
+        checkpoint_copy(d, seg, false);
+        set_step_counter(d.stream, d.d_it, first + 1);
+        a.store = true;
+        a.store_first = first;
+        for (int s = 0; s < n; ++s) enqueue_step(h, false);
+        a.store = false;
+        for (int t = first + n - 1; t >= first; --t) adjoint_step(d, t);
+        adjoint_correlate(d, first, n);
+        HIPCHK(h, hipGetLastError());
+    }
+    checkpoint_copy(d, a.nckpt - 1, false);   // back to the forward run's final state
+    set_step_counter(d.stream, d.d_it, d.time_steps + 1);
+    HIPCHK(h, hipGetLastError());
+    return adjoint_download(h, coef_grad, src_grad);
 }
 
 // ---- batched shots ----------------------------------------------------------
@@ -717,6 +902,15 @@ int32_t sdt_batch_destroy(sdt_batch *b)
     batch_free(b->b);
     delete b;
     return SDT_OK;
+}
+
+int32_t sdt_batch_adjoint(sdt_batch *b, const double *const *trace_grad, double *const *coef_grad,
+                          double *const *src_grad)
+{
+    (void)trace_grad;
+    (void)coef_grad;
+    (void)src_grad;
+    return bfail(b, SDT_ERR_ARG, "batched adjoint not available yet");
 }
 
 int32_t sdt_synchronize(sdt_handle *h)

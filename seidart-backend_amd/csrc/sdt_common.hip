@@ -172,6 +172,60 @@ void batch_step(Batch &b, int which)
     launch_tail_batch(b);
 }
 
+// ---- adjoint helpers ---------------------------------------------------------
+// Trace-gradient row t added to the adjoint of the recorded fields at the
+// receiver cells (atomics: receivers may share a cell).
+struct AdjTailArgs {
+    void *lam[3];             // adjoint array of each recorded component (or null)
+    const int64_t *rec_cell;
+    const double *tg;         // [time_steps][nrec][3]
+    int nrec;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) adj_tail_kernel(AdjTailArgs a, int t)
+{
+    for (int q = threadIdx.x; q < a.nrec; q += blockDim.x)
+        for (int m = 0; m < 3; ++m)
+            if (a.lam[m])
+                atomicAdd(reinterpret_cast<T *>(a.lam[m]) + a.rec_cell[q],
+                          (T)a.tg[((int64_t)t * a.nrec + q) * 3 + m]);
+}
+
+void launch_adj_tail(Device &d, int t, void *const lam_field[3])
+{
+    if (d.nrec == 0) return;
+    AdjTailArgs a;
+    for (int m = 0; m < 3; ++m) a.lam[m] = lam_field[m];
+    a.rec_cell = d.rec_cell;
+    a.tg = d.adj.trace_grad;
+    a.nrec = d.nrec;
+    if (d.precision == SDT_FP64)
+        hipLaunchKernelGGL(adj_tail_kernel<double>, dim3(1), dim3(256), 0, d.stream, a, t);
+    else
+        hipLaunchKernelGGL(adj_tail_kernel<float>, dim3(1), dim3(256), 0, d.stream, a, t);
+}
+
+__global__ void set_counter_kernel(int *p, int v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+void set_step_counter(hipStream_t s, int *d_it, int v)
+{
+    hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, d_it, v);
+}
+
+// an FP64 array in the device grid layout of d -> host (nx, ny, nz) order
+int download_grid_f64(Device &d, const double *dev, double *host)
+{
+    const size_t rows = (size_t)d.ny * d.nz;
+    if (hipMemcpy2D(host, (size_t)d.nx * 8, dev + d.xoff, (size_t)d.pitch * 8, (size_t)d.nx * 8, rows,
+                    hipMemcpyDeviceToHost) != hipSuccess)
+        return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
 void launch_tail(Device &d)
 {
     FieldPtrs fp;

@@ -51,6 +51,34 @@ struct TailArgs {
     int cplx;                        // 1: complex fields (interleaved re, im)
 };
 
+// Adjoint-state gradients (2D EM, 2D elastic).  The forward run saves its
+// state every `every` steps; the adjoint run walks the segments backwards:
+// it restores a segment's first state, recomputes the segment while the
+// forward kernels store the values the gradient needs (seg_fwd), steps the
+// adjoint fields back through the segment (storing the adjoint values the
+// gradient needs in seg_adj), then correlates the two per cell into the FP64
+// gradient accumulators.
+constexpr int kMaxAdjArrays = 24;
+constexpr int kAdjSrcSlots = 8;        // adjoint values recorded per step at the source cells
+struct Adjoint {
+    int every = 0;                     // SDT_OPT_CHECKPOINT (0 = no checkpoints)
+    int nckpt = 0;                     // slot k = state after min(k*every, time_steps) steps
+    int nstate = 0;                    // arrays per checkpoint
+    void *state[kMaxAdjArrays] = {};   // the device arrays a checkpoint holds
+    char *ckpt = nullptr;              // [nckpt][nstate][ncell_dev] elements of T
+    int nfwd = 0, nadj = 0;            // values per cell stored per step
+    void *seg_fwd = nullptr;           // [every][nfwd][ncell_dev]
+    void *seg_adj = nullptr;           // [every][nadj][ncell_dev]
+    void *lam[kMaxAdjArrays] = {};     // adjoint fields, memory variables, transfer arrays
+    int nlam = 0;
+    int ncoef = 0;                     // update coefficients per cell
+    double *grad = nullptr;            // [ncoef][ncell_dev]
+    double *src_lam = nullptr;         // [time_steps][kAdjSrcSlots] adjoint values at the source cells
+    double *trace_grad = nullptr;      // [time_steps][nrec][3]
+    bool store = false;                // forward kernels write seg_fwd
+    int store_first = 0;               // 0-based first step of the recomputed segment
+};
+
 struct Device {
     int physics = 0, precision = 0;
     int nx = 0, ny = 0, nz = 0;
@@ -125,6 +153,10 @@ struct Device {
     bool timing = false;
     double ms_stress = 0, ms_vel = 0;
     int64_t timed_launches = 0;
+
+    const double *coef_in = nullptr;   // update coefficients given directly (during setup only)
+    bool raw_source = false;           // SDT_OPT_RAW_SOURCE: series injected as given
+    Adjoint adj;
 };
 
 // Per-solver hooks, defined in the solver's .hip file.
@@ -146,6 +178,16 @@ int em2_setup(Device &d, const double *const *host_in);
 void em2_step(Device &d, bool timed);
 int em3_setup(Device &d, const double *const *host_in);
 void em3_step(Device &d, bool timed);
+
+// adjoint hooks of the 2D solvers: allocate the adjoint arrays, reverse step
+// t (0-based; its trace row is injected first), correlate the stored
+// segment [first, first + n) into the gradient
+int em2_adj_setup(Device &d);
+void em2_adj_step(Device &d, int t);
+void em2_adj_correlate(Device &d, int first, int n);
+int el2_adj_setup(Device &d);
+void el2_adj_step(Device &d, int t);
+void el2_adj_correlate(Device &d, int first, int n);
 
 // Many 2D shots advanced together: one launch per pass covers the tiles of
 // every shot (tile list entries name the shot), one tail launch covers every
@@ -175,7 +217,8 @@ struct Batch {
     int ntiles[kNumBatchShapes] = {};
     void *fps = nullptr;           // per-shot field pointers (device)
     TailArgs *tails = nullptr;     // per-shot tail arguments (device)
-    int choice = -1;               // autotuned tile height index, -1 = not yet
+    int choice = -1;               // autotuned tile shape index, -1 = not yet
+    bool store = false;            // forward E / velocity kernels write the segment store
     hipStream_t stream = nullptr;
 };
 int batch_setup(Batch &b, Device *const *shots, int n);
@@ -190,6 +233,9 @@ void launch_tail_batch(Batch &b);
 
 // common pieces (sdt_common.hip)
 void launch_tail(Device &d);
+void launch_adj_tail(Device &d, int t, void *const lam_field[3]);
+void set_step_counter(hipStream_t s, int *d_it, int v);
+int download_grid_f64(Device &d, const double *dev, double *host);
 // CPML classification.  A cell is an identity cell when a == 0, kappa == 1
 // and b is finite in both the full and the half coefficient set (its memory
 // variables then stay exactly zero and its derivatives pass unchanged).

@@ -65,11 +65,14 @@ EXPORTS = [
     "sdt_get_field", "sdt_set_field", "sdt_get_receivers", "sdt_get_norms", "sdt_set_timing",
     "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option", "sdt_get_info",
     "sdt_batch_create", "sdt_batch_run", "sdt_batch_info", "sdt_batch_destroy",
+    "sdt_set_coefficients", "sdt_adjoint", "sdt_batch_adjoint",
 ]
 
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
     1, 2, 3, 4, 5, 6, 7)
-INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES = 1, 2, 3
+OPT_CHECKPOINT, OPT_RAW_SOURCE = 8, 9
+INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES, INFO_COEFFICIENTS = 1, 2, 3, 4
+NUM_SRC = 9
 BATCH_INFO_TILE_ROWS, BATCH_INFO_TILES, BATCH_INFO_XCD_ORDER = 1, 2, 3
 
 
@@ -119,6 +122,9 @@ def _load():
         "sdt_batch_run": ([P, I32, C.POINTER(I32), C.POINTER(I32)], I32),
         "sdt_batch_info": ([P, I32, C.POINTER(I64)], I32),
         "sdt_batch_destroy": ([P], I32),
+        "sdt_set_coefficients": ([P, D, I64], I32),
+        "sdt_adjoint": ([P, D, I64, D, I64, D, I64], I32),
+        "sdt_batch_adjoint": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

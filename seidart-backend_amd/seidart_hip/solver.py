@@ -122,6 +122,32 @@ class Solver:
         s = np.ascontiguousarray(np.asarray(series, dtype=np.float64).ravel())
         L.check(L.lib.sdt_set_source(self._h, L.SOURCE_IDS[component], _dptr(s), s.size), self._h)
 
+    def set_coefficients(self, coef):
+        """2D EM / 2D elastic: the per-cell update coefficients, shape
+        (ncoef,) + grid (EM: aEx, bEx, bEz, dEx, eEx, dEz, eEz; elastic: c11,
+        c13, c15, c33, c35, c55, rho, 1/(1+gamma_x dt), 1/(1+gamma_z dt),
+        1/(1+gamma_xz dt)), in place of the model arrays; needs
+        set_option('raw_source', 1)."""
+        c = np.asarray(coef, dtype=np.float64)
+        if c.shape[1:] != self.shape:
+            raise ValueError(f"coefficient shape {c.shape} does not match grid {self.shape}")
+        flat = np.ascontiguousarray(np.stack([np.asfortranarray(x).ravel(order="F") for x in c]))
+        L.check(L.lib.sdt_set_coefficients(self._h, _dptr(flat), flat.size), self._h)
+        self.ncoef = c.shape[0]
+
+    def adjoint(self, trace_grad):
+        """After run() over all time_steps with the 'checkpoint' option:
+        trace_grad (time_steps, n_receivers, 3) = dJ/d(receivers()).  Returns
+        (dJ/d coefficients (ncoef,) + grid, dJ/d source series (9, time_steps))."""
+        g = np.ascontiguousarray(np.asarray(trace_grad, dtype=np.float64))
+        if g.shape != (self.time_steps, self.nrec, 3):
+            raise ValueError(f"trace_grad shape {g.shape} != {(self.time_steps, self.nrec, 3)}")
+        nc = self.info("coefficients")
+        cg = np.empty((nc, self.ncell), dtype=np.float64)
+        sg = np.empty((L.NUM_SRC, self.time_steps), dtype=np.float64)
+        L.check(L.lib.sdt_adjoint(self._h, _dptr(g), g.size, _dptr(cg), cg.size, _dptr(sg), sg.size), self._h)
+        return np.stack([row.reshape(self.shape, order="F") for row in cg]), sg
+
     def set_receivers(self, idx):
         """idx: (n, 3) 1-based unpadded (x, y, z) indices; y ignored in 2D."""
         r = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 3))
@@ -184,17 +210,20 @@ class Solver:
         elastic model has <= 256 distinct materials, 0 = per-cell arrays) or
         'concurrent' (1 = 3D elastic shell kernels on a second stream beside
         the interior kernels, 0 = sequential) or 'xcd_remap' (3D elastic block
-        order, bit 0 interior, bit 1 shell: XCD-contiguous tile ranges)."""
+        order, bit 0 interior, bit 1 shell: XCD-contiguous tile ranges) or
+        'checkpoint' (2D: steps between forward states saved for adjoint())
+        or 'raw_source' (1 = source series injected as given)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
              "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
-             "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP}[key]
+             "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP,
+             "checkpoint": L.OPT_CHECKPOINT, "raw_source": L.OPT_RAW_SOURCE}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     def info(self, key):
         """'materials' (distinct materials in the table, 0 = per-cell arrays),
         'shell_tiles' or 'device_bytes'; valid after the first run()."""
         k = {"materials": L.INFO_MATERIALS, "shell_tiles": L.INFO_SHELL_TILES,
-             "device_bytes": L.INFO_DEVICE_BYTES}[key]
+             "device_bytes": L.INFO_DEVICE_BYTES, "coefficients": L.INFO_COEFFICIENTS}[key]
         v = C.c_int64(0)
         L.check(L.lib.sdt_get_info(self._h, k, C.byref(v)), self._h)
         return v.value

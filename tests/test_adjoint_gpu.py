@@ -1,0 +1,124 @@
+"""Adjoint-state gradients (sdt_adjoint through seidart_hip.fwi) against the
+autodiff oracle (torch autograd of the float64 CPU restatement of the time
+stepping), against central finite differences, and FP32 against FP64."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _shot(cfg):
+    return dict(shape=cfg["shape"], spacing=cfg["spacing"], cpml=cfg["cpml"], dt=cfg["dt"], src=cfg["src"],
+                receivers=cfg["receivers"],
+                cpml_arrays={k: v for k, v in cfg["arrays"].items() if k.endswith("_cpml")})
+
+
+def _em2_case(steps=150, seed=4):
+    from oracle.adjoint_torch import em2_cfg_inputs
+    from seidart_hip import models
+    cfg = models.em2(nx=50, nz=40, steps=steps, src=(25, 12), nrec=8, seed=seed)
+    coef, src, A, dt, dx, dz, sij, rij = em2_cfg_inputs(cfg)
+    return cfg, coef, src, (A, dt, dx, dz, sij, rij)
+
+
+def _gpu_grad(cfg, coef, src, w, precision, checkpoint):
+    from seidart_hip import fwi
+    c = torch.tensor(coef, requires_grad=True)
+    s = torch.tensor(src, requires_grad=True)
+    tr = fwi.traces("em2", c, s, _shot(cfg), precision=precision, checkpoint=checkpoint)
+    (tr * torch.tensor(w)).sum().backward()
+    return tr.detach().numpy(), c.grad.numpy(), s.grad.numpy()
+
+
+@pytest.mark.parametrize("checkpoint", [40, 150])
+def test_em2_adjoint_matches_autograd(checkpoint):
+    from oracle.adjoint_torch import em2_traces
+    cfg, coef, src, o = _em2_case()
+    print(f"[adjoint] em2 shape={cfg['shape']} steps={cfg['time_steps']} checkpoint={checkpoint}", flush=True)
+    w = np.random.default_rng(0).standard_normal((cfg["time_steps"], len(cfg["receivers"]), 2))
+    tr, gc, gs = _gpu_grad(cfg, coef, src, w, "fp64", checkpoint)
+    c2 = torch.tensor(coef, requires_grad=True)
+    s2 = torch.tensor(src, requires_grad=True)
+    tr2 = em2_traces(c2, s2[0], s2[1], *o)
+    (tr2 * torch.tensor(w)).sum().backward()
+    assert rel_l2(tr, tr2.detach().numpy()) <= 1e-10
+    errs = [rel_l2(gc[q], c2.grad.numpy()[q]) for q in range(7)]
+    print("coefficient gradients rel L2:", [f"{e:.2e}" for e in errs], "source:",
+          f"{rel_l2(gs, s2.grad.numpy()):.2e}", flush=True)
+    assert max(errs) <= 1e-9
+    assert rel_l2(gs, s2.grad.numpy()) <= 1e-9
+
+
+def test_em2_adjoint_checkpoint_interval_invariant():
+    cfg, coef, src, _ = _em2_case(steps=120)
+    w = np.random.default_rng(1).standard_normal((120, len(cfg["receivers"]), 2))
+    ref = _gpu_grad(cfg, coef, src, w, "fp64", 120)
+    for every in (1, 7, 32):
+        got = _gpu_grad(cfg, coef, src, w, "fp64", every)
+        assert rel_l2(got[1], ref[1]) <= 1e-12, every
+        assert rel_l2(got[2], ref[2]) <= 1e-12, every
+
+
+def test_em2_adjoint_fp32_close_to_fp64():
+    cfg, coef, src, _ = _em2_case(steps=200)
+    w = np.random.default_rng(2).standard_normal((200, len(cfg["receivers"]), 2))
+    _, g64, s64 = _gpu_grad(cfg, coef, src, w, "fp64", 50)
+    _, g32, s32 = _gpu_grad(cfg, coef, src, w, "fp32", 50)
+    errs = [rel_l2(g32[q], g64[q]) for q in range(7)]
+    print("fp32 vs fp64 coefficient gradients rel L2:", [f"{e:.2e}" for e in errs], flush=True)
+    assert rel_l2(g32, g64) <= 1e-4
+    assert max(errs) <= 1e-3
+    assert rel_l2(s32, s64) <= 1e-4
+
+
+def test_em2_gradient_matches_finite_differences():
+    """dJ/d eps11 and dJ/d sigma11 at interior cells, J the L2 misfit of
+    traces against those of a perturbed model; central differences with a
+    1e-4 relative step in FP64."""
+    from seidart_hip import fwi, models
+    cfg = models.em2(nx=40, nz=30, steps=160, src=(20, 8), nrec=6, seed=9)
+    A = cfg["arrays"]
+    shot = _shot(cfg)
+    names = ("e11", "e13", "e33", "s11", "s13", "s33")
+    eps0 = [torch.tensor(A[n]) for n in names[:3]]
+    sig0 = [torch.tensor(A[n]) for n in names[3:]]
+    cp = cfg["cpml"]
+    si, sj = cfg["src"][0] + cp - 1, cfg["src"][1] + cp - 1
+    scale = fwi.em2_source_scale(A["e11"][si, sj], cfg["dt"])
+    src = torch.tensor(np.stack([cfg["sources"]["x"], cfg["sources"]["z"]]) * scale)
+    # observed data from a model with a bump in eps11 and sigma11
+    eps_t = [e.clone() for e in eps0]
+    sig_t = [s.clone() for s in sig0]
+    eps_t[0][cp + 15:cp + 25, cp + 14:cp + 20] *= 1.1
+    sig_t[0][cp + 15:cp + 25, cp + 14:cp + 20] *= 1.5
+    with torch.no_grad():
+        obs = fwi.traces("em2", fwi.em2_coefficients(eps_t, sig_t), src, shot, precision="fp64", checkpoint=40)
+
+    def J(eps, sig):
+        return fwi.l2_misfit(fwi.traces("em2", fwi.em2_coefficients(eps, sig), src, shot, precision="fp64",
+                                        checkpoint=40), obs)
+
+    e = [x.clone().requires_grad_(True) for x in eps0]
+    s = [x.clone().requires_grad_(True) for x in sig0]
+    J(e, s).backward()
+    rng = np.random.default_rng(3)
+    cells = [(int(rng.integers(cp + 2, cp + 38)), int(rng.integers(cp + 2, cp + 28))) for _ in range(8)]
+    fd, ad = [], []
+    for which, base, grad in (("eps11", eps0, e[0].grad), ("sig11", sig0, s[0].grad)):
+        for (i, j) in cells:
+            h = 1e-4 * float(base[0][i, j])
+            vals = []
+            for sign in (1.0, -1.0):
+                p = [x.clone() for x in base]
+                p[0][i, j] += sign * h
+                with torch.no_grad():
+                    vals.append(float(J(p, sig0) if which == "eps11" else J(eps0, p)))
+            fd.append((vals[0] - vals[1]) / (2 * h))
+            ad.append(float(grad[i, j]))
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    print("finite differences vs adjoint rel L2:", f"{err:.2e}", flush=True)
+    assert err <= 1e-3
