@@ -387,7 +387,7 @@ static int el2_batch_args_t(Batch &b, Device *const *shots, int n)
 {
     std::vector<El2Args<T>> a((size_t)n);
     for (int s = 0; s < n; ++s) a[(size_t)s] = el2_args<T>(*shots[s]);
-    if (hipMalloc(&b.args, a.size() * sizeof(El2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
+    if (!b.args && hipMalloc(&b.args, a.size() * sizeof(El2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(b.args, a.data(), a.size() * sizeof(El2Args<T>), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
     return SDT_OK;
@@ -453,6 +453,29 @@ void el2_adj_correlate(Device &d, int first, int n)
 {
     (void)d;
     (void)first;
+    (void)n;
+}
+
+int el2_batch_adj_args(Batch &b, Device *const *shots, int n)
+{
+    (void)b;
+    (void)shots;
+    (void)n;
+    return SDT_ERR_ARG;
+}
+
+void el2_batch_adj_step(Batch &b, int which, int t, int first)
+{
+    (void)b;
+    (void)which;
+    (void)t;
+    (void)first;
+}
+
+void el2_batch_adj_correlate(Batch &b, int which, int n)
+{
+    (void)b;
+    (void)which;
     (void)n;
 }
 

@@ -261,7 +261,7 @@ static int em2_batch_args_t(Batch &b, Device *const *shots, int n)
 {
     std::vector<Em2Args<T>> a((size_t)n);
     for (int s = 0; s < n; ++s) a[(size_t)s] = em2_args<T>(*shots[s]);
-    if (hipMalloc(&b.args, a.size() * sizeof(Em2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
+    if (!b.args && hipMalloc(&b.args, a.size() * sizeof(Em2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(b.args, a.data(), a.size() * sizeof(Em2Args<T>), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
     return SDT_OK;
@@ -344,6 +344,7 @@ struct Em2AdjArgs {
     T *lfz, *lfx, *lgz, *lgx;  // transfer terms
     T *adj;                    // segment store: [slot][2][stride] lam(Ex^{t+1}), lam(Ez^{t+1})
     double *src_lam;           // [time_steps][kAdjSrcSlots]
+    double *grad;              // [7][stride] gradient accumulators
 };
 
 template <typename T, bool PML>
@@ -454,8 +455,9 @@ __device__ __forceinline__ void em2_adj_h_tile(const Em2AdjArgs<T> &B, int tx, i
 // gradient of segment [first, first + n): per cell, sums over the stored
 // steps in FP64, scaled by dt once
 template <typename T, int R>
-__device__ __forceinline__ void em2_correlate_tile(const Em2AdjArgs<T> &B, double *grad, int tx, int band, int n)
+__device__ __forceinline__ void em2_correlate_tile(const Em2AdjArgs<T> &B, int tx, int band, int n)
 {
+    double *grad = B.grad;
     const Em2Args<T> &A = B.F;
     const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
     if (i < 1 || i > A.nx) return;
@@ -497,9 +499,37 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) em2_adj_h_kernel(Em2AdjArgs
 }
 
 template <typename T, int R>
-__global__ void __launch_bounds__(64 * kTile2dWaves) em2_correlate_kernel(Em2AdjArgs<T> B, double *grad, int n)
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_correlate_kernel(Em2AdjArgs<T> B, int n)
 {
-    em2_correlate_tile<T, R>(B, grad, blockIdx.x, blockIdx.y, n);
+    em2_correlate_tile<T, R>(B, blockIdx.x, blockIdx.y, n);
+}
+
+// batched forms: tile lists name the shot (all shots reverse the same step)
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_adj_e_batch_kernel(const Em2AdjArgs<T> *args,
+                                                                            const int4 *tiles, int t, int first)
+{
+    const int4 q = tiles[blockIdx.x];
+    if (q.x < 0) return;
+    em2_adj_e_tile<T, R>(args[q.x], q.y, q.z, t, first);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_adj_h_batch_kernel(const Em2AdjArgs<T> *args,
+                                                                            const int4 *tiles)
+{
+    const int4 q = tiles[blockIdx.x];
+    if (q.x < 0) return;
+    em2_adj_h_tile<T, R>(args[q.x], q.y, q.z);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) em2_correlate_batch_kernel(const Em2AdjArgs<T> *args,
+                                                                                const int4 *tiles, int n)
+{
+    const int4 q = tiles[blockIdx.x];
+    if (q.x < 0) return;
+    em2_correlate_tile<T, R>(args[q.x], q.y, q.z, n);
 }
 
 template <typename T>
@@ -518,6 +548,7 @@ static Em2AdjArgs<T> em2_adj_args(Device &d)
     B.lgx = l[10];
     B.adj = (T *)d.adj.seg_adj;
     B.src_lam = d.adj.src_lam;
+    B.grad = d.adj.grad;
     return B;
 }
 
@@ -553,7 +584,7 @@ static void em2_adj_correlate_t(Device &d, int n)
     constexpr int R = kSingleRows2d;
     const Em2AdjArgs<T> B = em2_adj_args<T>(d);
     const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
-    hipLaunchKernelGGL((em2_correlate_kernel<T, R>), grid, block, 0, d.stream, B, d.adj.grad, n);
+    hipLaunchKernelGGL((em2_correlate_kernel<T, R>), grid, block, 0, d.stream, B, n);
 }
 
 void em2_adj_correlate(Device &d, int first, int n)
@@ -561,6 +592,72 @@ void em2_adj_correlate(Device &d, int first, int n)
     (void)first;   // the store slots are relative to the segment start
     if (d.precision == SDT_FP64) em2_adj_correlate_t<double>(d, n);
     else em2_adj_correlate_t<float>(d, n);
+}
+
+template <typename T>
+static int em2_batch_adj_args_t(Batch &b, Device *const *shots, int n)
+{
+    std::vector<Em2AdjArgs<T>> a((size_t)n);
+    std::vector<AdjTailArgs> tails((size_t)n);
+    for (int s = 0; s < n; ++s) {
+        a[(size_t)s] = em2_adj_args<T>(*shots[s]);
+        AdjTailArgs &t = tails[(size_t)s];
+        t.lam[0] = a[(size_t)s].lex;
+        t.lam[1] = nullptr;
+        t.lam[2] = a[(size_t)s].lez;
+        t.rec_cell = shots[s]->rec_cell;
+        t.tg = shots[s]->adj.trace_grad;
+        t.nrec = shots[s]->nrec;
+    }
+    if (!b.adj_args && hipMalloc(&b.adj_args, a.size() * sizeof(Em2AdjArgs<T>)) != hipSuccess) return SDT_ERR_HIP;
+    if (!b.adj_tails && hipMalloc((void **)&b.adj_tails, tails.size() * sizeof(AdjTailArgs)) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (hipMemcpy(b.adj_args, a.data(), a.size() * sizeof(Em2AdjArgs<T>), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (hipMemcpy(b.adj_tails, tails.data(), tails.size() * sizeof(AdjTailArgs), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+int em2_batch_adj_args(Batch &b, Device *const *shots, int n)
+{
+    return b.precision == SDT_FP64 ? em2_batch_adj_args_t<double>(b, shots, n)
+                                   : em2_batch_adj_args_t<float>(b, shots, n);
+}
+
+template <typename T, int R>
+static void em2_batch_adj_launch(Batch &b, int which, int t, int first, int n, bool correlate)
+{
+    const Em2AdjArgs<T> *a = (const Em2AdjArgs<T> *)b.adj_args;
+    const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
+    if (correlate) {
+        hipLaunchKernelGGL((em2_correlate_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which], n);
+        return;
+    }
+    hipLaunchKernelGGL((em2_adj_e_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which], t, first);
+    hipLaunchKernelGGL((em2_adj_h_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
+}
+
+template <typename T>
+static void em2_batch_adj_p(Batch &b, int which, int t, int first, int n, bool correlate)
+{
+    switch (kBatchShapes[which].rows) {
+    case 1: em2_batch_adj_launch<T, 1>(b, which, t, first, n, correlate); break;
+    case 8: em2_batch_adj_launch<T, 8>(b, which, t, first, n, correlate); break;
+    default: em2_batch_adj_launch<T, 4>(b, which, t, first, n, correlate); break;
+    }
+}
+
+void em2_batch_adj_step(Batch &b, int which, int t, int first)
+{
+    if (b.precision == SDT_FP64) em2_batch_adj_p<double>(b, which, t, first, 0, false);
+    else em2_batch_adj_p<float>(b, which, t, first, 0, false);
+}
+
+void em2_batch_adj_correlate(Batch &b, int which, int n)
+{
+    if (b.precision == SDT_FP64) em2_batch_adj_p<double>(b, which, 0, 0, n, true);
+    else em2_batch_adj_p<float>(b, which, 0, 0, n, true);
 }
 
 void em2_step(Device &d, bool timed)

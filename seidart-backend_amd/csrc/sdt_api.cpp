@@ -353,15 +353,16 @@ static int ensure_graph(sdt_handle *h)
     return SDT_OK;
 }
 
-// checkpoint slot k <-> the state arrays (fields + memory variables)
-static void checkpoint_copy(Device &d, int k, bool save)
+// checkpoint slot k <-> the state arrays (fields + memory variables), on
+// stream s (the handle's, or its batch's)
+static void checkpoint_copy(Device &d, int k, bool save, hipStream_t s)
 {
     const Adjoint &a = d.adj;
     const size_t bytes = (size_t)d.ncell_dev * d.esize;
     for (int q = 0; q < a.nstate; ++q) {
         char *slot = a.ckpt + ((size_t)k * a.nstate + q) * bytes;
-        if (save) (void)hipMemcpyAsync(slot, a.state[q], bytes, hipMemcpyDeviceToDevice, d.stream);
-        else (void)hipMemcpyAsync(a.state[q], slot, bytes, hipMemcpyDeviceToDevice, d.stream);
+        if (save) (void)hipMemcpyAsync(slot, a.state[q], bytes, hipMemcpyDeviceToDevice, s);
+        else (void)hipMemcpyAsync(a.state[q], slot, bytes, hipMemcpyDeviceToDevice, s);
     }
 }
 
@@ -406,7 +407,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
         int n = left < chunk ? left : chunk;
         if (every > 0) {   // segments end at multiples of `every`; save each segment's first state
             const int into = h->steps_done % every;
-            if (into == 0) checkpoint_copy(d, h->steps_done / every, true);
+            if (into == 0) checkpoint_copy(d, h->steps_done / every, true, d.stream);
             n = n < every - into ? n : every - into;
         }
         const int first = h->steps_done + 1;
@@ -445,7 +446,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
         if (rc) return rc;
         h->steps_done += n;
         left -= n;
-        if (every > 0 && h->steps_done == d.time_steps) checkpoint_copy(d, d.adj.nckpt - 1, true);
+        if (every > 0 && h->steps_done == d.time_steps) checkpoint_copy(d, d.adj.nckpt - 1, true, d.stream);
     }
     return SDT_OK;
 }
@@ -731,7 +732,7 @@ int32_t sdt_adjoint(sdt_handle *h, const double *trace_grad, int64_t tcount, dou
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int first = seg * a.every;
         const int n = d.time_steps - first < a.every ? d.time_steps - first : a.every;
-        checkpoint_copy(d, seg, false);
+        checkpoint_copy(d, seg, false, d.stream);
         set_step_counter(d.stream, d.d_it, first + 1);
         a.store = true;
         a.store_first = first;
@@ -741,7 +742,7 @@ int32_t sdt_adjoint(sdt_handle *h, const double *trace_grad, int64_t tcount, dou
         adjoint_correlate(d, first, n);
         HIPCHK(h, hipGetLastError());
     }
-    checkpoint_copy(d, a.nckpt - 1, false);   // back to the forward run's final state
+    checkpoint_copy(d, a.nckpt - 1, false, d.stream);   // back to the forward run's final state
     set_step_counter(d.stream, d.d_it, d.time_steps + 1);
     HIPCHK(h, hipGetLastError());
     return adjoint_download(h, coef_grad, src_grad);
@@ -799,28 +800,15 @@ int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
     return SDT_OK;
 }
 
-int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_t *failed_step)
+// advance every unfinished shot of the batch `todo` steps on the batch's
+// stream: tile-shape tuning on the first steps (each candidate runs one
+// untimed and kTune timed real steps; every shape gives the same results),
+// then replays of a captured kSteps-step graph, then single steps
+static int batch_advance(sdt_batch *b, int todo)
 {
-    if (!b) return fail(nullptr, SDT_ERR_ARG, "null batch");
-    if (failed_shot) *failed_shot = -1;
-    if (failed_step) *failed_step = 0;
-    if (nsteps < 0) return bfail(b, SDT_ERR_ARG, "nsteps must be >= 0");
-    const int n = (int)b->shots.size();
-    std::vector<int> adv((size_t)n);
-    int todo = 0;
-    for (int q = 0; q < n; ++q) {
-        sdt_handle *h = b->shots[(size_t)q];
-        if (h->failed) return bfail(b, SDT_ERR_STATE, "a batch handle has already blown up");
-        const int left = h->d.time_steps - h->steps_done;
-        adv[(size_t)q] = nsteps < left ? nsteps : left;
-        todo = adv[(size_t)q] > todo ? adv[(size_t)q] : todo;
-    }
-    hipError_t e = hipSetDevice(b->shots[0]->p.device);
-    if (e != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
     Batch &B = b->b;
+    hipError_t e;
     int done = 0;
-    // autotune: each candidate tile height runs one untimed and kTune timed
-    // real steps (every height gives the same results), the fastest is kept
     constexpr int kTune = 4;
     if (B.choice < 0 && todo >= kNumBatchShapes * (kTune + 1)) {
         float best = 0;
@@ -851,6 +839,61 @@ int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_
     for (; b->graph && done + kSteps <= todo; done += kSteps)
         if ((e = hipGraphLaunch(b->graph, B.stream)) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
     for (; done < todo; ++done) batch_step(B, w);
+    return SDT_OK;
+}
+
+int32_t sdt_batch_run(sdt_batch *b, int32_t nsteps, int32_t *failed_shot, int32_t *failed_step)
+{
+    if (!b) return fail(nullptr, SDT_ERR_ARG, "null batch");
+    if (failed_shot) *failed_shot = -1;
+    if (failed_step) *failed_step = 0;
+    if (nsteps < 0) return bfail(b, SDT_ERR_ARG, "nsteps must be >= 0");
+    const int n = (int)b->shots.size();
+    std::vector<int> adv((size_t)n);
+    int todo = 0;
+    for (int q = 0; q < n; ++q) {
+        sdt_handle *h = b->shots[(size_t)q];
+        if (h->failed) return bfail(b, SDT_ERR_STATE, "a batch handle has already blown up");
+        const int left = h->d.time_steps - h->steps_done;
+        adv[(size_t)q] = nsteps < left ? nsteps : left;
+        todo = adv[(size_t)q] > todo ? adv[(size_t)q] : todo;
+    }
+    hipError_t e = hipSetDevice(b->shots[0]->p.device);
+    if (e != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    Batch &B = b->b;
+    // checkpointing shots (adjoint) advance in lockstep; pieces end at
+    // multiples of the interval and at each shot's last step, where the
+    // states are saved
+    const int every = b->shots[0]->d.adj.every;
+    for (int q = 0; q < n; ++q) {
+        const sdt_handle *h = b->shots[(size_t)q];
+        if (h->d.adj.every != every || (every > 0 && h->steps_done != b->shots[0]->steps_done))
+            return bfail(b, SDT_ERR_STATE, "checkpointing batch shots need one interval and one step count");
+    }
+    int rc = SDT_OK;
+    if (every <= 0) {
+        rc = batch_advance(b, todo);
+    } else {
+        const int pos = b->shots[0]->steps_done;
+        for (int done = 0; done < todo && rc == SDT_OK;) {
+            const int p = pos + done;
+            int piece = todo - done;
+            piece = piece < every - p % every ? piece : every - p % every;
+            for (int q = 0; q < n; ++q) {
+                Device &d = b->shots[(size_t)q]->d;
+                if (p >= d.time_steps) continue;
+                if (p % every == 0) checkpoint_copy(d, p / every, true, B.stream);
+                piece = piece < d.time_steps - p ? piece : d.time_steps - p;
+            }
+            rc = batch_advance(b, piece);
+            done += piece;
+            for (int q = 0; q < n; ++q) {
+                Device &d = b->shots[(size_t)q]->d;
+                if (pos + done == d.time_steps) checkpoint_copy(d, d.adj.nckpt - 1, true, B.stream);
+            }
+        }
+    }
+    if (rc) return rc;
     if ((e = hipGetLastError()) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
     if ((e = hipStreamSynchronize(B.stream)) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
     // every shot is checked and advanced; the first blow-up is reported
@@ -907,10 +950,62 @@ int32_t sdt_batch_destroy(sdt_batch *b)
 int32_t sdt_batch_adjoint(sdt_batch *b, const double *const *trace_grad, double *const *coef_grad,
                           double *const *src_grad)
 {
-    (void)trace_grad;
-    (void)coef_grad;
-    (void)src_grad;
-    return bfail(b, SDT_ERR_ARG, "batched adjoint not available yet");
+    if (!b || !trace_grad || !coef_grad) return fail(nullptr, SDT_ERR_ARG, "null argument");
+    const int n = (int)b->shots.size();
+    sdt_handle *h0 = b->shots[0];
+    const int every = h0->d.adj.every, nt = h0->d.time_steps;
+    for (int q = 0; q < n; ++q) {
+        const sdt_handle *h = b->shots[(size_t)q];
+        if (h->d.adj.every <= 0 || h->d.adj.every != every || h->d.time_steps != nt || h->steps_done != nt)
+            return bfail(b, SDT_ERR_STATE,
+                         "every shot must have run all of one time_steps with one SDT_OPT_CHECKPOINT interval");
+        if (h->failed) return bfail(b, SDT_ERR_STATE, "a batch handle has blown up");
+        if (!coef_grad[q] || (h->d.nrec > 0 && !trace_grad[q])) return bfail(b, SDT_ERR_ARG, "null per-shot array");
+    }
+    hipError_t e = hipSetDevice(h0->p.device);
+    if (e != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    Batch &B = b->b;
+    std::vector<Device *> devs((size_t)n);
+    for (int q = 0; q < n; ++q) {
+        sdt_handle *h = b->shots[(size_t)q];
+        devs[(size_t)q] = &h->d;
+        if (adjoint_alloc(h) != SDT_OK) return bfail(b, SDT_ERR_HIP, "adjoint allocation failed (out of device memory?)");
+        int rc = adjoint_reset(h, trace_grad[q]);
+        if (rc) return bfail(b, rc, h->err);
+    }
+    if (batch_adj_args(B, devs.data(), n) != SDT_OK) return bfail(b, SDT_ERR_HIP, "adjoint argument upload failed");
+    const int w = B.choice < 0 ? 1 : B.choice;
+    const int nseg = h0->d.adj.nckpt - 1;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int first = seg * every;
+        const int len = nt - first < every ? nt - first : every;
+        for (int q = 0; q < n; ++q) {
+            Device &d = *devs[(size_t)q];
+            d.adj.store_first = first;
+            checkpoint_copy(d, seg, false, B.stream);
+            set_step_counter(B.stream, d.d_it, first + 1);
+        }
+        if (batch_refresh_args(B, devs.data(), n) != SDT_OK) return bfail(b, SDT_ERR_HIP, "argument upload failed");
+        B.store = true;
+        for (int s = 0; s < len; ++s) batch_step(B, w);
+        B.store = false;
+        for (int t = first + len - 1; t >= first; --t) batch_adj_step(B, w, t, first);
+        batch_adj_correlate(B, w, len);
+        if ((e = hipGetLastError()) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    }
+    for (int q = 0; q < n; ++q) {   // back to the forward run's final states
+        Device &d = *devs[(size_t)q];
+        checkpoint_copy(d, d.adj.nckpt - 1, false, B.stream);
+        set_step_counter(B.stream, d.d_it, nt + 1);
+        d.adj.store_first = 0;
+    }
+    if ((e = hipStreamSynchronize(B.stream)) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
+    if (batch_refresh_args(B, devs.data(), n) != SDT_OK) return bfail(b, SDT_ERR_HIP, "argument upload failed");
+    for (int q = 0; q < n; ++q) {
+        int rc = adjoint_download(b->shots[(size_t)q], coef_grad[q], src_grad ? src_grad[q] : nullptr);
+        if (rc) return bfail(b, rc, b->shots[(size_t)q]->err);
+    }
+    return SDT_OK;
 }
 
 int32_t sdt_synchronize(sdt_handle *h)

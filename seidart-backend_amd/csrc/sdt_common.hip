@@ -162,6 +162,8 @@ void batch_free(Batch &b)
     if (b.fps) (void)hipFree(b.fps);
     if (b.tails) (void)hipFree(b.tails);
     if (b.args) (void)hipFree(b.args);
+    if (b.adj_args) (void)hipFree(b.adj_args);
+    if (b.adj_tails) (void)hipFree(b.adj_tails);
     b = Batch();
 }
 
@@ -175,21 +177,62 @@ void batch_step(Batch &b, int which)
 // ---- adjoint helpers ---------------------------------------------------------
 // Trace-gradient row t added to the adjoint of the recorded fields at the
 // receiver cells (atomics: receivers may share a cell).
-struct AdjTailArgs {
-    void *lam[3];             // adjoint array of each recorded component (or null)
-    const int64_t *rec_cell;
-    const double *tg;         // [time_steps][nrec][3]
-    int nrec;
-};
-
 template <typename T>
-__global__ void __launch_bounds__(256) adj_tail_kernel(AdjTailArgs a, int t)
+__device__ __forceinline__ void adj_tail_body(const AdjTailArgs &a, int t)
 {
     for (int q = threadIdx.x; q < a.nrec; q += blockDim.x)
         for (int m = 0; m < 3; ++m)
             if (a.lam[m])
                 atomicAdd(reinterpret_cast<T *>(a.lam[m]) + a.rec_cell[q],
                           (T)a.tg[((int64_t)t * a.nrec + q) * 3 + m]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) adj_tail_kernel(AdjTailArgs a, int t)
+{
+    adj_tail_body<T>(a, t);
+}
+
+// one workgroup per shot
+template <typename T>
+__global__ void __launch_bounds__(256) adj_tail_batch_kernel(const AdjTailArgs *a, int t)
+{
+    adj_tail_body<T>(a[blockIdx.x], t);
+}
+
+void launch_adj_tail_batch(Batch &b, int t)
+{
+    if (b.precision == SDT_FP64)
+        hipLaunchKernelGGL(adj_tail_batch_kernel<double>, dim3(b.nshots), dim3(256), 0, b.stream, b.adj_tails, t);
+    else
+        hipLaunchKernelGGL(adj_tail_batch_kernel<float>, dim3(b.nshots), dim3(256), 0, b.stream, b.adj_tails, t);
+}
+
+int batch_refresh_args(Batch &b, Device *const *shots, int n)
+{
+    if (hipStreamSynchronize(b.stream) != hipSuccess) return SDT_ERR_HIP;   // no launch reads them now
+    if (b.physics == SDT_ELASTIC2) return el2_batch_args(b, shots, n);
+    return em2_batch_args(b, shots, n);
+}
+
+int batch_adj_args(Batch &b, Device *const *shots, int n)
+{
+    if (hipStreamSynchronize(b.stream) != hipSuccess) return SDT_ERR_HIP;
+    if (b.physics == SDT_ELASTIC2) return el2_batch_adj_args(b, shots, n);
+    return em2_batch_adj_args(b, shots, n);
+}
+
+void batch_adj_step(Batch &b, int which, int t, int first)
+{
+    launch_adj_tail_batch(b, t);
+    if (b.physics == SDT_ELASTIC2) el2_batch_adj_step(b, which, t, first);
+    else em2_batch_adj_step(b, which, t, first);
+}
+
+void batch_adj_correlate(Batch &b, int which, int n)
+{
+    if (b.physics == SDT_ELASTIC2) el2_batch_adj_correlate(b, which, n);
+    else em2_batch_adj_correlate(b, which, n);
 }
 
 void launch_adj_tail(Device &d, int t, void *const lam_field[3])

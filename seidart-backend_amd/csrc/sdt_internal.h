@@ -79,6 +79,14 @@ struct Adjoint {
     int store_first = 0;               // 0-based first step of the recomputed segment
 };
 
+// trace-gradient injection of one adjoint step
+struct AdjTailArgs {
+    void *lam[3];             // adjoint array of each recorded component (or null)
+    const int64_t *rec_cell;
+    const double *tg;         // [time_steps][nrec][3]
+    int nrec;
+};
+
 struct Device {
     int physics = 0, precision = 0;
     int nx = 0, ny = 0, nz = 0;
@@ -219,6 +227,8 @@ struct Batch {
     TailArgs *tails = nullptr;     // per-shot tail arguments (device)
     int choice = -1;               // autotuned tile shape index, -1 = not yet
     bool store = false;            // forward E / velocity kernels write the segment store
+    void *adj_args = nullptr;      // per-shot adjoint kernel arguments (device)
+    AdjTailArgs *adj_tails = nullptr;
     hipStream_t stream = nullptr;
 };
 int batch_setup(Batch &b, Device *const *shots, int n);
@@ -230,6 +240,20 @@ int em2_batch_args(Batch &b, Device *const *shots, int n);
 void el2_batch_launch(Batch &b, int which);
 void em2_batch_launch(Batch &b, int which);
 void launch_tail_batch(Batch &b);
+// batched adjoint: (re)build the per-shot forward arguments (after changing
+// a shot's segment start) and adjoint arguments, reverse step t of every
+// shot, correlate the stored segment of every shot
+int batch_refresh_args(Batch &b, Device *const *shots, int n);
+int batch_adj_args(Batch &b, Device *const *shots, int n);
+void batch_adj_step(Batch &b, int which, int t, int first);
+void batch_adj_correlate(Batch &b, int which, int n);
+int em2_batch_adj_args(Batch &b, Device *const *shots, int n);
+void em2_batch_adj_step(Batch &b, int which, int t, int first);
+void em2_batch_adj_correlate(Batch &b, int which, int n);
+int el2_batch_adj_args(Batch &b, Device *const *shots, int n);
+void el2_batch_adj_step(Batch &b, int which, int t, int first);
+void el2_batch_adj_correlate(Batch &b, int which, int n);
+void launch_adj_tail_batch(Batch &b, int t);
 
 // common pieces (sdt_common.hip)
 void launch_tail(Device &d);
