@@ -32,7 +32,19 @@ struct El2Args {
     unsigned long long *vnorm;
     int64_t src_cell;   // excluded from the fused norm (the tail adds it)
     int nsteps;         // time_steps of the run (batched shots stop there)
+    // segment recomputation for the adjoint: per step (0-based t), slot
+    // t - store_first of `store` ([slot][kEl2Fwd][stride]) holds the
+    // CPML-corrected derivatives and pre-attenuation stresses of every cell
+    // (zero outside each update's region): 0 dvx/dx, 1 dvz/dz, 2 shear
+    // (sigma_xx/zz region), 3 sigma_xx and 4 sigma_zz before the 1/(1 +
+    // gamma dt) factor, 5 dvx/dx, 6 dvz/dz, 7 dvz/dx + dvx/dz (sigma_xz
+    // region), 8 sigma_xz before the factor, 9 dsxx/dx, 10 dsxz/dz (vx
+    // region), 11 dsxz/dx, 12 dszz/dz (vz region)
+    T *store;
+    int64_t stride;
+    int store_first;
 };
+constexpr int kEl2Fwd = 13;
 
 // memory recursion psi = b*psi + a*d ; d = d/kappa + psi (cpmlfdtd.f08:332-340)
 template <typename T>
@@ -46,14 +58,19 @@ __device__ __forceinline__ T cpml_fix(T d, T &psi, T a, T b, T ik)
 // operand is read before the first store: the compiler cannot tell the
 // arrays apart, so a store ahead of a load would serialise the memory
 // round trips.
-template <typename T, bool PML>
+template <typename T, bool PML, bool STORE>
 __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
     const bool r1 = i >= 2 && i <= nx - 2 && j >= 3 && j <= nz - 1;   // sigma_xx, sigma_zz
     const bool r2 = i >= 3 && i <= nx - 1 && j >= 2 && j <= nz - 2;   // sigma_xz
-    if (!r1 && !r2) return;
     const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+    T *st = nullptr;
+    if (STORE) {
+        st = A.store + (int64_t)(*A.d_it - 1 - A.store_first) * kEl2Fwd * A.stride + c;
+        for (int q = 0; q < 9; ++q) st[q * A.stride] = T(0);
+    }
+    if (!r1 && !r2) return;
     const T *vx = A.vx, *vz = A.vz;
     const int64_t sx = 1, sz = A.pitch;
     // stencil operands: x0 = vx(i,j), xl/xr = vx(i-+1,j), xd/xu = vx(i,j-+1), ...
@@ -112,8 +129,17 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
             dvz_dx = cpml_fix(dvz_dx, m[3], a, b, ik);
         }
         const T shear = dvz_dx + dvx_dz;
-        s_xx = (s_xx + (k11 * dvx_dx + k13 * dvz_dz + k15 * shear) * A.dt) * gx;
-        s_zz = (s_zz + (k13 * dvx_dx + k33 * dvz_dz + k35 * shear) * A.dt) * gz;
+        const T pxx = s_xx + (k11 * dvx_dx + k13 * dvz_dz + k15 * shear) * A.dt;
+        const T pzz = s_zz + (k13 * dvx_dx + k33 * dvz_dz + k35 * shear) * A.dt;
+        s_xx = pxx * gx;
+        s_zz = pzz * gz;
+        if (STORE) {
+            st[0] = dvx_dx;
+            st[A.stride] = dvz_dz;
+            st[2 * A.stride] = shear;
+            st[3 * A.stride] = pxx;
+            st[4 * A.stride] = pzz;
+        }
     }
     if (r2) {
         T dvx_dx = (T(27) * (x0 - xl) - (xr - xl2)) * A.rdx24;
@@ -126,7 +152,14 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
             dvz_dz = cpml_fix(dvz_dz, m[6], ah, bh, ikh);
             dvz_dx = cpml_fix(dvz_dx, m[7], ah, bh, ikh);
         }
-        s_xz = (s_xz + (k15 * dvx_dx + k35 * dvz_dz + k55 * (dvz_dx + dvx_dz)) * A.dt) * gxz;
+        const T pxz = s_xz + (k15 * dvx_dx + k35 * dvz_dz + k55 * (dvz_dx + dvx_dz)) * A.dt;
+        s_xz = pxz * gxz;
+        if (STORE) {
+            st[5 * A.stride] = dvx_dx;
+            st[6 * A.stride] = dvz_dz;
+            st[7 * A.stride] = dvz_dx + dvx_dz;
+            st[8 * A.stride] = pxz;
+        }
     }
     if (r1) {
         A.sxx[c] = s_xx;
@@ -160,7 +193,7 @@ __device__ __forceinline__ T tplus(T f0, T fp1, T fp2, T fm1, int corr)
 // cell.  Loads first, as in the stress update.  The norm is the square root
 // of the maximum squared magnitude (sqrt is monotone and correctly rounded,
 // so this equals the maximum of the magnitudes).
-template <typename T, bool PML>
+template <typename T, bool PML, bool STORE>
 __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
@@ -227,6 +260,11 @@ __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, 
                 d2 = cpml_fix(d2, m[1], a, b, ik);
             }
             vx = vx + A.dt * (d1 / rhoxx + d2 / rhozx);
+            if (STORE) {
+                T *st = A.store + (int64_t)(*A.d_it - 1 - A.store_first) * kEl2Fwd * A.stride + c;
+                st[9 * A.stride] = d1;
+                st[10 * A.stride] = d2;
+            }
         }
         if (rz) {
             const T rhoxz = scalar_mean(r, rzr, A.density_code);
@@ -238,7 +276,17 @@ __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, 
                 d2 = cpml_fix(d2, m[3], ah, bh, ikh);
             }
             vz = vz + A.dt * (d1 / rhoxz + d2 / rhozz);
+            if (STORE) {
+                T *st = A.store + (int64_t)(*A.d_it - 1 - A.store_first) * kEl2Fwd * A.stride + c;
+                st[11 * A.stride] = d1;
+                st[12 * A.stride] = d2;
+            }
         }
+    }
+    if (STORE) {   // zero outside the update regions
+        T *st = A.store + (int64_t)(*A.d_it - 1 - A.store_first) * kEl2Fwd * A.stride + c;
+        if (edge || !rx) st[9 * A.stride] = st[10 * A.stride] = T(0);
+        if (edge || !rz) st[11 * A.stride] = st[12 * A.stride] = T(0);
     }
     if (edge || rx) A.vx[c] = vx;
     if (edge || rz) A.vz[c] = vz;
@@ -290,6 +338,9 @@ static El2Args<T> el2_args(Device &d)
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
     A.nsteps = d.time_steps;
+    A.store = (T *)d.adj.seg_fwd;
+    A.stride = d.ncell_dev;
+    A.store_first = d.adj.store_first;
     return A;
 }
 
@@ -308,7 +359,7 @@ __device__ __forceinline__ bool el2_pml(const El2Args<T> &A, int i, int j)
     return !in_box(A.box, i, 1, j) || A.flags[((i - 1) >> 6) + (size_t)A.tiles_x * ((j - 1) >> 2)];
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __device__ __forceinline__ void el2_stress_tile(const El2Args<T> &A, int tx, int band)
 {
     const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
@@ -318,14 +369,14 @@ __device__ __forceinline__ void el2_stress_tile(const El2Args<T> &A, int tx, int
         const int j = j0 + r;
         if (j > A.nz) break;
         if (__any(valid && el2_pml(A, i, j))) {
-            if (valid) el2_stress_cell<T, true>(A, i, j);
+            if (valid) el2_stress_cell<T, true, STORE>(A, i, j);
         } else if (valid) {
-            el2_stress_cell<T, false>(A, i, j);
+            el2_stress_cell<T, false, STORE>(A, i, j);
         }
     }
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __device__ __forceinline__ double el2_velocity_tile(const El2Args<T> &A, int tx, int band)
 {
     const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
@@ -337,29 +388,29 @@ __device__ __forceinline__ double el2_velocity_tile(const El2Args<T> &A, int tx,
         if (j > A.nz) break;
         double v = 0.0;
         if (__any(valid && el2_pml(A, i, j))) {
-            if (valid) v = el2_velocity_cell<T, true>(A, i, j);
+            if (valid) v = el2_velocity_cell<T, true, STORE>(A, i, j);
         } else if (valid) {
-            v = el2_velocity_cell<T, false>(A, i, j);
+            v = el2_velocity_cell<T, false, STORE>(A, i, j);
         }
         nrm = (v > nrm || v != v) ? v : nrm;
     }
     return nrm;
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_kernel(El2Args<T> A)
 {
-    el2_stress_tile<T, R>(A, blockIdx.x, blockIdx.y);
+    el2_stress_tile<T, R, STORE>(A, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __global__ void __launch_bounds__(64 * kTile2dWaves) el2_velocity_kernel(El2Args<T> A)
 {
-    const double nrm = sqrt(el2_velocity_tile<T, R>(A, blockIdx.x, blockIdx.y));
+    const double nrm = sqrt(el2_velocity_tile<T, R, STORE>(A, blockIdx.x, blockIdx.y));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_batch_kernel(const El2Args<T> *args,
                                                                              const int4 *tiles)
 {
@@ -367,10 +418,10 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_batch_kernel(con
     if (t.x < 0) return;   // padding tile
     const El2Args<T> A = args[t.x];
     if (*A.d_it > A.nsteps) return;   // finished shot (uniform per workgroup)
-    el2_stress_tile<T, R>(A, t.y, t.z);
+    el2_stress_tile<T, R, STORE>(A, t.y, t.z);
 }
 
-template <typename T, int R>
+template <typename T, int R, bool STORE>
 __global__ void __launch_bounds__(64 * kTile2dWaves) el2_velocity_batch_kernel(const El2Args<T> *args,
                                                                                const int4 *tiles)
 {
@@ -378,7 +429,7 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) el2_velocity_batch_kernel(c
     if (t.x < 0) return;   // padding tile
     const El2Args<T> A = args[t.x];
     if (*A.d_it > A.nsteps) return;
-    const double nrm = sqrt(el2_velocity_tile<T, R>(A, t.y, t.z));
+    const double nrm = sqrt(el2_velocity_tile<T, R, STORE>(A, t.y, t.z));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -403,8 +454,13 @@ static void el2_batch_launch_t(Batch &b, int which)
 {
     const El2Args<T> *a = (const El2Args<T> *)b.args;
     const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
-    hipLaunchKernelGGL((el2_stress_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
-    hipLaunchKernelGGL((el2_velocity_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
+    if (b.store) {
+        hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, true>), grid, block, 0, b.stream, a, b.tiles[which]);
+        hipLaunchKernelGGL((el2_velocity_batch_kernel<T, R, true>), grid, block, 0, b.stream, a, b.tiles[which]);
+    } else {
+        hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, false>), grid, block, 0, b.stream, a, b.tiles[which]);
+        hipLaunchKernelGGL((el2_velocity_batch_kernel<T, R, false>), grid, block, 0, b.stream, a, b.tiles[which]);
+    }
 }
 
 template <typename T>
@@ -430,53 +486,569 @@ static void el2_step_t(Device &d, bool timed)
     const El2Args<T> A = el2_args<T>(d);
     const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    hipLaunchKernelGGL((el2_stress_kernel<T, R>), grid, block, 0, d.stream, A);
+    if (d.adj.store)
+        hipLaunchKernelGGL((el2_stress_kernel<T, R, true>), grid, block, 0, d.stream, A);
+    else
+        hipLaunchKernelGGL((el2_stress_kernel<T, R, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL((el2_velocity_kernel<T, R>), grid, block, 0, d.stream, A);
+    if (d.adj.store)
+        hipLaunchKernelGGL((el2_velocity_kernel<T, R, true>), grid, block, 0, d.stream, A);
+    else
+        hipLaunchKernelGGL((el2_velocity_kernel<T, R, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }
 
+// ---- adjoint state (discrete adjoint of the step above) ----------------------
+// Step t: sigma' = S(sigma^t, v^t, psi) (stress pass), v' = V(v^t, sigma',
+// psi) (velocity pass), then the point sources add to sigma' and v'
+// (sigma^{t+1}, v^{t+1}) and the receivers read v^{t+1}.  Its transpose, per
+// step, latest first:
+//   tail     trace-gradient row t into lam(v^{t+1}) at the receivers;
+//   V pass   lam(v^{t+1}) completed with the S-pass transfer terms of step
+//            t+1 (recorded at the source cell and in the segment store),
+//            pushed through V: own-cell lam(v^t) part, lam(psi_V), transfer
+//            terms P, Q, U, V (the derivative adjoints, scaled by
+//            1/(24 dx) or 1/(24 dz)) to lam(sigma');
+//   S pass   lam(sigma^{t+1}) (recorded at the source cells) plus the
+//            gathered V transfer terms = lam(sigma'), stored, pushed through
+//            S: own-cell lam(sigma^t), lam(psi_S), transfer terms to
+//            lam(v^t) gathered by the next V pass.
+// Gradients (per cell, summed over steps in FP64 by the correlation pass):
+//   c11: dt igx lxx a, c13: dt (igx lxx b + igz lzz a), c15: dt (igx lxx s +
+//   igxz lxz p), c33: dt igz lzz b, c35: dt (igz lzz s + igxz lxz q), c55:
+//   dt igxz lxz (r + u), igx: lxx pxx, igz: lzz pzz, igxz: lxz pxz, and for
+//   the density the four sums lvx d1x, lvx d2x, lvz d1z, lvz d2z, turned
+//   into d/drho through the face averages once all segments are done.
+constexpr int kEl2Adj = 5;    // stored adjoint values: lam sxx', szz', sxz', vx', vz'
+constexpr int kEl2Acc = 14;   // FP64 accumulators: the 10 coefficients + 4 density sums
+
+template <typename T>
+struct El2AdjArgs {
+    El2Args<T> F;
+    T *lvx, *lvz, *lxx, *lzz, *lxz;     // own-cell parts of the field adjoints
+    T *lm[12];                          // lam of the memory variables (forward order)
+    T *sx[8];                           // S transfers: Ax Bz Ez Fx (sigma_xx/zz region), Cx Dz Gz Hx (sigma_xz)
+    T *vt[4];                           // V transfers: P Q (vx region), U V (vz region)
+    T *adj;                             // segment store [slot][kEl2Adj][stride]
+    double *src_lam;                    // [time_steps][kAdjSrcSlots]
+    double *grad;                       // [kEl2Acc][stride]
+    int64_t src_cell2;                  // cell (is + 1, js + 1) of the sigma_xz source
+};
+
+// neighbour value of a transfer array, 0 outside the grid
+template <typename T>
+__device__ __forceinline__ T el2_nb(const T *f, const El2Args<T> &A, int64_t c, int i, int j, int di, int dj)
+{
+    const int ii = i + di, jj = j + dj;
+    if (ii < 1 || ii > A.nx || jj < 1 || jj > A.nz) return T(0);
+    return f[c + di + (int64_t)dj * A.pitch];
+}
+
+// CPML transpose: forward d' = d ik + psi', psi' = b psi + a d
+template <typename T>
+__device__ __forceinline__ T cpml_fix_adj(T ld, T &lpsi, T a, T b, T ik)
+{
+    const T L = lpsi + ld;
+    lpsi = b * L;
+    return ld * ik + a * L;
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ void el2_adj_v_cell(const El2AdjArgs<T> &B, int i, int j, int t, int first)
+{
+    const El2Args<T> &A = B.F;
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+    // complete lam(v^{t+1}) with the transposed stress-pass derivatives of step t+1
+    T gx = B.lvx[c], gz = B.lvz[c];
+    {
+        T *const *S = B.sx;
+        gx += el2_nb(S[0], A, c, i, j, 1, 0) - T(27) * S[0][c] + T(27) * el2_nb(S[0], A, c, i, j, -1, 0) -
+              el2_nb(S[0], A, c, i, j, -2, 0);
+        gx += T(27) * S[1][c] - T(27) * el2_nb(S[1], A, c, i, j, 0, 1) - el2_nb(S[1], A, c, i, j, 0, -1) +
+              el2_nb(S[1], A, c, i, j, 0, 2);
+        gx += T(27) * S[4][c] - T(27) * el2_nb(S[4], A, c, i, j, 1, 0) - el2_nb(S[4], A, c, i, j, -1, 0) +
+              el2_nb(S[4], A, c, i, j, 2, 0);
+        gx += T(27) * el2_nb(S[5], A, c, i, j, 0, -1) - T(27) * S[5][c] - el2_nb(S[5], A, c, i, j, 0, -2) +
+              el2_nb(S[5], A, c, i, j, 0, 1);
+        gz += el2_nb(S[3], A, c, i, j, 1, 0) - T(27) * S[3][c] + T(27) * el2_nb(S[3], A, c, i, j, -1, 0) -
+              el2_nb(S[3], A, c, i, j, -2, 0);
+        gz += T(27) * S[2][c] - T(27) * el2_nb(S[2], A, c, i, j, 0, 1) - el2_nb(S[2], A, c, i, j, 0, -1) +
+              el2_nb(S[2], A, c, i, j, 0, 2);
+        gz += T(27) * S[7][c] - T(27) * el2_nb(S[7], A, c, i, j, 1, 0) - el2_nb(S[7], A, c, i, j, -1, 0) +
+              el2_nb(S[7], A, c, i, j, 2, 0);
+        gz += T(27) * el2_nb(S[6], A, c, i, j, 0, -1) - T(27) * S[6][c] - el2_nb(S[6], A, c, i, j, 0, -2) +
+              el2_nb(S[6], A, c, i, j, 0, 1);
+    }
+    if (c == A.src_cell) {
+        B.src_lam[kAdjSrcSlots * (int64_t)t + 3] = (double)gx;
+        B.src_lam[kAdjSrcSlots * (int64_t)t + 4] = (double)gz;
+    }
+    T *as = B.adj + (int64_t)(t - first) * kEl2Adj * A.stride + c;
+    as[3 * A.stride] = gx;
+    as[4 * A.stride] = gz;
+    const bool edge = i == 1 || i == nx || j == 1 || j == nz;
+    const bool rx = i >= 3 && i <= nx - 1 && j >= 3 && j <= nz - 1;
+    const bool rz = i >= 2 && i <= nx - 2 && j >= 2 && j <= nz - 2;
+    T P = 0, Q = 0, U = 0, V = 0;
+    if (rx || rz) {
+        const T r = A.rho[c];
+        if (rx) {
+            const T rhoxx = scalar_mean(r, A.rho[c - 1], A.density_code);
+            const T rhozx = scalar_mean(r, A.rho[c - A.pitch], A.density_code);
+            T l1 = A.dt * gx / rhoxx, l2 = A.dt * gx / rhozx;
+            if (PML) {
+                const T a = ldnt(A.a + c), b = ldnt(A.b + c), ik = ldnt(A.ik + c);
+                l1 = cpml_fix_adj(l1, B.lm[8][c], a, b, ik);
+                l2 = cpml_fix_adj(l2, B.lm[9][c], a, b, ik);
+            }
+            P = l1 * A.rdx24;
+            Q = l2 * A.rdz24;
+        }
+        if (rz) {
+            const T rhoxz = scalar_mean(r, A.rho[c + 1], A.density_code);
+            const T rhozz = scalar_mean(r, A.rho[c + A.pitch], A.density_code);
+            T l1 = A.dt * gz / rhoxz, l2 = A.dt * gz / rhozz;
+            if (PML) {
+                const T ah = ldnt(A.ah + c), bh = ldnt(A.bh + c), ikh = ldnt(A.ikh + c);
+                l1 = cpml_fix_adj(l1, B.lm[10][c], ah, bh, ikh);
+                l2 = cpml_fix_adj(l2, B.lm[11][c], ah, bh, ikh);
+            }
+            U = l1 * A.rdx24;
+            V = l2 * A.rdz24;
+        }
+    }
+    B.vt[0][c] = P;
+    B.vt[1][c] = Q;
+    B.vt[2][c] = U;
+    B.vt[3][c] = V;
+    B.lvx[c] = edge ? T(0) : gx;
+    B.lvz[c] = edge ? T(0) : gz;
+}
+
+template <typename T, bool PML>
+__device__ __forceinline__ void el2_adj_s_cell(const El2AdjArgs<T> &B, int i, int j, int t, int first)
+{
+    const El2Args<T> &A = B.F;
+    const int nx = A.nx, nz = A.nz;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+    const int corr = A.corrected;
+    T lxx = B.lxx[c], lzz = B.lzz[c], lxz = B.lxz[c];
+    if (c == A.src_cell) {
+        B.src_lam[kAdjSrcSlots * (int64_t)t] = (double)lxx;
+        B.src_lam[kAdjSrcSlots * (int64_t)t + 2] = (double)lzz;
+    }
+    if (c == B.src_cell2) B.src_lam[kAdjSrcSlots * (int64_t)t + 1] = (double)lxz;
+    // lam(sigma') = lam(sigma^{t+1}) + transposed velocity-pass differences
+    {
+        T *const *V = B.vt;
+        lxx += T(27) * V[0][c] - T(27) * el2_nb(V[0], A, c, i, j, 1, 0) + el2_nb(V[0], A, c, i, j, 2, 0);
+        lxz += T(27) * V[1][c] - T(27) * el2_nb(V[1], A, c, i, j, 0, 1) + el2_nb(V[1], A, c, i, j, 0, 2);
+        lxz += -T(27) * V[2][c] + T(27) * el2_nb(V[2], A, c, i, j, -1, 0) - el2_nb(V[2], A, c, i, j, -2, 0);
+        lzz += -T(27) * V[3][c] + T(27) * el2_nb(V[3], A, c, i, j, 0, -1) - el2_nb(V[3], A, c, i, j, 0, -2);
+        if (corr) {
+            lxx -= el2_nb(V[0], A, c, i, j, -1, 0);
+            lxz -= el2_nb(V[1], A, c, i, j, 0, -1);
+            lxz += el2_nb(V[2], A, c, i, j, 1, 0);
+            lzz += el2_nb(V[3], A, c, i, j, 0, 1);
+        }
+    }
+    T *as = B.adj + (int64_t)(t - first) * kEl2Adj * A.stride + c;
+    as[0] = lxx;
+    as[A.stride] = lzz;
+    as[2 * A.stride] = lxz;
+    const bool r1 = i >= 2 && i <= nx - 2 && j >= 3 && j <= nz - 1;
+    const bool r2 = i >= 3 && i <= nx - 1 && j >= 2 && j <= nz - 2;
+    T s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0;
+    if (r1) {
+        const T gx = ldnt(A.igx + c), gz = ldnt(A.igz + c);
+        const T ax = A.dt * gx * lxx, az = A.dt * gz * lzz;
+        T lux = ax * ldnt(A.c11 + c) + az * ldnt(A.c13 + c);
+        T lwz = ax * ldnt(A.c13 + c) + az * ldnt(A.c33 + c);
+        const T lsh = ax * ldnt(A.c15 + c) + az * ldnt(A.c35 + c);
+        T lwx = lsh, luz = lsh;
+        if (PML) {
+            const T ah = ldnt(A.ah + c), bh = ldnt(A.bh + c), ikh = ldnt(A.ikh + c);
+            const T a = ldnt(A.a + c), b = ldnt(A.b + c), ik = ldnt(A.ik + c);
+            lux = cpml_fix_adj(lux, B.lm[0][c], ah, bh, ikh);
+            luz = cpml_fix_adj(luz, B.lm[1][c], ah, bh, ikh);
+            lwz = cpml_fix_adj(lwz, B.lm[2][c], a, b, ik);
+            lwx = cpml_fix_adj(lwx, B.lm[3][c], a, b, ik);
+        }
+        s0 = lux * A.rdx24;   // D+x vx
+        s1 = luz * A.rdz24;   // D-z vx
+        s2 = lwz * A.rdz24;   // D-z vz
+        s3 = lwx * A.rdx24;   // D+x vz
+        lxx *= gx;
+        lzz *= gz;
+    }
+    if (r2) {
+        const T g = ldnt(A.igxz + c);
+        const T al = A.dt * g * lxz;
+        T lux = al * ldnt(A.c15 + c), lwz = al * ldnt(A.c35 + c);
+        T lwx = al * ldnt(A.c55 + c), luz = lwx;
+        if (PML) {
+            const T ah = ldnt(A.ah + c), bh = ldnt(A.bh + c), ikh = ldnt(A.ikh + c);
+            lux = cpml_fix_adj(lux, B.lm[4][c], ah, bh, ikh);
+            luz = cpml_fix_adj(luz, B.lm[5][c], ah, bh, ikh);
+            lwz = cpml_fix_adj(lwz, B.lm[6][c], ah, bh, ikh);
+            lwx = cpml_fix_adj(lwx, B.lm[7][c], ah, bh, ikh);
+        }
+        s4 = lux * A.rdx24;   // D-x vx
+        s5 = luz * A.rdz24;   // D+z vx
+        s6 = lwz * A.rdz24;   // D+z vz
+        s7 = lwx * A.rdx24;   // D-x vz
+        lxz *= g;
+    }
+    B.sx[0][c] = s0;
+    B.sx[1][c] = s1;
+    B.sx[2][c] = s2;
+    B.sx[3][c] = s3;
+    B.sx[4][c] = s4;
+    B.sx[5][c] = s5;
+    B.sx[6][c] = s6;
+    B.sx[7][c] = s7;
+    B.lxx[c] = lxx;
+    B.lzz[c] = lzz;
+    B.lxz[c] = lxz;
+}
+
+template <typename T, int R, int PASS>   // PASS 0: velocity adjoint, 1: stress adjoint
+__device__ __forceinline__ void el2_adj_tile(const El2AdjArgs<T> &B, int tx, int band, int t, int first)
+{
+    const El2Args<T> &A = B.F;
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    const bool valid = i >= 1 && i <= A.nx;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        if (__any(valid && el2_pml(A, i, j))) {
+            if (valid) {
+                if (PASS == 0) el2_adj_v_cell<T, true>(B, i, j, t, first);
+                else el2_adj_s_cell<T, true>(B, i, j, t, first);
+            }
+        } else if (valid) {
+            if (PASS == 0) el2_adj_v_cell<T, false>(B, i, j, t, first);
+            else el2_adj_s_cell<T, false>(B, i, j, t, first);
+        }
+    }
+}
+
+// per-cell FP64 sums over the stored segment, folded into the gradients
+template <typename T, int R>
+__device__ __forceinline__ void el2_correlate_tile(const El2AdjArgs<T> &B, int tx, int band, int n)
+{
+    const El2Args<T> &A = B.F;
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    if (i < 1 || i > A.nx) return;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    const int64_t st = A.stride;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
+        double S[16] = {};
+        for (int s = 0; s < n; ++s) {
+            const T *f = A.store + (int64_t)s * kEl2Fwd * st + c;
+            const T *l = B.adj + (int64_t)s * kEl2Adj * st + c;
+            const double lxx = ldnt(l), lzz = ldnt(l + st), lxz = ldnt(l + 2 * st);
+            const double lvx = ldnt(l + 3 * st), lvz = ldnt(l + 4 * st);
+            const double a = ldnt(f), b = ldnt(f + st), sh = ldnt(f + 2 * st), pxx = ldnt(f + 3 * st),
+                         pzz = ldnt(f + 4 * st);
+            const double p = ldnt(f + 5 * st), q = ldnt(f + 6 * st), ru = ldnt(f + 7 * st), pxz = ldnt(f + 8 * st);
+            S[0] += lxx * a;
+            S[1] += lxx * b;
+            S[2] += lxx * sh;
+            S[3] += lzz * a;
+            S[4] += lzz * b;
+            S[5] += lzz * sh;
+            S[6] += lxx * pxx;
+            S[7] += lzz * pzz;
+            S[8] += lxz * p;
+            S[9] += lxz * q;
+            S[10] += lxz * ru;
+            S[11] += lxz * pxz;
+            S[12] += lvx * (double)ldnt(f + 9 * st);
+            S[13] += lvx * (double)ldnt(f + 10 * st);
+            S[14] += lvz * (double)ldnt(f + 11 * st);
+            S[15] += lvz * (double)ldnt(f + 12 * st);
+        }
+        const double dt = (double)A.dt, gx = (double)A.igx[c], gz = (double)A.igz[c], g = (double)A.igxz[c];
+        double *G = B.grad + c;
+        G[0] += dt * gx * S[0];
+        G[st] += dt * (gx * S[1] + gz * S[3]);
+        G[2 * st] += dt * (gx * S[2] + g * S[8]);
+        G[3 * st] += dt * gz * S[4];
+        G[4 * st] += dt * (gz * S[5] + g * S[9]);
+        G[5 * st] += dt * g * S[10];
+        G[7 * st] += S[6];
+        G[8 * st] += S[7];
+        G[9 * st] += S[11];
+        for (int q = 0; q < 4; ++q) G[(10 + q) * st] += S[12 + q];
+    }
+}
+
+// d/drho from the density sums: rho enters the face averages of vx
+// (rho(i-1), rho(i); rho(j-1), rho(j)) and vz (rho(i), rho(i+1); rho(j),
+// rho(j+1)) as d1 / mean, so d v'/d mean = -dt d1 / mean^2
+__device__ __forceinline__ double mean_d(double a, double b, int code, bool first)
+{
+    switch (code) {
+    case 1: { const double s = a + b, o = first ? b : a; return 2.0 * o * o / (s * s); }
+    case 2: return 0.5 * sqrt(first ? b / a : a / b);
+    case 3: return 0.5;
+    default: return first ? 1.0 : 0.0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ double mean_v(double a, double b, int code)
+{
+    switch (code) {
+    case 1: return 2.0 * a * b / (a + b);
+    case 2: return sqrt(a * b);
+    case 3: return 0.5 * (a + b);
+    default: return a;
+    }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void el2_rho_grad_tile(const El2AdjArgs<T> &B, int tx, int band)
+{
+    const El2Args<T> &A = B.F;
+    const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
+    if (i < 1 || i > A.nx) return;
+    const int j0 = (band * kTile2dWaves + (int)threadIdx.y) * R + 1;
+    const int64_t st = A.stride, pz = A.pitch;
+    const int code = A.density_code;
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j > A.nz) break;
+        const int64_t c = A.xoff + (int64_t)(i - 1) + pz * (j - 1);
+        const double *X = B.grad + 10 * st;
+        const double rc = (double)A.rho[c];
+        double g = 0.0;
+        auto term = [&](const double *x, int64_t cc, int ii, int jj, int64_t other, bool first) {
+            if (ii < 1 || ii > A.nx || jj < 1 || jj > A.nz) return;
+            const double sum = x[cc];
+            if (sum == 0.0) return;
+            const double a = (double)A.rho[cc], b = (double)A.rho[other];
+            const double m = mean_v<T>(a, b, code);
+            g -= sum / (m * m) * mean_d(a, b, code, first);
+        };
+        // vx at c uses (rho(c), rho(c-1)) and (rho(c), rho(c-pz)); vx at c+1 and c+pz use rho(c) second
+        if (i >= 2) term(X, c, i, j, c - 1, true);
+        if (i + 1 <= A.nx) term(X, c + 1, i + 1, j, c, false);
+        if (j >= 2) term(X + st, c, i, j, c - pz, true);
+        if (j + 1 <= A.nz) term(X + st, c + pz, i, j + 1, c, false);
+        // vz at c uses (rho(c), rho(c+1)) and (rho(c), rho(c+pz)); vz at c-1 and c-pz use rho(c) second
+        if (i + 1 <= A.nx) term(X + 2 * st, c, i, j, c + 1, true);
+        if (i >= 2) term(X + 2 * st, c - 1, i - 1, j, c, false);
+        if (j + 1 <= A.nz) term(X + 3 * st, c, i, j, c + pz, true);
+        if (j >= 2) term(X + 3 * st, c - pz, i, j - 1, c, false);
+        (void)rc;
+        B.grad[6 * st + c] += (double)A.dt * g;
+    }
+}
+
+template <typename T, int R, int PASS>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_adj_kernel(El2AdjArgs<T> B, int t, int first)
+{
+    el2_adj_tile<T, R, PASS>(B, blockIdx.x, blockIdx.y, t, first);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_correlate_kernel(El2AdjArgs<T> B, int n)
+{
+    el2_correlate_tile<T, R>(B, blockIdx.x, blockIdx.y, n);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_rho_grad_kernel(El2AdjArgs<T> B)
+{
+    el2_rho_grad_tile<T, R>(B, blockIdx.x, blockIdx.y);
+}
+
+template <typename T, int R, int PASS>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_adj_batch_kernel(const El2AdjArgs<T> *args,
+                                                                          const int4 *tiles, int t, int first)
+{
+    const int4 q = tiles[blockIdx.x];
+    if (q.x < 0) return;
+    el2_adj_tile<T, R, PASS>(args[q.x], q.y, q.z, t, first);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_correlate_batch_kernel(const El2AdjArgs<T> *args,
+                                                                                const int4 *tiles, int n)
+{
+    const int4 q = tiles[blockIdx.x];
+    if (q.x < 0) return;
+    el2_correlate_tile<T, R>(args[q.x], q.y, q.z, n);
+}
+
+template <typename T, int R>
+__global__ void __launch_bounds__(64 * kTile2dWaves) el2_rho_grad_batch_kernel(const El2AdjArgs<T> *args,
+                                                                               const int4 *tiles)
+{
+    const int4 q = tiles[blockIdx.x];
+    if (q.x < 0) return;
+    el2_rho_grad_tile<T, R>(args[q.x], q.y, q.z);
+}
+
+template <typename T>
+static El2AdjArgs<T> el2_adj_args(Device &d)
+{
+    El2AdjArgs<T> B;
+    B.F = el2_args<T>(d);
+    T **l = (T **)d.adj.lam;
+    B.lvx = l[0];
+    B.lvz = l[1];
+    B.lxx = l[2];
+    B.lzz = l[3];
+    B.lxz = l[4];
+    for (int q = 0; q < 12; ++q) B.lm[q] = l[5 + q];
+    for (int q = 0; q < 8; ++q) B.sx[q] = l[17 + q];
+    for (int q = 0; q < 4; ++q) B.vt[q] = l[25 + q];
+    B.adj = (T *)d.adj.seg_adj;
+    B.src_lam = d.adj.src_lam;
+    B.grad = d.adj.grad;
+    B.src_cell2 = cell_index(d, d.isrc + 1, 1, d.ksrc + 1);
+    return B;
+}
+
 int el2_adj_setup(Device &d)
 {
-    (void)d;
-    return SDT_ERR_ARG;
+    d.adj.nlam = 29;
+    for (int q = 0; q < d.adj.nlam; ++q)
+        if (!d.adj.lam[q] && alloc_grid_zeroed(d, &d.adj.lam[q]) != SDT_OK) return SDT_ERR_HIP;
+    return SDT_OK;
+}
+
+template <typename T>
+static void el2_adj_step_t(Device &d, int t)
+{
+    constexpr int R = kSingleRows2d;
+    const El2AdjArgs<T> B = el2_adj_args<T>(d);
+    void *lam_field[3] = {B.lvx, nullptr, B.lvz};
+    launch_adj_tail(d, t, lam_field);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((el2_adj_kernel<T, R, 0>), grid, block, 0, d.stream, B, t, d.adj.store_first);
+    hipLaunchKernelGGL((el2_adj_kernel<T, R, 1>), grid, block, 0, d.stream, B, t, d.adj.store_first);
 }
 
 void el2_adj_step(Device &d, int t)
 {
-    (void)d;
-    (void)t;
+    if (d.precision == SDT_FP64) el2_adj_step_t<double>(d, t);
+    else el2_adj_step_t<float>(d, t);
+}
+
+template <typename T>
+static void el2_adj_correlate_t(Device &d, int n)
+{
+    constexpr int R = kSingleRows2d;
+    const El2AdjArgs<T> B = el2_adj_args<T>(d);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((el2_correlate_kernel<T, R>), grid, block, 0, d.stream, B, n);
 }
 
 void el2_adj_correlate(Device &d, int first, int n)
 {
-    (void)d;
     (void)first;
-    (void)n;
+    if (d.precision == SDT_FP64) el2_adj_correlate_t<double>(d, n);
+    else el2_adj_correlate_t<float>(d, n);
+}
+
+template <typename T>
+static void el2_adj_finish_t(Device &d)
+{
+    constexpr int R = kSingleRows2d;
+    const El2AdjArgs<T> B = el2_adj_args<T>(d);
+    const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
+    hipLaunchKernelGGL((el2_rho_grad_kernel<T, R>), grid, block, 0, d.stream, B);
+}
+
+void el2_adj_finish(Device &d)
+{
+    if (d.precision == SDT_FP64) el2_adj_finish_t<double>(d);
+    else el2_adj_finish_t<float>(d);
+}
+
+template <typename T>
+static int el2_batch_adj_args_t(Batch &b, Device *const *shots, int n)
+{
+    std::vector<El2AdjArgs<T>> a((size_t)n);
+    std::vector<AdjTailArgs> tails((size_t)n);
+    for (int s = 0; s < n; ++s) {
+        a[(size_t)s] = el2_adj_args<T>(*shots[s]);
+        AdjTailArgs &t = tails[(size_t)s];
+        t.lam[0] = a[(size_t)s].lvx;
+        t.lam[1] = nullptr;
+        t.lam[2] = a[(size_t)s].lvz;
+        t.rec_cell = shots[s]->rec_cell;
+        t.tg = shots[s]->adj.trace_grad;
+        t.nrec = shots[s]->nrec;
+    }
+    if (!b.adj_args && hipMalloc(&b.adj_args, a.size() * sizeof(El2AdjArgs<T>)) != hipSuccess) return SDT_ERR_HIP;
+    if (!b.adj_tails && hipMalloc((void **)&b.adj_tails, tails.size() * sizeof(AdjTailArgs)) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (hipMemcpy(b.adj_args, a.data(), a.size() * sizeof(El2AdjArgs<T>), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (hipMemcpy(b.adj_tails, tails.data(), tails.size() * sizeof(AdjTailArgs), hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    return SDT_OK;
 }
 
 int el2_batch_adj_args(Batch &b, Device *const *shots, int n)
 {
-    (void)b;
-    (void)shots;
-    (void)n;
-    return SDT_ERR_ARG;
+    return b.precision == SDT_FP64 ? el2_batch_adj_args_t<double>(b, shots, n)
+                                   : el2_batch_adj_args_t<float>(b, shots, n);
+}
+
+// what: 0 adjoint step, 1 correlation, 2 density gradient
+template <typename T, int R>
+static void el2_batch_adj_launch(Batch &b, int which, int what, int t, int first, int n)
+{
+    const El2AdjArgs<T> *a = (const El2AdjArgs<T> *)b.adj_args;
+    const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
+    if (what == 0) {
+        hipLaunchKernelGGL((el2_adj_batch_kernel<T, R, 0>), grid, block, 0, b.stream, a, b.tiles[which], t, first);
+        hipLaunchKernelGGL((el2_adj_batch_kernel<T, R, 1>), grid, block, 0, b.stream, a, b.tiles[which], t, first);
+    } else if (what == 1) {
+        hipLaunchKernelGGL((el2_correlate_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which], n);
+    } else {
+        hipLaunchKernelGGL((el2_rho_grad_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
+    }
+}
+
+template <typename T>
+static void el2_batch_adj_p(Batch &b, int which, int what, int t, int first, int n)
+{
+    switch (kBatchShapes[which].rows) {
+    case 1: el2_batch_adj_launch<T, 1>(b, which, what, t, first, n); break;
+    case 8: el2_batch_adj_launch<T, 8>(b, which, what, t, first, n); break;
+    default: el2_batch_adj_launch<T, 4>(b, which, what, t, first, n); break;
+    }
 }
 
 void el2_batch_adj_step(Batch &b, int which, int t, int first)
 {
-    (void)b;
-    (void)which;
-    (void)t;
-    (void)first;
+    if (b.precision == SDT_FP64) el2_batch_adj_p<double>(b, which, 0, t, first, 0);
+    else el2_batch_adj_p<float>(b, which, 0, t, first, 0);
 }
 
 void el2_batch_adj_correlate(Batch &b, int which, int n)
 {
-    (void)b;
-    (void)which;
-    (void)n;
+    if (b.precision == SDT_FP64) el2_batch_adj_p<double>(b, which, 1, 0, 0, n);
+    else el2_batch_adj_p<float>(b, which, 1, 0, 0, n);
+}
+
+void el2_batch_adj_finish(Batch &b, int which)
+{
+    if (b.precision == SDT_FP64) el2_batch_adj_p<double>(b, which, 2, 0, 0, 0);
+    else el2_batch_adj_p<float>(b, which, 2, 0, 0, 0);
 }
 
 void el2_step(Device &d, bool timed)
@@ -499,16 +1071,23 @@ int el2_setup(Device &d, const double *const *in)
     const int ids[] = {SDT_C11, SDT_C13, SDT_C15, SDT_C33, SDT_C35, SDT_C55, SDT_RHO};
     d.nmat = 0;
     std::vector<double> zero((size_t)n, 0.0);
-    for (int id : ids) {
-        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
-        if (rc) return rc;
-    }
-    const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Z, SDT_GAMMA_XZ};
-    std::vector<double> inv((size_t)n);
-    for (int id : gids) {
-        for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
-        rc = upload_grid(d, inv.data(), &d.mat[d.nmat++]);
-        if (rc) return rc;
+    if (d.coef_in) {   // the ten coefficients given directly
+        for (int q = 0; q < 10; ++q) {
+            rc = upload_grid(d, d.coef_in + (size_t)q * n, &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
+    } else {
+        for (int id : ids) {
+            rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
+        const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Z, SDT_GAMMA_XZ};
+        std::vector<double> inv((size_t)n);
+        for (int id : gids) {
+            for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
+            rc = upload_grid(d, inv.data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
     }
     d.nmem = 12;
     for (int q = 0; q < d.nmem; ++q) {
@@ -546,6 +1125,17 @@ int el2_setup(Device &d, const double *const *in)
     t.rec_field[0] = SDT_F_VX;
     t.rec_field[1] = -1;
     t.rec_field[2] = SDT_F_VZ;
+
+    // adjoint layout: checkpoint = fields + memory variables; 10 coefficients
+    // (+ 4 density sums); per step kEl2Fwd forward and kEl2Adj adjoint values
+    Adjoint &a = d.adj;
+    a.nstate = 0;
+    for (int f : fids) a.state[a.nstate++] = d.field[f];
+    for (int q = 0; q < d.nmem; ++q) a.state[a.nstate++] = d.mem[q];
+    a.ncoef = 10;
+    a.ngrad = kEl2Acc;
+    a.nfwd = kEl2Fwd;
+    a.nadj = kEl2Adj;
     return upload_cpml_full(d, in);
 }
 

@@ -741,6 +741,7 @@ int em2_setup(Device &d, const double *const *in)
     for (int f : fids) a.state[a.nstate++] = d.field[f];
     for (int q = 0; q < d.nmem; ++q) a.state[a.nstate++] = d.mem[q];
     a.ncoef = 7;
+    a.ngrad = 7;
     a.nfwd = 4;
     a.nadj = 2;
     return upload_cpml_full(d, in);

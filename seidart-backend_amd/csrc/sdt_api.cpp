@@ -645,7 +645,7 @@ static int adjoint_alloc(sdt_handle *h)
     a.seg_fwd = p;
     if (alloc_grid(d, &p, (size_t)a.every * a.nadj * cell) != SDT_OK) return SDT_ERR_HIP;
     a.seg_adj = p;
-    if (alloc_grid(d, &p, (size_t)a.ncoef * d.ncell_dev * sizeof(double)) != SDT_OK) return SDT_ERR_HIP;
+    if (alloc_grid(d, &p, (size_t)a.ngrad * d.ncell_dev * sizeof(double)) != SDT_OK) return SDT_ERR_HIP;
     a.grad = (double *)p;
     if (hipMalloc((void **)&a.src_lam, (size_t)d.time_steps * kAdjSrcSlots * sizeof(double)) != hipSuccess)
         return SDT_ERR_HIP;
@@ -662,7 +662,7 @@ static int adjoint_reset(sdt_handle *h, const double *trace_grad)
     Adjoint &a = d.adj;
     const size_t cell = (size_t)d.ncell_dev * d.esize;
     for (int q = 0; q < a.nlam; ++q) HIPCHK(h, hipMemsetAsync(a.lam[q], 0, cell, d.stream));
-    HIPCHK(h, hipMemsetAsync(a.grad, 0, (size_t)a.ncoef * d.ncell_dev * sizeof(double), d.stream));
+    HIPCHK(h, hipMemsetAsync(a.grad, 0, (size_t)a.ngrad * d.ncell_dev * sizeof(double), d.stream));
     HIPCHK(h, hipMemsetAsync(a.src_lam, 0, (size_t)d.time_steps * kAdjSrcSlots * sizeof(double), d.stream));
     if (d.nrec > 0)
         HIPCHK(h, hipMemcpyAsync(a.trace_grad, trace_grad, (size_t)d.time_steps * d.nrec * 3 * sizeof(double),
@@ -742,6 +742,7 @@ int32_t sdt_adjoint(sdt_handle *h, const double *trace_grad, int64_t tcount, dou
         adjoint_correlate(d, first, n);
         HIPCHK(h, hipGetLastError());
     }
+    if (d.physics == SDT_ELASTIC2) el2_adj_finish(d);
     checkpoint_copy(d, a.nckpt - 1, false, d.stream);   // back to the forward run's final state
     set_step_counter(d.stream, d.d_it, d.time_steps + 1);
     HIPCHK(h, hipGetLastError());
@@ -993,6 +994,7 @@ int32_t sdt_batch_adjoint(sdt_batch *b, const double *const *trace_grad, double 
         batch_adj_correlate(B, w, len);
         if ((e = hipGetLastError()) != hipSuccess) return bfail(b, SDT_ERR_HIP, hipGetErrorString(e));
     }
+    if (B.physics == SDT_ELASTIC2) el2_batch_adj_finish(B, w);
     for (int q = 0; q < n; ++q) {   // back to the forward run's final states
         Device &d = *devs[(size_t)q];
         checkpoint_copy(d, d.adj.nckpt - 1, false, B.stream);

@@ -58,7 +58,7 @@ struct TailArgs {
 // adjoint fields back through the segment (storing the adjoint values the
 // gradient needs in seg_adj), then correlates the two per cell into the FP64
 // gradient accumulators.
-constexpr int kMaxAdjArrays = 24;
+constexpr int kMaxAdjArrays = 32;
 constexpr int kAdjSrcSlots = 8;        // adjoint values recorded per step at the source cells
 struct Adjoint {
     int every = 0;                     // SDT_OPT_CHECKPOINT (0 = no checkpoints)
@@ -72,7 +72,8 @@ struct Adjoint {
     void *lam[kMaxAdjArrays] = {};     // adjoint fields, memory variables, transfer arrays
     int nlam = 0;
     int ncoef = 0;                     // update coefficients per cell
-    double *grad = nullptr;            // [ncoef][ncell_dev]
+    int ngrad = 0;                     // FP64 accumulator arrays (>= ncoef; the first ncoef are the gradient)
+    double *grad = nullptr;            // [ngrad][ncell_dev]
     double *src_lam = nullptr;         // [time_steps][kAdjSrcSlots] adjoint values at the source cells
     double *trace_grad = nullptr;      // [time_steps][nrec][3]
     bool store = false;                // forward kernels write seg_fwd
@@ -196,6 +197,7 @@ void em2_adj_correlate(Device &d, int first, int n);
 int el2_adj_setup(Device &d);
 void el2_adj_step(Device &d, int t);
 void el2_adj_correlate(Device &d, int first, int n);
+void el2_adj_finish(Device &d);   // density gradient from its accumulated sums
 
 // Many 2D shots advanced together: one launch per pass covers the tiles of
 // every shot (tile list entries name the shot), one tail launch covers every
@@ -253,6 +255,7 @@ void em2_batch_adj_correlate(Batch &b, int which, int n);
 int el2_batch_adj_args(Batch &b, Device *const *shots, int n);
 void el2_batch_adj_step(Batch &b, int which, int t, int first);
 void el2_batch_adj_correlate(Batch &b, int which, int n);
+void el2_batch_adj_finish(Batch &b, int which);
 void launch_adj_tail_batch(Batch &b, int t);
 
 // common pieces (sdt_common.hip)

@@ -11,7 +11,8 @@ through the coefficient formulas below (FP64, differentiable).
 Coefficient order: see `EM2_COEFFS` / `EL2_COEFFS`.  A shot is a dict with
 keys shape, spacing, cpml (the padding width), dt, src (1-based unpadded
 (x, z)), receivers ((n, 3) 1-based unpadded), cpml_arrays (the kappa / alpha
-/ acoef / bcoef arrays and their _half forms).
+/ acoef / bcoef arrays and their _half forms) and, for the elastic solver,
+optionally density_method and stencil.
 """
 from __future__ import annotations
 
@@ -26,8 +27,9 @@ from .solver import Batch, Solver
 EPS0 = 8.85418782e-12                        # constants.f08:11
 EM2_COEFFS = ("aEx", "bEx", "bEz", "dEx", "eEx", "dEz", "eEz")
 EL2_COEFFS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "igx", "igz", "igxz")
-_SRC = {"em2": ("x", "z")}   # source components the raw series feed
-_TRACE = {"em2": (0, 2)}      # recorded components kept: (Ex, Ez)
+# source components the raw series feed, in the order of the src rows
+_SRC = {"em2": ("x", "z"), "elastic2": ("xx", "xz", "zz", "x", "z")}
+_TRACE = {"em2": (0, 2), "elastic2": (0, 2)}   # recorded components kept: (Ex, Ez) / (vx, vz)
 
 
 def em2_coefficients(eps, sig):
@@ -44,6 +46,14 @@ def em2_coefficients(eps, sig):
                         (e11 * s33 - e13 * s13) / det])
 
 
+def el2_coefficients(stiffness, rho, gamma, dt):
+    """(10, nx, nz) update coefficients of the 2D elastic solver from the
+    stiffness set (c11, c13, c15, c33, c35, c55), density and attenuation
+    (gamma_x, gamma_z, gamma_xz): the first seven as given, then
+    1/(1 + gamma dt) (cpmlfdtd.f08:344-353, 382-386)."""
+    return torch.stack(list(stiffness) + [rho] + [1.0 / (1.0 + g * dt) for g in gamma])
+
+
 def em2_source_scale(eps11_at_src, dt):
     """The library's default EM source scaling dt / (eps11 eps0) at the
     source cell (cpmlfdtd.f08:2678-2681); multiply a wavelet by it to get the
@@ -53,7 +63,8 @@ def em2_source_scale(eps11_at_src, dt):
 
 def _make_solver(physics, shot, precision, checkpoint, coef, src):
     s = Solver(physics, shot["shape"], shot["spacing"], shot["cpml"], shot["dt"], src.shape[-1], shot["src"],
-               precision=precision)
+               precision=precision, density_method=shot.get("density_method", "arithmetic"),
+               stencil=shot.get("stencil", "reference"))
     try:
         s.set_option("checkpoint", checkpoint)
         s.set_option("raw_source", 1)
@@ -108,8 +119,10 @@ class _Traces(torch.autograd.Function):
 def traces(physics, coef, src, shot, *, precision="fp32", checkpoint=100):
     """Differentiable receiver traces (steps, nrec, 2) of one shot.
 
-    coef: (ncoef, nx, nz) float64 tensor; src: (2, steps) float64 tensor of
-    raw series (EM: added to Ex, Ez at the source cell)."""
+    coef: (ncoef, nx, nz) float64 tensor; src: float64 tensor of raw series,
+    (2, steps) for "em2" (added to Ex, Ez at the source cell) or (5, steps)
+    for "elastic2" (added to sigma_xx, sigma_xz at (+1, +1), sigma_zz, vx,
+    vz; the library's default scaling divides each by the density there)."""
     return _Traces.apply(coef, src, physics, shot, precision, int(checkpoint))
 
 
@@ -180,5 +193,5 @@ def smooth(field, passes=10):
     return x[0, 0]
 
 
-__all__ = ["EM2_COEFFS", "EL2_COEFFS", "em2_coefficients", "em2_source_scale", "traces", "batch_traces",
-           "l2_misfit", "smooth"]
+__all__ = ["EM2_COEFFS", "EL2_COEFFS", "em2_coefficients", "el2_coefficients", "em2_source_scale", "traces",
+           "batch_traces", "l2_misfit", "smooth"]

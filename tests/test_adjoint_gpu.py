@@ -159,3 +159,131 @@ def test_em2_batch_adjoint_equals_single_shots(precision):
     err = rel_l2(c.grad.numpy(), gsum.numpy())
     print("batch vs single-shot gradient rel L2:", f"{err:.2e}", flush=True)
     assert err <= 1e-12
+
+
+# ---- 2D elastic --------------------------------------------------------------
+def _el2_case(steps=140, density="arithmetic", stencil="reference", gamma=0.0, seed=3):
+    from oracle.adjoint_torch import el2_cfg_inputs
+    from seidart_hip import models
+    cfg = models.elastic2(nx=44, nz=38, steps=steps, src=(22, 12), nrec=7, seed=seed, nlayers=3,
+                          density_method=density, gamma=gamma, tilt=(0.0, 25.0))
+    cfg["stencil"] = stencil
+    coef, src, A, dt, dx, dz, sij, rij = el2_cfg_inputs(cfg)
+    src = np.random.default_rng(seed).standard_normal(src.shape) * np.abs(src).max()
+    return cfg, coef, src, (A, dt, dx, dz, sij, rij)
+
+
+def _gpu_grad_el2(cfg, coef, src, w, precision, checkpoint):
+    from seidart_hip import fwi
+    c = torch.tensor(coef, requires_grad=True)
+    s = torch.tensor(src, requires_grad=True)
+    shot = _shot(cfg)
+    shot["density_method"] = cfg.get("density_method", "arithmetic")
+    shot["stencil"] = cfg.get("stencil", "reference")
+    tr = fwi.traces("elastic2", c, s, shot, precision=precision, checkpoint=checkpoint)
+    (tr * torch.tensor(w)).sum().backward()
+    return tr.detach().numpy(), c.grad.numpy(), s.grad.numpy()
+
+
+@pytest.mark.parametrize("density,stencil,gamma", [("arithmetic", "reference", 0.0),
+                                                   ("harmonic", "corrected", 3.0),
+                                                   ("geometric", "reference", 1.0)])
+def test_el2_adjoint_matches_autograd(density, stencil, gamma):
+    from oracle.adjoint_torch import el2_traces
+    cfg, coef, src, o = _el2_case(density=density, stencil=stencil, gamma=gamma)
+    print(f"[adjoint] elastic2 {density} {stencil} gamma={gamma} shape={cfg['shape']}", flush=True)
+    w = np.random.default_rng(0).standard_normal((cfg["time_steps"], len(cfg["receivers"]), 2))
+    tr, gc, gs = _gpu_grad_el2(cfg, coef, src, w, "fp64", 45)
+    c2 = torch.tensor(coef, requires_grad=True)
+    s2 = torch.tensor(src, requires_grad=True)
+    code = {"harmonic": 1, "geometric": 2, "arithmetic": 3}[density]
+    tr2 = el2_traces(c2, s2, *o, density_code=code, corrected=stencil == "corrected")
+    (tr2 * torch.tensor(w)).sum().backward()
+    assert rel_l2(tr, tr2.detach().numpy()) <= 1e-10
+    errs = [rel_l2(gc[q], c2.grad.numpy()[q]) for q in range(10)]
+    print("coefficient gradients rel L2:", [f"{e:.2e}" for e in errs], "source:",
+          f"{rel_l2(gs, s2.grad.numpy()):.2e}", flush=True)
+    assert max(errs) <= 1e-9
+    assert rel_l2(gs, s2.grad.numpy()) <= 1e-9
+
+
+def test_el2_adjoint_fp32_close_to_fp64():
+    cfg, coef, src, _ = _el2_case(steps=200, gamma=1.0)
+    w = np.random.default_rng(2).standard_normal((200, len(cfg["receivers"]), 2))
+    _, g64, s64 = _gpu_grad_el2(cfg, coef, src, w, "fp64", 50)
+    _, g32, s32 = _gpu_grad_el2(cfg, coef, src, w, "fp32", 50)
+    errs = [rel_l2(g32[q], g64[q]) for q in range(10)]
+    print("fp32 vs fp64 coefficient gradients rel L2:", [f"{e:.2e}" for e in errs], flush=True)
+    assert max(errs) <= 1e-3
+    assert rel_l2(s32, s64) <= 1e-4
+
+
+def test_el2_gradient_matches_finite_differences():
+    """dJ/dc11 and dJ/drho at interior cells, J the L2 misfit against traces
+    of a model with a stiffness and density bump; central differences with a
+    1e-4 relative step in FP64."""
+    from seidart_hip import fwi
+    cfg, coef, src, _ = _el2_case(steps=160, seed=6)
+    shot = _shot(cfg)
+    src_t = torch.tensor(src)
+    cp = cfg["cpml"]
+    true = coef.copy()
+    true[0, cp + 14:cp + 24, cp + 10:cp + 18] *= 1.1
+    true[6, cp + 14:cp + 24, cp + 10:cp + 18] *= 0.9
+    with torch.no_grad():
+        obs = fwi.traces("elastic2", torch.tensor(true), src_t, shot, precision="fp64", checkpoint=40)
+
+    def J(c):
+        return fwi.l2_misfit(fwi.traces("elastic2", c, src_t, shot, precision="fp64", checkpoint=40), obs)
+
+    c = torch.tensor(coef, requires_grad=True)
+    J(c).backward()
+    rng = np.random.default_rng(5)
+    cells = [(int(rng.integers(cp + 3, cp + 40)), int(rng.integers(cp + 3, cp + 34))) for _ in range(8)]
+    fd, ad = [], []
+    for q in (0, 6):
+        for (i, j) in cells:
+            h = 1e-4 * abs(coef[q, i, j])
+            vals = []
+            for sign in (1.0, -1.0):
+                p = coef.copy()
+                p[q, i, j] += sign * h
+                with torch.no_grad():
+                    vals.append(float(J(torch.tensor(p))))
+            fd.append((vals[0] - vals[1]) / (2 * h))
+            ad.append(float(c.grad[q, i, j]))
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
+    print("finite differences vs adjoint rel L2:", f"{err:.2e}", flush=True)
+    assert err <= 1e-3
+
+
+def test_el2_batch_adjoint_equals_single_shots():
+    from seidart_hip import fwi
+    cfg, coef, _, _ = _el2_case(steps=120)
+    base = _shot(cfg)
+    shots = []
+    for q, sx in enumerate((8, 22, 36)):
+        sh = dict(base)
+        sh["src"] = (sx, 8 + 5 * q)
+        shots.append(sh)
+    rng = np.random.default_rng(7)
+    nt = cfg["time_steps"]
+    src = torch.tensor(rng.standard_normal((3, 5, nt)) * 1e-3)
+    w = torch.tensor(rng.standard_normal((3, nt, len(cfg["receivers"]), 2)))
+    c = torch.tensor(coef, requires_grad=True)
+    s = src.clone().requires_grad_(True)
+    tb = fwi.batch_traces("elastic2", c, s, shots, precision="fp64", checkpoint=30)
+    (tb * w).sum().backward()
+    gsum = torch.zeros_like(c)
+    for q in range(3):
+        c1 = torch.tensor(coef, requires_grad=True)
+        s1 = src[q].clone().requires_grad_(True)
+        t1 = fwi.traces("elastic2", c1, s1, shots[q], precision="fp64", checkpoint=30)
+        (t1 * w[q]).sum().backward()
+        assert rel_l2(tb[q].detach().numpy(), t1.detach().numpy()) <= 1e-13
+        assert rel_l2(s.grad[q].numpy(), s1.grad.numpy()) <= 1e-12
+        gsum += c1.grad
+    err = rel_l2(c.grad.numpy(), gsum.numpy())
+    print("batch vs single-shot gradient rel L2:", f"{err:.2e}", flush=True)
+    assert err <= 1e-12
