@@ -287,6 +287,150 @@ def bench_batch2d(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_fwi2d(args):
+    """FWI gradient throughput: 600 x 300 (+10-cell CPML), 16 shots, 3000
+    steps, seeded 6-layer anisotropic true model with 10 Gaussian anomalies
+    (+-15% permittivity / stiffness), starting model = smoothed true model,
+    L2 trace misfit against FP64 data of the true model; checkpoints every
+    100 steps.  Timed: the batched FP32 forward + adjoint of all shots
+    (seidart_hip.fwi.batch_traces + backward) against the shots run one
+    after another with FP64 forward + adjoint.  Checked: the FP32 gradient
+    against the FP64 one, and the FP64 gradient against central differences
+    (relative step 1e-4) at --fd-cells random interior cells."""
+    import numpy as np
+    import torch
+    from seidart_hip import fwi, models
+
+    def rel(a, b):
+        return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+    out = {}
+    for physics in ("em2", "elastic2") if args.fwi_physics == "both" else (args.fwi_physics,):
+        t0 = time.perf_counter()
+        prob = models.fwi2d(physics, steps=args.fwi_steps, nshots=args.shots if args.shots != 64 else 16)
+        shots = prob["shots"]
+        ns, nt = len(shots), args.fwi_steps
+        dt = prob["cfg"]["dt"]
+        cp = prob["cfg"]["cpml"]
+        if physics == "em2":
+            names = ("e11", "e13", "e33", "s11", "s13", "s33")
+
+            def coefs(p):
+                return fwi.em2_coefficients(p[:3], p[3:])
+            # Ex dipole, scaled as the library does by default, from the starting model
+            src1 = np.stack([prob["wavelet"], np.zeros(nt)])
+            src = np.stack([src1 * fwi.em2_source_scale(prob["start"]["e11"][s["src"][0] + cp - 1,
+                                                                             s["src"][1] + cp - 1], dt)
+                            for s in shots])
+            fd_param = 0
+        else:
+            names = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "gamma_x", "gamma_z", "gamma_xz")
+
+            def coefs(p):
+                return fwi.el2_coefficients(p[:6], p[6], p[7:], dt)
+            # vertical force, scaled by 1/rho at the source as the library does by default
+            src = np.zeros((ns, 5, nt))
+            for q, s in enumerate(shots):
+                src[q, 4] = prob["wavelet"] / prob["start"]["rho"][s["src"][0] + cp - 1, s["src"][1] + cp - 1]
+            fd_param = 3   # c33
+        src_t = torch.tensor(src)
+        true = [torch.tensor(prob["true"][n]) for n in names]
+        start = [torch.tensor(prob["start"][n]) for n in names]
+        with torch.no_grad():
+            obs = fwi.batch_traces(physics, coefs(true), src_t, shots, precision="fp64")
+        t_setup = time.perf_counter() - t0
+        print(f"[fwi2d] {physics}: {ns} shots, {prob['cfg']['shape']}, {nt} steps, data {t_setup:.1f} s", flush=True)
+
+        # batched FP32 gradient
+        p32 = [x.clone().requires_grad_(True) for x in start]
+        torch.cuda.synchronize() if torch.cuda.is_available() else None
+        t0 = time.perf_counter()
+        J32 = fwi.l2_misfit(fwi.batch_traces(physics, coefs(p32), src_t, shots, precision="fp32",
+                                             checkpoint=args.checkpoint), obs)
+        t_fwd32 = time.perf_counter() - t0
+        J32.backward()
+        t32 = time.perf_counter() - t0
+        g32 = np.stack([x.grad.numpy() for x in p32])
+        print(f"[fwi2d] {physics}: batched fp32 forward {t_fwd32:.3f} s, forward+adjoint {t32:.3f} s", flush=True)
+
+        # serial FP64 forward + adjoint
+        p64 = [x.clone().requires_grad_(True) for x in start]
+        t0 = time.perf_counter()
+        J64 = 0.0
+        for q in range(ns if args.fwi_serial else 0):
+            Jq = fwi.l2_misfit(fwi.traces(physics, coefs(p64), src_t[q], shots[q], precision="fp64",
+                                          checkpoint=args.checkpoint), obs[q])
+            Jq.backward()
+            J64 += float(Jq.detach())
+        t64 = time.perf_counter() - t0
+        if not args.fwi_serial:
+            print(f"[fwi2d] {physics}: batched fp32 only", flush=True)
+            continue
+        g64 = np.stack([x.grad.numpy() for x in p64])
+        print(f"[fwi2d] {physics}: serial fp64 forward+adjoint {t64:.3f} s", flush=True)
+        agree = {n: rel(g32[k], g64[k]) for k, n in enumerate(names) if np.any(g64[k])}
+        # the same with the source near field muted (7 x 7 cells around each
+        # source cell), where the FP32 wavefields' rounding concentrates
+        mute = np.ones(g64.shape[1:], bool)
+        for s in shots:
+            si, sj = s["src"][0] + cp - 1, s["src"][1] + cp - 1
+            mute[max(si - 3, 0):si + 4, max(sj - 3, 0):sj + 4] = False
+        agree_muted = {n: rel(g32[k][mute], g64[k][mute]) for k, n in enumerate(names) if np.any(g64[k])}
+
+        # central differences of the FP64 misfit at random interior cells
+        fd = None
+        if args.fd_cells > 0:
+            rng = np.random.default_rng(11)
+            NX, NZ = prob["cfg"]["shape"]
+            cells = [(int(rng.integers(cp + 5, NX - cp - 5)), int(rng.integers(cp + 5, NZ - cp - 5)))
+                     for _ in range(args.fd_cells)]
+            base = [x.detach() for x in start]
+            t0 = time.perf_counter()
+            fdv, adv, ad32 = [], [], []
+            for (i, j) in cells:
+                h = 1e-4 * abs(float(base[fd_param][i, j]))
+                vals = []
+                for sgn in (1.0, -1.0):
+                    pp = [x.clone() for x in base]
+                    pp[fd_param][i, j] += sgn * h
+                    with torch.no_grad():
+                        vals.append(float(fwi.l2_misfit(fwi.batch_traces(physics, coefs(pp), src_t, shots,
+                                                                         precision="fp64"), obs)))
+                fdv.append((vals[0] - vals[1]) / (2 * h))
+                adv.append(g64[fd_param, i, j])
+                ad32.append(g32[fd_param, i, j])
+            fdv, adv, ad32 = np.array(fdv), np.array(adv), np.array(ad32)
+            fd = {"parameter": names[fd_param], "cells": len(cells), "step": "1e-4 relative, central",
+                  "fp64_adjoint_vs_fd_rel_l2": float(f"{rel(adv, fdv):.3e}"),
+                  "fp32_adjoint_vs_fd_rel_l2": float(f"{rel(ad32, fdv):.3e}"),
+                  "s": round(time.perf_counter() - t0, 1)}
+            print(f"[fwi2d] {physics}: finite differences {fd}", flush=True)
+        out[physics] = {
+            "shots": ns, "steps": nt, "grid": list(prob["cfg"]["shape"]), "checkpoint_every": args.checkpoint,
+            "misfit": float(f"{float(J32.detach()):.6e}"), "misfit_fp64": float(f"{J64:.6e}"),
+            "batched_fp32": {"s": round(t32, 4), "forward_s": round(t_fwd32, 4), "shots_per_s": round(ns / t32, 3)},
+            "serial_fp64": {"s": round(t64, 4), "shots_per_s": round(ns / t64, 3)},
+            "speedup": round(t64 / t32, 3),
+            "fp32_vs_fp64_gradient_rel_l2": {k: float(f"{v:.3e}") for k, v in agree.items()},
+            "fp32_vs_fp64_gradient_rel_l2_source_muted": {k: float(f"{v:.3e}") for k, v in agree_muted.items()},
+            "finite_differences": fd,
+        }
+    if not args.fwi_serial:
+        return
+    total_shots = sum(v["shots"] for v in out.values())
+    total_s = sum(v["batched_fp32"]["s"] for v in out.values())
+    line = {
+        "metric": "FWI gradient shots/s (2D, FP32 batched forward+adjoint; serial FP64 beside it)",
+        "value": round(total_shots / total_s, 3), "unit": "shots/s", "n_gpus": 1, "steps": args.fwi_steps,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded 6-layer anisotropic model + 10 Gaussian anomalies)",
+        "config": {"workload": f"fwi2d: 600x300 + 10-cell CPML, {args.fwi_steps} steps, 16 shots per solver "
+                               f"kind, checkpoints every {args.checkpoint} steps", "kinds": list(out)},
+        "roofline": None, "cpu_baseline": None, "fwi": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,10 +452,17 @@ def main():
                     help="also time the per-cell-array layout and report it beside the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
-    ap.add_argument("--workload", default="seismic3", choices=["seismic3", "electromag3", "batch2d"],
+    ap.add_argument("--workload", default="seismic3", choices=["seismic3", "electromag3", "batch2d", "fwi2d"],
                     help="seismic3: the 3D elastic 256^3 case (default); electromag3: the 3D EM "
                          "384x384x192 case; batch2d: uneven multi-shot 2D batches against serial runs")
-    ap.add_argument("--shots", type=int, default=64, help="batch2d: shots per solver kind")
+    ap.add_argument("--shots", type=int, default=64, help="batch2d: shots per solver kind (fwi2d: 16)")
+    ap.add_argument("--fwi-physics", default="both", choices=["both", "em2", "elastic2"])
+    ap.add_argument("--fwi-steps", type=int, default=3000)
+    ap.add_argument("--checkpoint", type=int, default=100, help="fwi2d: steps between saved forward states")
+    ap.add_argument("--fd-cells", type=int, default=50, help="fwi2d: cells of the finite-difference check")
+    ap.add_argument("--fwi-serial", type=int, default=1,
+                    help="fwi2d: also run the serial FP64 forward+adjoint and the checks (0 = batched only, "
+                         "for profiling; prints no JSON line)")
     ap.add_argument("--batch-physics", default="both", choices=["both", "elastic2", "em2"])
     ap.add_argument("--batch-serial", type=int, default=1,
                     help="batch2d: also time the serial single-handle runs (0 = batched run only, "
@@ -336,6 +487,9 @@ def main():
         return
     if args.workload == "batch2d":
         bench_batch2d(args)
+        return
+    if args.workload == "fwi2d":
+        bench_fwi2d(args)
         return
 
     import numpy as np

@@ -259,15 +259,6 @@ void set_step_counter(hipStream_t s, int *d_it, int v)
     hipLaunchKernelGGL(set_counter_kernel, dim3(1), dim3(64), 0, s, d_it, v);
 }
 
-// an FP64 array in the device grid layout of d -> host (nx, ny, nz) order
-int download_grid_f64(Device &d, const double *dev, double *host)
-{
-    const size_t rows = (size_t)d.ny * d.nz;
-    if (hipMemcpy2D(host, (size_t)d.nx * 8, dev + d.xoff, (size_t)d.pitch * 8, (size_t)d.nx * 8, rows,
-                    hipMemcpyDeviceToHost) != hipSuccess)
-        return SDT_ERR_HIP;
-    return SDT_OK;
-}
 
 void launch_tail(Device &d)
 {
@@ -354,8 +345,7 @@ int upload_grid(Device &d, const double *host, void **dev, int ncomp)
 {
     const size_t bytes = (size_t)d.ncell_dev * d.esize * ncomp;
     if (alloc_grid(d, dev, bytes) != SDT_OK) return SDT_ERR_HIP;
-    if (hipMemset(*dev, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
-    return store_grid(d, *dev, host, ncomp);
+    return store_grid(d, *dev, host, ncomp);   // writes the padding too
 }
 
 int alloc_grid_zeroed(Device &d, void **dev, int ncomp)
@@ -366,42 +356,54 @@ int alloc_grid_zeroed(Device &d, void **dev, int ncomp)
     return SDT_OK;
 }
 
-int store_grid(Device &d, void *dev, const double *host, int ncomp)
+// Host <-> device copies of whole grid arrays go through one contiguous
+// host image of the padded layout (zero padding), converted to the device
+// precision on the host: one copy per array instead of one per row.
+template <typename T>
+static int store_grid_t(Device &d, void *dev, const double *host, int ncomp)
 {
-    const size_t rows = (size_t)d.ny * d.nz, es = d.esize * ncomp;
-    char *dst = (char *)dev + d.xoff * es;
-    if (d.precision == SDT_FP64) {
-        if (hipMemcpy2D(dst, (size_t)d.pitch * es, host, (size_t)d.nx * es, (size_t)d.nx * es, rows,
-                        hipMemcpyHostToDevice) != hipSuccess)
-            return SDT_ERR_HIP;
-    } else {
-        const size_t nv = (size_t)d.ncell * ncomp;
-        std::vector<float> tmp(nv);
-        for (size_t i = 0; i < nv; ++i) tmp[i] = (float)host[i];
-        if (hipMemcpy2D(dst, (size_t)d.pitch * es, tmp.data(), (size_t)d.nx * es, (size_t)d.nx * es, rows,
-                        hipMemcpyHostToDevice) != hipSuccess)
-            return SDT_ERR_HIP;
+    const size_t rows = (size_t)d.ny * d.nz, nx = (size_t)d.nx * ncomp, pitch = (size_t)d.pitch * ncomp;
+    const size_t off = (size_t)d.xoff * ncomp;
+    std::vector<T> img(rows * pitch, T(0));
+    for (size_t r = 0; r < rows; ++r) {
+        T *dst = img.data() + r * pitch + off;
+        const double *src = host + r * nx;
+        for (size_t i = 0; i < nx; ++i) dst[i] = (T)src[i];
+    }
+    return hipMemcpy(dev, img.data(), img.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess ? SDT_OK
+                                                                                                 : SDT_ERR_HIP;
+}
+
+template <typename T>
+static int download_grid_t(Device &d, const void *dev, double *host, int ncomp)
+{
+    const size_t rows = (size_t)d.ny * d.nz, nx = (size_t)d.nx * ncomp, pitch = (size_t)d.pitch * ncomp;
+    const size_t off = (size_t)d.xoff * ncomp;
+    std::vector<T> img(rows * pitch);
+    if (hipMemcpy(img.data(), dev, img.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return SDT_ERR_HIP;
+    for (size_t r = 0; r < rows; ++r) {
+        const T *src = img.data() + r * pitch + off;
+        double *dst = host + r * nx;
+        for (size_t i = 0; i < nx; ++i) dst[i] = (double)src[i];
     }
     return SDT_OK;
 }
 
+int store_grid(Device &d, void *dev, const double *host, int ncomp)
+{
+    return d.precision == SDT_FP64 ? store_grid_t<double>(d, dev, host, ncomp) : store_grid_t<float>(d, dev, host, ncomp);
+}
+
 int download_grid(Device &d, const void *dev, double *host, int ncomp)
 {
-    const size_t rows = (size_t)d.ny * d.nz, es = d.esize * ncomp;
-    const char *src = (const char *)dev + d.xoff * es;
-    if (d.precision == SDT_FP64) {
-        if (hipMemcpy2D(host, (size_t)d.nx * es, src, (size_t)d.pitch * es, (size_t)d.nx * es, rows,
-                        hipMemcpyDeviceToHost) != hipSuccess)
-            return SDT_ERR_HIP;
-    } else {
-        const size_t nv = (size_t)d.ncell * ncomp;
-        std::vector<float> tmp(nv);
-        if (hipMemcpy2D(tmp.data(), (size_t)d.nx * es, src, (size_t)d.pitch * es, (size_t)d.nx * es, rows,
-                        hipMemcpyDeviceToHost) != hipSuccess)
-            return SDT_ERR_HIP;
-        for (size_t i = 0; i < nv; ++i) host[i] = tmp[i];
-    }
-    return SDT_OK;
+    return d.precision == SDT_FP64 ? download_grid_t<double>(d, dev, host, ncomp)
+                                   : download_grid_t<float>(d, dev, host, ncomp);
+}
+
+// an FP64 array in the device grid layout of d -> host (nx, ny, nz) order
+int download_grid_f64(Device &d, const double *dev, double *host)
+{
+    return download_grid_t<double>(d, dev, host, 1);
 }
 
 int build_material_table(Device &d, int row_len, const std::function<void(int64_t, double *)> &fill)

@@ -123,6 +123,8 @@ def traces(physics, coef, src, shot, *, precision="fp32", checkpoint=100):
     (2, steps) for "em2" (added to Ex, Ez at the source cell) or (5, steps)
     for "elastic2" (added to sigma_xx, sigma_xz at (+1, +1), sigma_zz, vx,
     vz; the library's default scaling divides each by the density there)."""
+    if not (torch.is_grad_enabled() and (coef.requires_grad or src.requires_grad)):
+        checkpoint = 0   # no backward pass can follow: no checkpoints
     return _Traces.apply(coef, src, physics, shot, precision, int(checkpoint))
 
 
@@ -134,6 +136,7 @@ class _BatchTraces(torch.autograd.Function):
         c = coef.detach().cpu().numpy()
         sv = src.detach().cpu().numpy()
         solvers = []
+        b = None
         try:
             for q, shot in enumerate(shots):
                 solvers.append(_make_solver(physics, shot, precision, checkpoint, c, sv[q]))
@@ -141,6 +144,8 @@ class _BatchTraces(torch.autograd.Function):
             b.run()
             out = np.stack([s.receivers()[:, :, list(_TRACE[physics])] for s in solvers])
         except Exception:
+            if b is not None:
+                b.close()   # before the handles it points to
             for s in solvers:
                 s.close()
             raise
@@ -177,6 +182,8 @@ def batch_traces(physics, coef, src, shots, *, precision="fp32", checkpoint=100)
     """Differentiable traces (nshot, steps, nrec, 2) of shots that share one
     model (same grid), run as one batch; the backward pass sums the
     per-shot coefficient gradients."""
+    if not (torch.is_grad_enabled() and (coef.requires_grad or src.requires_grad)):
+        checkpoint = 0
     return _BatchTraces.apply(coef, src, physics, list(shots), precision, int(checkpoint))
 
 

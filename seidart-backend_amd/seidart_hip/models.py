@@ -434,6 +434,64 @@ def batch2d(physics="elastic2", nshots=64, seed=3, nx=(300, 800), nz=(150, 400),
     return shots
 
 
+def _gaussians(shape, cpml, n, rng, amp=0.15, sigma=(8.0, 25.0)):
+    """Sum of n seeded Gaussian bumps of relative amplitude +-amp inside the
+    unpadded grid (one multiplicative factor field, 1 + sum)."""
+    NX, NZ = shape
+    ii, jj = np.meshgrid(np.arange(NX), np.arange(NZ), indexing="ij")
+    f = np.zeros(shape)
+    for _ in range(n):
+        ci = rng.uniform(cpml + 20, NX - cpml - 20)
+        cj = rng.uniform(cpml + 20, NZ - cpml - 20)
+        s = rng.uniform(*sigma)
+        f += rng.choice([-1.0, 1.0]) * amp * np.exp(-((ii - ci) ** 2 + (jj - cj) ** 2) / (2 * s * s))
+    return 1.0 + f
+
+
+def fwi2d(physics="em2", nx=600, nz=300, steps=3000, nshots=16, nrec=100, seed=8, nlayers=6, anomalies=10,
+          smooth_passes=30):
+    """Full-waveform-inversion problem: a seeded nlayers-layer anisotropic
+    model plus `anomalies` Gaussian anomalies of +-15% in permittivity (EM)
+    or stiffness (elastic) as the true model, the smoothed true model as the
+    starting model, nshots sources along a line near the top and nrec
+    receivers on that line.  Returns a dict: cfg (the true model's
+    configuration dict), true / start (parameter arrays by name), shots (one
+    dict per shot for seidart_hip.fwi), wavelet (steps,)."""
+    rng = np.random.default_rng(seed)
+    if physics == "em2":
+        cfg = em2(nx=nx, nz=nz, steps=steps, seed=seed, nlayers=nlayers, src=(nx // 2, 10), nrec=nrec)
+        names, scaled = ("e11", "e13", "e33", "s11", "s13", "s33"), ("e11", "e13", "e33")
+        f0 = 100e6
+    else:
+        cfg = elastic2(nx=nx, nz=nz, steps=steps, seed=seed, nlayers=nlayers, src=(nx // 2, 10), rec_z=10,
+                       nrec=nrec, tilt=(0.0, 0.0))
+        names = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "gamma_x", "gamma_z", "gamma_xz")
+        scaled = ("c11", "c13", "c15", "c33", "c35", "c55")
+        f0 = 50.0
+    A = cfg["arrays"]
+    bump = _gaussians(cfg["shape"], cfg["cpml"], anomalies, rng)
+    true = {n: np.asfortranarray(np.asarray(A[n], dtype=np.float64) * (bump if n in scaled else 1.0))
+            for n in names}
+    import torch  # the smoothing uses torch's pooling; models is imported without torch elsewhere
+
+    def smooth(a):
+        x = torch.as_tensor(a, dtype=torch.float64)[None, None]
+        for _ in range(smooth_passes):
+            x = torch.nn.functional.avg_pool2d(torch.nn.functional.pad(x, (1, 1, 1, 1), mode="replicate"), 3, 1)
+        return np.asfortranarray(x[0, 0].numpy())
+
+    start = {n: smooth(v) for n, v in true.items()}
+    cp = cfg["cpml"]
+    xs = np.linspace(40, nx - 40, nshots).round().astype(int)
+    cpml_arrays = {k: v for k, v in A.items() if k.endswith("_cpml")}
+    shots = [dict(shape=cfg["shape"], spacing=cfg["spacing"], cpml=cp, dt=cfg["dt"], src=(int(x), 10),
+                  receivers=cfg["receivers"], cpml_arrays=cpml_arrays,
+                  density_method=cfg.get("density_method", "arithmetic"))
+             for x in xs]
+    return dict(physics=physics, cfg=cfg, true=true, start=start, shots=shots,
+                wavelet=ricker(f0, cfg["dt"], steps))
+
+
 # ------------------------------------------------------ reference file set ----
 _IC_PREFIX = {"elastic2": "initialconditionV", "elastic25": "initialconditionV", "elastic3": "initialconditionV",
               "em25": "initialconditionE",
