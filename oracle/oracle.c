@@ -128,6 +128,102 @@ static inline double tplus(double s_m1, double s_0, double s_p1, double s_p2, in
     return -27.0 * s_0 + 27.0 * s_p1 - s_p2;
 }
 
+/* ------------------------------------------------------------------------
+ * plane-wave source: plane_wave_source.f08:67-230, faces and reference point
+ * as set up in cpmlfdtd.f08:188-232 (2D), 802-832 (2.5D), 1548-1605 (3D)
+ * ------------------------------------------------------------------------ */
+typedef struct pw_state {
+    double p[3], e[3], r0[3], loc[3];
+    int active[6];
+    int face[3];       /* cell index of the face normal to x, y, z (0 = none) */
+} pw_state;
+
+static void pw_setup(const oracle_run *r, int three_d, pw_state *w)
+{
+    const double deg = 3.1415926535897932384626433832795 / 180.0;
+    const double cx = cos(deg * r->pw_angles[2]), cy = cos(deg * r->pw_angles[0]), cz = cos(deg * r->pw_angles[1]);
+    const double sx = sin(deg * r->pw_angles[2]), sy = sin(deg * r->pw_angles[0]), sz = sin(deg * r->pw_angles[1]);
+    /* reshape fills column by column: Rx(2,3) = sx, Rx(3,2) = -sx, ... */
+    double Rx[3][3] = {{1, 0, 0}, {0, cx, sx}, {0, -sx, cx}};
+    double Ry[3][3] = {{cy, 0, -sy}, {0, 1, 0}, {sy, 0, cy}};
+    double Rz[3][3] = {{cz, sz, 0}, {-sz, cz, 0}, {0, 0, 1}};
+    double T[3][3], R[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            T[i][j] = 0.0;
+            for (int k = 0; k < 3; ++k) T[i][j] += Rz[i][k] * Rx[k][j];
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            R[i][j] = 0.0;
+            for (int k = 0; k < 3; ++k) R[i][j] += Ry[i][k] * T[k][j];
+        }
+    double p[3] = {R[0][2], R[1][2], R[2][2]};
+    double n = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (n > 0.0) for (int i = 0; i < 3; ++i) p[i] /= n;
+    double t[3] = {R[0][0], R[1][0], R[2][0]};
+    double dp = t[0] * p[0] + t[1] * p[1] + t[2] * p[2];
+    for (int i = 0; i < 3; ++i) t[i] -= dp * p[i];
+    n = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    if (n > 0.0) {
+        for (int i = 0; i < 3; ++i) w->e[i] = t[i] / n;
+    } else {
+        if (fabs(p[0]) < 0.9) { t[0] = 1; t[1] = 0; t[2] = 0; } else { t[0] = 0; t[1] = 1; t[2] = 0; }
+        dp = t[0] * p[0] + t[1] * p[1] + t[2] * p[2];
+        for (int i = 0; i < 3; ++i) t[i] -= dp * p[i];
+        n = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        for (int i = 0; i < 3; ++i) w->e[i] = n > 0.0 ? t[i] / n : 0.0;
+    }
+    for (int i = 0; i < 3; ++i) w->p[i] = p[i];
+    if (r->pw_kind == SDT_PW_P) for (int i = 0; i < 3; ++i) w->e[i] = p[i];
+    const double eps = 1.0e-6;   /* select_injection_faces */
+    for (int a = 0; a < 3; ++a) {
+        w->active[2 * a] = p[a] > eps;
+        w->active[2 * a + 1] = p[a] < -eps;
+    }
+    const int n_[3] = {r->nx, r->ny, r->nz};
+    const double h[3] = {r->dx, r->dy, r->dz};
+    for (int a = 0; a < 3; ++a) {
+        const double lo = h[a] * (1 + r->cpml), hi = h[a] * (n_[a] - r->cpml);
+        w->loc[a] = 0.5 * (lo + hi);
+        if (w->active[2 * a]) w->loc[a] = lo;
+        if (w->active[2 * a + 1]) w->loc[a] = hi;
+        w->r0[a] = w->loc[a];
+        w->face[a] = (w->active[2 * a] || w->active[2 * a + 1]) ? (int)lround(w->loc[a] / h[a]) : 0;
+    }
+    if (!three_d) { w->r0[1] = 0.0; w->loc[1] = 0.0; w->face[1] = 0; }
+}
+
+/* boundary_seismic_field, plane_wave_source.f08:196-230 */
+static void pw_field(const oracle_run *r, const pw_state *w, const double x[3], double t, double V[3], double st[6])
+{
+    const double tau = t - (w->p[0] * (x[0] - w->r0[0]) + w->p[1] * (x[1] - w->r0[1]) + w->p[2] * (x[2] - w->r0[2])) /
+                               r->pw_speed;
+    const double f0 = r->pw_f0, t0 = 1.5 / f0;
+    const double a = 3.1415926535897932384626433832795 * f0 * (tau - t0), g = exp(-a * a);
+    double wv = g;
+    if (r->pw_wavelet == 1) wv = -2.0 * a * g;
+    else if (r->pw_wavelet == 2) wv = (1.0 - 2.0 * a * a) * g;
+    const double s = r->pw_amp * wv, sc = -s / r->pw_speed;
+    const double *e = w->e, *p = w->p;
+    for (int i = 0; i < 3; ++i) V[i] = s * e[i];
+    st[0] = sc * e[0] * p[0];
+    st[1] = sc * e[1] * p[1];
+    st[2] = sc * e[2] * p[2];
+    st[3] = sc * (e[1] * p[2] + e[2] * p[1]);
+    st[4] = sc * (e[0] * p[2] + e[2] * p[0]);
+    st[5] = sc * (e[0] * p[1] + e[1] * p[0]);
+}
+
+/* add_plane_wave_stress (cpmlfdtd.f08:16-50) into s[6] = xx yy zz yz xz xy */
+static void pw_stress(const double *C21, const double st[6], double s[6])
+{
+    static const int ij[6][6] = {{0, 1, 2, 3, 4, 5},   {1, 6, 7, 8, 9, 10},   {2, 7, 11, 12, 13, 14},
+                                 {3, 8, 12, 15, 16, 17}, {4, 9, 13, 16, 18, 19}, {5, 10, 14, 17, 19, 20}};
+    for (int rw = 0; rw < 6; ++rw)
+        for (int q = 0; q < 6; ++q) s[rw] = s[rw] + C21[ij[rw][q]] * st[q];
+}
+
 /* ========================================================================
  * seismic2: cpmlfdtd.f08:57-597 (time loop 311-576)
  * ======================================================================== */
@@ -169,6 +265,8 @@ int32_t oracle_seismic2(oracle_run *r)
 
     const int is = r->isrc, js = r->ksrc;
     r->unstable_step = 0;
+    pw_state pw;
+    if (r->pw_kind) pw_setup(r, 0, &pw);
 
     for (int it = 1; it <= r->nsteps; ++it) {
         /* sigma_xx, sigma_zz: cpmlfdtd.f08:324-353 */
@@ -256,12 +354,33 @@ int32_t oracle_seismic2(oracle_run *r)
                 d_szz_dz = d_szz_dz / kappa_half[c] + m_dszz_dz[c];
                 vz[c] = vz[c] + dt * (d_sxz_dx / rhoxz + d_szz_dz / rhozz);
             }
+        if (r->pw_kind) {
+            /* plane wave, cpmlfdtd.f08:454-512 */
+            const double t = it * dt;
+            for (int f = 0; f < 3; f += 2) {
+                if (!pw.face[f]) continue;
+                const int lo = r->cpml + 2, hi = (f == 0 ? nz : nx) - r->cpml - 1;
+                for (int ip = lo; ip <= hi; ++ip) {
+                    const int i = f == 0 ? pw.face[0] : ip, j = f == 0 ? ip : pw.face[2];
+                    const double x[3] = {f == 0 ? pw.loc[0] : ip * dx, 0.0, f == 0 ? ip * dz : pw.loc[2]};
+                    double V[3], st[6];
+                    pw_field(r, &pw, x, t, V, st);
+                    const size_t c = I2(i, j);
+                    vx[c] = vx[c] + V[0];
+                    vz[c] = vz[c] + V[2];
+                    sigmaxx[c] = sigmaxx[c] + c11[c] * st[0] + c13[c] * st[2] + c15[c] * st[4];
+                    sigmazz[c] = sigmazz[c] + c13[c] * st[0] + c33[c] * st[2] + c35[c] * st[4];
+                    sigmaxz[c] = sigmaxz[c] + c15[c] * st[0] + c35[c] * st[2] + c55[c] * st[4];
+                }
+            }
+        } else {
         /* point source, cpmlfdtd.f08:519-523 (velocity sources carry no dt) */
         sigmaxx[I2(is, js)] = sigmaxx[I2(is, js)] + src_at(r, SDT_SRC_XX, it) / rho[I2(is, js)];
         sigmaxz[I2(is + 1, js + 1)] = sigmaxz[I2(is + 1, js + 1)] + src_at(r, SDT_SRC_XZ, it) / rho[I2(is + 1, js + 1)];
         sigmazz[I2(is, js)] = sigmazz[I2(is, js)] + src_at(r, SDT_SRC_ZZ, it) / rho[I2(is, js)];
         vx[I2(is, js)] = vx[I2(is, js)] + src_at(r, SDT_SRC_X, it) / rho[I2(is, js)];
         vz[I2(is, js)] = vz[I2(is, js)] + src_at(r, SDT_SRC_Z, it) / rho[I2(is, js)];
+        }
         /* Dirichlet edges, cpmlfdtd.f08:534-549 */
         for (int j = 1; j <= nz; ++j) {
             vx[I2(1, j)] = 0.0; vx[I2(nx, j)] = 0.0;
@@ -368,6 +487,8 @@ int32_t oracle_seismic3(oracle_run *r)
     const int is = r->isrc, js = r->jsrc, ks = r->ksrc;
     r->unstable_step = 0;
     enum { X = 0, Y = 1, Z = 2 };
+    pw_state pw;
+    if (r->pw_kind) pw_setup(r, 1, &pw);
 
     for (int it = 1; it <= r->nsteps; ++it) {
         /* Loop 1, sigma_xx/yy/zz: cpmlfdtd.f08:1761-1818 */
@@ -516,6 +637,36 @@ int32_t oracle_seismic3(oracle_run *r)
                     CPML(d3, m_szz_dz[c], acoef_half[c], bcoef_half[c], kappa_half[c]);
                     vz[c] = vz[c] + (d1 / rhoxz + d2 / rhoyz + d3 / rhozz) * dt;
                 }
+        if (r->pw_kind) {
+            /* plane wave, cpmlfdtd.f08:2133-2200: faces x, y, z in turn */
+            const double t = it * dt;
+            const int lo[3] = {r->cpml + 2, r->cpml + 2, r->cpml + 2};
+            const int hi[3] = {nx - r->cpml - 1, ny - r->cpml - 1, nz - r->cpml - 1};
+            for (int f = 0; f < 3; ++f) {
+                if (!pw.face[f]) continue;
+                const int a = f == 0 ? 1 : 0, b = f == 2 ? 1 : 2;
+                for (int u = lo[a]; u <= hi[a]; ++u)
+                    for (int w = lo[b]; w <= hi[b]; ++w) {
+                        int ijk[3];
+                        ijk[f] = pw.face[f];
+                        ijk[a] = u;
+                        ijk[b] = w;
+                        double x[3] = {ijk[0] * r->dx, ijk[1] * r->dy, ijk[2] * r->dz};
+                        x[f] = pw.loc[f];
+                        double V[3], st[6];
+                        pw_field(r, &pw, x, t, V, st);
+                        const size_t c = I3(ijk[0], ijk[1], ijk[2]);
+                        vx[c] = vx[c] + V[0];
+                        vy[c] = vy[c] + V[1];
+                        vz[c] = vz[c] + V[2];
+                        double C21[21], s6[6] = {sxx[c], syy[c], szz[c], syz[c], sxz[c], sxy[c]};
+                        for (int q = 0; q < 21; ++q) C21[q] = C[q][c];
+                        pw_stress(C21, st, s6);
+                        sxx[c] = s6[0]; syy[c] = s6[1]; szz[c] = s6[2];
+                        syz[c] = s6[3]; sxz[c] = s6[4]; sxy[c] = s6[5];
+                    }
+            }
+        } else {
         /* point source, cpmlfdtd.f08:2208-2219 */
         sxx[I3(is, js, ks)] = sxx[I3(is, js, ks)] + src_at(r, SDT_SRC_XX, it) / rho[I3(is, js, ks)];
         sxy[I3(is + 1, js + 1, ks)] = sxy[I3(is + 1, js + 1, ks)] + src_at(r, SDT_SRC_XY, it) / rho[I3(is + 1, js + 1, ks)];
@@ -526,6 +677,7 @@ int32_t oracle_seismic3(oracle_run *r)
         vx[I3(is + 1, js, ks)] = vx[I3(is + 1, js, ks)] + src_at(r, SDT_SRC_X, it) * dt / rho[I3(is + 1, js, ks)];
         vy[I3(is, js + 1, ks)] = vy[I3(is, js + 1, ks)] + src_at(r, SDT_SRC_Y, it) * dt / rho[I3(is, js, ks)];
         vz[I3(is, js, ks + 1)] = vz[I3(is, js, ks + 1)] + src_at(r, SDT_SRC_Z, it) * dt / rho[I3(is, js, ks + 1)];
+        }
         /* Dirichlet faces, cpmlfdtd.f08:2226-2248 */
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 1; k <= nz; ++k)
@@ -1008,6 +1160,8 @@ int32_t oracle_seismic25(oracle_run *r)
     const int is = r->isrc, ks = r->ksrc;
     r->unstable_step = 0;
     cplx_t *V[3] = {vx, vy, vz};
+    pw_state pw;
+    if (r->pw_kind) pw_setup(r, 0, &pw);
 
     for (int it = 1; it <= r->nsteps; ++it) {
         /* L1 sigma_xx, sigma_yy, sigma_zz: x forward half, z backward full (cpmlfdtd.f08:941-984) */
@@ -1116,6 +1270,32 @@ int32_t oracle_seismic25(oracle_run *r)
                 d3 = ccpml(d3, &MV[5][c], acoef_half[c], bcoef_half[c], kappa_half[c]);
                 vz[c] = cadd(vz[c], cmulr(cadd(cadd(cdivr(d1, rhoxz[c]), cdivr(d2, rhoyz[c])), cdivr(d3, rhozz[c])), dt));
             }
+        if (r->pw_kind) {
+            /* plane wave, cpmlfdtd.f08:1208-1262: x face then z face, real parts */
+            for (int f = 0; f < 3; f += 2) {
+                if (!pw.face[f]) continue;
+                const int lo = r->cpml + 2, hi = (f == 0 ? nz : nx) - r->cpml - 1;
+                for (int ip = lo; ip <= hi; ++ip) {
+                    const int i = f == 0 ? pw.face[0] : ip, k = f == 0 ? ip : pw.face[2];
+                    const double x[3] = {f == 0 ? pw.loc[0] : ip * dx, 0.0, f == 0 ? ip * dz : pw.loc[2]};
+                    double Vv[3], st[6];
+                    pw_field(r, &pw, x, it * dt, Vv, st);
+                    const size_t c = I2(i, k);
+                    vx[c].re = vx[c].re + Vv[0];
+                    vy[c].re = vy[c].re + Vv[1];
+                    vz[c].re = vz[c].re + Vv[2];
+                    double C21[21], s6[6] = {0, 0, 0, 0, 0, 0};
+                    for (int q = 0; q < 21; ++q) C21[q] = C[q][c];
+                    pw_stress(C21, st, s6);
+                    sxx[c].re = sxx[c].re + s6[0];
+                    syy[c].re = syy[c].re + s6[1];
+                    szz[c].re = szz[c].re + s6[2];
+                    syz[c].re = syz[c].re + s6[3];
+                    sxz[c].re = sxz[c].re + s6[4];
+                    sxy[c].re = sxy[c].re + s6[5];
+                }
+            }
+        } else {
         /* point source, cpmlfdtd.f08:1268-1276 (real series add to the real part) */
         sxx[I2(is, ks)].re += src_at(r, SDT_SRC_XX, it) / rho[I2(is, ks)];
         sxy[I2(is + 1, ks)].re += src_at(r, SDT_SRC_XY, it) / rho[I2(is + 1, ks)];
@@ -1126,6 +1306,7 @@ int32_t oracle_seismic25(oracle_run *r)
         vx[I2(is + 1, ks)].re += src_at(r, SDT_SRC_X, it) * dt / rho[I2(is + 1, ks)];
         vy[I2(is, ks)].re += src_at(r, SDT_SRC_Y, it) * dt / rho[I2(is, ks)];
         vz[I2(is, ks + 1)].re += src_at(r, SDT_SRC_Z, it) * dt / rho[I2(is, ks + 1)];
+        }
         /* Dirichlet edges, cpmlfdtd.f08:1280-1285 */
         for (int m = 0; m < 3; ++m) {
             for (int k = 1; k <= nz; ++k) { V[m][I2(1, k)] = cx(0, 0); V[m][I2(nx, k)] = cx(0, 0); }

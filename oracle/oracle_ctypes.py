@@ -48,7 +48,35 @@ class OracleRun(C.Structure):
         ("field_in", _D * NUM_FIELDS), ("field_out", _D * NUM_FIELDS),
         ("velnorm", _D), ("stressnorm", _D), ("unstable_step", C.c_int32), ("nthreads", C.c_int32),
         ("ky", C.c_double),
+        ("pw_kind", C.c_int32), ("pw_wavelet", C.c_int32), ("cpml", C.c_int32),
+        ("pw_angles", C.c_double * 3), ("pw_f0", C.c_double), ("pw_amp", C.c_double), ("pw_speed", C.c_double),
     ]
+
+
+_VOIGT = [(0, 0), (0, 1), (0, 2), (0, 3), (0, 4), (0, 5), (1, 1), (1, 2), (1, 3), (1, 4), (1, 5), (2, 2), (2, 3),
+          (2, 4), (2, 5), (3, 3), (3, 4), (3, 5), (4, 4), (4, 5), (5, 5)]
+
+
+def background_speed(arrays, two_d):
+    """sqrt(max over cells of the largest eigenvalue of the Kelvin-form
+    stiffness / min density), as cpmlfdtd.f08:228-229 (2D), 829-832 (2.5D)
+    and 1601-1605 (3D) compute it with LAPACK dsyev
+    (tensor_operations.f08:57-132, 157-181, 207-236); numpy's eigvalsh is
+    the same LAPACK family."""
+    names = ["c11", "c12", "c13", "c14", "c15", "c16", "c22", "c23", "c24", "c25", "c26",
+             "c33", "c34", "c35", "c36", "c44", "c45", "c46", "c55", "c56", "c66"]
+    rho = np.asarray(arrays["rho"], dtype=np.float64).ravel()
+    n = rho.size
+    keep = ("c11", "c13", "c15", "c33", "c35", "c55") if two_d else names
+    K = np.zeros((n, 6, 6))
+    s = np.array([1, 1, 1, np.sqrt(2.0), np.sqrt(2.0), np.sqrt(2.0)])
+    for q, (a, b) in enumerate(_VOIGT):
+        if names[q] not in keep or names[q] not in arrays:
+            continue
+        v = np.asarray(arrays[names[q]], dtype=np.float64).ravel() * s[a] * s[b]
+        K[:, a, b] = v
+        K[:, b, a] = v
+    return float(np.sqrt(np.linalg.eigvalsh(K).max() / rho.min()))
 
 
 def build():
@@ -130,6 +158,16 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
     else:
         r.isrc, r.jsrc, r.ksrc = src[0] + cp, 1, src[1] + cp
     r.density_code = DENSITY_CODES[cfg.get("density_method", "arithmetic")]
+    r.cpml = int(cp)
+    pw = cfg.get("plane_wave")
+    if pw:
+        r.pw_kind = {"s": 1, "p": 2}[pw["kind"]]
+        r.pw_wavelet = int(pw.get("wavelet", 0))
+        for q in range(3):
+            r.pw_angles[q] = float(pw["angles"][q])
+        r.pw_f0 = float(pw["f0"])
+        r.pw_amp = float(pw["amplitude"])
+        r.pw_speed = background_speed(cfg["arrays"], phys == "elastic2")
     r.stencil = 1 if cfg.get("stencil", "reference") == "corrected" else 0
     r.nthreads = nthreads
     keep = []

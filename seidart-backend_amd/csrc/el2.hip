@@ -43,6 +43,7 @@ struct El2Args {
     T *store;
     int64_t stride;
     int store_first;
+    PwFaces pwf;        // plane-wave cells, skipped by the fused norm
 };
 constexpr int kEl2Fwd = 13;
 
@@ -298,7 +299,7 @@ __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, 
         A.m[10][c] = m[2];
         A.m[11][c] = m[3];
     }
-    if (c == A.src_cell) return 0.0;
+    if (c == A.src_cell || pw_face(A.pwf, i, 1, j)) return 0.0;
     const double dvx = (double)vx, dvz = (double)vz;
     return dvx * dvx + dvz * dvz;
 }
@@ -341,6 +342,7 @@ static El2Args<T> el2_args(Device &d)
     A.store = (T *)d.adj.seg_fwd;
     A.stride = d.ncell_dev;
     A.store_first = d.adj.store_first;
+    A.pwf = d.pw.faces();
     return A;
 }
 
@@ -496,6 +498,7 @@ static void el2_step_t(Device &d, bool timed)
     else
         hipLaunchKernelGGL((el2_velocity_kernel<T, R, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_plane_wave(d);
     launch_tail(d);
 }
 

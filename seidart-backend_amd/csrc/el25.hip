@@ -115,6 +115,7 @@ struct El25Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t vfix[3];      // source cells excluded from the fused norm
+    PwFaces pwf;          // plane-wave cells, likewise
 };
 
 // sigma = (sigma + (K1 d1 + K2 d2 + K3 d3 + K4 yz + K5 xz + K6 xy) dt) * ig,
@@ -279,7 +280,7 @@ __device__ __forceinline__ double el25_velocity_cell(const El25Args<T> &A, int i
             A.v[2][c] = vz;
         }
     }
-    if (c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2]) return 0.0;
+    if (c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, 1, k)) return 0.0;
     const double s = ((double)vx.re * vx.re + (double)vx.im * vx.im) + ((double)vy.re * vy.re + (double)vy.im * vy.im) +
                      ((double)vz.re * vz.re + (double)vz.im * vz.im);
     return sqrt(s);
@@ -329,6 +330,7 @@ static El25Args<T> el25_args(Device &d)
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     for (int q = 0; q < 3; ++q) A.vfix[q] = d.tail.vfix[q];
+    A.pwf = d.pw.faces();
     return A;
 }
 
@@ -342,6 +344,7 @@ static void el25_step_t(Device &d, bool timed)
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
     hipLaunchKernelGGL(el25_velocity_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_plane_wave(d);
     launch_tail(d);
 }
 

@@ -57,6 +57,7 @@ static El3Args<T> el3_args(Device &d)
     A.vnorm = d.norms;
     A.snorm = d.norms + d.time_steps;
     for (int q = 0; q < 3; ++q) A.vfix[q] = d.tail.vfix[q];
+    A.pwf = d.pw.faces();
     for (int q = 0; q < 4; ++q) A.sfix[q] = d.tail.sfix[q];
     return A;
 }
@@ -70,6 +71,7 @@ static void el3_step_t(Device &d, bool timed)
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
     el3_fast_velocity<T>(d, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_plane_wave(d);
     launch_tail(d);
 }
 

@@ -45,6 +45,7 @@ struct El3Args {
     const int *d_it;
     unsigned long long *vnorm, *snorm;
     int64_t vfix[3], sfix[4];
+    PwFaces pwf;    // plane-wave cells: skipped by the fused norms like vfix / sfix
 };
 
 // psi = b*psi + a*d ; d = d/kappa + psi  (e.g. cpmlfdtd.f08:1777, 1787)
@@ -103,7 +104,7 @@ struct El3Tile {
                          // load from it, so they add no memory traffic
 };
 
-constexpr int kTabRow = 32;     // material-table row length (elements)
+constexpr int kTabRow = kEl3TabRow;   // material-table row length (elements)
 constexpr int kTabIg = 21, kTabRho = 27;
 constexpr int kMaxMaterials = 256;
 

@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
         if (act) {
             stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); stnt(&A.s[2][c], szz);
             stnt(&A.s[3][c], sxy); stnt(&A.s[4][c], sxz); stnt(&A.s[5][c], syz);
-            const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3];
+            const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3] ||
+                             pw_face(A.pwf, i, j, k);
             const double a0 = sxx, a1 = syy, a2 = szz, a3 = sxy, a4 = syz, a5 = sxz;
             const double n = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5));
             nrm = fix ? nrm : fmax_nan(nrm, n);
@@ -249,7 +250,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
         }
         if (act) {
             stnt(&A.v[0][c], vx); stnt(&A.v[1][c], vy); stnt(&A.v[2][c], vz);
-            const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2];
+            const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, j, k);
             const double a0 = vx, a1 = vy, a2 = vz;
             nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
         }
@@ -532,7 +533,8 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
         N[b][w * kShellLanes + lane] = part;
         __syncthreads();
         if (w == 0 && L.on) {
-            const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3];
+            const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3] ||
+                             pw_face(A.pwf, L.i, L.j, k);
             const double n = sqrt(N[b][lane] + N[b][kShellLanes + lane] + N[b][2 * kShellLanes + lane]);
             nrm = fix ? nrm : fmax_nan(nrm, n);
         }
@@ -648,7 +650,7 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3
         V2[b][w * kShellLanes + lane] = dv * dv;
         __syncthreads();
         if (w == 0 && L.on) {
-            const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2];
+            const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, L.i, L.j, k);
             const double n = sqrt(V2[b][lane] + V2[b][kShellLanes + lane] + V2[b][2 * kShellLanes + lane]);
             nrm = fix ? nrm : fmax_nan(nrm, n);
         }

@@ -22,6 +22,7 @@ struct sdt_handle {
     std::vector<double> series;       // [SDT_NUM_SRC][time_steps]
     std::vector<int32_t> rec_ijk;     // padded 1-based (i, j, k) per receiver
     std::vector<double> coef;         // update coefficients (sdt_set_coefficients)
+    PlaneWaveSpec pw;                 // sdt_set_plane_wave
     bool prepared = false;
     bool failed = false;
     int steps_done = 0;
@@ -130,6 +131,7 @@ int32_t sdt_create(const sdt_params *p, sdt_handle **out)
     d.dx = p->dx; d.dy = d3 ? p->dy : 1.0; d.dz = p->dz; d.dt = p->dt;
     d.time_steps = p->time_steps;
     d.isrc = is; d.jsrc = js; d.ksrc = ks;
+    d.cpml_width = p->cpml;
     d.density_code = p->density_code;
     d.stencil = p->stencil;
     d.ky = (p->physics == SDT_ELASTIC25 || p->physics == SDT_EM25) ? p->ky : 0.0;
@@ -258,6 +260,8 @@ static int prepare(sdt_handle *h)
         return fail(h, SDT_ERR_STATE, "coefficient input needs SDT_OPT_RAW_SOURCE = 1");
     if (d.adj.every > 0 && d.physics != SDT_ELASTIC2 && d.physics != SDT_EM2)
         return fail(h, SDT_ERR_ARG, "checkpointing for the adjoint is available for 2D elastic and 2D EM");
+    if (d.adj.every > 0 && h->pw.kind)
+        return fail(h, SDT_ERR_ARG, "the adjoint takes point sources, not a plane wave");
     d.coef_in = h->coef.empty() ? nullptr : h->coef.data();
     switch (d.physics) {
     case SDT_ELASTIC2: rc = el2_setup(d, in); break;
@@ -269,6 +273,8 @@ static int prepare(sdt_handle *h)
     }
     d.coef_in = nullptr;
     if (rc) return fail(h, rc, "device setup failed (out of device memory?)");
+    if (h->pw.kind && plane_wave_setup(d, h->pw, in) != SDT_OK)
+        return fail(h, SDT_ERR_ARG, "plane-wave source: the model needs positive density and stiffness");
     if (d.raw_source)
         for (int q = 0; q < d.tail.nterms; ++q) {
             d.tail.terms[q].mul = 1.0;
@@ -621,6 +627,28 @@ static int ncoef_of(int physics)
     return physics == SDT_EM2 ? 7 : physics == SDT_ELASTIC2 ? 10 : 0;
 }
 
+int32_t sdt_set_plane_wave(sdt_handle *h, int32_t kind, double angle_xz, double angle_xy, double angle_yz, double f0,
+                           double amplitude, int32_t wavelet)
+{
+    if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
+    if (h->prepared) return fail(h, SDT_ERR_STATE, "inputs are fixed once the first step has run");
+    const int ph = h->d.physics;
+    if (ph != SDT_ELASTIC2 && ph != SDT_ELASTIC25 && ph != SDT_ELASTIC3)
+        return fail(h, SDT_ERR_ARG, "plane-wave sources belong to the elastic solvers");
+    if (kind != SDT_PW_S && kind != SDT_PW_P) return fail(h, SDT_ERR_ARG, "plane-wave kind must be SDT_PW_S or SDT_PW_P");
+    if (!(f0 > 0) || !isfinite(f0) || !isfinite(amplitude) || !isfinite(angle_xz) || !isfinite(angle_xy) ||
+        !isfinite(angle_yz))
+        return fail(h, SDT_ERR_ARG, "plane-wave parameters must be finite, f0 > 0");
+    h->pw.kind = kind;
+    h->pw.angle_xz = angle_xz;
+    h->pw.angle_xy = angle_xy;
+    h->pw.angle_yz = angle_yz;
+    h->pw.f0 = f0;
+    h->pw.amplitude = amplitude;
+    h->pw.wavelet = wavelet >= 0 && wavelet <= 2 ? wavelet : 0;
+    return SDT_OK;
+}
+
 int32_t sdt_set_coefficients(sdt_handle *h, const double *coef, int64_t count)
 {
     if (!h || !coef) return fail(h, SDT_ERR_ARG, "null argument");
@@ -779,6 +807,7 @@ int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
             h->p.device != handles[0]->p.device)
             return fail(nullptr, SDT_ERR_ARG, "batch handles must share solver kind, precision and device");
         if (h->failed) return fail(nullptr, SDT_ERR_STATE, "a batch handle has already blown up");
+        if (h->pw.kind) return fail(nullptr, SDT_ERR_ARG, "batched shots take point sources, not a plane wave");
         int rc = prepare(h);
         if (rc) return rc;
     }

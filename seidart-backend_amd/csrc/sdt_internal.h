@@ -80,6 +80,48 @@ struct Adjoint {
     int store_first = 0;               // 0-based first step of the recomputed segment
 };
 
+// Plane-wave source: the cells of the active interior faces (face[a] = cell
+// index of the face normal to axis a, 0 = no face; lo/hi = cell ranges of
+// the other axes).  The fused norm reductions skip these cells; the
+// plane-wave pass folds their post-injection values in.
+struct PwFaces {
+    int face[3];
+    int lo[3], hi[3];
+};
+__host__ __device__ inline bool pw_face(const PwFaces &f, int i, int j, int k)
+{
+    return (i == f.face[0] && j >= f.lo[1] && j <= f.hi[1] && k >= f.lo[2] && k <= f.hi[2]) ||
+           (j == f.face[1] && i >= f.lo[0] && i <= f.hi[0] && k >= f.lo[2] && k <= f.hi[2]) ||
+           (k == f.face[2] && i >= f.lo[0] && i <= f.hi[0] && j >= f.lo[1] && j <= f.hi[1]);
+}
+
+struct PlaneWaveSpec {   // sdt_set_plane_wave arguments
+    int kind = 0;        // 0 none, SDT_PW_S, SDT_PW_P
+    double angle_xz = 0, angle_xy = 0, angle_yz = 0, f0 = 0, amplitude = 0;
+    int wavelet = 0;
+};
+
+struct PlaneWave {
+    int on = 0;
+    int face[3] = {}, lo[3] = {}, hi[3] = {};
+    double p[3] = {}, e[3] = {};   // propagation direction, polarisation
+    double r0[3] = {};             // reference point (m)
+    double loc[3] = {};            // face positions (m)
+    double h[3] = {};              // cell sizes
+    double v = 0, f0 = 0, amp = 0, dt = 0;   // background speed, wavelet, step
+    int wavelet = 0;
+    PwFaces faces() const
+    {
+        PwFaces f;
+        for (int a = 0; a < 3; ++a) {
+            f.face[a] = on ? face[a] : 0;
+            f.lo[a] = lo[a];
+            f.hi[a] = hi[a];
+        }
+        return f;
+    }
+};
+
 // trace-gradient injection of one adjoint step
 struct AdjTailArgs {
     void *lam[3];             // adjoint array of each recorded component (or null)
@@ -94,6 +136,7 @@ struct Device {
     double dx = 0, dy = 0, dz = 0, dt = 0;
     int time_steps = 0;
     int isrc = 0, jsrc = 0, ksrc = 0;   // padded 1-based
+    int cpml_width = 0;                // absorbing-layer width (cells)
     int density_code = 3, stencil = 0;
     double ky = 0.0;                   // 2.5D elastic y wavenumber
     int ncomp = 1;                     // values per field element (2 = complex)
@@ -166,6 +209,7 @@ struct Device {
     const double *coef_in = nullptr;   // update coefficients given directly (during setup only)
     bool raw_source = false;           // SDT_OPT_RAW_SOURCE: series injected as given
     Adjoint adj;
+    PlaneWave pw;                      // plane-wave source (elastic solvers)
 };
 
 // Per-solver hooks, defined in the solver's .hip file.
@@ -260,6 +304,11 @@ void launch_adj_tail_batch(Batch &b, int t);
 
 // common pieces (sdt_common.hip)
 void launch_tail(Device &d);
+// plane-wave source: geometry and background speed from the staged model
+// (before its arrays are released), and the per-step injection
+int plane_wave_setup(Device &d, const PlaneWaveSpec &spec, const double *const *host_in);
+void launch_plane_wave(Device &d);
+constexpr int kEl3TabRow = 32;         // 3D elastic material-table row length
 void launch_adj_tail(Device &d, int t, void *const lam_field[3]);
 void set_step_counter(hipStream_t s, int *d_it, int v);
 int download_grid_f64(Device &d, const double *dev, double *host);

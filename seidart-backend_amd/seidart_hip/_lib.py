@@ -65,7 +65,7 @@ EXPORTS = [
     "sdt_get_field", "sdt_set_field", "sdt_get_receivers", "sdt_get_norms", "sdt_set_timing",
     "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option", "sdt_get_info",
     "sdt_batch_create", "sdt_batch_run", "sdt_batch_info", "sdt_batch_destroy",
-    "sdt_set_coefficients", "sdt_adjoint", "sdt_batch_adjoint",
+    "sdt_set_coefficients", "sdt_adjoint", "sdt_batch_adjoint", "sdt_set_plane_wave",
 ]
 
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
@@ -73,6 +73,7 @@ OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCU
 OPT_CHECKPOINT, OPT_RAW_SOURCE = 8, 9
 INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES, INFO_COEFFICIENTS = 1, 2, 3, 4
 NUM_SRC = 9
+PW_S, PW_P = 1, 2
 BATCH_INFO_TILE_ROWS, BATCH_INFO_TILES, BATCH_INFO_XCD_ORDER = 1, 2, 3
 
 
@@ -125,6 +126,7 @@ def _load():
         "sdt_set_coefficients": ([P, D, I64], I32),
         "sdt_adjoint": ([P, D, I64, D, I64, D, I64], I32),
         "sdt_batch_adjoint": ([P, C.POINTER(D), C.POINTER(D), C.POINTER(D)], I32),
+        "sdt_set_plane_wave": ([P, I32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, I32], I32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -148,6 +148,16 @@ class Solver:
         L.check(L.lib.sdt_adjoint(self._h, _dptr(g), g.size, _dptr(cg), cg.size, _dptr(sg), sg.size), self._h)
         return np.stack([row.reshape(self.shape, order="F") for row in cg]), sg
 
+    def set_plane_wave(self, kind, angles, f0, amplitude, wavelet=0):
+        """Elastic solvers: a plane-wave source in place of the point source.
+        kind "s" (polarised across the direction: source types pw, pws,
+        pw_s, pw-s) or "p" (along it: pwp, pw_p, pw-p); angles = (x_z, x_y,
+        y_z) rotations in degrees; wavelet 0 gaus0, 1 gaus1, 2 gaus2."""
+        k = {"s": L.PW_S, "p": L.PW_P}[kind]
+        axz, axy, ayz = (float(a) for a in angles)
+        L.check(L.lib.sdt_set_plane_wave(self._h, k, axz, axy, ayz, float(f0), float(amplitude), int(wavelet)),
+                self._h)
+
     def set_receivers(self, idx):
         """idx: (n, 3) 1-based unpadded (x, y, z) indices; y ignored in 2D."""
         r = np.ascontiguousarray(np.asarray(idx, dtype=np.int32).reshape(-1, 3))
@@ -317,6 +327,9 @@ def make_solver(cfg, *, precision="fp64", device=0, options=None):
             s.set_source(comp, series)
         if len(cfg.get("receivers", ())):
             s.set_receivers(cfg["receivers"])
+        pw = cfg.get("plane_wave")
+        if pw:
+            s.set_plane_wave(pw["kind"], pw["angles"], pw["f0"], pw["amplitude"], pw.get("wavelet", 0))
     except Exception:
         s.close()
         raise

@@ -1,0 +1,85 @@
+"""Pins of the oracle's plane-wave source (plane_wave_source.f08 and the
+injection loops of cpmlfdtd.f08) by closed forms: the Kelvin-form background
+speed of an isotropic solid, and the fields one step after injection into a
+zero state for rotations whose direction and polarisation are known by hand
+(the rotation matrices are filled column by column, cpmlfdtd's reshape)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle_ctypes import background_speed, run_oracle
+
+
+def _iso(shape, lam, mu, rho):
+    A = {n: np.zeros(shape) for n in ("c11", "c12", "c13", "c14", "c15", "c16", "c22", "c23", "c24", "c25",
+                                      "c26", "c33", "c34", "c35", "c36", "c44", "c45", "c46", "c55", "c56", "c66")}
+    for n in ("c11", "c22", "c33"):
+        A[n][:] = lam + 2 * mu
+    for n in ("c12", "c13", "c23"):
+        A[n][:] = lam
+    for n in ("c44", "c55", "c66"):
+        A[n][:] = mu
+    A["rho"] = np.full(shape, rho)
+    return A
+
+
+def test_background_speed_isotropic_closed_form():
+    lam, mu, rho = 7.0e9, 3.0e9, 1000.0
+    A = _iso((6, 5), lam, mu, rho)
+    A["rho"][2, 3] = 800.0          # the smallest density sets the speed
+    # 3D Kelvin eigenvalues of an isotropic tensor: 3 lam + 2 mu once, 2 mu five times
+    assert background_speed(A, two_d=False) == pytest.approx(math.sqrt((3 * lam + 2 * mu) / 800.0), rel=1e-14)
+    # 2D keeps the (xx, zz, xz) block: eigenvalues 2 lam + 2 mu, 2 mu, 2 mu
+    assert background_speed(A, two_d=True) == pytest.approx(math.sqrt((2 * lam + 2 * mu) / 800.0), rel=1e-14)
+
+
+def _gaus(t, f0, kind):
+    a = math.pi * f0 * (t - 1.5 / f0)
+    g = math.exp(-a * a)
+    return {0: g, 1: -2 * a * g, 2: (1 - 2 * a * a) * g}[kind]
+
+
+@pytest.mark.parametrize("angles,kind,face,e_vec,p_vec", [
+    ((0.0, 0.0, 0.0), "s", "zmin", (1, 0, 0), (0, 0, 1)),      # R = I: p = z, e = x
+    ((90.0, 0.0, 0.0), "s", "xmax", (0, 0, 1), (-1, 0, 0)),    # Ry(90): p = (-sy, 0, cy), e = (cy, 0, sy)
+    ((90.0, 0.0, 0.0), "p", "xmax", (-1, 0, 0), (-1, 0, 0)),
+    ((180.0, 0.0, 0.0), "p", "zmax", (0, 0, -1), (0, 0, -1)),
+])
+def test_el2_one_step_injection_closed_form(angles, kind, face, e_vec, p_vec):
+    from seidart_hip import models
+    cfg = models.elastic2(nx=30, nz=24, steps=1, src=(10, 10), nrec=2, nlayers=2)
+    f0, amp, wav = 400.0, 2.5e-3, 2
+    cfg["plane_wave"] = dict(kind=kind, angles=angles, f0=f0, amplitude=amp, wavelet=wav)
+    fields = ("vx", "vz", "sxx", "szz", "sxz")
+    _, _, F, us = run_oracle(cfg, steps=1, fields=fields)
+    assert us == 0
+    A = cfg["arrays"]
+    nx, nz = cfg["shape"]
+    cp = cfg["cpml"]
+    v = background_speed(A, two_d=True)
+    s = amp * _gaus(cfg["dt"], f0, wav)     # on the face p.(r - r0) = 0, so tau = t = dt
+    sc = -s / v
+    e, p = np.array(e_vec, float), np.array(p_vec, float)
+    st = [sc * e[0] * p[0], sc * e[1] * p[1], sc * e[2] * p[2], sc * (e[1] * p[2] + e[2] * p[1]),
+          sc * (e[0] * p[2] + e[2] * p[0]), sc * (e[0] * p[1] + e[1] * p[0])]
+    if face in ("zmin", "zmax"):
+        k = cp + 1 if face == "zmin" else nz - cp     # 1-based face row
+        cells = [(i, k) for i in range(cp + 2, nx - cp)]
+    else:
+        i = cp + 1 if face == "xmin" else nx - cp
+        cells = [(i, k) for k in range(cp + 2, nz - cp)]
+    mask = np.zeros((nx, nz), bool)
+    for (i, k) in cells:
+        c = (i - 1, k - 1)
+        mask[c] = True
+        # cos(90 deg) is 6e-17 in floating point: absolute slack of 1e-12 |s|
+        assert F["vx"][c] == pytest.approx(s * e[0], rel=1e-12, abs=1e-12 * abs(s))
+        assert F["vz"][c] == pytest.approx(s * e[2], rel=1e-12, abs=1e-12 * abs(s))
+        want = {"sxx": A["c11"][c] * st[0] + A["c13"][c] * st[2] + A["c15"][c] * st[4],
+                "szz": A["c13"][c] * st[0] + A["c33"][c] * st[2] + A["c35"][c] * st[4],
+                "sxz": A["c15"][c] * st[0] + A["c35"][c] * st[2] + A["c55"][c] * st[4]}
+        for n, w in want.items():
+            assert F[n][c] == pytest.approx(w, rel=1e-12, abs=1e-12 * abs(sc) * A["c33"][c])
+    for n in fields:   # nothing else moved: the first update of a zero state is zero
+        assert not np.any(F[n][~mask])
