@@ -223,18 +223,23 @@ enum {
 };
 int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
 
-/* plane-wave source of the elastic solvers (2D, 2.5D, 3D), replacing the
- * point source: an analytic plane wave (wavelet `wavelet` = 0 gaus0, 1
- * gaus1, 2 gaus2 of centre frequency f0, times amplitude) travelling along
- * the direction the three rotation angles (degrees) give, polarised across
- * it (SDT_PW_S: source types 'pw', 'pws', 'pw_s', 'pw-s') or along it
- * (SDT_PW_P: 'pwp', 'pw_p', 'pw-p'), added every step to the velocities and
- * (through the stiffness) the stresses of the interior faces it enters
- * through, at the model's largest Kelvin-form stiffness eigenvalue over its
- * smallest density as speed.  Replaces plane_wave_source.f08:20-230 with
- * the injection loops cpmlfdtd.f08:188-232 / 454-512 (seismic2), 802-832 /
- * 1208-1262 (seismic25), 1548-1605 / 2133-2220 (seismic3).  Set before the
- * first step. */
+/* plane-wave source, replacing the point source: an analytic plane wave
+ * (wavelet `wavelet` = 0 gaus0, 1 gaus1, 2 gaus2 of centre frequency f0,
+ * times amplitude) travelling along the direction the three rotation angles
+ * (degrees) give, added every step on the interior faces it enters through.
+ * Elastic solvers (2D, 2.5D, 3D): polarised across the direction (SDT_PW_S:
+ * source types 'pw', 'pws', 'pw_s', 'pw-s') or along it (SDT_PW_P: 'pwp',
+ * 'pw_p', 'pw-p'), added to the velocities and (through the stiffness) the
+ * stresses, at the model's largest Kelvin-form stiffness eigenvalue over its
+ * smallest density as speed.  EM solvers (2D, 3D): source type 'pw' only
+ * (SDT_PW_S), E along the polarisation and H = E / eta along p x e, at speed
+ * c / sqrt(eps_min) and impedance eta = sqrt(mu0 / (eps0 eps_min)) of the
+ * smallest relative-permittivity eigenvalue; the 2.5D EM solver has none
+ * (SDT_ERR_ARG).  Replaces plane_wave_source.f08:20-230 with the injection
+ * loops cpmlfdtd.f08:188-232 / 454-512 (seismic2), 802-832 / 1208-1262
+ * (seismic25), 1548-1605 / 2133-2220 (seismic3), 2436-2475 / 2626-2673
+ * (electromag2), 3523-3580 / 3833-3903 (electromag3).  Set before the first
+ * step. */
 enum { SDT_PW_S = 1, SDT_PW_P = 2 };
 int32_t sdt_set_plane_wave(sdt_handle *h, int32_t kind, double angle_xz, double angle_xy, double angle_yz,
                            double f0, double amplitude, int32_t wavelet);

@@ -29,8 +29,8 @@ import numpy as np
 import torch
 import torch.nn.functional as F
 
-EPS0 = 8.85418782e-12                        # constants.f08:11
-MU0 = 4.0 * math.pi * 1.0e-7                 # constants.f08:10
+EPS0 = 8.85418782e-12                        # constants.f08:10
+MU0 = 4.0 * math.pi * 1.0e-7                 # constants.f08:9
 
 
 def em2_coefficients_np(e11, e13, e33, s11, s13, s33):

@@ -194,8 +194,9 @@ static void pw_setup(const oracle_run *r, int three_d, pw_state *w)
     if (!three_d) { w->r0[1] = 0.0; w->loc[1] = 0.0; w->face[1] = 0; }
 }
 
-/* boundary_seismic_field, plane_wave_source.f08:196-230 */
-static void pw_field(const oracle_run *r, const pw_state *w, const double x[3], double t, double V[3], double st[6])
+/* amplitude of the wave at position x, time t: amp * wavelet(tau) with
+ * tau = t - p.(x - r0) / v (plane_wave_source.f08:141-161, 180-183, 214-218) */
+static double pw_signal(const oracle_run *r, const pw_state *w, const double x[3], double t)
 {
     const double tau = t - (w->p[0] * (x[0] - w->r0[0]) + w->p[1] * (x[1] - w->r0[1]) + w->p[2] * (x[2] - w->r0[2])) /
                                r->pw_speed;
@@ -204,7 +205,28 @@ static void pw_field(const oracle_run *r, const pw_state *w, const double x[3], 
     double wv = g;
     if (r->pw_wavelet == 1) wv = -2.0 * a * g;
     else if (r->pw_wavelet == 2) wv = (1.0 - 2.0 * a * a) * g;
-    const double s = r->pw_amp * wv, sc = -s / r->pw_speed;
+    return r->pw_amp * wv;
+}
+
+/* boundary_em_field, plane_wave_source.f08:163-194: E = s e, H = (s / eta)
+ * (p x e) / |p x e| */
+static void pw_em_field(const oracle_run *r, const pw_state *w, const double x[3], double t, double E[3], double H[3])
+{
+    const double s = pw_signal(r, w, x, t);
+    const double *e = w->e, *p = w->p;
+    double hh[3] = {p[1] * e[2] - p[2] * e[1], p[2] * e[0] - p[0] * e[2], p[0] * e[1] - p[1] * e[0]};
+    const double hn = sqrt(hh[0] * hh[0] + hh[1] * hh[1] + hh[2] * hh[2]);
+    if (hn > 0.0) for (int i = 0; i < 3; ++i) hh[i] /= hn;
+    for (int i = 0; i < 3; ++i) {
+        E[i] = s * e[i];
+        H[i] = (s / r->pw_eta) * hh[i];
+    }
+}
+
+/* boundary_seismic_field, plane_wave_source.f08:196-230 */
+static void pw_field(const oracle_run *r, const pw_state *w, const double x[3], double t, double V[3], double st[6])
+{
+    const double s = pw_signal(r, w, x, t), sc = -s / r->pw_speed;
     const double *e = w->e, *p = w->p;
     for (int i = 0; i < 3; ++i) V[i] = s * e[i];
     st[0] = sc * e[0] * p[0];
@@ -776,6 +798,8 @@ int32_t oracle_electromag2(oracle_run *r)
     memcpy(Ez_old, Ez, n * sizeof(double));
 
     const int is = r->isrc, js = r->ksrc;
+    pw_state pw;
+    if (r->pw_kind) pw_setup(r, 0, &pw);
     r->unstable_step = 0;
     for (int it = 1; it <= r->nsteps; ++it) {
         /* Hy: cpmlfdtd.f08:2576-2593 */
@@ -805,9 +829,28 @@ int32_t oracle_electromag2(oracle_run *r)
                 Ez[c] = Ez_old[c] + (-aEz[c] * dHy_dz + bEz[c] * dHy_dx - dEz[c] * Ex_old[c] - eEz[c] * Ez_old[c]) * dt;
                 Ex[c] = Ex_old[c] + (-aEx[c] * dHy_dz + bEx[c] * dHy_dx - dEx[c] * Ex_old[c] - eEx[c] * Ez_old[c]) * dt;
             }
+        if (r->pw_kind) {
+            /* plane wave, cpmlfdtd.f08:2626-2673 */
+            const double t = it * dt;
+            for (int f = 0; f < 3; f += 2) {
+                if (!pw.face[f]) continue;
+                const int lo = r->cpml + 2, hi = (f == 0 ? nz : nx) - r->cpml - 1;
+                for (int ip = lo; ip <= hi; ++ip) {
+                    const int i = f == 0 ? pw.face[0] : ip, j = f == 0 ? ip : pw.face[2];
+                    const double x[3] = {f == 0 ? pw.loc[0] : ip * dx, 0.0, f == 0 ? ip * dz : pw.loc[2]};
+                    double E[3], H[3];
+                    pw_em_field(r, &pw, x, t, E, H);
+                    const size_t c = I2(i, j);
+                    Ex[c] = Ex[c] + E[0];
+                    Ez[c] = Ez[c] + E[2];
+                    Hy[c] = Hy[c] + H[1];
+                }
+            }
+        } else {
         /* point source, cpmlfdtd.f08:2678-2681 */
         Ex[I2(is, js)] = Ex[I2(is, js)] + src_at(r, SDT_SRC_X, it) * dt / epsilon11[I2(is, js)];
         Ez[I2(is, js)] = Ez[I2(is, js)] + src_at(r, SDT_SRC_Z, it) * dt / epsilon33[I2(is, js)];
+        }
         /* Dirichlet edges, cpmlfdtd.f08:2692-2711 */
         for (int j = 1; j <= nz; ++j) {
             Ex[I2(1, j)] = 0.0; Ex[I2(nx, j)] = 0.0;
@@ -906,6 +949,8 @@ int32_t oracle_electromag3(oracle_run *r)
     memcpy(Ez_old, Ez, n * sizeof(double));
 
     const int is = r->isrc, js = r->jsrc, ks = r->ksrc;
+    pw_state pw;
+    if (r->pw_kind) pw_setup(r, 1, &pw);
     r->unstable_step = 0;
     for (int it = 1; it <= r->nsteps; ++it) {
         /* Hx: cpmlfdtd.f08:3700-3718 */
@@ -981,10 +1026,38 @@ int32_t oracle_electromag3(oracle_run *r)
                     Ey[c] = Ey_old[c] + (aEy[c] * rhs_x + bEy[c] * rhs_y + cEy[c] * rhs_z) * dt;
                     Ez[c] = Ez_old[c] + (aEz[c] * rhs_x + bEz[c] * rhs_y + cEz[c] * rhs_z) * dt;
                 }
+        if (r->pw_kind) {
+            /* plane wave, cpmlfdtd.f08:3833-3903: faces x, y, z, cells
+             * cpml+3 .. n-cpml-2 along the face */
+            const double t = it * dt;
+            const int nn[3] = {nx, ny, nz};
+            const double hh[3] = {dx, dy, dz};
+            for (int f = 0; f < 3; ++f) {
+                if (!pw.face[f]) continue;
+                const int a = f == 0 ? 1 : 0, b = f == 2 ? 1 : 2;
+                for (int ia = r->cpml + 3; ia <= nn[a] - r->cpml - 2; ++ia)
+                    for (int ib = r->cpml + 3; ib <= nn[b] - r->cpml - 2; ++ib) {
+                        int ijk[3];
+                        double x[3];
+                        ijk[f] = pw.face[f];
+                        x[f] = pw.loc[f];
+                        ijk[a] = ia;
+                        x[a] = ia * hh[a];
+                        ijk[b] = ib;
+                        x[b] = ib * hh[b];
+                        double E[3], H[3];
+                        pw_em_field(r, &pw, x, t, E, H);
+                        const size_t c = I3(ijk[0], ijk[1], ijk[2]);
+                        Ex[c] = Ex[c] + E[0]; Ey[c] = Ey[c] + E[1]; Ez[c] = Ez[c] + E[2];
+                        Hx[c] = Hx[c] + H[0]; Hy[c] = Hy[c] + H[1]; Hz[c] = Hz[c] + H[2];
+                    }
+            }
+        } else {
         /* point source, cpmlfdtd.f08:3909-3914 (eps already scaled by eps0) */
         Ex[I3(is, js, ks)] = Ex[I3(is, js, ks)] + src_at(r, SDT_SRC_X, it) * dt / eps11[I3(is, js, ks)];
         Ey[I3(is, js, ks)] = Ey[I3(is, js, ks)] + src_at(r, SDT_SRC_Y, it) * dt / eps22[I3(is, js, ks)];
         Ez[I3(is, js, ks)] = Ez[I3(is, js, ks)] + src_at(r, SDT_SRC_Z, it) * dt / eps33[I3(is, js, ks)];
+        }
         /* Dirichlet faces for E and H, cpmlfdtd.f08:3921-3961 */
 #pragma omp parallel for collapse(2) schedule(static)
         for (int k = 1; k <= nz; ++k)

@@ -38,13 +38,15 @@ typedef struct oracle_run {
     int32_t unstable_step;       /* out: 0, or the 1-based step that blew up      */
     int32_t nthreads;            /* OpenMP threads, 0 = runtime default           */
     double ky;                   /* 2.5D elastic: y wavenumber (env KY)           */
-    /* plane-wave source of the elastic solvers (replaces the point source) */
+    /* plane-wave source (replaces the point source); the EM solvers take
+     * SDT_PW_S only (source type 'pw') */
     int32_t pw_kind;             /* 0 point source, SDT_PW_S, SDT_PW_P            */
     int32_t pw_wavelet;          /* 0 gaus0, 1 gaus1, 2 gaus2                     */
     int32_t cpml;                /* absorbing-layer width (face positions)        */
     double pw_angles[3];         /* x_z, x_y, y_z rotation (degrees)              */
     double pw_f0, pw_amp;
     double pw_speed;             /* background speed (cbackground), from the caller */
+    double pw_eta;               /* EM: background impedance (eta), from the caller */
 } oracle_run;
 
 /* 2.5D elastic (seismic25): fields, field_in/field_out and the ic_x..ic_z

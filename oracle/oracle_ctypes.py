@@ -50,6 +50,7 @@ class OracleRun(C.Structure):
         ("ky", C.c_double),
         ("pw_kind", C.c_int32), ("pw_wavelet", C.c_int32), ("cpml", C.c_int32),
         ("pw_angles", C.c_double * 3), ("pw_f0", C.c_double), ("pw_amp", C.c_double), ("pw_speed", C.c_double),
+        ("pw_eta", C.c_double),
     ]
 
 
@@ -77,6 +78,39 @@ def background_speed(arrays, two_d):
         K[:, a, b] = v
         K[:, b, a] = v
     return float(np.sqrt(np.linalg.eigvalsh(K).max() / rho.min()))
+
+
+CLIGHT = 2.9979458e8                                 # constants.f08:8
+MU0 = 4.0 * 3.141592653589793238462643 * 1.0e-7       # constants.f08:9
+EPS0 = 8.85418782e-12                                 # constants.f08:10
+
+
+def em_background(arrays, two_d):
+    """(speed, impedance) of the smallest relative-permittivity eigenvalue
+    over the model: clight / sqrt(eps_min), sqrt(mu0 / (eps0 eps_min))
+    (cpmlfdtd.f08:2472-2475, 3577-3580).  2D takes eigenvalues 2..3 of
+    [[e11, e13, 0], [e13, e33, 0], [0, 0, 0]] per cell
+    (tensor_operations.f08:135-154); 3D all three of the full tensor
+    (240-262)."""
+    def a(name):
+        return np.asarray(arrays[name], dtype=np.float64).ravel() if name in arrays else None
+    e11 = a("e11")
+    n = e11.size
+    z = np.zeros(n)
+    if two_d:
+        K = np.zeros((n, 3, 3))
+        K[:, 0, 0] = e11
+        K[:, 0, 1] = K[:, 1, 0] = a("e13") if a("e13") is not None else z
+        K[:, 1, 1] = a("e33")
+        emin = np.linalg.eigvalsh(K)[:, 1:].min()
+    else:
+        K = np.zeros((n, 3, 3))
+        for (i, j), nm in zip([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)],
+                              ["e11", "e12", "e13", "e22", "e23", "e33"]):
+            v = a(nm)
+            K[:, i, j] = K[:, j, i] = v if v is not None else z
+        emin = np.linalg.eigvalsh(K).min()
+    return float(CLIGHT / np.sqrt(emin)), float(np.sqrt(MU0 / (EPS0 * emin)))
 
 
 def build():
@@ -167,7 +201,10 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
             r.pw_angles[q] = float(pw["angles"][q])
         r.pw_f0 = float(pw["f0"])
         r.pw_amp = float(pw["amplitude"])
-        r.pw_speed = background_speed(cfg["arrays"], phys == "elastic2")
+        if phys in ("em2", "em3"):
+            r.pw_speed, r.pw_eta = em_background(cfg["arrays"], phys == "em2")
+        else:
+            r.pw_speed = background_speed(cfg["arrays"], phys == "elastic2")
     r.stencil = 1 if cfg.get("stencil", "reference") == "corrected" else 0
     r.nthreads = nthreads
     keep = []

@@ -33,6 +33,7 @@ struct Em2Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t src_cell;
+    PwFaces pwf;        // plane-wave face cells, excluded from the fused norm
     int nsteps;         // time_steps of the run (batched shots stop there)
     // segment recomputation for the adjoint: per step (0-based t), slot
     // t - store_first of `store` holds E_x, E_z before the update and the
@@ -76,7 +77,8 @@ __device__ __forceinline__ void em2_h_cell(const Em2Args<T> &A, int i, int j)
 }
 
 // Ex, Ez (cpmlfdtd.f08:2605-2620), zero edges, and the squared field
-// magnitude for the per-step norm, 0 at the source cell (the norm is the
+// magnitude for the per-step norm, 0 at the source cell and the plane-wave
+// face cells (their values change after this kernel; the norm is the
 // square root of the maximum, which equals the maximum of the magnitudes).
 // STORE: also write the values the adjoint gradient needs (see Em2Args).
 template <typename T, bool PML, bool STORE>
@@ -127,7 +129,7 @@ __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
     }
     A.Ex[c] = ex;
     A.Ez[c] = ez;
-    if (c == A.src_cell) return 0.0;
+    if (c == A.src_cell || pw_face(A.pwf, i, 1, j)) return 0.0;
     const double x = ex, z = ez;
     return x * x + z * z;
 }
@@ -166,6 +168,7 @@ static Em2Args<T> em2_args(Device &d)
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
+    A.pwf = d.pw.faces();
     A.nsteps = d.time_steps;
     A.store = (T *)d.adj.seg_fwd;
     A.stride = d.ncell_dev;
@@ -314,6 +317,7 @@ static void em2_step_t(Device &d, bool timed)
     else
         hipLaunchKernelGGL((em2_e_kernel<T, R, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_plane_wave(d);
     launch_tail(d);
 }
 

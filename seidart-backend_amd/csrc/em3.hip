@@ -33,6 +33,7 @@ struct Em3Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t src_cell;
+    PwFaces pwf;        // plane-wave face cells, excluded from the fused norm
     // material table (at most 256 distinct materials): id per padded cell,
     // rows of kEmRow = aEx bEx cEx bEy cEy cEz s11 s12 s13 s22 s23 s33
     const uint8_t *mid;
@@ -201,7 +202,7 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
         }
     }
     A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
-    if (c == A.src_cell) return 0.0;
+    if (c == A.src_cell || pw_face(A.pwf, i, j, k)) return 0.0;
     const double x = ex, y = ey, z = ez;
     return sqrt(x * x + y * y + z * z);
 }
@@ -256,6 +257,7 @@ static Em3Args<T> em3_args(Device &d)
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
+    A.pwf = d.pw.faces();
     return A;
 }
 
@@ -270,6 +272,7 @@ static void em3_step_t(Device &d, bool timed)
     if (d.mat_table) hipLaunchKernelGGL((em3_e_kernel<T, true>), grid, block, 0, d.stream, A);
     else hipLaunchKernelGGL((em3_e_kernel<T, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
+    launch_plane_wave(d);
     launch_tail(d);
 }
 

@@ -1,18 +1,20 @@
-// pw.hip -- plane-wave sources of the elastic solvers (seismic2, seismic25,
-// seismic3): an analytic plane wave (plane_wave_source.f08) injected every
-// step on the interior faces its direction selects, in place of the point
-// source (cpmlfdtd.f08:188-232 / 454-512, 802-832 / 1208-1262, 1548-1605 /
-// 2133-2220).
+// pw.hip -- plane-wave sources: an analytic plane wave (plane_wave_source.f08)
+// injected every step on the interior faces its direction selects, in place
+// of the point source.  Elastic (seismic2, seismic25, seismic3:
+// cpmlfdtd.f08:188-232 / 454-512, 802-832 / 1208-1262, 1548-1605 /
+// 2133-2220) and electromagnetic (electromag2, electromag3: 2436-2475 /
+// 2626-2673, 3523-3580 / 3833-3903).
 //
 // Host: propagation and polarisation vectors from the three rotation angles,
-// the active faces and reference point, and the background speed
-// sqrt(max Kelvin-form stiffness eigenvalue / min density) over the model
-// (a cyclic Jacobi eigensolver; cells repeating the previous cell's tensor
-// reuse its eigenvalue).  Device: one thread per face cell evaluates the
-// wave (wavelet, velocity, strain) and adds the velocity and C : strain to
-// the fields; a second pass, after every face is done, folds the injected
-// cells' velocity (and 3D stress) magnitudes into the step's norms, which
-// the fused kernels skip for these cells.
+// the active faces and reference point, and the background speed over the
+// model: sqrt(max Kelvin-form stiffness eigenvalue / min density), or
+// c / sqrt(min permittivity eigenvalue) with the matching impedance (a cyclic
+// Jacobi eigensolver; cells repeating the previous cell's tensor reuse its
+// eigenvalue).  Device: one thread per face cell evaluates the wave and adds
+// the velocity and C : strain (or E and H) to the fields; a second pass,
+// after every face is done, folds the injected cells' velocity / E (and 3D
+// stress) magnitudes into the step's norms, which the fused kernels skip for
+// these cells.
 #include <math.h>
 #include <string.h>
 
@@ -79,8 +81,9 @@ static void direction_vectors(double axz, double axy, double ayz, double p[3], d
     for (int i = 0; i < 3; ++i) e[i] = n > 0.0 ? t[i] / n : 0.0;
 }
 
-// largest eigenvalue of a symmetric n x n matrix (n <= 6), cyclic Jacobi
-static double jacobi_max_eig(double A[6][6], int n)
+// eigenvalues of a symmetric n x n matrix (n <= 6) on its diagonal after
+// cyclic Jacobi rotations
+static void jacobi_diagonalise(double A[6][6], int n)
 {
     for (int sweep = 0; sweep < 64; ++sweep) {
         double off = 0.0, diag = 0.0;
@@ -108,8 +111,21 @@ static double jacobi_max_eig(double A[6][6], int n)
                 }
             }
     }
+}
+
+static double jacobi_max_eig(double A[6][6], int n)
+{
+    jacobi_diagonalise(A, n);
     double m = A[0][0];
     for (int i = 1; i < n; ++i) m = A[i][i] > m ? A[i][i] : m;
+    return m;
+}
+
+static double jacobi_min_eig(double A[6][6], int n)
+{
+    jacobi_diagonalise(A, n);
+    double m = A[0][0];
+    for (int i = 1; i < n; ++i) m = A[i][i] < m ? A[i][i] : m;
     return m;
 }
 
@@ -135,12 +151,40 @@ static double max_kelvin_eig(const double *c21, bool two_d)
     return jacobi_max_eig(A, 6);
 }
 
-// max over cells of the Kelvin eigenvalue; threads over chunks of cells,
-// each reusing the previous cell's value when the tensor repeats
-static double max_stiffness_eig(const double *const *in, int64_t n, bool two_d)
+// Permittivity bounds (relative).  2D: the reference takes eigenvalues 2..3
+// (ascending) of [[e11, e13, 0], [e13, e33, 0], [0, 0, 0]]
+// (tensor_operations.f08:135-154), so a cell's smallest is the middle one of
+// {0, lambda-, lambda+}.  3D: the smallest eigenvalue of the full tensor
+// (tensor_operations.f08:240-262).
+static double min_eps_eig2(const double *e)   // e = e11, e13, e33
+{
+    const double m = 0.5 * (e[0] + e[2]), h = 0.5 * (e[0] - e[2]);
+    const double r = sqrt(h * h + e[1] * e[1]);
+    const double lo = m - r, hi = m + r;
+    return lo >= 0.0 ? lo : (hi <= 0.0 ? hi : 0.0);
+}
+
+static double min_eps_eig3(const double *e)   // e = e11 e12 e13 e22 e23 e33
+{
+    double A[6][6] = {};
+    A[0][0] = e[0];
+    A[0][1] = A[1][0] = e[1];
+    A[0][2] = A[2][0] = e[2];
+    A[1][1] = e[3];
+    A[1][2] = A[2][1] = e[4];
+    A[2][2] = e[5];
+    return jacobi_min_eig(A, 3);
+}
+
+// Reduction (max or min) of eval(tensor) over all cells, the tensor's
+// components read from `ncol` input arrays (null = 0); threads over chunks of
+// cells, each reusing the previous cell's value when the tensor repeats.
+template <typename F>
+static double cell_reduce(const double *const *col, int ncol, int64_t n, bool want_max, F eval)
 {
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<double> best((size_t)nt, 0.0);
+    std::vector<char> any((size_t)nt, 0);
     std::vector<std::thread> th;
     for (int w = 0; w < nt; ++w)
         th.emplace_back([&, w]() {
@@ -148,26 +192,36 @@ static double max_stiffness_eig(const double *const *in, int64_t n, bool two_d)
             double prev[21], cur[21], val = 0.0, m = 0.0;
             bool have = false;
             for (int64_t c = b; c < e; ++c) {
-                for (int q = 0; q < 21; ++q) cur[q] = in[SDT_C11 + q] ? in[SDT_C11 + q][c] : 0.0;
-                if (!have || memcmp(cur, prev, sizeof(cur)) != 0) {
-                    val = max_kelvin_eig(cur, two_d);
-                    memcpy(prev, cur, sizeof(cur));
+                for (int q = 0; q < ncol; ++q) cur[q] = col[q] ? col[q][c] : 0.0;
+                if (!have || memcmp(cur, prev, sizeof(double) * (size_t)ncol) != 0) {
+                    val = eval(cur);
+                    memcpy(prev, cur, sizeof(double) * (size_t)ncol);
+                    if (!have) m = val;
                     have = true;
                 }
-                m = val > m ? val : m;
+                m = want_max ? (val > m ? val : m) : (val < m ? val : m);
             }
             best[(size_t)w] = m;
+            any[(size_t)w] = have;
         });
     for (auto &t : th) t.join();
-    return *std::max_element(best.begin(), best.end());
+    bool have = false;
+    double m = 0.0;
+    for (int w = 0; w < nt; ++w) {
+        if (!any[(size_t)w]) continue;
+        const double v = best[(size_t)w];
+        m = !have ? v : (want_max ? std::max(m, v) : std::min(m, v));
+        have = true;
+    }
+    return m;
 }
 
 int plane_wave_setup(Device &d, const PlaneWaveSpec &spec, const double *const *in)
 {
     PlaneWave &w = d.pw;
     w = PlaneWave();
-    const bool d3 = d.physics == SDT_ELASTIC3;
-    const bool two_d = d.physics == SDT_ELASTIC2;
+    const bool em = d.physics == SDT_EM2 || d.physics == SDT_EM3;
+    const bool d3 = d.physics == SDT_ELASTIC3 || d.physics == SDT_EM3;
     double p[3], e[3];
     direction_vectors(spec.angle_xz, spec.angle_xy, spec.angle_yz, p, e);
     if (spec.kind == SDT_PW_P)
@@ -176,6 +230,9 @@ int plane_wave_setup(Device &d, const PlaneWaveSpec &spec, const double *const *
     const int n[3] = {d.nx, d.ny, d.nz};
     const double h[3] = {d.dx, d.dy, d.dz};
     const int cp = d.cpml_width;
+    // cell ranges along a face: cpml+2 .. n-cpml-1, except the 3D EM solver's
+    // cpml+3 .. n-cpml-2 (cpmlfdtd.f08:3569-3574)
+    const int inset = d.physics == SDT_EM3 ? 1 : 0;
     for (int a = 0; a < 3; ++a) {
         const double lo = h[a] * (1 + cp), hi = h[a] * (n[a] - cp), mid = 0.5 * (lo + hi);
         w.loc[a] = mid;
@@ -185,8 +242,8 @@ int plane_wave_setup(Device &d, const PlaneWaveSpec &spec, const double *const *
         if (axis && p[a] < -eps) w.loc[a] = hi;
         if (axis && (p[a] > eps || p[a] < -eps)) w.face[a] = (int)lround(w.loc[a] / h[a]);
         w.r0[a] = (a == 1 && !d3) ? 0.0 : w.loc[a];
-        w.lo[a] = cp + 2;
-        w.hi[a] = n[a] - cp - 1;
+        w.lo[a] = cp + 2 + inset;
+        w.hi[a] = n[a] - cp - 1 - inset;
         w.h[a] = h[a];
         w.p[a] = p[a];
         w.e[a] = e[a];
@@ -195,16 +252,42 @@ int plane_wave_setup(Device &d, const PlaneWaveSpec &spec, const double *const *
         w.lo[1] = w.hi[1] = 1;
         w.loc[1] = 0.0;   // r(2) = 0 in the 2D and 2.5D injection loops
     }
-    double rho_min = 0.0;
-    bool first = true;
-    for (int64_t c = 0; c < d.ncell; ++c) {
-        const double r = in[SDT_RHO] ? in[SDT_RHO][c] : 0.0;
-        if (first || r < rho_min) rho_min = r;
-        first = false;
+    if (em) {
+        // boundary_em_field (plane_wave_source.f08:163-194): speed and
+        // impedance of the smallest permittivity eigenvalue over the model
+        // (cpmlfdtd.f08:2472-2475, 3577-3580); H along p x e normalised
+        double emin;
+        if (d3) {
+            const double *col[6] = {in[SDT_E11], in[SDT_E12], in[SDT_E13], in[SDT_E22], in[SDT_E23], in[SDT_E33]};
+            emin = cell_reduce(col, 6, d.ncell, false, min_eps_eig3);
+        } else {
+            const double *col[3] = {in[SDT_E11], in[SDT_E13], in[SDT_E33]};
+            emin = cell_reduce(col, 3, d.ncell, false, min_eps_eig2);
+        }
+        if (!(emin > 0.0)) return SDT_ERR_ARG;
+        w.v = kClight / sqrt(emin);
+        w.eta = sqrt(kMu0 / (kEps0 * emin));
+        w.em = 1;
+        const double c[3] = {p[1] * e[2] - p[2] * e[1], p[2] * e[0] - p[0] * e[2], p[0] * e[1] - p[1] * e[0]};
+        const double cn = norm3(c);
+        for (int i = 0; i < 3; ++i) w.hh[i] = cn > 0.0 ? c[i] / cn : c[i];
+    } else {
+        double rho_min = 0.0;
+        bool first = true;
+        for (int64_t c = 0; c < d.ncell; ++c) {
+            const double r = in[SDT_RHO] ? in[SDT_RHO][c] : 0.0;
+            if (first || r < rho_min) rho_min = r;
+            first = false;
+        }
+        const double *col[21];
+        for (int q = 0; q < 21; ++q) col[q] = in[SDT_C11 + q];
+        const bool two_d = d.physics == SDT_ELASTIC2;
+        const double lmax = cell_reduce(col, 21, d.ncell, true, [two_d](const double *c21) {
+            return max_kelvin_eig(c21, two_d);
+        });
+        if (!(rho_min > 0.0) || !(lmax > 0.0)) return SDT_ERR_ARG;
+        w.v = sqrt(lmax / rho_min);
     }
-    const double lmax = max_stiffness_eig(in, d.ncell, two_d);
-    if (!(rho_min > 0.0) || !(lmax > 0.0)) return SDT_ERR_ARG;
-    w.v = sqrt(lmax / rho_min);
     w.f0 = spec.f0;
     w.amp = spec.amplitude;
     w.dt = d.dt;
@@ -233,6 +316,7 @@ struct PwArgs {
     int64_t xoff;
     int cplx;             // 1: complex fields, the wave adds to the real parts
     int dim;              // 2: seismic2 (strain 1, 3, 5 with six constants), 3: full
+    int three_d;          // 1: grid edges on all six sides (y too)
     const int *d_it;
     unsigned long long *vnorm, *snorm;
 };
@@ -283,15 +367,29 @@ __global__ void __launch_bounds__(256) pw_inject_kernel(PwArgs<T> A, int f)
     int i, j, k;
     if (!pw_cell(w, f, blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
     const int64_t c = pw_index(A, i, j, k);
-    // boundary_seismic_field (plane_wave_source.f08:196-230)
+    // A face on the grid edge (no absorbing layer): the Dirichlet step that
+    // follows the injection in the reference zeroes the velocities (E and H)
+    // there again, which the solvers' kernels have already done.
+    const bool edge = i == 1 || i == A.nx || k == 1 || k == A.nz || (A.three_d && (j == 1 || j == A.ny));
+    // boundary_seismic_field / boundary_em_field (plane_wave_source.f08:163-230)
     const double t = (double)(*A.d_it) * w.dt;
     // position of the cell: the face coordinate on the face's axis, index *
     // spacing on the others (y = 0 in 2D and 2.5D)
     const double r[3] = {f == 0 ? w.loc[0] : (double)i * w.h[0],
-                         (f == 1 || A.dim != 3) ? w.loc[1] : (double)j * w.h[1],
+                         (f == 1 || !A.three_d) ? w.loc[1] : (double)j * w.h[1],
                          f == 2 ? w.loc[2] : (double)k * w.h[2]};
     const double tau = t - (w.p[0] * (r[0] - w.r0[0]) + w.p[1] * (r[1] - w.r0[1]) + w.p[2] * (r[2] - w.r0[2])) / w.v;
     const double s = w.amp * pw_wavelet(tau, w.f0, w.wavelet);
+    if (w.em) {   // E += s e, H += (s / eta) hh (v = E fields, s[0..2] = H fields)
+        if (edge) return;
+        const double sh = s / w.eta;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            if (A.v[m]) pw_add(A, A.v[m], c, s * w.e[m]);
+            if (A.s[m]) pw_add(A, A.s[m], c, sh * w.hh[m]);
+        }
+        return;
+    }
     const double sc = -s / w.v;
     const double *e = w.e, *p = w.p;
     const double st[6] = {sc * e[0] * p[0], sc * e[1] * p[1], sc * e[2] * p[2], sc * (e[1] * p[2] + e[2] * p[1]),
@@ -305,9 +403,11 @@ __global__ void __launch_bounds__(256) pw_inject_kernel(PwArgs<T> A, int f)
 #pragma unroll
         for (int q = 0; q < 21; ++q) C[q] = A.C[q] ? (double)A.C[q][c] : 0.0;
     }
-    pw_add(A, A.v[0], c, s * e[0]);
-    if (A.v[1]) pw_add(A, A.v[1], c, s * e[1]);
-    pw_add(A, A.v[2], c, s * e[2]);
+    if (!edge) {
+        pw_add(A, A.v[0], c, s * e[0]);
+        if (A.v[1]) pw_add(A, A.v[1], c, s * e[1]);
+        pw_add(A, A.v[2], c, s * e[2]);
+    }
     if (A.dim == 2) {   // cpmlfdtd.f08:473-481
         pw_add(A, A.s[0], c, C[0] * st[0] + C[2] * st[2] + C[4] * st[4]);
         pw_add(A, A.s[2], c, C[2] * st[0] + C[11] * st[2] + C[13] * st[4]);
@@ -371,7 +471,15 @@ static void launch_plane_wave_t(Device &d)
     A.mid = nullptr;
     A.tab = nullptr;
     A.tab_row = 0;
-    if (d.physics == SDT_ELASTIC2) {
+    A.dim = 3;
+    if (d.pw.em) {   // E fields in v, H fields in s[0..2]
+        for (int m = 0; m < 3; ++m) {
+            A.v[m] = (T *)d.field[SDT_F_EX + m];
+            A.s[m] = (T *)d.field[SDT_F_HX + m];
+            A.s[m + 3] = nullptr;
+        }
+        if (d.physics == SDT_EM2) A.v[1] = A.s[0] = A.s[2] = nullptr;   // Ex, Ez, Hy
+    } else if (d.physics == SDT_ELASTIC2) {
         A.v[1] = nullptr;
         A.s[1] = A.s[3] = A.s[5] = nullptr;
         const int slot[6] = {0, 2, 4, 11, 13, 18};   // c11 c13 c15 c33 c35 c55
@@ -392,6 +500,7 @@ static void launch_plane_wave_t(Device &d)
     A.pitch = d.pitch;
     A.xoff = d.xoff;
     A.cplx = d.ncomp == 2 ? 1 : 0;
+    A.three_d = d.physics == SDT_ELASTIC3 || d.physics == SDT_EM3;
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     A.snorm = d3 ? d.norms + d.time_steps : nullptr;

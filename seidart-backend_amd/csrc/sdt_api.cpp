@@ -274,7 +274,9 @@ static int prepare(sdt_handle *h)
     d.coef_in = nullptr;
     if (rc) return fail(h, rc, "device setup failed (out of device memory?)");
     if (h->pw.kind && plane_wave_setup(d, h->pw, in) != SDT_OK)
-        return fail(h, SDT_ERR_ARG, "plane-wave source: the model needs positive density and stiffness");
+        return fail(h, SDT_ERR_ARG, (d.physics == SDT_EM2 || d.physics == SDT_EM3)
+                                        ? "plane-wave source: the permittivity needs a positive smallest eigenvalue"
+                                        : "plane-wave source: the model needs positive density and stiffness");
     if (d.raw_source)
         for (int q = 0; q < d.tail.nterms; ++q) {
             d.tail.terms[q].mul = 1.0;
@@ -633,9 +635,11 @@ int32_t sdt_set_plane_wave(sdt_handle *h, int32_t kind, double angle_xz, double 
     if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
     if (h->prepared) return fail(h, SDT_ERR_STATE, "inputs are fixed once the first step has run");
     const int ph = h->d.physics;
-    if (ph != SDT_ELASTIC2 && ph != SDT_ELASTIC25 && ph != SDT_ELASTIC3)
-        return fail(h, SDT_ERR_ARG, "plane-wave sources belong to the elastic solvers");
+    if (ph == SDT_EM25)   // cpmlfdtd.f08:2937-2996 sets up no injection for it
+        return fail(h, SDT_ERR_ARG, "the 2.5D electromagnetic solver has no plane-wave source");
     if (kind != SDT_PW_S && kind != SDT_PW_P) return fail(h, SDT_ERR_ARG, "plane-wave kind must be SDT_PW_S or SDT_PW_P");
+    if ((ph == SDT_EM2 || ph == SDT_EM3) && kind != SDT_PW_S)   // source type 'pw' only
+        return fail(h, SDT_ERR_ARG, "electromagnetic plane waves are transverse: kind SDT_PW_S");
     if (!(f0 > 0) || !isfinite(f0) || !isfinite(amplitude) || !isfinite(angle_xz) || !isfinite(angle_xy) ||
         !isfinite(angle_yz))
         return fail(h, SDT_ERR_ARG, "plane-wave parameters must be finite, f0 > 0");

@@ -13,8 +13,9 @@ namespace sdt {
 
 constexpr int kMaxMem = 32;     // memory-variable arrays per solver (3D elastic: 29)
 constexpr int kMaxMat = 32;     // device material arrays per solver (3D elastic: 28)
-constexpr double kEps0 = 8.85418782e-12;                  // constants.f08:11
-constexpr double kMu0 = 4.0 * 3.141592653589793238462643 * 1.0e-7;  // constants.f08:10
+constexpr double kEps0 = 8.85418782e-12;                  // constants.f08:10
+constexpr double kMu0 = 4.0 * 3.141592653589793238462643 * 1.0e-7;  // constants.f08:9
+constexpr double kClight = 2.9979458e8;                    // constants.f08:8
 constexpr double kStability = 1.0e25;                    // constants.f08:12
 
 // One point-source injection: field[cell] += (series[it] * mul) / div.
@@ -110,6 +111,9 @@ struct PlaneWave {
     double h[3] = {};              // cell sizes
     double v = 0, f0 = 0, amp = 0, dt = 0;   // background speed, wavelet, step
     int wavelet = 0;
+    int em = 0;                    // 1: electromagnetic (E along e, H along hh)
+    double eta = 0;                // background impedance (ohm)
+    double hh[3] = {};             // unit p x e
     PwFaces faces() const
     {
         PwFaces f;

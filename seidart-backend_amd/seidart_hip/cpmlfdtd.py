@@ -23,9 +23,16 @@ Differences from the reference, by design:
 * electromag3 writes its 3D snapshots with the write_image3 format; the
   reference's call to write_image stops at the end of step 1 for 3D grids
   (seidartio.f08:315-324);
-* plane-wave sources ('pw', 'pws', 'pwp', ...) are not supported yet and raise
-  NotImplementedError (electromag25's 'pw' branch leaves its point-source
-  series uninitialised in the reference, cpmlfdtd.f08:2992-2996).
+* electromag25 with source type 'pw' raises ValueError: the reference sets up
+  the plane-wave geometry but has no injection loop for it and adds its
+  never-loaded point-source series instead (cpmlfdtd.f08:2937-2996,
+  3260-3266).
+
+Plane-wave sources: the elastic solvers take 'pw', 'pws', 'pw_s', 'pw-s'
+(S) and 'pwp', 'pw_p', 'pw-p' (P) after trimming (plane_wave_source.f08:
+20-35); electromag2 / electromag3 take exactly 'pw' (trailing blanks
+ignored, Fortran string equality, cpmlfdtd.f08:2436, 3523) and run any other
+type as a point source.  No source series is read for a plane wave.
 """
 from __future__ import annotations
 
@@ -38,6 +45,8 @@ from . import fortran_io as fio
 from .solver import Solver
 
 PLANE_WAVE_TYPES = {"pw", "pws", "pw_s", "pw-s", "pwp", "pw_p", "pw-p"}   # plane_wave_source.f08:20-27
+P_WAVE_TYPES = {"pwp", "pw_p", "pw-p"}                                  # plane_wave_source.f08:29-35
+WAVELETS = {"gaus0": 0, "gaus1": 1, "gaus2": 2}                         # plane_wave_source.f08:150-160
 
 
 @dataclasses.dataclass
@@ -83,11 +92,38 @@ def _series(workdir, name, n):
     return fio.read_record(os.path.join(workdir, name + ".dat"), np.float64, n)
 
 
-def _check_source_type(source):
-    st = (source.source_type or "").strip()
-    if st in PLANE_WAVE_TYPES:
-        raise NotImplementedError("plane-wave sources are not implemented in seidart_hip yet")
-    return st
+def _source_type(source):
+    return (source.source_type or "").strip()
+
+
+def _wavelet(source):
+    """`select case` on the wavelet name; anything else is gaus0
+    (plane_wave_source.f08:150-160)."""
+    return WAVELETS.get((source.source_wavelet or "").rstrip(), 0)
+
+
+def _plane_wave(s, source, kind):
+    s.set_plane_wave(kind, (source.x_z_rotation, source.x_y_rotation, source.y_z_rotation),
+                     source.source_frequency, source.amplitude, _wavelet(source))
+
+
+def _elastic_plane_wave(s, source):
+    """Elastic solvers: a plane wave when the trimmed source type is one of
+    PLANE_WAVE_TYPES (is_plane_wave_source); True when it is."""
+    st = _source_type(source)
+    if st not in PLANE_WAVE_TYPES:
+        return False
+    _plane_wave(s, source, "p" if st in P_WAVE_TYPES else "s")
+    return True
+
+
+def _em_plane_wave(s, source):
+    """EM solvers: a plane wave when source_type == 'pw' (Fortran equality:
+    trailing blanks ignored); True when it is."""
+    if (source.source_type or "").rstrip() != "pw":
+        return False
+    _plane_wave(s, source, "s")
+    return True
 
 
 def _cpml_inputs(s, workdir, shape):
@@ -116,7 +152,7 @@ def seismic2(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *,
     """2D elastic; inputs c11 c13 c15 c33 c35 c55 rho gamma_{x,z,xz} CPML
     initialconditionV{x,z} and seismicsource{x,z} ('ac') or
     seismicsource{xx,xz,zz} (cpmlfdtd.f08:159-266); writes Vx/Vz snapshots."""
-    st = _check_source_type(source)
+    st = _source_type(source)
     shape = (domain.nx, domain.nz)
     with Solver("elastic2", shape, (domain.dx, domain.dz), domain.cpml, source.dt, source.time_steps,
                 (source.xind, source.zind), precision=precision, density_method=density_method.strip(),
@@ -126,9 +162,10 @@ def seismic2(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *,
         _cpml_inputs(s, workdir, shape)
         s.set_array("ic_x", _read(workdir, "initialconditionVx", shape))
         s.set_array("ic_z", _read(workdir, "initialconditionVz", shape))
-        comps = ("x", "z") if st == "ac" else ("xx", "xz", "zz")
-        for c in comps:
-            s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
+        if not _elastic_plane_wave(s, source):
+            comps = ("x", "z") if st == "ac" else ("xx", "xz", "zz")
+            for c in comps:
+                s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
         _time_loop(s, workdir, source, (("vx", "Vx"), ("vz", "Vz")), SINGLE_OUTPUT, False, snapshot_every)
 
 
@@ -137,7 +174,7 @@ def seismic3(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *,
     """3D elastic; inputs c11..c66 rho gamma_* CPML initialconditionV{x,y,z}
     and seismicsource{x,y,z} ('ac') or seismicsource{xx,xy,xz,yy,yz,zz}
     (cpmlfdtd.f08:1501-1645); writes Vx/Vy/Vz snapshots and the norm files."""
-    st = _check_source_type(source)
+    st = _source_type(source)
     shape = (domain.nx, domain.ny, domain.nz)
     with Solver("elastic3", shape, (domain.dx, domain.dy, domain.dz), domain.cpml, source.dt,
                 source.time_steps, (source.xind, source.yind, source.zind), precision=precision,
@@ -149,9 +186,10 @@ def seismic3(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *,
         _cpml_inputs(s, workdir, shape)
         for c in "xyz":
             s.set_array("ic_" + c, _read(workdir, "initialconditionV" + c, shape))
-        comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
-        for c in comps:
-            s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
+        if not _elastic_plane_wave(s, source):
+            comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
+            for c in comps:
+                s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
         _time_loop(s, workdir, source, (("vx", "Vx"), ("vy", "Vy"), ("vz", "Vz")), SINGLE_OUTPUT, True,
                    snapshot_every)
         vn, sn = s.norms()
@@ -163,7 +201,6 @@ def electromag2(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
                 snapshot_every=1):
     """2D EM; inputs e11 e13 e33 s11 s13 s33 CPML initialconditionE{x,z} and
     electromagneticsource{x,z} (cpmlfdtd.f08:2413-2506); writes Ex/Ez."""
-    _check_source_type(source)
     shape = (domain.nx, domain.nz)
     with Solver("em2", shape, (domain.dx, domain.dz), domain.cpml, source.dt, source.time_steps,
                 (source.xind, source.zind), precision=precision, device=device) as s:
@@ -172,8 +209,9 @@ def electromag2(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
         _cpml_inputs(s, workdir, shape)
         s.set_array("ic_x", _read(workdir, "initialconditionEx", shape))
         s.set_array("ic_z", _read(workdir, "initialconditionEz", shape))
-        for c in ("x", "z"):
-            s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
+        if not _em_plane_wave(s, source):
+            for c in ("x", "z"):
+                s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
         _time_loop(s, workdir, source, (("ex", "Ex"), ("ez", "Ez")), SINGLE_OUTPUT, False, snapshot_every)
 
 
@@ -181,7 +219,6 @@ def electromag3(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
                 snapshot_every=1):
     """3D EM; inputs e11..e33 s11..s33 CPML initialconditionE{x,y,z} and
     electromagneticsource{x,y,z} (cpmlfdtd.f08:3481-3618); writes Ex/Ey/Ez."""
-    _check_source_type(source)
     shape = (domain.nx, domain.ny, domain.nz)
     with Solver("em3", shape, (domain.dx, domain.dy, domain.dz), domain.cpml, source.dt,
                 source.time_steps, (source.xind, source.yind, source.zind), precision=precision,
@@ -191,8 +228,9 @@ def electromag3(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
         _cpml_inputs(s, workdir, shape)
         for c in "xyz":
             s.set_array("ic_" + c, _read(workdir, "initialconditionE" + c, shape))
-        for c in "xyz":
-            s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
+        if not _em_plane_wave(s, source):
+            for c in "xyz":
+                s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
         _time_loop(s, workdir, source, (("ex", "Ex"), ("ey", "Ey"), ("ez", "Ez")), SINGLE_OUTPUT, True,
                    snapshot_every)
 
@@ -235,7 +273,7 @@ def seismic25(domain, source, *, workdir=".", precision="fp64", device=0, ky=Non
     Output: S25.<first>-<last>.<xind>.<yind>.<zind>.blk.zst blocks of the
     real parts of vx, vy, vz (default), or with FDTD_LEGACY_OUTPUT=TRUE
     per-step Vx/Vy/Vz images of the real parts (cpmlfdtd.f08:1295-1305)."""
-    st = _check_source_type(source)
+    st = _source_type(source)
     ky = _env_ky() if ky is None else float(ky)
     density_method = _env_density() if density_method is None else density_method
     single = _env_logical("SINGLE_OUTPUT") if SINGLE_OUTPUT is None else bool(SINGLE_OUTPUT)
@@ -258,9 +296,10 @@ def seismic25(domain, source, *, workdir=".", precision="fp64", device=0, ky=Non
         for c in "xyz":
             s.set_array("ic_" + c, fio.read_array(os.path.join(workdir, "initialconditionV" + c + ".dat"),
                                                   shape, np.complex128))
-        comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
-        for c in comps:
-            s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
+        if not _elastic_plane_wave(s, source):
+            comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
+            for c in comps:
+                s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
         for it in range(1, source.time_steps + 1):
             s.run(1)
             vx, vy, vz = s.field("vx"), s.field("vy"), s.field("vz")
@@ -282,7 +321,9 @@ def electromag25(domain, source, *, workdir=".", precision="fp64", device=0, ky=
     (cpmlfdtd.f08:2896-3036).  Output: EM25.<first>-<last>.<xind>.<yind>.
     <zind>.blk.zst blocks of the real parts of Ex, Ey, Ez, or with
     FDTD_LEGACY_OUTPUT=TRUE per-step Ex/Ey/Ez images (cpmlfdtd.f08:3313-3327)."""
-    _check_source_type(source)
+    if (source.source_type or "").rstrip() == "pw":
+        raise ValueError("electromag25 has no plane-wave source: the reference sets up its geometry but injects "
+                         "nothing and adds its never-loaded point-source series (cpmlfdtd.f08:2937-2996)")
     ky = _env_ky() if ky is None else float(ky)
     single = _env_logical("SINGLE_OUTPUT") if SINGLE_OUTPUT is None else bool(SINGLE_OUTPUT)
     if legacy_output is None:

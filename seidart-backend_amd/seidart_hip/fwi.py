@@ -24,7 +24,7 @@ import torch
 from . import _lib as L
 from .solver import Batch, Solver
 
-EPS0 = 8.85418782e-12                        # constants.f08:11
+EPS0 = 8.85418782e-12                        # constants.f08:10
 EM2_COEFFS = ("aEx", "bEx", "bEz", "dEx", "eEx", "dEz", "eEz")
 EL2_COEFFS = ("c11", "c13", "c15", "c33", "c35", "c55", "rho", "igx", "igz", "igxz")
 # source components the raw series feed, in the order of the src rows

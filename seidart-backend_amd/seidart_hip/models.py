@@ -26,8 +26,8 @@ import math
 
 import numpy as np
 
-EPS0 = 8.85418782e-12       # constants.f08:11
-CLIGHT = 2.9979458e8        # constants.f08:9
+EPS0 = 8.85418782e-12       # constants.f08:10
+CLIGHT = 2.9979458e8        # constants.f08:8
 
 VOIGT_NAMES = ["c11", "c12", "c13", "c14", "c15", "c16", "c22", "c23", "c24", "c25", "c26",
                "c33", "c34", "c35", "c36", "c44", "c45", "c46", "c55", "c56", "c66"]
@@ -549,6 +549,14 @@ def write_inputs(cfg, workdir, source_type="ac"):
            "source_frequency": 0.0, "x-z_rotation": 0.0, "x-y_rotation": 0.0, "y-z_rotation": 0.0,
            "amplitude": 1.0, "source_type": source_type, "source_wavelet": "gaus2",
            "dt": float(cfg["dt"]), "time_steps": int(n)}
+    pw = cfg.get("plane_wave")
+    if pw:   # the JSON fields of a plane-wave source (seidartio.f08:82-98)
+        src.update({"x-z_rotation": float(pw["angles"][0]), "x-y_rotation": float(pw["angles"][1]),
+                    "y-z_rotation": float(pw["angles"][2]), "source_frequency": float(pw["f0"]),
+                    "amplitude": float(pw["amplitude"]),
+                    "source_wavelet": ("gaus0", "gaus1", "gaus2")[int(pw.get("wavelet", 0))]})
+        if source_type == "ac":
+            src["source_type"] = "pwp" if pw["kind"] == "p" else "pw"
     doc = {"Domain": {"dim": 3.0 if is3d else (2.5 if is25 else 2.0), "nx": int(nx), "ny": int(ny), "nz": int(nz),
                       "dx": float(dx), "dy": float(dy), "dz": float(dz), "cpml": int(cp), "nmats": 1,
                       "image_file": "model.png"},

@@ -149,10 +149,12 @@ class Solver:
         return np.stack([row.reshape(self.shape, order="F") for row in cg]), sg
 
     def set_plane_wave(self, kind, angles, f0, amplitude, wavelet=0):
-        """Elastic solvers: a plane-wave source in place of the point source.
-        kind "s" (polarised across the direction: source types pw, pws,
-        pw_s, pw-s) or "p" (along it: pwp, pw_p, pw-p); angles = (x_z, x_y,
-        y_z) rotations in degrees; wavelet 0 gaus0, 1 gaus1, 2 gaus2."""
+        """A plane-wave source in place of the point source.  Elastic
+        solvers: kind "s" (polarised across the direction: source types pw,
+        pws, pw_s, pw-s) or "p" (along it: pwp, pw_p, pw-p).  2D and 3D EM:
+        kind "s" only (source type pw; E across the direction, H = E / eta
+        along p x e); the 2.5D EM solver has none.  angles = (x_z, x_y, y_z)
+        rotations in degrees; wavelet 0 gaus0, 1 gaus1, 2 gaus2."""
         k = {"s": L.PW_S, "p": L.PW_P}[kind]
         axz, axy, ayz = (float(a) for a in angles)
         L.check(L.lib.sdt_set_plane_wave(self._h, k, axz, axy, ayz, float(f0), float(amplitude), int(wavelet)),

@@ -149,3 +149,42 @@ def test_cli_electromag25_blocks(tmp_path, monkeypatch):
     for q, key in enumerate(("ex", "ey", "ez")):
         want = f[key].real[6:-6, 6:-6].astype(np.float32)
         assert rel_l2(blk[:, 0, :, q, 3], want) < 1e-6, key
+
+
+@pytest.mark.parametrize("physics,source_type", [("elastic2", "pw_p"), ("em2", "pw"), ("elastic3", " pws "),
+                                                 ("em3", "pw")])
+def test_cli_plane_wave(tmp_path, monkeypatch, physics, source_type):
+    """Plane-wave sources from input.json (rotations, source_frequency,
+    amplitude, source_wavelet) through the CLI, snapshots against the
+    oracle's plane wave."""
+    from seidart_hip import cli, fortran_io as fio, models
+    if physics == "elastic2":
+        cfg = models.elastic2(nx=50, nz=40, cpml=6, steps=60, src=(25, 10), rec_z=10, nrec=4)
+        pw = dict(kind="p", angles=(30.0, 0.0, 0.0), f0=400.0, amplitude=1e-3, wavelet=1)
+        chans = (("Vx", "vx"), ("Vz", "vz"))
+    elif physics == "elastic3":
+        cfg = models.elastic3(nx=20, ny=18, nz=22, cpml=4, steps=40, src=(10, 9, 6), nrec=4)
+        pw = dict(kind="s", angles=(20.0, 30.0, 10.0), f0=2500.0, amplitude=1e-3, wavelet=2)
+        chans = (("Vx", "vx"), ("Vy", "vy"), ("Vz", "vz"))
+    elif physics == "em2":
+        cfg = models.em2(nx=50, nz=40, steps=60, src=(25, 12), nrec=4)
+        pw = dict(kind="s", angles=(-40.0, 0.0, 0.0), f0=300e6, amplitude=1.0, wavelet=0)
+        chans = (("Ex", "ex"), ("Ez", "ez"))
+    else:
+        cfg = models.em3(nx=20, ny=18, nz=22, cpml=4, steps=50, src=(10, 9, 6), nrec=4)
+        pw = dict(kind="s", angles=(30.0, 20.0, 10.0), f0=400e6, amplitude=1.0, wavelet=2)
+        chans = (("Ex", "ex"), ("Ey", "ey"), ("Ez", "ez"))
+    cfg["plane_wave"] = pw
+    models.write_inputs(cfg, str(tmp_path), source_type=source_type)
+    monkeypatch.chdir(tmp_path)
+    print(f"[dropin] cli plane wave {physics} {source_type!r}", flush=True)
+    seismic = "true" if physics.startswith("elastic") else "false"
+    assert cli.main(["input.json", f"seismic={seismic}", "density_method=arithmetic"]) == 0
+    _, _, f = _oracle(cfg, tuple(k for _, k in chans))
+    src = cfg["src"]
+    three = physics.endswith("3")
+    for ch, key in chans:
+        name = fio.snapshot_name(ch, cfg["time_steps"], src[0], src[-1], src[1] if three else None)
+        got = fio.read_array(str(tmp_path / name), cfg["shape"], np.float32)
+        assert np.abs(f[key]).max() > 0
+        assert rel_l2(got, f[key].astype(np.float32)) < 1e-6, key

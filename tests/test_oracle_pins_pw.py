@@ -1,14 +1,15 @@
 """Pins of the oracle's plane-wave source (plane_wave_source.f08 and the
 injection loops of cpmlfdtd.f08) by closed forms: the Kelvin-form background
-speed of an isotropic solid, and the fields one step after injection into a
-zero state for rotations whose direction and polarisation are known by hand
-(the rotation matrices are filled column by column, cpmlfdtd's reshape)."""
+speed of an isotropic solid, the EM speed and impedance of simple
+permittivity tensors, and the fields one step after injection into a zero
+state for rotations whose direction and polarisation are known by hand (the
+rotation matrices are filled column by column, cpmlfdtd's reshape)."""
 import math
 
 import numpy as np
 import pytest
 
-from oracle.oracle_ctypes import background_speed, run_oracle
+from oracle.oracle_ctypes import CLIGHT, EPS0, MU0, background_speed, em_background, run_oracle
 
 
 def _iso(shape, lam, mu, rho):
@@ -83,3 +84,83 @@ def test_el2_one_step_injection_closed_form(angles, kind, face, e_vec, p_vec):
             assert F[n][c] == pytest.approx(w, rel=1e-12, abs=1e-12 * abs(sc) * A["c33"][c])
     for n in fields:   # nothing else moved: the first update of a zero state is zero
         assert not np.any(F[n][~mask])
+
+
+def test_em_background_closed_form():
+    shape2 = (5, 4)
+    A = {"e11": np.full(shape2, 4.0), "e13": np.zeros(shape2), "e33": np.full(shape2, 4.0)}
+    v, eta = em_background(A, two_d=True)
+    assert v == pytest.approx(CLIGHT / 2.0, rel=1e-15)
+    assert eta == pytest.approx(math.sqrt(MU0 / (EPS0 * 4.0)), rel=1e-15)
+    # [[3, 1], [1, 3]] has eigenvalues 2 and 4; the smallest over the model wins
+    A["e11"][1, 2], A["e13"][1, 2], A["e33"][1, 2] = 3.0, 1.0, 3.0
+    assert em_background(A, two_d=True)[0] == pytest.approx(CLIGHT / math.sqrt(2.0), rel=1e-14)
+    shape3 = (3, 4, 2)
+    B = {n: np.zeros(shape3) for n in ("e12", "e13", "e23")}
+    for n in ("e11", "e22", "e33"):
+        B[n] = np.full(shape3, 6.0)
+    # [[4, 1, 0], [1, 4, 0], [0, 0, 9]]: eigenvalues 3, 5, 9
+    B["e11"][2, 1, 0], B["e12"][2, 1, 0], B["e22"][2, 1, 0], B["e33"][2, 1, 0] = 4.0, 1.0, 4.0, 9.0
+    v, eta = em_background(B, two_d=False)
+    assert v == pytest.approx(CLIGHT / math.sqrt(3.0), rel=1e-14)
+    assert eta == pytest.approx(math.sqrt(MU0 / (EPS0 * 3.0)), rel=1e-14)
+
+
+def _em_cfg(physics, cpml=4):
+    from seidart_hip import models
+    if physics == "em2":
+        return models.em2(nx=26, nz=22, cpml=cpml, steps=1, src=(10, 10), nrec=2, nlayers=2)
+    return models.em3(nx=14, ny=12, nz=16, cpml=cpml, steps=1, src=(7, 6, 8), nrec=2, nlayers=2)
+
+
+# E along e, H along p x e (boundary_em_field, plane_wave_source.f08:163-194)
+@pytest.mark.parametrize("physics", ["em2", "em3"])
+@pytest.mark.parametrize("angles,face,e_vec,h_vec", [
+    ((0.0, 0.0, 0.0), "zmin", (1, 0, 0), (0, 1, 0)),      # p = z, e = x, z x x = y
+    ((90.0, 0.0, 0.0), "xmax", (0, 0, 1), (0, 1, 0)),     # p = -x, e = z, (-x) x z = y
+])
+def test_em_one_step_injection_closed_form(physics, angles, face, e_vec, h_vec):
+    cfg = _em_cfg(physics)
+    f0, amp, wav = 200e6, 3.0, 1
+    cfg["plane_wave"] = dict(kind="s", angles=angles, f0=f0, amplitude=amp, wavelet=wav)
+    three = physics == "em3"
+    fields = ("ex", "ey", "ez", "hx", "hy", "hz") if three else ("ex", "ez", "hy")
+    _, _, F, us = run_oracle(cfg, steps=1, fields=fields)
+    assert us == 0
+    v, eta = em_background(cfg["arrays"], two_d=not three)
+    s = amp * _gaus(cfg["dt"], f0, wav)     # on the face tau = t = dt
+    n = cfg["shape"]
+    cp = cfg["cpml"]
+    # cells along a face: cpml+2 .. n-cpml-1 in 2D, cpml+3 .. n-cpml-2 in 3D
+    lo, hi = (cp + 3, lambda m: m - cp - 2) if three else (cp + 2, lambda m: m - cp - 1)
+    ax = 0 if face[0] == "x" else len(n) - 1
+    fidx = cp + 1 if face.endswith("min") else n[ax] - cp
+    others = [a for a in range(len(n)) if a != ax]
+    mask = np.zeros(n, bool)
+    sl = [None] * len(n)
+    sl[ax] = fidx - 1
+    for a in others:
+        sl[a] = slice(lo - 1, hi(n[a]))
+    mask[tuple(sl)] = True
+    want = {"ex": s * e_vec[0], "ey": s * e_vec[1], "ez": s * e_vec[2],
+            "hx": s / eta * h_vec[0], "hy": s / eta * h_vec[1], "hz": s / eta * h_vec[2]}
+    for name in fields:
+        got = F[name]
+        w = want[name]
+        scale = abs(s) if name[0] == "e" else abs(s) / eta
+        assert np.allclose(got[mask], w, rtol=1e-12, atol=1e-12 * scale), name
+        assert not np.any(got[~mask]), name    # the first update of a zero state is zero
+
+
+@pytest.mark.parametrize("physics", ["em2", "em3"])
+def test_em_face_on_grid_edge_is_zeroed(physics):
+    """With no absorbing layer the face lies on the grid edge, which the
+    Dirichlet step after the injection zeroes again (cpmlfdtd.f08:2692-2711,
+    3921-3961)."""
+    cfg = _em_cfg(physics, cpml=0)
+    cfg["plane_wave"] = dict(kind="s", angles=(0.0, 0.0, 0.0), f0=200e6, amplitude=3.0, wavelet=1)
+    fields = ("ex", "ez", "hy")
+    _, _, F, us = run_oracle(cfg, steps=1, fields=fields)
+    assert us == 0
+    for name in fields:
+        assert not np.any(F[name]), name
