@@ -164,3 +164,92 @@ def test_em_face_on_grid_edge_is_zeroed(physics):
     assert us == 0
     for name in fields:
         assert not np.any(F[name]), name
+
+
+# R = Ry(x_z) (Rz(x_y) Rx(y_z)) with the matrices filled column by column
+# (plane_wave_source.f08:67-98); p = R z, e = R x, worked out by hand.  The
+# last case tells the composition order apart: Rx Ry would send p to -x.
+@pytest.mark.parametrize("angles,face,e_vec,h_vec", [
+    ((0.0, 0.0, 90.0), "ymin", (1, 0, 0), (0, 0, -1)),    # Rx(90): p = y, e = x, y x x = -z
+    ((0.0, 90.0, 0.0), "zmin", (0, -1, 0), (1, 0, 0)),    # Rz(90): p = z, e = -y, z x -y = x
+    ((90.0, 90.0, 0.0), "xmax", (0, -1, 0), (0, 0, 1)),   # Ry(90) Rz(90): p = -x, e = -y
+    ((90.0, 0.0, 90.0), "ymin", (0, 0, 1), (1, 0, 0)),    # Ry(90) Rx(90): p = y, e = z
+])
+def test_em3_rotation_composition_closed_form(angles, face, e_vec, h_vec):
+    cfg = _em_cfg("em3")
+    f0, amp, wav = 200e6, 3.0, 0
+    cfg["plane_wave"] = dict(kind="s", angles=angles, f0=f0, amplitude=amp, wavelet=wav)
+    fields = ("ex", "ey", "ez", "hx", "hy", "hz")
+    _, _, F, us = run_oracle(cfg, steps=1, fields=fields)
+    assert us == 0
+    _, eta = em_background(cfg["arrays"], two_d=False)
+    s = amp * _gaus(cfg["dt"], f0, wav)
+    n, cp = cfg["shape"], cfg["cpml"]
+    ax = "xyz".index(face[0])
+    mask = np.zeros(n, bool)
+    sl = [slice(cp + 2, n[a] - cp - 2) for a in range(3)]     # 0-based cpml+3 .. n-cpml-2
+    sl[ax] = cp if face.endswith("min") else n[ax] - cp - 1
+    mask[tuple(sl)] = True
+    want = {"ex": s * e_vec[0], "ey": s * e_vec[1], "ez": s * e_vec[2],
+            "hx": s / eta * h_vec[0], "hy": s / eta * h_vec[1], "hz": s / eta * h_vec[2]}
+    for name in fields:
+        scale = abs(s) if name[0] == "e" else abs(s) / eta
+        assert np.allclose(F[name][mask], want[name], rtol=1e-12, atol=1e-12 * scale), name
+        assert not np.any(F[name][~mask]), name
+
+
+# 3D stress injection: sigma_I += C_IJ strain_J over the Voigt columns
+# (add_plane_wave_stress, cpmlfdtd.f08:16-50).  Along z, a P wave has only
+# strain_zz = -s/v (column 3), an S wave polarised along x only the
+# engineering strain_xz = -s/v (column 5).
+@pytest.mark.parametrize("kind,col", [("p", 3), ("s", 5)])
+def test_el3_one_step_stress_columns(kind, col):
+    from seidart_hip import models
+    cfg = models.elastic3(nx=14, ny=12, nz=16, cpml=3, steps=1, src=(7, 6, 8), nrec=2, nlayers=2)
+    f0, amp, wav = 2000.0, 1e-3, 2
+    cfg["plane_wave"] = dict(kind=kind, angles=(0.0, 0.0, 0.0), f0=f0, amplitude=amp, wavelet=wav)
+    fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "syz", "sxz", "sxy")
+    _, _, F, us = run_oracle(cfg, steps=1, fields=fields)
+    assert us == 0
+    A = cfg["arrays"]
+    v = background_speed(A, two_d=False)
+    s = amp * _gaus(cfg["dt"], f0, wav)
+    n, cp = cfg["shape"], cfg["cpml"]
+    mask = np.zeros(n, bool)
+    mask[cp + 1:n[0] - cp - 1, cp + 1:n[1] - cp - 1, cp] = True    # z face cpml+1, cells cpml+2 .. n-cpml-1
+    assert np.allclose(F["vz" if kind == "p" else "vx"][mask], s, rtol=1e-12)
+    row = {"sxx": 1, "syy": 2, "szz": 3, "syz": 4, "sxz": 5, "sxy": 6}
+    for name, r in row.items():
+        a, b = sorted((r, col))
+        want = A[f"c{a}{b}"][mask] * (-s / v)
+        assert np.allclose(F[name][mask], want, rtol=1e-12, atol=1e-12 * abs(s / v) * A["c33"].max()), name
+    for name in fields:
+        assert not np.any(F[name][~mask]), name
+
+
+def test_wavelets_are_gaussian_derivatives():
+    """gaus1 and gaus2 against central differences of the oracle's own gaus0
+    (plane_wave_source.f08:141-161): with a = pi f0 (t - t0) and g = exp(-a^2),
+    gaus1 = g' / (pi f0) and gaus2 = -g'' / (2 (pi f0)^2).  The value injected
+    at a face cell one step after a zero state is amplitude * wavelet(dt), so
+    varying dt samples the wavelet."""
+    f0, amp = 200e6, 1.0
+    cp = 4
+
+    def sample(t, wav):
+        cfg = _em_cfg("em2", cpml=cp)
+        cfg["dt"] = t
+        cfg["plane_wave"] = dict(kind="s", angles=(0.0, 0.0, 0.0), f0=f0, amplitude=amp, wavelet=wav)
+        _, _, F, us = run_oracle(cfg, steps=1, fields=("ex",))
+        assert us == 0
+        return F["ex"][cp + 4, cp]     # a cell of the z face (row cpml+1)
+
+    w = math.pi * f0
+    h = 2e-3 / f0
+    for t in (2e-9, 5.5e-9, 7.5e-9, 9e-9, 12e-9):
+        g = [sample(t + k * h, 0) for k in (-1, 0, 1)]
+        d1 = (g[2] - g[0]) / (2 * h)
+        d2 = (g[2] - 2 * g[1] + g[0]) / (h * h)
+        assert sample(t, 1) == pytest.approx(d1 / w, rel=1e-4, abs=1e-6)
+        assert sample(t, 2) == pytest.approx(-d2 / (2 * w * w), rel=1e-4, abs=1e-5)
+    assert sample(1.5 / f0, 0) == pytest.approx(amp, rel=1e-15)      # peak at t0 = 1.5 / f0
