@@ -35,24 +35,31 @@ HBM_PEAK_GBS = 8000.0
 # + written and the six coefficient arrays)
 EL3 = dict(F=9, M=28, PSI=29, C=6)
 EL3_STRESS = dict(reads_v=3, rw_sigma=6, mat=27, psi=20, C=6)
+# isotropic-form arrays: c11, c12, c44, rho and six attenuation divisors per
+# cell (the stress pass reads all but rho)
+EL3_ISO_M, EL3_ISO_STRESS_M = 10, 9
 
 
-def algorithmic_bytes(shape, cpml, w, table=False):
+def algorithmic_bytes(shape, cpml, w, table=False, iso=False, n_ig=6):
     """Bytes per step and per stress pass.  table=False is the model with one
     array per material parameter (M = 28 per cell); table=True replaces those
-    by the one-byte material id read once per pass."""
+    by the one-byte material id read once per pass; iso=True counts the
+    isotropic-form arrays (M = 10); n_ig < 6 counts that many distinct
+    attenuation-divisor arrays (equal ones are held and read once)."""
     nx, ny, nz = shape
     n = nx * ny * nz
     n_int = max(nx - 2 * cpml, 0) * max(ny - 2 * cpml, 0) * max(nz - 2 * cpml, 0)
     n_pml = n - n_int
-    M, Ms = (0, 0) if table else (EL3["M"], EL3_STRESS["mat"])
+    M, Ms = (0, 0) if table else ((EL3_ISO_M, EL3_ISO_STRESS_M) if iso else (EL3["M"], EL3_STRESS["mat"]))
+    if not table:
+        M, Ms = M - (6 - n_ig), Ms - (6 - n_ig)
     step = w * (n * (2 * EL3["F"] + M) + n_pml * (2 * EL3["PSI"] + EL3["C"])) + (2 * n if table else 0)
     s = EL3_STRESS
     stress = w * (n * (s["reads_v"] + 2 * s["rw_sigma"] + Ms) + n_pml * (2 * s["psi"] + s["C"])) + (n if table else 0)
     return step, stress
 
 
-def timed_run(cfg, args, material_table):
+def timed_run(cfg, args, material_table, isotropic=1):
     """W untimed steps (the first uploads every input), then K steps timed by
     the wall clock between synchronisations (captured step graphs, no per-
     kernel events), then KT more steps with HIP events around each kernel pass
@@ -69,6 +76,7 @@ def timed_run(cfg, args, material_table):
     s.set_option("material_table", material_table)
     s.set_option("concurrent", args.concurrent)
     s.set_option("xcd_remap", args.xcd_remap)
+    s.set_option("isotropic", isotropic)
     s.set_arrays(cfg["arrays"])
     for comp, series in cfg["sources"].items():
         series = np.concatenate([np.asarray(series, dtype=np.float64), np.zeros(KT)])
@@ -85,11 +93,12 @@ def timed_run(cfg, args, material_table):
     s.synchronize()
     ms_stress, ms_vel, launches = s.kernel_times()
     nmat = s.materials()
+    iso = s.info("isotropic")
     rec = s.receivers()
     s.close()
     if not np.all(np.isfinite(rec)):
         raise RuntimeError("non-finite receiver traces")
-    return elapsed, ms_stress, ms_vel, launches, nmat
+    return elapsed, ms_stress, ms_vel, launches, nmat, iso
 
 
 def measured_traffic():
@@ -139,7 +148,7 @@ def bench_em3(args):
     shape = cfg["shape"]
     ncell = int(np.prod(shape))
     w = 8 if args.precision == "fp64" else 4
-    elapsed, ms_h, ms_e, launches, nmat = timed_run(cfg, args, args.material_table)
+    elapsed, ms_h, ms_e, launches, nmat, _ = timed_run(cfg, args, args.material_table)
     nx, ny, nz = shape
     cp = cfg["cpml"]
     n_pml = ncell - (nx - 2 * cp) * (ny - 2 * cp) * (nz - 2 * cp)
@@ -431,6 +440,65 @@ def bench_fwi2d(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_seismic3_iso(args):
+    """The isotropic fast path: a smoothly varying isotropic 256^3 + 12-cell
+    CPML model (every cell its own material, so no material table), timed
+    with the three isotropic arrays (c11, c12, c44) and again with all 21
+    stiffness arrays on the same model."""
+    import numpy as np
+    from seidart_hip import models
+
+    K, W = args.steps, args.warmup
+    n = args.n
+    cfg = models.elastic3_iso(nx=n, ny=n, nz=n, steps=W + K, src=(n // 2, n // 2, 20), nrec=min(256, n))
+    shape = cfg["shape"]
+    ncell = int(np.prod(shape))
+    w = 8 if args.precision == "fp64" else 4
+    fast = timed_run(cfg, args, 1, isotropic=1)
+    general = timed_run(cfg, args, 1, isotropic=0)
+    if fast[5] != 1 or general[5] != 0 or fast[4] != 0:
+        raise RuntimeError(f"isotropic path not selected as expected: {fast[4:]}, {general[4:]}")
+
+    gam = [cfg["arrays"].get(g) for g in ("gamma_x", "gamma_y", "gamma_z", "gamma_xy", "gamma_xz", "gamma_yz")]
+    distinct = []
+    for g in gam:
+        g = np.zeros(1) if g is None else g
+        if not any(g.shape == h.shape and np.array_equal(g, h) for h in distinct):
+            distinct.append(g)
+    n_ig = len(distinct)
+
+    def summary(run, iso):
+        elapsed, ms_stress, ms_vel, launches = run[:4]
+        step_bytes, stress_bytes = algorithmic_bytes(shape, cfg["cpml"], w, iso=iso, n_ig=n_ig)
+        avg_stress_s = ms_stress / max(launches, 1) / 1e3
+        return {"value": round(ncell * K / elapsed / 1e9, 4), "ms_per_step": round(elapsed / K * 1e3, 4),
+                "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+                "stress_pass_ms": round(avg_stress_s * 1e3, 4),
+                "velocity_pass_ms": round(ms_vel / max(launches, 1), 4),
+                "stress_achieved": round(stress_bytes / avg_stress_s / 1e9, 1), "stress_bytes": stress_bytes}
+
+    f, g = summary(fast, True), summary(general, False)
+    line = {
+        "metric": "GCell-updates/s (3D elastic CPML-FDTD, isotropic per-cell model, "
+                  + ("FP64)" if args.precision == "fp64" else "FP32)"),
+        "value": f["value"], "unit": "GCell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": f["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (smooth isotropic model with 0.5 % per-cell jitter, Ricker source)",
+        "config": {"workload": f"seismic3 isotropic {n}^3 + {cfg['cpml']}-cell CPML ({shape[0]}^3 padded), "
+                               "per-cell arrays, attenuation", "cells": ncell, "precision": args.precision},
+        "hbm_fraction_step": f["hbm_fraction_step"],
+        "roofline": {"bound": "hbm", "kernel": "el3 stress pass, isotropic arrays",
+                     "achieved": f["stress_achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(f["stress_achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": f["stress_bytes"], "avg_launch_ms": f["stress_pass_ms"]},
+        "cpu_baseline": None, "distinct_attenuation_arrays": n_ig,
+        "isotropic_path": f, "general_path_same_model": g,
+        "speedup_vs_general_path": round(g["ms_per_step"] / f["ms_per_step"], 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -452,7 +520,8 @@ def main():
                     help="also time the per-cell-array layout and report it beside the main line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
-    ap.add_argument("--workload", default="seismic3", choices=["seismic3", "electromag3", "batch2d", "fwi2d"],
+    ap.add_argument("--workload", default="seismic3",
+                    choices=["seismic3", "seismic3iso", "electromag3", "batch2d", "fwi2d"],
                     help="seismic3: the 3D elastic 256^3 case (default); electromag3: the 3D EM "
                          "384x384x192 case; batch2d: uneven multi-shot 2D batches against serial runs")
     ap.add_argument("--shots", type=int, default=64, help="batch2d: shots per solver kind (fwi2d: 16)")
@@ -491,6 +560,9 @@ def main():
     if args.workload == "fwi2d":
         bench_fwi2d(args)
         return
+    if args.workload == "seismic3iso":
+        bench_seismic3_iso(args)
+        return
 
     import numpy as np
     from seidart_hip import models
@@ -506,7 +578,7 @@ def main():
 
     main_run = timed_run(cfg, args, args.material_table)
     dense_run = timed_run(cfg, args, 0) if (args.material_table and args.compare_layouts) else None
-    elapsed, ms_stress, ms_vel, launches, nmat = main_run
+    elapsed, ms_stress, ms_vel, launches, nmat, _ = main_run
 
     value = ncell * K / elapsed / 1e9
     step_bytes, stress_bytes = algorithmic_bytes(shape, cfg["cpml"], w)
@@ -566,7 +638,7 @@ def main():
         "setup_s": round(t_gen, 2),
     }
     if dense_run is not None:
-        e2, s2, v2, l2, _ = dense_run
+        e2, s2, v2, l2, _, _ = dense_run
         line["per_cell_arrays_layout"] = {
             "value": round(ncell * K / e2 / 1e9, 4), "ms_per_step": round(e2 / K * 1e3, 4),
             "hbm_fraction_step": round(step_bytes * K / e2 / 1e9 / HBM_PEAK_GBS, 4),

@@ -39,6 +39,7 @@ static El3Args<T> el3_args(Device &d)
     A.tab = (const T *)d.tab;
     A.shell = (T *)d.shell;
     A.remap = d.xcd_remap;
+    A.ig_same = d.ig_same ? 1 : 0;
     A.snq = d.shell_snq;
     A.cid = d.shell_cid;
     A.ctab = (const T *)d.shell_ctab;
@@ -100,6 +101,27 @@ static int el3_material_table(Device &d, const double *const *in)
     });
 }
 
+// Isotropic form of every cell's stiffness: c11 = c22 = c33, c12 = c13 = c23,
+// c44 = c55 = c66 and the other twelve entries 0 (exact comparisons; absent
+// arrays are 0).  The stress updates then read c11, c12, c44 only, with the
+// same products and sums as from the full tensor (the zero entries add exact
+// zeros).
+static bool el3_isotropic(const double *const *in, int64_t n)
+{
+    auto at = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
+    static const int zero_ids[] = {SDT_C14, SDT_C15, SDT_C16, SDT_C24, SDT_C25, SDT_C26,
+                                   SDT_C34, SDT_C35, SDT_C36, SDT_C45, SDT_C46, SDT_C56};
+    for (int64_t c = 0; c < n; ++c) {
+        const double a = at(SDT_C11, c), l = at(SDT_C12, c), m = at(SDT_C44, c);
+        if (at(SDT_C22, c) != a || at(SDT_C33, c) != a || at(SDT_C13, c) != l || at(SDT_C23, c) != l ||
+            at(SDT_C55, c) != m || at(SDT_C66, c) != m)
+            return false;
+        for (int id : zero_ids)
+            if (at(id, c) != 0.0) return false;
+    }
+    return true;
+}
+
 int el3_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -125,19 +147,47 @@ int el3_setup(Device &d, const double *const *in)
     d.nmat = 0;
     rc = d.mat_table_opt ? el3_material_table(d, in) : SDT_OK;
     if (rc) return rc;
-    if (!d.mat_table) {
+    d.mat_iso = !d.mat_table && d.iso_opt && el3_isotropic(in, n);
+    if (d.mat_iso) {
+        // c11, c12, c44 in their Voigt slots 0, 1, 15; the other 18 slots stay null
+        for (int id : {SDT_C11, SDT_C12, SDT_C44, SDT_RHO}) {
+            if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
+            rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[id - SDT_C11]);
+            if (rc) return rc;
+        }
+        d.nmat = SDT_RHO - SDT_C11 + 1;
+    } else if (!d.mat_table) {
         for (int id = SDT_C11; id <= SDT_RHO; ++id) {
             if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
             rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
             if (rc) return rc;
         }
-        std::vector<double> inv((size_t)n);
+    }
+    d.ig_same = false;
+    if (!d.mat_table) {
+        // attenuation divisors 1/(1 + gamma dt); a direction whose array
+        // equals an earlier one's shares its device array, and when all six
+        // are equal the stress kernels load one value per cell
+        std::vector<double> inv[6];
         const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};
-        for (int id : gids) {
-            for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
-            rc = upload_grid(d, inv.data(), &d.mat[d.nmat++]);
+        int same = 0;
+        for (int q = 0; q < 6; ++q) {
+            const int id = gids[q];
+            inv[q].resize((size_t)n);
+            for (int64_t c = 0; c < n; ++c) inv[q][(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
+            int p = 0;   // an earlier array with the same values (aliased ones are left empty)
+            while (p < q && (inv[p].empty() || memcmp(inv[p].data(), inv[q].data(), sizeof(double) * (size_t)n) != 0))
+                ++p;
+            if (p < q) {
+                d.mat[d.nmat++] = d.mat[22 + p];
+                std::vector<double>().swap(inv[q]);
+                if (p == 0) ++same;
+                continue;
+            }
+            rc = upload_grid(d, inv[q].data(), &d.mat[d.nmat++]);
             if (rc) return rc;
         }
+        d.ig_same = same == 5;
     }
     for (int f = SDT_F_VX; f <= SDT_F_SYZ; ++f) {
         if (f <= SDT_F_VZ && in[SDT_IC_X + f]) rc = upload_grid(d, in[SDT_IC_X + f], &d.field[f]);

@@ -42,6 +42,7 @@ struct El3Args {
     T dt;
     int density_code, corrected;
     int remap;      // bit 0: XCD-contiguous block order of the interior kernels, bit 1: of the shell kernels
+    int ig_same;    // 1: the six ig arrays hold equal values (one array, loaded once per cell)
     const int *d_it;
     unsigned long long *vnorm, *snorm;
     int64_t vfix[3], sfix[4];
@@ -103,6 +104,22 @@ struct El3Tile {
     int ax, ay, alane;   // an active cell of the tile and its lane: inactive lanes
                          // load from it, so they add no memory traffic
 };
+
+// Where the stress kernels read a cell's stiffness from: all 21 per-cell
+// arrays, the material table, or the isotropic arrays c11, c12, c44.
+enum { kMatDense = 0, kMatTable = 1, kMatIso = 2 };
+
+// Voigt slot q (c11 c12 c13 c14 c15 c16 c22 ... c66 order) of an isotropic-form
+// tensor: c11 = c22 = c33, c12 = c13 = c23, c44 = c55 = c66, the rest 0.  With
+// q a compile-time constant after unrolling, only the three arrays are read.
+template <typename T>
+__device__ __forceinline__ T iso_entry(int q, T c11, T c12, T c44)
+{
+    return (q == 0 || q == 6 || q == 11) ? c11
+         : (q == 1 || q == 2 || q == 7)   ? c12
+         : (q == 15 || q == 18 || q == 20) ? c44
+                                           : T(0);
+}
 
 constexpr int kTabRow = kEl3TabRow;   // material-table row length (elements)
 constexpr int kTabIg = 21, kTabRho = 27;

@@ -66,10 +66,10 @@ __device__ __forceinline__ void load_stiff(Stiff<T> &K, const T *const *C, int64
 template <typename T>
 using const_row = __attribute__((address_space(4))) const T *;
 
-template <typename T, bool TAB>
+template <typename T, int MAT>
 __device__ __forceinline__ void load_material(Stiff<T> &K, T (&ig)[6], const El3Args<T> &A, int64_t c)
 {
-    if (TAB) {
+    if (MAT == kMatTable) {
         const int m = A.mid[c];
         const int m0 = __builtin_amdgcn_readfirstlane(m);
         if (__all(m == m0)) {
@@ -86,9 +86,21 @@ __device__ __forceinline__ void load_material(Stiff<T> &K, T (&ig)[6], const El3
             for (int q = 0; q < 6; ++q) ig[q] = row[kTabIg + q];
         }
     } else {
-        load_stiff(K, A.C, c);
+        if (MAT == kMatIso) {
+            const T c11 = ldnt(&A.C[0][c]), c12 = ldnt(&A.C[1][c]), c44 = ldnt(&A.C[15][c]);
 #pragma unroll
-        for (int q = 0; q < 6; ++q) ig[q] = ldnt(&A.ig[q][c]);
+            for (int q = 0; q < 21; ++q) K.v[q] = iso_entry(q, c11, c12, c44);
+        } else {
+            load_stiff(K, A.C, c);
+        }
+        if (A.ig_same) {   // wave-uniform branch
+            const T g = ldnt(&A.ig[0][c]);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ig[q] = g;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) ig[q] = ldnt(&A.ig[q][c]);
+        }
     }
 }
 template <typename T, bool TAB>
@@ -109,7 +121,7 @@ __device__ __forceinline__ T supk(T s, const Stiff<T> &K, int i0, int i1, int i2
 }
 
 // ------------------------------------------------------------------ interior
-template <typename T, int TY, bool TAB>
+template <typename T, int TY, int MAT>
 __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, int kc)
 {
     const int i0 = A.box[0], j0 = A.box[2];
@@ -151,7 +163,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
         // coefficient sets leave a derivative unchanged
         Stiff<T> K;
         T ig[6];
-        load_material<T, TAB>(K, ig, A, c);
+        load_material<T, MAT>(K, ig, A, c);
         T sxx, syy, szz, sxy, sxz, syz;
         {   // L1: D+x, D-y, D-z
             const T yz = dym[2] + dzm[1], xz = dzm[0] + dxp[2], xy = dxp[1] + dym[0];
@@ -395,6 +407,13 @@ struct DenseRow {    // per-cell arrays (cached loads: the three wavefronts shar
     int64_t c;
     __device__ T operator[](int q) const { return q < 21 ? A->C[q][c] : A->ig[q - 21][c]; }
 };
+template <typename T>
+struct IsoRow {      // isotropic-form per-cell arrays c11, c12, c44
+    const El3Args<T> *A;
+    int64_t c;
+    T c11, c12, c44;
+    __device__ T operator[](int q) const { return q < 21 ? iso_entry(q, c11, c12, c44) : A->ig[q - 21][c]; }
+};
 
 template <typename T, typename R>
 __device__ __forceinline__ T supr(T s, const R &K, int i0, int i1, int i2, int i3, int i4, int i5, T dxx, T dyy,
@@ -453,11 +472,15 @@ __device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K
     return 2.0 * (a4 * a4 + a5 * a5);
 }
 
-template <typename T, bool TAB, int W>
+template <typename T, int MAT, int W>
 __device__ __forceinline__ double shell_stresses_at(const El3Args<T> &A, int64_t c, const T *D, int lane, bool L1,
                                                     bool L2, bool L3, bool L4)
 {
-    if (TAB) {
+    if (MAT == kMatIso) {
+        const IsoRow<T> K{&A, c, A.C[0][c], A.C[1][c], A.C[15][c]};
+        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+    }
+    if (MAT == kMatTable) {
         const int m = A.mid[c];
         const int m0 = __builtin_amdgcn_readfirstlane(m);
         if (__all(m == m0)) {
@@ -476,7 +499,7 @@ __device__ __forceinline__ double shell_stresses_at(const El3Args<T> &A, int64_t
 // one LDS buffer) and two stresses of plane k (from the other), so the loads
 // of both phases are in flight together.  Buffers alternate per plane; every
 // buffer is rewritten only after the barrier that follows its last read.
-template <typename T, bool TAB>
+template <typename T, int MAT>
 __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
 {
     __shared__ T D[2][3 * K_NKIND * kShellLanes];
@@ -527,9 +550,9 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
         bool L1, L2, L3, L4;
         nests(k, L1, L2, L3, L4);
         double part;
-        if (w == 0) part = shell_stresses_at<T, TAB, 0>(A, c, D[b], lane, L1, L2, L3, L4);
-        else if (w == 1) part = shell_stresses_at<T, TAB, 1>(A, c, D[b], lane, L1, L2, L3, L4);
-        else part = shell_stresses_at<T, TAB, 2>(A, c, D[b], lane, L1, L2, L3, L4);
+        if (w == 0) part = shell_stresses_at<T, MAT, 0>(A, c, D[b], lane, L1, L2, L3, L4);
+        else if (w == 1) part = shell_stresses_at<T, MAT, 1>(A, c, D[b], lane, L1, L2, L3, L4);
+        else part = shell_stresses_at<T, MAT, 2>(A, c, D[b], lane, L1, L2, L3, L4);
         N[b][w * kShellLanes + lane] = part;
         __syncthreads();
         if (w == 0 && L.on) {
@@ -677,10 +700,10 @@ static bool box_empty(const Device &d)
     return d.box[1] < d.box[0] || d.box[3] < d.box[2] || d.box[5] < d.box[4];
 }
 
-template <typename T, int TY, bool TAB>
+template <typename T, int TY, int MAT>
 static void int_stress(Device &d, const El3Args<T> &A, int kc)
 {
-    hipLaunchKernelGGL((el3_int_stress_kernel<T, TY, TAB>), interior_grid(d, kc, TY), dim3(64, TY, 1), 0, d.stream,
+    hipLaunchKernelGGL((el3_int_stress_kernel<T, TY, MAT>), interior_grid(d, kc, TY), dim3(64, TY, 1), 0, d.stream,
                        A, kc);
 }
 
@@ -709,18 +732,18 @@ static void shell_stream_end(Device &d, int pass)
     (void)hipStreamWaitEvent(d.stream, d.join[pass], 0);
 }
 
-template <typename T, bool TAB>
+template <typename T, int MAT>
 static void fast_stress(Device &d, const El3Args<T> &A)
 {
     const int kc = el3_kc(d), ty = el3_ty(d);
     const hipStream_t ss = shell_stream_begin(d, 0);
     if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_stress_kernel<T, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss, A,
+        hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss, A,
                            (const El3Tile *)d.tiles);
     if (!box_empty(d)) {
-        if (ty == 16) int_stress<T, 16, TAB>(d, A, kc);
-        else if (ty == 8) int_stress<T, 8, TAB>(d, A, kc);
-        else int_stress<T, 4, TAB>(d, A, kc);
+        if (ty == 16) int_stress<T, 16, MAT>(d, A, kc);
+        else if (ty == 8) int_stress<T, 8, MAT>(d, A, kc);
+        else int_stress<T, 4, MAT>(d, A, kc);
     }
     shell_stream_end(d, 0);
 }
@@ -744,8 +767,9 @@ static void fast_velocity(Device &d, const El3Args<T> &A)
 template <typename T>
 void el3_fast_stress(Device &d, const El3Args<T> &A)
 {
-    if (d.mat_table) fast_stress<T, true>(d, A);
-    else fast_stress<T, false>(d, A);
+    if (d.mat_table) fast_stress<T, kMatTable>(d, A);
+    else if (d.mat_iso) fast_stress<T, kMatIso>(d, A);
+    else fast_stress<T, kMatDense>(d, A);
 }
 
 template <typename T>

@@ -593,6 +593,11 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
         if (value != 0 && value != 1) return fail(h, SDT_ERR_ARG, "material table option must be 0 or 1");
         h->d.mat_table_opt = (int)value;
         return SDT_OK;
+    case SDT_OPT_ISOTROPIC:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "material layout is fixed once the first step has run");
+        if (value != 0 && value != 1) return fail(h, SDT_ERR_ARG, "isotropic option must be 0 or 1");
+        h->d.iso_opt = (int)value;
+        return SDT_OK;
     case SDT_OPT_STAGGER:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "allocation layout is fixed once the first step has run");
         h->d.stagger = value != 0;
@@ -619,6 +624,7 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
     case SDT_INFO_SHELL_TILES: *value = h->d.ntiles; return SDT_OK;
     case SDT_INFO_DEVICE_BYTES: *value = (int64_t)h->d.grid_bytes; return SDT_OK;
     case SDT_INFO_COEFFICIENTS: *value = h->d.adj.ncoef; return SDT_OK;
+    case SDT_INFO_ISOTROPIC: *value = h->d.mat_iso ? 1 : 0; return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown info key");
 }

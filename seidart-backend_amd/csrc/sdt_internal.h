@@ -187,6 +187,9 @@ struct Device {
     int zchunk = 0;                    // SDT_OPT_ZCHUNK
     int mat_table_opt = 1;             // SDT_OPT_MATERIAL_TABLE
     bool mat_table = false;            // per-cell material ids + table in use
+    int iso_opt = 1;                   // SDT_OPT_ISOTROPIC
+    bool mat_iso = false;              // 3D elastic: only c11, c12, c44 held (isotropic form)
+    bool ig_same = false;              // 3D elastic per-cell arrays: the six attenuation divisors are equal
     int nmaterials = 0;                // distinct materials found at setup
     uint8_t *mid = nullptr;            // material id per (padded) cell
     void *tab = nullptr;               // material rows

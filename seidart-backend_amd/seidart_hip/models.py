@@ -273,6 +273,46 @@ def elastic3(nx=256, ny=256, nz=256, d=0.5, cpml=12, steps=2000, seed=1, nlayers
                 receivers=recs, density_method=density_method, vmax=vmax)
 
 
+def elastic3_iso(nx=256, ny=256, nz=256, d=0.5, cpml=12, steps=2000, seed=11, f0=100.0, src=(128, 128, 20),
+                 cfl=0.45, nrec=256, density_method="arithmetic", with_cpml=True, attenuation=True):
+    """3D elastic, smoothly varying isotropic model (a P speed gradient with
+    depth and lateral undulations, Vp/Vs about 1.8, seeded per-cell jitter of
+    0.5 %, so no two cells share a material and the per-cell arrays are used):
+    c11 = c22 = c33 = lambda + 2 mu, c12 = c13 = c23 = lambda, c44 = c55 = c66
+    = mu, the other twelve entries absent (0).  The repeated entries share one
+    array."""
+    rng = np.random.default_rng(seed)
+    NX, NY, NZ = nx + 2 * cpml, ny + 2 * cpml, nz + 2 * cpml
+    shape = (NX, NY, NZ)
+    x = np.linspace(0.0, 1.0, NX)[:, None, None]
+    y = np.linspace(0.0, 1.0, NY)[None, :, None]
+    z = np.linspace(0.0, 1.0, NZ)[None, None, :]
+    jit = lambda: 1.0 + 0.005 * rng.standard_normal(shape)
+    vp = (3000.0 + 1200.0 * z + 150.0 * np.sin(2 * math.pi * 1.5 * x) * np.cos(2 * math.pi * y)) * jit()
+    vs = vp / (1.8 + 0.1 * np.cos(2 * math.pi * z)) * jit()
+    rho = (1800.0 + 500.0 * z + 50.0 * np.sin(2 * math.pi * (x + y))) * jit()
+    mu = np.asfortranarray(rho * vs * vs)
+    lam = np.asfortranarray(rho * (vp * vp - 2.0 * vs * vs))
+    c11 = np.asfortranarray(lam + 2.0 * mu)
+    arrays = {"c11": c11, "c22": c11, "c33": c11, "c12": lam, "c13": lam, "c23": lam,
+              "c44": mu, "c55": mu, "c66": mu, "rho": np.asfortranarray(rho)}
+    if attenuation:
+        g = np.asfortranarray(0.002 + 0.006 * z * np.ones(shape))
+        for gname in ("gamma_x", "gamma_y", "gamma_z", "gamma_xy", "gamma_xz", "gamma_yz"):
+            arrays[gname] = g
+    vmax = float(np.sqrt(c11 / rho).max())
+    dt = cfl * d / (math.sqrt(3.0) * vmax)
+    arrays.update(cpml_arrays(shape, (d, d, d), cpml, vmax, dt, f0) if with_cpml else no_cpml(shape))
+    for c in ("ic_x", "ic_y", "ic_z"):
+        arrays[c] = np.zeros(shape, order="F")
+    nrec = min(nrec, nx)
+    recs = np.array([[1 + (nx - 1) * k // max(nrec - 1, 1), ny // 2, 1] for k in range(nrec)], dtype=np.int32)
+    return dict(physics="elastic3", shape=shape, spacing=(d, d, d), cpml=cpml, dt=dt,
+                time_steps=steps, src=src, arrays=arrays,
+                sources={"x": np.zeros(steps), "y": np.zeros(steps), "z": ricker(f0, dt, steps)},
+                receivers=recs, density_method=density_method, vmax=vmax)
+
+
 def elastic25(nx=120, nz=100, dx=1.0, dz=1.0, cpml=10, steps=400, seed=5, nlayers=3, f0=60.0,
               src=(60, 30), cfl=0.4, nrec=24, ky=0.05, density_method="harmonic", anomaly=True,
               with_cpml=True):

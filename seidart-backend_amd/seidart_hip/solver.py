@@ -224,18 +224,23 @@ class Solver:
         the interior kernels, 0 = sequential) or 'xcd_remap' (3D elastic block
         order, bit 0 interior, bit 1 shell: XCD-contiguous tile ranges) or
         'checkpoint' (2D: steps between forward states saved for adjoint())
-        or 'raw_source' (1 = source series injected as given)."""
+        or 'raw_source' (1 = source series injected as given) or 'isotropic'
+        (3D elastic without a material table: 1 = hold only c11, c12, c44 when
+        every cell's stiffness has the isotropic form, 0 = all 21 arrays)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
              "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
              "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP,
-             "checkpoint": L.OPT_CHECKPOINT, "raw_source": L.OPT_RAW_SOURCE}[key]
+             "checkpoint": L.OPT_CHECKPOINT, "raw_source": L.OPT_RAW_SOURCE,
+             "isotropic": L.OPT_ISOTROPIC}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     def info(self, key):
         """'materials' (distinct materials in the table, 0 = per-cell arrays),
-        'shell_tiles' or 'device_bytes'; valid after the first run()."""
+        'shell_tiles', 'device_bytes', 'coefficients' or 'isotropic' (1 = the
+        3D elastic isotropic arrays are in use); valid after the first run()."""
         k = {"materials": L.INFO_MATERIALS, "shell_tiles": L.INFO_SHELL_TILES,
-             "device_bytes": L.INFO_DEVICE_BYTES, "coefficients": L.INFO_COEFFICIENTS}[key]
+             "device_bytes": L.INFO_DEVICE_BYTES, "coefficients": L.INFO_COEFFICIENTS,
+             "isotropic": L.INFO_ISOTROPIC}[key]
         v = C.c_int64(0)
         L.check(L.lib.sdt_get_info(self._h, k, C.byref(v)), self._h)
         return v.value
