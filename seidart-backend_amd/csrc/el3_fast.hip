@@ -113,6 +113,31 @@ __device__ __forceinline__ T rho_at(const El3Args<T> &A, int64_t c)
     return A.tab[m * kTabRow + kTabRho];
 }
 
+// Density in two steps, so that the per-lane load (the material id, or the
+// density itself without a table) can be issued a plane ahead of the cached
+// table read.
+template <typename T>
+struct RhoPre {
+    int m;
+    T r;
+};
+template <typename T, bool TAB>
+__device__ __forceinline__ RhoPre<T> rho_pre(const El3Args<T> &A, int64_t c)
+{
+    RhoPre<T> p;
+    p.m = TAB ? (int)A.mid[c] : 0;
+    p.r = TAB ? T(0) : A.rho[c];
+    return p;
+}
+template <typename T, bool TAB>
+__device__ __forceinline__ T rho_get(const El3Args<T> &A, const RhoPre<T> &p)
+{
+    if (!TAB) return p.r;
+    const int m0 = __builtin_amdgcn_readfirstlane(p.m);
+    if (__all(p.m == m0)) return ((const_row<T>)(A.tab))[m0 * kTabRow + kTabRho];
+    return A.tab[p.m * kTabRow + kTabRho];
+}
+
 template <typename T>
 __device__ __forceinline__ T supk(T s, const Stiff<T> &K, int i0, int i1, int i2, int i3, int i4, int i5, T dxx, T dyy,
                                   T dzz, T yz, T xz, T xy, T dt, T ig)
@@ -342,6 +367,26 @@ __device__ __forceinline__ void shell_coefs(const El3Args<T> &A, const T *blk, i
     }
 }
 
+// The same in two steps (the velocity kernel loads the slot's dictionary id a
+// plane ahead, then reads the cached row).
+template <typename T>
+__device__ __forceinline__ int coef_id(const El3Args<T> &A, int64_t slot)
+{
+    return A.ctab ? (int)A.cid[slot] : 0;
+}
+template <typename T>
+__device__ __forceinline__ void shell_coefs_id(const El3Args<T> &A, const T *blk, int id, T (&cf)[6])
+{
+    if (A.ctab) {
+        const T *row = A.ctab + 8 * id;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = row[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T cpml_upd(T d, const T *psi_p, T a, T b, T ik, T &psi_out)
 {
@@ -427,7 +472,7 @@ __device__ __forceinline__ T supr(T s, const R &K, int i0, int i1, int i2, int i
 // Returns this wavefront's share of the stress norm's sum of squares.
 template <typename T, int W, typename R>
 __device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K, int64_t c, const T *D, int lane,
-                                                 bool L1, bool L2, bool L3, bool L4)
+                                                 bool L1, bool L2, bool L3, bool L4, T sa, T sb)
 {
     auto d = [&](int m, int kind) { return D[(m * K_NKIND + kind) * kShellLanes + lane]; };
     const T dt = A.dt;
@@ -436,14 +481,14 @@ __device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K
         const T dxx = d(0, K_XPH), dyy = d(1, K_YMF), dzz = d(2, K_ZMF);
         const T yz = d(1, K_ZMF) + d(2, K_YMF), xz = d(0, K_ZMF) + d(2, K_XPH), xy = d(1, K_XPH) + d(0, K_YMF);
         if (W == 0) {
-            T sxx = ldnt(&A.s[0][c]), syy = ldnt(&A.s[1][c]);
+            T sxx = sa, syy = sb;
             const T n0 = supr(sxx, K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 0);
             const T n1 = supr(syy, K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 1);
             if (L1) { sxx = n0; syy = n1; stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); }
             const double a0 = sxx, a1 = syy;
             return a0 * a0 + a1 * a1;
         }
-        T szz = ldnt(&A.s[2][c]), sxy = ldnt(&A.s[3][c]);
+        T szz = sa, sxy = sb;
         const T n2 = supr(szz, K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 2);
         if (L1) { szz = n2; stnt(&A.s[2][c], szz); }
         {   // L2 derivatives: D-x full, D+y half, D-z half (vx's uncorrected)
@@ -455,7 +500,7 @@ __device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K
         const double a2 = szz, a3 = sxy;
         return a2 * a2 + 2.0 * (a3 * a3);
     }
-    T sxz = ldnt(&A.s[4][c]), syz = ldnt(&A.s[5][c]);
+    T sxz = sa, syz = sb;
     {   // L3 derivatives: D-x full, D-y full, D+z half
         const T dxx = d(0, K_XMF), dyy = d(1, K_YMF), dzz = d(2, K_ZPH);
         const T yz = d(1, K_ZPH) + d(2, K_YMF), xz = d(0, K_ZPH) + d(2, K_XMF), xy = d(1, K_XMF) + d(0, K_YMF);
@@ -472,26 +517,53 @@ __device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K
     return 2.0 * (a4 * a4 + a5 * a5);
 }
 
+// Phase-2 operands of one plane that do not depend on phase 1: this
+// wavefront's two stresses and the handle of the cell's material (table id,
+// or the isotropic entries).
+template <typename T>
+struct ShellPre {
+    T sa, sb;
+    int m;
+    T c11, c12, c44;
+};
+
+template <typename T, int MAT>
+__device__ __forceinline__ ShellPre<T> shell_pre(const El3Args<T> &A, int64_t c, int w)
+{
+    ShellPre<T> P;
+    P.sa = ldnt(&A.s[2 * w][c]);
+    P.sb = ldnt(&A.s[2 * w + 1][c]);
+    P.m = 0;
+    P.c11 = P.c12 = P.c44 = T(0);
+    if (MAT == kMatTable) P.m = A.mid[c];
+    if (MAT == kMatIso) {
+        P.c11 = A.C[0][c];
+        P.c12 = A.C[1][c];
+        P.c44 = A.C[15][c];
+    }
+    return P;
+}
+
 template <typename T, int MAT, int W>
 __device__ __forceinline__ double shell_stresses_at(const El3Args<T> &A, int64_t c, const T *D, int lane, bool L1,
-                                                    bool L2, bool L3, bool L4)
+                                                    bool L2, bool L3, bool L4, const ShellPre<T> &P)
 {
     if (MAT == kMatIso) {
-        const IsoRow<T> K{&A, c, A.C[0][c], A.C[1][c], A.C[15][c]};
-        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+        const IsoRow<T> K{&A, c, P.c11, P.c12, P.c44};
+        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
     }
     if (MAT == kMatTable) {
-        const int m = A.mid[c];
+        const int m = P.m;
         const int m0 = __builtin_amdgcn_readfirstlane(m);
         if (__all(m == m0)) {
             const ScalarRow<T> K{(const_row<T>)(A.tab) + m0 * kTabRow};
-            return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+            return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
         }
         const LaneRow<T> K{A.tab + m * kTabRow};
-        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
     }
     const DenseRow<T> K{&A, c};
-    return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4);
+    return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
 }
 
 // The stress kernel runs its two phases as a pipeline over the planes: in one
@@ -545,14 +617,21 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
     double nrm = 0.0;
     for (int k = tl.k0; k <= tl.k1; ++k) {
         const int b = (k - tl.k0) & 1;
-        if (k < tl.k1) phase1(k + 1, b ^ 1);
         const int64_t c = L.col + sz * (k - 1);
+        // per-cell arrays: load the phase-2 operands ahead of phase 1 (their
+        // latency is exposed otherwise); with the table (scalar row loads)
+        // the kernel measured faster loading them after it
+        constexpr bool ahead = MAT != kMatTable;
+        ShellPre<T> P;
+        if (ahead) P = shell_pre<T, MAT>(A, c, w);
+        if (k < tl.k1) phase1(k + 1, b ^ 1);
+        if (!ahead) P = shell_pre<T, MAT>(A, c, w);
         bool L1, L2, L3, L4;
         nests(k, L1, L2, L3, L4);
         double part;
-        if (w == 0) part = shell_stresses_at<T, MAT, 0>(A, c, D[b], lane, L1, L2, L3, L4);
-        else if (w == 1) part = shell_stresses_at<T, MAT, 1>(A, c, D[b], lane, L1, L2, L3, L4);
-        else part = shell_stresses_at<T, MAT, 2>(A, c, D[b], lane, L1, L2, L3, L4);
+        if (w == 0) part = shell_stresses_at<T, MAT, 0>(A, c, D[b], lane, L1, L2, L3, L4, P);
+        else if (w == 1) part = shell_stresses_at<T, MAT, 1>(A, c, D[b], lane, L1, L2, L3, L4, P);
+        else part = shell_stresses_at<T, MAT, 2>(A, c, D[b], lane, L1, L2, L3, L4, P);
         N[b][w * kShellLanes + lane] = part;
         __syncthreads();
         if (w == 0 && L.on) {
@@ -571,7 +650,7 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
 // value (the old one outside the nest's range).
 template <typename T, bool CORR, bool TAB, int M>
 __device__ __forceinline__ T shell_velocity(const El3Args<T> &A, const ShellCol &L, T (&qs)[5], T (&qr)[3], int64_t c,
-                                            T *blk, int k, const T (&cf)[6])
+                                            T *blk, int k, const T (&cf)[6], T rx_nb)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz, i = L.i, j = L.j;
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
@@ -595,7 +674,7 @@ __device__ __forceinline__ T shell_velocity(const El3Args<T> &A, const ShellCol 
         d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
         d2 = cpml_upd(d2, m2, ah, bh, ikh, p1);
         d3 = cpml_upd(d3, m3, ah, bh, ikh, p2);
-        r1 = scalar_mean(r, rho_at<T, TAB>(A, c + L.xm1), dc);
+        r1 = scalar_mean(r, rx_nb, dc);   // x neighbour i-1
         r3 = scalar_mean(r, qr[0], dc);
         in = i >= 3 && j >= 3 && k >= 3;
     } else if (M == 1) {
@@ -607,7 +686,7 @@ __device__ __forceinline__ T shell_velocity(const El3Args<T> &A, const ShellCol 
         d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
         d2 = cpml_upd(d2, m2, ah, bh, ikh, p1);
         d3 = cpml_upd(d3, m3, a, b, ik, p2);
-        r1 = scalar_mean(r, rho_at<T, TAB>(A, c + L.xp1), dc);
+        r1 = scalar_mean(r, rx_nb, dc);   // x neighbour i+1
         r3 = scalar_mean(r, qr[0], dc);
         in = i <= nx - 2 && j <= ny - 2 && k >= 3;
     } else {
@@ -619,7 +698,7 @@ __device__ __forceinline__ T shell_velocity(const El3Args<T> &A, const ShellCol 
         d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
         d2 = cpml_upd(d2, m2, a, b, ik, p1);
         d3 = cpml_upd(d3, m3, ah, bh, ikh, p2);
-        r1 = scalar_mean(r, rho_at<T, TAB>(A, c + L.xp1), dc);
+        r1 = scalar_mean(r, rx_nb, dc);   // x neighbour i+1
         r3 = scalar_mean(r, qr[2], dc);
         in = i <= nx - 2 && j >= 3 && k <= nz - 2;
     }
@@ -656,19 +735,35 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3
     const int lsel = L.on ? lane : tl.alane;
     T *blk = shell_block(A, tl, lsel);
     int64_t slot = tl.base + lsel;
+    // x neighbour of the x-face density mean: i-1 for vx, i+1 for vy and vz
+    const int64_t xo = w == 0 ? L.xm1 : L.xp1;
+    // per-lane loads of plane k, issued during plane k-1 (after its own loads)
+    auto zp1_of = [&](int k) { return sz * (min(k + 1, nz) - k); };
+    auto zp2_of = [&](int k) { return sz * (min(k + 2, nz) - k); };
+    const int64_t c0 = L.col + sz * (tl.k0 - 1);
+    T qs_next = sq[c0 + zp2_of(tl.k0)];
+    RhoPre<T> rz_next = rho_pre<T, TAB>(A, c0 + zp1_of(tl.k0)), rx_next = rho_pre<T, TAB>(A, c0 + xo);
+    int cid_next = coef_id(A, slot);
     for (int k = tl.k0; k <= tl.k1; ++k, blk += (int64_t)A.snq * kShellLanes, slot += kShellLanes) {
         const int b = (k - tl.k0) & 1;
         const int64_t c = L.col + sz * (k - 1);
-        const int64_t zp1 = sz * (min(k + 1, nz) - k), zp2 = sz * (min(k + 2, nz) - k);
-        qs[4] = sq[c + zp2];
-        qr[2] = rho_at<T, TAB>(A, c + zp1);
+        qs[4] = qs_next;
+        qr[2] = rho_get<T, TAB>(A, rz_next);
+        const T rx_nb = rho_get<T, TAB>(A, rx_next);
         // read by all three wavefronts: cached loads
         T cf[6];
-        shell_coefs(A, blk, slot, cf);
+        shell_coefs_id(A, blk, cid_next, cf);
         T v;
-        if (w == 0) v = shell_velocity<T, CORR, TAB, 0>(A, L, qs, qr, c, blk, k, cf);
-        else if (w == 1) v = shell_velocity<T, CORR, TAB, 1>(A, L, qs, qr, c, blk, k, cf);
-        else v = shell_velocity<T, CORR, TAB, 2>(A, L, qs, qr, c, blk, k, cf);
+        if (w == 0) v = shell_velocity<T, CORR, TAB, 0>(A, L, qs, qr, c, blk, k, cf, rx_nb);
+        else if (w == 1) v = shell_velocity<T, CORR, TAB, 1>(A, L, qs, qr, c, blk, k, cf, rx_nb);
+        else v = shell_velocity<T, CORR, TAB, 2>(A, L, qs, qr, c, blk, k, cf, rx_nb);
+        if (k < tl.k1) {
+            const int64_t cn = c + sz;
+            qs_next = sq[cn + zp2_of(k + 1)];
+            rz_next = rho_pre<T, TAB>(A, cn + zp1_of(k + 1));
+            rx_next = rho_pre<T, TAB>(A, cn + xo);
+            cid_next = coef_id(A, slot + kShellLanes);
+        }
         const double dv = v;
         V2[b][w * kShellLanes + lane] = dv * dv;
         __syncthreads();
