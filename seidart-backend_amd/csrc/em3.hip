@@ -81,6 +81,9 @@ __device__ __forceinline__ void em3_e_update(const R &M, T exo, T eyo, T ezo, T 
     ez = ezo + (M[2] * rx + M[4] * ry + M[5] * rz) * dt;
 }
 
+// Hx, Hy, Hz (cpmlfdtd.f08:3700-3778).  Every operand is loaded before the
+// first store: the compiler cannot tell the arrays apart, so a store ahead of
+// a load would serialise the memory round trips of the three components.
 template <typename T, bool PML>
 __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
 {
@@ -93,48 +96,56 @@ __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, in
         A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0);
         return;
     }
+    const T ex0 = Ex[c], exy = Ex[c + sy], exz = Ex[c + sz];
+    const T ey0 = Ey[c], eyx = Ey[c + 1], eyz = Ey[c + sz];
+    const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
+    const T h0 = face ? T(0) : A.H[0][c], h1 = face ? T(0) : A.H[1][c], h2 = face ? T(0) : A.H[2][c];
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
+    T m_eydz = 0, m_ezdy = 0, m_exdz = 0, m_ezdx = 0, m_exdy = 0, m_eydx = 0;
     if (PML) {
         a = A.a[c]; b = A.b[c]; ik = A.ik[c];
         ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+        m_eydz = A.m_dEy_dz[c]; m_ezdy = A.m_dEz_dy[c];
+        m_exdz = A.m_dEx_dz[c]; m_ezdx = A.m_dEz_dx[c];
+        m_exdy = A.m_dEx_dy[c]; m_eydx = A.m_dEy_dx[c];
     }
-    {   // Hx: cpmlfdtd.f08:3700-3718
-        T dEz_dy = (Ez[c + sy] - Ez[c]) * A.rdy;
-        T dEy_dz = (Ey[c + sz] - Ey[c]) * A.rdz;
-        if (PML) {
-            A.m_dEy_dz[c] = bh * A.m_dEy_dz[c] + ah * dEy_dz;
-            A.m_dEz_dy[c] = bh * A.m_dEz_dy[c] + ah * dEz_dy;
-            dEz_dy = dEz_dy * ikh + A.m_dEz_dy[c];
-            dEy_dz = dEy_dz * ikh + A.m_dEy_dz[c];
-        }
-        const T hold = face ? T(0) : A.H[0][c];
-        A.H[0][c] = hold + A.dbH * (dEy_dz - dEz_dy);
+    // Hx: cpmlfdtd.f08:3700-3718
+    T dEz_dy = (ezy - ez0) * A.rdy;
+    T dEy_dz = (eyz - ey0) * A.rdz;
+    if (PML) {
+        m_eydz = bh * m_eydz + ah * dEy_dz;
+        m_ezdy = bh * m_ezdy + ah * dEz_dy;
+        dEz_dy = dEz_dy * ikh + m_ezdy;
+        dEy_dz = dEy_dz * ikh + m_eydz;
     }
-    {   // Hy: cpmlfdtd.f08:3730-3749
-        T dEx_dz = (Ex[c + sz] - Ex[c]) * A.rdz;
-        T dEz_dx = (Ez[c + 1] - Ez[c]) * A.rdx;
-        if (PML) {
-            A.m_dEx_dz[c] = b * A.m_dEx_dz[c] + a * dEx_dz;
-            A.m_dEz_dx[c] = b * A.m_dEz_dx[c] + a * dEz_dx;
-            dEx_dz = dEx_dz * ik + A.m_dEx_dz[c];
-            dEz_dx = dEz_dx * ik + A.m_dEz_dx[c];
-        }
-        const T hold = face ? T(0) : A.H[1][c];
-        A.H[1][c] = hold + A.dbH * (dEz_dx - dEx_dz);
+    const T hx = h0 + A.dbH * (dEy_dz - dEz_dy);
+    // Hy: cpmlfdtd.f08:3730-3749
+    T dEx_dz = (exz - ex0) * A.rdz;
+    T dEz_dx = (ezx - ez0) * A.rdx;
+    if (PML) {
+        m_exdz = b * m_exdz + a * dEx_dz;
+        m_ezdx = b * m_ezdx + a * dEz_dx;
+        dEx_dz = dEx_dz * ik + m_exdz;
+        dEz_dx = dEz_dx * ik + m_ezdx;
     }
-    if (k >= 2) {   // Hz: cpmlfdtd.f08:3761-3778 (k from 2)
-        T dEx_dy = (Ex[c + sy] - Ex[c]) * A.rdy;
-        T dEy_dx = (Ey[c + 1] - Ey[c]) * A.rdx;
-        if (PML) {
-            A.m_dEx_dy[c] = b * A.m_dEx_dy[c] + a * dEx_dy;
-            A.m_dEy_dx[c] = b * A.m_dEy_dx[c] + a * dEy_dx;
-            dEx_dy = dEx_dy * ik + A.m_dEx_dy[c];
-            dEy_dx = dEy_dx * ik + A.m_dEy_dx[c];
-        }
-        const T hold = face ? T(0) : A.H[2][c];
-        A.H[2][c] = hold + A.dbH * (dEx_dy - dEy_dx);
-    } else {
-        A.H[2][c] = T(0);
+    const T hy = h1 + A.dbH * (dEz_dx - dEx_dz);
+    // Hz: cpmlfdtd.f08:3761-3778 (k from 2)
+    T dEx_dy = (exy - ex0) * A.rdy;
+    T dEy_dx = (eyx - ey0) * A.rdx;
+    if (PML) {
+        m_exdy = b * m_exdy + a * dEx_dy;
+        m_eydx = b * m_eydx + a * dEy_dx;
+        dEx_dy = dEx_dy * ik + m_exdy;
+        dEy_dx = dEy_dx * ik + m_eydx;
+    }
+    const T hz = k >= 2 ? h2 + A.dbH * (dEx_dy - dEy_dx) : T(0);
+    A.H[0][c] = hx;
+    A.H[1][c] = hy;
+    A.H[2][c] = hz;
+    if (PML) {
+        A.m_dEy_dz[c] = m_eydz; A.m_dEz_dy[c] = m_ezdy;
+        A.m_dEx_dz[c] = m_exdz; A.m_dEz_dx[c] = m_ezdx;
+        if (k >= 2) { A.m_dEx_dy[c] = m_exdy; A.m_dEy_dx[c] = m_eydx; }
     }
 }
 
@@ -150,6 +161,8 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
     else em3_h_cell<T, false>(A, i, j, k);
 }
 
+// Ex, Ey, Ez (cpmlfdtd.f08:3795-3828), zero faces, and the field magnitude
+// for the norm; as in the H update, all loads come before the first store.
 template <typename T, bool PML, bool TAB>
 __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
 {
@@ -160,31 +173,39 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
     if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
         ex = ey = ez = T(0);
     } else {
-        // cpmlfdtd.f08:3795-3828
         const T *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
-        T dHz_dy = (Hz[c] - Hz[c - sy]) * A.rdy;
-        T dHy_dz = (Hy[c] - Hy[c - sz]) * A.rdz;
-        T dHz_dx = (Hz[c] - Hz[c - 1]) * A.rdx;
-        T dHx_dz = (Hx[c] - Hx[c - sz]) * A.rdz;
-        T dHy_dx = (Hy[c] - Hy[c - 1]) * A.rdx;
-        T dHx_dy = (Hx[c] - Hx[c - sy]) * A.rdy;
-        if (PML) {
-            const T a = A.a[c], b = A.b[c], ik = A.ik[c];
-            const T ah = A.ah[c], bh = A.bh[c], ikh = A.ikh[c];
-            A.m_dHz_dy[c] = bh * A.m_dHz_dy[c] + ah * dHz_dy;
-            A.m_dHy_dz[c] = b * A.m_dHy_dz[c] + a * dHy_dz;
-            A.m_dHz_dx[c] = bh * A.m_dHz_dx[c] + ah * dHz_dx;
-            A.m_dHx_dz[c] = bh * A.m_dHx_dz[c] + ah * dHx_dz;
-            A.m_dHx_dy[c] = bh * A.m_dHx_dy[c] + ah * dHx_dy;
-            A.m_dHy_dx[c] = bh * A.m_dHy_dx[c] + ah * dHy_dx;
-            dHz_dy = dHz_dy * ikh + A.m_dHz_dy[c];
-            dHy_dz = dHy_dz * ik + A.m_dHy_dz[c];
-            dHz_dx = dHz_dx * ikh + A.m_dHz_dx[c];
-            dHx_dz = dHx_dz * ikh + A.m_dHx_dz[c];
-            dHx_dy = dHx_dy * ikh + A.m_dHx_dy[c];
-            dHy_dx = dHy_dx * ikh + A.m_dHy_dx[c];
-        }
+        const T hx0 = Hx[c], hxy = Hx[c - sy], hxz = Hx[c - sz];
+        const T hy0 = Hy[c], hyx = Hy[c - 1], hyz = Hy[c - sz];
+        const T hz0 = Hz[c], hzx = Hz[c - 1], hzy = Hz[c - sy];
         const T exo = A.E[0][c], eyo = A.E[1][c], ezo = A.E[2][c];
+        T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
+        T m_hzdy = 0, m_hydz = 0, m_hzdx = 0, m_hxdz = 0, m_hxdy = 0, m_hydx = 0;
+        if (PML) {
+            a = A.a[c]; b = A.b[c]; ik = A.ik[c];
+            ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+            m_hzdy = A.m_dHz_dy[c]; m_hydz = A.m_dHy_dz[c]; m_hzdx = A.m_dHz_dx[c];
+            m_hxdz = A.m_dHx_dz[c]; m_hxdy = A.m_dHx_dy[c]; m_hydx = A.m_dHy_dx[c];
+        }
+        T dHz_dy = (hz0 - hzy) * A.rdy;
+        T dHy_dz = (hy0 - hyz) * A.rdz;
+        T dHz_dx = (hz0 - hzx) * A.rdx;
+        T dHx_dz = (hx0 - hxz) * A.rdz;
+        T dHy_dx = (hy0 - hyx) * A.rdx;
+        T dHx_dy = (hx0 - hxy) * A.rdy;
+        if (PML) {
+            m_hzdy = bh * m_hzdy + ah * dHz_dy;
+            m_hydz = b * m_hydz + a * dHy_dz;
+            m_hzdx = bh * m_hzdx + ah * dHz_dx;
+            m_hxdz = bh * m_hxdz + ah * dHx_dz;
+            m_hxdy = bh * m_hxdy + ah * dHx_dy;
+            m_hydx = bh * m_hydx + ah * dHy_dx;
+            dHz_dy = dHz_dy * ikh + m_hzdy;
+            dHy_dz = dHy_dz * ik + m_hydz;
+            dHz_dx = dHz_dx * ikh + m_hzdx;
+            dHx_dz = dHx_dz * ikh + m_hxdz;
+            dHx_dy = dHx_dy * ikh + m_hxdy;
+            dHy_dx = dHy_dx * ikh + m_hydx;
+        }
         const T cx = dHz_dy - dHy_dz, cy = dHx_dz - dHz_dx, cz = dHy_dx - dHx_dy;
         if (TAB) {
             const int m = A.mid[c];
@@ -199,6 +220,10 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
         } else {
             const EmDenseRow<T> M{&A, c};
             em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
+        }
+        if (PML) {
+            A.m_dHz_dy[c] = m_hzdy; A.m_dHy_dz[c] = m_hydz; A.m_dHz_dx[c] = m_hzdx;
+            A.m_dHx_dz[c] = m_hxdz; A.m_dHx_dy[c] = m_hxdy; A.m_dHy_dx[c] = m_hydx;
         }
     }
     A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;

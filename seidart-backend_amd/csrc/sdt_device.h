@@ -32,8 +32,13 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, doub
     atomicMax(slot, (unsigned long long)__double_as_longlong(v));
 }
 
-// Workgroup max-reduction of a non-negative value followed by one atomic per
-// workgroup.  blockDim.x*blockDim.y*blockDim.z must be a multiple of 64.
+// Workgroup max-reduction of a non-negative value followed by at most one
+// atomic per workgroup: the slot is read first (agent-scope atomic load, past
+// the CU's L1) and the atomic is skipped when it cannot raise the value.  Once
+// a wave fills the grid every workgroup has a non-zero maximum, and one atomic
+// from each of ~1e5 workgroups on a single address serialises at L2 (it
+// doubled the 3D EM E update); a stale read only costs an extra atomic.
+// blockDim.x*blockDim.y*blockDim.z must be a multiple of 64.
 __device__ __forceinline__ void block_max_to(unsigned long long *slot, double v)
 {
     __shared__ double wave_max[16];
@@ -49,7 +54,10 @@ __device__ __forceinline__ void block_max_to(unsigned long long *slot, double v)
         const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
         double m = wave_max[0];
         for (int w = 1; w < nw; ++w) m = (wave_max[w] > m || wave_max[w] != wave_max[w]) ? wave_max[w] : m;
-        if (m > 0.0 || m != m) atomic_max_nonneg(slot, m);
+        if (m > 0.0 || m != m) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+            if (bits > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(slot, bits);
+        }
     }
 }
 
