@@ -252,13 +252,17 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
     }
     qr[0] = rho_at<T, TAB>(A, c - sz);
     qr[1] = rho_at<T, TAB>(A, c);
+    // per-lane density loads of plane k (ids with the table), issued during
+    // plane k-1 after its own loads; the cached table reads follow in plane k
+    RhoPre<T> rz_next = rho_pre<T, TAB>(A, c + sz), rxm_next = rho_pre<T, TAB>(A, c - 1),
+              rxp_next = rho_pre<T, TAB>(A, c + 1);
     double nrm = 0.0;
     for (int k = kb; k <= ke; ++k, c += sz) {
         qxz[4] = sxz[c + 2 * sz];
         qyz[4] = syz[c + 2 * sz];
         qzz[4] = szz[c + 2 * sz];
-        qr[2] = rho_at<T, TAB>(A, c + sz);
-        const T r = qr[1], rxm = rho_at<T, TAB>(A, c - 1), rxp = rho_at<T, TAB>(A, c + 1);
+        qr[2] = rho_get<T, TAB>(A, rz_next);
+        const T r = qr[1], rxm = rho_get<T, TAB>(A, rxm_next), rxp = rho_get<T, TAB>(A, rxp_next);
         const T sxx_m2 = sxx[c - 2], sxx_m1 = sxx[c - 1], sxx_0 = sxx[c], sxx_p1 = CORR ? sxx[c + 1] : T(0);
         const T sxy_ym2 = sxy[c - 2 * sy], sxy_ym1 = sxy[c - sy], sxy_0 = sxy[c], sxy_yp1 = CORR ? sxy[c + sy] : T(0);
         const T sxy_xm1 = CORR ? sxy[c - 1] : T(0), sxy_xp1 = sxy[c + 1], sxy_xp2 = sxy[c + 2];
@@ -290,6 +294,11 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
             const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, j, k);
             const double a0 = vx, a1 = vy, a2 = vz;
             nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
+        }
+        if (k < ke) {
+            rz_next = rho_pre<T, TAB>(A, c + 2 * sz);
+            rxm_next = rho_pre<T, TAB>(A, c + sz - 1);
+            rxp_next = rho_pre<T, TAB>(A, c + sz + 1);
         }
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
