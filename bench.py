@@ -69,7 +69,8 @@ def timed_run(cfg, args, material_table, isotropic=1):
     K, W = args.steps, args.warmup
     KT = max(1, min(K, 100))
     s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K + KT, cfg["src"],
-               precision=args.precision, density_method=cfg.get("density_method", "arithmetic"))
+               precision=args.precision, density_method=cfg.get("density_method", "arithmetic"),
+               ky=cfg.get("ky", 0.0))
     s.set_option("zchunk", args.zchunk)
     s.set_option("stagger", args.stagger)
     s.set_option("tile_rows", args.tile_rows)
@@ -499,6 +500,54 @@ def bench_seismic3_iso(args):
     print(json.dumps(line), flush=True)
 
 
+EL25 = dict(F=9, M=32, PSI=21, C=6)   # complex fields and memory variables (2 values each), real materials
+
+
+def bench_seismic25(args):
+    """The 2.5D elastic solver on a 2000 x 1000 (+10-cell CPML) x-z grid: a
+    seeded layered model of rotated ice tensors (21 stiffness arrays per
+    cell), and a smooth isotropic model timed with the isotropic arrays and
+    with all 21."""
+    import numpy as np
+    from seidart_hip import models
+
+    K, W = args.steps, args.warmup
+    w = 8 if args.precision == "fp64" else 4
+    nx, nz = 2000, 1000
+    runs = {}
+    for name, cfg, iso in (
+            ("anisotropic", models.elastic25(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64), 1),
+            ("isotropic", models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64), 1),
+            ("isotropic_general_path", models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40),
+                                                            nrec=64), 0)):
+        elapsed, ms_s, ms_v, launches, _, used = timed_run(cfg, args, 1, isotropic=iso)
+        shape = cfg["shape"]
+        n = int(np.prod(shape))
+        n_int = (shape[0] - 2 * cfg["cpml"]) * (shape[1] - 2 * cfg["cpml"])
+        M = (3 + 6 + 5) if used else EL25["M"]
+        step_bytes = w * (n * (2 * 2 * EL25["F"] + M) + (n - n_int) * (2 * 2 * EL25["PSI"] + EL25["C"]))
+        runs[name] = {"value": round(n * K / elapsed / 1e9, 4), "ms_per_step": round(elapsed / K * 1e3, 4),
+                      "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+                      "stress_pass_ms": round(ms_s / max(launches, 1), 4),
+                      "velocity_pass_ms": round(ms_v / max(launches, 1), 4), "isotropic_arrays": bool(used),
+                      "step_bytes": step_bytes}
+        print(f"[seismic25] {name}: {runs[name]}", flush=True)
+    a = runs["anisotropic"]
+    line = {
+        "metric": "GCell-updates/s (2.5D elastic CPML-FDTD, " + ("FP64)" if args.precision == "fp64" else "FP32)"),
+        "value": a["value"], "unit": "GCell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": a["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded layered rotated-ice model; smooth isotropic model)",
+        "config": {"workload": f"seismic25 {nx}x{nz} + 10-cell CPML, complex fields, ky=0.05",
+                   "precision": args.precision},
+        "roofline": {"bound": "hbm", "kernel": "whole step (byte model)", "achieved": None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": a["hbm_fraction_step"], "traffic": None},
+        "cpu_baseline": None, "runs": runs,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -521,7 +570,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
     ap.add_argument("--workload", default="seismic3",
-                    choices=["seismic3", "seismic3iso", "electromag3", "batch2d", "fwi2d"],
+                    choices=["seismic3", "seismic3iso", "seismic25", "electromag3", "batch2d", "fwi2d"],
                     help="seismic3: the 3D elastic 256^3 case (default); electromag3: the 3D EM "
                          "384x384x192 case; batch2d: uneven multi-shot 2D batches against serial runs")
     ap.add_argument("--shots", type=int, default=64, help="batch2d: shots per solver kind (fwi2d: 16)")
@@ -562,6 +611,9 @@ def main():
         return
     if args.workload == "seismic3iso":
         bench_seismic3_iso(args)
+        return
+    if args.workload == "seismic25":
+        bench_seismic25(args)
         return
 
     import numpy as np

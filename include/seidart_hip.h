@@ -214,8 +214,8 @@ enum {
                                    kernels; set = each XCD takes a contiguous range of tiles,
                                    clear = the dispatcher's round-robin order (default 0: the
                                    round-robin order measured faster at 280^3)               */
-    SDT_OPT_ISOTROPIC = 10,     /* 3D elastic: 1 = when the per-cell arrays are in use (no material
-                                   table) and every cell's stiffness is isotropic in form
+    SDT_OPT_ISOTROPIC = 10,     /* 3D and 2.5D elastic: 1 = when the per-cell arrays are in use (no
+                                   material table) and every cell's stiffness is isotropic in form
                                    (c11 = c22 = c33, c12 = c13 = c23, c44 = c55 = c66, all
                                    other entries 0), keep only c11, c12, c44 and read 3 arrays
                                    per cell instead of 21 (default; same arithmetic, results
@@ -257,7 +257,7 @@ enum {
     SDT_INFO_SHELL_TILES = 2,   /* 3D elastic CPML-shell tiles                                    */
     SDT_INFO_DEVICE_BYTES = 3,  /* device memory held by the handle's grid arrays                 */
     SDT_INFO_COEFFICIENTS = 4,  /* update coefficients per cell (2D EM 7, 2D elastic 10, else 0)  */
-    SDT_INFO_ISOTROPIC = 5      /* 1 = the 3D elastic isotropic arrays (c11, c12, c44) are in use */
+    SDT_INFO_ISOTROPIC = 5      /* 1 = the isotropic arrays (c11, c12, c44) are in use (3D, 2.5D elastic) */
 };
 int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 

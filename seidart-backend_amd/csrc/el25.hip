@@ -47,13 +47,19 @@ template <typename T>
 __device__ __forceinline__ Cx<T> iky(T ky, Cx<T> f) { return cmk(-ky * f.im, ky * f.re); }
 // psi = b*psi + a*d ; d = d/kappa + psi (1/kappa precomputed)
 template <typename T>
-__device__ __forceinline__ Cx<T> cfix(Cx<T> d, Cx<T> *psi, int64_t c, T a, T b, T ik)
+__device__ __forceinline__ Cx<T> cfix(Cx<T> d, Cx<T> &m, T a, T b, T ik)   // m: the memory variable, updated in place
 {
-    Cx<T> m = psi[c];
     m.re = b * m.re + a * d.re;
     m.im = b * m.im + a * d.im;
-    psi[c] = m;
     return cmk(d.re * ik + m.re, d.im * ik + m.im);
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cfix_mem(Cx<T> d, Cx<T> *psi, int64_t c, T a, T b, T ik)   // psi[c] updated
+{
+    Cx<T> m = psi[c];
+    const Cx<T> r = cfix(d, m, a, b, ik);
+    psi[c] = m;
+    return r;
 }
 // 4th-order staggered difference 27(f1 - f0) - (f2 - fm1), scaled
 template <typename T>
@@ -129,8 +135,11 @@ __device__ __forceinline__ Cx<T> cstress(Cx<T> s, T k1, T k2, T k3, T k4, T k5, 
     return cmk((s.re + re * dt) * ig, (s.im + im * dt) * ig);
 }
 
-// the four stress nests, cpmlfdtd.f08:941-1113
-template <typename T, bool PML>
+// the four stress nests, cpmlfdtd.f08:941-1113.  ISO: isotropic-form
+// stiffness, only c11, c12, c44 read.  (Loading every operand before the
+// first store, as the 2D and velocity kernels do, needs about twice the
+// registers here and measured slower.)
+template <typename T, bool PML, bool ISO>
 __device__ __forceinline__ void el25_stress_cell(const El25Args<T> &A, int i, int k)
 {
     const int nx = A.nx, nz = A.nz;
@@ -155,64 +164,67 @@ __device__ __forceinline__ void el25_stress_cell(const El25Args<T> &A, int i, in
         dy[q] = iky(ky, v0);
         if (L1 || L4) {
             dxp[q] = cd4(v[c - 1], v0, v[c + 1], v[c + 2], rx);
-            if (PML) dxp[q] = cfix(dxp[q], A.m[P_XPH + q], c, ah, bh, ikh);
+            if (PML) dxp[q] = cfix_mem(dxp[q], A.m[P_XPH + q], c, ah, bh, ikh);
         }
         if (L2 || L3) {
             dxm[q] = cd4(v[c - 2], v[c - 1], v0, v[c + 1], rx);
-            if (PML) dxm[q] = cfix(dxm[q], A.m[P_XMF + q], c, a, b, ik);
+            if (PML) dxm[q] = cfix_mem(dxm[q], A.m[P_XMF + q], c, a, b, ik);
         }
         if (L1 || L2) {
             const Cx<T> d = cd4(v[c - 2 * sz], v[c - sz], v0, v[c + sz], rz);
             dzmF[q] = d;
             dzmH[q] = d;
-            if (PML && L1) dzmF[q] = cfix(d, A.m[P_ZMF + q], c, a, b, ik);
-            if (PML && L2) dzmH[q] = cfix(d, A.m[P_ZMH + q], c, ah, bh, ikh);
+            if (PML && L1) dzmF[q] = cfix_mem(d, A.m[P_ZMF + q], c, a, b, ik);
+            if (PML && L2) dzmH[q] = cfix_mem(d, A.m[P_ZMH + q], c, ah, bh, ikh);
         }
         if (L3 || L4) {
             dzp[q] = cd4(v[c - sz], v0, v[c + sz], v[c + 2 * sz], rz);
-            if (PML) dzp[q] = cfix(dzp[q], A.m[P_ZPH + q], c, ah, bh, ikh);
+            if (PML) dzp[q] = cfix_mem(dzp[q], A.m[P_ZPH + q], c, ah, bh, ikh);
         }
     }
     const T *const *M = A.mat;
+    T c11 = 0, c12 = 0, c44 = 0;
+    if (ISO) { c11 = M[0][c]; c12 = M[1][c]; c44 = M[15][c]; }
+    auto K = [&](int q) { return ISO ? iso_entry(q, c11, c12, c44) : M[q][c]; };
     // Voigt slots of the 21 stiffness arrays: c11 0, c12 1, c13 2, c14 3, c15 4,
     // c16 5, c22 6, c23 7, c24 8, c25 9, c26 10, c33 11, c34 12, c35 13, c36 14,
     // c44 15, c45 16, c46 17, c55 18, c56 19, c66 20
     if (L1) {
         const Cx<T> yz = dzmF[1] + dy[2], xz = dzmF[0] + dxp[2], xy = dxp[1] + dy[0];
-        A.s[0][c] = cstress(A.s[0][c], M[0][c], M[1][c], M[2][c], M[3][c], M[4][c], M[5][c], dxp[0], dy[1], dzmF[2],
+        A.s[0][c] = cstress(A.s[0][c], K(0), K(1), K(2), K(3), K(4), K(5), dxp[0], dy[1], dzmF[2],
                             yz, xz, xy, dt, M[M_IG + 0][c]);
-        A.s[1][c] = cstress(A.s[1][c], M[1][c], M[6][c], M[7][c], M[8][c], M[9][c], M[10][c], dxp[0], dy[1], dzmF[2],
+        A.s[1][c] = cstress(A.s[1][c], K(1), K(6), K(7), K(8), K(9), K(10), dxp[0], dy[1], dzmF[2],
                             yz, xz, xy, dt, M[M_IG + 1][c]);
-        A.s[2][c] = cstress(A.s[2][c], M[2][c], M[7][c], M[11][c], M[12][c], M[13][c], M[14][c], dxp[0], dy[1],
+        A.s[2][c] = cstress(A.s[2][c], K(2), K(7), K(11), K(12), K(13), K(14), dxp[0], dy[1],
                             dzmF[2], yz, xz, xy, dt, M[M_IG + 2][c]);
     }
     if (L2) {
         const Cx<T> yz = dzmH[1] + dy[2], xz = dzmH[0] + dxm[2], xy = dxm[1] + dy[0];
-        A.s[3][c] = cstress(A.s[3][c], M[5][c], M[10][c], M[14][c], M[17][c], M[19][c], M[20][c], dxm[0], dy[1],
+        A.s[3][c] = cstress(A.s[3][c], K(5), K(10), K(14), K(17), K(19), K(20), dxm[0], dy[1],
                             dzmH[2], yz, xz, xy, dt, M[M_IG + 3][c]);
     }
     if (L3) {
         const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxm[2], xy = dxm[1] + dy[0];
-        A.s[4][c] = cstress(A.s[4][c], M[4][c], M[9][c], M[13][c], M[16][c], M[18][c], M[19][c], dxm[0], dy[1],
+        A.s[4][c] = cstress(A.s[4][c], K(4), K(9), K(13), K(16), K(18), K(19), dxm[0], dy[1],
                             dzp[2], yz, xz, xy, dt, M[M_IG + 4][c]);
     }
     if (L4) {
         const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxp[2], xy = dxp[1] + dy[0];
-        A.s[5][c] = cstress(A.s[5][c], M[3][c], M[8][c], M[12][c], M[15][c], M[16][c], M[17][c], dxp[0], dy[1],
+        A.s[5][c] = cstress(A.s[5][c], K(3), K(8), K(12), K(15), K(16), K(17), dxp[0], dy[1],
                             dzp[2], yz, xz, xy, dt, M[M_IG + 5][c]);
     }
 }
 
-template <typename T>
+template <typename T, bool ISO>
 __global__ void __launch_bounds__(256) el25_stress_kernel(El25Args<T> A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
     const int k = blockIdx.y * 4 + threadIdx.y + 1;
     if (i > A.nx || k > A.nz) return;
     if (!in_box(A.box, i, 1, k) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y])
-        el25_stress_cell<T, true>(A, i, k);
+        el25_stress_cell<T, true, ISO>(A, i, k);
     else
-        el25_stress_cell<T, false>(A, i, k);
+        el25_stress_cell<T, false, ISO>(A, i, k);
 }
 
 // v += (d1 / rho_x + d2 / rho_y + d3 / rho_z) * dt
@@ -223,7 +235,8 @@ __device__ __forceinline__ Cx<T> vupd(Cx<T> v, Cx<T> d1, Cx<T> d2, Cx<T> d3, T r
 }
 
 // vx, vy, vz (cpmlfdtd.f08:1130-1200), Dirichlet edges (1280-1285) and the
-// per-step velocity norm (1291) without the source cells
+// per-step velocity norm (1291) without the source cells; loads before
+// stores as in the stress update
 template <typename T, bool PML>
 __device__ __forceinline__ double el25_velocity_cell(const El25Args<T> &A, int i, int k)
 {
@@ -246,38 +259,55 @@ __device__ __forceinline__ double el25_velocity_cell(const El25Args<T> &A, int i
             a = A.a[c]; b = A.b[c]; ik = A.ik[c];
             ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
         }
-        if (k >= 3 && k <= nz - 1 && i >= 3 && i <= nx - 1) {
+        const bool LX = k >= 3 && k <= nz - 1 && i >= 3 && i <= nx - 1;
+        const bool LY = k >= 3 && k <= nz - 1 && i >= 2 && i <= nx - 2;
+        const bool LZ = k >= 2 && k <= nz - 2 && i >= 2 && i <= nx - 2;
+        const Cx<T> z0 = cmk(T(0), T(0));
+        Cx<T> mx1 = z0, mx3 = z0, my1 = z0, my3 = z0, mz1 = z0, mz3 = z0;
+        if (PML) {
+            if (LX) { mx1 = A.m[P_VX_X][c]; mx3 = A.m[P_VX_Z][c]; }
+            if (LY) { my1 = A.m[P_VY_X][c]; my3 = A.m[P_VY_Z][c]; }
+            if (LZ) { mz1 = A.m[P_VZ_X][c]; mz3 = A.m[P_VZ_Z][c]; }
+        }
+        Cx<T> nvx = vx, nvy = vy, nvz = vz;
+        if (LX) {
             Cx<T> d1 = ctm(A.s[0], c, (int64_t)1, corr, rx);
             const Cx<T> d2 = iky(ky, A.s[3][c]);
             Cx<T> d3 = ctm(A.s[4], c, sz, corr, rz);
             if (PML) {
-                d1 = cfix(d1, A.m[P_VX_X], c, ah, bh, ikh);
-                d3 = cfix(d3, A.m[P_VX_Z], c, ah, bh, ikh);
+                d1 = cfix(d1, mx1, ah, bh, ikh);
+                d3 = cfix(d3, mx3, ah, bh, ikh);
             }
-            vx = vupd(vx, d1, d2, d3, M[M_RHO_PI][c], rho, M[M_RHO_PJ][c], dt);
-            A.v[0][c] = vx;
+            nvx = vupd(vx, d1, d2, d3, M[M_RHO_PI][c], rho, M[M_RHO_PJ][c], dt);
         }
-        if (k >= 3 && k <= nz - 1 && i >= 2 && i <= nx - 2) {
+        if (LY) {
             Cx<T> d1 = ctp(A.s[3], c, (int64_t)1, corr, rx);
             const Cx<T> d2 = iky(ky, A.s[1][c]);
             Cx<T> d3 = ctm(A.s[5], c, sz, corr, rz);
             if (PML) {
-                d1 = cfix(d1, A.m[P_VY_X], c, ah, bh, ikh);
-                d3 = cfix(d3, A.m[P_VY_Z], c, a, b, ik);
+                d1 = cfix(d1, my1, ah, bh, ikh);
+                d3 = cfix(d3, my3, a, b, ik);
             }
-            vy = vupd(vy, d1, d2, d3, M[M_RHO_MI][c], rho, M[M_RHO_PJ][c], dt);
-            A.v[1][c] = vy;
+            nvy = vupd(vy, d1, d2, d3, M[M_RHO_MI][c], rho, M[M_RHO_PJ][c], dt);
         }
-        if (k >= 2 && k <= nz - 2 && i >= 2 && i <= nx - 2) {
+        if (LZ) {
             Cx<T> d1 = ctp(A.s[4], c, (int64_t)1, corr, rx);
             const Cx<T> d2 = iky(ky, A.s[5][c]);
             Cx<T> d3 = ctp(A.s[2], c, sz, corr, rz);
             if (PML) {
-                d1 = cfix(d1, A.m[P_VZ_X], c, ah, bh, ikh);
-                d3 = cfix(d3, A.m[P_VZ_Z], c, ah, bh, ikh);
+                d1 = cfix(d1, mz1, ah, bh, ikh);
+                d3 = cfix(d3, mz3, ah, bh, ikh);
             }
-            vz = vupd(vz, d1, d2, d3, M[M_RHO_MI][c], rho, M[M_RHO_MJ][c], dt);
-            A.v[2][c] = vz;
+            nvz = vupd(vz, d1, d2, d3, M[M_RHO_MI][c], rho, M[M_RHO_MJ][c], dt);
+        }
+        // stores
+        if (LX) { vx = nvx; A.v[0][c] = vx; }
+        if (LY) { vy = nvy; A.v[1][c] = vy; }
+        if (LZ) { vz = nvz; A.v[2][c] = vz; }
+        if (PML) {
+            if (LX) { A.m[P_VX_X][c] = mx1; A.m[P_VX_Z][c] = mx3; }
+            if (LY) { A.m[P_VY_X][c] = my1; A.m[P_VY_Z][c] = my3; }
+            if (LZ) { A.m[P_VZ_X][c] = mz1; A.m[P_VZ_Z][c] = mz3; }
         }
     }
     if (c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, 1, k)) return 0.0;
@@ -340,7 +370,8 @@ static void el25_step_t(Device &d, bool timed)
     const El25Args<T> A = el25_args<T>(d);
     const dim3 grid(d.tiles_x, d.tiles_z), block(64, 4);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    hipLaunchKernelGGL(el25_stress_kernel<T>, grid, block, 0, d.stream, A);
+    if (d.mat_iso) hipLaunchKernelGGL((el25_stress_kernel<T, true>), grid, block, 0, d.stream, A);
+    else hipLaunchKernelGGL((el25_stress_kernel<T, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
     hipLaunchKernelGGL(el25_velocity_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
@@ -382,9 +413,10 @@ static void smear(std::vector<double> &a, int nx, int nz, int code, int dir)
             for (int i = 1; i <= nx; ++i) A(i, k) = mean(A(i, k), A(i, k + 1));
 }
 
-// Device material slots: c11..c66 (21), 1/(1 + gamma*dt) for x, y, z, xy,
-// xz, yz, then rho and its four averaged copies ('+i' rhoxx, '+j' rhozx =
-// rhozy, '-i' rhoxy = rhoxz, '-j' rhozz; cpmlfdtd.f08:889-898).
+// Device material slots: c11..c66 (21; only c11, c12, c44 for an
+// isotropic-form model), 1/(1 + gamma*dt) for x, y, z, xy, xz, yz, then rho
+// and its four averaged copies ('+i' rhoxx, '+j' rhozx = rhozy, '-i' rhoxy =
+// rhoxz, '-j' rhozz; cpmlfdtd.f08:889-898).
 int el25_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -396,9 +428,12 @@ int el25_setup(Device &d, const double *const *in)
     set_layout(d);
     const int64_t n = d.ncell;
     std::vector<double> zero((size_t)n, 0.0);
+    // isotropic form: only c11, c12, c44 (Voigt slots 0, 1, 15) are held
+    d.mat_iso = d.iso_opt && stiffness_isotropic(in, n);
     d.nmat = 0;
-    for (int id = SDT_C11; id <= SDT_C66; ++id) {
-        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
+    for (int id = SDT_C11; id <= SDT_C66; ++id, ++d.nmat) {
+        if (d.mat_iso && id != SDT_C11 && id != SDT_C12 && id != SDT_C44) continue;
+        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat]);
         if (rc) return rc;
     }
     const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};

@@ -101,27 +101,6 @@ static int el3_material_table(Device &d, const double *const *in)
     });
 }
 
-// Isotropic form of every cell's stiffness: c11 = c22 = c33, c12 = c13 = c23,
-// c44 = c55 = c66 and the other twelve entries 0 (exact comparisons; absent
-// arrays are 0).  The stress updates then read c11, c12, c44 only, with the
-// same products and sums as from the full tensor (the zero entries add exact
-// zeros).
-static bool el3_isotropic(const double *const *in, int64_t n)
-{
-    auto at = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
-    static const int zero_ids[] = {SDT_C14, SDT_C15, SDT_C16, SDT_C24, SDT_C25, SDT_C26,
-                                   SDT_C34, SDT_C35, SDT_C36, SDT_C45, SDT_C46, SDT_C56};
-    for (int64_t c = 0; c < n; ++c) {
-        const double a = at(SDT_C11, c), l = at(SDT_C12, c), m = at(SDT_C44, c);
-        if (at(SDT_C22, c) != a || at(SDT_C33, c) != a || at(SDT_C13, c) != l || at(SDT_C23, c) != l ||
-            at(SDT_C55, c) != m || at(SDT_C66, c) != m)
-            return false;
-        for (int id : zero_ids)
-            if (at(id, c) != 0.0) return false;
-    }
-    return true;
-}
-
 int el3_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -147,7 +126,7 @@ int el3_setup(Device &d, const double *const *in)
     d.nmat = 0;
     rc = d.mat_table_opt ? el3_material_table(d, in) : SDT_OK;
     if (rc) return rc;
-    d.mat_iso = !d.mat_table && d.iso_opt && el3_isotropic(in, n);
+    d.mat_iso = !d.mat_table && d.iso_opt && stiffness_isotropic(in, n);
     if (d.mat_iso) {
         // c11, c12, c44 in their Voigt slots 0, 1, 15; the other 18 slots stay null
         for (int id : {SDT_C11, SDT_C12, SDT_C44, SDT_RHO}) {

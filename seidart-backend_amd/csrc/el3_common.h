@@ -109,18 +109,6 @@ struct El3Tile {
 // arrays, the material table, or the isotropic arrays c11, c12, c44.
 enum { kMatDense = 0, kMatTable = 1, kMatIso = 2 };
 
-// Voigt slot q (c11 c12 c13 c14 c15 c16 c22 ... c66 order) of an isotropic-form
-// tensor: c11 = c22 = c33, c12 = c13 = c23, c44 = c55 = c66, the rest 0.  With
-// q a compile-time constant after unrolling, only the three arrays are read.
-template <typename T>
-__device__ __forceinline__ T iso_entry(int q, T c11, T c12, T c44)
-{
-    return (q == 0 || q == 6 || q == 11) ? c11
-         : (q == 1 || q == 2 || q == 7)   ? c12
-         : (q == 15 || q == 18 || q == 20) ? c44
-                                           : T(0);
-}
-
 constexpr int kTabRow = kEl3TabRow;   // material-table row length (elements)
 constexpr int kTabIg = 21, kTabRho = 27;
 constexpr int kMaxMaterials = 256;

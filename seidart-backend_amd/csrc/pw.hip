@@ -493,7 +493,7 @@ static void launch_plane_wave_t(Device &d)
     } else if (d.mat_iso) {   // isotropic form: slots of c11, c12, c44 repeat, the rest are 0
         static const int slot[21] = {0, 1, 1, -1, -1, -1, 0, 1, -1, -1, -1, 0, -1, -1, -1, 15, -1, -1, 15, -1, 15};
         for (int q = 0; q < 21; ++q) A.C[q] = slot[q] < 0 ? nullptr : (const T *)d.mat[slot[q]];
-        A.dim = 3;
+        A.dim = d3 ? 3 : 25;
     } else {
         for (int q = 0; q < 21; ++q) A.C[q] = (const T *)d.mat[q];
         A.dim = d3 ? 3 : 25;

@@ -509,4 +509,25 @@ int compute_tile_flags(Device &d, const double *const *host_in)
     return SDT_OK;
 }
 
+// Isotropic form of every cell's stiffness: c11 = c22 = c33, c12 = c13 = c23,
+// c44 = c55 = c66 and the other twelve entries 0 (exact comparisons; absent
+// arrays are 0).  The stress updates then read c11, c12, c44 only, with the
+// same products and sums as from the full tensor (the zero entries add exact
+// zeros).
+bool stiffness_isotropic(const double *const *in, int64_t n)
+{
+    auto at = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
+    static const int zero_ids[] = {SDT_C14, SDT_C15, SDT_C16, SDT_C24, SDT_C25, SDT_C26,
+                                   SDT_C34, SDT_C35, SDT_C36, SDT_C45, SDT_C46, SDT_C56};
+    for (int64_t c = 0; c < n; ++c) {
+        const double a = at(SDT_C11, c), l = at(SDT_C12, c), m = at(SDT_C44, c);
+        if (at(SDT_C22, c) != a || at(SDT_C33, c) != a || at(SDT_C13, c) != l || at(SDT_C23, c) != l ||
+            at(SDT_C55, c) != m || at(SDT_C66, c) != m)
+            return false;
+        for (int id : zero_ids)
+            if (at(id, c) != 0.0) return false;
+    }
+    return true;
+}
+
 }  // namespace sdt

@@ -314,6 +314,8 @@ void launch_tail(Device &d);
 // plane-wave source: geometry and background speed from the staged model
 // (before its arrays are released), and the per-step injection
 int plane_wave_setup(Device &d, const PlaneWaveSpec &spec, const double *const *host_in);
+// every cell's stiffness (c11..c66 inputs, absent = 0) has the isotropic form
+bool stiffness_isotropic(const double *const *host_in, int64_t n);
 void launch_plane_wave(Device &d);
 constexpr int kEl3TabRow = 32;         // 3D elastic material-table row length
 void launch_adj_tail(Device &d, int t, void *const lam_field[3]);

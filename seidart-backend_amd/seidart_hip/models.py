@@ -354,6 +354,41 @@ def elastic25(nx=120, nz=100, dx=1.0, dz=1.0, cpml=10, steps=400, seed=5, nlayer
                 receivers=recs, density_method=density_method, vmax=vmax)
 
 
+def elastic25_iso(nx=120, nz=100, dx=1.0, dz=1.0, cpml=10, steps=400, seed=12, f0=60.0, src=(60, 30), cfl=0.4,
+                  nrec=24, ky=0.05, density_method="harmonic", with_cpml=True):
+    """2.5D elastic, smoothly varying isotropic model with 0.5 % per-cell
+    jitter (as elastic3_iso, on the x-z grid); the repeated stiffness entries
+    share one array, the other twelve are absent."""
+    rng = np.random.default_rng(seed)
+    NX, NZ = nx + 2 * cpml, nz + 2 * cpml
+    shape = (NX, NZ)
+    x = np.linspace(0.0, 1.0, NX)[:, None]
+    z = np.linspace(0.0, 1.0, NZ)[None, :]
+    jit = lambda: 1.0 + 0.005 * rng.standard_normal(shape)
+    vp = (3000.0 + 1200.0 * z + 150.0 * np.sin(2 * math.pi * 1.5 * x)) * jit()
+    vs = vp / (1.8 + 0.1 * np.cos(2 * math.pi * z)) * jit()
+    rho = (1800.0 + 500.0 * z + 50.0 * np.sin(2 * math.pi * x)) * jit()
+    mu = np.asfortranarray(rho * vs * vs)
+    lam = np.asfortranarray(rho * (vp * vp - 2.0 * vs * vs))
+    c11 = np.asfortranarray(lam + 2.0 * mu)
+    arrays = {"c11": c11, "c22": c11, "c33": c11, "c12": lam, "c13": lam, "c23": lam,
+              "c44": mu, "c55": mu, "c66": mu, "rho": np.asfortranarray(rho)}
+    g = np.asfortranarray(0.002 + 0.006 * z * np.ones(shape))
+    for gname in ("gamma_x", "gamma_y", "gamma_z", "gamma_xy", "gamma_xz", "gamma_yz"):
+        arrays[gname] = g
+    vmax = float(np.sqrt(c11 / rho).max())
+    dt = cfl * min(dx, dz) / (math.sqrt(2.0) * vmax)
+    arrays.update(cpml_arrays(shape, (dx, dz), cpml, vmax, dt, f0) if with_cpml else no_cpml(shape))
+    for c in ("ic_x", "ic_y", "ic_z"):
+        arrays[c] = np.zeros(shape, dtype=np.complex128, order="F")
+    nrec = min(nrec, nx)
+    recs = np.array([[1 + (nx - 1) * k // max(nrec - 1, 1), 1, src[1]] for k in range(nrec)], dtype=np.int32)
+    return dict(physics="elastic25", shape=shape, spacing=(dx, dz), cpml=cpml, dt=dt, time_steps=steps,
+                src=src, arrays=arrays, ky=float(ky),
+                sources={"x": np.zeros(steps), "y": np.zeros(steps), "z": ricker(f0, dt, steps)},
+                receivers=recs, density_method=density_method, vmax=vmax)
+
+
 def em3(nx=384, ny=384, nz=192, d=0.05, cpml=10, steps=2000, seed=2, nlayers=5, f0=100e6,
         src=(192, 192, 20), cfl=0.5, nrec=128, with_cpml=True):
     """3D EM, five seeded SPD permittivity/conductivity layers (the

@@ -104,3 +104,43 @@ def test_partly_equal_attenuation_arrays(isotropic):
     print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k, v in errs.items():
         assert v <= 1e-10, (k, v)
+
+
+def _cfg25(**kw):
+    from seidart_hip import models
+    return models.elastic25_iso(nx=70, nz=56, steps=200, src=(35, 12), nrec=10, **kw)
+
+
+FIELDS25 = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
+
+def _run25(cfg, **options):
+    from seidart_hip.solver import make_solver
+    s = make_solver(cfg, options=options or None)
+    try:
+        s.run(cfg["time_steps"])
+        return s.receivers(), s.norms()[0], {f: s.field(f) for f in FIELDS25}, s.info("isotropic")
+    finally:
+        s.close()
+
+
+def test_isotropic_path_25d_matches_oracle_and_general_path():
+    """2.5D elastic: the isotropic arrays against the oracle and against the
+    21-array path on the same model."""
+    from oracle.oracle_ctypes import run_oracle
+    cfg = _cfg25()
+    print("[isotropic] 2.5D oracle parity and general path", flush=True)
+    a_rec, a_vn, a_f, a_iso = _run25(cfg)
+    b_rec, b_vn, b_f, b_iso = _run25(cfg, isotropic=0)
+    assert a_iso == 1 and b_iso == 0
+    o_rec, (o_vn, _), o_f, us = run_oracle(cfg, fields=FIELDS25)
+    assert us == 0 and np.abs(o_rec).max() > 0
+    errs = {"receivers": rel_l2(a_rec, o_rec), "velnorm": rel_l2(a_vn, o_vn)}
+    errs.update({k: rel_l2(a_f[k], o_f[k]) for k in FIELDS25})
+    gen = {"receivers": rel_l2(a_rec, b_rec), "velnorm": rel_l2(a_vn, b_vn)}
+    gen.update({k: rel_l2(a_f[k], b_f[k]) for k in FIELDS25})
+    print({k: f"{v:.2e}" for k, v in errs.items()}, {k: f"{v:.2e}" for k, v in gen.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-10, (k, v)
+    for k, v in gen.items():
+        assert v <= 1e-13, (k, v)
