@@ -361,6 +361,23 @@ __device__ __forceinline__ T *shell_block(const El3Args<T> &A, const El3Tile &tl
     return A.shell + (tl.base / kShellLanes) * (int64_t)(A.snq * kShellLanes) + lane;
 }
 
+// CPML coefficients of a shell slot, the source fixed at compile time (no
+// branch, so the scheduler can issue the id load with the plane's other
+// loads): from the dictionary (one-byte id, cached table) or from the
+// block's coefficient streams.
+template <typename T, bool CTAB>
+__device__ __forceinline__ void shell_coefs_t(const El3Args<T> &A, const T *blk, int64_t slot, T (&cf)[6])
+{
+    if (CTAB) {
+        const T *row = A.ctab + 8 * (int)A.cid[slot];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = row[r];
+    } else {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+    }
+}
+
 // CPML coefficients of a shell slot: from the dictionary (one-byte id, cached
 // table) or from the block's coefficient streams.
 template <typename T>
@@ -407,22 +424,33 @@ __device__ __forceinline__ T cpml_upd(T d, const T *psi_p, T a, T b, T ik, T &ps
 // (cpmlfdtd.f08:1761-1992) use seven (direction, stencil, coefficient set)
 // derivatives per component; each memory variable is updated once, on the
 // union of the ranges of the nests that read it.
-template <typename T, int M>
+// INSIDE: read the CPML coefficients here, after the velocity loads and in
+// their block, so the dictionary id's latency overlaps theirs (measured
+// faster with the isotropic arrays, slower with the table or all 21 arrays);
+// otherwise the caller passes them.
+template <typename T, int M, bool CTAB, bool INSIDE>
 __device__ __forceinline__ void shell_derivs(const El3Args<T> &A, const ShellCol &L, T (&q)[5], int64_t c,
                                              int64_t zp2, T *blk, bool L1, bool L2, bool L3, bool L4,
-                                             const T (&cf)[6], T *D, int lane)
+                                             int64_t slot, const T (&cf_in)[6], T *D, int lane)
 {
     const T *v = A.v[M];
     q[4] = v[c + zp2];
     const T v0 = q[2];
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2];
-    const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
     const T dxp = dp4(v[c + L.xm1], v0, v[c + L.xp1], v[c + L.xp2], rx);
     const T dxm = dm4(v[c + L.xm2], v[c + L.xm1], v0, v[c + L.xp1], rx);
     const T dyp = dp4(v[c + L.ym1], v0, v[c + L.yp1], v[c + L.yp2], ry);
     const T dym = dm4(v[c + L.ym2], v[c + L.ym1], v0, v[c + L.yp1], ry);
     const T dzp = dp4(q[1], v0, q[3], q[4], rz);
     const T dzm = dm4(q[0], q[1], v0, q[3], rz);
+    T cf[6];
+    if (INSIDE) {
+        shell_coefs_t<T, CTAB>(A, blk, slot, cf);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 6; ++r) cf[r] = cf_in[r];
+    }
+    const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
     T *mm = blk + M * K_NKIND * kShellLanes;   // this component's seven streams
     T *Dm = D + M * K_NKIND * kShellLanes + lane;
     T p0, p1, p2, p3, p4, p5 = T(0), p6;
@@ -580,7 +608,7 @@ __device__ __forceinline__ double shell_stresses_at(const El3Args<T> &A, int64_t
 // one LDS buffer) and two stresses of plane k (from the other), so the loads
 // of both phases are in flight together.  Buffers alternate per plane; every
 // buffer is rewritten only after the barrier that follows its last read.
-template <typename T, int MAT>
+template <typename T, int MAT, bool CTAB>
 __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
 {
     __shared__ T D[2][3 * K_NKIND * kShellLanes];
@@ -613,12 +641,14 @@ __global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Ar
         T *const blk = blk0 + (int64_t)(k - tl.k0) * bstride;
         bool L1, L2, L3, L4;
         nests(k, L1, L2, L3, L4);
-        // read by all three wavefronts: cached loads
+        const int64_t slot = tl.base + (int64_t)(k - tl.k0) * kShellLanes + lsel;
+        // coefficients read by all three wavefronts: cached loads
+        constexpr bool inside = MAT == kMatIso;
         T cf[6];
-        shell_coefs(A, blk, tl.base + (int64_t)(k - tl.k0) * kShellLanes + lsel, cf);
-        if (w == 0) shell_derivs<T, 0>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
-        else if (w == 1) shell_derivs<T, 1>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
-        else shell_derivs<T, 2>(A, L, q, c, zp2, blk, L1, L2, L3, L4, cf, D[b], lane);
+        if (!inside) shell_coefs_t<T, CTAB>(A, blk, slot, cf);
+        if (w == 0) shell_derivs<T, 0, CTAB, inside>(A, L, q, c, zp2, blk, L1, L2, L3, L4, slot, cf, D[b], lane);
+        else if (w == 1) shell_derivs<T, 1, CTAB, inside>(A, L, q, c, zp2, blk, L1, L2, L3, L4, slot, cf, D[b], lane);
+        else shell_derivs<T, 2, CTAB, inside>(A, L, q, c, zp2, blk, L1, L2, L3, L4, slot, cf, D[b], lane);
         q[0] = q[1]; q[1] = q[2]; q[2] = q[3]; q[3] = q[4];
     };
     phase1(tl.k0, 0);
@@ -841,9 +871,14 @@ static void fast_stress(Device &d, const El3Args<T> &A)
 {
     const int kc = el3_kc(d), ty = el3_ty(d);
     const hipStream_t ss = shell_stream_begin(d, 0);
-    if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss, A,
-                           (const El3Tile *)d.tiles);
+    if (d.ntiles > 0) {
+        if (A.ctab)
+            hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT, true>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss,
+                               A, (const El3Tile *)d.tiles);
+        else
+            hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT, false>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss,
+                               A, (const El3Tile *)d.tiles);
+    }
     if (!box_empty(d)) {
         if (ty == 16) int_stress<T, 16, MAT>(d, A, kc);
         else if (ty == 8) int_stress<T, 8, MAT>(d, A, kc);
