@@ -59,17 +59,19 @@ def algorithmic_bytes(shape, cpml, w, table=False, iso=False, n_ig=6):
     return step, stress
 
 
-def timed_run(cfg, args, material_table, isotropic=1):
+def timed_run(cfg, args, material_table, isotropic=1, precision=None):
     """W untimed steps (the first uploads every input), then K steps timed by
     the wall clock between synchronisations (captured step graphs, no per-
     kernel events), then KT more steps with HIP events around each kernel pass
-    on the solver's stream for the per-pass durations."""
+    on the solver's stream for the per-pass durations.  Returns the timings,
+    the material count, whether the isotropic arrays were used and the
+    receiver traces of all W + K + KT steps."""
     import numpy as np
     from seidart_hip.solver import Solver
     K, W = args.steps, args.warmup
     KT = max(1, min(K, 100))
     s = Solver(cfg["physics"], cfg["shape"], cfg["spacing"], cfg["cpml"], cfg["dt"], W + K + KT, cfg["src"],
-               precision=args.precision, density_method=cfg.get("density_method", "arithmetic"),
+               precision=precision or args.precision, density_method=cfg.get("density_method", "arithmetic"),
                ky=cfg.get("ky", 0.0))
     s.set_option("zchunk", args.zchunk)
     s.set_option("stagger", args.stagger)
@@ -99,7 +101,15 @@ def timed_run(cfg, args, material_table, isotropic=1):
     s.close()
     if not np.all(np.isfinite(rec)):
         raise RuntimeError("non-finite receiver traces")
-    return elapsed, ms_stress, ms_vel, launches, nmat, iso
+    return elapsed, ms_stress, ms_vel, launches, nmat, iso, rec
+
+
+def fp32_trace_error(cfg, args, rec32):
+    """Relative L2 difference of the FP32 receiver traces from an FP64 run of
+    the same workload and step count."""
+    import numpy as np
+    rec64 = timed_run(cfg, args, args.material_table, precision="fp64")[6]
+    return float(f"{np.linalg.norm((rec32 - rec64).ravel()) / np.linalg.norm(rec64.ravel()):.3e}")
 
 
 def measured_traffic():
@@ -149,7 +159,7 @@ def bench_em3(args):
     shape = cfg["shape"]
     ncell = int(np.prod(shape))
     w = 8 if args.precision == "fp64" else 4
-    elapsed, ms_h, ms_e, launches, nmat, _ = timed_run(cfg, args, args.material_table)
+    elapsed, ms_h, ms_e, launches, nmat, _, rec = timed_run(cfg, args, args.material_table)
     nx, ny, nz = shape
     cp = cfg["cpml"]
     n_pml = ncell - (nx - 2 * cp) * (ny - 2 * cp) * (nz - 2 * cp)
@@ -174,6 +184,8 @@ def bench_em3(args):
                      "h_kernel_avg_ms": round(ms_h / max(launches, 1), 4)},
         "cpu_baseline": None,
     }
+    if args.precision == "fp32":
+        line["fp32_vs_fp64_trace_rel_l2"] = fp32_trace_error(cfg, args, rec)
     print(json.dumps(line), flush=True)
 
 
@@ -458,7 +470,7 @@ def bench_seismic3_iso(args):
     fast = timed_run(cfg, args, 1, isotropic=1)
     general = timed_run(cfg, args, 1, isotropic=0)
     if fast[5] != 1 or general[5] != 0 or fast[4] != 0:
-        raise RuntimeError(f"isotropic path not selected as expected: {fast[4:]}, {general[4:]}")
+        raise RuntimeError(f"isotropic path not selected as expected: {fast[4:6]}, {general[4:6]}")
 
     gam = [cfg["arrays"].get(g) for g in ("gamma_x", "gamma_y", "gamma_z", "gamma_xy", "gamma_xz", "gamma_yz")]
     distinct = []
@@ -520,7 +532,7 @@ def bench_seismic25(args):
             ("isotropic", models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64), 1),
             ("isotropic_general_path", models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40),
                                                             nrec=64), 0)):
-        elapsed, ms_s, ms_v, launches, _, used = timed_run(cfg, args, 1, isotropic=iso)
+        elapsed, ms_s, ms_v, launches, _, used, _ = timed_run(cfg, args, 1, isotropic=iso)
         shape = cfg["shape"]
         n = int(np.prod(shape))
         n_int = (shape[0] - 2 * cfg["cpml"]) * (shape[1] - 2 * cfg["cpml"])
@@ -630,7 +642,7 @@ def main():
 
     main_run = timed_run(cfg, args, args.material_table)
     dense_run = timed_run(cfg, args, 0) if (args.material_table and args.compare_layouts) else None
-    elapsed, ms_stress, ms_vel, launches, nmat, _ = main_run
+    elapsed, ms_stress, ms_vel, launches, nmat, _, rec = main_run
 
     value = ncell * K / elapsed / 1e9
     step_bytes, stress_bytes = algorithmic_bytes(shape, cfg["cpml"], w)
@@ -690,11 +702,13 @@ def main():
         "setup_s": round(t_gen, 2),
     }
     if dense_run is not None:
-        e2, s2, v2, l2, _, _ = dense_run
+        e2, s2, v2, l2, _, _, _ = dense_run
         line["per_cell_arrays_layout"] = {
             "value": round(ncell * K / e2 / 1e9, 4), "ms_per_step": round(e2 / K * 1e3, 4),
             "hbm_fraction_step": round(step_bytes * K / e2 / 1e9 / HBM_PEAK_GBS, 4),
             "stress_pass_ms": round(s2 / max(l2, 1), 4), "velocity_pass_ms": round(v2 / max(l2, 1), 4)}
+    if args.precision == "fp32":
+        line["fp32_vs_fp64_trace_rel_l2"] = fp32_trace_error(cfg, args, rec)
     print(json.dumps(line), flush=True)
 
 

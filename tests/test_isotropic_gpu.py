@@ -144,3 +144,21 @@ def test_isotropic_path_25d_matches_oracle_and_general_path():
         assert v <= 1e-10, (k, v)
     for k, v in gen.items():
         assert v <= 1e-13, (k, v)
+
+
+def test_isotropic_path_25d_plane_wave_matches_oracle():
+    """2.5D isotropic arrays with a plane-wave source (the injection reads the
+    stiffness through the isotropic slots)."""
+    from oracle.oracle_ctypes import run_oracle
+    cfg = _cfg25()
+    cfg["plane_wave"] = dict(kind="p", angles=(25.0, 0.0, 10.0), f0=300.0, amplitude=1e-3, wavelet=2)
+    print("[isotropic] 2.5D plane wave", flush=True)
+    g_rec, g_vn, g_f, iso = _run25(cfg)
+    assert iso == 1
+    o_rec, (o_vn, _), o_f, us = run_oracle(cfg, fields=FIELDS25)
+    assert us == 0 and np.abs(o_rec).max() > 0
+    errs = {"receivers": rel_l2(g_rec, o_rec), "velnorm": rel_l2(g_vn, o_vn)}
+    errs.update({k: rel_l2(g_f[k], o_f[k]) for k in FIELDS25})
+    print({k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= 1e-10, (k, v)
