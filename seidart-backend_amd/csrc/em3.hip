@@ -26,6 +26,7 @@ struct Em3Args {
     T *m_dEy_dz, *m_dEz_dy, *m_dEx_dz, *m_dEz_dx, *m_dEx_dy, *m_dEy_dx;
     T *m_dHz_dy, *m_dHy_dz, *m_dHz_dx, *m_dHx_dz, *m_dHx_dy, *m_dHy_dx;
     const uint8_t *flags;
+    int any_flags;      // 0: no tile inside the box holds a CPML cell (the flags need no load)
     int box[6];
     int nx, ny, nz, tiles_x, tiles_y, pitch;
     int64_t xoff;
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
     const int k = blockIdx.z + 1;
     if (i > A.nx || j > A.ny) return;
     const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-    if (!in_box(A.box, i, j, k) || A.flags[t]) em3_h_cell<T, true>(A, i, j, k);
+    if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) em3_h_cell<T, true>(A, i, j, k);
     else em3_h_cell<T, false>(A, i, j, k);
 }
 
@@ -241,7 +242,7 @@ __global__ void __launch_bounds__(256) em3_e_kernel(Em3Args<T> A)
     double nrm = 0.0;
     if (i <= A.nx && j <= A.ny) {
         const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (!in_box(A.box, i, j, k) || A.flags[t]) nrm = em3_e_cell<T, true, TAB>(A, i, j, k);
+        if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) nrm = em3_e_cell<T, true, TAB>(A, i, j, k);
         else nrm = em3_e_cell<T, false, TAB>(A, i, j, k);
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
@@ -270,6 +271,7 @@ static Em3Args<T> em3_args(Device &d)
                 &A.m_dHz_dy, &A.m_dHy_dz, &A.m_dHz_dx, &A.m_dHx_dz, &A.m_dHx_dy, &A.m_dHy_dx};
     for (int q = 0; q < 12; ++q) *mm[q] = (T *)d.mem[q];
     A.flags = d.tile_flags;
+    A.any_flags = d.nflagged > 0 ? 1 : 0;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
     A.pitch = d.pitch; A.xoff = d.xoff;
