@@ -191,6 +191,16 @@ def test_em3_fp64(options):
     _check(g, o, TOL64)
 
 
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+def test_em3_odd_plane_count(options):
+    """An odd padded grid (46 x 42 x 39) with its point source near the
+    bottom CPML slab."""
+    from seidart_hip import models
+    cfg = models.em3(nx=34, ny=30, nz=27, cpml=6, steps=150, src=(17, 15, 9), nrec=8)
+    g, o = _run_both(cfg, ("ex", "ey", "ez", "hx", "hy", "hz"), options=options)
+    _check(g, o, TOL64)
+
+
 def test_em3_heterogeneous_permittivity():
     """More than 256 distinct materials: the per-cell arrays are used."""
     from seidart_hip import models
