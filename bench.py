@@ -517,9 +517,9 @@ EL25 = dict(F=9, M=32, PSI=21, C=6)   # complex fields and memory variables (2 v
 
 def bench_seismic25(args):
     """The 2.5D elastic solver on a 2000 x 1000 (+10-cell CPML) x-z grid: a
-    seeded layered model of rotated ice tensors (21 stiffness arrays per
-    cell), and a smooth isotropic model timed with the isotropic arrays and
-    with all 21."""
+    seeded layered model of rotated ice tensors (21 stiffness entries per
+    cell) timed with the material table and with the per-cell arrays, and a
+    smooth isotropic model timed with the isotropic arrays and with all 21."""
     import numpy as np
     from seidart_hip import models
 
@@ -527,12 +527,14 @@ def bench_seismic25(args):
     w = 8 if args.precision == "fp64" else 4
     nx, nz = 2000, 1000
     runs = {}
-    for name, cfg, iso in (
-            ("anisotropic", models.elastic25(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64), 1),
-            ("isotropic", models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64), 1),
-            ("isotropic_general_path", models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40),
-                                                            nrec=64), 0)):
-        elapsed, ms_s, ms_v, launches, _, used, _ = timed_run(cfg, args, 1, isotropic=iso)
+    layered = models.elastic25(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64)
+    smooth = models.elastic25_iso(nx=nx, nz=nz, steps=W + K, src=(nx // 2, 40), nrec=64)
+    for name, cfg, table, iso in (
+            ("anisotropic", layered, args.material_table, 1),
+            ("anisotropic_per_cell_arrays", layered, 0, 1),
+            ("isotropic", smooth, 1, 1),
+            ("isotropic_general_path", smooth, 1, 0)):
+        elapsed, ms_s, ms_v, launches, nmat, used, _ = timed_run(cfg, args, table, isotropic=iso)
         shape = cfg["shape"]
         n = int(np.prod(shape))
         n_int = (shape[0] - 2 * cfg["cpml"]) * (shape[1] - 2 * cfg["cpml"])
@@ -542,7 +544,7 @@ def bench_seismic25(args):
                       "hbm_fraction_step": round(step_bytes * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
                       "stress_pass_ms": round(ms_s / max(launches, 1), 4),
                       "velocity_pass_ms": round(ms_v / max(launches, 1), 4), "isotropic_arrays": bool(used),
-                      "step_bytes": step_bytes}
+                      "materials_in_table": nmat, "step_bytes": step_bytes}
         print(f"[seismic25] {name}: {runs[name]}", flush=True)
     a = runs["anisotropic"]
     line = {
@@ -552,7 +554,9 @@ def bench_seismic25(args):
         "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic (seeded layered rotated-ice model; smooth isotropic model)",
         "config": {"workload": f"seismic25 {nx}x{nz} + 10-cell CPML, complex fields, ky=0.05",
-                   "precision": args.precision},
+                   "precision": args.precision,
+                   "material_layout": f"table ({a['materials_in_table']} materials, 1-byte id per cell)"
+                                      if a["materials_in_table"] else "per-cell arrays"},
         "roofline": {"bound": "hbm", "kernel": "whole step (byte model)", "achieved": None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": a["hbm_fraction_step"], "traffic": None},
         "cpu_baseline": None, "runs": runs,

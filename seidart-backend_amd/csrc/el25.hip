@@ -122,6 +122,43 @@ struct El25Args {
     unsigned long long *vnorm;
     int64_t vfix[3];      // source cells excluded from the fused norm
     PwFaces pwf;          // plane-wave cells, likewise
+    // material table (at most 256 distinct (c11..c66, attenuation) tuples):
+    // id per padded cell, rows of kEl25TabRow (21 stiffness, 6 divisors)
+    const uint8_t *mid;
+    const T *tab;
+};
+
+// Where the stress update reads a cell's stiffness and attenuation divisors
+// from: the 27 per-cell arrays, the material table, or the isotropic arrays
+// c11, c12, c44 with per-cell divisors.
+enum { kEl25Dense = 0, kEl25Table = 1, kEl25Iso = 2 };
+
+// Row accessors: entry q < 21 is Voigt slot q of the stiffness, 21 + q the
+// divisor of nest q (M_IG + q).
+template <typename T>
+using el25_const_row = __attribute__((address_space(4))) const T *;
+template <typename T>
+struct El25ScalarRow {   // wave-uniform table row, scalar loads
+    el25_const_row<T> p;
+    __device__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct El25LaneRow {     // per-lane table row (cached)
+    const T *p;
+    __device__ T operator[](int q) const { return p[q]; }
+};
+template <typename T>
+struct El25DenseRow {    // per-cell arrays
+    const T *const *M;
+    int64_t c;
+    __device__ T operator[](int q) const { return M[q][c]; }
+};
+template <typename T>
+struct El25IsoRow {      // isotropic form: c11, c12, c44, per-cell divisors
+    const T *const *M;
+    int64_t c;
+    T c11, c12, c44;
+    __device__ T operator[](int q) const { return q < 21 ? iso_entry(q, c11, c12, c44) : M[q][c]; }
 };
 
 // sigma = (sigma + (K1 d1 + K2 d2 + K3 d3 + K4 yz + K5 xz + K6 xy) dt) * ig,
@@ -135,11 +172,48 @@ __device__ __forceinline__ Cx<T> cstress(Cx<T> s, T k1, T k2, T k3, T k4, T k5, 
     return cmk((s.re + re * dt) * ig, (s.im + im * dt) * ig);
 }
 
-// the four stress nests, cpmlfdtd.f08:941-1113.  ISO: isotropic-form
-// stiffness, only c11, c12, c44 read.  (Loading every operand before the
-// first store, as the 2D and velocity kernels do, needs about twice the
-// registers here and measured slower.)
-template <typename T, bool PML, bool ISO>
+// The six stress updates of the four nests (cpmlfdtd.f08:974-1113) from the
+// cell's derivatives; K is the row accessor of the cell's material.
+// Voigt slots of the 21 stiffness entries: c11 0, c12 1, c13 2, c14 3, c15 4,
+// c16 5, c22 6, c23 7, c24 8, c25 9, c26 10, c33 11, c34 12, c35 13, c36 14,
+// c44 15, c45 16, c46 17, c55 18, c56 19, c66 20
+template <typename T, typename R>
+__device__ __forceinline__ void el25_stresses(const El25Args<T> &A, int64_t c, const R &K, bool L1, bool L2,
+                                              bool L3, bool L4, const Cx<T> (&dy)[3], const Cx<T> (&dxp)[3],
+                                              const Cx<T> (&dxm)[3], const Cx<T> (&dzmF)[3],
+                                              const Cx<T> (&dzmH)[3], const Cx<T> (&dzp)[3])
+{
+    const T dt = A.dt;
+    if (L1) {
+        const Cx<T> yz = dzmF[1] + dy[2], xz = dzmF[0] + dxp[2], xy = dxp[1] + dy[0];
+        A.s[0][c] = cstress(A.s[0][c], K[0], K[1], K[2], K[3], K[4], K[5], dxp[0], dy[1], dzmF[2],
+                            yz, xz, xy, dt, K[M_IG + 0]);
+        A.s[1][c] = cstress(A.s[1][c], K[1], K[6], K[7], K[8], K[9], K[10], dxp[0], dy[1], dzmF[2],
+                            yz, xz, xy, dt, K[M_IG + 1]);
+        A.s[2][c] = cstress(A.s[2][c], K[2], K[7], K[11], K[12], K[13], K[14], dxp[0], dy[1],
+                            dzmF[2], yz, xz, xy, dt, K[M_IG + 2]);
+    }
+    if (L2) {
+        const Cx<T> yz = dzmH[1] + dy[2], xz = dzmH[0] + dxm[2], xy = dxm[1] + dy[0];
+        A.s[3][c] = cstress(A.s[3][c], K[5], K[10], K[14], K[17], K[19], K[20], dxm[0], dy[1],
+                            dzmH[2], yz, xz, xy, dt, K[M_IG + 3]);
+    }
+    if (L3) {
+        const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxm[2], xy = dxm[1] + dy[0];
+        A.s[4][c] = cstress(A.s[4][c], K[4], K[9], K[13], K[16], K[18], K[19], dxm[0], dy[1],
+                            dzp[2], yz, xz, xy, dt, K[M_IG + 4]);
+    }
+    if (L4) {
+        const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxp[2], xy = dxp[1] + dy[0];
+        A.s[5][c] = cstress(A.s[5][c], K[3], K[8], K[12], K[15], K[16], K[17], dxp[0], dy[1],
+                            dzp[2], yz, xz, xy, dt, K[M_IG + 5]);
+    }
+}
+
+// the four stress nests, cpmlfdtd.f08:941-1113.  (Loading every operand
+// before the first store, as the 2D and velocity kernels do, needs about
+// twice the registers here and measured slower.)
+template <typename T, bool PML, int MAT>
 __device__ __forceinline__ void el25_stress_cell(const El25Args<T> &A, int i, int k)
 {
     const int nx = A.nx, nz = A.nz;
@@ -150,7 +224,9 @@ __device__ __forceinline__ void el25_stress_cell(const El25Args<T> &A, int i, in
     const bool L3 = k >= 2 && k <= nz - 2 && i >= 3 && i <= nx - 1;
     const bool L4 = k >= 2 && k <= nz - 2 && i >= 2 && i <= nx - 2;
     if (!(L1 || L2 || L3 || L4)) return;
-    const T rx = A.rdx24, rz = A.rdz24, ky = A.ky, dt = A.dt;
+    // the material id first: its row loads wait on it
+    const int m = MAT == kEl25Table ? (int)A.mid[c] : 0;
+    const T rx = A.rdx24, rz = A.rdz24, ky = A.ky;
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
     if (PML) {
         a = A.a[c]; b = A.b[c]; ik = A.ik[c];
@@ -182,49 +258,35 @@ __device__ __forceinline__ void el25_stress_cell(const El25Args<T> &A, int i, in
             if (PML) dzp[q] = cfix_mem(dzp[q], A.m[P_ZPH + q], c, ah, bh, ikh);
         }
     }
-    const T *const *M = A.mat;
-    T c11 = 0, c12 = 0, c44 = 0;
-    if (ISO) { c11 = M[0][c]; c12 = M[1][c]; c44 = M[15][c]; }
-    auto K = [&](int q) { return ISO ? iso_entry(q, c11, c12, c44) : M[q][c]; };
-    // Voigt slots of the 21 stiffness arrays: c11 0, c12 1, c13 2, c14 3, c15 4,
-    // c16 5, c22 6, c23 7, c24 8, c25 9, c26 10, c33 11, c34 12, c35 13, c36 14,
-    // c44 15, c45 16, c46 17, c55 18, c56 19, c66 20
-    if (L1) {
-        const Cx<T> yz = dzmF[1] + dy[2], xz = dzmF[0] + dxp[2], xy = dxp[1] + dy[0];
-        A.s[0][c] = cstress(A.s[0][c], K(0), K(1), K(2), K(3), K(4), K(5), dxp[0], dy[1], dzmF[2],
-                            yz, xz, xy, dt, M[M_IG + 0][c]);
-        A.s[1][c] = cstress(A.s[1][c], K(1), K(6), K(7), K(8), K(9), K(10), dxp[0], dy[1], dzmF[2],
-                            yz, xz, xy, dt, M[M_IG + 1][c]);
-        A.s[2][c] = cstress(A.s[2][c], K(2), K(7), K(11), K(12), K(13), K(14), dxp[0], dy[1],
-                            dzmF[2], yz, xz, xy, dt, M[M_IG + 2][c]);
-    }
-    if (L2) {
-        const Cx<T> yz = dzmH[1] + dy[2], xz = dzmH[0] + dxm[2], xy = dxm[1] + dy[0];
-        A.s[3][c] = cstress(A.s[3][c], K(5), K(10), K(14), K(17), K(19), K(20), dxm[0], dy[1],
-                            dzmH[2], yz, xz, xy, dt, M[M_IG + 3][c]);
-    }
-    if (L3) {
-        const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxm[2], xy = dxm[1] + dy[0];
-        A.s[4][c] = cstress(A.s[4][c], K(4), K(9), K(13), K(16), K(18), K(19), dxm[0], dy[1],
-                            dzp[2], yz, xz, xy, dt, M[M_IG + 4][c]);
-    }
-    if (L4) {
-        const Cx<T> yz = dzp[1] + dy[2], xz = dzp[0] + dxp[2], xy = dxp[1] + dy[0];
-        A.s[5][c] = cstress(A.s[5][c], K(3), K(8), K(12), K(15), K(16), K(17), dxp[0], dy[1],
-                            dzp[2], yz, xz, xy, dt, M[M_IG + 5][c]);
+    if (MAT == kEl25Table) {
+        const int m0 = __builtin_amdgcn_readfirstlane(m);
+        if (__all(m == m0)) {
+            const El25ScalarRow<T> K{(el25_const_row<T>)(A.tab) + m0 * kEl25TabRow};
+            el25_stresses(A, c, K, L1, L2, L3, L4, dy, dxp, dxm, dzmF, dzmH, dzp);
+        } else {
+            const El25LaneRow<T> K{A.tab + m * kEl25TabRow};
+            el25_stresses(A, c, K, L1, L2, L3, L4, dy, dxp, dxm, dzmF, dzmH, dzp);
+        }
+    } else if (MAT == kEl25Iso) {
+        const T *const *M = A.mat;
+        const El25IsoRow<T> K{M, c, M[0][c], M[1][c], M[15][c]};
+        el25_stresses(A, c, K, L1, L2, L3, L4, dy, dxp, dxm, dzmF, dzmH, dzp);
+    } else {
+        const El25DenseRow<T> K{A.mat, c};
+        el25_stresses(A, c, K, L1, L2, L3, L4, dy, dxp, dxm, dzmF, dzmH, dzp);
     }
 }
 
-template <typename T, bool ISO>
+template <typename T, int MAT>
 __global__ void __launch_bounds__(256) el25_stress_kernel(El25Args<T> A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
     const int k = blockIdx.y * 4 + threadIdx.y + 1;
     if (i > A.nx || k > A.nz) return;
     if (!in_box(A.box, i, 1, k) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y])
-        el25_stress_cell<T, true, ISO>(A, i, k);
+        el25_stress_cell<T, true, MAT>(A, i, k);
     else
-        el25_stress_cell<T, false, ISO>(A, i, k);
+        el25_stress_cell<T, false, MAT>(A, i, k);
 }
 
 // v += (d1 / rho_x + d2 / rho_y + d3 / rho_z) * dt
@@ -361,6 +423,8 @@ static El25Args<T> el25_args(Device &d)
     A.vnorm = d.norms;
     for (int q = 0; q < 3; ++q) A.vfix[q] = d.tail.vfix[q];
     A.pwf = d.pw.faces();
+    A.mid = d.mid;
+    A.tab = (const T *)d.tab;
     return A;
 }
 
@@ -370,8 +434,9 @@ static void el25_step_t(Device &d, bool timed)
     const El25Args<T> A = el25_args<T>(d);
     const dim3 grid(d.tiles_x, d.tiles_z), block(64, 4);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    if (d.mat_iso) hipLaunchKernelGGL((el25_stress_kernel<T, true>), grid, block, 0, d.stream, A);
-    else hipLaunchKernelGGL((el25_stress_kernel<T, false>), grid, block, 0, d.stream, A);
+    if (d.mat_table) hipLaunchKernelGGL((el25_stress_kernel<T, kEl25Table>), grid, block, 0, d.stream, A);
+    else if (d.mat_iso) hipLaunchKernelGGL((el25_stress_kernel<T, kEl25Iso>), grid, block, 0, d.stream, A);
+    else hipLaunchKernelGGL((el25_stress_kernel<T, kEl25Dense>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
     hipLaunchKernelGGL(el25_velocity_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
@@ -416,7 +481,10 @@ static void smear(std::vector<double> &a, int nx, int nz, int code, int dir)
 // Device material slots: c11..c66 (21; only c11, c12, c44 for an
 // isotropic-form model), 1/(1 + gamma*dt) for x, y, z, xy, xz, yz, then rho
 // and its four averaged copies ('+i' rhoxx, '+j' rhozx = rhozy, '-i' rhoxy =
-// rhoxz, '-j' rhozz; cpmlfdtd.f08:889-898).
+// rhoxz, '-j' rhozz; cpmlfdtd.f08:889-898).  A model with at most 256
+// distinct (stiffness, divisor) tuples keeps those 27 entries in a material
+// table instead (one byte of id per cell); the smeared densities vary from
+// row to row and stay per-cell arrays.
 int el25_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -430,18 +498,30 @@ int el25_setup(Device &d, const double *const *in)
     std::vector<double> zero((size_t)n, 0.0);
     // isotropic form: only c11, c12, c44 (Voigt slots 0, 1, 15) are held
     d.mat_iso = d.iso_opt && stiffness_isotropic(in, n);
-    d.nmat = 0;
-    for (int id = SDT_C11; id <= SDT_C66; ++id, ++d.nmat) {
-        if (d.mat_iso && id != SDT_C11 && id != SDT_C12 && id != SDT_C44) continue;
-        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat]);
+    const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};
+    auto divisor = [&](int q, int64_t c) { return 1.0 / (1.0 + (in[gids[q]] ? in[gids[q]][c] : 0.0) * d.dt); };
+    if (!d.mat_iso && d.mat_table_opt) {
+        rc = build_material_table(d, kEl25TabRow, [&](int64_t c, double *r) {
+            for (int q = 0; q < 21; ++q) r[q] = in[SDT_C11 + q] ? in[SDT_C11 + q][c] : 0.0;
+            for (int q = 0; q < 6; ++q) r[M_IG + q] = divisor(q, c);
+        });
         if (rc) return rc;
     }
-    const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Y, SDT_GAMMA_Z, SDT_GAMMA_XY, SDT_GAMMA_XZ, SDT_GAMMA_YZ};
+    d.nmat = 0;
     std::vector<double> tmp((size_t)n);
-    for (int id : gids) {
-        for (int64_t c = 0; c < n; ++c) tmp[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
-        rc = upload_grid(d, tmp.data(), &d.mat[d.nmat++]);
-        if (rc) return rc;
+    if (d.mat_table) {
+        d.nmat = M_RHO;   // stiffness and divisors live in the table
+    } else {
+        for (int id = SDT_C11; id <= SDT_C66; ++id, ++d.nmat) {
+            if (d.mat_iso && id != SDT_C11 && id != SDT_C12 && id != SDT_C44) continue;
+            rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat]);
+            if (rc) return rc;
+        }
+        for (int q = 0; q < 6; ++q) {
+            for (int64_t c = 0; c < n; ++c) tmp[(size_t)c] = divisor(q, c);
+            rc = upload_grid(d, tmp.data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
     }
     const double *rho = in[SDT_RHO] ? in[SDT_RHO] : zero.data();
     rc = upload_grid(d, rho, &d.mat[d.nmat++]);

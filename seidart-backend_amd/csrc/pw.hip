@@ -485,11 +485,11 @@ static void launch_plane_wave_t(Device &d)
         const int slot[6] = {0, 2, 4, 11, 13, 18};   // c11 c13 c15 c33 c35 c55
         for (int q = 0; q < 6; ++q) A.C[slot[q]] = (const T *)d.mat[q];
         A.dim = 2;
-    } else if (d3 && d.mat_table) {
+    } else if (d.mat_table) {   // 3D or 2.5D elastic: stiffness from the material table
         A.mid = d.mid;
         A.tab = (const T *)d.tab;
-        A.tab_row = kEl3TabRow;
-        A.dim = 3;
+        A.tab_row = d3 ? kEl3TabRow : kEl25TabRow;
+        A.dim = d3 ? 3 : 25;
     } else if (d.mat_iso) {   // isotropic form: slots of c11, c12, c44 repeat, the rest are 0
         static const int slot[21] = {0, 1, 1, -1, -1, -1, 0, 1, -1, -1, -1, 0, -1, -1, -1, 15, -1, -1, 15, -1, 15};
         for (int q = 0; q < 21; ++q) A.C[q] = slot[q] < 0 ? nullptr : (const T *)d.mat[slot[q]];

@@ -247,13 +247,38 @@ def test_fp32_close_to_fp64(physics):
 EL25_FIELDS = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
 
 
-def test_elastic25_fp64():
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+def test_elastic25_fp64(options):
     """2.5D elastic: layered rotated-ice model with a lateral anomaly,
-    attenuation, harmonic density smear, ky = 0.05, 400 steps."""
+    attenuation, harmonic density smear, ky = 0.05, 400 steps; stiffness
+    from the material table (default) and from the 27 per-cell arrays."""
     from seidart_hip import models
     cfg = models.elastic25()
-    g, o = _run_both(cfg, EL25_FIELDS)
+    g, o = _run_both(cfg, EL25_FIELDS, options=options)
     _check(g, o, TOL64)
+
+
+def test_elastic25_material_table_equals_per_cell_arrays():
+    """The table holds the same FP64 stiffness and divisors as the per-cell
+    arrays; the runs agree to rounding (FMA contraction may differ when the
+    row sits in scalar registers).  The anomaly's edge puts wavefronts with
+    two materials on the per-lane row path."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    cfg = models.elastic25(nx=70, nz=60, cpml=8, steps=200, src=(35, 20), nrec=8, ky=0.1)
+    out = []
+    for opt in (1, 0):
+        s = make_solver(cfg, options={"material_table": opt})
+        try:
+            s.run(cfg["time_steps"])
+            out.append((s.receivers(), s.norms()[0], {f: s.field(f) for f in EL25_FIELDS}, s.materials()))
+        finally:
+            s.close()
+    (r1, n1, f1, m1), (r0, n0, f0, m0) = out
+    assert m1 > 1 and m0 == 0
+    assert rel_l2(r1, r0) < 1e-13 and rel_l2(n1, n0) < 1e-13
+    for k in EL25_FIELDS:
+        assert rel_l2(f1[k], f0[k]) < 1e-13, k
 
 
 @pytest.mark.parametrize("density,ky", [("arithmetic", 0.0), ("geometric", 0.3), ("none", 0.12)])
