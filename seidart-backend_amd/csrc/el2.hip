@@ -44,8 +44,36 @@ struct El2Args {
     int64_t stride;
     int store_first;
     PwFaces pwf;        // plane-wave cells, skipped by the fused norm
+    // material table (a model with at most 256 distinct (stiffness,
+    // attenuation) tuples and no adjoint): id per padded cell, rows of
+    // kEl2TabRow with c11 c13 c15 c33 c35 c55 in their Voigt slots (0 2 4 11
+    // 13 18) and the divisors x, z, xz at 21..23; null: per-cell arrays.
+    // The density stays a per-cell array.
+    const uint8_t *mid;
+    const T *tab;
 };
 constexpr int kEl2Fwd = 13;
+
+template <typename T>
+using el2_const_row = __attribute__((address_space(4))) const T *;
+
+// Stiffness and divisors of a cell from its table row: scalar loads when the
+// wavefront's cells share one material, per-lane cached loads otherwise.
+template <typename T>
+__device__ __forceinline__ void el2_row(const T *tab, int m, T &k11, T &k13, T &k15, T &k33, T &k35, T &k55, T &gx,
+                                        T &gz, T &gxz)
+{
+    const int m0 = __builtin_amdgcn_readfirstlane(m);
+    if (__all(m == m0)) {
+        const el2_const_row<T> p = (el2_const_row<T>)tab + m0 * kEl2TabRow;
+        k11 = p[0]; k13 = p[2]; k15 = p[4]; k33 = p[11]; k35 = p[13]; k55 = p[18];
+        gx = p[21]; gz = p[22]; gxz = p[23];
+    } else {
+        const T *p = tab + m * kEl2TabRow;
+        k11 = p[0]; k13 = p[2]; k15 = p[4]; k33 = p[11]; k35 = p[13]; k55 = p[18];
+        gx = p[21]; gz = p[22]; gxz = p[23];
+    }
+}
 
 // memory recursion psi = b*psi + a*d ; d = d/kappa + psi (cpmlfdtd.f08:332-340)
 template <typename T>
@@ -72,6 +100,10 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
         for (int q = 0; q < 9; ++q) st[q * A.stride] = T(0);
     }
     if (!r1 && !r2) return;
+    // uniform over the launch's shot; recording runs (the adjoint's) never
+    // hold a table
+    const bool tab = !STORE && A.mid != nullptr;
+    const int mat = tab ? (int)A.mid[c] : 0;
     const T *vx = A.vx, *vz = A.vz;
     const int64_t sx = 1, sz = A.pitch;
     // stencil operands: x0 = vx(i,j), xl/xr = vx(i-+1,j), xd/xu = vx(i,j-+1), ...
@@ -88,11 +120,13 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
         zd2 = vz[c - 2 * sz];
         s_xx = A.sxx[c];
         s_zz = A.szz[c];
-        k11 = ldnt(A.c11 + c);
-        k13 = ldnt(A.c13 + c);
-        k33 = ldnt(A.c33 + c);
-        gx = ldnt(A.igx + c);
-        gz = ldnt(A.igz + c);
+        if (!tab) {
+            k11 = ldnt(A.c11 + c);
+            k13 = ldnt(A.c13 + c);
+            k33 = ldnt(A.c33 + c);
+            gx = ldnt(A.igx + c);
+            gz = ldnt(A.igz + c);
+        }
     }
     if (r2) {
         xl2 = vx[c - 2 * sx];
@@ -100,11 +134,15 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
         zl2 = vz[c - 2 * sx];
         zu2 = vz[c + 2 * sz];
         s_xz = A.sxz[c];
-        k55 = ldnt(A.c55 + c);
-        gxz = ldnt(A.igxz + c);
+        if (!tab) {
+            k55 = ldnt(A.c55 + c);
+            gxz = ldnt(A.igxz + c);
+        }
     }
-    k15 = ldnt(A.c15 + c);
-    k35 = ldnt(A.c35 + c);
+    if (!tab) {
+        k15 = ldnt(A.c15 + c);
+        k35 = ldnt(A.c35 + c);
+    }
     if (PML) {
         ah = ldnt(A.ah + c);
         bh = ldnt(A.bh + c);
@@ -118,6 +156,7 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
         if (r2)
             for (int q = 4; q < 8; ++q) m[q] = ldnt(A.m[q] + c);
     }
+    if (tab) el2_row(A.tab, mat, k11, k13, k15, k33, k35, k55, gx, gz, gxz);
     if (r1) {
         T dvx_dx = (T(27) * (xr - x0) - (xr2 - xl)) * A.rdx24;
         T dvz_dz = (T(27) * (z0 - zd) - (zu - zd2)) * A.rdz24;
@@ -343,6 +382,8 @@ static El2Args<T> el2_args(Device &d)
     A.stride = d.ncell_dev;
     A.store_first = d.adj.store_first;
     A.pwf = d.pw.faces();
+    A.mid = d.mat_table ? d.mid : nullptr;
+    A.tab = d.mat_table ? (const T *)d.tab : nullptr;
     return A;
 }
 
@@ -1061,7 +1102,10 @@ void el2_step(Device &d, bool timed)
 }
 
 // Material layout on the device: c11 c13 c15 c33 c35 c55 rho and the three
-// attenuation divisors precomputed as 1/(1 + gamma*dt) in FP64.
+// attenuation divisors precomputed as 1/(1 + gamma*dt) in FP64.  Without an
+// adjoint and with at most 256 distinct (stiffness, divisor) tuples, those
+// nine come from a material table (one byte of id per cell) and only the
+// density is a per-cell array (slot 6).
 int el2_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -1080,15 +1124,26 @@ int el2_setup(Device &d, const double *const *in)
             if (rc) return rc;
         }
     } else {
-        for (int id : ids) {
-            rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
+        const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Z, SDT_GAMMA_XZ};
+        auto divisor = [&](int q, int64_t c) { return 1.0 / (1.0 + (in[gids[q]] ? in[gids[q]][c] : 0.0) * d.dt); };
+        if (d.mat_table_opt && d.adj.every == 0) {
+            static const int voigt[6] = {0, 2, 4, 11, 13, 18};
+            rc = build_material_table(d, kEl2TabRow, [&](int64_t c, double *r) {
+                for (int q = 0; q < 6; ++q) r[voigt[q]] = in[ids[q]] ? in[ids[q]][c] : 0.0;
+                for (int q = 0; q < 3; ++q) r[21 + q] = divisor(q, c);
+            });
             if (rc) return rc;
         }
-        const int gids[] = {SDT_GAMMA_X, SDT_GAMMA_Z, SDT_GAMMA_XZ};
+        for (int q = 0; q < 7; ++q, ++d.nmat) {
+            if (d.mat_table && q != 6) continue;   // only the density beside the table
+            rc = upload_grid(d, in[ids[q]] ? in[ids[q]] : zero.data(), &d.mat[d.nmat]);
+            if (rc) return rc;
+        }
         std::vector<double> inv((size_t)n);
-        for (int id : gids) {
-            for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = 1.0 / (1.0 + (in[id] ? in[id][c] : 0.0) * d.dt);
-            rc = upload_grid(d, inv.data(), &d.mat[d.nmat++]);
+        for (int q = 0; q < 3; ++q, ++d.nmat) {
+            if (d.mat_table) continue;
+            for (int64_t c = 0; c < n; ++c) inv[(size_t)c] = divisor(q, c);
+            rc = upload_grid(d, inv.data(), &d.mat[d.nmat]);
             if (rc) return rc;
         }
     }

@@ -482,8 +482,14 @@ static void launch_plane_wave_t(Device &d)
     } else if (d.physics == SDT_ELASTIC2) {
         A.v[1] = nullptr;
         A.s[1] = A.s[3] = A.s[5] = nullptr;
-        const int slot[6] = {0, 2, 4, 11, 13, 18};   // c11 c13 c15 c33 c35 c55
-        for (int q = 0; q < 6; ++q) A.C[slot[q]] = (const T *)d.mat[q];
+        if (d.mat_table) {   // rows hold c11 c13 c15 c33 c35 c55 in their Voigt slots
+            A.mid = d.mid;
+            A.tab = (const T *)d.tab;
+            A.tab_row = kEl2TabRow;
+        } else {
+            const int slot[6] = {0, 2, 4, 11, 13, 18};   // c11 c13 c15 c33 c35 c55
+            for (int q = 0; q < 6; ++q) A.C[slot[q]] = (const T *)d.mat[q];
+        }
         A.dim = 2;
     } else if (d.mat_table) {   // 3D or 2.5D elastic: stiffness from the material table
         A.mid = d.mid;

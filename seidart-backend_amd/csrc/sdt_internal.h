@@ -319,6 +319,8 @@ bool stiffness_isotropic(const double *const *host_in, int64_t n);
 void launch_plane_wave(Device &d);
 constexpr int kEl3TabRow = 32;         // 3D elastic material-table row length
 constexpr int kEl25TabRow = 32;        // 2.5D elastic material-table row length
+constexpr int kEl2TabRow = 32;         // 2D elastic material-table row length
+constexpr int kEm2TabRow = 8;          // 2D EM material-table row length
 void launch_adj_tail(Device &d, int t, void *const lam_field[3]);
 void set_step_counter(hipStream_t s, int *d_it, int v);
 int download_grid_f64(Device &d, const double *dev, double *host);

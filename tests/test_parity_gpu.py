@@ -37,12 +37,38 @@ def _check(g, o, tol, stress_norm=False):
     return errs
 
 
-def test_elastic2_config0_fp64():
-    """2D elastic, 200x200 (+10 CPML), 1000 steps, two tilted ice layers."""
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+def test_elastic2_config0_fp64(options):
+    """2D elastic, 200x200 (+10 CPML), 1000 steps, two tilted ice layers;
+    stiffness from the material table (default) and from per-cell arrays."""
     from seidart_hip import models
     cfg = models.elastic2()
-    g, o = _run_both(cfg, ("vx", "vz", "sxx", "szz", "sxz"))
+    g, o = _run_both(cfg, ("vx", "vz", "sxx", "szz", "sxz"), options=options)
     _check(g, o, TOL64)
+
+
+def test_elastic2_material_table_equals_per_cell_arrays():
+    """Table and per-cell arrays hold the same values: the runs agree to
+    rounding; the table is used without an adjoint and dropped with one."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    cfg = models.elastic2(nx=90, nz=70, cpml=8, steps=300, src=(45, 20), rec_z=20, nrec=8, gamma=2.0,
+                          density_method="geometric")
+    out = []
+    for opts in ({"material_table": 1}, {"material_table": 0}, {"material_table": 1, "checkpoint": 50}):
+        s = make_solver(cfg, options=opts)
+        try:
+            s.run(cfg["time_steps"])
+            out.append((s.receivers(), s.norms()[0], {f: s.field(f) for f in ("vx", "vz", "sxx", "sxz")},
+                        s.materials()))
+        finally:
+            s.close()
+    (r1, n1, f1, m1), (r0, n0, f0, m0), (r2, n2, f2, m2) = out
+    assert m1 > 1 and m0 == 0 and m2 == 0
+    assert rel_l2(r1, r0) < 1e-13 and rel_l2(n1, n0) < 1e-13
+    assert np.array_equal(r2, r0) and np.array_equal(n2, n0)
+    for k in f0:
+        assert rel_l2(f1[k], f0[k]) < 1e-13, k
 
 
 @pytest.mark.parametrize("density", ["harmonic", "geometric", "none"])
