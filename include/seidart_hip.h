@@ -220,13 +220,15 @@ enum {
                                    other entries 0), keep only c11, c12, c44 and read 3 arrays
                                    per cell instead of 21 (default; same arithmetic, results
                                    equal to the general path), 0 = always all 21 arrays      */
-    SDT_OPT_MATERIAL_TABLE = 5  /* 3D elastic, 2.5D elastic and 3D EM: 1 = when the model holds
-                                   at most 256 distinct material tuples ((c11..c66, gamma_*,
-                                   rho); 2.5D: (c11..c66, gamma_*), the smeared densities stay
-                                   per cell; or (inverse permittivity, conductivity)), keep one
-                                   byte of material id per cell and a table of the tuples
-                                   instead of the per-cell arrays (default; same values, results
-                                   equal to rounding), 0 = always the per-cell arrays       */
+    SDT_OPT_MATERIAL_TABLE = 5  /* every solver but 2.5D EM: 1 = when the model holds at most
+                                   256 distinct material tuples ((c11..c66, gamma_*, rho);
+                                   2.5D and 2D elastic: (c11..c66, gamma_*), the density stays
+                                   per cell; 2D EM: the seven update coefficients; 3D EM:
+                                   (inverse permittivity, conductivity)), keep one byte of
+                                   material id per cell and a table of the tuples instead of
+                                   the per-cell arrays (default; same values, results equal to
+                                   rounding; the 2D solvers keep per-cell arrays when an
+                                   adjoint is planned), 0 = always the per-cell arrays     */
 };
 int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value);
 

@@ -41,7 +41,33 @@ struct Em2Args {
     T *store;
     int64_t stride;
     int store_first;
+    // material table (a model with at most 256 distinct coefficient tuples
+    // and no adjoint): id per padded cell, rows of kEm2TabRow holding aEx bEx
+    // bEz dEx eEx dEz eEz; null: per-cell arrays
+    const uint8_t *mid;
+    const T *tab;
 };
+
+template <typename T>
+using em2_const_row = __attribute__((address_space(4))) const T *;
+
+// The seven update coefficients of a cell from its table row: scalar loads
+// when the wavefront's cells share one material, per-lane cached loads
+// otherwise.
+template <typename T>
+__device__ __forceinline__ void em2_row(const T *tab, int m, T (&co)[7])
+{
+    const int m0 = __builtin_amdgcn_readfirstlane(m);
+    if (__all(m == m0)) {
+        const em2_const_row<T> p = (em2_const_row<T>)tab + m0 * kEm2TabRow;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) co[q] = p[q];
+    } else {
+        const T *p = tab + m * kEm2TabRow;
+#pragma unroll
+        for (int q = 0; q < 7; ++q) co[q] = p[q];
+    }
+}
 
 // Hy (cpmlfdtd.f08:2576-2593); every operand is read before the first store
 template <typename T, bool PML>
@@ -93,10 +119,17 @@ __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
         ex = ez = T(0);
         if (STORE) st[0] = st[A.stride] = st[2 * A.stride] = st[3 * A.stride] = T(0);
     } else {
+        // uniform over the launch's shot; recording runs (the adjoint's)
+        // never hold a table
+        const bool tab = !STORE && A.mid != nullptr;
+        const int mat = tab ? (int)A.mid[c] : 0;
         const T h0 = A.Hy[c], hd = A.Hy[c - A.pitch], hl = A.Hy[c - 1];
         const T exo = A.Ex[c], ezo = A.Ez[c];
-        const T bEx = ldnt(A.bEx + c), bEz = ldnt(A.bEz + c), dEz = ldnt(A.dEz + c), eEz = ldnt(A.eEz + c);
-        const T aEx = ldnt(A.aEx + c), dEx = ldnt(A.dEx + c), eEx = ldnt(A.eEx + c);
+        T co[7];   // aEx bEx bEz dEx eEx dEz eEz
+        if (!tab) {
+            co[1] = ldnt(A.bEx + c); co[2] = ldnt(A.bEz + c); co[5] = ldnt(A.dEz + c); co[6] = ldnt(A.eEz + c);
+            co[0] = ldnt(A.aEx + c); co[3] = ldnt(A.dEx + c); co[4] = ldnt(A.eEx + c);
+        }
         T mx = 0, mz = 0, a = 0, b = 0, ik = 0, ah = 0, bh = 0, ikh = 0;
         if (PML) {
             mx = ldnt(A.m_dHy_dx + c);
@@ -108,6 +141,8 @@ __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
             bh = ldnt(A.bh + c);
             ikh = ldnt(A.ikh + c);
         }
+        if (tab) em2_row(A.tab, mat, co);
+        const T aEx = co[0], bEx = co[1], bEz = co[2], dEx = co[3], eEx = co[4], dEz = co[5], eEz = co[6];
         T dHy_dz = (h0 - hd) * A.rdz;
         T dHy_dx = (h0 - hl) * A.rdx;
         if (PML) {
@@ -173,6 +208,8 @@ static Em2Args<T> em2_args(Device &d)
     A.store = (T *)d.adj.seg_fwd;
     A.stride = d.ncell_dev;
     A.store_first = d.adj.store_first;
+    A.mid = d.mat_table ? d.mid : nullptr;
+    A.tab = d.mat_table ? (const T *)d.tab : nullptr;
     return A;
 }
 
@@ -672,7 +709,9 @@ void em2_step(Device &d, bool timed)
 
 // Coefficients of cpmlfdtd.f08:2519-2540 (permittivity scaled by eps0,
 // determinant clamped at 1e-18*eps0^2), computed in FP64 on the host and
-// stored as aEx, bEx (= aEz), bEz, dEx, eEx, dEz, eEz.
+// stored as aEx, bEx (= aEz), bEz, dEx, eEx, dEz, eEz: per-cell arrays, or,
+// without an adjoint and with at most 256 distinct tuples, table rows with
+// one byte of id per cell.
 int em2_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -700,9 +739,16 @@ int em2_setup(Device &d, const double *const *in)
         co[5][(size_t)c] = (e11 * s13 - e13 * s11) / det;
         co[6][(size_t)c] = (e11 * s33 - e13 * s13) / det;
     }
+    if (!d.coef_in && d.mat_table_opt && d.adj.every == 0) {
+        rc = build_material_table(d, kEm2TabRow, [&](int64_t c, double *r) {
+            for (int q = 0; q < 7; ++q) r[q] = co[q][(size_t)c];
+        });
+        if (rc) return rc;
+    }
     d.nmat = 0;
-    for (int q = 0; q < 7; ++q) {
-        rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
+    for (int q = 0; q < 7; ++q, ++d.nmat) {
+        if (d.mat_table) continue;   // the rows hold them
+        rc = upload_grid(d, co[q].data(), &d.mat[d.nmat]);
         if (rc) return rc;
     }
     d.nmem = 4;

@@ -29,11 +29,12 @@ def _outputs(s):
     return s.receivers(), s.norms()[0], {f: s.field(f) for f in names}
 
 
-def test_batch_mixes_material_layouts():
-    """2D elastic shots with the material table and with per-cell arrays in
-    one batch: each equals its own run alone."""
+@pytest.mark.parametrize("physics", ["elastic2", "em2"])
+def test_batch_mixes_material_layouts(physics):
+    """Shots with the material table and with per-cell arrays in one batch:
+    each equals its own run alone."""
     from seidart_hip.solver import Batch, make_solver
-    cfgs = _shots("elastic2")
+    cfgs = _shots(physics)
     opts = [{"material_table": q % 2} for q in range(len(cfgs))]
     solo = []
     for c, o in zip(cfgs, opts):

@@ -216,11 +216,35 @@ def test_unstable_run_reports_step():
     assert e.value.step == 1
 
 
-def test_em2_fp64():
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+def test_em2_fp64(options):
     from seidart_hip import models
     cfg = models.em2(nx=120, nz=80, steps=400, src=(60, 20), nrec=16)
-    g, o = _run_both(cfg, ("ex", "ez", "hy"))
+    g, o = _run_both(cfg, ("ex", "ez", "hy"), options=options)
     _check(g, o, TOL64)
+
+
+def test_em2_material_table_equals_per_cell_arrays():
+    """Table and per-cell coefficient arrays hold the same values: the runs
+    agree to rounding; with an adjoint planned no table is built."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    cfg = models.em2(nx=110, nz=84, steps=300, src=(55, 20), nrec=8)
+    out = []
+    for opts in ({"material_table": 1}, {"material_table": 0}, {"material_table": 1, "checkpoint": 60}):
+        s = make_solver(cfg, options=opts)
+        try:
+            s.run(cfg["time_steps"])
+            out.append((s.receivers(), s.norms()[0], {f: s.field(f) for f in ("ex", "ez", "hy")},
+                        s.materials()))
+        finally:
+            s.close()
+    (r1, n1, f1, m1), (r0, n0, f0, m0), (r2, n2, f2, m2) = out
+    assert m1 > 1 and m0 == 0 and m2 == 0
+    assert rel_l2(r1, r0) < 1e-13 and rel_l2(n1, n0) < 1e-13
+    assert np.array_equal(r2, r0) and np.array_equal(n2, n0)
+    for k in f0:
+        assert rel_l2(f1[k], f0[k]) < 1e-13, k
 
 
 @pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"zchunk": 5}, {"zchunk": 1, "material_table": 0}])
