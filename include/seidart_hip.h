@@ -220,11 +220,11 @@ enum {
                                    other entries 0), keep only c11, c12, c44 and read 3 arrays
                                    per cell instead of 21 (default; same arithmetic, results
                                    equal to the general path), 0 = always all 21 arrays      */
-    SDT_OPT_MATERIAL_TABLE = 5  /* every solver but 2.5D EM: 1 = when the model holds at most
-                                   256 distinct material tuples ((c11..c66, gamma_*, rho);
-                                   2.5D and 2D elastic: (c11..c66, gamma_*), the density stays
-                                   per cell; 2D EM: the seven update coefficients; 3D EM:
-                                   (inverse permittivity, conductivity)), keep one byte of
+    SDT_OPT_MATERIAL_TABLE = 5  /* every solver: 1 = when the model holds at most 256
+                                   distinct material tuples (3D elastic: (c11..c66, gamma_*,
+                                   rho); 2.5D and 2D elastic: (c11..c66, gamma_*), the density
+                                   stays per cell; 2D EM: the seven update coefficients; 3D
+                                   and 2.5D EM: (inverse permittivity, conductivity)), keep one byte of
                                    material id per cell and a table of the tuples instead of
                                    the per-cell arrays (default; same values, results equal to
                                    rounding; the 2D solvers keep per-cell arrays when an

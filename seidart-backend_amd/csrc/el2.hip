@@ -522,10 +522,19 @@ void el2_batch_launch(Batch &b, int which)
     else el2_batch_launch_p<float>(b, which);
 }
 
+template <typename T, int R>
+static void el2_step_r(Device &d, bool timed);
+
 template <typename T>
 static void el2_step_t(Device &d, bool timed)
 {
-    constexpr int R = kSingleRows2d;
+    if (small_grid_2d(d)) el2_step_r<T, 1>(d, timed);
+    else el2_step_r<T, kSingleRows2d>(d, timed);
+}
+
+template <typename T, int R>
+static void el2_step_r(Device &d, bool timed)
+{
     const El2Args<T> A = el2_args<T>(d);
     const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);

@@ -73,7 +73,39 @@ struct Em25Args {
     const int *d_it;
     unsigned long long *vnorm;
     int64_t src_cell;
+    // material table (at most 256 distinct (inverse permittivity,
+    // conductivity) tuples): id per padded cell, rows of kEm25TabRow holding
+    // aEx bEx cEx bEy cEy cEz s11 s12 s13 s22 s23 s33; null: per-cell arrays
+    const uint8_t *mid;
+    const T *tab;
 };
+
+template <typename T>
+using em25_const_row = __attribute__((address_space(4))) const T *;
+
+// The twelve E-update coefficients of a cell: from its table row (scalar
+// loads when the wavefront's cells share one material, per-lane cached loads
+// otherwise) or from the per-cell arrays.
+template <typename T>
+__device__ __forceinline__ void em25_coefs(const Em25Args<T> &A, int64_t c, int m, T (&co)[12])
+{
+    if (A.mid) {
+        const int m0 = __builtin_amdgcn_readfirstlane(m);
+        if (__all(m == m0)) {
+            const em25_const_row<T> p = (em25_const_row<T>)A.tab + m0 * kEm25TabRow;
+#pragma unroll
+            for (int q = 0; q < 12; ++q) co[q] = p[q];
+        } else {
+            const T *p = A.tab + m * kEm25TabRow;
+#pragma unroll
+            for (int q = 0; q < 12; ++q) co[q] = p[q];
+        }
+        return;
+    }
+    const T *const arr[12] = {A.aEx, A.bEx, A.cEx, A.bEy, A.cEy, A.cEz, A.s11, A.s12, A.s13, A.s22, A.s23, A.s33};
+#pragma unroll
+    for (int q = 0; q < 12; ++q) co[q] = arr[q][c];
+}
 
 // Hx (cpmlfdtd.f08:3106-3116), Hy (3131-3146), Hz (3160-3170)
 template <typename T, bool PML>
@@ -137,6 +169,7 @@ __device__ __forceinline__ double em25_e_cell(const Em25Args<T> &A, int i, int k
         ex = ey = ez = czk(T(0), T(0));
         A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
     } else if (k >= 3 && k <= nz - 2 && i >= 3 && i <= nx - 2) {
+        const int mat = A.mid ? (int)A.mid[c] : 0;   // first: the row loads wait on it
         const Cz<T> *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
         const T rx = A.rdx24, rz = A.rdz24;
         const Cz<T> dHz_dy = ikyz(A.ky, Hz[c]);
@@ -152,8 +185,10 @@ __device__ __forceinline__ double em25_e_cell(const Em25Args<T> &A, int i, int k
             dHz_dx = fixz(dHz_dx, A.m[Q_HZ_DX], c, a, b, ik);
             dHy_dx = fixz(dHy_dx, A.m[Q_HY_DX], c, a, b, ik);
         }
-        const T s11 = A.s11[c], s12 = A.s12[c], s13 = A.s13[c], s22 = A.s22[c], s23 = A.s23[c], s33 = A.s33[c];
-        const T aEx = A.aEx[c], bEx = A.bEx[c], cEx = A.cEx[c], bEy = A.bEy[c], cEy = A.cEy[c], cEz = A.cEz[c];
+        T co[12];
+        em25_coefs(A, c, mat, co);
+        const T aEx = co[0], bEx = co[1], cEx = co[2], bEy = co[3], cEy = co[4], cEz = co[5];
+        const T s11 = co[6], s12 = co[7], s13 = co[8], s22 = co[9], s23 = co[10], s33 = co[11];
         const T dt = A.dt;
         const T rxr = dHz_dy.re - dHy_dz.re - (s11 * ex.re + s12 * ey.re + s13 * ez.re);
         const T rxi = dHz_dy.im - dHy_dz.im - (s11 * ex.im + s12 * ey.im + s13 * ez.im);
@@ -220,6 +255,8 @@ static Em25Args<T> em25_args(Device &d)
     A.d_it = d.d_it;
     A.vnorm = d.norms;
     A.src_cell = d.tail.vfix[0];
+    A.mid = d.mat_table ? d.mid : nullptr;
+    A.tab = d.mat_table ? (const T *)d.tab : nullptr;
     return A;
 }
 
@@ -244,7 +281,8 @@ void em25_step(Device &d, bool timed)
 
 // Inverse permittivity by adjugate/determinant of the absolute permittivity
 // (cpmlfdtd.f08:3040-3066), FP64 on the host, six unique entries; the six
-// conductivities as given.
+// conductivities as given.  Per-cell arrays, or table rows with one byte of
+// id per cell when the model has at most 256 distinct tuples.
 int em25_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -270,16 +308,27 @@ int em25_setup(Device &d, const double *const *in)
         co[4][(size_t)c] = -(e11 * e23 - e12 * e13) / det;
         co[5][(size_t)c] = (e11 * e22 - e12 * e12) / det;
     }
-    d.nmat = 0;
-    for (int q = 0; q < 6; ++q) {
-        rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
+    // few distinct materials (layered / blocky models): one-byte ids and a
+    // table of (inverse permittivity, conductivity) rows
+    if (d.mat_table_opt) {
+        rc = build_material_table(d, kEm25TabRow, [&](int64_t c, double *r) {
+            for (int q = 0; q < 6; ++q) r[q] = co[q][(size_t)c];
+            for (int q = 0; q < 6; ++q) r[6 + q] = val(SDT_S11 + q, c);
+        });
         if (rc) return rc;
     }
-    std::vector<double> zero;
-    for (int id = SDT_S11; id <= SDT_S33; ++id) {
-        if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
-        rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
-        if (rc) return rc;
+    d.nmat = 0;
+    if (!d.mat_table) {
+        for (int q = 0; q < 6; ++q) {
+            rc = upload_grid(d, co[q].data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
+        std::vector<double> zero;
+        for (int id = SDT_S11; id <= SDT_S33; ++id) {
+            if (!in[id] && zero.empty()) zero.assign((size_t)n, 0.0);
+            rc = upload_grid(d, in[id] ? in[id] : zero.data(), &d.mat[d.nmat++]);
+            if (rc) return rc;
+        }
     }
     d.nmem = Q_TOTAL;
     for (int q = 0; q < d.nmem; ++q) {

@@ -271,6 +271,14 @@ constexpr int kNumBatchShapes = 4;
 constexpr BatchShape kBatchShapes[kNumBatchShapes] = {{4, false}, {8, false}, {4, true}, {8, true}};
 inline int tiles_x_2d(const Device &d) { return (int)((d.xoff + d.nx + 63) / 64); }
 inline int tiles_z_2d(const Device &d, int rows) { return (d.nz + kTile2dWaves * rows - 1) / (kTile2dWaves * rows); }
+// A 2D elastic single-shot grid too small to give every CU a few
+// workgroups at kSingleRows2d rows per wavefront (the 220 x 220 case gives
+// 56) runs one row per wavefront: four times the workgroups, a quarter of
+// the serial row latency per wavefront; same per-cell arithmetic either way.
+// (Measured on 64 shots of 300-800 x 150-400: 2D elastic 7.4 -> 8.8
+// GCell/s, 2D EM 10.8 -> 10.1, so the 2D EM updates keep kSingleRows2d.)
+constexpr int kSmallGrid2dTiles = 1024;
+inline bool small_grid_2d(const Device &d) { return tiles_x_2d(d) * tiles_z_2d(d, kSingleRows2d) < kSmallGrid2dTiles; }
 struct Batch {
     int physics = 0, precision = 0, nshots = 0;
     void *args = nullptr;          // per-shot kernel arguments (device)
@@ -321,6 +329,7 @@ constexpr int kEl3TabRow = 32;         // 3D elastic material-table row length
 constexpr int kEl25TabRow = 32;        // 2.5D elastic material-table row length
 constexpr int kEl2TabRow = 32;         // 2D elastic material-table row length
 constexpr int kEm2TabRow = 8;          // 2D EM material-table row length
+constexpr int kEm25TabRow = 16;        // 2.5D EM material-table row length
 void launch_adj_tail(Device &d, int t, void *const lam_field[3]);
 void set_step_counter(hipStream_t s, int *d_it, int v);
 int download_grid_f64(Device &d, const double *dev, double *host);

@@ -219,8 +219,7 @@ class Solver:
         """key: 'zchunk' (planes per z-marching workgroup, 0 auto), 'graph'
         (0/1), 'stagger' (0/1), 'tile_rows' (0 auto, 4, 8, 16) or
         'material_table' (1 = one-byte material ids + table when the model
-        has <= 256 distinct materials, every solver but 'em25'; 0 = per-cell
-        arrays) or
+        has <= 256 distinct materials; 0 = per-cell arrays) or
         'concurrent' (1 = 3D elastic shell kernels on a second stream beside
         the interior kernels, 0 = sequential) or 'xcd_remap' (3D elastic block
         order, bit 0 interior, bit 1 shell: XCD-contiguous tile ranges) or

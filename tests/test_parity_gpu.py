@@ -359,14 +359,41 @@ def test_elastic25_complex_ic_corrected_stencil():
 EM25_FIELDS = ("ex", "ey", "ez", "hx", "hy", "hz")
 
 
-@pytest.mark.parametrize("ky", [0.0, 5.0, 30.0])
-def test_em25_fp64(ky):
+@pytest.mark.parametrize("ky,options", [(0.0, {}), (5.0, {}), (30.0, {}), (5.0, {"material_table": 0})])
+def test_em25_fp64(ky, options):
     """2.5D EM: layered SPD permittivity / conductivity, 4th-order
-    differences, i*ky y derivatives, three-component dipole."""
+    differences, i*ky y derivatives, three-component dipole; coefficients
+    from the material table (default) and from per-cell arrays."""
     from seidart_hip import models
     cfg = models.em25(ky=ky)
-    g, o = _run_both(cfg, EM25_FIELDS)
+    g, o = _run_both(cfg, EM25_FIELDS, options=options)
     _check(g, o, TOL64)
+
+
+def test_em25_material_table_equals_per_cell_arrays():
+    """Table and per-cell arrays hold the same values: the runs agree to
+    rounding; a per-cell jitter of one conductivity overflows the table and
+    falls back to the arrays."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    cfg = models.em25(nx=70, nz=60, cpml=8, steps=200, src=(35, 20), nrec=8, ky=12.0)
+    jit = dict(cfg)
+    jit["arrays"] = dict(cfg["arrays"])
+    rng = np.random.default_rng(5)
+    jit["arrays"]["s11"] = cfg["arrays"]["s11"] * (1.0 + 0.01 * rng.random(cfg["shape"]))
+    out = []
+    for c, opts in ((cfg, {"material_table": 1}), (cfg, {"material_table": 0}), (jit, {})):
+        s = make_solver(c, options=opts)
+        try:
+            s.run(c["time_steps"])
+            out.append((s.receivers(), s.norms()[0], {f: s.field(f) for f in EM25_FIELDS}, s.materials()))
+        finally:
+            s.close()
+    (r1, n1, f1, m1), (r0, n0, f0, m0), (_, _, _, mj) = out
+    assert m1 > 1 and m0 == 0 and mj == 0
+    assert rel_l2(r1, r0) < 1e-13 and rel_l2(n1, n0) < 1e-13
+    for k in EM25_FIELDS:
+        assert rel_l2(f1[k], f0[k]) < 1e-13, k
 
 
 def test_em25_complex_ic():
