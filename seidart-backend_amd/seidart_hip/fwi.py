@@ -173,7 +173,10 @@ class _BatchTraces(torch.autograd.Function):
                 s.close()
             ctx.batch = ctx.solvers = None
         shape = solvers[0].shape
-        gc = sum(np.stack([row.reshape(shape, order="F") for row in x]) for x in cgs)
+        acc = np.zeros_like(cgs[0])
+        for x in cgs:   # shot order, as a running sum
+            acc += x
+        gc = np.stack([row.reshape(shape, order="F") for row in acc])
         gsrc = np.stack([np.stack([sg[L.SOURCE_IDS[c]] for c in _SRC[physics]]) for sg in sgs])
         return torch.from_numpy(gc), torch.from_numpy(gsrc), None, None, None, None
 
