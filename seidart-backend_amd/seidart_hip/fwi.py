@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import _lib as L
-from .solver import Batch, Solver
+from .solver import Batch, Solver, flatten_coefficients
 
 EPS0 = 8.85418782e-12                        # constants.f08:10
 EM2_COEFFS = ("aEx", "bEx", "bEz", "dEx", "eEx", "dEz", "eEz")
@@ -133,7 +133,7 @@ class _BatchTraces(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, coef, src, physics, shots, precision, checkpoint):
-        c = coef.detach().cpu().numpy()
+        c = flatten_coefficients(coef.detach().cpu().numpy())   # once for every shot
         sv = src.detach().cpu().numpy()
         solvers = []
         b = None

@@ -21,6 +21,13 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def flatten_coefficients(coef):
+    """(ncoef,) + grid per-cell coefficients -> the C-ABI's layout: one
+    Fortran-order grid after the other, contiguous FP64."""
+    c = np.asarray(coef, dtype=np.float64)
+    return np.ascontiguousarray(np.stack([np.asfortranarray(x).ravel(order="F") for x in c]).ravel())
+
+
 class Solver:
     """One simulation on one GPU.
 
@@ -127,13 +134,20 @@ class Solver:
         (ncoef,) + grid (EM: aEx, bEx, bEz, dEx, eEx, dEz, eEz; elastic: c11,
         c13, c15, c33, c35, c55, rho, 1/(1+gamma_x dt), 1/(1+gamma_z dt),
         1/(1+gamma_xz dt)), in place of the model arrays; needs
-        set_option('raw_source', 1)."""
+        set_option('raw_source', 1).  A 1-D array is taken as the output of
+        flatten_coefficients (shots sharing one model flatten it once)."""
         c = np.asarray(coef, dtype=np.float64)
-        if c.shape[1:] != self.shape:
-            raise ValueError(f"coefficient shape {c.shape} does not match grid {self.shape}")
-        flat = np.ascontiguousarray(np.stack([np.asfortranarray(x).ravel(order="F") for x in c]))
+        if c.ndim == 1:
+            ncell = int(np.prod(self.shape))
+            if c.size == 0 or c.size % ncell:
+                raise ValueError(f"flat coefficient length {c.size} is not a multiple of {ncell} cells")
+            flat, ncoef = np.ascontiguousarray(c), c.size // ncell
+        else:
+            if c.shape[1:] != self.shape:
+                raise ValueError(f"coefficient shape {c.shape} does not match grid {self.shape}")
+            flat, ncoef = flatten_coefficients(c), c.shape[0]
         L.check(L.lib.sdt_set_coefficients(self._h, _dptr(flat), flat.size), self._h)
-        self.ncoef = c.shape[0]
+        self.ncoef = ncoef
 
     def adjoint(self, trace_grad):
         """After run() over all time_steps with the 'checkpoint' option:
