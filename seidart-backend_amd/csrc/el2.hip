@@ -87,7 +87,7 @@ __device__ __forceinline__ T cpml_fix(T d, T &psi, T a, T b, T ik)
 // operand is read before the first store: the compiler cannot tell the
 // arrays apart, so a store ahead of a load would serialise the memory
 // round trips.
-template <typename T, bool PML, bool STORE>
+template <typename T, bool PML, bool STORE, bool TAB>
 __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
@@ -100,9 +100,9 @@ __device__ __forceinline__ void el2_stress_cell(const El2Args<T> &A, int i, int 
         for (int q = 0; q < 9; ++q) st[q * A.stride] = T(0);
     }
     if (!r1 && !r2) return;
-    // uniform over the launch's shot; recording runs (the adjoint's) never
-    // hold a table
-    const bool tab = !STORE && A.mid != nullptr;
+    // TAB: the launch may meet a table (compiled out otherwise); uniform over
+    // the launch's shot; recording runs (the adjoint's) never hold a table
+    const bool tab = TAB && !STORE && A.mid != nullptr;
     const int mat = tab ? (int)A.mid[c] : 0;
     const T *vx = A.vx, *vz = A.vz;
     const int64_t sx = 1, sz = A.pitch;
@@ -402,7 +402,7 @@ __device__ __forceinline__ bool el2_pml(const El2Args<T> &A, int i, int j)
     return !in_box(A.box, i, 1, j) || A.flags[((i - 1) >> 6) + (size_t)A.tiles_x * ((j - 1) >> 2)];
 }
 
-template <typename T, int R, bool STORE>
+template <typename T, int R, bool STORE, bool TAB>
 __device__ __forceinline__ void el2_stress_tile(const El2Args<T> &A, int tx, int band)
 {
     const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
@@ -412,9 +412,9 @@ __device__ __forceinline__ void el2_stress_tile(const El2Args<T> &A, int tx, int
         const int j = j0 + r;
         if (j > A.nz) break;
         if (__any(valid && el2_pml(A, i, j))) {
-            if (valid) el2_stress_cell<T, true, STORE>(A, i, j);
+            if (valid) el2_stress_cell<T, true, STORE, TAB>(A, i, j);
         } else if (valid) {
-            el2_stress_cell<T, false, STORE>(A, i, j);
+            el2_stress_cell<T, false, STORE, TAB>(A, i, j);
         }
     }
 }
@@ -440,10 +440,10 @@ __device__ __forceinline__ double el2_velocity_tile(const El2Args<T> &A, int tx,
     return nrm;
 }
 
-template <typename T, int R, bool STORE>
+template <typename T, int R, bool STORE, bool TAB>
 __global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_kernel(El2Args<T> A)
 {
-    el2_stress_tile<T, R, STORE>(A, blockIdx.x, blockIdx.y);
+    el2_stress_tile<T, R, STORE, TAB>(A, blockIdx.x, blockIdx.y);
 }
 
 template <typename T, int R, bool STORE>
@@ -453,7 +453,7 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) el2_velocity_kernel(El2Args
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
-template <typename T, int R, bool STORE>
+template <typename T, int R, bool STORE, bool TAB>
 __global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_batch_kernel(const El2Args<T> *args,
                                                                              const int4 *tiles)
 {
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) el2_stress_batch_kernel(con
     if (t.x < 0) return;   // padding tile
     const El2Args<T> A = args[t.x];
     if (*A.d_it > A.nsteps) return;   // finished shot (uniform per workgroup)
-    el2_stress_tile<T, R, STORE>(A, t.y, t.z);
+    el2_stress_tile<T, R, STORE, TAB>(A, t.y, t.z);
 }
 
 template <typename T, int R, bool STORE>
@@ -480,7 +480,11 @@ template <typename T>
 static int el2_batch_args_t(Batch &b, Device *const *shots, int n)
 {
     std::vector<El2Args<T>> a((size_t)n);
-    for (int s = 0; s < n; ++s) a[(size_t)s] = el2_args<T>(*shots[s]);
+    b.any_table = false;
+    for (int s = 0; s < n; ++s) {
+        a[(size_t)s] = el2_args<T>(*shots[s]);
+        b.any_table = b.any_table || a[(size_t)s].mid != nullptr;
+    }
     if (!b.args && hipMalloc(&b.args, a.size() * sizeof(El2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(b.args, a.data(), a.size() * sizeof(El2Args<T>), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
@@ -498,10 +502,15 @@ static void el2_batch_launch_t(Batch &b, int which)
     const El2Args<T> *a = (const El2Args<T> *)b.args;
     const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
     if (b.store) {
-        hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, true>), grid, block, 0, b.stream, a, b.tiles[which]);
+        hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, true, false>), grid, block, 0, b.stream, a, b.tiles[which]);
         hipLaunchKernelGGL((el2_velocity_batch_kernel<T, R, true>), grid, block, 0, b.stream, a, b.tiles[which]);
     } else {
-        hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, false>), grid, block, 0, b.stream, a, b.tiles[which]);
+        if (b.any_table)
+            hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, false, true>), grid, block, 0, b.stream, a,
+                               b.tiles[which]);
+        else
+            hipLaunchKernelGGL((el2_stress_batch_kernel<T, R, false, false>), grid, block, 0, b.stream, a,
+                               b.tiles[which]);
         hipLaunchKernelGGL((el2_velocity_batch_kernel<T, R, false>), grid, block, 0, b.stream, a, b.tiles[which]);
     }
 }
@@ -539,9 +548,11 @@ static void el2_step_r(Device &d, bool timed)
     const dim3 grid(tiles_x_2d(d), tiles_z_2d(d, R)), block(64, kTile2dWaves);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
     if (d.adj.store)
-        hipLaunchKernelGGL((el2_stress_kernel<T, R, true>), grid, block, 0, d.stream, A);
+        hipLaunchKernelGGL((el2_stress_kernel<T, R, true, false>), grid, block, 0, d.stream, A);
+    else if (d.mat_table)
+        hipLaunchKernelGGL((el2_stress_kernel<T, R, false, true>), grid, block, 0, d.stream, A);
     else
-        hipLaunchKernelGGL((el2_stress_kernel<T, R, false>), grid, block, 0, d.stream, A);
+        hipLaunchKernelGGL((el2_stress_kernel<T, R, false, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
     if (d.adj.store)
         hipLaunchKernelGGL((el2_velocity_kernel<T, R, true>), grid, block, 0, d.stream, A);

@@ -107,7 +107,7 @@ __device__ __forceinline__ void em2_h_cell(const Em2Args<T> &A, int i, int j)
 // face cells (their values change after this kernel; the norm is the
 // square root of the maximum, which equals the maximum of the magnitudes).
 // STORE: also write the values the adjoint gradient needs (see Em2Args).
-template <typename T, bool PML, bool STORE>
+template <typename T, bool PML, bool STORE, bool TAB>
 __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
 {
     const int nx = A.nx, nz = A.nz;
@@ -119,9 +119,10 @@ __device__ __forceinline__ double em2_e_cell(const Em2Args<T> &A, int i, int j)
         ex = ez = T(0);
         if (STORE) st[0] = st[A.stride] = st[2 * A.stride] = st[3 * A.stride] = T(0);
     } else {
-        // uniform over the launch's shot; recording runs (the adjoint's)
-        // never hold a table
-        const bool tab = !STORE && A.mid != nullptr;
+        // TAB: the launch may meet a table (compiled out otherwise); uniform
+        // over the launch's shot; recording runs (the adjoint's) never hold
+        // a table
+        const bool tab = TAB && !STORE && A.mid != nullptr;
         const int mat = tab ? (int)A.mid[c] : 0;
         const T h0 = A.Hy[c], hd = A.Hy[c - A.pitch], hl = A.Hy[c - 1];
         const T exo = A.Ex[c], ezo = A.Ez[c];
@@ -241,7 +242,7 @@ __device__ __forceinline__ void em2_h_tile(const Em2Args<T> &A, int tx, int band
     }
 }
 
-template <typename T, int R, bool STORE>
+template <typename T, int R, bool STORE, bool TAB>
 __device__ __forceinline__ double em2_e_tile(const Em2Args<T> &A, int tx, int band)
 {
     const int i = tx * 64 + (int)threadIdx.x - (int)A.xoff + 1;
@@ -253,9 +254,9 @@ __device__ __forceinline__ double em2_e_tile(const Em2Args<T> &A, int tx, int ba
         if (j > A.nz) break;
         double v = 0.0;
         if (__any(valid && em2_pml(A, i, j))) {
-            if (valid) v = em2_e_cell<T, true, STORE>(A, i, j);
+            if (valid) v = em2_e_cell<T, true, STORE, TAB>(A, i, j);
         } else if (valid) {
-            v = em2_e_cell<T, false, STORE>(A, i, j);
+            v = em2_e_cell<T, false, STORE, TAB>(A, i, j);
         }
         nrm = (v > nrm || v != v) ? v : nrm;
     }
@@ -268,10 +269,10 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) em2_h_kernel(Em2Args<T> A)
     em2_h_tile<T, R>(A, blockIdx.x, blockIdx.y);
 }
 
-template <typename T, int R, bool STORE>
+template <typename T, int R, bool STORE, bool TAB>
 __global__ void __launch_bounds__(64 * kTile2dWaves) em2_e_kernel(Em2Args<T> A)
 {
-    const double nrm = sqrt(em2_e_tile<T, R, STORE>(A, blockIdx.x, blockIdx.y));
+    const double nrm = sqrt(em2_e_tile<T, R, STORE, TAB>(A, blockIdx.x, blockIdx.y));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -285,14 +286,14 @@ __global__ void __launch_bounds__(64 * kTile2dWaves) em2_h_batch_kernel(const Em
     em2_h_tile<T, R>(A, t.y, t.z);
 }
 
-template <typename T, int R, bool STORE>
+template <typename T, int R, bool STORE, bool TAB>
 __global__ void __launch_bounds__(64 * kTile2dWaves) em2_e_batch_kernel(const Em2Args<T> *args, const int4 *tiles)
 {
     const int4 t = tiles[blockIdx.x];
     if (t.x < 0) return;   // padding tile
     const Em2Args<T> A = args[t.x];
     if (*A.d_it > A.nsteps) return;
-    const double nrm = sqrt(em2_e_tile<T, R, STORE>(A, t.y, t.z));
+    const double nrm = sqrt(em2_e_tile<T, R, STORE, TAB>(A, t.y, t.z));
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
@@ -300,7 +301,11 @@ template <typename T>
 static int em2_batch_args_t(Batch &b, Device *const *shots, int n)
 {
     std::vector<Em2Args<T>> a((size_t)n);
-    for (int s = 0; s < n; ++s) a[(size_t)s] = em2_args<T>(*shots[s]);
+    b.any_table = false;
+    for (int s = 0; s < n; ++s) {
+        a[(size_t)s] = em2_args<T>(*shots[s]);
+        b.any_table = b.any_table || a[(size_t)s].mid != nullptr;
+    }
     if (!b.args && hipMalloc(&b.args, a.size() * sizeof(Em2Args<T>)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(b.args, a.data(), a.size() * sizeof(Em2Args<T>), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
@@ -319,9 +324,11 @@ static void em2_batch_launch_t(Batch &b, int which)
     const dim3 grid(b.ntiles[which]), block(64, kTile2dWaves);
     hipLaunchKernelGGL((em2_h_batch_kernel<T, R>), grid, block, 0, b.stream, a, b.tiles[which]);
     if (b.store)
-        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, true>), grid, block, 0, b.stream, a, b.tiles[which]);
+        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, true, false>), grid, block, 0, b.stream, a, b.tiles[which]);
+    else if (b.any_table)
+        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, false, true>), grid, block, 0, b.stream, a, b.tiles[which]);
     else
-        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, false>), grid, block, 0, b.stream, a, b.tiles[which]);
+        hipLaunchKernelGGL((em2_e_batch_kernel<T, R, false, false>), grid, block, 0, b.stream, a, b.tiles[which]);
 }
 
 template <typename T>
@@ -350,9 +357,11 @@ static void em2_step_t(Device &d, bool timed)
     hipLaunchKernelGGL((em2_h_kernel<T, R>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
     if (d.adj.store)
-        hipLaunchKernelGGL((em2_e_kernel<T, R, true>), grid, block, 0, d.stream, A);
+        hipLaunchKernelGGL((em2_e_kernel<T, R, true, false>), grid, block, 0, d.stream, A);
+    else if (d.mat_table)
+        hipLaunchKernelGGL((em2_e_kernel<T, R, false, true>), grid, block, 0, d.stream, A);
     else
-        hipLaunchKernelGGL((em2_e_kernel<T, R, false>), grid, block, 0, d.stream, A);
+        hipLaunchKernelGGL((em2_e_kernel<T, R, false, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_plane_wave(d);
     launch_tail(d);

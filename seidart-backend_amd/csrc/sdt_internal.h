@@ -288,6 +288,7 @@ struct Batch {
     TailArgs *tails = nullptr;     // per-shot tail arguments (device)
     int choice = -1;               // autotuned tile shape index, -1 = not yet
     bool store = false;            // forward E / velocity kernels write the segment store
+    bool any_table = false;        // some shot reads a material table (else the kernels without it)
     void *adj_args = nullptr;      // per-shot adjoint kernel arguments (device)
     AdjTailArgs *adj_tails = nullptr;
     hipStream_t stream = nullptr;

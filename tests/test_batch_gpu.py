@@ -32,7 +32,10 @@ def _outputs(s):
 @pytest.mark.parametrize("physics", ["elastic2", "em2"])
 def test_batch_mixes_material_layouts(physics):
     """Shots with the material table and with per-cell arrays in one batch:
-    each equals its own run alone."""
+    each equals its own run alone to FP32 rounding (a batch holding a table
+    shot runs the table-capable kernel for every shot, whose per-cell-array
+    branch the compiler may contract differently from the table-free kernel
+    a lone per-cell-array shot runs; a wrong layout would differ by O(1))."""
     from seidart_hip.solver import Batch, make_solver
     cfgs = _shots(physics)
     opts = [{"material_table": q % 2} for q in range(len(cfgs))]
@@ -52,10 +55,9 @@ def test_batch_mixes_material_layouts(physics):
         for q, s in enumerate(shots):
             rec, vn, f = _outputs(s)
             r0, v0, f0, _ = solo[q]
-            # same per-cell arithmetic as the single-shot kernels
-            assert rel_l2(rec, r0) <= 1e-13 and rel_l2(vn, v0) <= 1e-13, q
+            assert rel_l2(rec, r0) <= 1e-5 and rel_l2(vn, v0) <= 1e-5, q
             for k in f:
-                assert rel_l2(f[k], f0[k]) <= 1e-13, (q, k)
+                assert rel_l2(f[k], f0[k]) <= 1e-5, (q, k)
     finally:
         for s in shots:
             s.close()
