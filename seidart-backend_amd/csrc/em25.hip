@@ -83,13 +83,13 @@ struct Em25Args {
 template <typename T>
 using em25_const_row = __attribute__((address_space(4))) const T *;
 
-// The twelve E-update coefficients of a cell: from its table row (scalar
-// loads when the wavefront's cells share one material, per-lane cached loads
-// otherwise) or from the per-cell arrays.
-template <typename T>
+// The twelve E-update coefficients of a cell: from its table row (TAB;
+// scalar loads when the wavefront's cells share one material, per-lane cached
+// loads otherwise) or from the per-cell arrays.
+template <typename T, bool TAB>
 __device__ __forceinline__ void em25_coefs(const Em25Args<T> &A, int64_t c, int m, T (&co)[12])
 {
-    if (A.mid) {
+    if (TAB) {
         const int m0 = __builtin_amdgcn_readfirstlane(m);
         if (__all(m == m0)) {
             const em25_const_row<T> p = (em25_const_row<T>)A.tab + m0 * kEm25TabRow;
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(256) em25_h_kernel(Em25Args<T> A)
 
 // E (cpmlfdtd.f08:3190-3228), Dirichlet faces (3246-3258) and the norm
 // (3309) without the source cell
-template <typename T, bool PML>
+template <typename T, bool PML, bool TAB>
 __device__ __forceinline__ double em25_e_cell(const Em25Args<T> &A, int i, int k)
 {
     const int nx = A.nx, nz = A.nz;
@@ -169,7 +169,7 @@ __device__ __forceinline__ double em25_e_cell(const Em25Args<T> &A, int i, int k
         ex = ey = ez = czk(T(0), T(0));
         A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
     } else if (k >= 3 && k <= nz - 2 && i >= 3 && i <= nx - 2) {
-        const int mat = A.mid ? (int)A.mid[c] : 0;   // first: the row loads wait on it
+        const int mat = TAB ? (int)A.mid[c] : 0;   // first: the row loads wait on it
         const Cz<T> *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
         const T rx = A.rdx24, rz = A.rdz24;
         const Cz<T> dHz_dy = ikyz(A.ky, Hz[c]);
@@ -186,7 +186,7 @@ __device__ __forceinline__ double em25_e_cell(const Em25Args<T> &A, int i, int k
             dHy_dx = fixz(dHy_dx, A.m[Q_HY_DX], c, a, b, ik);
         }
         T co[12];
-        em25_coefs(A, c, mat, co);
+        em25_coefs<T, TAB>(A, c, mat, co);
         const T aEx = co[0], bEx = co[1], cEx = co[2], bEy = co[3], cEy = co[4], cEz = co[5];
         const T s11 = co[6], s12 = co[7], s13 = co[8], s22 = co[9], s23 = co[10], s33 = co[11];
         const T dt = A.dt;
@@ -207,7 +207,7 @@ __device__ __forceinline__ double em25_e_cell(const Em25Args<T> &A, int i, int k
     return sqrt(s);
 }
 
-template <typename T>
+template <typename T, bool TAB>
 __global__ void __launch_bounds__(256) em25_e_kernel(Em25Args<T> A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x + 1;
@@ -215,9 +215,9 @@ __global__ void __launch_bounds__(256) em25_e_kernel(Em25Args<T> A)
     double nrm = 0.0;
     if (i <= A.nx && k <= A.nz) {
         if (!in_box(A.box, i, 1, k) || A.flags[blockIdx.x + (size_t)A.tiles_x * blockIdx.y])
-            nrm = em25_e_cell<T, true>(A, i, k);
+            nrm = em25_e_cell<T, true, TAB>(A, i, k);
         else
-            nrm = em25_e_cell<T, false>(A, i, k);
+            nrm = em25_e_cell<T, false, TAB>(A, i, k);
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
@@ -268,7 +268,8 @@ static void em25_step_t(Device &d, bool timed)
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
     hipLaunchKernelGGL(em25_h_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    hipLaunchKernelGGL(em25_e_kernel<T>, grid, block, 0, d.stream, A);
+    if (d.mat_table) hipLaunchKernelGGL((em25_e_kernel<T, true>), grid, block, 0, d.stream, A);
+    else hipLaunchKernelGGL((em25_e_kernel<T, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_tail(d);
 }
