@@ -403,8 +403,11 @@ __device__ __forceinline__ void em2_adj_e_cell(const Em2AdjArgs<T> &B, int i, in
     const Em2Args<T> &A = B.F;
     const int nx = A.nx, nz = A.nz;
     const int64_t c = A.xoff + (int64_t)(i - 1) + (int64_t)A.pitch * (j - 1);
-    const T gzd = j >= 2 ? B.lgz[c - A.pitch] : T(0);
-    const T gxl = i >= 2 ? B.lgx[c - 1] : T(0);
+    // unconditional loads (the own element at the low faces), so they issue
+    // with the cell's other loads
+    const T gzd_v = B.lgz[j >= 2 ? c - A.pitch : c], gxl_v = B.lgx[i >= 2 ? c - 1 : c];
+    const T gzd = j >= 2 ? gzd_v : T(0);
+    const T gxl = i >= 2 ? gxl_v : T(0);
     const T lx = B.lex[c] + (gzd - B.lgz[c]) * A.rdz;
     const T lz = B.lez[c] + (gxl - B.lgx[c]) * A.rdx;
     if (c == A.src_cell) {
