@@ -247,44 +247,43 @@ __device__ __forceinline__ double el2_velocity_cell(const El2Args<T> &A, int i, 
     T r = 0, rxl = 0, rxd = 0, rzr = 0, rzu = 0;
     T f[8] = {}, g[8] = {};
     T a = 0, b = 0, ik = 0, ah = 0, bh = 0, ikh = 0, m[4] = {};
-    if (rx || rz) r = ldnt(A.rho + c);
-    if (rx) {
-        rxl = A.rho[c - sx];
-        rxd = A.rho[c - sz];
-        f[0] = A.sxx[c];
-        f[1] = A.sxx[c - sx];
-        f[2] = A.sxx[c - 2 * sx];
-        f[3] = corr ? A.sxx[c + sx] : T(0);
-        f[4] = A.sxz[c];
-        f[5] = A.sxz[c - sz];
-        f[6] = A.sxz[c - 2 * sz];
-        f[7] = corr ? A.sxz[c + sz] : T(0);
-        if (PML) {
-            a = ldnt(A.a + c);
-            b = ldnt(A.b + c);
-            ik = ldnt(A.ik + c);
-            m[0] = ldnt(A.m[8] + c);
-            m[1] = ldnt(A.m[9] + c);
-        }
-    }
-    if (rz) {
-        rzr = A.rho[c + sx];
-        rzu = A.rho[c + sz];
-        g[0] = A.sxz[c];
-        g[1] = A.sxz[c + sx];
-        g[2] = A.sxz[c + 2 * sx];
-        g[3] = corr ? A.sxz[c - sx] : T(0);
-        g[4] = A.szz[c];
-        g[5] = A.szz[c + sz];
-        g[6] = A.szz[c + 2 * sz];
-        g[7] = corr ? A.szz[c - sz] : T(0);
-        if (PML) {
-            ah = ldnt(A.ah + c);
-            bh = ldnt(A.bh + c);
-            ikh = ldnt(A.ikh + c);
-            m[2] = ldnt(A.m[10] + c);
-            m[3] = ldnt(A.m[11] + c);
-        }
+    // Every load is issued, whatever the cell's region: outside a region its
+    // neighbour loads read the cell's own element instead (always in the
+    // allocation) and the values go unused, so the compiler can batch the
+    // loads instead of guarding them branch by branch.
+    auto nx_ = [&](bool ok, int64_t off) { return ok ? c + off : c; };
+    r = ldnt(A.rho + c);
+    rxl = A.rho[nx_(rx, -sx)];
+    rxd = A.rho[nx_(rx, -sz)];
+    f[0] = A.sxx[c];
+    f[1] = A.sxx[nx_(rx, -sx)];
+    f[2] = A.sxx[nx_(rx, -2 * sx)];
+    f[3] = corr ? A.sxx[nx_(rx, sx)] : T(0);
+    f[4] = A.sxz[c];
+    f[5] = A.sxz[nx_(rx, -sz)];
+    f[6] = A.sxz[nx_(rx, -2 * sz)];
+    f[7] = corr ? A.sxz[nx_(rx, sz)] : T(0);
+    rzr = A.rho[nx_(rz, sx)];
+    rzu = A.rho[nx_(rz, sz)];
+    g[0] = f[4];
+    g[1] = A.sxz[nx_(rz, sx)];
+    g[2] = A.sxz[nx_(rz, 2 * sx)];
+    g[3] = corr ? A.sxz[nx_(rz, -sx)] : T(0);
+    g[4] = A.szz[c];
+    g[5] = A.szz[nx_(rz, sz)];
+    g[6] = A.szz[nx_(rz, 2 * sz)];
+    g[7] = corr ? A.szz[nx_(rz, -sz)] : T(0);
+    if (PML) {
+        a = ldnt(A.a + c);
+        b = ldnt(A.b + c);
+        ik = ldnt(A.ik + c);
+        ah = ldnt(A.ah + c);
+        bh = ldnt(A.bh + c);
+        ikh = ldnt(A.ikh + c);
+        m[0] = ldnt(A.m[8] + c);
+        m[1] = ldnt(A.m[9] + c);
+        m[2] = ldnt(A.m[10] + c);
+        m[3] = ldnt(A.m[11] + c);
     }
     if (edge) {
         vx = T(0);
