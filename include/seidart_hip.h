@@ -157,7 +157,8 @@ int32_t sdt_device_count(int32_t *count);
 /* last error message of a handle, or of the calling thread when h is NULL */
 const char *sdt_last_error(const sdt_handle *h);
 
-/* lifetime */
+/* lifetime; a handle that belongs to a live batch is not destroyed
+ * (SDT_ERR_STATE) until sdt_batch_destroy releases it */
 int32_t sdt_create(const sdt_params *p, sdt_handle **out);
 int32_t sdt_destroy(sdt_handle *h);
 
@@ -173,7 +174,9 @@ int32_t sdt_set_source(sdt_handle *h, int32_t component, const double *series, i
 int32_t sdt_set_receivers(sdt_handle *h, int32_t n, const int32_t *ijk);
 
 /* advance nsteps time steps (continues from the current step).  On
- * SDT_ERR_UNSTABLE, *failed_step (if non-NULL) receives the 1-based step. */
+ * SDT_ERR_UNSTABLE, *failed_step (if non-NULL) receives the 1-based step.
+ * A handle that belongs to a live batch is advanced only by sdt_batch_run
+ * (SDT_ERR_STATE here). */
 int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step);
 /* steps completed so far */
 int32_t sdt_steps_done(const sdt_handle *h, int32_t *steps);
@@ -273,7 +276,10 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
  * from captured hipGraphs.  The tile height is chosen by timing each
  * candidate on the first steps of the run.  Handles stay owned by the caller
  * and must outlive the batch; their fields, receivers and norms are read with
- * the usual calls afterwards. */
+ * the usual calls afterwards.  A handle may appear once in one batch
+ * (SDT_ERR_ARG for a repeat, SDT_ERR_STATE for a handle of another live
+ * batch); while the batch lives, sdt_run, sdt_adjoint and sdt_destroy on its
+ * handles return SDT_ERR_STATE. */
 typedef struct sdt_batch sdt_batch;
 int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out);
 /* advance every shot by min(nsteps, its remaining steps); on a blow-up

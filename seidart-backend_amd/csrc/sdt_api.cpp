@@ -15,6 +15,8 @@
 
 using namespace sdt;
 
+struct sdt_batch;
+
 struct sdt_handle {
     Device d;
     sdt_params p{};
@@ -30,6 +32,7 @@ struct sdt_handle {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     std::vector<hipEvent_t> tev;       // timing events, 3 per timed step
+    const sdt_batch *owner = nullptr;  // the batch this handle belongs to (until sdt_batch_destroy)
 };
 
 static thread_local std::string g_err;
@@ -187,6 +190,7 @@ static void release(sdt_handle *h)
 int32_t sdt_destroy(sdt_handle *h)
 {
     if (!h) return SDT_OK;
+    if (h->owner) return fail(h, SDT_ERR_STATE, "the handle belongs to a batch: destroy the batch first");
     release(h);
     delete h;
     return SDT_OK;
@@ -399,6 +403,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
 {
     if (!h) return fail(nullptr, SDT_ERR_ARG, "null handle");
     if (failed_step) *failed_step = 0;
+    if (h->owner) return fail(h, SDT_ERR_STATE, "the handle belongs to a batch: advance it with sdt_batch_run");
     if (h->failed) return fail(h, SDT_ERR_STATE, "the run has already blown up");
     if (nsteps < 0) return fail(h, SDT_ERR_ARG, "nsteps must be >= 0");
     if (h->steps_done + nsteps > h->d.time_steps)
@@ -751,6 +756,7 @@ int32_t sdt_adjoint(sdt_handle *h, const double *trace_grad, int64_t tcount, dou
                     double *src_grad, int64_t scount)
 {
     if (!h || !coef_grad || (!trace_grad && tcount)) return fail(h, SDT_ERR_ARG, "null argument");
+    if (h->owner) return fail(h, SDT_ERR_STATE, "the handle belongs to a batch: use sdt_batch_adjoint");
     Device &d = h->d;
     Adjoint &a = d.adj;
     if (d.physics != SDT_EM2 && d.physics != SDT_ELASTIC2)
@@ -807,6 +813,16 @@ int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
 {
     if (!handles || !out || n < 1) return fail(nullptr, SDT_ERR_ARG, "bad batch arguments");
     *out = nullptr;
+    // a handle listed twice would have its tiles updated twice per launch
+    // (racing on its fields) and its sources injected twice per step; the
+    // check compares pointers only, before any handle is dereferenced
+    for (int q = 0; q < n; ++q) {
+        if (!handles[q]) return fail(nullptr, SDT_ERR_ARG, "null handle in batch");
+        for (int r = 0; r < q; ++r)
+            if (handles[r] == handles[q])
+                return fail(nullptr, SDT_ERR_ARG,
+                            "handle " + std::to_string(q) + " repeats handle " + std::to_string(r) + " in the batch");
+    }
     const int phys = handles[0] ? handles[0]->p.physics : -1;
     if (phys != SDT_ELASTIC2 && phys != SDT_EM2)
         return fail(nullptr, SDT_ERR_ARG, "batches hold 2D elastic or 2D EM handles");
@@ -817,6 +833,7 @@ int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
             h->p.device != handles[0]->p.device)
             return fail(nullptr, SDT_ERR_ARG, "batch handles must share solver kind, precision and device");
         if (h->failed) return fail(nullptr, SDT_ERR_STATE, "a batch handle has already blown up");
+        if (h->owner) return fail(nullptr, SDT_ERR_STATE, "a handle already belongs to another batch");
         if (h->pw.kind) return fail(nullptr, SDT_ERR_ARG, "batched shots take point sources, not a plane wave");
         int rc = prepare(h);
         if (rc) return rc;
@@ -836,6 +853,7 @@ int32_t sdt_batch_create(sdt_handle *const *handles, int32_t n, sdt_batch **out)
         delete b;
         return fail(nullptr, rc, "batch setup failed");
     }
+    for (sdt_handle *h : b->shots) h->owner = b;
     *out = b;
     return SDT_OK;
 }
@@ -983,6 +1001,7 @@ int32_t sdt_batch_destroy(sdt_batch *b)
     for (auto ev : b->ev)
         if (ev) (void)hipEventDestroy(ev);
     batch_free(b->b);
+    for (sdt_handle *h : b->shots) h->owner = nullptr;
     delete b;
     return SDT_OK;
 }

@@ -4,8 +4,10 @@ write_image2/3 366-426, write_array 186-200).
 
 A record is a 4-byte little-endian length marker, the payload, and the same
 marker again.  Payloads larger than 2**31 - 9 bytes are split into
-subrecords whose markers are negative on every subrecord but the last (the
-gfortran convention).
+subrecords (the gfortran convention): a subrecord's head marker is negative
+when more subrecords follow it, its tail marker is negative when it
+continues an earlier subrecord, so the head of the first and the tail of the
+last are the only positive markers of a split record.
 """
 from __future__ import annotations
 
@@ -60,10 +62,9 @@ def write_record(path, array, dtype=None):
         while True:
             size = min(n - pos, _MAX_SUB)
             last = pos + size >= n
-            marker = size if last else -size
-            f.write(struct.pack("<i", marker))
+            f.write(struct.pack("<i", size if last else -size))
             f.write(payload[pos:pos + size])
-            f.write(struct.pack("<i", marker))
+            f.write(struct.pack("<i", -size if pos > 0 else size))
             pos += size
             if last:
                 break

@@ -84,8 +84,10 @@ class Solver:
 
     # ---- lifetime ---------------------------------------------------------
     def close(self):
+        """Free the handle's device memory.  Raises SdtError (and keeps the
+        handle) while a live Batch holds it."""
         if self._h:
-            L.lib.sdt_destroy(self._h)
+            L.check(L.lib.sdt_destroy(self._h), self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

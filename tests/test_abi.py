@@ -95,3 +95,18 @@ def test_oracle_library_builds_and_loads():
     from oracle import oracle_ctypes as O
     lib = O.lib()
     assert lib.oracle_scalar_mean(2.0, 2.0, 3) == 2.0
+
+
+def test_batch_rejects_repeated_handle_before_touching_it():
+    """A handle listed twice in one batch would be stepped twice per launch;
+    sdt_batch_create refuses it by comparing the pointers alone, so the
+    stand-in addresses below are never dereferenced."""
+    from seidart_hip import _lib as L
+    b = C.c_void_p()
+    hs = (C.c_void_p * 3)(0x1000, 0x2000, 0x1000)
+    assert L.lib.sdt_batch_create(hs, 3, C.byref(b)) == L.ERR_ARG
+    assert "repeats handle 0" in L.lib.sdt_last_error(None).decode()
+    assert not b
+    hs = (C.c_void_p * 2)(0x1000, None)
+    assert L.lib.sdt_batch_create(hs, 2, C.byref(b)) == L.ERR_ARG
+    assert "null handle" in L.lib.sdt_last_error(None).decode()

@@ -152,3 +152,33 @@ def test_batch_rejects_mixed_kinds():
     finally:
         a.close()
         c.close()
+
+
+def test_batch_owns_its_handles_until_destroyed():
+    """While a batch lives its handles cannot be run, adjointed or destroyed
+    on their own, nor join a second batch; destroying the batch releases
+    them."""
+    import ctypes as C
+    from seidart_hip import SdtError, models
+    from seidart_hip import _lib as L
+    from seidart_hip.solver import Batch, make_solver
+    a = make_solver(models.elastic2(nx=40, nz=40, steps=20, src=(20, 20), nrec=2))
+    c = make_solver(models.elastic2(nx=44, nz=40, steps=20, src=(20, 20), nrec=2))
+    b = Batch([a, c])
+    try:
+        assert L.lib.sdt_run(a._h, 1, None) == L.ERR_STATE
+        assert L.lib.sdt_destroy(c._h) == L.ERR_STATE
+        assert "belongs to a batch" in L.lib.sdt_last_error(c._h).decode()
+        with pytest.raises(SdtError):
+            a.close()
+        assert a._h   # still alive
+        with pytest.raises(SdtError):
+            Batch([c])
+        b.run(5)
+    finally:
+        b.close()
+    a.run(3)   # released: runs on its own again
+    assert a.steps_done == 8
+    a.close()
+    c.close()
+    assert not a._h and not c._h

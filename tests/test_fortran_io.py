@@ -34,7 +34,17 @@ def test_subrecords(tmp_path, monkeypatch):
     p = str(tmp_path / "big.dat")
     fio.write_record(p, a)
     raw = open(p, "rb").read()
-    assert struct.unpack("<i", raw[:4])[0] == -16          # continued
+    # 56 bytes in subrecords of 16, 16, 16, 8: (head, tail) markers are
+    # (-16, 16), (-16, -16), (-16, -16), (8, -8) -- head negative when more
+    # follow, tail negative when the subrecord continues an earlier one
+    markers, pos = [], 0
+    while pos < len(raw):
+        head = struct.unpack("<i", raw[pos:pos + 4])[0]
+        size = abs(head)
+        tail = struct.unpack("<i", raw[pos + 4 + size:pos + 8 + size])[0]
+        markers.append((head, tail))
+        pos += size + 8
+    assert markers == [(-16, 16), (-16, -16), (-16, -16), (8, -8)]
     np.testing.assert_array_equal(fio.read_record(p, np.float64, 7), a)
 
 
