@@ -14,6 +14,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# the same source built with the reference's production flags (-O3,
+# fast-math, FP contraction): timed as the CPU baseline, never the checker
+FAST_LIB_PATH = os.path.join(_HERE, "liboracle_fast.so")
 
 NUM_ARRAYS, NUM_SRC, NUM_FIELDS = 51, 9, 9
 
@@ -117,10 +120,10 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def _load():
-    if not os.path.exists(LIB_PATH):
+def _load(path):
+    if not os.path.exists(path):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     for name in ("oracle_seismic2", "oracle_seismic25", "oracle_seismic3", "oracle_electromag2",
                  "oracle_electromag25", "oracle_electromag3"):
         fn = getattr(lib, name)
@@ -133,14 +136,16 @@ def _load():
     return lib
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        _lib = _load()
-    return _lib
+def lib(fast=False):
+    """The checker library, or with fast=True the production-flag build
+    used only for CPU-baseline timing."""
+    path = FAST_LIB_PATH if fast else LIB_PATH
+    if path not in _libs:
+        _libs[path] = _load(path)
+    return _libs[path]
 
 
 def scalar_mean(a, b, code):
@@ -162,12 +167,13 @@ _ENTRY = {"elastic2": "oracle_seismic2", "elastic25": "oracle_seismic25", "em25"
           "em3": "oracle_electromag3"}
 
 
-def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
+def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0, fast=False):
     """Run a configuration dict through the oracle.
 
     Returns (receivers[steps, nrec, 3], (velnorm, stressnorm), {field: array},
     unstable_step).  For "elastic25" the receivers, fields and ic_* arrays are
-    complex128 and cfg["ky"] is the y wavenumber (likewise "em25")."""
+    complex128 and cfg["ky"] is the y wavenumber (likewise "em25").
+    fast=True runs the production-flag build (CPU-baseline timing only)."""
     phys = cfg["physics"]
     is3d = phys.endswith("3")
     shape = tuple(cfg["shape"])
@@ -256,7 +262,7 @@ def run_oracle(cfg, *, steps=None, fields=(), field_in=None, nthreads=0):
     sn = np.zeros(nsteps)
     r.velnorm = vn.ctypes.data_as(_D)
     r.stressnorm = sn.ctypes.data_as(_D)
-    getattr(lib(), _ENTRY[phys])(C.byref(r))
+    getattr(lib(fast), _ENTRY[phys])(C.byref(r))
     fields_out = {k: v.reshape(shape, order="F") for k, v in outs.items()}
     if cplx:
         rec_out = rec_out.view(np.complex128)
