@@ -700,6 +700,47 @@ def bench_fwi2d(args):
     }
 
 
+# ------------------------------------------------------------------ snapshots (drop-in time loop)
+def bench_snapshots(args):
+    """The drop-in's time loop (seidart_hip.cpmlfdtd, what the seidartfdtd CLI
+    runs) on the 256^3 + 12-cell CPML model, FP64, writing FP32 snapshots of
+    vx, vy, vz after every step (write_image3 files, 3 x 87.8 MB per step) to
+    a tmpfs directory: device-packed FP32 snapshots, pinned double-buffered
+    copies, writer threads overlapping the file writes with the next steps.
+    The model comes from memory rather than from input files (reading the
+    inputs is setup, outside the loop either way)."""
+    import shutil
+    import tempfile
+    import types
+    import numpy as np
+    from seidart_hip import models
+    from seidart_hip.cpmlfdtd import _time_loop
+    from seidart_hip.solver import make_solver
+    K = args.snapshot_steps
+    n = args.n
+    cfg = models.elastic3(nx=n, ny=n, nz=n, steps=K, src=(n // 2, n // 2, 20), nrec=min(256, n))
+    base = "/dev/shm" if os.path.isdir("/dev/shm") else tempfile.gettempdir()
+    work = tempfile.mkdtemp(prefix="sdt_snap_", dir=base)
+    log(f"[snapshots] {cfg['shape']} {K} steps -> {work}")
+    src = types.SimpleNamespace(time_steps=K, xind=n // 2, yind=n // 2, zind=20)
+    s = make_solver(cfg, precision="fp64")
+    try:
+        s.run(0)
+        s.synchronize()
+        st = _time_loop(s, work, src, (("vx", "Vx"), ("vy", "Vy"), ("vz", "Vz")), True, True, 1)
+        files = len(os.listdir(work))
+    finally:
+        s.close()
+        shutil.rmtree(work, ignore_errors=True)
+    ncell = int(np.prod(cfg["shape"]))
+    return {"metric": "GCell-updates/s (3D elastic drop-in time loop with FP32 snapshots every step)",
+            "value": round(ncell * K / st["loop_s"] / 1e9, 4), "unit": "GCell-updates/s", "steps": K,
+            "ms_per_step": round(st["loop_s"] / K * 1e3, 3), "dtype": "f64",
+            "config": {"workload": f"seismic3 {n}^3 + 12-cell CPML, vx vy vz FP32 snapshots every step to tmpfs "
+                                   f"({base})", "files": files},
+            "snapshot_GB_per_s": round(st["bytes"] / st["loop_s"] / 1e9, 2), "bytes_written": st["bytes"]}
+
+
 # ------------------------------------------------------------------ other solvers
 def bench_seismic3_iso(args):
     """The isotropic fast path: a smoothly varying isotropic 256^3 + 12-cell
@@ -882,6 +923,7 @@ def subrecords(args, rec64):
         ("electromag3_fp32", lambda: bench_em3(_sub(args, steps=K2, warmup=W2, precision="fp32"))),
         ("batch2d_64shots_fp32", lambda: bench_batch2d(args)),
         ("fwi2d_600x300_16shots", lambda: bench_fwi2d(_sub(args, fd_cells=args.sub_fd_cells))),
+        ("seismic3_snapshots_every_step", lambda: bench_snapshots(args)),
     ]
     want = set(args.subrecords.split(",")) if args.subrecords not in ("all", "none") else None
     out = {}
@@ -923,14 +965,16 @@ def main():
     ap.add_argument("--subrecords", default="all",
                     help="seismic3 workload: the other configurations timed into the line's 'configs' "
                          "('all', 'none', or a comma list of name prefixes: seismic3_fp32, seismic2, "
-                         "electromag3, batch2d, fwi2d)")
+                         "electromag3, batch2d, fwi2d, seismic3_snapshots)")
     ap.add_argument("--sub-fd-cells", type=int, default=8,
                     help="finite-difference cells of the FWI sub-record (--workload fwi2d: --fd-cells)")
+    ap.add_argument("--snapshot-steps", type=int, default=20,
+                    help="snapshots workload: steps of the drop-in time loop (3 snapshot files each)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain 2 GiB copy kernel (HBM counter calibration) and exit")
     ap.add_argument("--workload", default="seismic3",
                     choices=["seismic3", "seismic3iso", "seismic25", "seismic2", "electromag3", "electromag25",
-                             "batch2d", "fwi2d"],
+                             "batch2d", "fwi2d", "snapshots"],
                     help="seismic3: the 3D elastic 256^3 case (default); electromag3: the 3D EM "
                          "384x384x192 case; seismic25 / electromag25: the 2.5D solvers on 2000x1000; "
                          "seismic2: the 2D elastic 200x200 case beside the CPU oracle; seismic3iso: an "
@@ -967,7 +1011,7 @@ def main():
 
     single = {"electromag3": bench_em3, "batch2d": bench_batch2d, "fwi2d": bench_fwi2d,
               "seismic3iso": bench_seismic3_iso, "seismic25": bench_seismic25, "electromag25": bench_em25,
-              "seismic2": bench_seismic2}
+              "seismic2": bench_seismic2, "snapshots": bench_snapshots}
     if args.workload in single:
         print(json.dumps(single[args.workload](args)), flush=True)
         return

@@ -191,6 +191,22 @@ int32_t sdt_get_receivers(sdt_handle *h, double *out, int64_t count);
 /* per-step max norms: velocity (or E) and, for 3D elastic, stress; NULL skips */
 int32_t sdt_get_norms(sdt_handle *h, double *vel, double *stress, int64_t count);
 
+/* snapshot of one field, as write_image2/3 (seidartio.f08:366-426) write it
+ * after every step: Fortran order, unpadded, FP32 when single = 1 (the
+ * reference's SINGLE_OUTPUT default, main.f08:94-116) or FP64 when single =
+ * 0, count = nx*ny*nz values into out (float or double).  The field is
+ * packed and converted on the device into a staging buffer the handle
+ * allocates on first use, then copied to out asynchronously on the handle's
+ * stream: out must stay valid, and is complete, after the next
+ * sdt_synchronize (or any blocking call on the handle).  Pinned host memory
+ * (sdt_host_alloc) lets the copy overlap host work.  Real-valued solvers only
+ * (the 2.5D solvers write zstd blocks instead). */
+int32_t sdt_snapshot(sdt_handle *h, int32_t field_id, int32_t single, void *out, int64_t count);
+/* pinned (page-locked) host memory for snapshot buffers; no reference
+ * counterpart */
+int32_t sdt_host_alloc(int64_t bytes, void **out);
+int32_t sdt_host_free(void *p);
+
 /* kernel timing with HIP events on the handle's stream.  When enabled, each
  * step records events around the stress (H) and velocity (E) kernels;
  * sdt_get_kernel_times returns the summed milliseconds and the number of

@@ -178,6 +178,7 @@ static void release(sdt_handle *h)
     drop(d.rec_out);
     drop(d.adj.src_lam);      // the other adjoint arrays are grid allocations
     drop(d.adj.trace_grad);
+    drop(d.snap);
     for (auto ev : h->tev) (void)hipEventDestroy(ev);
     for (int q = 0; q < 2; ++q) {
         if (d.fork[q]) (void)hipEventDestroy(d.fork[q]);
@@ -509,6 +510,57 @@ int32_t sdt_set_field(sdt_handle *h, int32_t f, const double *host, int64_t coun
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(d.stream));
     if (store_grid(d, d.field[f], host, d.ncomp) != SDT_OK) return fail(h, SDT_ERR_HIP, "field upload failed");
+    return SDT_OK;
+}
+
+// H fields whose Dirichlet faces the EM kernels apply lazily (em2.hip, em3.hip)
+static bool lazy_h_faces(const sdt_handle *h, int f)
+{
+    const Device &d = h->d;
+    const bool hfield = (d.physics == SDT_EM2 && f == SDT_F_HY) || (d.physics == SDT_EM3 && f >= SDT_F_HX);
+    return hfield && h->steps_done > 0;
+}
+
+static int snapshot_check(sdt_handle *h, int32_t f, int32_t single, const void *out, int64_t count)
+{
+    if (!h || !out) return fail(h, SDT_ERR_ARG, "null argument");
+    if (f < 0 || f >= SDT_NUM_FIELDS || !nfields_of(h->d.physics, f))
+        return fail(h, SDT_ERR_ARG, "field not present in this solver");
+    if (h->d.ncomp != 1) return fail(h, SDT_ERR_ARG, "snapshots are written for the real-valued solvers");
+    if (single != 0 && single != 1) return fail(h, SDT_ERR_ARG, "single must be 0 (FP64) or 1 (FP32)");
+    if (count != h->d.ncell) return fail(h, SDT_ERR_ARG, "count must equal nx*ny*nz");
+    return SDT_OK;
+}
+
+int32_t sdt_snapshot(sdt_handle *h, int32_t f, int32_t single, void *out, int64_t count)
+{
+    int rc = snapshot_check(h, f, single, out, count);
+    if (rc) return rc;
+    rc = prepare(h);
+    if (rc) return rc;
+    Device &d = h->d;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    if (!d.snap) HIPCHK(h, hipMalloc(&d.snap, (size_t)d.ncell * sizeof(double)));
+    launch_pack(d, d.field[f], d.snap, single, 0, d.nz, lazy_h_faces(h, f), d.stream);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(out, d.snap, (size_t)count * (single ? 4 : 8), hipMemcpyDeviceToHost, d.stream));
+    return SDT_OK;
+}
+
+int32_t sdt_host_alloc(int64_t bytes, void **out)
+{
+    if (!out || bytes <= 0) return fail(nullptr, SDT_ERR_ARG, "bad pinned allocation request");
+    *out = nullptr;
+    hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipHostMalloc");
+    return SDT_OK;
+}
+
+int32_t sdt_host_free(void *p)
+{
+    if (!p) return SDT_OK;
+    hipError_t e = hipHostFree(p);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipHostFree");
     return SDT_OK;
 }
 

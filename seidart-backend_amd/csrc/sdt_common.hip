@@ -406,6 +406,52 @@ int download_grid_f64(Device &d, const double *dev, double *host)
     return download_grid_t<double>(d, dev, host, 1);
 }
 
+// ---- snapshots ---------------------------------------------------------------
+// Planes k0 .. k0+nk-1 (0-based) of a grid array in the device layout (rows
+// of `pitch` elements, x cells from xoff) -> Fortran order (x fastest, no
+// padding) in dst, converted to the output type.  With zero_faces the cells
+// on the x and z faces (and the y faces when ny > 1) read 0: the Dirichlet
+// faces of H that the EM kernels apply lazily (write_image2/3 of
+// seidartio.f08:366-426 write the field after the Dirichlet step).
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) pack_kernel(const TS *__restrict__ src, TD *__restrict__ dst, int nx, int ny,
+                                                   int nz, int k0, int nk, int pitch, int64_t xoff, int zero_faces)
+{
+    const int64_t n = (int64_t)nx * ny * nk;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = (int)(e % nx);
+    const int64_t r = e / nx;
+    const int j = (int)(r % ny), k = k0 + (int)(r / ny);
+    TS v = src[xoff + i + (int64_t)pitch * (j + (int64_t)ny * k)];
+    if (zero_faces && (i == 0 || i == nx - 1 || k == 0 || k == nz - 1 || (ny > 1 && (j == 0 || j == ny - 1))))
+        v = TS(0);
+    dst[e] = (TD)v;
+}
+
+void launch_pack(const Device &d, const void *src, void *dst, int single, int k0, int nk, bool zero_faces,
+                 hipStream_t s)
+{
+    const int64_t n = (int64_t)d.nx * d.ny * nk;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    const int zf = zero_faces ? 1 : 0;
+    if (d.precision == SDT_FP64) {
+        if (single)
+            hipLaunchKernelGGL((pack_kernel<double, float>), grid, block, 0, s, (const double *)src, (float *)dst,
+                               d.nx, d.ny, d.nz, k0, nk, d.pitch, d.xoff, zf);
+        else
+            hipLaunchKernelGGL((pack_kernel<double, double>), grid, block, 0, s, (const double *)src,
+                               (double *)dst, d.nx, d.ny, d.nz, k0, nk, d.pitch, d.xoff, zf);
+    } else {
+        if (single)
+            hipLaunchKernelGGL((pack_kernel<float, float>), grid, block, 0, s, (const float *)src, (float *)dst,
+                               d.nx, d.ny, d.nz, k0, nk, d.pitch, d.xoff, zf);
+        else
+            hipLaunchKernelGGL((pack_kernel<float, double>), grid, block, 0, s, (const float *)src, (double *)dst,
+                               d.nx, d.ny, d.nz, k0, nk, d.pitch, d.xoff, zf);
+    }
+}
+
 int build_material_table(Device &d, int row_len, const std::function<void(int64_t, double *)> &fill)
 {
     const int64_t n = d.ncell;

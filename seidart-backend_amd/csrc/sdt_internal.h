@@ -212,6 +212,7 @@ struct Device {
     bool timing = false;
     double ms_stress = 0, ms_vel = 0;
     int64_t timed_launches = 0;
+    void *snap = nullptr;                // snapshot staging (ncell doubles), allocated on first use
 
     const double *coef_in = nullptr;   // update coefficients given directly (during setup only)
     bool raw_source = false;           // SDT_OPT_RAW_SOURCE: series injected as given
@@ -334,6 +335,11 @@ constexpr int kEm25TabRow = 16;        // 2.5D EM material-table row length
 void launch_adj_tail(Device &d, int t, void *const lam_field[3]);
 void set_step_counter(hipStream_t s, int *d_it, int v);
 int download_grid_f64(Device &d, const double *dev, double *host);
+// snapshot pack: planes [k0, k0 + nk) of a device grid array -> Fortran order
+// in dst (FP32 when single, else FP64); zero_faces reads the Dirichlet faces
+// of a lazily zeroed H field as 0
+void launch_pack(const Device &d, const void *src, void *dst, int single, int k0, int nk, bool zero_faces,
+                 hipStream_t s);
 // CPML classification.  A cell is an identity cell when a == 0, kappa == 1
 // and b is finite in both the full and the half coefficient set (its memory
 // variables then stay exactly zero and its derivatives pass unchanged).

@@ -132,19 +132,74 @@ def _cpml_inputs(s, workdir, shape):
             s.set_array(stem + sfx, _read(workdir, stem + sfx, shape))
 
 
-def _snapshots(s, workdir, source, it, names, single, is3d):
-    for fname, channel in names:
-        a = s.field(fname)
-        path = os.path.join(workdir, fio.snapshot_name(channel, it, source.xind, source.zind,
-                                                       source.yind if is3d else None))
-        fio.write_record(path, a, np.float32 if single else np.float64)
+class _SnapshotPipeline:
+    """Per-step snapshots written while the next steps run: each snapshot
+    step packs every component on the device (FP32 when single) into one of
+    `depth` sets of pinned host buffers; the files of a set are written in
+    16 MiB chunks by a pool of writer threads (positioned writes into the
+    pre-sized file, then a rename) while the solver advances, and a set is
+    reused once its files are written.  The files are those of
+    write_image2/3 (seidartio.f08:366-426), byte for byte."""
+
+    def __init__(self, s, workdir, source, names, single, is3d, depth=3, writers=None):
+        from concurrent.futures import ThreadPoolExecutor
+        from .solver import PinnedArray
+        self.s, self.workdir, self.source, self.names = s, workdir, source, names
+        self.single, self.is3d = single, is3d
+        dt = np.float32 if single else np.float64
+        self.sets = [[PinnedArray(s.ncell, dt) for _ in names] for _ in range(depth)]
+        self.busy = [[] for _ in range(depth)]
+        self.chunks = ThreadPoolExecutor(max_workers=writers or max(2, min(16, os.cpu_count() or 2)))
+        self.pool = ThreadPoolExecutor(max_workers=len(names))   # per-file completion and rename
+        self.count = 0
+        self.bytes = 0
+
+    def emit(self, it):
+        q = self.count % len(self.sets)
+        for f in self.busy[q]:
+            f.result()
+        bufs = self.sets[q]
+        for (fname, _), b in zip(self.names, bufs):
+            self.s.snapshot(fname, b.array, self.single)
+        self.s.synchronize()
+        src = self.source
+        self.busy[q] = [self.pool.submit(fio.start_record(
+                            os.path.join(self.workdir, fio.snapshot_name(
+                                channel, it, src.xind, src.zind, src.yind if self.is3d else None)),
+                            b.array, self.chunks))
+                        for (_, channel), b in zip(self.names, bufs)]
+        self.count += 1
+        self.bytes += sum(b.array.nbytes + 8 for b in bufs)
+
+    def close(self):
+        try:
+            for fs in self.busy:
+                for f in fs:
+                    f.result()
+        finally:
+            self.pool.shutdown()
+            self.chunks.shutdown()
 
 
 def _time_loop(s, workdir, source, names, single, is3d, snapshot_every):
-    for it in range(1, source.time_steps + 1):
-        s.run(1)
-        if snapshot_every and it % snapshot_every == 0:
-            _snapshots(s, workdir, source, it, names, single, is3d)
+    """Run every step, writing the snapshots of every `snapshot_every`-th
+    step (0: none).  Returns {"loop_s", "snapshots", "bytes"}."""
+    import time
+    t0 = time.perf_counter()
+    pipe = _SnapshotPipeline(s, workdir, source, names, single, is3d) if snapshot_every else None
+    try:
+        it = 0
+        while it < source.time_steps:
+            n = min(snapshot_every, source.time_steps - it) if snapshot_every else source.time_steps - it
+            s.run(n)
+            it += n
+            if pipe and it % snapshot_every == 0:
+                pipe.emit(it)
+    finally:
+        if pipe:
+            pipe.close()
+    return {"loop_s": time.perf_counter() - t0, "snapshots": pipe.count if pipe else 0,
+            "bytes": pipe.bytes if pipe else 0}
 
 
 def seismic2(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *, workdir=".",
@@ -166,7 +221,7 @@ def seismic2(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *,
             comps = ("x", "z") if st == "ac" else ("xx", "xz", "zz")
             for c in comps:
                 s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
-        _time_loop(s, workdir, source, (("vx", "Vx"), ("vz", "Vz")), SINGLE_OUTPUT, False, snapshot_every)
+        return _time_loop(s, workdir, source, (("vx", "Vx"), ("vz", "Vz")), SINGLE_OUTPUT, False, snapshot_every)
 
 
 def seismic3(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *, workdir=".",
@@ -190,11 +245,12 @@ def seismic3(domain, source, density_method="arithmetic", SINGLE_OUTPUT=True, *,
             comps = ("x", "y", "z") if st == "ac" else ("xx", "xy", "xz", "yy", "yz", "zz")
             for c in comps:
                 s.set_source(c, _series(workdir, "seismicsource" + c, source.time_steps))
-        _time_loop(s, workdir, source, (("vx", "Vx"), ("vy", "Vy"), ("vz", "Vz")), SINGLE_OUTPUT, True,
-                   snapshot_every)
+        stats = _time_loop(s, workdir, source, (("vx", "Vx"), ("vy", "Vy"), ("vz", "Vz")), SINGLE_OUTPUT, True,
+                           snapshot_every)
         vn, sn = s.norms()
         fio.write_record(os.path.join(workdir, "velocity_norm.dat"), vn)
         fio.write_record(os.path.join(workdir, "stress_norm.dat"), sn)
+        return stats
 
 
 def electromag2(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="fp64", device=0,
@@ -212,7 +268,7 @@ def electromag2(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
         if not _em_plane_wave(s, source):
             for c in ("x", "z"):
                 s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
-        _time_loop(s, workdir, source, (("ex", "Ex"), ("ez", "Ez")), SINGLE_OUTPUT, False, snapshot_every)
+        return _time_loop(s, workdir, source, (("ex", "Ex"), ("ez", "Ez")), SINGLE_OUTPUT, False, snapshot_every)
 
 
 def electromag3(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="fp64", device=0,
@@ -231,8 +287,8 @@ def electromag3(domain, source, SINGLE_OUTPUT=True, *, workdir=".", precision="f
         if not _em_plane_wave(s, source):
             for c in "xyz":
                 s.set_source(c, _series(workdir, "electromagneticsource" + c, source.time_steps))
-        _time_loop(s, workdir, source, (("ex", "Ex"), ("ey", "Ey"), ("ez", "Ez")), SINGLE_OUTPUT, True,
-                   snapshot_every)
+        return _time_loop(s, workdir, source, (("ex", "Ex"), ("ey", "Ey"), ("ez", "Ez")), SINGLE_OUTPUT, True,
+                          snapshot_every)
 
 
 def _env_logical(name, default=True):

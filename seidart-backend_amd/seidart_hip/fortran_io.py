@@ -71,6 +71,80 @@ def write_record(path, array, dtype=None):
     os.replace(tmp, path)
 
 
+def write_record_raw(path, payload):
+    """Write one record whose payload is already in file order: `payload` is
+    a contiguous array (or buffer) whose bytes are written as they are, with
+    no copy (snapshots packed on the device)."""
+    mv = memoryview(payload).cast("B")
+    tmp = path + ".tmp"
+    with open(tmp, "wb") as f:
+        pos, n = 0, len(mv)
+        while True:
+            size = min(n - pos, _MAX_SUB)
+            last = pos + size >= n
+            f.write(struct.pack("<i", size if last else -size))
+            f.write(mv[pos:pos + size])
+            f.write(struct.pack("<i", -size if pos > 0 else size))
+            pos += size
+            if last:
+                break
+    os.replace(tmp, path)
+
+
+def record_layout(n):
+    """Subrecords of an n-byte payload: (file offset of the head marker,
+    head marker, tail marker, payload offset, size) each, and the file size."""
+    out, pos, off = [], 0, 0
+    while True:
+        size = min(n - pos, _MAX_SUB)
+        last = pos + size >= n
+        out.append((off, size if last else -size, -size if pos > 0 else size, pos, size))
+        off += size + 8
+        pos += size
+        if last:
+            return out, off
+
+
+def _pwrite_all(fd, mv, off):
+    while len(mv):
+        k = os.pwrite(fd, mv, off)
+        mv, off = mv[k:], off + k
+
+
+def start_record(path, payload, pool, chunk=16 << 20):
+    """Queue the write of one record (payload already in file order, kept
+    alive by the caller until the write ends) as parallel chunk writes on the
+    thread pool `pool`.  Returns a callable that waits for them and then
+    moves the finished file to `path`; the bytes equal write_record_raw's."""
+    mv = memoryview(payload).cast("B")
+    subs, total = record_layout(len(mv))
+    tmp = path + ".tmp"
+    fd = os.open(tmp, os.O_WRONLY | os.O_CREAT | os.O_TRUNC, 0o666)
+    futs = []
+    try:
+        os.ftruncate(fd, total)
+        for off, head, tail, ppos, size in subs:
+            _pwrite_all(fd, memoryview(struct.pack("<i", head)), off)
+            _pwrite_all(fd, memoryview(struct.pack("<i", tail)), off + 4 + size)
+            for c in range(0, size, chunk):
+                m = min(chunk, size - c)
+                futs.append(pool.submit(_pwrite_all, fd, mv[ppos + c:ppos + c + m], off + 4 + c))
+    except BaseException:
+        for f in futs:
+            f.cancel()
+        os.close(fd)
+        raise
+
+    def finish():
+        try:
+            for f in futs:
+                f.result()
+        finally:
+            os.close(fd)
+        os.replace(tmp, path)
+    return finish
+
+
 def snapshot_name(channel, it, xind, zind, yind=None):
     """`(a2,'.',i6.6,'.',i0,'.',i0[,'.',i0],'.dat')` of write_image2/3
     (seidartio.f08:378-379, 413-414)."""

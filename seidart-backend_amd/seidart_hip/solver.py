@@ -21,6 +21,26 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+class PinnedArray:
+    """A 1-D numpy array over pinned (page-locked) host memory from
+    sdt_host_alloc, freed when this object is collected.  Device-to-host
+    snapshot copies into it run asynchronously."""
+
+    def __init__(self, n, dtype):
+        dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        L.check(L.lib.sdt_host_alloc(int(n) * dtype.itemsize, C.byref(p)))
+        self._p = p
+        buf = (C.c_char * (int(n) * dtype.itemsize)).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(n))
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            self.array = None
+            L.lib.sdt_host_free(self._p)
+            self._p = None
+
+
 def flatten_coefficients(coef):
     """(ncoef,) + grid per-cell coefficients -> the C-ABI's layout: one
     Fortran-order grid after the other, contiguous FP64."""
@@ -209,6 +229,19 @@ class Solver:
             out = flat = np.empty(self.ncell, dtype=np.float64)
         L.check(L.lib.sdt_get_field(self._h, self.fields[name], _dptr(flat), flat.size), self._h)
         return out.reshape(self.shape, order="F")
+
+    def snapshot(self, name, out, single=True):
+        """Queue a snapshot of field `name` (as the reference's write_image2/3
+        write it: Fortran order, FP32 when single) into `out`, a contiguous
+        float32 / float64 array of ncell values (pinned: see PinnedArray).
+        The copy is asynchronous: `out` holds the snapshot after the next
+        synchronize() or blocking call."""
+        dt = np.float32 if single else np.float64
+        if out.dtype != dt or out.size != self.ncell or not out.flags.c_contiguous:
+            raise ValueError(f"snapshot buffer must be a contiguous {np.dtype(dt).name} array of {self.ncell} values")
+        L.check(L.lib.sdt_snapshot(self._h, self.fields[name], 1 if single else 0,
+                                   out.ctypes.data_as(C.c_void_p), out.size), self._h)
+        return out
 
     def set_field(self, name, a):
         flat = self._complex_grid(a) if self.complex else np.ascontiguousarray(self._grid_array(a))

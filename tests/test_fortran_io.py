@@ -51,3 +51,34 @@ def test_subrecords(tmp_path, monkeypatch):
 def test_snapshot_names():
     assert fio.snapshot_name("Vx", 1, 100, 40) == "Vx.000001.100.40.dat"
     assert fio.snapshot_name("Ez", 123456, 7, 9, yind=8) == "Ez.123456.7.8.9.dat"
+
+
+def test_raw_record_writer_matches_record_writer(tmp_path, monkeypatch):
+    """write_record_raw (payload already in file order, as device snapshots
+    are packed) writes the same bytes as write_record, with and without
+    subrecords."""
+    a = np.random.default_rng(1).standard_normal((5, 4, 3))
+    for dt in (np.float32, np.float64):
+        p1, p2 = str(tmp_path / "a.dat"), str(tmp_path / "b.dat")
+        fio.write_record(p1, a, dt)
+        fio.write_record_raw(p2, np.asfortranarray(a).ravel(order="F").astype(dt))
+        assert open(p1, "rb").read() == open(p2, "rb").read()
+    monkeypatch.setattr(fio, "_MAX_SUB", 40)
+    fio.write_record(p1, a, np.float64)
+    fio.write_record_raw(p2, np.asfortranarray(a).ravel(order="F"))
+    assert open(p1, "rb").read() == open(p2, "rb").read()
+
+
+def test_parallel_chunked_record_matches(tmp_path, monkeypatch):
+    """start_record (chunks written by a thread pool at their offsets)
+    produces the bytes of write_record_raw, also across subrecords."""
+    from concurrent.futures import ThreadPoolExecutor
+    a = np.random.default_rng(2).standard_normal(1000).astype(np.float32)
+    with ThreadPoolExecutor(4) as pool:
+        for max_sub in (fio._MAX_SUB, 1000):
+            monkeypatch.setattr(fio, "_MAX_SUB", max_sub)
+            p1, p2 = str(tmp_path / "a.dat"), str(tmp_path / "b.dat")
+            fio.write_record_raw(p1, a)
+            fio.start_record(p2, a, pool, chunk=96)()
+            assert open(p1, "rb").read() == open(p2, "rb").read()
+            assert not (tmp_path / "b.dat.tmp").exists()
