@@ -202,6 +202,18 @@ int32_t sdt_get_norms(sdt_handle *h, double *vel, double *stress, int64_t count)
  * (sdt_host_alloc) lets the copy overlap host work.  Real-valued solvers only
  * (the 2.5D solvers write zstd blocks instead). */
 int32_t sdt_snapshot(sdt_handle *h, int32_t field_id, int32_t single, void *out, int64_t count);
+/* sdt_snapshot for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  Same arguments and output as sdt_snapshot,
+ * but it allocates nothing: the field is packed through a fixed 16 MiB device
+ * scratch (one per GPU, held until the process ends) in chunks of whole x-y
+ * planes, and the call returns when the copy into out is complete.  The
+ * handle must already be set up (its first sdt_run, which may take 0 steps;
+ * SDT_ERR_STATE otherwise) and one x-y plane of the output (nx*ny*4 or
+ * nx*ny*8 bytes) must fit the scratch (SDT_ERR_ARG otherwise).  Every call
+ * on one GPU uses the same scratch: calls that overlap on one GPU (from
+ * several host threads) race. */
+int32_t sdt_snapshot_noalloc(sdt_handle *h, int32_t field_id, int32_t single, void *out, int64_t count);
 /* pinned (page-locked) host memory for snapshot buffers; no reference
  * counterpart */
 int32_t sdt_host_alloc(int64_t bytes, void **out);

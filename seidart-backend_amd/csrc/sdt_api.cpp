@@ -8,6 +8,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -544,6 +545,64 @@ int32_t sdt_snapshot(sdt_handle *h, int32_t f, int32_t single, void *out, int64_
     launch_pack(d, d.field[f], d.snap, single, 0, d.nz, lazy_h_faces(h, f), d.stream);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(out, d.snap, (size_t)count * (single ? 4 : 8), hipMemcpyDeviceToHost, d.stream));
+    return SDT_OK;
+}
+
+// ---- allocation-free snapshots ----------------------------------------------
+// Device scratch of sdt_snapshot_noalloc: one per GPU, held until the process
+// ends.  Host code never uses the symbol itself (in host code it is only a
+// stand-in, not a device address): the device address is looked up once per
+// device with that device current and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_for(sdt_handle *h, void **ptr)
+{
+    const int dev = h->p.device;
+    if (dev < 0 || dev >= kMaxDevices) return fail(h, SDT_ERR_ARG, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        HIPCHK(h, hipSetDevice(dev));
+        void *p = nullptr;
+        HIPCHK(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = g_scratch_ptr[dev];
+    return SDT_OK;
+}
+
+int32_t sdt_snapshot_noalloc(sdt_handle *h, int32_t f, int32_t single, void *out, int64_t count)
+{
+    int rc = snapshot_check(h, f, single, out, count);
+    if (rc) return rc;
+    if (!h->prepared)
+        return fail(h, SDT_ERR_STATE, "run the handle first (sdt_run with 0 steps sets it up): "
+                                      "this entry point allocates nothing");
+    Device &d = h->d;
+    const size_t esz = single ? 4 : 8;
+    const size_t plane = (size_t)d.nx * d.ny * esz;
+    if (plane > kScratchBytes)
+        return fail(h, SDT_ERR_ARG, "one x-y plane (" + std::to_string(plane) + " bytes) exceeds the " +
+                                        std::to_string(kScratchBytes >> 20) + " MiB snapshot scratch");
+    void *ptr = nullptr;
+    rc = scratch_for(h, &ptr);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->p.device));
+    // planes in chunks that fit the scratch; on one stream the next chunk's
+    // pack waits for the previous chunk's copy out of the scratch
+    const int per = (int)(kScratchBytes / plane);
+    char *dst = (char *)out;
+    for (int k0 = 0; k0 < d.nz; k0 += per) {
+        const int nk = d.nz - k0 < per ? d.nz - k0 : per;
+        launch_pack(d, d.field[f], ptr, single, k0, nk, lazy_h_faces(h, f), d.stream);
+        HIPCHK(h, hipGetLastError());
+        HIPCHK(h, hipMemcpyAsync(dst + (size_t)k0 * plane, ptr, (size_t)nk * plane, hipMemcpyDeviceToHost,
+                                 d.stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(d.stream));
     return SDT_OK;
 }
 

@@ -66,7 +66,7 @@ EXPORTS = [
     "sdt_get_kernel_times", "sdt_synchronize", "sdt_set_option", "sdt_get_info",
     "sdt_batch_create", "sdt_batch_run", "sdt_batch_info", "sdt_batch_destroy",
     "sdt_set_coefficients", "sdt_adjoint", "sdt_batch_adjoint", "sdt_set_plane_wave",
-    "sdt_snapshot", "sdt_host_alloc", "sdt_host_free",
+    "sdt_snapshot", "sdt_host_alloc", "sdt_host_free", "sdt_snapshot_noalloc",
 ]
 
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
@@ -131,6 +131,7 @@ def _load():
         "sdt_snapshot": ([P, I32, I32, P, I64], I32),
         "sdt_host_alloc": ([I64, C.POINTER(P)], I32),
         "sdt_host_free": ([P], I32),
+        "sdt_snapshot_noalloc": ([P, I32, I32, P, I64], I32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
