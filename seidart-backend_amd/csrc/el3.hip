@@ -41,6 +41,7 @@ static El3Args<T> el3_args(Device &d)
     A.remap = d.xcd_remap;
     A.ig_same = d.ig_same ? 1 : 0;
     A.snq = d.shell_snq;
+    A.nslots = d.nslots;
     A.cid = d.shell_cid;
     A.ctab = (const T *)d.shell_ctab;
     A.flags = d.tile_flags;
@@ -214,8 +215,7 @@ int el3_setup(Device &d, const double *const *in)
 
 // CPML-shell tiles (see El3Tile) and their compact memory-variable and CPML
 // coefficient storage.  Tiles are listed plane-chunk by plane-chunk, rows
-// then columns, so that consecutive tiles (which the XCD remapping keeps on
-// one XCD) share stencil halos.
+// then columns, so that consecutive tiles share stencil halos.
 int el3_build_shell(Device &d, const double *const *in, int kc)
 {
     std::vector<El3Tile> tiles;
@@ -231,31 +231,29 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         if (kind == 1) return j < b[2] || j > b[3];
         return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
     };
-    // a tile covers kShellLanes cells of a plane: (1 << txl) x (kShellLanes >> txl)
-    auto add = [&](int x0, int y0, int k0, int k1, int kind, int txl) {
-        const int tw = 1 << txl;
-        for (int lane = 0; lane < kShellLanes; ++lane) {
-            const int i = x0 + (lane & (tw - 1)), j = y0 + (lane >> txl);
-            if (region(kind, i, j)) {
-                tiles.push_back(El3Tile{x0, y0, k0, k1, kind, txl, base, i, j, lane});
-                base += (int64_t)(k1 - k0 + 1) * kShellLanes;
-                return;
-            }
-        }
+    // a tile is listed when one of its cells lies in its region
+    auto add = [&](int x0, int y0, int k0, int k1, int kind, int wide) {
+        const int tw = wide ? 64 : 16, th = kShellThreads / tw;
+        for (int ty = 0; ty < th; ++ty)
+            for (int tx = 0; tx < tw; ++tx)
+                if (region(kind, x0 + tx, y0 + ty)) {
+                    tiles.push_back(El3Tile{x0, y0, k0, k1, kind, wide, base});
+                    base += (int64_t)(k1 - k0 + 1) * kShellThreads;
+                    return;
+                }
     };
-    const int rows64 = kShellLanes >> 6, rows16 = kShellLanes >> 4;   // tile heights of 64- and 16-wide tiles
     auto zslab = [&](int ka, int kb) {
         for (int k0 = ka; k0 <= kb; k0 += kc) {
             const int k1 = k0 + kc - 1 < kb ? k0 + kc - 1 : kb;
-            for (int y0 = 1; y0 <= d.ny; y0 += rows64)
-                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 0, 6);
+            for (int y0 = 1; y0 <= d.ny; y0 += 4)
+                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 0, 1);
         }
     };
     // 16-wide tiles aligned to the line grid that cover columns [lo, hi]
     auto xstrip = [&](int lo, int hi, int y0, int k0, int k1) {
         if (lo > hi) return;
         int x0 = xa + ((lo - xa) / line) * line;
-        for (; x0 <= hi; x0 += 16) add(x0, y0, k0, k1, 2, 4);
+        for (; x0 <= hi; x0 += 16) add(x0, y0, k0, k1, 2, 0);
     };
     if (empty) {
         zslab(1, d.nz);
@@ -263,14 +261,14 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         zslab(1, b[4] - 1);
         for (int k0 = b[4]; k0 <= b[5]; k0 += kc) {
             const int k1 = k0 + kc - 1 < b[5] ? k0 + kc - 1 : b[5];
-            for (int y0 = 1; y0 < b[2]; y0 += rows64)
-                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 6);
-            for (int y0 = b[2]; y0 <= b[3]; y0 += rows16) {
+            for (int y0 = 1; y0 < b[2]; y0 += 4)
+                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 1);
+            for (int y0 = b[2]; y0 <= b[3]; y0 += 16) {
                 xstrip(1, b[0] - 1, y0, k0, k1);
                 xstrip(b[1] + 1, d.nx, y0, k0, k1);
             }
-            for (int y0 = b[3] + 1; y0 <= d.ny; y0 += rows64)
-                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 6);
+            for (int y0 = b[3] + 1; y0 <= d.ny; y0 += 4)
+                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 1);
         }
         zslab(b[5] + 1, d.nz);
     }
@@ -281,6 +279,7 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
     if (hipMemcpy(d.tiles, tiles.data(), tiles.size() * sizeof(El3Tile), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
     // CPML coefficients of every slot: a, b, 1/kappa, a_half, b_half, 1/kappa_half
+    // (identity values for slots outside the grid)
     const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
     auto coef = [&](int q, int i, int j, int k) {
         const double *src = in[cids[q]];
@@ -289,6 +288,14 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         if (src && i >= 1 && i <= d.nx && j >= 1 && j <= d.ny)
             v = src[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))];
         return inverse ? 1.0 / v : v;
+    };
+    auto for_slots = [&](const std::function<void(int64_t, int, int, int)> &f) {
+        for (const El3Tile &t : tiles) {
+            const int tw = t.wide ? 64 : 16;
+            for (int k = t.k0; k <= t.k1; ++k)
+                for (int th = 0; th < kShellThreads; ++th)
+                    f(t.base + (int64_t)(k - t.k0) * kShellThreads + th, t.x0 + th % tw, t.y0 + th / tw, k);
+        }
     };
     // Coefficient dictionary: CPML profiles give few distinct tuples (tens
     // for the usual per-axis profiles), so the shell keeps a one-byte id per
@@ -302,33 +309,29 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
     bool dict = true;
     {
         std::unordered_multimap<uint64_t, int> index;
-        for (const El3Tile &t : tiles) {
-            const int tw = 1 << t.txl, th = kShellLanes >> t.txl;
-            for (int k = t.k0; k <= t.k1 && dict; ++k)
-                for (int ty = 0; ty < th && dict; ++ty)
-                    for (int tx = 0; tx < tw; ++tx) {
-                        Tup u;
-                        memset(&u, 0, sizeof(u));
-                        for (int q = 0; q < 6; ++q) u.v[q] = coef(q, t.x0 + tx, t.y0 + ty, k);
-                        uint64_t h = 1469598103934665603ull;
-                        for (int q = 0; q < 6; ++q) {
-                            uint64_t w;
-                            memcpy(&w, &u.v[q], 8);
-                            h = (h ^ w) * 1099511628211ull;
-                        }
-                        int id = -1;
-                        auto range = index.equal_range(h);
-                        for (auto it = range.first; it != range.second; ++it)
-                            if (tups[(size_t)it->second] == u) { id = it->second; break; }
-                        if (id < 0) {
-                            if ((int)tups.size() == 256) { dict = false; break; }
-                            id = (int)tups.size();
-                            tups.push_back(u);
-                            index.emplace(h, id);
-                        }
-                        ids[(size_t)(t.base + (int64_t)(k - t.k0) * kShellLanes + ty * tw + tx)] = (uint8_t)id;
-                    }
-        }
+        for_slots([&](int64_t slot, int i, int j, int k) {
+            if (!dict) return;
+            Tup u;
+            memset(&u, 0, sizeof(u));
+            for (int q = 0; q < 6; ++q) u.v[q] = coef(q, i, j, k);
+            uint64_t h = 1469598103934665603ull;
+            for (int q = 0; q < 6; ++q) {
+                uint64_t w;
+                memcpy(&w, &u.v[q], 8);
+                h = (h ^ w) * 1099511628211ull;
+            }
+            int id = -1;
+            auto range = index.equal_range(h);
+            for (auto it = range.first; it != range.second; ++it)
+                if (tups[(size_t)it->second] == u) { id = it->second; break; }
+            if (id < 0) {
+                if ((int)tups.size() == 256) { dict = false; return; }
+                id = (int)tups.size();
+                tups.push_back(u);
+                index.emplace(h, id);
+            }
+            ids[(size_t)slot] = (uint8_t)id;
+        });
     }
     d.shell_snq = dict ? M_TOTAL : S_NQ;
     if (dict) {
@@ -340,26 +343,26 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         if (upload_converted(d, flat.data(), &d.shell_ctab, (int64_t)flat.size()) != SDT_OK) return SDT_ERR_HIP;
         d.shell_ncoef = (int)tups.size();
     }
-    // Shell storage, blocked by tile-plane: the snq streams of one tile-plane
-    // (memory variables, then without the dictionary a, b, 1/kappa, a_half,
-    // b_half, 1/kappa_half) are kShellLanes-element runs of one contiguous
-    // block, so a workgroup's accesses of a plane fall in one block (one TLB
-    // entry, open DRAM rows).
-    const int snq = d.shell_snq;
-    const int64_t nblk = d.nslots / kShellLanes;
-    std::vector<double> blk((size_t)(nblk * snq * kShellLanes), 0.0);
-    if (!dict)
-        for (int q = 0; q < 6; ++q)
-            for (const El3Tile &t : tiles) {
-                const int tw = 1 << t.txl, th = kShellLanes >> t.txl;
-                for (int k = t.k0; k <= t.k1; ++k) {
-                    const int64_t tp = t.base / kShellLanes + (k - t.k0);
-                    double *row = &blk[(size_t)((tp * snq + S_CF + q) * kShellLanes)];
-                    for (int ty = 0; ty < th; ++ty)
-                        for (int tx = 0; tx < tw; ++tx) row[ty * tw + tx] = coef(q, t.x0 + tx, t.y0 + ty, k);
-                }
+    // Shell storage, stream-major: stream q of slot s at q * nslots + s (the
+    // M_TOTAL memory variables, zero at the start, then without the
+    // dictionary the six coefficient streams).
+    if (alloc_zeroed(d, &d.shell, (int64_t)d.shell_snq * d.nslots) != SDT_OK) return SDT_ERR_HIP;
+    if (!dict) {
+        std::vector<double> cs((size_t)d.nslots);
+        for (int q = 0; q < 6; ++q) {
+            for_slots([&](int64_t slot, int i, int j, int k) { cs[(size_t)slot] = coef(q, i, j, k); });
+            const size_t bytes = (size_t)d.nslots * d.esize;
+            char *dst = (char *)d.shell + (size_t)(S_CF + q) * bytes;
+            hipError_t e;
+            if (d.precision == SDT_FP64) {
+                e = hipMemcpy(dst, cs.data(), bytes, hipMemcpyHostToDevice);
+            } else {
+                std::vector<float> f(cs.begin(), cs.end());
+                e = hipMemcpy(dst, f.data(), bytes, hipMemcpyHostToDevice);
             }
-    if (upload_converted(d, blk.data(), &d.shell, (int64_t)blk.size()) != SDT_OK) return SDT_ERR_HIP;
+            if (e != hipSuccess) return SDT_ERR_HIP;
+        }
+    }
     return SDT_OK;
 }
 

@@ -27,8 +27,9 @@ struct El3Args {
     // xy xz yz (6), rho, 4 unused
     const uint8_t *mid;
     const T *tab;
-    T *shell;       // CPML-shell storage: [tile-plane][snq][kShellLanes]
-    int snq;        // streams per tile-plane block: M_TOTAL (+ 6 coefficient streams without ctab)
+    T *shell;       // CPML-shell storage: snq streams of nslots values (stream q of slot s at q * nslots + s)
+    int snq;        // streams: M_TOTAL memory variables (+ 6 coefficient streams without ctab)
+    int64_t nslots;
     // CPML coefficient dictionary of the shell (at most 256 distinct
     // (a, b, 1/kappa, a_half, b_half, 1/kappa_half) tuples): id per slot and
     // rows of 8 (6 used); ctab == nullptr keeps the coefficients in the block
@@ -92,17 +93,15 @@ __device__ __forceinline__ T tplus5(T fm1, T f0, T fp1, T fp2, int corr)
     return corr ? T(27) * (fp1 - f0) - (fp2 - fm1) : -T(27) * f0 + T(27) * fp1 - fp2;
 }
 
-// One z-marching tile of the CPML shell: x-y footprint (1 << txl) x
-// (kShellLanes >> txl) cells from (x0, y0), planes k0..k1, region kind (0 z
-// slab, 1 y strip, 2 x strip) and the first compact slot of its memory
-// variables (slot = base + (k - k0) * kShellLanes + lane).
-constexpr int kShellLanes = 64;
+// One z-marching tile of the CPML shell: kShellThreads cells of a plane, 64 x 4
+// (wide: z slabs and y strips) or 16 x 16 (x strips) from (x0, y0), planes
+// k0..k1, region kind (0 z slab, 1 y strip, 2 x strip) and the first slot of
+// its memory variables (slot = base + (k - k0) * kShellThreads + thread).
+constexpr int kShellThreads = 256;
 struct El3Tile {
     int x0, y0, k0, k1;
-    int kind, txl;
+    int kind, wide;
     int64_t base;
-    int ax, ay, alane;   // an active cell of the tile and its lane: inactive lanes
-                         // load from it, so they add no memory traffic
 };
 
 // Where the stress kernels read a cell's stiffness from: all 21 per-cell

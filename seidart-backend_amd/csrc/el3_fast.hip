@@ -9,13 +9,13 @@
 //    load of a plane is unconditional so the compiler issues them together.
 //
 //  * shell kernels over the cells outside the box (the CPML slabs and the
-//    Dirichlet faces), as a list of z-marching tiles of 64 cells per plane:
-//    64 x 1 over the z slabs and the y strips, 16 x 4 over the (12-cell) x
-//    strips, three cooperating wavefronts per tile.  Their memory variables
-//    and CPML coefficients are stored compactly in tile order, so those
-//    streams are contiguous per wavefront.  Neighbour indices are clamped
-//    into the grid so every load is unconditional; nest ranges only mask the
-//    stores.
+//    Dirichlet faces), as a list of z-marching tiles of 256 cells per plane,
+//    one thread per cell like the interior kernels: 64 x 4 over the z slabs
+//    and the y strips, 16 x 16 over the (12-cell) x strips.  Their memory
+//    variables and CPML coefficient ids are stored compactly in tile order,
+//    so those streams are contiguous per wavefront.  Neighbour indices are
+//    clamped into the grid so every load is unconditional; nest ranges only
+//    mask the stores.
 //
 // Both write disjoint cells of the same arrays and fold into the same per-step
 // norm slots, so their order within a step does not matter.
@@ -310,63 +310,58 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 }
 
 // --------------------------------------------------------------------- shell
-// A shell tile is kShellLanes (64) cells of a plane (64 x 1 over the z slabs
-// and y strips, 16 x 4 over the x strips) marched over planes k0..k1.  Its
-// workgroup is three wavefronts over the same 64 cells, so that no lane has to
-// hold a whole cell's state:
-//   stress pass: wavefront m computes the seven CPML-corrected derivatives of
-//     velocity component m (updating its memory variables) into LDS; after a
-//     barrier each wavefront updates two of the six stresses from them;
-//   velocity pass: wavefront m updates velocity component m (its three
-//     memory variables); the three meet in LDS only for the norm.
-// Lanes outside the grid or the tile's region load from an active cell of the
-// tile (no extra traffic) and store nothing.
-__device__ __forceinline__ bool shell_lane(const El3Tile &t, const int (&b)[6], int i, int j, int nx, int ny)
+// The CPML shell (the cells outside the box: the z slabs, y strips and x
+// strips, Dirichlet faces included) as a list of z-marching tiles of
+// kShellThreads cells per plane, one thread per cell: 64 x 4 over the z slabs
+// and y strips, 16 x 16 over the (12-cell) x strips, so every wavefront reads
+// whole 128-byte lines.  A thread keeps its column's z stencil planes in
+// registers, issues every load of a plane before its first store, and
+// updates all of its cell's memory variables and fields itself -- no LDS and
+// no barriers, so each wavefront keeps a plane's loads in flight like the
+// interior kernels.  Memory variables are stored stream-major (stream q of
+// slot s at shell[q * nslots + s], slot = tile base + (k - k0) * 256 +
+// thread), so a wavefront's access of one stream is 64 consecutive values.
+// Lanes outside the grid or the tile's region compute on clamped indices and
+// store nothing; neighbour indices are clamped into the grid, and the nest
+// ranges of cpmlfdtd.f08 mask the stores.
+__device__ __forceinline__ bool shell_lane(int kind, const int (&b)[6], int i, int j, int nx, int ny)
 {
-    if (i < 1 || i > nx || j > ny) return false;
-    if (t.kind == 0) return true;
-    if (t.kind == 1) return j < b[2] || j > b[3];
+    if (i < 1 || i > nx || j < 1 || j > ny) return false;
+    if (kind == 0) return true;
+    if (kind == 1) return j < b[2] || j > b[3];
     return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
 }
 
-struct ShellCol {
+struct ShellCell {
     bool on;
-    int i, j;                 // cell (i, j) of the lane (the anchor cell when !on)
-    int64_t col;              // element of (i, j, 1)
-    int64_t xm2, xm1, xp1, xp2, ym2, ym1, yp1, yp2;   // neighbour offsets clamped into the grid
+    int i, j;                                     // clamped into the grid
+    int64_t col;                                  // element of (i, j, 1)
+    int xm2, xm1, xp1, xp2, ym2, ym1, yp1, yp2;   // neighbour offsets clamped into the grid (elements)
 };
 
-__device__ __forceinline__ ShellCol shell_col(const El3Tile &tl, const int (&box)[6], int lane, int nx, int ny,
-                                              int64_t sy, int64_t xoff)
+__device__ __forceinline__ ShellCell shell_cell(const El3Tile &tl, const int (&box)[6], int nx, int ny, int64_t sy,
+                                                int64_t xoff)
 {
-    ShellCol L;
-    const int tx = lane & ((1 << tl.txl) - 1), ty = lane >> tl.txl;
-    const int i0 = tl.x0 + tx, j0 = tl.y0 + ty;
-    L.on = shell_lane(tl, box, i0, j0, nx, ny);
-    L.i = L.on ? i0 : tl.ax;
-    L.j = L.on ? j0 : tl.ay;
-    const int i = L.i, j = L.j;
+    ShellCell L;
+    const int tw = tl.wide ? 64 : 16;
+    const int t = threadIdx.x;
+    const int i0 = tl.x0 + t % tw, j0 = tl.y0 + t / tw;
+    L.on = shell_lane(tl.kind, box, i0, j0, nx, ny);
+    const int i = min(max(i0, 1), nx), j = min(max(j0, 1), ny);
+    L.i = i;
+    L.j = j;
     L.xm2 = max(i - 2, 1) - i; L.xm1 = max(i - 1, 1) - i; L.xp1 = min(i + 1, nx) - i; L.xp2 = min(i + 2, nx) - i;
-    L.ym2 = (max(j - 2, 1) - j) * sy; L.ym1 = (max(j - 1, 1) - j) * sy;
-    L.yp1 = (min(j + 1, ny) - j) * sy; L.yp2 = (min(j + 2, ny) - j) * sy;
+    L.ym2 = (max(j - 2, 1) - j) * (int)sy; L.ym1 = (max(j - 1, 1) - j) * (int)sy;
+    L.yp1 = (min(j + 1, ny) - j) * (int)sy; L.yp2 = (min(j + 2, ny) - j) * (int)sy;
     L.col = xoff + (int64_t)(i - 1) + sy * (j - 1);
     return L;
 }
 
-// First element of a lane's streams in the tile's first plane; the next
-// plane's block follows after S_NQ * kShellLanes elements.
-template <typename T>
-__device__ __forceinline__ T *shell_block(const El3Args<T> &A, const El3Tile &tl, int lane)
-{
-    return A.shell + (tl.base / kShellLanes) * (int64_t)(A.snq * kShellLanes) + lane;
-}
-
-// CPML coefficients of a shell slot, the source fixed at compile time (no
-// branch, so the scheduler can issue the id load with the plane's other
-// loads): from the dictionary (one-byte id, cached table) or from the
-// block's coefficient streams.
+// CPML coefficients a, b, 1/kappa, a_half, b_half, 1/kappa_half of a shell
+// slot: from the dictionary (one-byte id, cached rows of 8) or from the six
+// coefficient streams after the memory variables.
 template <typename T, bool CTAB>
-__device__ __forceinline__ void shell_coefs_t(const El3Args<T> &A, const T *blk, int64_t slot, T (&cf)[6])
+__device__ __forceinline__ void shell_coefs(const El3Args<T> &A, int64_t slot, T (&cf)[6])
 {
     if (CTAB) {
         const T *row = A.ctab + 8 * (int)A.cid[slot];
@@ -374,129 +369,51 @@ __device__ __forceinline__ void shell_coefs_t(const El3Args<T> &A, const T *blk,
         for (int r = 0; r < 6; ++r) cf[r] = row[r];
     } else {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
-    }
-}
-
-// CPML coefficients of a shell slot: from the dictionary (one-byte id, cached
-// table) or from the block's coefficient streams.
-template <typename T>
-__device__ __forceinline__ void shell_coefs(const El3Args<T> &A, const T *blk, int64_t slot, T (&cf)[6])
-{
-    if (A.ctab) {
-        const T *row = A.ctab + 8 * (int)A.cid[slot];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = row[r];
-    } else {
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
-    }
-}
-
-// The same in two steps (the velocity kernel loads the slot's dictionary id a
-// plane ahead, then reads the cached row).
-template <typename T>
-__device__ __forceinline__ int coef_id(const El3Args<T> &A, int64_t slot)
-{
-    return A.ctab ? (int)A.cid[slot] : 0;
-}
-template <typename T>
-__device__ __forceinline__ void shell_coefs_id(const El3Args<T> &A, const T *blk, int id, T (&cf)[6])
-{
-    if (A.ctab) {
-        const T *row = A.ctab + 8 * id;
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = row[r];
-    } else {
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = blk[(S_CF + r) * kShellLanes];
+        for (int r = 0; r < 6; ++r) cf[r] = ldnt(&A.shell[(S_CF + r) * A.nslots + slot]);
     }
 }
 
 template <typename T>
-__device__ __forceinline__ T cpml_upd(T d, const T *psi_p, T a, T b, T ik, T &psi_out)
+__device__ __forceinline__ T cpml_upd(T d, T psi, T a, T b, T ik, T &psi_out)
 {
-    psi_out = b * ldnt(psi_p) + a * d;
+    psi_out = b * psi + a * d;
     return d * ik + psi_out;
 }
 
-// Stress pass, phase 1 for velocity component M: the four stress nests
-// (cpmlfdtd.f08:1761-1992) use seven (direction, stencil, coefficient set)
-// derivatives per component; each memory variable is updated once, on the
-// union of the ranges of the nests that read it.
-// INSIDE: read the CPML coefficients here, after the velocity loads and in
-// their block, so the dictionary id's latency overlaps theirs (measured
-// faster with the isotropic arrays, slower with the table or all 21 arrays);
-// otherwise the caller passes them.
-template <typename T, int M, bool CTAB, bool INSIDE>
-__device__ __forceinline__ void shell_derivs(const El3Args<T> &A, const ShellCol &L, T (&q)[5], int64_t c,
-                                             int64_t zp2, T *blk, bool L1, bool L2, bool L3, bool L4,
-                                             int64_t slot, const T (&cf_in)[6], T *D, int lane)
-{
-    const T *v = A.v[M];
-    q[4] = v[c + zp2];
-    const T v0 = q[2];
-    const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2];
-    const T dxp = dp4(v[c + L.xm1], v0, v[c + L.xp1], v[c + L.xp2], rx);
-    const T dxm = dm4(v[c + L.xm2], v[c + L.xm1], v0, v[c + L.xp1], rx);
-    const T dyp = dp4(v[c + L.ym1], v0, v[c + L.yp1], v[c + L.yp2], ry);
-    const T dym = dm4(v[c + L.ym2], v[c + L.ym1], v0, v[c + L.yp1], ry);
-    const T dzp = dp4(q[1], v0, q[3], q[4], rz);
-    const T dzm = dm4(q[0], q[1], v0, q[3], rz);
-    T cf[6];
-    if (INSIDE) {
-        shell_coefs_t<T, CTAB>(A, blk, slot, cf);
-    } else {
-#pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = cf_in[r];
-    }
-    const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
-    T *mm = blk + M * K_NKIND * kShellLanes;   // this component's seven streams
-    T *Dm = D + M * K_NKIND * kShellLanes + lane;
-    T p0, p1, p2, p3, p4, p5 = T(0), p6;
-    Dm[K_XPH * kShellLanes] = cpml_upd(dxp, &mm[K_XPH * kShellLanes], ah, bh, ikh, p0);
-    Dm[K_XMF * kShellLanes] = cpml_upd(dxm, &mm[K_XMF * kShellLanes], a, b, ik, p1);
-    Dm[K_YMF * kShellLanes] = cpml_upd(dym, &mm[K_YMF * kShellLanes], a, b, ik, p2);
-    Dm[K_YPH * kShellLanes] = cpml_upd(dyp, &mm[K_YPH * kShellLanes], ah, bh, ikh, p3);
-    Dm[K_ZMF * kShellLanes] = cpml_upd(dzm, &mm[K_ZMF * kShellLanes], a, b, ik, p4);
-    // vx's z derivative enters sigma_xy uncorrected (cpmlfdtd.f08:1858-1865)
-    Dm[K_ZMH * kShellLanes] = M == 0 ? dzm : cpml_upd(dzm, &mm[K_ZMH * kShellLanes], ah, bh, ikh, p5);
-    Dm[K_ZPH * kShellLanes] = cpml_upd(dzp, &mm[K_ZPH * kShellLanes], ah, bh, ikh, p6);
-    if (L1 || L4) stnt(&mm[K_XPH * kShellLanes], p0);
-    if (L2 || L3) stnt(&mm[K_XMF * kShellLanes], p1);
-    if (L1 || L3) stnt(&mm[K_YMF * kShellLanes], p2);
-    if (L2 || L4) stnt(&mm[K_YPH * kShellLanes], p3);
-    if (L1) stnt(&mm[K_ZMF * kShellLanes], p4);
-    if (M != 0 && L2) stnt(&mm[K_ZMH * kShellLanes], p5);
-    if (L3 || L4) stnt(&mm[K_ZPH * kShellLanes], p6);
-}
-
-// Row accessors of the stiffness entries (0..20) and attenuation divisors
-// (21..26) of one cell.
+// Row accessors of a cell's stiffness entries (0..20) and attenuation
+// divisors (21..26), read where they are used.
 template <typename T>
 struct ScalarRow {   // wave-uniform table row, scalar loads
     const_row<T> p;
     __device__ T operator[](int q) const { return p[q]; }
 };
 template <typename T>
-struct LaneRow {     // per-lane table row
+struct LaneRow {     // per-lane table row (cached)
     const T *p;
     __device__ T operator[](int q) const { return p[q]; }
 };
 template <typename T>
-struct DenseRow {    // per-cell arrays (cached loads: the three wavefronts share entries)
-    const El3Args<T> *A;
+struct DenseRow {    // per-cell arrays (C: the 21 stiffness arrays, ig: the six divisor arrays)
+    const T *const *C;
+    const T *const *ig;
     int64_t c;
-    __device__ T operator[](int q) const { return q < 21 ? A->C[q][c] : A->ig[q - 21][c]; }
+    int ig_same;
+    __device__ T operator[](int q) const { return q < 21 ? ldnt(&C[q][c]) : ldnt(&ig[ig_same ? 0 : q - 21][c]); }
 };
 template <typename T>
 struct IsoRow {      // isotropic-form per-cell arrays c11, c12, c44
-    const El3Args<T> *A;
+    const T *const *ig;
     int64_t c;
+    int ig_same;
     T c11, c12, c44;
-    __device__ T operator[](int q) const { return q < 21 ? iso_entry(q, c11, c12, c44) : A->ig[q - 21][c]; }
+    __device__ T operator[](int q) const
+    {
+        return q < 21 ? iso_entry(q, c11, c12, c44) : ldnt(&ig[ig_same ? 0 : q - 21][c]);
+    }
 };
 
+// (sigma + (sum_slot C[row][slot] d_slot) dt) * 1/(1 + gamma dt), slots in the
+// reference's order dxx, dyy, dzz, yz, xz, xy
 template <typename T, typename R>
 __device__ __forceinline__ T supr(T s, const R &K, int i0, int i1, int i2, int i3, int i4, int i5, T dxx, T dyy,
                                   T dzz, T yz, T xz, T xy, T dt, int ig)
@@ -504,315 +421,234 @@ __device__ __forceinline__ T supr(T s, const R &K, int i0, int i1, int i2, int i
     return (s + (K[i0] * dxx + K[i1] * dyy + K[i2] * dzz + K[i3] * yz + K[i4] * xz + K[i5] * xy) * dt) * K[ig];
 }
 
-// Stress pass, phase 2 for wavefront W: W = 0 sigma_xx, sigma_yy (L1);
-// W = 1 sigma_zz (L1), sigma_xy (L2); W = 2 sigma_xz (L3), sigma_yz (L4).
-// Returns this wavefront's share of the stress norm's sum of squares.
-template <typename T, int W, typename R>
-__device__ __forceinline__ double shell_stresses(const El3Args<T> &A, const R &K, int64_t c, const T *D, int lane,
-                                                 bool L1, bool L2, bool L3, bool L4, T sa, T sb)
+// The six stress updates of one shell cell from its 21 corrected
+// derivatives D[component][kind]; masked by the nest ranges.
+template <typename T, typename R>
+__device__ __forceinline__ void shell_stress_update(const R &K, const T (&D)[3][K_NKIND], const T (&so)[6], T dt,
+                                                    bool L1, bool L2, bool L3, bool L4, T (&sn)[6])
 {
-    auto d = [&](int m, int kind) { return D[(m * K_NKIND + kind) * kShellLanes + lane]; };
-    const T dt = A.dt;
-    if (W == 0 || W == 1) {
-        // L1 derivatives: D+x half, D-y full, D-z full
-        const T dxx = d(0, K_XPH), dyy = d(1, K_YMF), dzz = d(2, K_ZMF);
-        const T yz = d(1, K_ZMF) + d(2, K_YMF), xz = d(0, K_ZMF) + d(2, K_XPH), xy = d(1, K_XPH) + d(0, K_YMF);
-        if (W == 0) {
-            T sxx = sa, syy = sb;
-            const T n0 = supr(sxx, K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 0);
-            const T n1 = supr(syy, K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 1);
-            if (L1) { sxx = n0; syy = n1; stnt(&A.s[0][c], sxx); stnt(&A.s[1][c], syy); }
-            const double a0 = sxx, a1 = syy;
-            return a0 * a0 + a1 * a1;
-        }
-        T szz = sa, sxy = sb;
-        const T n2 = supr(szz, K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 2);
-        if (L1) { szz = n2; stnt(&A.s[2][c], szz); }
-        {   // L2 derivatives: D-x full, D+y half, D-z half (vx's uncorrected)
-            const T ex = d(0, K_XMF), ey = d(1, K_YPH), ez = d(2, K_ZMH);
-            const T eyz = d(1, K_ZMH) + d(2, K_YPH), exz = d(0, K_ZMH) + d(2, K_XMF), exy = d(1, K_XMF) + d(0, K_YPH);
-            const T n3 = supr(sxy, K, 5, 10, 14, 17, 19, 20, ex, ey, ez, eyz, exz, exy, dt, kTabIg + 3);
-            if (L2) { sxy = n3; stnt(&A.s[3][c], sxy); }
-        }
-        const double a2 = szz, a3 = sxy;
-        return a2 * a2 + 2.0 * (a3 * a3);
+    {   // L1: D+x half, D-y full, D-z full
+        const T dxx = D[0][K_XPH], dyy = D[1][K_YMF], dzz = D[2][K_ZMF];
+        const T yz = D[1][K_ZMF] + D[2][K_YMF], xz = D[0][K_ZMF] + D[2][K_XPH], xy = D[1][K_XPH] + D[0][K_YMF];
+        sn[0] = L1 ? supr(so[0], K, 0, 1, 2, 3, 4, 5, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 0) : so[0];
+        sn[1] = L1 ? supr(so[1], K, 1, 6, 7, 8, 9, 10, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 1) : so[1];
+        sn[2] = L1 ? supr(so[2], K, 2, 7, 11, 12, 13, 14, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 2) : so[2];
     }
-    T sxz = sa, syz = sb;
-    {   // L3 derivatives: D-x full, D-y full, D+z half
-        const T dxx = d(0, K_XMF), dyy = d(1, K_YMF), dzz = d(2, K_ZPH);
-        const T yz = d(1, K_ZPH) + d(2, K_YMF), xz = d(0, K_ZPH) + d(2, K_XMF), xy = d(1, K_XMF) + d(0, K_YMF);
-        const T n4 = supr(sxz, K, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 4);
-        if (L3) { sxz = n4; stnt(&A.s[4][c], sxz); }
+    {   // L2: D-x full, D+y half, D-z half (vx's uncorrected)
+        const T dxx = D[0][K_XMF], dyy = D[1][K_YPH], dzz = D[2][K_ZMH];
+        const T yz = D[1][K_ZMH] + D[2][K_YPH], xz = D[0][K_ZMH] + D[2][K_XMF], xy = D[1][K_XMF] + D[0][K_YPH];
+        sn[3] = L2 ? supr(so[3], K, 5, 10, 14, 17, 19, 20, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 3) : so[3];
     }
-    {   // L4 derivatives: D+x half, D+y half, D+z half
-        const T dxx = d(0, K_XPH), dyy = d(1, K_YPH), dzz = d(2, K_ZPH);
-        const T yz = d(1, K_ZPH) + d(2, K_YPH), xz = d(0, K_ZPH) + d(2, K_XPH), xy = d(1, K_XPH) + d(0, K_YPH);
-        const T n5 = supr(syz, K, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 5);
-        if (L4) { syz = n5; stnt(&A.s[5][c], syz); }
+    {   // L3: D-x full, D-y full, D+z half
+        const T dxx = D[0][K_XMF], dyy = D[1][K_YMF], dzz = D[2][K_ZPH];
+        const T yz = D[1][K_ZPH] + D[2][K_YMF], xz = D[0][K_ZPH] + D[2][K_XMF], xy = D[1][K_XMF] + D[0][K_YMF];
+        sn[4] = L3 ? supr(so[4], K, 4, 9, 13, 16, 18, 19, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 4) : so[4];
     }
-    const double a4 = syz, a5 = sxz;
-    return 2.0 * (a4 * a4 + a5 * a5);
+    {   // L4: D+x half, D+y half, D+z half
+        const T dxx = D[0][K_XPH], dyy = D[1][K_YPH], dzz = D[2][K_ZPH];
+        const T yz = D[1][K_ZPH] + D[2][K_YPH], xz = D[0][K_ZPH] + D[2][K_XPH], xy = D[1][K_XPH] + D[0][K_YPH];
+        sn[5] = L4 ? supr(so[5], K, 3, 8, 12, 15, 16, 17, dxx, dyy, dzz, yz, xz, xy, dt, kTabIg + 5) : so[5];
+    }
 }
 
-// Phase-2 operands of one plane that do not depend on phase 1: this
-// wavefront's two stresses and the handle of the cell's material (table id,
-// or the isotropic entries).
-template <typename T>
-struct ShellPre {
-    T sa, sb;
-    int m;
-    T c11, c12, c44;
-};
-
-template <typename T, int MAT>
-__device__ __forceinline__ ShellPre<T> shell_pre(const El3Args<T> &A, int64_t c, int w)
-{
-    ShellPre<T> P;
-    P.sa = ldnt(&A.s[2 * w][c]);
-    P.sb = ldnt(&A.s[2 * w + 1][c]);
-    P.m = 0;
-    P.c11 = P.c12 = P.c44 = T(0);
-    if (MAT == kMatTable) P.m = A.mid[c];
-    if (MAT == kMatIso) {
-        P.c11 = A.C[0][c];
-        P.c12 = A.C[1][c];
-        P.c44 = A.C[15][c];
-    }
-    return P;
-}
-
-template <typename T, int MAT, int W>
-__device__ __forceinline__ double shell_stresses_at(const El3Args<T> &A, int64_t c, const T *D, int lane, bool L1,
-                                                    bool L2, bool L3, bool L4, const ShellPre<T> &P)
-{
-    if (MAT == kMatIso) {
-        const IsoRow<T> K{&A, c, P.c11, P.c12, P.c44};
-        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
-    }
-    if (MAT == kMatTable) {
-        const int m = P.m;
-        const int m0 = __builtin_amdgcn_readfirstlane(m);
-        if (__all(m == m0)) {
-            const ScalarRow<T> K{(const_row<T>)(A.tab) + m0 * kTabRow};
-            return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
-        }
-        const LaneRow<T> K{A.tab + m * kTabRow};
-        return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
-    }
-    const DenseRow<T> K{&A, c};
-    return shell_stresses<T, W>(A, K, c, D, lane, L1, L2, L3, L4, P.sa, P.sb);
-}
-
-// The stress kernel runs its two phases as a pipeline over the planes: in one
-// barrier interval a wavefront computes the derivatives of plane k+1 (into
-// one LDS buffer) and two stresses of plane k (from the other), so the loads
-// of both phases are in flight together.  Buffers alternate per plane; every
-// buffer is rewritten only after the barrier that follows its last read.
+// Stress pass (cpmlfdtd.f08:1761-1992): per velocity component the seven
+// (direction, stencil, coefficient set) derivatives the four nests read,
+// each memory variable updated once on the union of the ranges of the nests
+// that read it (and stored at once), then the six stresses of the nests
+// whose range holds the cell.  vx's z derivative enters sigma_xy
+// uncorrected (1858-1865).
 template <typename T, int MAT, bool CTAB>
-__global__ void __launch_bounds__(3 * kShellLanes) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
+__global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args<T> A, const El3Tile *tiles)
 {
-    __shared__ T D[2][3 * K_NKIND * kShellLanes];
-    __shared__ double N[2][3 * kShellLanes];
     const El3Tile tl = tiles[(A.remap & 2) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x];
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int lane = threadIdx.x & (kShellLanes - 1);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kShellLanes);
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
-    const ShellCol L = shell_col(tl, A.box, lane, nx, ny, sy, A.xoff);
+    const ShellCell L = shell_cell(tl, A.box, nx, ny, sy, A.xoff);
     const int i = L.i, j = L.j;
-    auto nests = [&](int k, bool &L1, bool &L2, bool &L3, bool &L4) {
-        L1 = L.on && i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1;
-        L2 = L.on && i >= 3 && i <= nx - 1 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1;
-        L3 = L.on && i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2;
-        L4 = L.on && i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 2 && k <= nz - 2;
-    };
-    // z queue of this wavefront's velocity component
-    const T *vq = w == 0 ? A.v[0] : (w == 1 ? A.v[1] : A.v[2]);
-    T q[5];
+    const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
+    T q[3][5];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) q[s] = vq[L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
-    const int lsel = L.on ? lane : tl.alane;
-    T *const blk0 = shell_block(A, tl, lsel);
-    const int64_t bstride = (int64_t)A.snq * kShellLanes;
-    // phase 1 of plane k into buffer b
-    auto phase1 = [&](int k, int b) {
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) q[m][s] = A.v[m][L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
+    const int64_t ns = A.nslots;
+    double nrm = 0.0;
+    int64_t slot = tl.base + threadIdx.x;
+    for (int k = tl.k0; k <= tl.k1; ++k, slot += kShellThreads) {
         const int64_t c = L.col + sz * (k - 1);
         const int64_t zp2 = sz * (min(k + 2, nz) - k);
-        T *const blk = blk0 + (int64_t)(k - tl.k0) * bstride;
-        bool L1, L2, L3, L4;
-        nests(k, L1, L2, L3, L4);
-        const int64_t slot = tl.base + (int64_t)(k - tl.k0) * kShellLanes + lsel;
-        // coefficients read by all three wavefronts: cached loads
-        constexpr bool inside = MAT == kMatIso;
+        const bool L1 = L.on && i >= 2 && i <= nx - 2 && j >= 3 && j <= ny - 1 && k >= 3 && k <= nz - 1;
+        const bool L2 = L.on && i >= 3 && i <= nx - 1 && j >= 2 && j <= ny - 2 && k >= 3 && k <= nz - 1;
+        const bool L3 = L.on && i >= 3 && i <= nx - 1 && j >= 3 && j <= ny - 1 && k >= 2 && k <= nz - 2;
+        const bool L4 = L.on && i >= 2 && i <= nx - 2 && j >= 2 && j <= ny - 2 && k >= 2 && k <= nz - 2;
         T cf[6];
-        if (!inside) shell_coefs_t<T, CTAB>(A, blk, slot, cf);
-        if (w == 0) shell_derivs<T, 0, CTAB, inside>(A, L, q, c, zp2, blk, L1, L2, L3, L4, slot, cf, D[b], lane);
-        else if (w == 1) shell_derivs<T, 1, CTAB, inside>(A, L, q, c, zp2, blk, L1, L2, L3, L4, slot, cf, D[b], lane);
-        else shell_derivs<T, 2, CTAB, inside>(A, L, q, c, zp2, blk, L1, L2, L3, L4, slot, cf, D[b], lane);
-        q[0] = q[1]; q[1] = q[2]; q[2] = q[3]; q[3] = q[4];
-    };
-    phase1(tl.k0, 0);
-    __syncthreads();
-    double nrm = 0.0;
-    for (int k = tl.k0; k <= tl.k1; ++k) {
-        const int b = (k - tl.k0) & 1;
-        const int64_t c = L.col + sz * (k - 1);
-        // per-cell arrays: load the phase-2 operands ahead of phase 1 (their
-        // latency is exposed otherwise); with the table (scalar row loads)
-        // the kernel measured faster loading them after it
-        constexpr bool ahead = MAT != kMatTable;
-        ShellPre<T> P;
-        if (ahead) P = shell_pre<T, MAT>(A, c, w);
-        if (k < tl.k1) phase1(k + 1, b ^ 1);
-        if (!ahead) P = shell_pre<T, MAT>(A, c, w);
-        bool L1, L2, L3, L4;
-        nests(k, L1, L2, L3, L4);
-        double part;
-        if (w == 0) part = shell_stresses_at<T, MAT, 0>(A, c, D[b], lane, L1, L2, L3, L4, P);
-        else if (w == 1) part = shell_stresses_at<T, MAT, 1>(A, c, D[b], lane, L1, L2, L3, L4, P);
-        else part = shell_stresses_at<T, MAT, 2>(A, c, D[b], lane, L1, L2, L3, L4, P);
-        N[b][w * kShellLanes + lane] = part;
-        __syncthreads();
-        if (w == 0 && L.on) {
+        shell_coefs<T, CTAB>(A, slot, cf);
+        const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
+        T so[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) so[s] = ldnt(&A.s[s][c]);
+        const int mid = MAT == kMatTable ? (int)A.mid[c] : 0;
+        T ic11 = T(0), ic12 = T(0), ic44 = T(0);
+        if (MAT == kMatIso) { ic11 = ldnt(&A.C[0][c]); ic12 = ldnt(&A.C[1][c]); ic44 = ldnt(&A.C[15][c]); }
+        T *ps = A.shell + slot;
+        T D[3][K_NKIND];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const T *v = A.v[m];
+            q[m][4] = v[c + zp2];
+            const T v0 = q[m][2];
+            const T xm2 = v[c + L.xm2], xm1 = v[c + L.xm1], xp1 = v[c + L.xp1], xp2 = v[c + L.xp2];
+            const T ym2 = v[c + L.ym2], ym1 = v[c + L.ym1], yp1 = v[c + L.yp1], yp2 = v[c + L.yp2];
+            T pin[K_NKIND];
+#pragma unroll
+            for (int kd = 0; kd < K_NKIND; ++kd)
+                pin[kd] = (m == 0 && kd == K_ZMH) ? T(0) : ldnt(&ps[(m * K_NKIND + kd) * ns]);
+            const T dxp = dp4(xm1, v0, xp1, xp2, rx), dxm = dm4(xm2, xm1, v0, xp1, rx);
+            const T dyp = dp4(ym1, v0, yp1, yp2, ry), dym = dm4(ym2, ym1, v0, yp1, ry);
+            const T dzp = dp4(q[m][1], v0, q[m][3], q[m][4], rz), dzm = dm4(q[m][0], q[m][1], v0, q[m][3], rz);
+            T P[K_NKIND];
+            D[m][K_XPH] = cpml_upd(dxp, pin[K_XPH], ah, bh, ikh, P[K_XPH]);
+            D[m][K_XMF] = cpml_upd(dxm, pin[K_XMF], a, b, ik, P[K_XMF]);
+            D[m][K_YMF] = cpml_upd(dym, pin[K_YMF], a, b, ik, P[K_YMF]);
+            D[m][K_YPH] = cpml_upd(dyp, pin[K_YPH], ah, bh, ikh, P[K_YPH]);
+            D[m][K_ZMF] = cpml_upd(dzm, pin[K_ZMF], a, b, ik, P[K_ZMF]);
+            if (m == 0) { D[m][K_ZMH] = dzm; P[K_ZMH] = T(0); }
+            else D[m][K_ZMH] = cpml_upd(dzm, pin[K_ZMH], ah, bh, ikh, P[K_ZMH]);
+            D[m][K_ZPH] = cpml_upd(dzp, pin[K_ZPH], ah, bh, ikh, P[K_ZPH]);
+            if (L1 || L4) stnt(&ps[(m * K_NKIND + K_XPH) * ns], P[K_XPH]);
+            if (L2 || L3) stnt(&ps[(m * K_NKIND + K_XMF) * ns], P[K_XMF]);
+            if (L1 || L3) stnt(&ps[(m * K_NKIND + K_YMF) * ns], P[K_YMF]);
+            if (L2 || L4) stnt(&ps[(m * K_NKIND + K_YPH) * ns], P[K_YPH]);
+            if (L1) stnt(&ps[(m * K_NKIND + K_ZMF) * ns], P[K_ZMF]);
+            if (m != 0 && L2) stnt(&ps[(m * K_NKIND + K_ZMH) * ns], P[K_ZMH]);
+            if (L3 || L4) stnt(&ps[(m * K_NKIND + K_ZPH) * ns], P[K_ZPH]);
+            q[m][0] = q[m][1]; q[m][1] = q[m][2]; q[m][2] = q[m][3]; q[m][3] = q[m][4];
+        }
+        T sn[6];
+        if constexpr (MAT == kMatTable) {
+            const int m0 = __builtin_amdgcn_readfirstlane(mid);
+            if (__all(mid == m0))
+                shell_stress_update(ScalarRow<T>{(const_row<T>)(A.tab) + m0 * kTabRow}, D, so, dt, L1, L2, L3, L4, sn);
+            else
+                shell_stress_update(LaneRow<T>{A.tab + mid * kTabRow}, D, so, dt, L1, L2, L3, L4, sn);
+        } else if constexpr (MAT == kMatIso) {
+            shell_stress_update(IsoRow<T>{A.ig, c, A.ig_same, ic11, ic12, ic44}, D, so, dt, L1, L2, L3, L4, sn);
+        } else {
+            shell_stress_update(DenseRow<T>{A.C, A.ig, c, A.ig_same}, D, so, dt, L1, L2, L3, L4, sn);
+        }
+        if (L1) { stnt(&A.s[0][c], sn[0]); stnt(&A.s[1][c], sn[1]); stnt(&A.s[2][c], sn[2]); }
+        if (L2) stnt(&A.s[3][c], sn[3]);
+        if (L3) stnt(&A.s[4][c], sn[4]);
+        if (L4) stnt(&A.s[5][c], sn[5]);
+        if (L.on) {
             const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3] ||
-                             pw_face(A.pwf, L.i, L.j, k);
-            const double n = sqrt(N[b][lane] + N[b][kShellLanes + lane] + N[b][2 * kShellLanes + lane]);
+                             pw_face(A.pwf, i, j, k);
+            const double a0 = sn[0], a1 = sn[1], a2 = sn[2], a3 = sn[3], a4 = sn[5], a5 = sn[4];
+            const double n = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5));
             nrm = fix ? nrm : fmax_nan(nrm, n);
         }
     }
     block_max_to(&A.snorm[*A.d_it - 1], nrm);
 }
 
-// Velocity pass for component M (cpmlfdtd.f08:2014-2038 vx, 2055-2080 vy,
-// 2098-2127 vz): three stress derivatives, their memory variables, the
-// averaged densities; Dirichlet faces (2226-2248) give 0.  Returns the new
-// value (the old one outside the nest's range).
-template <typename T, bool CORR, bool TAB, int M>
-__device__ __forceinline__ T shell_velocity(const El3Args<T> &A, const ShellCol &L, T (&qs)[5], T (&qr)[3], int64_t c,
-                                            T *blk, int k, const T (&cf)[6], T rx_nb)
+// Velocity pass (cpmlfdtd.f08:2014-2038 vx, 2055-2080 vy, 2098-2127 vz): the
+// three components of the cell, each from three stress differences with
+// their memory variables and the averaged densities; Dirichlet faces
+// (2226-2248) give 0.
+template <typename T, bool CORR, bool TAB, bool CTAB>
+__global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Args<T> A, const El3Tile *tiles)
 {
-    const int nx = A.nx, ny = A.ny, nz = A.nz, i = L.i, j = L.j;
+    const El3Tile tl = tiles[(A.remap & 2) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x];
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
+    const ShellCell L = shell_cell(tl, A.box, nx, ny, sy, A.xoff);
+    const int i = L.i, j = L.j;
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
     const int dc = A.density_code;
-    const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
-    const bool face = i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz;
-    const T r = qr[1];
-    const T ryy = scalar_mean(r, r, dc);
-    T v = ldnt(&A.v[M][c]);
-    T p0, p1, p2;
-    T d1, d2, d3;
-    T *m1, *m2, *m3;
-    T r1, r3;
-    bool in;
-    if (M == 0) {
-        const T *sxx = A.s[0], *sxy = A.s[3];
-        d1 = tminus5(sxx[c + L.xm2], sxx[c + L.xm1], sxx[c], CORR ? sxx[c + L.xp1] : T(0), CORR) * rx;
-        d2 = tminus5(sxy[c + L.ym2], sxy[c + L.ym1], sxy[c], CORR ? sxy[c + L.yp1] : T(0), CORR) * ry;
-        d3 = tminus5(qs[0], qs[1], qs[2], qs[3], CORR) * rz;
-        m1 = blk + M_SXX_X * kShellLanes; m2 = blk + M_SXY_Y * kShellLanes; m3 = blk + M_SXZ_Z * kShellLanes;
-        d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
-        d2 = cpml_upd(d2, m2, ah, bh, ikh, p1);
-        d3 = cpml_upd(d3, m3, ah, bh, ikh, p2);
-        r1 = scalar_mean(r, rx_nb, dc);   // x neighbour i-1
-        r3 = scalar_mean(r, qr[0], dc);
-        in = i >= 3 && j >= 3 && k >= 3;
-    } else if (M == 1) {
-        const T *sxy = A.s[3], *syy = A.s[1];
-        d1 = tplus5(CORR ? sxy[c + L.xm1] : T(0), sxy[c], sxy[c + L.xp1], sxy[c + L.xp2], CORR) * rx;
-        d2 = tplus5(CORR ? syy[c + L.ym1] : T(0), syy[c], syy[c + L.yp1], syy[c + L.yp2], CORR) * ry;
-        d3 = tminus5(qs[0], qs[1], qs[2], qs[3], CORR) * rz;
-        m1 = blk + M_SXY_X * kShellLanes; m2 = blk + M_SYY_Y * kShellLanes; m3 = blk + M_SYZ_Z * kShellLanes;
-        d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
-        d2 = cpml_upd(d2, m2, ah, bh, ikh, p1);
-        d3 = cpml_upd(d3, m3, a, b, ik, p2);
-        r1 = scalar_mean(r, rx_nb, dc);   // x neighbour i+1
-        r3 = scalar_mean(r, qr[0], dc);
-        in = i <= nx - 2 && j <= ny - 2 && k >= 3;
-    } else {
-        const T *sxz = A.s[4], *syz = A.s[5];
-        d1 = tplus5(CORR ? sxz[c + L.xm1] : T(0), sxz[c], sxz[c + L.xp1], sxz[c + L.xp2], CORR) * rx;
-        d2 = tminus5(syz[c + L.ym2], syz[c + L.ym1], syz[c], CORR ? syz[c + L.yp1] : T(0), CORR) * ry;
-        d3 = tplus5(qs[1], qs[2], qs[3], qs[4], CORR) * rz;
-        m1 = blk + M_SXZ_X * kShellLanes; m2 = blk + M_SYZ_Y * kShellLanes; m3 = blk + M_SZZ_Z * kShellLanes;
-        d1 = cpml_upd(d1, m1, ah, bh, ikh, p0);
-        d2 = cpml_upd(d2, m2, a, b, ik, p1);
-        d3 = cpml_upd(d3, m3, ah, bh, ikh, p2);
-        r1 = scalar_mean(r, rx_nb, dc);   // x neighbour i+1
-        r3 = scalar_mean(r, qr[2], dc);
-        in = i <= nx - 2 && j >= 3 && k <= nz - 2;
-    }
-    const T n = v + (d1 / r1 + d2 / ryy + d3 / r3) * dt;
-    if (L.on && !face && in) {
-        stnt(m1, p0); stnt(m2, p1); stnt(m3, p2);
-        v = n;
-    }
-    if (face) v = T(0);
-    if (L.on) stnt(&A.v[M][c], v);
-    return v;
-}
-
-// One barrier per plane: the norm of a plane needs all three components, so
-// each wavefront leaves v^2 in LDS (double-buffered) for wavefront 0.
-template <typename T, bool CORR, bool TAB>
-__global__ void __launch_bounds__(3 * kShellLanes) el3_shell_velocity_kernel(El3Args<T> A, const El3Tile *tiles)
-{
-    __shared__ double V2[2][3 * kShellLanes];
-    const El3Tile tl = tiles[(A.remap & 2) ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x];
-    const int nz = A.nz;
-    const int lane = threadIdx.x & (kShellLanes - 1);
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kShellLanes);
-    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * A.ny;
-    const ShellCol L = shell_col(tl, A.box, lane, A.nx, A.ny, sy, A.xoff);
-    // z queue: sigma_xz (vx), sigma_yz (vy), sigma_zz (vz), planes k-2..k+2
-    const T *sq = w == 0 ? A.s[4] : (w == 1 ? A.s[5] : A.s[2]);
-    T qs[5], qr[3];
+    const T *sxx = A.s[0], *syy = A.s[1], *szz = A.s[2], *sxy = A.s[3], *sxz = A.s[4], *syz = A.s[5];
+    auto zc = [&](int k) { return sz * (min(max(k, 1), nz) - 1); };
+    T qxz[5], qyz[5], qzz[5], qr[3];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) qs[s] = sq[L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
-    qr[0] = rho_at<T, TAB>(A, L.col + sz * (max(tl.k0 - 1, 1) - 1));
-    qr[1] = rho_at<T, TAB>(A, L.col + sz * (tl.k0 - 1));
+    for (int s = 0; s < 4; ++s) {
+        qxz[s] = sxz[L.col + zc(tl.k0 - 2 + s)];
+        qyz[s] = syz[L.col + zc(tl.k0 - 2 + s)];
+        qzz[s] = szz[L.col + zc(tl.k0 - 2 + s)];
+    }
+    qr[0] = rho_at<T, TAB>(A, L.col + zc(tl.k0 - 1));
+    qr[1] = rho_at<T, TAB>(A, L.col + zc(tl.k0));
+    RhoPre<T> rz_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0 + 1)),
+              rxm_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0) + L.xm1),
+              rxp_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0) + L.xp1);
+    const int64_t ns = A.nslots;
     double nrm = 0.0;
-    const int lsel = L.on ? lane : tl.alane;
-    T *blk = shell_block(A, tl, lsel);
-    int64_t slot = tl.base + lsel;
-    // x neighbour of the x-face density mean: i-1 for vx, i+1 for vy and vz
-    const int64_t xo = w == 0 ? L.xm1 : L.xp1;
-    // per-lane loads of plane k, issued during plane k-1 (after its own loads)
-    auto zp1_of = [&](int k) { return sz * (min(k + 1, nz) - k); };
-    auto zp2_of = [&](int k) { return sz * (min(k + 2, nz) - k); };
-    const int64_t c0 = L.col + sz * (tl.k0 - 1);
-    T qs_next = sq[c0 + zp2_of(tl.k0)];
-    RhoPre<T> rz_next = rho_pre<T, TAB>(A, c0 + zp1_of(tl.k0)), rx_next = rho_pre<T, TAB>(A, c0 + xo);
-    int cid_next = coef_id(A, slot);
-    for (int k = tl.k0; k <= tl.k1; ++k, blk += (int64_t)A.snq * kShellLanes, slot += kShellLanes) {
-        const int b = (k - tl.k0) & 1;
+    int64_t slot = tl.base + threadIdx.x;
+    for (int k = tl.k0; k <= tl.k1; ++k, slot += kShellThreads) {
         const int64_t c = L.col + sz * (k - 1);
-        qs[4] = qs_next;
+        qxz[4] = sxz[L.col + zc(k + 2)];
+        qyz[4] = syz[L.col + zc(k + 2)];
+        qzz[4] = szz[L.col + zc(k + 2)];
         qr[2] = rho_get<T, TAB>(A, rz_next);
-        const T rx_nb = rho_get<T, TAB>(A, rx_next);
-        // read by all three wavefronts: cached loads
+        const T r = qr[1], rxm = rho_get<T, TAB>(A, rxm_next), rxp = rho_get<T, TAB>(A, rxp_next);
+        const T sxx_m2 = sxx[c + L.xm2], sxx_m1 = sxx[c + L.xm1], sxx_0 = sxx[c], sxx_p1 = CORR ? sxx[c + L.xp1] : T(0);
+        const T sxy_ym2 = sxy[c + L.ym2], sxy_ym1 = sxy[c + L.ym1], sxy_0 = sxy[c],
+                sxy_yp1 = CORR ? sxy[c + L.yp1] : T(0);
+        const T sxy_xm1 = CORR ? sxy[c + L.xm1] : T(0), sxy_xp1 = sxy[c + L.xp1], sxy_xp2 = sxy[c + L.xp2];
+        const T syy_m1 = CORR ? syy[c + L.ym1] : T(0), syy_0 = syy[c], syy_p1 = syy[c + L.yp1],
+                syy_p2 = syy[c + L.yp2];
+        const T sxz_xm1 = CORR ? sxz[c + L.xm1] : T(0), sxz_xp1 = sxz[c + L.xp1], sxz_xp2 = sxz[c + L.xp2];
+        const T syz_ym2 = syz[c + L.ym2], syz_ym1 = syz[c + L.ym1], syz_yp1 = CORR ? syz[c + L.yp1] : T(0);
         T cf[6];
-        shell_coefs_id(A, blk, cid_next, cf);
-        T v;
-        if (w == 0) v = shell_velocity<T, CORR, TAB, 0>(A, L, qs, qr, c, blk, k, cf, rx_nb);
-        else if (w == 1) v = shell_velocity<T, CORR, TAB, 1>(A, L, qs, qr, c, blk, k, cf, rx_nb);
-        else v = shell_velocity<T, CORR, TAB, 2>(A, L, qs, qr, c, blk, k, cf, rx_nb);
-        if (k < tl.k1) {
-            const int64_t cn = c + sz;
-            qs_next = sq[cn + zp2_of(k + 1)];
-            rz_next = rho_pre<T, TAB>(A, cn + zp1_of(k + 1));
-            rx_next = rho_pre<T, TAB>(A, cn + xo);
-            cid_next = coef_id(A, slot + kShellLanes);
+        shell_coefs<T, CTAB>(A, slot, cf);
+        const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
+        T *ps = A.shell + slot;
+        T pin[9];
+#pragma unroll
+        for (int s = 0; s < 9; ++s) pin[s] = ldnt(&ps[(M_SXX_X + s) * ns]);
+        T vx = ldnt(&A.v[0][c]), vy = ldnt(&A.v[1][c]), vz = ldnt(&A.v[2][c]);
+        const bool face = i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz;
+        const T ryy = scalar_mean(r, r, dc);
+        T p[9];
+        T nvx, nvy, nvz;
+        {   // vx: T-x sigma_xx, T-y sigma_xy, T-z sigma_xz, all half-set recursions
+            const T d1 = cpml_upd(tminus5(sxx_m2, sxx_m1, sxx_0, sxx_p1, CORR) * rx, pin[0], ah, bh, ikh, p[0]);
+            const T d2 = cpml_upd(tminus5(sxy_ym2, sxy_ym1, sxy_0, sxy_yp1, CORR) * ry, pin[1], ah, bh, ikh, p[1]);
+            const T d3 = cpml_upd(tminus5(qxz[0], qxz[1], qxz[2], qxz[3], CORR) * rz, pin[2], ah, bh, ikh, p[2]);
+            nvx = vx + (d1 / scalar_mean(r, rxm, dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
         }
-        const double dv = v;
-        V2[b][w * kShellLanes + lane] = dv * dv;
-        __syncthreads();
-        if (w == 0 && L.on) {
-            const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, L.i, L.j, k);
-            const double n = sqrt(V2[b][lane] + V2[b][kShellLanes + lane] + V2[b][2 * kShellLanes + lane]);
-            nrm = fix ? nrm : fmax_nan(nrm, n);
+        {   // vy: T+x sigma_xy (half), T+y sigma_yy (half), T-z sigma_yz (full)
+            const T d1 = cpml_upd(tplus5(sxy_xm1, sxy_0, sxy_xp1, sxy_xp2, CORR) * rx, pin[3], ah, bh, ikh, p[3]);
+            const T d2 = cpml_upd(tplus5(syy_m1, syy_0, syy_p1, syy_p2, CORR) * ry, pin[4], ah, bh, ikh, p[4]);
+            const T d3 = cpml_upd(tminus5(qyz[0], qyz[1], qyz[2], qyz[3], CORR) * rz, pin[5], a, b, ik, p[5]);
+            nvy = vy + (d1 / scalar_mean(r, rxp, dc) + d2 / ryy + d3 / scalar_mean(r, qr[0], dc)) * dt;
+        }
+        {   // vz: T+x sigma_xz (half), T-y sigma_yz (full), T+z sigma_zz (half)
+            const T d1 = cpml_upd(tplus5(sxz_xm1, qxz[2], sxz_xp1, sxz_xp2, CORR) * rx, pin[6], ah, bh, ikh, p[6]);
+            const T d2 = cpml_upd(tminus5(syz_ym2, syz_ym1, qyz[2], syz_yp1, CORR) * ry, pin[7], a, b, ik, p[7]);
+            const T d3 = cpml_upd(tplus5(qzz[1], qzz[2], qzz[3], qzz[4], CORR) * rz, pin[8], ah, bh, ikh, p[8]);
+            nvz = vz + (d1 / scalar_mean(r, rxp, dc) + d2 / ryy + d3 / scalar_mean(r, qr[2], dc)) * dt;
+        }
+        const bool inx = L.on && !face && i >= 3 && j >= 3 && k >= 3;
+        const bool iny = L.on && !face && i <= nx - 2 && j <= ny - 2 && k >= 3;
+        const bool inz = L.on && !face && i <= nx - 2 && j >= 3 && k <= nz - 2;
+        if (inx) { stnt(&ps[M_SXX_X * ns], p[0]); stnt(&ps[M_SXY_Y * ns], p[1]); stnt(&ps[M_SXZ_Z * ns], p[2]); vx = nvx; }
+        if (iny) { stnt(&ps[M_SXY_X * ns], p[3]); stnt(&ps[M_SYY_Y * ns], p[4]); stnt(&ps[M_SYZ_Z * ns], p[5]); vy = nvy; }
+        if (inz) { stnt(&ps[M_SXZ_X * ns], p[6]); stnt(&ps[M_SYZ_Y * ns], p[7]); stnt(&ps[M_SZZ_Z * ns], p[8]); vz = nvz; }
+        if (face) { vx = T(0); vy = T(0); vz = T(0); }
+        if (L.on) {
+            stnt(&A.v[0][c], vx); stnt(&A.v[1][c], vy); stnt(&A.v[2][c], vz);
+            const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, j, k);
+            const double a0 = vx, a1 = vy, a2 = vz;
+            nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
+        }
+        if (k < tl.k1) {
+            rz_next = rho_pre<T, TAB>(A, L.col + zc(k + 2));
+            rxm_next = rho_pre<T, TAB>(A, c + sz + L.xm1);
+            rxp_next = rho_pre<T, TAB>(A, c + sz + L.xp1);
         }
 #pragma unroll
-        for (int s = 0; s < 4; ++s) qs[s] = qs[s + 1];
+        for (int s = 0; s < 4; ++s) {
+            qxz[s] = qxz[s + 1]; qyz[s] = qyz[s + 1]; qzz[s] = qzz[s + 1];
+        }
         qr[0] = qr[1]; qr[1] = qr[2];
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
@@ -873,10 +709,10 @@ static void fast_stress(Device &d, const El3Args<T> &A)
     const hipStream_t ss = shell_stream_begin(d, 0);
     if (d.ntiles > 0) {
         if (A.ctab)
-            hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT, true>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss,
+            hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT, true>), dim3(d.ntiles), dim3(kShellThreads), 0, ss,
                                A, (const El3Tile *)d.tiles);
         else
-            hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT, false>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss,
+            hipLaunchKernelGGL((el3_shell_stress_kernel<T, MAT, false>), dim3(d.ntiles), dim3(kShellThreads), 0, ss,
                                A, (const El3Tile *)d.tiles);
     }
     if (!box_empty(d)) {
@@ -892,9 +728,14 @@ static void fast_velocity(Device &d, const El3Args<T> &A)
 {
     const int kc = el3_kc(d), ty = el3_ty(d);
     const hipStream_t ss = shell_stream_begin(d, 1);
-    if (d.ntiles > 0)
-        hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB>), dim3(d.ntiles), dim3(3 * kShellLanes), 0, ss,
-                           A, (const El3Tile *)d.tiles);
+    if (d.ntiles > 0) {
+        if (A.ctab)
+            hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB, true>), dim3(d.ntiles), dim3(kShellThreads), 0,
+                               ss, A, (const El3Tile *)d.tiles);
+        else
+            hipLaunchKernelGGL((el3_shell_velocity_kernel<T, CORR, TAB, false>), dim3(d.ntiles), dim3(kShellThreads),
+                               0, ss, A, (const El3Tile *)d.tiles);
+    }
     if (!box_empty(d)) {
         if (ty == 16) int_velocity<T, CORR, 16, TAB>(d, A, kc);
         else if (ty == 8) int_velocity<T, CORR, 8, TAB>(d, A, kc);
