@@ -251,6 +251,12 @@ enum {
                                    other entries 0), keep only c11, c12, c44 and read 3 arrays
                                    per cell instead of 21 (default; same arithmetic, results
                                    equal to the general path), 0 = always all 21 arrays      */
+    SDT_OPT_FUSED = 11,         /* 3D EM: 1 = the H and E updates of a step run as one kernel
+                                   (E, H and the H-update memory variables are kept in two
+                                   buffers each, the current one swapped every step; a third
+                                   fewer bytes per step), 0 = one kernel per half step, fields
+                                   updated in place, -1 = fused for FP64, not for FP32
+                                   (default: the faster of the two for each precision)      */
     SDT_OPT_MATERIAL_TABLE = 5  /* every solver: 1 = when the model holds at most 256
                                    distinct material tuples (3D elastic: (c11..c66, gamma_*,
                                    rho); 2.5D and 2D elastic: (c11..c66, gamma_*), the density
@@ -291,7 +297,8 @@ enum {
     SDT_INFO_SHELL_TILES = 2,   /* 3D elastic CPML-shell tiles                                    */
     SDT_INFO_DEVICE_BYTES = 3,  /* device memory held by the handle's grid arrays                 */
     SDT_INFO_COEFFICIENTS = 4,  /* update coefficients per cell (2D EM 7, 2D elastic 10, else 0)  */
-    SDT_INFO_ISOTROPIC = 5      /* 1 = the isotropic arrays (c11, c12, c44) are in use (3D, 2.5D elastic) */
+    SDT_INFO_ISOTROPIC = 5,     /* 1 = the isotropic arrays (c11, c12, c44) are in use (3D, 2.5D elastic) */
+    SDT_INFO_FUSED = 6          /* 1 = the fused 3D EM step is in use                      */
 };
 int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 

@@ -30,7 +30,7 @@ struct sdt_handle {
     bool failed = false;
     int steps_done = 0;
     std::string err;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[2] = {};      // captured steps, per buffer parity (Device::parity)
     int graph_steps = 0;
     std::vector<hipEvent_t> tev;       // timing events, 3 per timed step
     const sdt_batch *owner = nullptr;  // the batch this handle belongs to (until sdt_batch_destroy)
@@ -165,7 +165,8 @@ static void release(sdt_handle *h)
     Device &d = h->d;
     (void)hipSetDevice(h->p.device);
     if (d.stream) (void)hipStreamSynchronize(d.stream);
-    if (h->graph) (void)hipGraphExecDestroy(h->graph);
+    for (auto g : h->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     auto drop = [](void *p) { if (p) (void)hipFree(p); };
     free_grids(d);   // fields, memory variables, materials, CPML coefficients
     drop(d.tile_flags);
@@ -352,15 +353,24 @@ static void enqueue_step(sdt_handle *h, bool timed)
 // replay.
 static constexpr int kGraphSteps = 32;
 
+static void drop_graphs(sdt_handle *h)
+{
+    for (auto &g : h->graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
 static int ensure_graph(sdt_handle *h)
 {
-    if (h->graph) return SDT_OK;
     Device &d = h->d;
+    if (h->graph[d.parity]) return SDT_OK;
+    const int par = d.parity;
     hipGraph_t g = nullptr;
     HIPCHK(h, hipStreamBeginCapture(d.stream, hipStreamCaptureModeThreadLocal));
     for (int s = 0; s < kGraphSteps; ++s) enqueue_step(h, false);
     HIPCHK(h, hipStreamEndCapture(d.stream, &g));
-    hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&h->graph[par], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
     h->graph_steps = kGraphSteps;
@@ -452,7 +462,7 @@ int32_t sdt_run(sdt_handle *h, int32_t nsteps, int32_t *failed_step)
             if (use_graph && n >= kGraphSteps) {
                 rc = ensure_graph(h);
                 if (rc) return rc;
-                for (; s + kGraphSteps <= n; s += kGraphSteps) HIPCHK(h, hipGraphLaunch(h->graph, d.stream));
+                for (; s + kGraphSteps <= n; s += kGraphSteps) HIPCHK(h, hipGraphLaunch(h->graph[d.parity], d.stream));
             }
             for (; s < n; ++s) enqueue_step(h, false);
         }
@@ -691,23 +701,22 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
         return SDT_OK;
     case SDT_OPT_CONCURRENT:
         h->d.concurrent = value != 0;
-        if (h->graph) {   // captured steps follow the old setting: capture again
-            (void)hipGraphExecDestroy(h->graph);
-            h->graph = nullptr;
-        }
+        drop_graphs(h);   // captured steps follow the old setting: capture again
         return SDT_OK;
     case SDT_OPT_XCD_REMAP:
         if (value < 0 || value > 3) return fail(h, SDT_ERR_ARG, "XCD remap option must be 0..3");
         h->d.xcd_remap = (int)value;
-        if (h->graph) {
-            (void)hipGraphExecDestroy(h->graph);
-            h->graph = nullptr;
-        }
+        drop_graphs(h);
         return SDT_OK;
     case SDT_OPT_MATERIAL_TABLE:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "material layout is fixed once the first step has run");
         if (value != 0 && value != 1) return fail(h, SDT_ERR_ARG, "material table option must be 0 or 1");
         h->d.mat_table_opt = (int)value;
+        return SDT_OK;
+    case SDT_OPT_FUSED:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "the step's kernels are fixed once the first step has run");
+        if (value < -1 || value > 1) return fail(h, SDT_ERR_ARG, "fused option must be -1, 0 or 1");
+        h->d.fused_opt = (int)value;
         return SDT_OK;
     case SDT_OPT_ISOTROPIC:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "material layout is fixed once the first step has run");
@@ -741,6 +750,7 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
     case SDT_INFO_DEVICE_BYTES: *value = (int64_t)h->d.grid_bytes; return SDT_OK;
     case SDT_INFO_COEFFICIENTS: *value = h->d.adj.ncoef; return SDT_OK;
     case SDT_INFO_ISOTROPIC: *value = h->d.mat_iso ? 1 : 0; return SDT_OK;
+    case SDT_INFO_FUSED: *value = h->d.fused ? 1 : 0; return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown info key");
 }

@@ -12,6 +12,7 @@
 namespace sdt {
 
 constexpr int kMaxMem = 32;     // memory-variable arrays per solver (3D elastic: 29)
+constexpr int kAltMem = 16;     // 3D EM fused step: second buffers of mem[0..5] start here
 constexpr int kMaxMat = 32;     // device material arrays per solver (3D elastic: 28)
 constexpr double kEps0 = 8.85418782e-12;                  // constants.f08:10
 constexpr double kMu0 = 4.0 * 3.141592653589793238462643 * 1.0e-7;  // constants.f08:9
@@ -156,6 +157,13 @@ struct Device {
     int64_t ncell_dev = 0;
 
     void *field[SDT_NUM_FIELDS] = {};
+    // 3D EM with the fused step: the other buffer of each of E and H (the
+    // step reads field[] and writes alt[], then the two are swapped), and of
+    // the H-update memory variables mem[0..5] in mem[kAltMem + 0..5]
+    void *alt[SDT_NUM_FIELDS] = {};
+    int parity = 0;                    // buffer swaps so far, mod 2
+    int fused_opt = -1;                // SDT_OPT_FUSED (-1 auto)
+    bool fused = false;                // fused 3D EM step in use
     void *mem[kMaxMem] = {};
     int nmem = 0;
     void *mat[kMaxMat] = {};

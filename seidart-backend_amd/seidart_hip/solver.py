@@ -275,21 +275,24 @@ class Solver:
         'checkpoint' (2D: steps between forward states saved for adjoint())
         or 'raw_source' (1 = source series injected as given) or 'isotropic'
         (3D elastic without a material table: 1 = hold only c11, c12, c44 when
-        every cell's stiffness has the isotropic form, 0 = all 21 arrays)."""
+        every cell's stiffness has the isotropic form, 0 = all 21 arrays) or
+        'fused' (3D EM: 1 = one kernel per step, 0 = one per half step, -1 =
+        fused for FP64 only, the default)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
              "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
              "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP,
              "checkpoint": L.OPT_CHECKPOINT, "raw_source": L.OPT_RAW_SOURCE,
-             "isotropic": L.OPT_ISOTROPIC}[key]
+             "isotropic": L.OPT_ISOTROPIC, "fused": L.OPT_FUSED}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     def info(self, key):
         """'materials' (distinct materials in the table, 0 = per-cell arrays),
-        'shell_tiles', 'device_bytes', 'coefficients' or 'isotropic' (1 = the
-        3D elastic isotropic arrays are in use); valid after the first run()."""
+        'shell_tiles', 'device_bytes', 'coefficients', 'isotropic' (1 = the
+        3D elastic isotropic arrays are in use) or 'fused' (1 = the fused 3D
+        EM step is in use); valid after the first run()."""
         k = {"materials": L.INFO_MATERIALS, "shell_tiles": L.INFO_SHELL_TILES,
              "device_bytes": L.INFO_DEVICE_BYTES, "coefficients": L.INFO_COEFFICIENTS,
-             "isotropic": L.INFO_ISOTROPIC}[key]
+             "isotropic": L.INFO_ISOTROPIC, "fused": L.INFO_FUSED}[key]
         v = C.c_int64(0)
         L.check(L.lib.sdt_get_info(self._h, k, C.byref(v)), self._h)
         return v.value

@@ -247,7 +247,8 @@ def test_em2_material_table_equals_per_cell_arrays():
         assert rel_l2(f1[k], f0[k]) < 1e-13, k
 
 
-@pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"zchunk": 5}, {"zchunk": 1, "material_table": 0}])
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"zchunk": 5}, {"zchunk": 1, "material_table": 0},
+                                     {"fused": 0}, {"fused": 0, "material_table": 0}, {"tile_rows": 4, "zchunk": 7}])
 def test_em3_fp64(options):
     from seidart_hip import models
     cfg = models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)
@@ -265,6 +266,31 @@ def test_em3_odd_plane_count(options):
     cfg = models.em3(nx=34, ny=30, nz=27, cpml=6, steps=150, src=(17, 15, 9), nrec=8)
     g, o = _run_both(cfg, ("ex", "ey", "ez", "hx", "hy", "hz"), options=options)
     _check(g, o, TOL64)
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_em3_runs_split_across_graph_replays(fused):
+    """The fused step swaps E / H buffers every step and replays captured
+    32-step graphs per buffer parity: a run split into pieces of odd and
+    even lengths (direct steps and replays at both parities) must equal the
+    oracle's uninterrupted run, in FP64 and in FP32 (against FP64 to 1e-3)."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    from oracle.oracle_ctypes import run_oracle
+    cfg = models.em3(nx=30, ny=28, nz=24, cpml=5, steps=150, src=(15, 14, 9), nrec=6)
+    o_rec, (o_vn, _), o_f, _ = run_oracle(cfg, fields=("ex", "hz"))
+    for precision, tol in (("fp64", TOL64), ("fp32", 1e-3)):
+        s = make_solver(cfg, precision=precision, options={"fused": fused})
+        try:
+            for n in (5, 40, 3, 37, 33, 32):
+                s.run(n)
+            rec, (vn, _) = s.receivers(), s.norms()
+            errs = [rel_l2(rec, o_rec), rel_l2(vn, o_vn), rel_l2(s.field("ex"), o_f["ex"]),
+                    rel_l2(s.field("hz"), o_f["hz"])]
+            print(precision, [f"{e:.2e}" for e in errs], flush=True)
+            assert max(errs) <= tol
+        finally:
+            s.close()
 
 
 def test_em3_heterogeneous_permittivity():
