@@ -109,17 +109,7 @@ int el3_setup(Device &d, const double *const *in)
     d.tile_z = 1;
     int rc = compute_tile_flags(d, in);
     if (rc) return rc;
-    // The branch-free interior kernels need every nest active on the box
-    // (cpmlfdtd.f08:1761-2127 ranges intersect to [3, n-2] per axis) and no
-    // CPML cell inside it; otherwise every cell takes the shell kernels.
-    const int lim[3] = {d.nx, d.ny, d.nz};
-    for (int a = 0; a < 3; ++a) {
-        d.box[2 * a] = d.box[2 * a] < 3 ? 3 : d.box[2 * a];
-        d.box[2 * a + 1] = d.box[2 * a + 1] > lim[a] - 2 ? lim[a] - 2 : d.box[2 * a + 1];
-    }
-    const bool empty = d.nflagged > 0 || d.box[1] < d.box[0] || d.box[3] < d.box[2] || d.box[5] < d.box[4];
-    if (empty)
-        for (int a = 0; a < 3; ++a) { d.box[2 * a] = 1; d.box[2 * a + 1] = 0; }
+    host::el3_interior_box(d.box, d.nflagged, d.nx, d.ny, d.nz);
     set_layout(d);
 
     const int64_t n = d.ncell;
@@ -218,62 +208,10 @@ int el3_setup(Device &d, const double *const *in)
 // then columns, so that consecutive tiles share stencil halos.
 int el3_build_shell(Device &d, const double *const *in, int kc)
 {
-    std::vector<El3Tile> tiles;
-    const int *b = d.box;
-    const bool empty = b[1] < b[0];
-    const int line = (int)(128 / d.esize);
-    // x origin of line-aligned tiles: cell i sits at row element xoff + i - 1
-    const int xa = 1 - (int)d.xoff;
-    int64_t base = 0;
-    auto region = [&](int kind, int i, int j) {
-        if (i < 1 || i > d.nx || j < 1 || j > d.ny) return false;
-        if (kind == 0) return true;
-        if (kind == 1) return j < b[2] || j > b[3];
-        return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
-    };
-    // a tile is listed when one of its cells lies in its region
-    auto add = [&](int x0, int y0, int k0, int k1, int kind, int wide) {
-        const int tw = wide ? 64 : 16, th = kShellThreads / tw;
-        for (int ty = 0; ty < th; ++ty)
-            for (int tx = 0; tx < tw; ++tx)
-                if (region(kind, x0 + tx, y0 + ty)) {
-                    tiles.push_back(El3Tile{x0, y0, k0, k1, kind, wide, base});
-                    base += (int64_t)(k1 - k0 + 1) * kShellThreads;
-                    return;
-                }
-    };
-    auto zslab = [&](int ka, int kb) {
-        for (int k0 = ka; k0 <= kb; k0 += kc) {
-            const int k1 = k0 + kc - 1 < kb ? k0 + kc - 1 : kb;
-            for (int y0 = 1; y0 <= d.ny; y0 += 4)
-                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 0, 1);
-        }
-    };
-    // 16-wide tiles aligned to the line grid that cover columns [lo, hi]
-    auto xstrip = [&](int lo, int hi, int y0, int k0, int k1) {
-        if (lo > hi) return;
-        int x0 = xa + ((lo - xa) / line) * line;
-        for (; x0 <= hi; x0 += 16) add(x0, y0, k0, k1, 2, 0);
-    };
-    if (empty) {
-        zslab(1, d.nz);
-    } else {
-        zslab(1, b[4] - 1);
-        for (int k0 = b[4]; k0 <= b[5]; k0 += kc) {
-            const int k1 = k0 + kc - 1 < b[5] ? k0 + kc - 1 : b[5];
-            for (int y0 = 1; y0 < b[2]; y0 += 4)
-                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 1);
-            for (int y0 = b[2]; y0 <= b[3]; y0 += 16) {
-                xstrip(1, b[0] - 1, y0, k0, k1);
-                xstrip(b[1] + 1, d.nx, y0, k0, k1);
-            }
-            for (int y0 = b[3] + 1; y0 <= d.ny; y0 += 4)
-                for (int x0 = xa; x0 <= d.nx; x0 += 64) add(x0, y0, k0, k1, 1, 1);
-        }
-        zslab(b[5] + 1, d.nz);
-    }
+    host::ShellTiles st = host::shell_tiles(d.nx, d.ny, d.nz, d.box, d.xoff, (int)d.esize, kc);
+    const std::vector<El3Tile> &tiles = st.tiles;
     d.ntiles = (int)tiles.size();
-    d.nslots = base;
+    d.nslots = st.nslots;
     if (d.ntiles == 0) return SDT_OK;
     if (hipMalloc(&d.tiles, tiles.size() * sizeof(El3Tile)) != hipSuccess) return SDT_ERR_HIP;
     if (hipMemcpy(d.tiles, tiles.data(), tiles.size() * sizeof(El3Tile), hipMemcpyHostToDevice) != hipSuccess)
@@ -289,59 +227,28 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
             v = src[(int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1))];
         return inverse ? 1.0 / v : v;
     };
-    auto for_slots = [&](const std::function<void(int64_t, int, int, int)> &f) {
-        for (const El3Tile &t : tiles) {
-            const int tw = t.wide ? 64 : 16;
-            for (int k = t.k0; k <= t.k1; ++k)
-                for (int th = 0; th < kShellThreads; ++th)
-                    f(t.base + (int64_t)(k - t.k0) * kShellThreads + th, t.x0 + th % tw, t.y0 + th / tw, k);
-        }
-    };
+    // slot -> cell, for the dictionary's fill
+    std::vector<int> cell((size_t)d.nslots * 3);
+    host::for_shell_slots(tiles, [&](int64_t slot, int i, int j, int k) {
+        cell[(size_t)slot * 3] = i;
+        cell[(size_t)slot * 3 + 1] = j;
+        cell[(size_t)slot * 3 + 2] = k;
+    });
     // Coefficient dictionary: CPML profiles give few distinct tuples (tens
     // for the usual per-axis profiles), so the shell keeps a one-byte id per
-    // slot and a small table when there are at most 256 of them.
-    struct Tup {
-        double v[8];
-        bool operator==(const Tup &o) const { return memcmp(v, o.v, sizeof(v)) == 0; }
-    };
-    std::vector<Tup> tups;
-    std::vector<uint8_t> ids((size_t)d.nslots, 0);
-    bool dict = true;
-    {
-        std::unordered_multimap<uint64_t, int> index;
-        for_slots([&](int64_t slot, int i, int j, int k) {
-            if (!dict) return;
-            Tup u;
-            memset(&u, 0, sizeof(u));
-            for (int q = 0; q < 6; ++q) u.v[q] = coef(q, i, j, k);
-            uint64_t h = 1469598103934665603ull;
-            for (int q = 0; q < 6; ++q) {
-                uint64_t w;
-                memcpy(&w, &u.v[q], 8);
-                h = (h ^ w) * 1099511628211ull;
-            }
-            int id = -1;
-            auto range = index.equal_range(h);
-            for (auto it = range.first; it != range.second; ++it)
-                if (tups[(size_t)it->second] == u) { id = it->second; break; }
-            if (id < 0) {
-                if ((int)tups.size() == 256) { dict = false; return; }
-                id = (int)tups.size();
-                tups.push_back(u);
-                index.emplace(h, id);
-            }
-            ids[(size_t)slot] = (uint8_t)id;
-        });
-    }
+    // slot and a small table (rows of 8, 6 used) when there are at most 256.
+    const host::RowDictionary cd = host::dedupe_rows(d.nslots, 8, [&](int64_t slot, double *r) {
+        const int *c = &cell[(size_t)slot * 3];
+        for (int q = 0; q < 6; ++q) r[q] = coef(q, c[0], c[1], c[2]);
+    });
+    const bool dict = cd.ok;
     d.shell_snq = dict ? M_TOTAL : S_NQ;
     if (dict) {
         if (hipMalloc((void **)&d.shell_cid, (size_t)d.nslots) != hipSuccess) return SDT_ERR_HIP;
-        if (hipMemcpy(d.shell_cid, ids.data(), (size_t)d.nslots, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(d.shell_cid, cd.ids.data(), (size_t)d.nslots, hipMemcpyHostToDevice) != hipSuccess)
             return SDT_ERR_HIP;
-        std::vector<double> flat(tups.size() * 8);
-        for (size_t r = 0; r < tups.size(); ++r) memcpy(&flat[r * 8], tups[r].v, sizeof(tups[r].v));
-        if (upload_converted(d, flat.data(), &d.shell_ctab, (int64_t)flat.size()) != SDT_OK) return SDT_ERR_HIP;
-        d.shell_ncoef = (int)tups.size();
+        if (upload_converted(d, cd.rows.data(), &d.shell_ctab, (int64_t)cd.rows.size()) != SDT_OK) return SDT_ERR_HIP;
+        d.shell_ncoef = cd.count();
     }
     // Shell storage, stream-major: stream q of slot s at q * nslots + s (the
     // M_TOTAL memory variables, zero at the start, then without the
@@ -350,7 +257,10 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
     if (!dict) {
         std::vector<double> cs((size_t)d.nslots);
         for (int q = 0; q < 6; ++q) {
-            for_slots([&](int64_t slot, int i, int j, int k) { cs[(size_t)slot] = coef(q, i, j, k); });
+            for (int64_t slot = 0; slot < d.nslots; ++slot) {
+                const int *c = &cell[(size_t)slot * 3];
+                cs[(size_t)slot] = coef(q, c[0], c[1], c[2]);
+            }
             const size_t bytes = (size_t)d.nslots * d.esize;
             char *dst = (char *)d.shell + (size_t)(S_CF + q) * bytes;
             hipError_t e;

@@ -2,6 +2,7 @@
 // el3_fast.hip).
 #pragma once
 
+#include "host_layout.h"
 #include "sdt_device.h"
 #include "sdt_internal.h"
 
@@ -92,17 +93,6 @@ __device__ __forceinline__ T tplus5(T fm1, T f0, T fp1, T fp2, int corr)
 {
     return corr ? T(27) * (fp1 - f0) - (fp2 - fm1) : -T(27) * f0 + T(27) * fp1 - fp2;
 }
-
-// One z-marching tile of the CPML shell: kShellThreads cells of a plane, 64 x 4
-// (wide: z slabs and y strips) or 16 x 16 (x strips) from (x0, y0), planes
-// k0..k1, region kind (0 z slab, 1 y strip, 2 x strip) and the first slot of
-// its memory variables (slot = base + (k - k0) * kShellThreads + thread).
-constexpr int kShellThreads = 256;
-struct El3Tile {
-    int x0, y0, k0, k1;
-    int kind, wide;
-    int64_t base;
-};
 
 // Where the stress kernels read a cell's stiffness from: all 21 per-cell
 // arrays, the material table, or the isotropic arrays c11, c12, c44.

@@ -324,12 +324,16 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 // Lanes outside the grid or the tile's region compute on clamped indices and
 // store nothing; neighbour indices are clamped into the grid, and the nest
 // ranges of cpmlfdtd.f08 mask the stores.
+// the region test of host::shell_region (kinds 0 z slab, 1 / 3 y strip below
+// / above the box, 2 / 4 x strip left / right of it)
 __device__ __forceinline__ bool shell_lane(int kind, const int (&b)[6], int i, int j, int nx, int ny)
 {
     if (i < 1 || i > nx || j < 1 || j > ny) return false;
     if (kind == 0) return true;
-    if (kind == 1) return j < b[2] || j > b[3];
-    return j >= b[2] && j <= b[3] && (i < b[0] || i > b[1]);
+    if (kind == 1) return j < b[2];
+    if (kind == 3) return j > b[3];
+    const bool rows = j >= b[2] && j <= b[3];
+    return kind == 2 ? rows && i < b[0] : rows && i > b[1];
 }
 
 struct ShellCell {

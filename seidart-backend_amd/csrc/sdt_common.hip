@@ -115,24 +115,14 @@ int batch_setup(Batch &b, Device *const *shots, int n)
     b.stream = shots[0]->stream;
     // tile lists for every candidate tile height
     for (int w = 0; w < kNumBatchShapes; ++w) {
-        std::vector<int4> t;
+        std::vector<host::BatchTile> t;
         for (int s = 0; s < n; ++s) {
             const int tx = tiles_x_2d(*shots[s]), tz = tiles_z_2d(*shots[s], kBatchShapes[w].rows);
             for (int z = 0; z < tz; ++z)
-                for (int x = 0; x < tx; ++x) t.push_back(make_int4(s, x, z, 0));
+                for (int x = 0; x < tx; ++x) t.push_back(host::BatchTile{s, x, z, 0});
         }
-        if (kBatchShapes[w].xcd_order) {
-            // natural tile q: group g = q / 8 goes to XCD g % 8, slot
-            // (g / 8) * 8 + q % 8; list position = slot * 8 + XCD.  Padding
-            // tiles (shot -1) fill the list to a multiple of 64.
-            t.resize((t.size() + 63) / 64 * 64, make_int4(-1, 0, 0, 0));
-            std::vector<int4> p(t.size());
-            for (size_t q = 0; q < t.size(); ++q) {
-                const size_t g = q / 8, slot = (g / 8) * 8 + q % 8;
-                p[slot * 8 + g % 8] = t[q];
-            }
-            t.swap(p);
-        }
+        if (kBatchShapes[w].xcd_order) t = host::batch_order(t);
+        static_assert(sizeof(host::BatchTile) == sizeof(int4), "batch tiles are read as int4");
         b.ntiles[w] = (int)t.size();
         if (hipMalloc((void **)&b.tiles[w], t.size() * sizeof(int4)) != hipSuccess) return SDT_ERR_HIP;
         if (hipMemcpy(b.tiles[w], t.data(), t.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
@@ -331,11 +321,10 @@ int upload_cpml_full(Device &d, const double *const *in)
 
 void set_layout(Device &d)
 {
-    const int line = (int)(128 / d.esize);
     const bool empty = d.box[1] < d.box[0];
-    const int first = empty ? 1 : d.box[0];
-    d.xoff = (line - (first - 1) % line) % line;
-    d.pitch = (int)((d.xoff + d.nx + line - 1) / line * line);
+    const host::RowLayout r = host::row_layout(d.nx, empty ? 1 : d.box[0], (int)d.esize);
+    d.xoff = r.xoff;
+    d.pitch = r.pitch;
     d.ncell_dev = (int64_t)d.pitch * d.ny * d.nz;
 }
 
@@ -362,14 +351,8 @@ int alloc_grid_zeroed(Device &d, void **dev, int ncomp)
 template <typename T>
 static int store_grid_t(Device &d, void *dev, const double *host, int ncomp)
 {
-    const size_t rows = (size_t)d.ny * d.nz, nx = (size_t)d.nx * ncomp, pitch = (size_t)d.pitch * ncomp;
-    const size_t off = (size_t)d.xoff * ncomp;
-    std::vector<T> img(rows * pitch, T(0));
-    for (size_t r = 0; r < rows; ++r) {
-        T *dst = img.data() + r * pitch + off;
-        const double *src = host + r * nx;
-        for (size_t i = 0; i < nx; ++i) dst[i] = (T)src[i];
-    }
+    std::vector<T> img;
+    host::pack_rows<T>(host, (size_t)d.ny * d.nz, (size_t)d.nx, (size_t)d.pitch, (size_t)d.xoff, ncomp, img);
     return hipMemcpy(dev, img.data(), img.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess ? SDT_OK
                                                                                                  : SDT_ERR_HIP;
 }
@@ -377,15 +360,10 @@ static int store_grid_t(Device &d, void *dev, const double *host, int ncomp)
 template <typename T>
 static int download_grid_t(Device &d, const void *dev, double *host, int ncomp)
 {
-    const size_t rows = (size_t)d.ny * d.nz, nx = (size_t)d.nx * ncomp, pitch = (size_t)d.pitch * ncomp;
-    const size_t off = (size_t)d.xoff * ncomp;
-    std::vector<T> img(rows * pitch);
+    const size_t rows = (size_t)d.ny * d.nz;
+    std::vector<T> img(rows * d.pitch * ncomp);
     if (hipMemcpy(img.data(), dev, img.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) return SDT_ERR_HIP;
-    for (size_t r = 0; r < rows; ++r) {
-        const T *src = img.data() + r * pitch + off;
-        double *dst = host + r * nx;
-        for (size_t i = 0; i < nx; ++i) dst[i] = (double)src[i];
-    }
+    host::unpack_rows<T>(img, rows, (size_t)d.nx, (size_t)d.pitch, (size_t)d.xoff, ncomp, host);
     return SDT_OK;
 }
 
@@ -454,45 +432,10 @@ void launch_pack(const Device &d, const void *src, void *dst, int single, int k0
 
 int build_material_table(Device &d, int row_len, const std::function<void(int64_t, double *)> &fill)
 {
-    const int64_t n = d.ncell;
-    const size_t rb = (size_t)row_len * sizeof(double);
-    auto hash = [&](const double *r) {
-        uint64_t h = 1469598103934665603ull;
-        for (int q = 0; q < row_len; ++q) {
-            uint64_t w;
-            memcpy(&w, &r[q], 8);
-            h = (h ^ w) * 1099511628211ull;
-            h ^= h >> 29;
-        }
-        return h;
-    };
-    std::vector<double> rows;                       // distinct rows, row_len each
-    std::unordered_multimap<uint64_t, int> index;
-    std::vector<uint8_t> ids((size_t)n);
-    std::vector<double> cur((size_t)row_len), prev((size_t)row_len);
-    int prev_id = -1;
-    for (int64_t c = 0; c < n; ++c) {
-        std::fill(cur.begin(), cur.end(), 0.0);
-        fill(c, cur.data());
-        if (prev_id >= 0 && memcmp(cur.data(), prev.data(), rb) == 0) {   // runs along x
-            ids[(size_t)c] = (uint8_t)prev_id;
-            continue;
-        }
-        const uint64_t h = hash(cur.data());
-        int id = -1;
-        auto range = index.equal_range(h);
-        for (auto it = range.first; it != range.second; ++it)
-            if (memcmp(&rows[(size_t)it->second * row_len], cur.data(), rb) == 0) { id = it->second; break; }
-        if (id < 0) {
-            if ((int)(rows.size() / row_len) == 256) return SDT_OK;   // too many: per-cell arrays
-            id = (int)(rows.size() / row_len);
-            rows.insert(rows.end(), cur.begin(), cur.end());
-            index.emplace(h, id);
-        }
-        ids[(size_t)c] = (uint8_t)id;
-        prev.swap(cur);
-        prev_id = id;
-    }
+    const host::RowDictionary md = host::dedupe_rows(d.ncell, row_len, fill);
+    if (!md.ok) return SDT_OK;   // too many materials: per-cell arrays
+    const std::vector<uint8_t> &ids = md.ids;
+    const std::vector<double> &rows = md.rows;
     // ids in the padded layout (1 byte per element)
     const size_t bytes = (size_t)d.ncell_dev;
     if (hipMalloc((void **)&d.mid, bytes) != hipSuccess) return SDT_ERR_HIP;
@@ -506,51 +449,18 @@ int build_material_table(Device &d, int row_len, const std::function<void(int64_
     return SDT_OK;
 }
 
-static inline bool trivial(double a, double b, double kappa)
-{
-    return a == 0.0 && kappa == 1.0 && isfinite(b);
-}
-
 int compute_tile_flags(Device &d, const double *const *host_in)
 {
-    d.tiles_x = (d.nx + d.tile_x - 1) / d.tile_x;
-    d.tiles_y = (d.ny + d.tile_y - 1) / d.tile_y;
-    d.tiles_z = (d.nz + d.tile_z - 1) / d.tile_z;
-    const size_t ntiles = (size_t)d.tiles_x * d.tiles_y * d.tiles_z;
-    std::vector<uint8_t> flags(ntiles, 0);
-    const double *a = host_in[SDT_ACOEF], *b = host_in[SDT_BCOEF], *k = host_in[SDT_KAPPA];
-    const double *ah = host_in[SDT_ACOEF_HALF], *bh = host_in[SDT_BCOEF_HALF],
-                 *kh = host_in[SDT_KAPPA_HALF];
-    auto ident = [&](int i, int j, int kk) {   // 1-based
-        const int64_t c = (int64_t)(i - 1) + (int64_t)d.nx * ((int64_t)(j - 1) + (int64_t)d.ny * (kk - 1));
-        return trivial(a ? a[c] : 0.0, b ? b[c] : 0.0, k ? k[c] : 1.0) &&
-               trivial(ah ? ah[c] : 0.0, bh ? bh[c] : 0.0, kh ? kh[c] : 1.0);
-    };
-    const int ic = (d.nx + 1) / 2, jc = (d.ny + 1) / 2, kc = (d.nz + 1) / 2;
-    int *bx = d.box;
-    if (!ident(ic, jc, kc)) {
-        bx[0] = 1; bx[1] = 0; bx[2] = 1; bx[3] = 0; bx[4] = 1; bx[5] = 0;   // empty
-    } else {
-        bx[0] = ic; while (bx[0] > 1 && ident(bx[0] - 1, jc, kc)) --bx[0];
-        bx[1] = ic; while (bx[1] < d.nx && ident(bx[1] + 1, jc, kc)) ++bx[1];
-        bx[2] = jc; while (bx[2] > 1 && ident(ic, bx[2] - 1, kc)) --bx[2];
-        bx[3] = jc; while (bx[3] < d.ny && ident(ic, bx[3] + 1, kc)) ++bx[3];
-        bx[4] = kc; while (bx[4] > 1 && ident(ic, jc, bx[4] - 1)) --bx[4];
-        bx[5] = kc; while (bx[5] < d.nz && ident(ic, jc, bx[5] + 1)) ++bx[5];
-    }
-    // cells inside the box that are not identity cells flag their tile
-    for (int kk = bx[4]; kk <= bx[5]; ++kk)
-        for (int j = bx[2]; j <= bx[3]; ++j)
-            for (int i = bx[0]; i <= bx[1]; ++i)
-                if (!ident(i, j, kk)) {
-                    const size_t t = (size_t)((i - 1) / d.tile_x) +
-                                     (size_t)d.tiles_x * ((size_t)((j - 1) / d.tile_y) +
-                                                          (size_t)d.tiles_y * ((kk - 1) / d.tile_z));
-                    d.nflagged += flags[t] ? 0 : 1;
-                    flags[t] = 1;
-                }
-    if (hipMalloc((void **)&d.tile_flags, ntiles) != hipSuccess) return SDT_ERR_HIP;
-    if (hipMemcpy(d.tile_flags, flags.data(), ntiles, hipMemcpyHostToDevice) != hipSuccess)
+    const host::BoxScan bs = host::scan_box(d.nx, d.ny, d.nz, d.tile_x, d.tile_y, d.tile_z, host_in[SDT_ACOEF],
+                                            host_in[SDT_BCOEF], host_in[SDT_KAPPA], host_in[SDT_ACOEF_HALF],
+                                            host_in[SDT_BCOEF_HALF], host_in[SDT_KAPPA_HALF]);
+    d.tiles_x = bs.tiles_x;
+    d.tiles_y = bs.tiles_y;
+    d.tiles_z = bs.tiles_z;
+    for (int q = 0; q < 6; ++q) d.box[q] = bs.box[q];
+    d.nflagged += bs.nflagged;
+    if (hipMalloc((void **)&d.tile_flags, bs.flags.size()) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy(d.tile_flags, bs.flags.data(), bs.flags.size(), hipMemcpyHostToDevice) != hipSuccess)
         return SDT_ERR_HIP;
     return SDT_OK;
 }

@@ -8,6 +8,7 @@
 #include <functional>
 
 #include "../../include/seidart_hip.h"
+#include "host_layout.h"
 
 namespace sdt {
 

@@ -103,6 +103,37 @@ def test_elastic3_small_fp64(options):
     _check(g, o, TOL64, stress_norm=True)
 
 
+@pytest.mark.parametrize("ny,cpml", [(2, 5), (3, 5), (1, 5)])
+def test_elastic3_thin_interior_box(ny, cpml):
+    """Interior boxes one row (ny 2) and two rows (ny 3) tall: the last 4-row
+    y-strip tile below the box reaches past it into rows the strip above owns,
+    and every CPML-shell cell must still be updated by exactly one tile (a
+    cell updated by two tiles gets its stress increment twice, depending on
+    their timing).  ny 1 leaves no CPML-free centre: every cell takes the
+    shell kernels."""
+    import numpy as np
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    from oracle.oracle_ctypes import run_oracle
+    cfg = models.elastic3(nx=20, ny=ny, nz=18, cpml=cpml, steps=30, src=(10, 1, 6), nrec=4)
+    fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    rng = np.random.default_rng(ny * 10 + cpml)
+    # every cell active from the first step: velocities O(1), stresses O(rho c v)
+    ic = {f: rng.uniform(-1.0, 1.0, cfg["shape"]) * (1.0 if f.startswith("v") else 4e6) for f in fields}
+    print(f"[parity] elastic3 thin box shape={cfg['shape']}", flush=True)
+    s = make_solver(cfg)
+    try:
+        for k, a in ic.items():
+            s.set_field(k, a)
+        s.run()
+        g = (s.receivers(), *s.norms(), {f: s.field(f) for f in fields})
+    finally:
+        s.close()
+    o_rec, (o_vn, o_sn), o_f, us = run_oracle(cfg, fields=fields, field_in=ic)
+    assert us == 0
+    _check(g, (o_rec, o_vn, o_sn, o_f), TOL64, stress_norm=True)
+
+
 def test_elastic3_corrected_stencil():
     from seidart_hip import models
     cfg = models.elastic3(nx=30, ny=26, nz=28, cpml=5, steps=120, src=(15, 13, 9), nrec=8)
