@@ -26,17 +26,28 @@
 
 namespace sdt {
 
-// One z-marching tile of the CPML shell: kShellThreads cells of a plane, 64 x 4
-// (wide: z slabs and y strips) or 16 x 16 (x strips) from (x0, y0), planes
-// k0..k1, region kind (0 z slab; 1, 3 the y strip below, above the box; 2, 4
-// the x strip left, right of it) and the first slot of its memory variables
+// One z-marching tile of the CPML shell: up to kShellThreads cells of a plane,
+// tw columns by kShellThreads / tw rows from (x0, y0) (64 x 4 over z slabs and
+// y strips, strip width by as many rows as fit over x strips), planes k0..k1,
+// region kind (0 z slab; 1, 3 the y strip below, above the box; 2, 4 the x
+// strip left, right of it) and the first slot of its memory variables
 // (slot = base + (k - k0) * kShellThreads + thread).
 constexpr int kShellThreads = 256;
 struct El3Tile {
     int x0, y0, k0, k1;
-    int kind, wide;
+    int kind, tw;
     int64_t base;
 };
+
+// Column (i, j) of thread t of a tile; false for the threads past its last
+// full row of tw columns (they own slots but no cell).  The kernels use the
+// same mapping.
+inline bool tile_lane(const El3Tile &t, int thread, int &i, int &j)
+{
+    i = t.x0 + thread % t.tw;
+    j = t.y0 + thread / t.tw;
+    return thread < t.tw * (kShellThreads / t.tw);
+}
 
 namespace host {
 
@@ -181,38 +192,42 @@ struct ShellTiles {
 };
 
 // Tiles of the cells outside the box (all cells when the box is empty), plane
-// chunk by plane chunk of kc planes, rows then columns: z slabs below and
-// above the box and y strips in 64 x 4 tiles, x strips in 16 x 16 tiles, every
-// tile's x origin on the 128-byte line grid of the padded rows.  A tile is
-// listed when one of its columns lies in its region; its slots follow the
-// previous tile's.
-inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t xoff, int esize, int kc)
+// chunk by plane chunk of kc planes: z slabs below and above the box and y
+// strips in 64 x 4 tiles whose x origin is on the 128-byte line grid of the
+// padded rows, rows then columns; x strips (left, then right of the box) in
+// tiles as wide as the strip (at most 64 columns) and kShellThreads / width
+// rows tall, so a narrow strip keeps almost every lane busy.  A tile is listed
+// when one of its columns lies in its region; its slots follow the previous
+// tile's.
+inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t xoff, int kc)
 {
     ShellTiles st;
     st.nslots = 0;
     const bool empty = b[1] < b[0];
-    const int line = 128 / esize;
     const int xa = 1 - (int)xoff;   // x origin of line-aligned tiles
-    auto add = [&](int x0, int y0, int k0, int k1, int kind, int wide) {
-        const int tw = wide ? 64 : 16, th = kShellThreads / tw;
-        for (int ty = 0; ty < th; ++ty)
-            for (int tx = 0; tx < tw; ++tx)
-                if (shell_region(kind, b, x0 + tx, y0 + ty, nx, ny)) {
-                    st.tiles.push_back(El3Tile{x0, y0, k0, k1, kind, wide, st.nslots});
-                    st.nslots += (int64_t)(k1 - k0 + 1) * kShellThreads;
-                    return;
-                }
+    auto add = [&](int x0, int y0, int k0, int k1, int kind, int tw) {
+        const El3Tile t{x0, y0, k0, k1, kind, tw, st.nslots};
+        for (int th = 0; th < kShellThreads; ++th) {
+            int i, j;
+            if (tile_lane(t, th, i, j) && shell_region(kind, b, i, j, nx, ny)) {
+                st.tiles.push_back(t);
+                st.nslots += (int64_t)(k1 - k0 + 1) * kShellThreads;
+                return;
+            }
+        }
     };
     auto zslab = [&](int ka, int kb) {
         for (int k0 = ka; k0 <= kb; k0 += kc) {
             const int k1 = k0 + kc - 1 < kb ? k0 + kc - 1 : kb;
             for (int y0 = 1; y0 <= ny; y0 += 4)
-                for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 0, 1);
+                for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 0, 64);
         }
     };
-    auto xstrip = [&](int lo, int hi, int y0, int k0, int k1, int kind) {
+    auto xstrip = [&](int lo, int hi, int k0, int k1, int kind) {
         if (lo > hi) return;
-        for (int x0 = xa + ((lo - xa) / line) * line; x0 <= hi; x0 += 16) add(x0, y0, k0, k1, kind, 0);
+        const int tw = hi - lo + 1 < 64 ? hi - lo + 1 : 64, rows = kShellThreads / tw;
+        for (int y0 = b[2]; y0 <= b[3]; y0 += rows)
+            for (int x0 = lo; x0 <= hi; x0 += tw) add(x0, y0, k0, k1, kind, tw);
     };
     if (empty) {
         zslab(1, nz);
@@ -222,29 +237,28 @@ inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t
     for (int k0 = b[4]; k0 <= b[5]; k0 += kc) {
         const int k1 = k0 + kc - 1 < b[5] ? k0 + kc - 1 : b[5];
         for (int y0 = 1; y0 < b[2]; y0 += 4)
-            for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 1, 1);
-        for (int y0 = b[2]; y0 <= b[3]; y0 += 16) {
-            xstrip(1, b[0] - 1, y0, k0, k1, 2);
-            xstrip(b[1] + 1, nx, y0, k0, k1, 4);
-        }
+            for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 1, 64);
+        xstrip(1, b[0] - 1, k0, k1, 2);
+        xstrip(b[1] + 1, nx, k0, k1, 4);
         for (int y0 = b[3] + 1; y0 <= ny; y0 += 4)
-            for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 3, 1);
+            for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 3, 64);
     }
     zslab(b[5] + 1, nz);
     return st;
 }
 
-// f(slot, i, j, k) for every slot of every tile (lanes outside the grid or
-// the region included, with their unclamped column)
+// f(slot, i, j, k) for every slot of every tile (threads outside the grid,
+// the region or the tile's full rows included, with their unclamped column)
 template <typename F>
 void for_shell_slots(const std::vector<El3Tile> &tiles, F &&f)
 {
-    for (const El3Tile &t : tiles) {
-        const int tw = t.wide ? 64 : 16;
+    for (const El3Tile &t : tiles)
         for (int k = t.k0; k <= t.k1; ++k)
-            for (int th = 0; th < kShellThreads; ++th)
-                f(t.base + (int64_t)(k - t.k0) * kShellThreads + th, t.x0 + th % tw, t.y0 + th / tw, k);
-    }
+            for (int th = 0; th < kShellThreads; ++th) {
+                int i, j;
+                (void)tile_lane(t, th, i, j);
+                f(t.base + (int64_t)(k - t.k0) * kShellThreads + th, i, j, k);
+            }
 }
 
 // ------------------------------------------------------------ dictionaries

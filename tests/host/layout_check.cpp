@@ -7,9 +7,10 @@
 //   * row_layout: the box's first column starts a 128-byte line, the pitch is
 //     whole lines and holds the row;
 //   * shell_tiles: every cell outside the box is covered by exactly one
-//     (tile, lane, plane), no cell inside it is, tiles start on the line
-//     grid, chunks hold at most kc planes, and slots number the tiles'
-//     lanes without gaps or overlaps;
+//     (tile, lane, plane), no cell inside it is, z-slab and y-strip tiles
+//     start on the line grid and x-strip tiles inside their strip, chunks hold
+//     at most kc planes, and slots number the tiles' lanes without gaps or
+//     overlaps;
 //   * dedupe_rows: every entry's id maps back to its row bit for bit, ids are
 //     numbered in order of first appearance, and more than 256 distinct rows
 //     is reported;
@@ -114,20 +115,23 @@ static void check_grid(int nx, int ny, int nz, int w, int strays, int esize, int
     CHECK((rl.xoff + (empty ? 1 : box[0]) - 1) % line == 0, "first column off the line grid");
     CHECK(rl.pitch % line == 0 && rl.pitch >= rl.xoff + nx && rl.pitch < rl.xoff + nx + line, "pitch %d", rl.pitch);
 
-    const host::ShellTiles st = host::shell_tiles(nx, ny, nz, box, rl.xoff, esize, kc);
+    const host::ShellTiles st = host::shell_tiles(nx, ny, nz, box, rl.xoff, kc);
     std::vector<int> cover((size_t)nx * ny * nz, 0);
     int64_t base = 0;
     for (const El3Tile &t : st.tiles) {
         CHECK(t.base == base, "tile base %lld, expected %lld", (long long)t.base, (long long)base);
         CHECK(t.k0 >= 1 && t.k1 <= nz && t.k0 <= t.k1 && t.k1 - t.k0 + 1 <= kc, "planes %d..%d", t.k0, t.k1);
-        CHECK((rl.xoff + t.x0 - 1) % 16 == 0 && (!t.wide || (rl.xoff + t.x0 - 1) % line == 0),
-              "tile x0 %d off the line grid", t.x0);
-        CHECK(t.kind >= 0 && t.kind <= 4 && t.wide == (t.kind != 2 && t.kind != 4), "tile shape of kind %d", t.kind);
-        const int tw = t.wide ? 64 : 16;
+        const bool xs = t.kind == 2 || t.kind == 4;
+        CHECK(t.kind >= 0 && t.kind <= 4 && (xs ? t.tw >= 1 && t.tw <= 64 : t.tw == 64), "tile shape of kind %d",
+              t.kind);
+        CHECK(xs || (rl.xoff + t.x0 - 1) % line == 0, "tile x0 %d off the line grid", t.x0);
+        if (xs)   // x-strip tiles start inside their strip and are no wider
+            CHECK(t.kind == 2 ? t.x0 >= 1 && t.x0 <= box[0] - 1 : t.x0 >= box[1] + 1 && t.x0 <= nx,
+                  "x-strip tile x0 %d width %d", t.x0, t.tw);
         bool any = false;
         for (int th = 0; th < kShellThreads; ++th) {
-            const int i = t.x0 + th % tw, j = t.y0 + th / tw;
-            if (!host::shell_region(t.kind, box, i, j, nx, ny)) continue;
+            int i, j;
+            if (!tile_lane(t, th, i, j) || !host::shell_region(t.kind, box, i, j, nx, ny)) continue;
             any = true;
             for (int kk = t.k0; kk <= t.k1; ++kk) {
                 const bool inside = !empty && i >= box[0] && i <= box[1] && j >= box[2] && j <= box[3] &&
