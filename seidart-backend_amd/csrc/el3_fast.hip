@@ -11,8 +11,8 @@
 //  * shell kernels over the cells outside the box (the CPML slabs and the
 //    Dirichlet faces), as a list of z-marching tiles of 256 cells per plane,
 //    one thread per cell like the interior kernels: 64 x 4 over the z slabs
-//    and the y strips, strip-wide (12 x 21 for 12-cell strips) over the x
-//    strips.  Their memory
+//    and the y strips, strip-wide (12 x 20 for 12-cell strips, each
+//    wavefront on 5 whole rows) over the x strips.  Their memory
 //    variables and CPML coefficient ids are stored compactly in tile order,
 //    so those streams are contiguous per wavefront.  Neighbour indices are
 //    clamped into the grid so every load is unconditional; nest ranges only
@@ -315,8 +315,8 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 // strips, Dirichlet faces included) as a list of z-marching tiles of
 // kShellThreads cells per plane, one thread per cell: 64 x 4 over the z slabs
 // and y strips (every wavefront reads whole 128-byte lines), as wide as the
-// strip over the x strips (12 x 21 for 12-cell strips, so nearly every lane
-// has a cell; threads past the last full row have none).  A thread keeps its
+// strip over the x strips (12 x 20 for 12-cell strips: each wavefront takes
+// 5 whole rows and leaves 4 threads idle, so it touches one line per row).  A thread keeps its
 // column's z stencil planes in
 // registers, issues every load of a plane before its first store, and
 // updates all of its cell's memory variables and fields itself -- no LDS and
@@ -350,10 +350,9 @@ __device__ __forceinline__ ShellCell shell_cell(const El3Tile &tl, const int (&b
                                                 int64_t xoff)
 {
     ShellCell L;
-    const int tw = tl.tw;
-    const int t = threadIdx.x;
-    const int i0 = tl.x0 + t % tw, j0 = tl.y0 + t / tw;
-    L.on = t < tw * (kShellThreads / tw) && shell_lane(tl.kind, box, i0, j0, nx, ny);
+    const int tw = tl.tw, rpw = 64 / tw, l = threadIdx.x % 64;   // host::tile_lane
+    const int i0 = tl.x0 + l % tw, j0 = tl.y0 + (threadIdx.x / 64) * rpw + l / tw;
+    L.on = l < rpw * tw && shell_lane(tl.kind, box, i0, j0, nx, ny);
     const int i = min(max(i0, 1), nx), j = min(max(j0, 1), ny);
     L.i = i;
     L.j = j;

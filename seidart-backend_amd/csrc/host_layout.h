@@ -27,8 +27,9 @@
 namespace sdt {
 
 // One z-marching tile of the CPML shell: up to kShellThreads cells of a plane,
-// tw columns by kShellThreads / tw rows from (x0, y0) (64 x 4 over z slabs and
-// y strips, strip width by as many rows as fit over x strips), planes k0..k1,
+// tw columns from (x0, y0), each wavefront of 64 threads on 64 / tw whole rows
+// (64 x 4 over z slabs and y strips; over x strips as wide as the strip, e.g.
+// 12 x 20 with 4 idle threads per wavefront for a 12-cell CPML), planes k0..k1,
 // region kind (0 z slab; 1, 3 the y strip below, above the box; 2, 4 the x
 // strip left, right of it) and the first slot of its memory variables
 // (slot = base + (k - k0) * kShellThreads + thread).
@@ -39,14 +40,18 @@ struct El3Tile {
     int64_t base;
 };
 
-// Column (i, j) of thread t of a tile; false for the threads past its last
-// full row of tw columns (they own slots but no cell).  The kernels use the
-// same mapping.
+// Rows of a tile: 64 / tw per wavefront, four wavefronts.
+inline int tile_rows(int tw) { return 4 * (64 / tw); }
+
+// Column (i, j) of thread t of a tile: wavefront w = t / 64 takes rows
+// w * (64 / tw) ...; false for the threads past the wavefront's last whole row
+// (they own slots but no cell).  The kernels use the same mapping.
 inline bool tile_lane(const El3Tile &t, int thread, int &i, int &j)
 {
-    i = t.x0 + thread % t.tw;
-    j = t.y0 + thread / t.tw;
-    return thread < t.tw * (kShellThreads / t.tw);
+    const int rpw = 64 / t.tw, l = thread % 64;
+    i = t.x0 + l % t.tw;
+    j = t.y0 + (thread / 64) * rpw + l / t.tw;
+    return l < rpw * t.tw;
 }
 
 namespace host {
@@ -195,8 +200,8 @@ struct ShellTiles {
 // chunk by plane chunk of kc planes: z slabs below and above the box and y
 // strips in 64 x 4 tiles whose x origin is on the 128-byte line grid of the
 // padded rows, rows then columns; x strips (left, then right of the box) in
-// tiles as wide as the strip (at most 64 columns) and kShellThreads / width
-// rows tall, so a narrow strip keeps almost every lane busy.  A tile is listed
+// tiles as wide as the strip (at most 64 columns), each wavefront on whole
+// rows, so a narrow strip keeps most lanes busy.  A tile is listed
 // when one of its columns lies in its region; its slots follow the previous
 // tile's.
 inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t xoff, int kc)
@@ -225,7 +230,7 @@ inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t
     };
     auto xstrip = [&](int lo, int hi, int k0, int k1, int kind) {
         if (lo > hi) return;
-        const int tw = hi - lo + 1 < 64 ? hi - lo + 1 : 64, rows = kShellThreads / tw;
+        const int tw = hi - lo + 1 < 64 ? hi - lo + 1 : 64, rows = tile_rows(tw);
         for (int y0 = b[2]; y0 <= b[3]; y0 += rows)
             for (int x0 = lo; x0 <= hi; x0 += tw) add(x0, y0, k0, k1, kind, tw);
     };
