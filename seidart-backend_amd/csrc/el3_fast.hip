@@ -508,7 +508,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
             T pin[K_NKIND];
 #pragma unroll
             for (int kd = 0; kd < K_NKIND; ++kd)
-                pin[kd] = (m == 0 && kd == K_ZMH) ? T(0) : ldnt(&ps[(m * K_NKIND + kd) * ns]);
+                pin[kd] = (m == 0 && kd == K_ZMH) || !L.on ? T(0) : ldnt(&ps[(m * K_NKIND + kd) * ns]);
             const T dxp = dp4(xm1, v0, xp1, xp2, rx), dxm = dm4(xm2, xm1, v0, xp1, rx);
             const T dyp = dp4(ym1, v0, yp1, yp2, ry), dym = dm4(ym2, ym1, v0, yp1, ry);
             const T dzp = dp4(q[m][1], v0, q[m][3], q[m][4], rz), dzm = dm4(q[m][0], q[m][1], v0, q[m][3], rz);
@@ -609,7 +609,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
         T *ps = A.shell + slot;
         T pin[9];
 #pragma unroll
-        for (int s = 0; s < 9; ++s) pin[s] = ldnt(&ps[(M_SXX_X + s) * ns]);
+        for (int s = 0; s < 9; ++s) pin[s] = L.on ? ldnt(&ps[(M_SXX_X + s) * ns]) : T(0);
         T vx = ldnt(&A.v[0][c]), vy = ldnt(&A.v[1][c]), vz = ldnt(&A.v[2][c]);
         const bool face = i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz;
         const T ryy = scalar_mean(r, r, dc);
