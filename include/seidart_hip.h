@@ -298,7 +298,9 @@ enum {
     SDT_INFO_DEVICE_BYTES = 3,  /* device memory held by the handle's grid arrays                 */
     SDT_INFO_COEFFICIENTS = 4,  /* update coefficients per cell (2D EM 7, 2D elastic 10, else 0)  */
     SDT_INFO_ISOTROPIC = 5,     /* 1 = the isotropic arrays (c11, c12, c44) are in use (3D, 2.5D elastic) */
-    SDT_INFO_FUSED = 6          /* 1 = the fused 3D EM step is in use                      */
+    SDT_INFO_FUSED = 6,         /* 1 = the fused 3D EM step is in use                      */
+    SDT_INFO_CPML_TUPLES = 7    /* 3D EM: distinct CPML coefficient tuples held as a one-byte
+                                   id per cell and a table, 0 = six per-cell arrays        */
 };
 int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 

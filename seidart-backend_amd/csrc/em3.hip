@@ -22,6 +22,10 @@ struct Em3Args {
     const T *aEx, *bEx, *cEx, *bEy, *cEy, *cEz;            // aEy=bEx, aEz=cEx, bEz=cEy
     const T *s11, *s12, *s13, *s22, *s23, *s33;
     const T *a, *b, *ik, *ah, *bh, *ikh;
+    // CPML coefficients as a dictionary (ctab != nullptr): id per padded cell,
+    // rows of 8 = a b 1/kappa a_half b_half 1/kappa_half
+    const uint8_t *cid;
+    const T *ctab;
     // E-side (H update) and H-side (E update) memory variables
     T *m_dEy_dz, *m_dEz_dy, *m_dEx_dz, *m_dEz_dx, *m_dEx_dy, *m_dEy_dx;
     T *m_dHz_dy, *m_dHy_dz, *m_dHz_dx, *m_dHx_dz, *m_dHx_dy, *m_dHy_dx;
@@ -46,6 +50,20 @@ struct Em3Args {
 };
 
 constexpr int kEmRow = 16;
+
+// A CPML cell's coefficients: from the dictionary when there is one (a
+// wave-uniform test of a kernel argument), else from the six arrays.
+template <typename T>
+__device__ __forceinline__ void em3_coefs(const Em3Args<T> &A, int64_t c, T &a, T &b, T &ik, T &ah, T &bh, T &ikh)
+{
+    if (A.ctab) {
+        const T *r = A.ctab + 8 * (int)A.cid[c];
+        a = r[0]; b = r[1]; ik = r[2]; ah = r[3]; bh = r[4]; ikh = r[5];
+    } else {
+        a = A.a[c]; b = A.b[c]; ik = A.ik[c];
+        ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+    }
+}
 
 template <typename T>
 using em_const_row = __attribute__((address_space(4))) const T *;
@@ -108,8 +126,7 @@ __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, in
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
     T m_eydz = 0, m_ezdy = 0, m_exdz = 0, m_ezdx = 0, m_exdy = 0, m_eydx = 0;
     if (PML) {
-        a = A.a[c]; b = A.b[c]; ik = A.ik[c];
-        ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+        em3_coefs(A, c, a, b, ik, ah, bh, ikh);
         m_eydz = A.m_dEy_dz[c]; m_ezdy = A.m_dEz_dy[c];
         m_exdz = A.m_dEx_dz[c]; m_ezdx = A.m_dEz_dx[c];
         m_exdy = A.m_dEx_dy[c]; m_eydx = A.m_dEy_dx[c];
@@ -186,8 +203,7 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
         T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
         T m_hzdy = 0, m_hydz = 0, m_hzdx = 0, m_hxdz = 0, m_hxdy = 0, m_hydx = 0;
         if (PML) {
-            a = A.a[c]; b = A.b[c]; ik = A.ik[c];
-            ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+            em3_coefs(A, c, a, b, ik, ah, bh, ikh);
             m_hzdy = A.m_dHz_dy[c]; m_hydz = A.m_dHy_dz[c]; m_hzdx = A.m_dHz_dx[c];
             m_hxdz = A.m_dHx_dz[c]; m_hxdy = A.m_dHx_dy[c]; m_hydx = A.m_dHy_dx[c];
         }
@@ -291,8 +307,7 @@ __device__ __forceinline__ void em3_h_new(const Em3Args<T> &A, int i, int j, int
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
     T m_eydz = 0, m_ezdy = 0, m_exdz = 0, m_ezdx = 0, m_exdy = 0, m_eydx = 0;
     if (pml) {
-        a = A.a[c]; b = A.b[c]; ik = A.ik[c];
-        ah = A.ah[c]; bh = A.bh[c]; ikh = A.ikh[c];
+        em3_coefs(A, c, a, b, ik, ah, bh, ikh);
         m_eydz = A.mo[0][c]; m_ezdy = A.mo[1][c]; m_exdz = A.mo[2][c];
         m_ezdx = A.mo[3][c]; m_exdy = A.mo[4][c]; m_eydx = A.mo[5][c];
     }
@@ -349,8 +364,7 @@ __device__ __forceinline__ EPre<T> em3_e_pre(const Em3Args<T> &A, int64_t c, boo
     P.a = 0; P.b = 0; P.ik = 1; P.ah = 0; P.bh = 0; P.ikh = 1;
     P.mhzdy = P.mhydz = P.mhzdx = P.mhxdz = P.mhxdy = P.mhydx = T(0);
     if (pml) {
-        P.a = A.a[c]; P.b = A.b[c]; P.ik = A.ik[c];
-        P.ah = A.ah[c]; P.bh = A.bh[c]; P.ikh = A.ikh[c];
+        em3_coefs(A, c, P.a, P.b, P.ik, P.ah, P.bh, P.ikh);
         P.mhzdy = A.m_dHz_dy[c]; P.mhydz = A.m_dHy_dz[c]; P.mhzdx = A.m_dHz_dx[c];
         P.mhxdz = A.m_dHx_dz[c]; P.mhxdy = A.m_dHx_dy[c]; P.mhydx = A.m_dHy_dx[c];
     }
@@ -482,6 +496,8 @@ static Em3Args<T> em3_args(Device &d)
     A.ah = (const T *)d.cpml[3];
     A.bh = (const T *)d.cpml[4];
     A.ikh = (const T *)d.cpml[5];
+    A.cid = d.cpml_cid;
+    A.ctab = (const T *)d.cpml_ctab;
     T **mm[] = {&A.m_dEy_dz, &A.m_dEz_dy, &A.m_dEx_dz, &A.m_dEz_dx, &A.m_dEx_dy, &A.m_dEy_dx,
                 &A.m_dHz_dy, &A.m_dHy_dz, &A.m_dHz_dx, &A.m_dHx_dz, &A.m_dHx_dy, &A.m_dHy_dx};
     for (int q = 0; q < 12; ++q) *mm[q] = (T *)d.mem[q];
@@ -668,7 +684,7 @@ int em3_setup(Device &d, const double *const *in)
     t.nsfix = 0;
     t.rec_field[0] = SDT_F_EX; t.rec_field[1] = SDT_F_EY; t.rec_field[2] = SDT_F_EZ;
 
-    return upload_cpml_full(d, in);
+    return upload_cpml_dict(d, in);
 }
 
 }  // namespace sdt

@@ -173,6 +173,7 @@ static void release(sdt_handle *h)
     drop(d.tiles);
     drop(d.mid);   // d.tab is one of the grid allocations freed above
     drop(d.shell_cid);
+    drop(d.cpml_cid);   // d.cpml_ctab is a grid allocation
     drop(d.series);
     drop(d.norms);
     drop(d.d_it);
@@ -751,6 +752,7 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
     case SDT_INFO_COEFFICIENTS: *value = h->d.adj.ncoef; return SDT_OK;
     case SDT_INFO_ISOTROPIC: *value = h->d.mat_iso ? 1 : 0; return SDT_OK;
     case SDT_INFO_FUSED: *value = h->d.fused ? 1 : 0; return SDT_OK;
+    case SDT_INFO_CPML_TUPLES: *value = h->d.cpml_ncoef; return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown info key");
 }

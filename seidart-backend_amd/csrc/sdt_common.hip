@@ -302,6 +302,29 @@ int alloc_zeroed(Device &d, void **dev, int64_t n)
     return SDT_OK;
 }
 
+int upload_cpml_dict(Device &d, const double *const *in)
+{
+    const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};
+    const host::RowDictionary cd = host::dedupe_rows(d.ncell, 8, [&](int64_t c, double *r) {
+        for (int q = 0; q < 6; ++q) {
+            const double *src = in[cids[q]];
+            const bool inverse = (q == 2 || q == 5);
+            const double v = src ? src[c] : (inverse ? 1.0 : 0.0);
+            r[q] = inverse ? 1.0 / v : v;
+        }
+    });
+    if (!cd.ok) return upload_cpml_full(d, in);
+    const size_t bytes = (size_t)d.ncell_dev;
+    if (hipMalloc((void **)&d.cpml_cid, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemset(d.cpml_cid, 0, bytes) != hipSuccess) return SDT_ERR_HIP;
+    if (hipMemcpy2D(d.cpml_cid + d.xoff, (size_t)d.pitch, cd.ids.data(), (size_t)d.nx, (size_t)d.nx,
+                    (size_t)d.ny * d.nz, hipMemcpyHostToDevice) != hipSuccess)
+        return SDT_ERR_HIP;
+    if (upload_converted(d, cd.rows.data(), &d.cpml_ctab, (int64_t)cd.rows.size()) != SDT_OK) return SDT_ERR_HIP;
+    d.cpml_ncoef = cd.count();
+    return SDT_OK;
+}
+
 int upload_cpml_full(Device &d, const double *const *in)
 {
     const int cids[6] = {SDT_ACOEF, SDT_BCOEF, SDT_KAPPA, SDT_ACOEF_HALF, SDT_BCOEF_HALF, SDT_KAPPA_HALF};

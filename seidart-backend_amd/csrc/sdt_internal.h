@@ -170,6 +170,11 @@ struct Device {
     void *mat[kMaxMat] = {};
     int nmat = 0;
     void *cpml[6] = {};                // a, b, 1/kappa (full), a, b, 1/kappa (half)
+    // or, when the grid holds at most 256 distinct such tuples: a one-byte id
+    // per padded cell and a table of rows of 8 (6 used) (upload_cpml_dict)
+    uint8_t *cpml_cid = nullptr;
+    void *cpml_ctab = nullptr;
+    int cpml_ncoef = 0;
     uint8_t *tile_flags = nullptr;     // 1 = tile holds a CPML cell inside the interior box
     int64_t nflagged = 0;              // tiles flagged inside the box
     void *tiles = nullptr;             // 3D elastic CPML-shell tile list (device)
@@ -361,6 +366,9 @@ int compute_tile_flags(Device &d, const double *const *host_in);
 // full-grid CPML coefficient arrays: a, b, 1/kappa for the full and half
 // sets (identity defaults where not supplied)
 int upload_cpml_full(Device &d, const double *const *host_in);
+// The same coefficients as a dictionary (cpml_cid, cpml_ctab) when there are
+// at most 256 distinct tuples, otherwise as upload_cpml_full
+int upload_cpml_dict(Device &d, const double *const *host_in);
 // choose pitch / xoff from d.box (call after compute_tile_flags)
 void set_layout(Device &d);
 inline int64_t cell_index(const Device &d, int i, int j, int k)

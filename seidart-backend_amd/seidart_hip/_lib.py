@@ -72,7 +72,8 @@ EXPORTS = [
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
     1, 2, 3, 4, 5, 6, 7)
 OPT_CHECKPOINT, OPT_RAW_SOURCE, OPT_ISOTROPIC, OPT_FUSED = 8, 9, 10, 11
-INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES, INFO_COEFFICIENTS, INFO_ISOTROPIC, INFO_FUSED = 1, 2, 3, 4, 5, 6
+(INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES, INFO_COEFFICIENTS, INFO_ISOTROPIC, INFO_FUSED,
+ INFO_CPML_TUPLES) = 1, 2, 3, 4, 5, 6, 7
 NUM_SRC = 9
 PW_S, PW_P = 1, 2
 BATCH_INFO_TILE_ROWS, BATCH_INFO_TILES, BATCH_INFO_XCD_ORDER = 1, 2, 3

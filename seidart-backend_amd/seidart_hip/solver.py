@@ -288,11 +288,13 @@ class Solver:
     def info(self, key):
         """'materials' (distinct materials in the table, 0 = per-cell arrays),
         'shell_tiles', 'device_bytes', 'coefficients', 'isotropic' (1 = the
-        3D elastic isotropic arrays are in use) or 'fused' (1 = the fused 3D
-        EM step is in use); valid after the first run()."""
+        3D elastic isotropic arrays are in use), 'fused' (1 = the fused 3D
+        EM step is in use) or 'cpml_tuples' (3D EM: distinct CPML coefficient
+        tuples held as a dictionary, 0 = per-cell arrays); valid after the
+        first run()."""
         k = {"materials": L.INFO_MATERIALS, "shell_tiles": L.INFO_SHELL_TILES,
              "device_bytes": L.INFO_DEVICE_BYTES, "coefficients": L.INFO_COEFFICIENTS,
-             "isotropic": L.INFO_ISOTROPIC, "fused": L.INFO_FUSED}[key]
+             "isotropic": L.INFO_ISOTROPIC, "fused": L.INFO_FUSED, "cpml_tuples": L.INFO_CPML_TUPLES}[key]
         v = C.c_int64(0)
         L.check(L.lib.sdt_get_info(self._h, k, C.byref(v)), self._h)
         return v.value

@@ -334,6 +334,35 @@ def test_em3_heterogeneous_permittivity():
     _check(g, o, TOL64)
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+@pytest.mark.parametrize("perturb", [False, True])
+def test_em3_cpml_coefficient_forms(fused, perturb):
+    """The CPML coefficients as a dictionary (a one-byte id per cell: the
+    layered profiles give few distinct tuples) and, once they differ in every
+    cell (more than 256 tuples), as the six per-cell arrays."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    from oracle.oracle_ctypes import run_oracle
+    cfg = models.em3(nx=30, ny=28, nz=26, cpml=5, steps=120, src=(15, 14, 9), nrec=8)
+    if perturb:
+        rng = np.random.default_rng(12)
+        for k in ("acoef_cpml", "bcoef_cpml", "acoef_half_cpml", "bcoef_half_cpml"):
+            a = cfg["arrays"][k]
+            cfg["arrays"][k] = np.asfortranarray(a * (1.0 + 1e-3 * rng.random(a.shape)))
+    fields = ("ex", "ey", "ez", "hx", "hy", "hz")
+    s = make_solver(cfg, options={"fused": fused})
+    try:
+        s.run()
+        g = (s.receivers(), *s.norms(), {f: s.field(f) for f in fields})
+        tuples = s.info("cpml_tuples")
+    finally:
+        s.close()
+    assert (tuples == 0) == perturb and tuples <= 256
+    o_rec, (o_vn, o_sn), o_f, us = run_oracle(cfg, fields=fields)
+    assert us == 0
+    _check(g, (o_rec, o_vn, o_sn, o_f), TOL64)
+
+
 @pytest.mark.parametrize("physics", ["elastic2", "elastic25", "elastic3", "em2", "em25", "em3"])
 def test_fp32_close_to_fp64(physics):
     """FP32 wavefields with FP64-precomputed coefficients: trace error vs the
