@@ -203,6 +203,8 @@ def timed_run(cfg, args, material_table, isotropic=1, precision=None, extra=None
     s.set_option("concurrent", args.concurrent)
     s.set_option("xcd_remap", args.xcd_remap)
     s.set_option("isotropic", isotropic)
+    if cfg["physics"] == "em3":
+        s.set_option("fused", args.fused)
     s.set_arrays(cfg["arrays"])
     for comp, series in cfg["sources"].items():
         series = np.asarray(series, dtype=np.float64)
@@ -990,6 +992,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per interior workgroup tile (0 = auto)")
     ap.add_argument("--material-table", type=int, default=1,
                     help="1: material ids + table when the model allows (default), 0: per-cell arrays")
+    ap.add_argument("--fused", type=int, default=-1, choices=[-1, 0, 1],
+                    help="3D EM: one kernel per step (1), one per half step (0), or by precision (-1)")
     ap.add_argument("--concurrent", type=int, default=0,
                     help="1: CPML-shell kernels on a second stream beside the interior kernels")
     ap.add_argument("--xcd-remap", type=int, default=0,

@@ -244,7 +244,9 @@ enum {
     SDT_OPT_XCD_REMAP = 7,      /* 3D elastic block order: bit 0 interior kernels, bit 1 shell
                                    kernels; set = each XCD takes a contiguous range of tiles,
                                    clear = the dispatcher's round-robin order (default 0: the
-                                   round-robin order measured faster at 280^3)               */
+                                   round-robin order measured faster at 280^3).  3D EM half-step
+                                   kernels: bit 0 set = column order, each XCD runs tile columns
+                                   through all planes in turn (default 0: measured slower)    */
     SDT_OPT_ISOTROPIC = 10,     /* 3D and 2.5D elastic: 1 = when the per-cell arrays are in use (no
                                    material table) and every cell's stiffness is isotropic in form
                                    (c11 = c22 = c33, c12 = c13 = c23, c44 = c55 = c66, all
@@ -255,8 +257,8 @@ enum {
                                    (E, H and the H-update memory variables are kept in two
                                    buffers each, the current one swapped every step; a third
                                    fewer bytes per step), 0 = one kernel per half step, fields
-                                   updated in place, -1 = fused for FP64, not for FP32
-                                   (default: the faster of the two for each precision)      */
+                                   updated in place, -1 = the half-step kernels (default: the
+                                   faster form in both precisions at 404 x 404 x 212)        */
     SDT_OPT_MATERIAL_TABLE = 5  /* every solver: 1 = when the model holds at most 256
                                    distinct material tuples (3D elastic: (c11..c66, gamma_*,
                                    rho); 2.5D and 2D elastic: (c11..c66, gamma_*), the density
@@ -299,8 +301,9 @@ enum {
     SDT_INFO_COEFFICIENTS = 4,  /* update coefficients per cell (2D EM 7, 2D elastic 10, else 0)  */
     SDT_INFO_ISOTROPIC = 5,     /* 1 = the isotropic arrays (c11, c12, c44) are in use (3D, 2.5D elastic) */
     SDT_INFO_FUSED = 6,         /* 1 = the fused 3D EM step is in use                      */
-    SDT_INFO_CPML_TUPLES = 7    /* 3D EM: distinct CPML coefficient tuples held as a one-byte
-                                   id per cell and a table, 0 = six per-cell arrays        */
+    SDT_INFO_CPML_TUPLES = 7    /* 3D EM and the 3D elastic CPML shell: distinct CPML
+                                   coefficient tuples held as a one-byte id per cell and a
+                                   table, 0 = six per-cell arrays (streams)               */
 };
 int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value);
 

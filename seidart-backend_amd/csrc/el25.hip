@@ -493,7 +493,8 @@ int el25_setup(Device &d, const double *const *in)
     d.ncomp = 2;
     int rc = compute_tile_flags(d, in);
     if (rc) return rc;
-    set_layout(d);
+    // the kernels' 64-wide tiles start at column 1: put it on a line boundary
+    set_layout(d, 1);
     const int64_t n = d.ncell;
     std::vector<double> zero((size_t)n, 0.0);
     // isotropic form: only c11, c12, c44 (Voigt slots 0, 1, 15) are held

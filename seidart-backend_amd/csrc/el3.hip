@@ -250,25 +250,28 @@ int el3_build_shell(Device &d, const double *const *in, int kc)
         if (upload_converted(d, cd.rows.data(), &d.shell_ctab, (int64_t)cd.rows.size()) != SDT_OK) return SDT_ERR_HIP;
         d.shell_ncoef = cd.count();
     }
-    // Shell storage, stream-major: stream q of slot s at q * nslots + s (the
-    // M_TOTAL memory variables, zero at the start, then without the
-    // dictionary the six coefficient streams).
+    // Shell storage in tile-plane blocks: the 256 slots of a tile-plane keep
+    // their snq streams together, stream q of slot s at
+    // ((s / 256) * snq + q) * 256 + s % 256 (the M_TOTAL memory variables,
+    // zero at the start, then without the dictionary the six coefficient
+    // streams).
     if (alloc_zeroed(d, &d.shell, (int64_t)d.shell_snq * d.nslots) != SDT_OK) return SDT_ERR_HIP;
     if (!dict) {
         std::vector<double> cs((size_t)d.nslots);
+        const size_t row = (size_t)kShellThreads * d.esize, nblk = (size_t)d.nslots / kShellThreads;
         for (int q = 0; q < 6; ++q) {
             for (int64_t slot = 0; slot < d.nslots; ++slot) {
                 const int *c = &cell[(size_t)slot * 3];
                 cs[(size_t)slot] = coef(q, c[0], c[1], c[2]);
             }
-            const size_t bytes = (size_t)d.nslots * d.esize;
-            char *dst = (char *)d.shell + (size_t)(S_CF + q) * bytes;
+            char *dst = (char *)d.shell + (size_t)(S_CF + q) * row;
+            const size_t dpitch = (size_t)d.shell_snq * row;
             hipError_t e;
             if (d.precision == SDT_FP64) {
-                e = hipMemcpy(dst, cs.data(), bytes, hipMemcpyHostToDevice);
+                e = hipMemcpy2D(dst, dpitch, cs.data(), row, row, nblk, hipMemcpyHostToDevice);
             } else {
                 std::vector<float> f(cs.begin(), cs.end());
-                e = hipMemcpy(dst, f.data(), bytes, hipMemcpyHostToDevice);
+                e = hipMemcpy2D(dst, dpitch, f.data(), row, row, nblk, hipMemcpyHostToDevice);
             }
             if (e != hipSuccess) return SDT_ERR_HIP;
         }

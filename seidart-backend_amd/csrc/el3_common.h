@@ -28,7 +28,7 @@ struct El3Args {
     // xy xz yz (6), rho, 4 unused
     const uint8_t *mid;
     const T *tab;
-    T *shell;       // CPML-shell storage: snq streams of nslots values (stream q of slot s at q * nslots + s)
+    T *shell;       // CPML-shell storage: per tile-plane (256 slots) a block of snq streams of 256 values
     int snq;        // streams: M_TOTAL memory variables (+ 6 coefficient streams without ctab)
     int64_t nslots;
     // CPML coefficient dictionary of the shell (at most 256 distinct

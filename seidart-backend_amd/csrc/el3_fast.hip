@@ -363,6 +363,16 @@ __device__ __forceinline__ ShellCell shell_cell(const El3Tile &tl, const int (&b
     return L;
 }
 
+// Element of stream q of shell slot `slot`: the snq streams of a tile-plane
+// (256 slots) are stored together as one block, stream after stream, so that
+// a tile's memory-variable traffic of a plane is one contiguous region.
+template <typename T>
+__device__ __forceinline__ int64_t shell_at(const El3Args<T> &A, int64_t slot, int q)
+{
+    static_assert(kShellThreads == 256, "tile-plane blocks of 256 slots");
+    return ((slot >> 8) * A.snq + q) * kShellThreads + (slot & (kShellThreads - 1));
+}
+
 // CPML coefficients a, b, 1/kappa, a_half, b_half, 1/kappa_half of a shell
 // slot: from the dictionary (one-byte id, cached rows of 8) or from the six
 // coefficient streams after the memory variables.
@@ -375,7 +385,7 @@ __device__ __forceinline__ void shell_coefs(const El3Args<T> &A, int64_t slot, T
         for (int r = 0; r < 6; ++r) cf[r] = row[r];
     } else {
 #pragma unroll
-        for (int r = 0; r < 6; ++r) cf[r] = ldnt(&A.shell[(S_CF + r) * A.nslots + slot]);
+        for (int r = 0; r < 6; ++r) cf[r] = ldnt(&A.shell[shell_at(A, slot, S_CF + r)]);
     }
 }
 
@@ -477,7 +487,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
     for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int s = 0; s < 4; ++s) q[m][s] = A.v[m][L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
-    const int64_t ns = A.nslots;
+    constexpr int64_t ns = kShellThreads;   // stream stride inside a tile-plane block
     double nrm = 0.0;
     int64_t slot = tl.base + threadIdx.x;
     for (int k = tl.k0; k <= tl.k1; ++k, slot += kShellThreads) {
@@ -496,7 +506,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
         const int mid = MAT == kMatTable ? (int)A.mid[c] : 0;
         T ic11 = T(0), ic12 = T(0), ic44 = T(0);
         if (MAT == kMatIso) { ic11 = ldnt(&A.C[0][c]); ic12 = ldnt(&A.C[1][c]); ic44 = ldnt(&A.C[15][c]); }
-        T *ps = A.shell + slot;
+        T *ps = A.shell + shell_at(A, slot, 0);
         T D[3][K_NKIND];
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
@@ -585,7 +595,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
     RhoPre<T> rz_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0 + 1)),
               rxm_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0) + L.xm1),
               rxp_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0) + L.xp1);
-    const int64_t ns = A.nslots;
+    constexpr int64_t ns = kShellThreads;   // stream stride inside a tile-plane block
     double nrm = 0.0;
     int64_t slot = tl.base + threadIdx.x;
     for (int k = tl.k0; k <= tl.k1; ++k, slot += kShellThreads) {
@@ -606,7 +616,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
         T cf[6];
         shell_coefs<T, CTAB>(A, slot, cf);
         const T a = cf[0], b = cf[1], ik = cf[2], ah = cf[3], bh = cf[4], ikh = cf[5];
-        T *ps = A.shell + slot;
+        T *ps = A.shell + shell_at(A, slot, 0);
         T pin[9];
 #pragma unroll
         for (int s = 0; s < 9; ++s) pin[s] = L.on ? ldnt(&ps[(M_SXX_X + s) * ns]) : T(0);

@@ -292,7 +292,8 @@ int em25_setup(Device &d, const double *const *in)
     d.ncomp = 2;
     int rc = compute_tile_flags(d, in);
     if (rc) return rc;
-    set_layout(d);
+    // the kernels' 64-wide tiles start at column 1: put it on a line boundary
+    set_layout(d, 1);
     const int64_t n = d.ncell;
     auto val = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
     std::vector<double> co[6];

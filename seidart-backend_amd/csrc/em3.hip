@@ -31,6 +31,7 @@ struct Em3Args {
     T *m_dHz_dy, *m_dHy_dz, *m_dHz_dx, *m_dHx_dz, *m_dHx_dy, *m_dHy_dx;
     const uint8_t *flags;
     int any_flags;      // 0: no tile inside the box holds a CPML cell (the flags need no load)
+    int col_order;      // half-step kernels: 1D grid in XCD column order
     int box[6];
     int nx, ny, nz, tiles_x, tiles_y, pitch;
     int64_t xoff;
@@ -50,6 +51,8 @@ struct Em3Args {
 };
 
 constexpr int kEmRow = 16;
+// planes per workgroup of the fused kernel when the z-chunk option is 0 (auto)
+constexpr int kEm3FusedChunk = 12;
 
 // A CPML cell's coefficients: from the dictionary when there is one (a
 // wave-uniform test of a kernel argument), else from the six arrays.
@@ -104,206 +107,24 @@ __device__ __forceinline__ void em3_e_update(const R &M, T exo, T eyo, T ezo, T 
     ez = ezo + (M[2] * rx + M[4] * ry + M[5] * rz) * dt;
 }
 
-// Hx, Hy, Hz (cpmlfdtd.f08:3700-3778).  Every operand is loaded before the
-// first store: the compiler cannot tell the arrays apart, so a store ahead of
-// a load would serialise the memory round trips of the three components.
-template <typename T, bool PML>
-__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
-{
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
-    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    const bool face = i == 1 || j == 1 || k == 1;
-    const T *Ex = A.E[0], *Ey = A.E[1], *Ez = A.E[2];
-    if (i == nx || j == ny || k == nz) {
-        A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0);
-        return;
-    }
-    const T ex0 = Ex[c], exy = Ex[c + sy], exz = Ex[c + sz];
-    const T ey0 = Ey[c], eyx = Ey[c + 1], eyz = Ey[c + sz];
-    const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
-    const T h0 = face ? T(0) : A.H[0][c], h1 = face ? T(0) : A.H[1][c], h2 = face ? T(0) : A.H[2][c];
-    T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
-    T m_eydz = 0, m_ezdy = 0, m_exdz = 0, m_ezdx = 0, m_exdy = 0, m_eydx = 0;
-    if (PML) {
-        em3_coefs(A, c, a, b, ik, ah, bh, ikh);
-        m_eydz = A.m_dEy_dz[c]; m_ezdy = A.m_dEz_dy[c];
-        m_exdz = A.m_dEx_dz[c]; m_ezdx = A.m_dEz_dx[c];
-        m_exdy = A.m_dEx_dy[c]; m_eydx = A.m_dEy_dx[c];
-    }
-    // Hx: cpmlfdtd.f08:3700-3718
-    T dEz_dy = (ezy - ez0) * A.rdy;
-    T dEy_dz = (eyz - ey0) * A.rdz;
-    if (PML) {
-        m_eydz = bh * m_eydz + ah * dEy_dz;
-        m_ezdy = bh * m_ezdy + ah * dEz_dy;
-        dEz_dy = dEz_dy * ikh + m_ezdy;
-        dEy_dz = dEy_dz * ikh + m_eydz;
-    }
-    const T hx = h0 + A.dbH * (dEy_dz - dEz_dy);
-    // Hy: cpmlfdtd.f08:3730-3749
-    T dEx_dz = (exz - ex0) * A.rdz;
-    T dEz_dx = (ezx - ez0) * A.rdx;
-    if (PML) {
-        m_exdz = b * m_exdz + a * dEx_dz;
-        m_ezdx = b * m_ezdx + a * dEz_dx;
-        dEx_dz = dEx_dz * ik + m_exdz;
-        dEz_dx = dEz_dx * ik + m_ezdx;
-    }
-    const T hy = h1 + A.dbH * (dEz_dx - dEx_dz);
-    // Hz: cpmlfdtd.f08:3761-3778 (k from 2)
-    T dEx_dy = (exy - ex0) * A.rdy;
-    T dEy_dx = (eyx - ey0) * A.rdx;
-    if (PML) {
-        m_exdy = b * m_exdy + a * dEx_dy;
-        m_eydx = b * m_eydx + a * dEy_dx;
-        dEx_dy = dEx_dy * ik + m_exdy;
-        dEy_dx = dEy_dx * ik + m_eydx;
-    }
-    const T hz = k >= 2 ? h2 + A.dbH * (dEx_dy - dEy_dx) : T(0);
-    A.H[0][c] = hx;
-    A.H[1][c] = hy;
-    A.H[2][c] = hz;
-    if (PML) {
-        A.m_dEy_dz[c] = m_eydz; A.m_dEz_dy[c] = m_ezdy;
-        A.m_dEx_dz[c] = m_exdz; A.m_dEz_dx[c] = m_ezdx;
-        if (k >= 2) { A.m_dEx_dy[c] = m_exdy; A.m_dEy_dx[c] = m_eydx; }
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    if (i > A.nx || j > A.ny) return;
-    const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-    if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) em3_h_cell<T, true>(A, i, j, k);
-    else em3_h_cell<T, false>(A, i, j, k);
-}
-
-// Ex, Ey, Ez (cpmlfdtd.f08:3795-3828), zero faces, and the field magnitude
-// for the norm; as in the H update, all loads come before the first store.
-template <typename T, bool PML, bool TAB>
-__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
-{
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
-    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    T ex, ey, ez;
-    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
-        ex = ey = ez = T(0);
-    } else {
-        const T *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
-        const T hx0 = Hx[c], hxy = Hx[c - sy], hxz = Hx[c - sz];
-        const T hy0 = Hy[c], hyx = Hy[c - 1], hyz = Hy[c - sz];
-        const T hz0 = Hz[c], hzx = Hz[c - 1], hzy = Hz[c - sy];
-        const T exo = A.E[0][c], eyo = A.E[1][c], ezo = A.E[2][c];
-        T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
-        T m_hzdy = 0, m_hydz = 0, m_hzdx = 0, m_hxdz = 0, m_hxdy = 0, m_hydx = 0;
-        if (PML) {
-            em3_coefs(A, c, a, b, ik, ah, bh, ikh);
-            m_hzdy = A.m_dHz_dy[c]; m_hydz = A.m_dHy_dz[c]; m_hzdx = A.m_dHz_dx[c];
-            m_hxdz = A.m_dHx_dz[c]; m_hxdy = A.m_dHx_dy[c]; m_hydx = A.m_dHy_dx[c];
-        }
-        T dHz_dy = (hz0 - hzy) * A.rdy;
-        T dHy_dz = (hy0 - hyz) * A.rdz;
-        T dHz_dx = (hz0 - hzx) * A.rdx;
-        T dHx_dz = (hx0 - hxz) * A.rdz;
-        T dHy_dx = (hy0 - hyx) * A.rdx;
-        T dHx_dy = (hx0 - hxy) * A.rdy;
-        if (PML) {
-            m_hzdy = bh * m_hzdy + ah * dHz_dy;
-            m_hydz = b * m_hydz + a * dHy_dz;
-            m_hzdx = bh * m_hzdx + ah * dHz_dx;
-            m_hxdz = bh * m_hxdz + ah * dHx_dz;
-            m_hxdy = bh * m_hxdy + ah * dHx_dy;
-            m_hydx = bh * m_hydx + ah * dHy_dx;
-            dHz_dy = dHz_dy * ikh + m_hzdy;
-            dHy_dz = dHy_dz * ik + m_hydz;
-            dHz_dx = dHz_dx * ikh + m_hzdx;
-            dHx_dz = dHx_dz * ikh + m_hxdz;
-            dHx_dy = dHx_dy * ikh + m_hxdy;
-            dHy_dx = dHy_dx * ikh + m_hydx;
-        }
-        const T cx = dHz_dy - dHy_dz, cy = dHx_dz - dHz_dx, cz = dHy_dx - dHx_dy;
-        if (TAB) {
-            const int m = A.mid[c];
-            const int m0 = __builtin_amdgcn_readfirstlane(m);
-            if (__all(m == m0)) {
-                const EmScalarRow<T> M{(em_const_row<T>)(A.tab) + m0 * kEmRow};
-                em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
-            } else {
-                const EmLaneRow<T> M{A.tab + m * kEmRow};
-                em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
-            }
-        } else {
-            const EmDenseRow<T> M{&A, c};
-            em3_e_update(M, exo, eyo, ezo, cx, cy, cz, A.dt, ex, ey, ez);
-        }
-        if (PML) {
-            A.m_dHz_dy[c] = m_hzdy; A.m_dHy_dz[c] = m_hydz; A.m_dHz_dx[c] = m_hzdx;
-            A.m_dHx_dz[c] = m_hxdz; A.m_dHx_dy[c] = m_hxdy; A.m_dHy_dx[c] = m_hydx;
-        }
-    }
-    A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
-    if (c == A.src_cell || pw_face(A.pwf, i, j, k)) return 0.0;
-    const double x = ex, y = ey, z = ez;
-    return sqrt(x * x + y * y + z * z);
-}
-
-template <typename T, bool TAB>
-__global__ void __launch_bounds__(256) em3_e_kernel(Em3Args<T> A)
-{
-    const int i = blockIdx.x * 64 + threadIdx.x + 1;
-    const int j = blockIdx.y * 4 + threadIdx.y + 1;
-    const int k = blockIdx.z + 1;
-    double nrm = 0.0;
-    if (i <= A.nx && j <= A.ny) {
-        const size_t t = blockIdx.x + (size_t)A.tiles_x * (blockIdx.y + (size_t)A.tiles_y * blockIdx.z);
-        if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) nrm = em3_e_cell<T, true, TAB>(A, i, j, k);
-        else nrm = em3_e_cell<T, false, TAB>(A, i, j, k);
-    }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
-}
-
-// ---- fused step ---------------------------------------------------------------
-// One kernel per step: a 64 x 4 tile marches kc planes; per plane each thread
-// updates H of its cell (cpmlfdtd.f08:3700-3778) into LDS and memory, the
-// tile also recomputes H of the column left of it and of the row below it
-// (the E update reads H at i-1 and j-1), and after one barrier each thread
-// updates E of its cell (3795-3828) from H of this plane (LDS) and of the
-// plane below (registers).  E, H and the H-update memory variables are read
-// from the previous step's buffers and written to the other ones, so a
-// halo recomputation never sees a value a neighbouring tile already
-// advanced.  Per cell and step the fields are read once and written once
-// (the two half-step kernels read H and E twice).
+// ---- cell updates shared by the half-step and fused kernels ----------------
+// The kernels read the previous step's E, H and H-update memory variables
+// through Eo, Ho, mo and write the new ones through E, H and m_dE*: the same
+// arrays in the half-step kernels (updates in place), the other buffers of a
+// pair in the fused kernel.
 template <typename T>
 __device__ __forceinline__ double fmax_nan3(double a, double b) { return (b > a || b != b) ? b : a; }
 
-// H of cell (i, j, k) after this step's H update; `store` writes it and the
-// cell's memory variables (the owner), otherwise they are only returned (a
-// halo cell).
+// H of a cell (cpmlfdtd.f08:3700-3778) from its operands: this step's E of
+// the cell and of its +x, +y, +z neighbours and the previous H (0 on the
+// lazily zeroed faces).  In CPML cells the coefficients and memory variables
+// are read here.  `store` writes H and the memory variables (the cell's
+// owner); otherwise they are only returned (a halo cell of the fused kernel).
 template <typename T>
-__device__ __forceinline__ void em3_h_new(const Em3Args<T> &A, int i, int j, int k, bool store, T &hx, T &hy, T &hz)
+__device__ __forceinline__ void em3_h_math(const Em3Args<T> &A, int64_t c, int k, bool pml, bool store, T ex0,
+                                           T ey0, T ez0, T exy, T ezy, T eyx, T ezx, T exz, T eyz, T h0, T h1,
+                                           T h2, T &hx, T &hy, T &hz)
 {
-    const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
-    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    if (i == nx || j == ny || k == nz) {
-        hx = hy = hz = T(0);
-        if (store) { A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0); }
-        return;
-    }
-    const bool face = i == 1 || j == 1 || k == 1;
-    const T *Ex = A.Eo[0], *Ey = A.Eo[1], *Ez = A.Eo[2];
-    const T ex0 = Ex[c], exy = Ex[c + sy], exz = Ex[c + sz];
-    const T ey0 = Ey[c], eyx = Ey[c + 1], eyz = Ey[c + sz];
-    const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
-    const T h0 = face ? T(0) : A.Ho[0][c], h1 = face ? T(0) : A.Ho[1][c], h2 = face ? T(0) : A.Ho[2][c];
-    const size_t t = (size_t)((i - 1) >> 6) + (size_t)A.tiles_x * ((size_t)((j - 1) >> 2) + (size_t)A.tiles_y * (k - 1));
-    const bool pml = !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]);
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
     T m_eydz = 0, m_ezdy = 0, m_exdz = 0, m_ezdx = 0, m_exdy = 0, m_eydx = 0;
     if (pml) {
@@ -330,7 +151,7 @@ __device__ __forceinline__ void em3_h_new(const Em3Args<T> &A, int i, int j, int
     }
     hx = h0 + A.dbH * (dEy_dz - dEz_dy);
     hy = h1 + A.dbH * (dEz_dx - dEx_dz);
-    hz = k >= 2 ? h2 + A.dbH * (dEx_dy - dEy_dx) : T(0);
+    hz = k >= 2 ? h2 + A.dbH * (dEx_dy - dEy_dx) : T(0);   // Hz nest from k = 2
     if (store) {
         A.H[0][c] = hx; A.H[1][c] = hy; A.H[2][c] = hz;
         if (pml) {
@@ -339,6 +160,30 @@ __device__ __forceinline__ void em3_h_new(const Em3Args<T> &A, int i, int j, int
             if (k >= 2) { A.m_dEx_dy[c] = m_exdy; A.m_dEy_dx[c] = m_eydx; }
         }
     }
+}
+
+// H of cell (i, j, k) after this step's H update, its operands loaded here
+// (the fused kernel).
+template <typename T>
+__device__ __forceinline__ void em3_h_new(const Em3Args<T> &A, int i, int j, int k, bool store, T &hx, T &hy, T &hz)
+{
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    if (i == nx || j == ny || k == nz) {
+        hx = hy = hz = T(0);
+        if (store) { A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0); }
+        return;
+    }
+    const bool face = i == 1 || j == 1 || k == 1;
+    const T *Ex = A.Eo[0], *Ey = A.Eo[1], *Ez = A.Eo[2];
+    const T ex0 = Ex[c], exy = Ex[c + sy], exz = Ex[c + sz];
+    const T ey0 = Ey[c], eyx = Ey[c + 1], eyz = Ey[c + sz];
+    const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
+    const T h0 = face ? T(0) : A.Ho[0][c], h1 = face ? T(0) : A.Ho[1][c], h2 = face ? T(0) : A.Ho[2][c];
+    const size_t t = (size_t)((i - 1) >> 6) + (size_t)A.tiles_x * ((size_t)((j - 1) >> 2) + (size_t)A.tiles_y * (k - 1));
+    const bool pml = !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]);
+    em3_h_math(A, c, k, pml, store, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
 }
 
 // E-update operands of a cell that do not depend on this step's H: the old
@@ -427,6 +272,112 @@ __device__ __forceinline__ double em3_e_new(const Em3Args<T> &A, int i, int j, i
     return sqrt(x * x + y * y + z * z);
 }
 
+// ---- half-step kernels ----------------------------------------------------
+// One kernel for the H update and one for the E update, a 64 x 4 tile of one
+// plane per workgroup.  Every operand is loaded before the first store: the
+// compiler cannot tell the arrays apart, so a store ahead of a load would
+// serialise the memory round trips.
+//
+// Block order.  The dispatcher deals workgroup b to XCD b % 8, and each XCD
+// has its own L2.  In the grid's own order a tile and the tiles above and
+// below it (whose E or H planes it reads) land on different XCDs.  In column
+// order (a 1D grid) XCD g takes the tile columns m = g, g + 8, g + 16, ... in
+// turn and runs each through every plane, so those tiles share one L2 at
+// about the same time.
+struct Em3Block {
+    int bx, by, bz;
+    bool valid;
+};
+
+template <typename T>
+__device__ __forceinline__ Em3Block em3_block(const Em3Args<T> &A)
+{
+    if (!A.col_order) return {(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, true};
+    const unsigned b = blockIdx.x, g = b & 7u, s = b >> 3;
+    const unsigned m = (s / (unsigned)A.nz) * 8u + g, kz = s % (unsigned)A.nz;
+    const unsigned ncol = (unsigned)A.tiles_x * (unsigned)A.tiles_y;
+    return {(int)(m % (unsigned)A.tiles_x), (int)(m / (unsigned)A.tiles_x), (int)kz, m < ncol};
+}
+
+// Hx, Hy, Hz of cell (i, j, k), cpmlfdtd.f08:3700-3778, updated in place.
+template <typename T, bool PML>
+__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
+{
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const bool face = i == 1 || j == 1 || k == 1;
+    const T *Ex = A.E[0], *Ey = A.E[1], *Ez = A.E[2];
+    if (i == nx || j == ny || k == nz) {   // outside every H nest
+        A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0);
+        return;
+    }
+    const T ex0 = Ex[c], exy = Ex[c + sy], exz = Ex[c + sz];
+    const T ey0 = Ey[c], eyx = Ey[c + 1], eyz = Ey[c + sz];
+    const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
+    const T h0 = face ? T(0) : A.H[0][c], h1 = face ? T(0) : A.H[1][c], h2 = face ? T(0) : A.H[2][c];
+    T hx, hy, hz;
+    em3_h_math(A, c, k, PML, true, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
+{
+    const Em3Block B = em3_block(A);
+    const int i = B.bx * 64 + threadIdx.x + 1;
+    const int j = B.by * 4 + threadIdx.y + 1;
+    const int k = B.bz + 1;
+    if (!B.valid || i > A.nx || j > A.ny) return;
+    const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
+    if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) em3_h_cell<T, true>(A, i, j, k);
+    else em3_h_cell<T, false>(A, i, j, k);
+}
+
+// Ex, Ey, Ez of cell (i, j, k), cpmlfdtd.f08:3795-3828, zero faces, in place;
+// returns the cell's |E| for the norm.
+template <typename T, bool PML, bool TAB>
+__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
+{
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
+    const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
+    const EPre<T> P = em3_e_pre<T, TAB>(A, c, PML);
+    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz)   // no H neighbour loads on faces
+        return em3_e_new<T, TAB>(A, i, j, k, P, T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0));
+    const T *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
+    const T hx0 = Hx[c], hxy = Hx[c - sy], hxz = Hx[c - sz];
+    const T hy0 = Hy[c], hyx = Hy[c - 1], hyz = Hy[c - sz];
+    const T hz0 = Hz[c], hzx = Hz[c - 1], hzy = Hz[c - sy];
+    return em3_e_new<T, TAB>(A, i, j, k, P, hx0, hy0, hz0, hxy, hyx, hzx, hzy, hxz, hyz);
+}
+
+template <typename T, bool TAB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) em3_e_kernel(Em3Args<T> A)
+{
+    const Em3Block B = em3_block(A);
+    const int i = B.bx * 64 + threadIdx.x + 1;
+    const int j = B.by * 4 + threadIdx.y + 1;
+    const int k = B.bz + 1;
+    double nrm = 0.0;
+    if (B.valid && i <= A.nx && j <= A.ny) {
+        const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
+        if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) nrm = em3_e_cell<T, true, TAB>(A, i, j, k);
+        else nrm = em3_e_cell<T, false, TAB>(A, i, j, k);
+    }
+    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+}
+
+// ---- fused step ---------------------------------------------------------------
+// One kernel per step: a 64 x TY tile marches kc planes; per plane each thread
+// updates H of its cell (cpmlfdtd.f08:3700-3778) into LDS and memory, the
+// tile also recomputes H of the column left of it and of the row below it
+// (the E update reads H at i-1 and j-1), and after one barrier each thread
+// updates E of its cell (3795-3828) from H of this plane (LDS) and of the
+// plane below (registers).  E, H and the H-update memory variables are read
+// from the previous step's buffers and written to the other ones, so a
+// halo recomputation never sees a value a neighbouring tile already
+// advanced.  Per cell and step the fields are read once and written once
+// (the two half-step kernels read H and E twice).
 // TY rows of 64 cells per tile, TY wavefronts: wavefront 0 also updates the
 // row below the tile, wavefront 1 (its first TY lanes) the column left of it.
 template <typename T, bool TAB, int TY>
@@ -503,6 +454,7 @@ static Em3Args<T> em3_args(Device &d)
     for (int q = 0; q < 12; ++q) *mm[q] = (T *)d.mem[q];
     A.flags = d.tile_flags;
     A.any_flags = d.nflagged > 0 ? 1 : 0;
+    A.col_order = 0;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
     A.pitch = d.pitch; A.xoff = d.xoff;
@@ -534,13 +486,16 @@ static void em3_fused_step_t(Device &d, bool timed)
         A.mo[q] = (const T *)d.mem[q];
         *mn[q] = (T *)d.mem[kAltMem + q];
     }
-    const int kc = d.zchunk > 0 ? d.zchunk : 6;
-    const int ty = d.tile_rows == 4 ? 4 : 8;
+    const int kc = d.zchunk > 0 ? d.zchunk : kEm3FusedChunk;
+    const int ty = d.tile_rows == 4 || d.tile_rows == 16 ? d.tile_rows : 8;
     const dim3 grid(d.tiles_x, (d.ny + ty - 1) / ty, (d.nz + kc - 1) / kc), block(64, ty, 1);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
     if (ty == 4) {
         if (d.mat_table) hipLaunchKernelGGL((em3_fused_kernel<T, true, 4>), grid, block, 0, d.stream, A, kc);
         else hipLaunchKernelGGL((em3_fused_kernel<T, false, 4>), grid, block, 0, d.stream, A, kc);
+    } else if (ty == 16) {
+        if (d.mat_table) hipLaunchKernelGGL((em3_fused_kernel<T, true, 16>), grid, block, 0, d.stream, A, kc);
+        else hipLaunchKernelGGL((em3_fused_kernel<T, false, 16>), grid, block, 0, d.stream, A, kc);
     } else {
         if (d.mat_table) hipLaunchKernelGGL((em3_fused_kernel<T, true, 8>), grid, block, 0, d.stream, A, kc);
         else hipLaunchKernelGGL((em3_fused_kernel<T, false, 8>), grid, block, 0, d.stream, A, kc);
@@ -572,8 +527,17 @@ static void em3_step_t(Device &d, bool timed)
         em3_fused_step_t<T>(d, timed);
         return;
     }
-    const Em3Args<T> A = em3_args<T>(d);
-    const dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z), block(64, 4, 1);
+    Em3Args<T> A = em3_args<T>(d);
+    for (int q = 0; q < 3; ++q) {   // updates in place
+        A.Eo[q] = A.E[q];
+        A.Ho[q] = A.H[q];
+    }
+    const T *mp[] = {A.m_dEy_dz, A.m_dEz_dy, A.m_dEx_dz, A.m_dEz_dx, A.m_dEx_dy, A.m_dEy_dx};
+    for (int q = 0; q < 6; ++q) A.mo[q] = mp[q];
+    A.col_order = d.xcd_remap & 1;
+    dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z);
+    if (A.col_order) grid = dim3(8 * ((d.tiles_x * d.tiles_y + 7) / 8) * d.tiles_z, 1, 1);
+    const dim3 block(64, 4, 1);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
     hipLaunchKernelGGL(em3_h_kernel<T>, grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
@@ -599,7 +563,9 @@ int em3_setup(Device &d, const double *const *in)
     d.tile_z = 1;
     int rc = compute_tile_flags(d, in);
     if (rc) return rc;
-    set_layout(d);
+    // the kernels' 64-wide tiles start at column 1: put it on a line boundary
+    // so that a tile row covers whole lines that no other tile touches
+    set_layout(d, 1);
     const int64_t n = d.ncell;
     auto val = [&](int id, int64_t c) { return in[id] ? in[id][c] : 0.0; };
     auto inv_eps = [&](int64_t c, double *co) {
@@ -651,10 +617,9 @@ int em3_setup(Device &d, const double *const *in)
         else rc = alloc_grid_zeroed(d, &d.field[f]);
         if (rc) return rc;
     }
-    // fused step: second buffers of E, H and the H-update memory variables
-    // auto: FP64 runs fused (1.62 vs 1.66 ms per step at 404 x 404 x 212);
-    // FP32 keeps the half-step kernels (1.00 vs 1.04 ms)
-    d.fused = d.fused_opt == 1 || (d.fused_opt < 0 && d.precision == SDT_FP64);
+    // fused step (on request): second buffers of E, H and the H-update
+    // memory variables
+    d.fused = d.fused_opt == 1;
     if (d.fused) {
         for (int f = SDT_F_EX; f <= SDT_F_HZ; ++f) {
             rc = alloc_grid_zeroed(d, &d.alt[f]);

@@ -752,7 +752,7 @@ int32_t sdt_get_info(sdt_handle *h, int32_t key, int64_t *value)
     case SDT_INFO_COEFFICIENTS: *value = h->d.adj.ncoef; return SDT_OK;
     case SDT_INFO_ISOTROPIC: *value = h->d.mat_iso ? 1 : 0; return SDT_OK;
     case SDT_INFO_FUSED: *value = h->d.fused ? 1 : 0; return SDT_OK;
-    case SDT_INFO_CPML_TUPLES: *value = h->d.cpml_ncoef; return SDT_OK;
+    case SDT_INFO_CPML_TUPLES: *value = h->d.physics == SDT_ELASTIC3 ? h->d.shell_ncoef : h->d.cpml_ncoef; return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown info key");
 }

@@ -342,10 +342,11 @@ int upload_cpml_full(Device &d, const double *const *in)
     return SDT_OK;
 }
 
-void set_layout(Device &d)
+void set_layout(Device &d, int first)
 {
     const bool empty = d.box[1] < d.box[0];
-    const host::RowLayout r = host::row_layout(d.nx, empty ? 1 : d.box[0], (int)d.esize);
+    if (first <= 0) first = empty ? 1 : d.box[0];
+    const host::RowLayout r = host::row_layout(d.nx, first, (int)d.esize);
     d.xoff = r.xoff;
     d.pitch = r.pitch;
     d.ncell_dev = (int64_t)d.pitch * d.ny * d.nz;

@@ -369,8 +369,9 @@ int upload_cpml_full(Device &d, const double *const *host_in);
 // The same coefficients as a dictionary (cpml_cid, cpml_ctab) when there are
 // at most 256 distinct tuples, otherwise as upload_cpml_full
 int upload_cpml_dict(Device &d, const double *const *host_in);
-// choose pitch / xoff from d.box (call after compute_tile_flags)
-void set_layout(Device &d);
+// choose pitch / xoff (call after compute_tile_flags): column `first` starts
+// a 128-byte line; 0 = the interior box's first column (d.box)
+void set_layout(Device &d, int first = 0);
 inline int64_t cell_index(const Device &d, int i, int j, int k)
 {
     return d.xoff + (int64_t)(i - 1) + (int64_t)d.pitch * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1));

@@ -276,8 +276,8 @@ class Solver:
         or 'raw_source' (1 = source series injected as given) or 'isotropic'
         (3D elastic without a material table: 1 = hold only c11, c12, c44 when
         every cell's stiffness has the isotropic form, 0 = all 21 arrays) or
-        'fused' (3D EM: 1 = one kernel per step, 0 = one per half step, -1 =
-        fused for FP64 only, the default)."""
+        'fused' (3D EM: 1 = one kernel per step, 0 or -1 = one per half step,
+        the default)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
              "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
              "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP,

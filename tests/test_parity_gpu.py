@@ -159,6 +159,35 @@ def test_elastic3_block_orders_agree(remap):
         assert np.array_equal(f0[k], f1[k]), k
 
 
+@pytest.mark.parametrize("perturb", [False, True])
+def test_elastic3_cpml_coefficient_forms(perturb):
+    """The CPML shell's coefficients as a dictionary (a one-byte id per slot)
+    and, once they differ in every cell (more than 256 tuples), as six
+    streams inside the tile-plane blocks of the memory variables."""
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    from oracle.oracle_ctypes import run_oracle
+    cfg = models.elastic3(nx=30, ny=26, nz=28, cpml=5, steps=100, src=(15, 13, 9), nrec=8)
+    if perturb:
+        rng = np.random.default_rng(13)
+        for k in ("acoef_cpml", "bcoef_cpml", "acoef_half_cpml", "bcoef_half_cpml"):
+            a = cfg["arrays"][k]
+            cfg["arrays"][k] = np.asfortranarray(a * (1.0 + 1e-3 * rng.random(a.shape)))
+    fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    print(f"[parity] elastic3 CPML coefficients perturbed={perturb}", flush=True)
+    s = make_solver(cfg)
+    try:
+        s.run()
+        g = (s.receivers(), *s.norms(), {f: s.field(f) for f in fields})
+        tuples = s.info("cpml_tuples")
+    finally:
+        s.close()
+    assert (tuples == 0) == perturb and tuples <= 256
+    o_rec, (o_vn, o_sn), o_f, us = run_oracle(cfg, fields=fields)
+    assert us == 0
+    _check(g, (o_rec, o_vn, o_sn, o_f), TOL64, stress_norm=True)
+
+
 def test_elastic3_interior_and_cpml_paths_agree():
     """A vanishing but non-zero CPML coefficient everywhere forces every tile
     through the CPML body; results must equal the face-specialised run."""
