@@ -300,8 +300,8 @@ __device__ __forceinline__ Em3Block em3_block(const Em3Args<T> &A)
 }
 
 // Hx, Hy, Hz of cell (i, j, k), cpmlfdtd.f08:3700-3778, updated in place.
-template <typename T, bool PML>
-__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k)
+template <typename T>
+__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k, bool pml)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
@@ -317,7 +317,7 @@ __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, in
     const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
     const T h0 = face ? T(0) : A.H[0][c], h1 = face ? T(0) : A.H[1][c], h2 = face ? T(0) : A.H[2][c];
     T hx, hy, hz;
-    em3_h_math(A, c, k, PML, true, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
+    em3_h_math(A, c, k, pml, true, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
 }
 
 template <typename T>
@@ -329,19 +329,21 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
     const int k = B.bz + 1;
     if (!B.valid || i > A.nx || j > A.ny) return;
     const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
-    if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) em3_h_cell<T, true>(A, i, j, k);
-    else em3_h_cell<T, false>(A, i, j, k);
+    // one code path with a per-lane CPML flag: a wavefront holding both kinds
+    // of cell (the x faces) issues its field loads once, the CPML lanes add
+    // theirs
+    em3_h_cell(A, i, j, k, !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
 }
 
 // Ex, Ey, Ez of cell (i, j, k), cpmlfdtd.f08:3795-3828, zero faces, in place;
 // returns the cell's |E| for the norm.
-template <typename T, bool PML, bool TAB>
-__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k)
+template <typename T, bool TAB>
+__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k, bool pml)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
     const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
-    const EPre<T> P = em3_e_pre<T, TAB>(A, c, PML);
+    const EPre<T> P = em3_e_pre<T, TAB>(A, c, pml);
     if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz)   // no H neighbour loads on faces
         return em3_e_new<T, TAB>(A, i, j, k, P, T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0));
     const T *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
@@ -361,8 +363,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) e
     double nrm = 0.0;
     if (B.valid && i <= A.nx && j <= A.ny) {
         const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
-        if (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t])) nrm = em3_e_cell<T, true, TAB>(A, i, j, k);
-        else nrm = em3_e_cell<T, false, TAB>(A, i, j, k);
+        nrm = em3_e_cell<T, TAB>(A, i, j, k, !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
     }
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }

@@ -307,9 +307,13 @@ def test_em2_material_table_equals_per_cell_arrays():
         assert rel_l2(f1[k], f0[k]) < 1e-13, k
 
 
-@pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"zchunk": 5}, {"zchunk": 1, "material_table": 0},
-                                     {"fused": 0}, {"fused": 0, "material_table": 0}, {"tile_rows": 4, "zchunk": 7}])
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"xcd_remap": 1}, {"xcd_remap": 1, "material_table": 0},
+                                     {"fused": 1}, {"fused": 1, "material_table": 0}, {"fused": 1, "zchunk": 5},
+                                     {"fused": 1, "tile_rows": 4, "zchunk": 7},
+                                     {"fused": 1, "tile_rows": 16, "zchunk": 3}])
 def test_em3_fp64(options):
+    """Half-step kernels (default; grid order and XCD column order) and the
+    fused step (chunks of planes, 4-, 8- and 16-row tiles)."""
     from seidart_hip import models
     cfg = models.em3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=8)
     cfg["sources"]["y"] = 0.5 * cfg["sources"]["x"]
@@ -318,7 +322,7 @@ def test_em3_fp64(options):
     _check(g, o, TOL64)
 
 
-@pytest.mark.parametrize("options", [{}, {"material_table": 0}])
+@pytest.mark.parametrize("options", [{}, {"material_table": 0}, {"xcd_remap": 1}, {"fused": 1}])
 def test_em3_odd_plane_count(options):
     """An odd padded grid (46 x 42 x 39) with its point source near the
     bottom CPML slab."""
