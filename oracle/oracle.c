@@ -15,10 +15,23 @@
  *
  * Pinning (what fixes each part independently of this text) is listed in the
  * test files: tests/test_oracle_pins.py (linear-field patch tests, CPML memory
- * recursion closed form, d'Alembert/Christoffel plane waves in the corrected
- * stencil mode, Yee cavity eigenfrequency, conductive decay, linearity and
- * superposition, source offsets).  The 3-term velocity-stencil weights of the
- * reference mode are pinned only by the cited reference lines.
+ * recursion closed form with distinct full and half sets, Christoffel plane
+ * waves in the corrected stencil mode, Yee plane waves, EM one-step closed
+ * forms, conductive decay, linearity and superposition, scalar_mean against
+ * textbook means, blow-up threshold), tests/test_oracle_pins_25.py (2.5D:
+ * ky = 0 against 2D, the y-mode oscillators, array_averaging2d impulse
+ * responses), tests/test_oracle_pins_pw.py (plane waves), and
+ * tests/test_oracle_stencil_pins.py (the reference-mode 3-term velocity
+ * stencils of 2D, 2.5D and 3D differ from the pinned corrected mode by
+ * exactly the omitted term).
+ *
+ * Parity unpinned (restated from the cited reference lines only; nothing
+ * outside the reference fixes them and the reference cannot be built here):
+ * which coefficient set (full or half CPML profile) each derivative of each
+ * 3D and 2.5D nest takes, the uncorrected dvx_dz of 3D sigma_xy
+ * (cpmlfdtd.f08:1858-1865), the asymmetric source offsets and the dt-less 2D
+ * velocity source (519-523, 1268-1276, 2208-2219), and the assignment of the
+ * smeared 2.5D densities to velocity terms (889-898, 1141-1198).
  *
  * Index helpers take the reference's 1-based (i, j, k).
  */
