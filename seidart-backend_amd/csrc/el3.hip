@@ -200,7 +200,10 @@ int el3_setup(Device &d, const double *const *in)
     t.sfix[3] = cell(is, js + 1, ks + 1);
     t.rec_field[0] = SDT_F_VX; t.rec_field[1] = SDT_F_VY; t.rec_field[2] = SDT_F_VZ;
 
-    return el3_build_shell(d, in, d.zchunk > 0 ? d.zchunk : 8);
+    // planes per shell tile: short chunks keep the planes that the resident
+    // shell workgroups touch at one time close together (280^3 FP64: 1.891 ms
+    // per step at 3 planes, 1.902 at 4, 1.920 at 8, 1.949 at 16)
+    return el3_build_shell(d, in, d.zchunk > 0 ? d.zchunk : kEl3ShellChunk);
 }
 
 // CPML-shell tiles (see El3Tile) and their compact memory-variable and CPML

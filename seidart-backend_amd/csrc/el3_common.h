@@ -15,6 +15,8 @@ enum { M_SXX_X = 21, M_SXY_Y, M_SXZ_Z, M_SXY_X, M_SYY_Y, M_SYZ_Z, M_SXZ_X, M_SYZ
 // streams of a shell tile-plane block: the M_TOTAL memory variables, then the
 // CPML coefficients a, b, 1/kappa, a_half, b_half, 1/kappa_half
 enum { S_CF = M_TOTAL, S_NQ = M_TOTAL + 6 };
+// planes per z-marching tile of the CPML shell when the z-chunk option is 0
+constexpr int kEl3ShellChunk = 3;
 
 template <typename T>
 struct El3Args {
