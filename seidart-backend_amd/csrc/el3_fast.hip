@@ -671,7 +671,15 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
 }
 
 // ----------------------------------------------------------------- launchers
-static int el3_kc(const Device &d) { return d.zchunk > 0 ? d.zchunk : 8; }
+// planes per interior workgroup when the z-chunk option is 0: 8, except the
+// stress pass with per-cell material arrays (27 more streams per cell), which
+// runs faster with the resident workgroups on fewer planes (280^3 FP64: stress
+// pass 2.131 ms at 3 planes against 2.189 at 8; the velocity passes and the
+// table layout's stress pass are fastest at 8)
+static int el3_kc(const Device &d, bool dense_stress = false)
+{
+    return d.zchunk > 0 ? d.zchunk : dense_stress ? 3 : 8;
+}
 
 static int el3_ty(const Device &d) { return d.tile_rows == 8 || d.tile_rows == 16 ? d.tile_rows : 4; }
 
@@ -721,7 +729,7 @@ static void shell_stream_end(Device &d, int pass)
 template <typename T, int MAT>
 static void fast_stress(Device &d, const El3Args<T> &A)
 {
-    const int kc = el3_kc(d), ty = el3_ty(d);
+    const int kc = el3_kc(d, MAT == kMatDense), ty = el3_ty(d);
     const hipStream_t ss = shell_stream_begin(d, 0);
     if (d.ntiles > 0) {
         if (A.ctab)
