@@ -40,6 +40,7 @@ static El3Args<T> el3_args(Device &d)
     A.shell = (T *)d.shell;
     A.remap = d.xcd_remap;
     A.ig_same = d.ig_same ? 1 : 0;
+    A.edge = d.concurrent ? 0 : 1;
     A.snq = d.shell_snq;
     A.nslots = d.nslots;
     A.cid = d.shell_cid;

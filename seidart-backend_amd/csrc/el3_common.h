@@ -47,6 +47,9 @@ struct El3Args {
     int density_code, corrected;
     int remap;      // bit 0: XCD-contiguous block order of the interior kernels, bit 1: of the shell kernels
     int ig_same;    // 1: the six ig arrays hold equal values (one array, loaded once per cell)
+    int edge;       // 1: the shell kernels of a pass finish before its interior kernels start, and the
+                    // interior kernels store column box[1] + 1 again (the shell's new value) so that
+                    // the line holding their last column is written whole
     const int *d_it;
     unsigned long long *vnorm, *snorm;
     int64_t vfix[3], sfix[4];
