@@ -121,9 +121,9 @@ __device__ __forceinline__ double fmax_nan3(double a, double b) { return (b > a 
 // are read here.  `store` writes H and the memory variables (the cell's
 // owner); otherwise they are only returned (a halo cell of the fused kernel).
 template <typename T>
-__device__ __forceinline__ void em3_h_math(const Em3Args<T> &A, int64_t c, int k, bool pml, bool store, T ex0,
-                                           T ey0, T ez0, T exy, T ezy, T eyx, T ezx, T exz, T eyz, T h0, T h1,
-                                           T h2, T &hx, T &hy, T &hz)
+__device__ __forceinline__ void em3_h_math(const Em3Args<T> &A, int64_t c, int k, bool pml, bool store, bool mst,
+                                           T ex0, T ey0, T ez0, T exy, T ezy, T eyx, T ezx, T exz, T eyz, T h0,
+                                           T h1, T h2, T &hx, T &hy, T &hz)
 {
     T a = 0, b = 0, ik = 1, ah = 0, bh = 0, ikh = 1;
     T m_eydz = 0, m_ezdy = 0, m_exdz = 0, m_ezdx = 0, m_exdy = 0, m_eydx = 0;
@@ -154,7 +154,7 @@ __device__ __forceinline__ void em3_h_math(const Em3Args<T> &A, int64_t c, int k
     hz = k >= 2 ? h2 + A.dbH * (dEx_dy - dEy_dx) : T(0);   // Hz nest from k = 2
     if (store) {
         A.H[0][c] = hx; A.H[1][c] = hy; A.H[2][c] = hz;
-        if (pml) {
+        if (mst) {   // 0 from the lanes without CPML (their memory variables stay 0)
             A.m_dEy_dz[c] = m_eydz; A.m_dEz_dy[c] = m_ezdy;
             A.m_dEx_dz[c] = m_exdz; A.m_dEz_dx[c] = m_ezdx;
             if (k >= 2) { A.m_dEx_dy[c] = m_exdy; A.m_dEy_dx[c] = m_eydx; }
@@ -183,7 +183,7 @@ __device__ __forceinline__ void em3_h_new(const Em3Args<T> &A, int i, int j, int
     const T h0 = face ? T(0) : A.Ho[0][c], h1 = face ? T(0) : A.Ho[1][c], h2 = face ? T(0) : A.Ho[2][c];
     const size_t t = (size_t)((i - 1) >> 6) + (size_t)A.tiles_x * ((size_t)((j - 1) >> 2) + (size_t)A.tiles_y * (k - 1));
     const bool pml = !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]);
-    em3_h_math(A, c, k, pml, store, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
+    em3_h_math(A, c, k, pml, store, pml, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
 }
 
 // E-update operands of a cell that do not depend on this step's H: the old
@@ -218,15 +218,20 @@ __device__ __forceinline__ EPre<T> em3_e_pre(const Em3Args<T> &A, int64_t c, boo
 
 // E of cell (i, j, k) from this step's H (own cell; x- and y-neighbours; the
 // plane below) and the preloaded operands; returns the cell's |E| for the norm.
-template <typename T, bool TAB>
+// Columns past nx (the row padding) take 0 like the faces.  The memory
+// variables are stored by the CPML cells (WL = false) or, with WL, by every
+// lane with mst set (0 from the lanes without CPML, whose memory variables
+// stay 0), so that a line holding CPML cells is written whole.
+template <typename T, bool TAB, bool WL = false>
 __device__ __forceinline__ double em3_e_new(const Em3Args<T> &A, int i, int j, int k, const EPre<T> &P, T hx0,
-                                            T hy0, T hz0, T hxy, T hyx, T hzx, T hzy, T hxz, T hyz)
+                                            T hy0, T hz0, T hxy, T hyx, T hzx, T hzy, T hxz, T hyz, bool mst = false)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
     const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
     T ex, ey, ez;
-    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz) {
+    T m_hzdy = 0, m_hydz = 0, m_hzdx = 0, m_hxdz = 0, m_hxdy = 0, m_hydx = 0;
+    if (i == 1 || i >= nx || j == 1 || j == ny || k == 1 || k == nz) {
         ex = ey = ez = T(0);
     } else {
         T dHz_dy = (hz0 - hzy) * A.rdy;
@@ -236,20 +241,22 @@ __device__ __forceinline__ double em3_e_new(const Em3Args<T> &A, int i, int j, i
         T dHy_dx = (hy0 - hyx) * A.rdx;
         T dHx_dy = (hx0 - hxy) * A.rdy;
         if (P.pml) {
-            const T m_hzdy = P.bh * P.mhzdy + P.ah * dHz_dy;
-            const T m_hydz = P.b * P.mhydz + P.a * dHy_dz;
-            const T m_hzdx = P.bh * P.mhzdx + P.ah * dHz_dx;
-            const T m_hxdz = P.bh * P.mhxdz + P.ah * dHx_dz;
-            const T m_hxdy = P.bh * P.mhxdy + P.ah * dHx_dy;
-            const T m_hydx = P.bh * P.mhydx + P.ah * dHy_dx;
+            m_hzdy = P.bh * P.mhzdy + P.ah * dHz_dy;
+            m_hydz = P.b * P.mhydz + P.a * dHy_dz;
+            m_hzdx = P.bh * P.mhzdx + P.ah * dHz_dx;
+            m_hxdz = P.bh * P.mhxdz + P.ah * dHx_dz;
+            m_hxdy = P.bh * P.mhxdy + P.ah * dHx_dy;
+            m_hydx = P.bh * P.mhydx + P.ah * dHy_dx;
             dHz_dy = dHz_dy * P.ikh + m_hzdy;
             dHy_dz = dHy_dz * P.ik + m_hydz;
             dHz_dx = dHz_dx * P.ikh + m_hzdx;
             dHx_dz = dHx_dz * P.ikh + m_hxdz;
             dHx_dy = dHx_dy * P.ikh + m_hxdy;
             dHy_dx = dHy_dx * P.ikh + m_hydx;
-            A.m_dHz_dy[c] = m_hzdy; A.m_dHy_dz[c] = m_hydz; A.m_dHz_dx[c] = m_hzdx;
-            A.m_dHx_dz[c] = m_hxdz; A.m_dHx_dy[c] = m_hxdy; A.m_dHy_dx[c] = m_hydx;
+            if (!WL) {
+                A.m_dHz_dy[c] = m_hzdy; A.m_dHy_dz[c] = m_hydz; A.m_dHz_dx[c] = m_hzdx;
+                A.m_dHx_dz[c] = m_hxdz; A.m_dHx_dy[c] = m_hxdy; A.m_dHy_dx[c] = m_hydx;
+            }
         }
         const T cx = dHz_dy - dHy_dz, cy = dHx_dz - dHz_dx, cz = dHy_dx - dHx_dy;
         if constexpr (TAB) {
@@ -267,6 +274,10 @@ __device__ __forceinline__ double em3_e_new(const Em3Args<T> &A, int i, int j, i
         }
     }
     A.E[0][c] = ex; A.E[1][c] = ey; A.E[2][c] = ez;
+    if (WL && mst) {
+        A.m_dHz_dy[c] = m_hzdy; A.m_dHy_dz[c] = m_hydz; A.m_dHz_dx[c] = m_hzdx;
+        A.m_dHx_dz[c] = m_hxdz; A.m_dHx_dy[c] = m_hxdy; A.m_dHy_dx[c] = m_hydx;
+    }
     if (c == A.src_cell || pw_face(A.pwf, i, j, k)) return 0.0;
     const double x = ex, y = ey, z = ez;
     return sqrt(x * x + y * y + z * z);
@@ -299,17 +310,38 @@ __device__ __forceinline__ Em3Block em3_block(const Em3Args<T> &A)
     return {(int)(m % (unsigned)A.tiles_x), (int)(m / (unsigned)A.tiles_x), (int)kz, m < ncol};
 }
 
-// Hx, Hy, Hz of cell (i, j, k), cpmlfdtd.f08:3700-3778, updated in place.
+// Whole-line stores.  A 128-byte line that a kernel writes only in part costs
+// the memory a read of the line besides the write.  In the half-step kernels
+// each wavefront holds one row segment starting on a line, so the lanes of a
+// line are 128 / sizeof(T) consecutive lanes: the lanes over the row padding
+// (columns past nx) store zeros with the row, and when a line holds a CPML
+// cell every lane of it stores the memory variables (0 from the lanes without
+// CPML: their memory variables stay 0, as cpmlfdtd.f08's do where a = 0).
 template <typename T>
-__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k, bool pml)
+__device__ __forceinline__ bool em3_line_any(bool p)
+{
+    constexpr int L = 128 / (int)sizeof(T);
+    const uint64_t m = __ballot(p);
+    const int base = (int)(threadIdx.x & 63u) & ~(L - 1);
+    return ((m >> base) & (L >= 64 ? ~0ull : ((1ull << L) - 1))) != 0;
+}
+
+// Hx, Hy, Hz of cell (i, j, k), cpmlfdtd.f08:3700-3778, updated in place;
+// columns past nx (the row padding) take 0 like the cells outside every nest.
+template <typename T>
+__device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, int k, bool pml, bool mst)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
     const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
     const bool face = i == 1 || j == 1 || k == 1;
     const T *Ex = A.E[0], *Ey = A.E[1], *Ez = A.E[2];
-    if (i == nx || j == ny || k == nz) {   // outside every H nest
+    if (i >= nx || j == ny || k == nz) {   // outside every H nest
         A.H[0][c] = T(0); A.H[1][c] = T(0); A.H[2][c] = T(0);
+        if (mst) {
+            A.m_dEy_dz[c] = T(0); A.m_dEz_dy[c] = T(0); A.m_dEx_dz[c] = T(0); A.m_dEz_dx[c] = T(0);
+            if (k >= 2) { A.m_dEx_dy[c] = T(0); A.m_dEy_dx[c] = T(0); }
+        }
         return;
     }
     const T ex0 = Ex[c], exy = Ex[c + sy], exz = Ex[c + sz];
@@ -317,7 +349,7 @@ __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, in
     const T ez0 = Ez[c], ezx = Ez[c + 1], ezy = Ez[c + sy];
     const T h0 = face ? T(0) : A.H[0][c], h1 = face ? T(0) : A.H[1][c], h2 = face ? T(0) : A.H[2][c];
     T hx, hy, hz;
-    em3_h_math(A, c, k, pml, true, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
+    em3_h_math(A, c, k, pml, true, mst, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
 }
 
 template <typename T>
@@ -327,30 +359,34 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
     const int i = B.bx * 64 + threadIdx.x + 1;
     const int j = B.by * 4 + threadIdx.y + 1;
     const int k = B.bz + 1;
-    if (!B.valid || i > A.nx || j > A.ny) return;
+    const bool row = B.valid && i <= A.pitch - (int)A.xoff && j <= A.ny;   // the cells and the row padding
     const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
     // one code path with a per-lane CPML flag: a wavefront holding both kinds
     // of cell (the x faces) issues its field loads once, the CPML lanes add
     // theirs
-    em3_h_cell(A, i, j, k, !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
+    const bool pml = row && i < A.nx && j < A.ny && k < A.nz &&
+                     (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
+    const bool mst = em3_line_any<T>(pml);
+    if (!row) return;
+    em3_h_cell(A, i, j, k, pml, mst);
 }
 
-// Ex, Ey, Ez of cell (i, j, k), cpmlfdtd.f08:3795-3828, zero faces, in place;
-// returns the cell's |E| for the norm.
+// Ex, Ey, Ez of cell (i, j, k), cpmlfdtd.f08:3795-3828, zero faces (and row
+// padding), in place; returns the cell's |E| for the norm.
 template <typename T, bool TAB>
-__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k, bool pml)
+__device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k, bool pml, bool mst)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
     const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
     const EPre<T> P = em3_e_pre<T, TAB>(A, c, pml);
-    if (i == 1 || i == nx || j == 1 || j == ny || k == 1 || k == nz)   // no H neighbour loads on faces
-        return em3_e_new<T, TAB>(A, i, j, k, P, T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0));
+    if (i == 1 || i >= nx || j == 1 || j == ny || k == 1 || k == nz)   // no H neighbour loads on faces
+        return em3_e_new<T, TAB, true>(A, i, j, k, P, T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), mst);
     const T *Hx = A.H[0], *Hy = A.H[1], *Hz = A.H[2];
     const T hx0 = Hx[c], hxy = Hx[c - sy], hxz = Hx[c - sz];
     const T hy0 = Hy[c], hyx = Hy[c - 1], hyz = Hy[c - sz];
     const T hz0 = Hz[c], hzx = Hz[c - 1], hzy = Hz[c - sy];
-    return em3_e_new<T, TAB>(A, i, j, k, P, hx0, hy0, hz0, hxy, hyx, hzx, hzy, hxz, hyz);
+    return em3_e_new<T, TAB, true>(A, i, j, k, P, hx0, hy0, hz0, hxy, hyx, hzx, hzy, hxz, hyz, mst);
 }
 
 template <typename T, bool TAB>
@@ -360,11 +396,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) e
     const int i = B.bx * 64 + threadIdx.x + 1;
     const int j = B.by * 4 + threadIdx.y + 1;
     const int k = B.bz + 1;
+    const bool row = B.valid && i <= A.pitch - (int)A.xoff && j <= A.ny;   // the cells and the row padding
+    const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
+    // CPML cells of the E nests (the faces take 0 and no memory variables)
+    const bool pml = row && i > 1 && i < A.nx && j > 1 && j < A.ny && k > 1 && k < A.nz &&
+                     (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
+    const bool mst = em3_line_any<T>(pml);
     double nrm = 0.0;
-    if (B.valid && i <= A.nx && j <= A.ny) {
-        const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
-        nrm = em3_e_cell<T, TAB>(A, i, j, k, !in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
-    }
+    if (row) nrm = em3_e_cell<T, TAB>(A, i, j, k, pml, mst);
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);
 }
 
