@@ -20,9 +20,10 @@
 //
 // Whole-line stores.  A 128-byte line that a kernel writes only in part costs
 // the memory a read of the line besides the write, so the kernels write whole
-// lines where they can: every lane of a shell wavefront stores its memory
-// variables (lanes without a cell, or outside the nest ranges that read a
-// variable, store into slots no update reads), and the interior kernels store
+// lines where they can: every lane of a memory-variable line (16 slots in
+// FP64) that holds a cell stores its memory variables (lanes without a cell,
+// or outside the nest ranges that read a variable, store into slots no update
+// reads; lines without a cell are skipped), and the interior kernels store
 // the shell cell next to their last column again, with the value the shell
 // kernel of the same pass gave it, so that the line holding a row's last
 // interior cells is written by one store (280^3 FP64: 1.894 -> 1.758 ms per
@@ -505,6 +506,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
     const ShellCell L = shell_cell(tl, A.box, nx, ny, sy, A.xoff);
+    const bool mst = line_any<T>(L.on);   // memory-variable lines holding a cell (stored whole)
     const int i = L.i, j = L.j;
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
     T q[3][5];
@@ -556,13 +558,15 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
             if (m == 0) { D[m][K_ZMH] = dzm; P[K_ZMH] = T(0); }
             else D[m][K_ZMH] = cpml_upd(dzm, pin[K_ZMH], ah, bh, ikh, P[K_ZMH]);
             D[m][K_ZPH] = cpml_upd(dzp, pin[K_ZPH], ah, bh, ikh, P[K_ZPH]);
-            stnt(&ps[(m * K_NKIND + K_XPH) * ns], P[K_XPH]);
-            stnt(&ps[(m * K_NKIND + K_XMF) * ns], P[K_XMF]);
-            stnt(&ps[(m * K_NKIND + K_YMF) * ns], P[K_YMF]);
-            stnt(&ps[(m * K_NKIND + K_YPH) * ns], P[K_YPH]);
-            stnt(&ps[(m * K_NKIND + K_ZMF) * ns], P[K_ZMF]);
-            if (m != 0) stnt(&ps[(m * K_NKIND + K_ZMH) * ns], P[K_ZMH]);
-            stnt(&ps[(m * K_NKIND + K_ZPH) * ns], P[K_ZPH]);
+            if (mst) {
+                stnt(&ps[(m * K_NKIND + K_XPH) * ns], P[K_XPH]);
+                stnt(&ps[(m * K_NKIND + K_XMF) * ns], P[K_XMF]);
+                stnt(&ps[(m * K_NKIND + K_YMF) * ns], P[K_YMF]);
+                stnt(&ps[(m * K_NKIND + K_YPH) * ns], P[K_YPH]);
+                stnt(&ps[(m * K_NKIND + K_ZMF) * ns], P[K_ZMF]);
+                if (m != 0) stnt(&ps[(m * K_NKIND + K_ZMH) * ns], P[K_ZMH]);
+                stnt(&ps[(m * K_NKIND + K_ZPH) * ns], P[K_ZPH]);
+            }
             q[m][0] = q[m][1]; q[m][1] = q[m][2]; q[m][2] = q[m][3]; q[m][3] = q[m][4];
         }
         T sn[6];
@@ -603,6 +607,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int64_t sy = A.pitch, sz = (int64_t)A.pitch * ny;
     const ShellCell L = shell_cell(tl, A.box, nx, ny, sy, A.xoff);
+    const bool mst = line_any<T>(L.on);   // memory-variable lines holding a cell (stored whole)
     const int i = L.i, j = L.j;
     const T rx = A.rd24[0], ry = A.rd24[1], rz = A.rd24[2], dt = A.dt;
     const int dc = A.density_code;
@@ -671,11 +676,13 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
         const bool inx = L.on && !face && i >= 3 && j >= 3 && k >= 3;
         const bool iny = L.on && !face && i <= nx - 2 && j <= ny - 2 && k >= 3;
         const bool inz = L.on && !face && i <= nx - 2 && j >= 3 && k <= nz - 2;
-        // memory variables from every lane (whole lines); only the cells of
-        // the update ranges below ever read theirs back
-        stnt(&ps[M_SXX_X * ns], p[0]); stnt(&ps[M_SXY_Y * ns], p[1]); stnt(&ps[M_SXZ_Z * ns], p[2]);
-        stnt(&ps[M_SXY_X * ns], p[3]); stnt(&ps[M_SYY_Y * ns], p[4]); stnt(&ps[M_SYZ_Z * ns], p[5]);
-        stnt(&ps[M_SXZ_X * ns], p[6]); stnt(&ps[M_SYZ_Y * ns], p[7]); stnt(&ps[M_SZZ_Z * ns], p[8]);
+        // memory variables from every lane of a line holding a cell (whole
+        // lines); only the cells of the update ranges below read theirs back
+        if (mst) {
+            stnt(&ps[M_SXX_X * ns], p[0]); stnt(&ps[M_SXY_Y * ns], p[1]); stnt(&ps[M_SXZ_Z * ns], p[2]);
+            stnt(&ps[M_SXY_X * ns], p[3]); stnt(&ps[M_SYY_Y * ns], p[4]); stnt(&ps[M_SYZ_Z * ns], p[5]);
+            stnt(&ps[M_SXZ_X * ns], p[6]); stnt(&ps[M_SYZ_Y * ns], p[7]); stnt(&ps[M_SZZ_Z * ns], p[8]);
+        }
         if (inx) vx = nvx;
         if (iny) vy = nvy;
         if (inz) vz = nvz;

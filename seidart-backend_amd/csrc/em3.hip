@@ -310,21 +310,11 @@ __device__ __forceinline__ Em3Block em3_block(const Em3Args<T> &A)
     return {(int)(m % (unsigned)A.tiles_x), (int)(m / (unsigned)A.tiles_x), (int)kz, m < ncol};
 }
 
-// Whole-line stores.  A 128-byte line that a kernel writes only in part costs
-// the memory a read of the line besides the write.  In the half-step kernels
-// each wavefront holds one row segment starting on a line, so the lanes of a
-// line are 128 / sizeof(T) consecutive lanes: the lanes over the row padding
+// Whole-line stores (line_any): in the half-step kernels each wavefront
+// holds one row segment starting on a line, so the lanes over the row padding
 // (columns past nx) store zeros with the row, and when a line holds a CPML
 // cell every lane of it stores the memory variables (0 from the lanes without
 // CPML: their memory variables stay 0, as cpmlfdtd.f08's do where a = 0).
-template <typename T>
-__device__ __forceinline__ bool em3_line_any(bool p)
-{
-    constexpr int L = 128 / (int)sizeof(T);
-    const uint64_t m = __ballot(p);
-    const int base = (int)(threadIdx.x & 63u) & ~(L - 1);
-    return ((m >> base) & (L >= 64 ? ~0ull : ((1ull << L) - 1))) != 0;
-}
 
 // Hx, Hy, Hz of cell (i, j, k), cpmlfdtd.f08:3700-3778, updated in place;
 // columns past nx (the row padding) take 0 like the cells outside every nest.
@@ -366,7 +356,7 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
     // theirs
     const bool pml = row && i < A.nx && j < A.ny && k < A.nz &&
                      (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
-    const bool mst = em3_line_any<T>(pml);
+    const bool mst = line_any<T>(pml);
     if (!row) return;
     em3_h_cell(A, i, j, k, pml, mst);
 }
@@ -401,7 +391,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) e
     // CPML cells of the E nests (the faces take 0 and no memory variables)
     const bool pml = row && i > 1 && i < A.nx && j > 1 && j < A.ny && k > 1 && k < A.nz &&
                      (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
-    const bool mst = em3_line_any<T>(pml);
+    const bool mst = line_any<T>(pml);
     double nrm = 0.0;
     if (row) nrm = em3_e_cell<T, TAB>(A, i, j, k, pml, mst);
     block_max_to(&A.vnorm[*A.d_it - 1], nrm);

@@ -51,6 +51,21 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long *slot, doub
 // from each of ~1e5 workgroups on a single address serialises at L2 (it
 // doubled the 3D EM E update); a stale read only costs an extra atomic.
 // blockDim.x*blockDim.y*blockDim.z must be a multiple of 64.
+// Whole-line stores.  A 128-byte line that a kernel writes only in part costs
+// the memory a read of the line besides the write.  Where the lanes of a
+// wavefront store to consecutive elements, lane 0 on a line boundary, each
+// group of 128 / sizeof(T) lanes shares one line: line_any tells a lane
+// whether any lane of its group has p set (call it with every lane of the
+// wavefront active).
+template <typename T>
+__device__ __forceinline__ bool line_any(bool p)
+{
+    constexpr int L = 128 / (int)sizeof(T);
+    const unsigned long long m = __ballot(p);
+    const int base = (int)__lane_id() & ~(L - 1);
+    return ((m >> base) & (L >= 64 ? ~0ull : ((1ull << L) - 1))) != 0;
+}
+
 __device__ __forceinline__ void block_max_to(unsigned long long *slot, double v)
 {
     __shared__ double wave_max[16];
