@@ -212,7 +212,7 @@ int el3_setup(Device &d, const double *const *in)
 // then columns, so that consecutive tiles share stencil halos.
 int el3_build_shell(Device &d, const double *const *in, int kc)
 {
-    host::ShellTiles st = host::shell_tiles(d.nx, d.ny, d.nz, d.box, d.xoff, kc);
+    host::ShellTiles st = host::shell_tiles(d.nx, d.ny, d.nz, d.box, d.xoff, d.pitch, kc);
     const std::vector<El3Tile> &tiles = st.tiles;
     d.ntiles = (int)tiles.size();
     d.nslots = st.nslots;

@@ -10,9 +10,11 @@
 //
 //  * shell kernels over the cells outside the box (the CPML slabs and the
 //    Dirichlet faces), as a list of z-marching tiles of 256 cells per plane,
-//    one thread per cell like the interior kernels: 64 x 4 over the z slabs
-//    and the y strips, strip-wide (12 x 20 for 12-cell strips, each
-//    wavefront on 5 whole rows) over the x strips.  Their memory
+//    one thread per cell like the interior kernels: 256 consecutive
+//    elements of a plane over the z slabs and the y strips (rows follow one
+//    another; only the row padding leaves threads idle), strip-wide (12 x 20
+//    for 12-cell strips, each wavefront on 5 whole rows) over the x strips.
+//    Their memory
 //    variables and CPML coefficient ids are stored compactly in tile order,
 //    so those streams are contiguous per wavefront.  Neighbour indices are
 //    clamped into the grid so every load is unconditional; nest ranges mask
@@ -338,17 +340,21 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 // --------------------------------------------------------------------- shell
 // The CPML shell (the cells outside the box: the z slabs, y strips and x
 // strips, Dirichlet faces included) as a list of z-marching tiles of
-// kShellThreads cells per plane, one thread per cell: 64 x 4 over the z slabs
-// and y strips (every wavefront reads whole 128-byte lines), as wide as the
-// strip over the x strips (12 x 20 for 12-cell strips: each wavefront takes
-// 5 whole rows and leaves 4 threads idle, so it touches one line per row).  A thread keeps its
-// column's z stencil planes in
-// registers, issues every load of a plane before its first store, and
+// kShellThreads cells per plane, one thread per cell: 256 consecutive
+// elements of a plane over the z slabs and y strips (every wavefront reads
+// whole 128-byte lines, and only the row padding leaves threads idle: 280^3
+// FP64 1.732 -> 1.705 ms per step against 64 x 4 tiles, whose last tile of a
+// row was half empty), as wide as the strip over the x strips (12 x 20 for
+// 12-cell strips: each wavefront takes 5 whole rows and leaves 4 threads
+// idle, so it touches one line per row).  A thread keeps its column's z
+// stencil planes in registers, issues every load of a plane before its first
+// store, and
 // updates all of its cell's memory variables and fields itself -- no LDS and
 // no barriers, so each wavefront keeps a plane's loads in flight like the
-// interior kernels.  Memory variables are stored stream-major (stream q of
-// slot s at shell[q * nslots + s], slot = tile base + (k - k0) * 256 +
-// thread), so a wavefront's access of one stream is 64 consecutive values.
+// interior kernels.  Memory variables are stored in tile-plane blocks
+// (stream q of slot s at ((s / 256) * snq + q) * 256 + s % 256, slot = tile
+// base + (k - k0) * 256 + thread), so a wavefront's access of one stream is
+// 64 consecutive values.
 // Lanes outside the grid or the tile's region compute on clamped indices and
 // store no field; neighbour indices are clamped into the grid, and the nest
 // ranges of cpmlfdtd.f08 mask the field stores.
@@ -375,9 +381,18 @@ __device__ __forceinline__ ShellCell shell_cell(const El3Tile &tl, const int (&b
                                                 int64_t xoff)
 {
     ShellCell L;
-    const int tw = tl.tw, rpw = 64 / tw, l = threadIdx.x % 64;   // host::tile_lane
-    const int i0 = tl.x0 + l % tw, j0 = tl.y0 + (threadIdx.x / 64) * rpw + l / tw;
-    L.on = l < rpw * tw && shell_lane(tl.kind, box, i0, j0, nx, ny);
+    int i0, j0;   // host::tile_lane
+    if (tl.tw == 0) {   // z slab / y strip: consecutive elements of the plane
+        const int e = tl.x0 + (int)threadIdx.x;
+        j0 = e / tl.pitch + 1;
+        i0 = e - (j0 - 1) * tl.pitch - tl.xoff + 1;
+        L.on = shell_lane(tl.kind, box, i0, j0, nx, ny);
+    } else {            // x strip: whole rows of tw cells per wavefront
+        const int tw = tl.tw, rpw = 64 / tw, l = threadIdx.x % 64;
+        i0 = tl.x0 + l % tw;
+        j0 = tl.y0 + (threadIdx.x / 64) * rpw + l / tw;
+        L.on = l < rpw * tw && shell_lane(tl.kind, box, i0, j0, nx, ny);
+    }
     const int i = min(max(i0, 1), nx), j = min(max(j0, 1), ny);
     L.i = i;
     L.j = j;

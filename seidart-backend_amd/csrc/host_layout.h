@@ -27,27 +27,37 @@
 namespace sdt {
 
 // One z-marching tile of the CPML shell: up to kShellThreads cells of a plane,
-// tw columns from (x0, y0), each wavefront of 64 threads on 64 / tw whole rows
-// (64 x 4 over z slabs and y strips; over x strips as wide as the strip, e.g.
-// 12 x 20 with 4 idle threads per wavefront for a 12-cell CPML), planes k0..k1,
-// region kind (0 z slab; 1, 3 the y strip below, above the box; 2, 4 the x
-// strip left, right of it) and the first slot of its memory variables
-// (slot = base + (k - k0) * kShellThreads + thread).
+// planes k0..k1, region kind (0 z slab; 1, 3 the y strip below, above the box;
+// 2, 4 the x strip left, right of it) and the first slot of its memory
+// variables (slot = base + (k - k0) * kShellThreads + thread).  Over the z
+// slabs and y strips (tw = 0) thread t takes element x0 + t of the plane, rows
+// of `pitch` elements with column 1 at element xoff of its row: every
+// wavefront covers 64 consecutive elements (whole lines) and only the row
+// padding leaves threads idle.  Over the x strips, tw columns from (x0, y0),
+// each wavefront of 64 threads on 64 / tw whole rows (12 x 20 with 4 idle
+// threads per wavefront for a 12-cell CPML).
 constexpr int kShellThreads = 256;
 struct El3Tile {
     int x0, y0, k0, k1;
     int kind, tw;
     int64_t base;
+    int pitch, xoff;
 };
 
-// Rows of a tile: 64 / tw per wavefront, four wavefronts.
+// Rows of an x-strip tile: 64 / tw per wavefront, four wavefronts.
 inline int tile_rows(int tw) { return 4 * (64 / tw); }
 
-// Column (i, j) of thread t of a tile: wavefront w = t / 64 takes rows
-// w * (64 / tw) ...; false for the threads past the wavefront's last whole row
-// (they own slots but no cell).  The kernels use the same mapping.
+// Column (i, j) of thread t of a tile (possibly outside the grid); false for
+// the threads of an x-strip tile past its wavefront's last whole row (they own
+// slots but no cell).  The kernels use the same mapping.
 inline bool tile_lane(const El3Tile &t, int thread, int &i, int &j)
 {
+    if (t.tw == 0) {
+        const int e = t.x0 + thread;
+        j = e / t.pitch + 1;
+        i = e - (j - 1) * t.pitch - t.xoff + 1;
+        return true;
+    }
     const int rpw = 64 / t.tw, l = thread % 64;
     i = t.x0 + l % t.tw;
     j = t.y0 + (thread / 64) * rpw + l / t.tw;
@@ -204,14 +214,13 @@ struct ShellTiles {
 // rows, so a narrow strip keeps most lanes busy.  A tile is listed
 // when one of its columns lies in its region; its slots follow the previous
 // tile's.
-inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t xoff, int kc)
+inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t xoff, int pitch, int kc)
 {
     ShellTiles st;
     st.nslots = 0;
     const bool empty = b[1] < b[0];
-    const int xa = 1 - (int)xoff;   // x origin of line-aligned tiles
     auto add = [&](int x0, int y0, int k0, int k1, int kind, int tw) {
-        const El3Tile t{x0, y0, k0, k1, kind, tw, st.nslots};
+        const El3Tile t{x0, y0, k0, k1, kind, tw, st.nslots, pitch, (int)xoff};
         for (int th = 0; th < kShellThreads; ++th) {
             int i, j;
             if (tile_lane(t, th, i, j) && shell_region(kind, b, i, j, nx, ny)) {
@@ -221,11 +230,14 @@ inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t
             }
         }
     };
+    // rows ja..jb of planes k0..k1 as linear tiles (elements from row ja's first)
+    auto rows = [&](int ja, int jb, int k0, int k1, int kind) {
+        for (int e = (ja - 1) * pitch; e < jb * pitch; e += kShellThreads) add(e, 0, k0, k1, kind, 0);
+    };
     auto zslab = [&](int ka, int kb) {
         for (int k0 = ka; k0 <= kb; k0 += kc) {
             const int k1 = k0 + kc - 1 < kb ? k0 + kc - 1 : kb;
-            for (int y0 = 1; y0 <= ny; y0 += 4)
-                for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 0, 64);
+            rows(1, ny, k0, k1, 0);
         }
     };
     auto xstrip = [&](int lo, int hi, int k0, int k1, int kind) {
@@ -241,12 +253,10 @@ inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t
     zslab(1, b[4] - 1);
     for (int k0 = b[4]; k0 <= b[5]; k0 += kc) {
         const int k1 = k0 + kc - 1 < b[5] ? k0 + kc - 1 : b[5];
-        for (int y0 = 1; y0 < b[2]; y0 += 4)
-            for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 1, 64);
+        if (b[2] > 1) rows(1, b[2] - 1, k0, k1, 1);
         xstrip(1, b[0] - 1, k0, k1, 2);
         xstrip(b[1] + 1, nx, k0, k1, 4);
-        for (int y0 = b[3] + 1; y0 <= ny; y0 += 4)
-            for (int x0 = xa; x0 <= nx; x0 += 64) add(x0, y0, k0, k1, 3, 64);
+        if (b[3] < ny) rows(b[3] + 1, ny, k0, k1, 3);
     }
     zslab(b[5] + 1, nz);
     return st;

@@ -115,16 +115,19 @@ static void check_grid(int nx, int ny, int nz, int w, int strays, int esize, int
     CHECK((rl.xoff + (empty ? 1 : box[0]) - 1) % line == 0, "first column off the line grid");
     CHECK(rl.pitch % line == 0 && rl.pitch >= rl.xoff + nx && rl.pitch < rl.xoff + nx + line, "pitch %d", rl.pitch);
 
-    const host::ShellTiles st = host::shell_tiles(nx, ny, nz, box, rl.xoff, kc);
+    const host::ShellTiles st = host::shell_tiles(nx, ny, nz, box, rl.xoff, rl.pitch, kc);
     std::vector<int> cover((size_t)nx * ny * nz, 0);
     int64_t base = 0;
     for (const El3Tile &t : st.tiles) {
         CHECK(t.base == base, "tile base %lld, expected %lld", (long long)t.base, (long long)base);
         CHECK(t.k0 >= 1 && t.k1 <= nz && t.k0 <= t.k1 && t.k1 - t.k0 + 1 <= kc, "planes %d..%d", t.k0, t.k1);
         const bool xs = t.kind == 2 || t.kind == 4;
-        CHECK(t.kind >= 0 && t.kind <= 4 && (xs ? t.tw >= 1 && t.tw <= 64 : t.tw == 64), "tile shape of kind %d",
+        CHECK(t.kind >= 0 && t.kind <= 4 && (xs ? t.tw >= 1 && t.tw <= 64 : t.tw == 0), "tile shape of kind %d",
               t.kind);
-        CHECK(xs || (rl.xoff + t.x0 - 1) % line == 0, "tile x0 %d off the line grid", t.x0);
+        // linear tiles (z slabs, y strips) start on a line, inside the plane
+        CHECK(xs || (t.x0 % line == 0 && t.x0 >= 0 && t.x0 < ny * rl.pitch && t.pitch == rl.pitch &&
+                     t.xoff == rl.xoff),
+              "linear tile at element %d", t.x0);
         if (xs)   // x-strip tiles start inside their strip and are no wider
             CHECK(t.kind == 2 ? t.x0 >= 1 && t.x0 <= box[0] - 1 : t.x0 >= box[1] + 1 && t.x0 <= nx,
                   "x-strip tile x0 %d width %d", t.x0, t.tw);
