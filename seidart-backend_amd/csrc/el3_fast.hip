@@ -12,8 +12,8 @@
 //    Dirichlet faces), as a list of z-marching tiles of 256 cells per plane,
 //    one thread per cell like the interior kernels: 256 consecutive
 //    elements of a plane over the z slabs and the y strips (rows follow one
-//    another; only the row padding leaves threads idle), strip-wide (12 x 20
-//    for 12-cell strips, each wavefront on 5 whole rows) over the x strips.
+//    another; only the row padding leaves threads idle), and over the x
+//    strips 256 consecutive cells of the strip, numbered row by row.
 //    Their memory
 //    variables and CPML coefficient ids are stored compactly in tile order,
 //    so those streams are contiguous per wavefront.  Neighbour indices are
@@ -344,12 +344,13 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
 // elements of a plane over the z slabs and y strips (every wavefront reads
 // whole 128-byte lines, and only the row padding leaves threads idle: 280^3
 // FP64 1.732 -> 1.705 ms per step against 64 x 4 tiles, whose last tile of a
-// row was half empty), as wide as the strip over the x strips (12 x 20 for
-// 12-cell strips: each wavefront takes 5 whole rows and leaves 4 threads
-// idle, so it touches one line per row).  A thread keeps its column's z
-// stencil planes in registers, issues every load of a plane before its first
-// store, and
-// updates all of its cell's memory variables and fields itself -- no LDS and
+// row was half empty), and over the x strips 256 consecutive cells of the
+// strip numbered row by row (a wavefront on about five rows of a 12- or
+// 13-cell strip, no thread idle: 1.697 -> 1.679 ms per step against tiles of
+// whole rows per wavefront, which left 4 or 12 threads idle).  A thread keeps
+// its column's z stencil planes in registers, issues every load of a plane
+// before its first store, and updates all of its cell's memory variables and
+// fields itself -- no LDS and
 // no barriers, so each wavefront keeps a plane's loads in flight like the
 // interior kernels.  Memory variables are stored in tile-plane blocks
 // (stream q of slot s at ((s / 256) * snq + q) * 256 + s % 256, slot = tile
@@ -387,11 +388,11 @@ __device__ __forceinline__ ShellCell shell_cell(const El3Tile &tl, const int (&b
         j0 = e / tl.pitch + 1;
         i0 = e - (j0 - 1) * tl.pitch - tl.xoff + 1;
         L.on = shell_lane(tl.kind, box, i0, j0, nx, ny);
-    } else {            // x strip: whole rows of tw cells per wavefront
-        const int tw = tl.tw, rpw = 64 / tw, l = threadIdx.x % 64;
-        i0 = tl.x0 + l % tw;
-        j0 = tl.y0 + (threadIdx.x / 64) * rpw + l / tw;
-        L.on = l < rpw * tw && shell_lane(tl.kind, box, i0, j0, nx, ny);
+    } else {            // x strip: the strip's cells row by row, every thread on one
+        const int w = -tl.tw, n = tl.n0 + (int)threadIdx.x, r = n / w;
+        i0 = tl.x0 + (n - r * w);
+        j0 = tl.y0 + r;
+        L.on = shell_lane(tl.kind, box, i0, j0, nx, ny);
     }
     const int i = min(max(i0, 1), nx), j = min(max(j0, 1), ny);
     L.i = i;

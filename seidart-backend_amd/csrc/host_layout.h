@@ -33,23 +33,22 @@ namespace sdt {
 // slabs and y strips (tw = 0) thread t takes element x0 + t of the plane, rows
 // of `pitch` elements with column 1 at element xoff of its row: every
 // wavefront covers 64 consecutive elements (whole lines) and only the row
-// padding leaves threads idle.  Over the x strips, tw columns from (x0, y0),
-// each wavefront of 64 threads on 64 / tw whole rows (12 x 20 with 4 idle
-// threads per wavefront for a 12-cell CPML).
+// padding leaves threads idle.  Over the x strips (tw = -w for a strip w
+// columns wide from column x0, rows from y0) thread t takes the strip's cell
+// n0 + t, row by row: column x0 + (n0 + t) mod w of row y0 + (n0 + t) / w, so
+// no thread is idle whatever the width.
 constexpr int kShellThreads = 256;
 struct El3Tile {
     int x0, y0, k0, k1;
     int kind, tw;
     int64_t base;
     int pitch, xoff;
+    int n0;
 };
 
-// Rows of an x-strip tile: 64 / tw per wavefront, four wavefronts.
-inline int tile_rows(int tw) { return 4 * (64 / tw); }
-
-// Column (i, j) of thread t of a tile (possibly outside the grid); false for
-// the threads of an x-strip tile past its wavefront's last whole row (they own
-// slots but no cell).  The kernels use the same mapping.
+// Column (i, j) of thread t of a tile (possibly outside the grid or the
+// tile's region: such threads own slots but no cell).  The kernels use the
+// same mapping.
 inline bool tile_lane(const El3Tile &t, int thread, int &i, int &j)
 {
     if (t.tw == 0) {
@@ -58,10 +57,10 @@ inline bool tile_lane(const El3Tile &t, int thread, int &i, int &j)
         i = e - (j - 1) * t.pitch - t.xoff + 1;
         return true;
     }
-    const int rpw = 64 / t.tw, l = thread % 64;
-    i = t.x0 + l % t.tw;
-    j = t.y0 + (thread / 64) * rpw + l / t.tw;
-    return l < rpw * t.tw;
+    const int w = -t.tw, n = t.n0 + thread, r = n / w;   // x strip
+    i = t.x0 + (n - r * w);
+    j = t.y0 + r;
+    return true;
 }
 
 namespace host {
@@ -208,19 +207,18 @@ struct ShellTiles {
 
 // Tiles of the cells outside the box (all cells when the box is empty), plane
 // chunk by plane chunk of kc planes: z slabs below and above the box and y
-// strips in 64 x 4 tiles whose x origin is on the 128-byte line grid of the
-// padded rows, rows then columns; x strips (left, then right of the box) in
-// tiles as wide as the strip (at most 64 columns), each wavefront on whole
-// rows, so a narrow strip keeps most lanes busy.  A tile is listed
-// when one of its columns lies in its region; its slots follow the previous
-// tile's.
+// strips as linear tiles of 256 consecutive elements of the padded rows
+// (starting on the 128-byte line grid); x strips (left, then right of the
+// box) with their cells numbered row by row, 256 to a tile, so that no lane
+// is idle whatever the strip's width.  A tile is listed when one of its
+// lanes lies in its region; its slots follow the previous tile's.
 inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t xoff, int pitch, int kc)
 {
     ShellTiles st;
     st.nslots = 0;
     const bool empty = b[1] < b[0];
-    auto add = [&](int x0, int y0, int k0, int k1, int kind, int tw) {
-        const El3Tile t{x0, y0, k0, k1, kind, tw, st.nslots, pitch, (int)xoff};
+    auto add = [&](int x0, int y0, int k0, int k1, int kind, int tw, int n0 = 0) {
+        const El3Tile t{x0, y0, k0, k1, kind, tw, st.nslots, pitch, (int)xoff, n0};
         for (int th = 0; th < kShellThreads; ++th) {
             int i, j;
             if (tile_lane(t, th, i, j) && shell_region(kind, b, i, j, nx, ny)) {
@@ -242,9 +240,8 @@ inline ShellTiles shell_tiles(int nx, int ny, int nz, const int (&b)[6], int64_t
     };
     auto xstrip = [&](int lo, int hi, int k0, int k1, int kind) {
         if (lo > hi) return;
-        const int tw = hi - lo + 1 < 64 ? hi - lo + 1 : 64, rows = tile_rows(tw);
-        for (int y0 = b[2]; y0 <= b[3]; y0 += rows)
-            for (int x0 = lo; x0 <= hi; x0 += tw) add(x0, y0, k0, k1, kind, tw);
+        const int w = hi - lo + 1, ncell = w * (b[3] - b[2] + 1);
+        for (int n0 = 0; n0 < ncell; n0 += kShellThreads) add(lo, b[2], k0, k1, kind, -w, n0);
     };
     if (empty) {
         zslab(1, nz);

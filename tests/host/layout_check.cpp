@@ -122,15 +122,15 @@ static void check_grid(int nx, int ny, int nz, int w, int strays, int esize, int
         CHECK(t.base == base, "tile base %lld, expected %lld", (long long)t.base, (long long)base);
         CHECK(t.k0 >= 1 && t.k1 <= nz && t.k0 <= t.k1 && t.k1 - t.k0 + 1 <= kc, "planes %d..%d", t.k0, t.k1);
         const bool xs = t.kind == 2 || t.kind == 4;
-        CHECK(t.kind >= 0 && t.kind <= 4 && (xs ? t.tw >= 1 && t.tw <= 64 : t.tw == 0), "tile shape of kind %d",
-              t.kind);
+        CHECK(t.kind >= 0 && t.kind <= 4 && (xs ? t.tw < 0 : t.tw == 0), "tile shape of kind %d", t.kind);
         // linear tiles (z slabs, y strips) start on a line, inside the plane
         CHECK(xs || (t.x0 % line == 0 && t.x0 >= 0 && t.x0 < ny * rl.pitch && t.pitch == rl.pitch &&
                      t.xoff == rl.xoff),
               "linear tile at element %d", t.x0);
-        if (xs)   // x-strip tiles start inside their strip and are no wider
-            CHECK(t.kind == 2 ? t.x0 >= 1 && t.x0 <= box[0] - 1 : t.x0 >= box[1] + 1 && t.x0 <= nx,
-                  "x-strip tile x0 %d width %d", t.x0, t.tw);
+        if (xs)   // x-strip tiles cover their whole strip's width from its first row
+            CHECK((t.kind == 2 ? t.x0 == 1 && -t.tw == box[0] - 1 : t.x0 == box[1] + 1 && -t.tw == nx - box[1]) &&
+                      t.y0 == box[2] && t.n0 >= 0 && t.n0 % kShellThreads == 0,
+                  "x-strip tile x0 %d width %d", t.x0, -t.tw);
         bool any = false;
         for (int th = 0; th < kShellThreads; ++th) {
             int i, j;
