@@ -32,6 +32,7 @@ struct Em3Args {
     const uint8_t *flags;
     int any_flags;      // 0: no tile inside the box holds a CPML cell (the flags need no load)
     int col_order;      // half-step kernels: 1D grid in XCD column order
+    int lin;            // half-step kernels: each workgroup takes 256 consecutive elements of a plane
     int box[6];
     int nx, ny, nz, tiles_x, tiles_y, pitch;
     int64_t xoff;
@@ -316,6 +317,40 @@ __device__ __forceinline__ Em3Block em3_block(const Em3Args<T> &A)
 // cell every lane of it stores the memory variables (0 from the lanes without
 // CPML: their memory variables stay 0, as cpmlfdtd.f08's do where a = 0).
 
+// The cell of a half-step kernel's thread.  With `lin` a workgroup takes 256
+// consecutive elements of plane blockIdx.y + 1, rows following one another,
+// so that no lane is idle beyond the row padding (64 x 4 tiles of 404-cell
+// rows leave 32 of every 448 lanes idle); otherwise a 64 x 4 tile, in grid or
+// XCD column order.  `row`: the lane holds a cell or a padding element of a
+// row of the grid.
+struct Em3Lane {
+    int i, j, k;
+    bool row;
+    size_t t;   // CPML flag tile (64 x 4 x 1 cells from column 1) of the cell, when it is one
+};
+
+template <typename T, bool LIN>
+__device__ __forceinline__ Em3Lane em3_lane(const Em3Args<T> &A)
+{
+    Em3Lane L;
+    if constexpr (LIN) {
+        const int e = (int)blockIdx.x * 256 + (int)(threadIdx.x + 64 * threadIdx.y);
+        L.j = e / A.pitch + 1;
+        L.i = e - (L.j - 1) * A.pitch - (int)A.xoff + 1;
+        L.k = (int)blockIdx.y + 1;
+        L.row = L.j <= A.ny;
+        L.t = (size_t)((L.i - 1) >> 6) + (size_t)A.tiles_x * ((size_t)((L.j - 1) >> 2) + (size_t)A.tiles_y * (L.k - 1));
+    } else {
+        const Em3Block B = em3_block(A);
+        L.i = B.bx * 64 + threadIdx.x + 1;
+        L.j = B.by * 4 + threadIdx.y + 1;
+        L.k = B.bz + 1;
+        L.row = B.valid && L.i <= A.pitch - (int)A.xoff && L.j <= A.ny;
+        L.t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
+    }
+    return L;
+}
+
 // Hx, Hy, Hz of cell (i, j, k), cpmlfdtd.f08:3700-3778, updated in place;
 // columns past nx (the row padding) take 0 like the cells outside every nest.
 template <typename T>
@@ -342,20 +377,17 @@ __device__ __forceinline__ void em3_h_cell(const Em3Args<T> &A, int i, int j, in
     em3_h_math(A, c, k, pml, true, mst, ex0, ey0, ez0, exy, ezy, eyx, ezx, exz, eyz, h0, h1, h2, hx, hy, hz);
 }
 
-template <typename T>
+template <typename T, bool LIN>
 __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
 {
-    const Em3Block B = em3_block(A);
-    const int i = B.bx * 64 + threadIdx.x + 1;
-    const int j = B.by * 4 + threadIdx.y + 1;
-    const int k = B.bz + 1;
-    const bool row = B.valid && i <= A.pitch - (int)A.xoff && j <= A.ny;   // the cells and the row padding
-    const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
+    const Em3Lane L = em3_lane<T, LIN>(A);
+    const int i = L.i, j = L.j, k = L.k;
+    const bool row = L.row;
     // one code path with a per-lane CPML flag: a wavefront holding both kinds
     // of cell (the x faces) issues its field loads once, the CPML lanes add
     // theirs
-    const bool pml = row && i < A.nx && j < A.ny && k < A.nz &&
-                     (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
+    const bool pml = row && i >= 1 && i < A.nx && j < A.ny && k < A.nz &&
+                     (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[L.t]));
     const bool mst = line_any<T>(pml);
     if (!row) return;
     em3_h_cell(A, i, j, k, pml, mst);
@@ -379,18 +411,15 @@ __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, 
     return em3_e_new<T, TAB, true>(A, i, j, k, P, hx0, hy0, hz0, hxy, hyx, hzx, hzy, hxz, hyz, mst);
 }
 
-template <typename T, bool TAB>
+template <typename T, bool TAB, bool LIN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) em3_e_kernel(Em3Args<T> A)
 {
-    const Em3Block B = em3_block(A);
-    const int i = B.bx * 64 + threadIdx.x + 1;
-    const int j = B.by * 4 + threadIdx.y + 1;
-    const int k = B.bz + 1;
-    const bool row = B.valid && i <= A.pitch - (int)A.xoff && j <= A.ny;   // the cells and the row padding
-    const size_t t = B.bx + (size_t)A.tiles_x * (B.by + (size_t)A.tiles_y * B.bz);
+    const Em3Lane L = em3_lane<T, LIN>(A);
+    const int i = L.i, j = L.j, k = L.k;
+    const bool row = L.row;
     // CPML cells of the E nests (the faces take 0 and no memory variables)
     const bool pml = row && i > 1 && i < A.nx && j > 1 && j < A.ny && k > 1 && k < A.nz &&
-                     (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[t]));
+                     (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[L.t]));
     const bool mst = line_any<T>(pml);
     double nrm = 0.0;
     if (row) nrm = em3_e_cell<T, TAB>(A, i, j, k, pml, mst);
@@ -485,6 +514,7 @@ static Em3Args<T> em3_args(Device &d)
     A.flags = d.tile_flags;
     A.any_flags = d.nflagged > 0 ? 1 : 0;
     A.col_order = 0;
+    A.lin = 0;
     for (int q = 0; q < 6; ++q) A.box[q] = d.box[q];
     A.nx = d.nx; A.ny = d.ny; A.nz = d.nz;
     A.pitch = d.pitch; A.xoff = d.xoff;
@@ -565,14 +595,25 @@ static void em3_step_t(Device &d, bool timed)
     const T *mp[] = {A.m_dEy_dz, A.m_dEz_dy, A.m_dEx_dz, A.m_dEz_dx, A.m_dEx_dy, A.m_dEy_dx};
     for (int q = 0; q < 6; ++q) A.mo[q] = mp[q];
     A.col_order = d.xcd_remap & 1;
+    // linear workgroups in FP64 (404x404x212: 25.5 -> 26.2 GCell/s); in FP32
+    // the E update ran slower with them (49.3 -> 46.7 GCell/s), so the 64 x 4
+    // tiles stay
+    A.lin = !A.col_order && sizeof(T) == 8;
     dim3 grid(d.tiles_x, d.tiles_y, d.tiles_z);
+    if (A.lin) grid = dim3((unsigned)((d.ny * (int64_t)d.pitch + 255) / 256), d.nz, 1);
     if (A.col_order) grid = dim3(8 * ((d.tiles_x * d.tiles_y + 7) / 8) * d.tiles_z, 1, 1);
     const dim3 block(64, 4, 1);
     if (timed) (void)hipEventRecord(d.ev[0], d.stream);
-    hipLaunchKernelGGL(em3_h_kernel<T>, grid, block, 0, d.stream, A);
+    if (A.lin) hipLaunchKernelGGL((em3_h_kernel<T, true>), grid, block, 0, d.stream, A);
+    else hipLaunchKernelGGL((em3_h_kernel<T, false>), grid, block, 0, d.stream, A);
     if (timed) (void)hipEventRecord(d.ev[1], d.stream);
-    if (d.mat_table) hipLaunchKernelGGL((em3_e_kernel<T, true>), grid, block, 0, d.stream, A);
-    else hipLaunchKernelGGL((em3_e_kernel<T, false>), grid, block, 0, d.stream, A);
+    if (A.lin) {
+        if (d.mat_table) hipLaunchKernelGGL((em3_e_kernel<T, true, true>), grid, block, 0, d.stream, A);
+        else hipLaunchKernelGGL((em3_e_kernel<T, false, true>), grid, block, 0, d.stream, A);
+    } else {
+        if (d.mat_table) hipLaunchKernelGGL((em3_e_kernel<T, true, false>), grid, block, 0, d.stream, A);
+        else hipLaunchKernelGGL((em3_e_kernel<T, false, false>), grid, block, 0, d.stream, A);
+    }
     if (timed) (void)hipEventRecord(d.ev[2], d.stream);
     launch_plane_wave(d);
     launch_tail(d);
