@@ -245,7 +245,9 @@ int main(int argc, char **argv)
     // the 3D elastic shapes the tests and the benchmark use, then random ones
     const int fixed[][7] = {{40, 36, 32, 6, 0, 8, 8}, {28, 28, 28, 6, 0, 8, 8}, {32, 30, 28, 6, 0, 4, 8},
                             {20, 18, 16, 4, 3, 8, 3}, {7, 7, 7, 2, 0, 8, 1}, {5, 5, 5, 0, 0, 4, 8},
-                            {1, 1, 1, 0, 0, 8, 8}, {70, 9, 40, 10, 0, 8, 12}};
+                            {1, 1, 1, 0, 0, 8, 8}, {70, 9, 40, 10, 0, 8, 12},
+                            // CPML strips wider than a wavefront (66 cells: a strip row spans two)
+                            {140, 140, 140, 66, 0, 8, 3}, {141, 139, 138, 66, 0, 4, 3}};
     for (const auto &f : fixed) check_grid(f[0], f[1], f[2], f[3], f[4], f[5], f[6]);
     for (int r = 0; r < rounds; ++r) {
         const int nx = uni(1, 90), ny = uni(1, 40), nz = uni(1, 30);

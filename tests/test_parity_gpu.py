@@ -134,6 +134,35 @@ def test_elastic3_thin_interior_box(ny, cpml):
     _check(g, (o_rec, o_vn, o_sn, o_f), TOL64, stress_norm=True)
 
 
+def test_elastic3_cpml_wider_than_a_wavefront():
+    """A 66-cell CPML on every side (144^3 with a 12^3 interior box): the
+    x-strip tiles number a 66-cell strip's cells row by row, so a strip row
+    spans two wavefronts and a wavefront holds the end of one row and the
+    start of the next; the z slabs and y strips take linear tiles of whole
+    rows.  Seeded random fields on every cell, 4 steps, every field against
+    the oracle."""
+    import numpy as np
+    from seidart_hip import models
+    from seidart_hip.solver import make_solver
+    from oracle.oracle_ctypes import run_oracle
+    cfg = models.elastic3(nx=12, ny=12, nz=12, cpml=66, steps=4, src=(6, 6, 6), nrec=4, nlayers=2)
+    fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
+    rng = np.random.default_rng(66)
+    ic = {f: rng.uniform(-1.0, 1.0, cfg["shape"]) * (1.0 if f.startswith("v") else 4e6) for f in fields}
+    print(f"[parity] elastic3 wide CPML shape={cfg['shape']}", flush=True)
+    s = make_solver(cfg)
+    try:
+        for k, a in ic.items():
+            s.set_field(k, a)
+        s.run()
+        g = (s.receivers(), *s.norms(), {f: s.field(f) for f in fields})
+    finally:
+        s.close()
+    o_rec, (o_vn, o_sn), o_f, us = run_oracle(cfg, fields=fields, field_in=ic)
+    assert us == 0
+    _check(g, (o_rec, o_vn, o_sn, o_f), TOL64, stress_norm=True)
+
+
 def test_elastic3_corrected_stencil():
     from seidart_hip import models
     cfg = models.elastic3(nx=30, ny=26, nz=28, cpml=5, steps=120, src=(15, 13, 9), nrec=8)
