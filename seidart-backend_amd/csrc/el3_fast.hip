@@ -597,10 +597,20 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
         } else {
             shell_stress_update(DenseRow<T>{A.C, A.ig, c, A.ig_same}, D, so, dt, L1, L2, L3, L4, sn);
         }
-        if (L1) { stnt(&A.s[0][c], sn[0]); stnt(&A.s[1][c], sn[1]); stnt(&A.s[2][c], sn[2]); }
-        if (L2) stnt(&A.s[3][c], sn[3]);
-        if (L3) stnt(&A.s[4][c], sn[4]);
-        if (L4) stnt(&A.s[5][c], sn[5]);
+        if (tl.tw < 0) {
+            // x strip (uniform per workgroup): ordinary stores, so the row pieces
+            // of neighbouring rows may meet in L2 before they go to memory
+            // (280^3 FP64 1.692 -> 1.682 ms per step against non-temporal ones)
+            if (L1) { A.s[0][c] = sn[0]; A.s[1][c] = sn[1]; A.s[2][c] = sn[2]; }
+            if (L2) A.s[3][c] = sn[3];
+            if (L3) A.s[4][c] = sn[4];
+            if (L4) A.s[5][c] = sn[5];
+        } else {
+            if (L1) { stnt(&A.s[0][c], sn[0]); stnt(&A.s[1][c], sn[1]); stnt(&A.s[2][c], sn[2]); }
+            if (L2) stnt(&A.s[3][c], sn[3]);
+            if (L3) stnt(&A.s[4][c], sn[4]);
+            if (L4) stnt(&A.s[5][c], sn[5]);
+        }
         if (L.on) {
             const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3] ||
                              pw_face(A.pwf, i, j, k);
@@ -703,8 +713,12 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
         if (iny) vy = nvy;
         if (inz) vz = nvz;
         if (face) { vx = T(0); vy = T(0); vz = T(0); }
-        if (L.on) {
+        if (L.on && tl.tw < 0) {   // x strip: ordinary stores, as in the stress kernel
+            A.v[0][c] = vx; A.v[1][c] = vy; A.v[2][c] = vz;
+        } else if (L.on) {
             stnt(&A.v[0][c], vx); stnt(&A.v[1][c], vy); stnt(&A.v[2][c], vz);
+        }
+        if (L.on) {
             const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, j, k);
             const double a0 = vx, a1 = vy, a2 = vz;
             nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
