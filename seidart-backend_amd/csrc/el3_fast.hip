@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
     for (int m = 0; m < 3; ++m)
 #pragma unroll
         for (int s = 0; s < 4; ++s) q[m][s] = A.v[m][c + (s - 2) * sz];
-    double nrm = 0.0;
+    double nrm = 0.0;   // largest squared magnitude (sqrt is monotone and correctly rounded)
     for (int k = kb; k <= ke; ++k, c += sz) {
         T dxp[3], dxm[3], dyp[3], dym[3], dzp[3], dzm[3];
 #pragma unroll
@@ -237,15 +237,15 @@ __global__ void __launch_bounds__(64 * TY) el3_int_stress_kernel(El3Args<T> A, i
             const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3] ||
                              pw_face(A.pwf, i, j, k);
             const double a0 = sxx, a1 = syy, a2 = szz, a3 = sxy, a4 = syz, a5 = sxz;
-            const double n = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5));
-            nrm = fix ? nrm : fmax_nan(nrm, n);
+            const double n2 = a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5);
+            nrm = fix ? nrm : fmax_nan(nrm, n2);
         }
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             q[m][0] = q[m][1]; q[m][1] = q[m][2]; q[m][2] = q[m][3]; q[m][3] = q[m][4];
         }
     }
-    block_max_to(&A.snorm[*A.d_it - 1], nrm);
+    block_max_to(&A.snorm[*A.d_it - 1], sqrt(nrm));
 }
 
 template <typename T, bool CORR, int TY, bool TAB>
@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
     // plane k-1 after its own loads; the cached table reads follow in plane k
     RhoPre<T> rz_next = rho_pre<T, TAB>(A, c + sz), rxm_next = rho_pre<T, TAB>(A, c - 1),
               rxp_next = rho_pre<T, TAB>(A, c + 1);
-    double nrm = 0.0;
+    double nrm = 0.0;   // largest squared magnitude (sqrt is monotone and correctly rounded)
     for (int k = kb; k <= ke; ++k, c += sz) {
         qxz[4] = sxz[c + 2 * sz];
         qyz[4] = syz[c + 2 * sz];
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
         if (act) {
             const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, j, k);
             const double a0 = vx, a1 = vy, a2 = vz;
-            nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
+            nrm = fix ? nrm : fmax_nan(nrm, a0 * a0 + a1 * a1 + a2 * a2);
         }
         if (k < ke) {
             rz_next = rho_pre<T, TAB>(A, c + 2 * sz);
@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(64 * TY) el3_int_velocity_kernel(El3Args<T> A,
         }
         qr[0] = qr[1]; qr[1] = qr[2];
     }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+    block_max_to(&A.vnorm[*A.d_it - 1], sqrt(nrm));
 }
 
 // --------------------------------------------------------------------- shell
@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
 #pragma unroll
         for (int s = 0; s < 4; ++s) q[m][s] = A.v[m][L.col + sz * (min(max(tl.k0 - 2 + s, 1), nz) - 1)];
     constexpr int64_t ns = kShellThreads;   // stream stride inside a tile-plane block
-    double nrm = 0.0;
+    double nrm = 0.0;   // largest squared magnitude (sqrt is monotone and correctly rounded)
     int64_t slot = tl.base + threadIdx.x;
     for (int k = tl.k0; k <= tl.k1; ++k, slot += kShellThreads) {
         const int64_t c = L.col + sz * (k - 1);
@@ -615,11 +615,11 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_stress_kernel(El3Args
             const bool fix = c == A.sfix[0] || c == A.sfix[1] || c == A.sfix[2] || c == A.sfix[3] ||
                              pw_face(A.pwf, i, j, k);
             const double a0 = sn[0], a1 = sn[1], a2 = sn[2], a3 = sn[3], a4 = sn[5], a5 = sn[4];
-            const double n = sqrt(a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5));
-            nrm = fix ? nrm : fmax_nan(nrm, n);
+            const double n2 = a0 * a0 + a1 * a1 + a2 * a2 + 2.0 * (a3 * a3 + a4 * a4 + a5 * a5);
+            nrm = fix ? nrm : fmax_nan(nrm, n2);
         }
     }
-    block_max_to(&A.snorm[*A.d_it - 1], nrm);
+    block_max_to(&A.snorm[*A.d_it - 1], sqrt(nrm));
 }
 
 // Velocity pass (cpmlfdtd.f08:2014-2038 vx, 2055-2080 vy, 2098-2127 vz): the
@@ -652,7 +652,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
               rxm_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0) + L.xm1),
               rxp_next = rho_pre<T, TAB>(A, L.col + zc(tl.k0) + L.xp1);
     constexpr int64_t ns = kShellThreads;   // stream stride inside a tile-plane block
-    double nrm = 0.0;
+    double nrm = 0.0;   // largest squared magnitude (sqrt is monotone and correctly rounded)
     int64_t slot = tl.base + threadIdx.x;
     for (int k = tl.k0; k <= tl.k1; ++k, slot += kShellThreads) {
         const int64_t c = L.col + sz * (k - 1);
@@ -721,7 +721,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
         if (L.on) {
             const bool fix = c == A.vfix[0] || c == A.vfix[1] || c == A.vfix[2] || pw_face(A.pwf, i, j, k);
             const double a0 = vx, a1 = vy, a2 = vz;
-            nrm = fix ? nrm : fmax_nan(nrm, sqrt(a0 * a0 + a1 * a1 + a2 * a2));
+            nrm = fix ? nrm : fmax_nan(nrm, a0 * a0 + a1 * a1 + a2 * a2);
         }
         if (k < tl.k1) {
             rz_next = rho_pre<T, TAB>(A, L.col + zc(k + 2));
@@ -734,7 +734,7 @@ __global__ void __launch_bounds__(kShellThreads) el3_shell_velocity_kernel(El3Ar
         }
         qr[0] = qr[1]; qr[1] = qr[2];
     }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+    block_max_to(&A.vnorm[*A.d_it - 1], sqrt(nrm));
 }
 
 // ----------------------------------------------------------------- launchers
