@@ -218,7 +218,7 @@ __device__ __forceinline__ EPre<T> em3_e_pre(const Em3Args<T> &A, int64_t c, boo
 }
 
 // E of cell (i, j, k) from this step's H (own cell; x- and y-neighbours; the
-// plane below) and the preloaded operands; returns the cell's |E| for the norm.
+// plane below) and the preloaded operands; returns the cell's squared |E| for the norm.
 // Columns past nx (the row padding) take 0 like the faces.  The memory
 // variables are stored by the CPML cells (WL = false) or, with WL, by every
 // lane with mst set (0 from the lanes without CPML, whose memory variables
@@ -281,7 +281,7 @@ __device__ __forceinline__ double em3_e_new(const Em3Args<T> &A, int i, int j, i
     }
     if (c == A.src_cell || pw_face(A.pwf, i, j, k)) return 0.0;
     const double x = ex, y = ey, z = ez;
-    return sqrt(x * x + y * y + z * z);
+    return x * x + y * y + z * z;
 }
 
 // ---- half-step kernels ----------------------------------------------------
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(256) em3_h_kernel(Em3Args<T> A)
 }
 
 // Ex, Ey, Ez of cell (i, j, k), cpmlfdtd.f08:3795-3828, zero faces (and row
-// padding), in place; returns the cell's |E| for the norm.
+// padding), in place; returns the cell's squared |E| for the norm.
 template <typename T, bool TAB>
 __device__ __forceinline__ double em3_e_cell(const Em3Args<T> &A, int i, int j, int k, bool pml, bool mst)
 {
@@ -421,9 +421,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) e
     const bool pml = row && i > 1 && i < A.nx && j > 1 && j < A.ny && k > 1 && k < A.nz &&
                      (!in_box(A.box, i, j, k) || (A.any_flags && A.flags[L.t]));
     const bool mst = line_any<T>(pml);
-    double nrm = 0.0;
+    double nrm = 0.0;   // largest squared |E| (sqrt is monotone and correctly rounded)
     if (row) nrm = em3_e_cell<T, TAB>(A, i, j, k, pml, mst);
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+    block_max_to(&A.vnorm[*A.d_it - 1], sqrt(nrm));
 }
 
 // ---- fused step ---------------------------------------------------------------
@@ -457,7 +457,7 @@ __global__ void __launch_bounds__(64 * TY) em3_fused_kernel(Em3Args<T> A, int kc
     const int hi = hrow ? i : i0 - 1, hj = hrow ? j0 - 1 : j0 + tx;
     T hxm = T(0), hym = T(0), unused;
     if (valid && kb > 1) em3_h_new(A, i, j, kb - 1, false, hxm, hym, unused);
-    double nrm = 0.0;
+    double nrm = 0.0;   // largest squared |E| (sqrt is monotone and correctly rounded)
     for (int k = kb; k <= ke; ++k) {
         const int b = (k - kb) & 1;
         const int64_t c = A.xoff + (int64_t)(i - 1) + sy * (j - 1) + sz * (k - 1);
@@ -484,7 +484,7 @@ __global__ void __launch_bounds__(64 * TY) em3_fused_kernel(Em3Args<T> A, int kc
         hxm = hx;
         hym = hy;
     }
-    block_max_to(&A.vnorm[*A.d_it - 1], nrm);
+    block_max_to(&A.vnorm[*A.d_it - 1], sqrt(nrm));
 }
 
 template <typename T>
