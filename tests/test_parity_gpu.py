@@ -92,10 +92,12 @@ def test_elastic2_stress_source_and_attenuation():
 
 @pytest.mark.parametrize("options", [{}, {"zchunk": 5}, {"zchunk": 1}, {"zchunk": 7, "stagger": 0},
                                      {"material_table": 0}, {"material_table": 0, "zchunk": 3},
-                                     {"concurrent": 0}, {"concurrent": 0, "graph": 0}])
+                                     {"concurrent": 1}, {"concurrent": 1, "graph": 0}, {"graph": 0}])
 def test_elastic3_small_fp64(options):
     """z-marching chunk heights that put chunk seams inside and outside the
-    CPML slabs and the interior box."""
+    CPML slabs and the interior box; the shell kernels on a second stream
+    beside the interior kernels (concurrent: the shell then stores the column
+    next to the box's last one itself)."""
     from seidart_hip import models
     cfg = models.elastic3(nx=40, ny=36, nz=32, cpml=6, steps=200, src=(20, 18, 10), nrec=16)
     fields = ("vx", "vy", "vz", "sxx", "syy", "szz", "sxy", "sxz", "syz")
