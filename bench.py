@@ -1041,7 +1041,7 @@ def main():
     ap.add_argument("--fwi-steps", type=int, default=3000)
     ap.add_argument("--checkpoint", type=int, default=100, help="fwi2d: steps between saved forward states")
     ap.add_argument("--fd-cells", type=int, default=50, help="fwi2d: cells of the finite-difference check")
-    ap.add_argument("--near-offset-mute", type=int, default=5,
+    ap.add_argument("--near-offset-mute", type=int, default=-1,
                     help="fwi2d: receivers within this many cells of a shot's source are left out of its "
                          "misfit (-1: keep every trace)")
     ap.add_argument("--fwi-serial", type=int, default=1,

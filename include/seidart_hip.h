@@ -353,9 +353,17 @@ int32_t sdt_batch_destroy(sdt_batch *b);
 enum {
     SDT_OPT_CHECKPOINT = 8,     /* steps between saved forward states for sdt_adjoint
                                    (0 = none, default); set before the first step      */
-    SDT_OPT_RAW_SOURCE = 9      /* 1 = source series are added to the fields as given
+    SDT_OPT_RAW_SOURCE = 9,     /* 1 = source series are added to the fields as given
                                    (no dt / permittivity or density scaling); set before
                                    the first step                                         */
+    SDT_OPT_SOURCE_WINDOW = 12  /* 2D EM in FP32: 1 = the cells within 7 of the point source
+                                   (inside the CPML-free box, off the faces) are updated
+                                   again in FP64 every step with FP64 coefficients, the
+                                   source is added there in FP64 and receivers there
+                                   record the FP64 values (default: FP32 rounding of the
+                                   large near-source field otherwise dominates near-offset
+                                   traces and FWI residuals), 0 = FP32 throughout; no
+                                   effect in FP64; set before the first step              */
 };
 /* replace the model inputs by the update coefficients above (count = ncoef *
  * nx * nz, coefficient-major); needs SDT_OPT_RAW_SOURCE = 1 */

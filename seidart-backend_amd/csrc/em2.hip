@@ -12,6 +12,7 @@
 // the E update); field read-back masks the faces.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "sdt_device.h"
@@ -724,6 +725,26 @@ void em2_step(Device &d, bool timed)
 // stored as aEx, bEx (= aEz), bEz, dEx, eEx, dEz, eEz: per-cell arrays, or,
 // without an adjoint and with at most 256 distinct tuples, table rows with
 // one byte of id per cell.
+size_t src_window_bytes(const SrcWindow &w) { return 3 * (size_t)(w.ni + 1) * (w.nj + 1) * sizeof(double); }
+
+// the window's (and ring's) values of field f from an unpadded FP64 host
+// field in Fortran order (nx, nz); cells past the grid (none for E) stay
+int src_window_load(Device &d, int f, const double *host)
+{
+    const SrcWindow &w = d.tail.win;
+    if (!w.v || (f != SDT_F_EX && f != SDT_F_EZ && f != SDT_F_HY)) return SDT_OK;
+    const int m = w.ni + 1, mm = m * (w.nj + 1);
+    std::vector<double> v((size_t)mm, 0.0);
+    for (int jj = 0; jj <= w.nj; ++jj)
+        for (int ii = 0; ii <= w.ni; ++ii) {
+            const int i = w.i0 + ii, j = w.j0 + jj;
+            if (i <= d.nx && j <= d.nz) v[(size_t)(ii + m * jj)] = host[(int64_t)(i - 1) + (int64_t)d.nx * (j - 1)];
+        }
+    double *dst = w.v + (size_t)(f == SDT_F_EX ? 0 : f == SDT_F_EZ ? 1 : 2) * mm;
+    return hipMemcpy(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess ? SDT_OK
+                                                                                                   : SDT_ERR_HIP;
+}
+
 int em2_setup(Device &d, const double *const *in)
 {
     d.tile_x = 64;
@@ -795,6 +816,42 @@ int em2_setup(Device &d, const double *const *in)
     t.rec_field[0] = SDT_F_EX;
     t.rec_field[1] = -1;
     t.rec_field[2] = SDT_F_EZ;
+
+    // FP64 source window (FP32 runs): up to kSrcWinHalf cells on each side
+    // of the source, inside the CPML-free box and off the faces
+    if (d.precision == SDT_FP32 && d.src_window_opt && !face) {
+        const int ilo = std::max({is - kSrcWinHalf, d.box[0], 2}), ihi = std::min({is + kSrcWinHalf, d.box[1], nx - 1});
+        const int jlo = std::max({js - kSrcWinHalf, d.box[4], 2}), jhi = std::min({js + kSrcWinHalf, d.box[5], nz - 1});
+        if (ilo <= is && is <= ihi && jlo <= js && js <= jhi) {
+            SrcWindow &w = t.win;
+            w.i0 = ilo;
+            w.j0 = jlo;
+            w.ni = ihi - ilo + 1;
+            w.nj = jhi - jlo + 1;
+            w.pitch = d.pitch;
+            w.base = cell_index(d, ilo, 1, jlo);
+            w.rdx = 1.0 / d.dx;
+            w.rdz = 1.0 / d.dz;
+            w.dt = d.dt;
+            w.dbHy = d.dt / kMu0;
+            const size_t nv = src_window_bytes(w) / sizeof(double), nc = (size_t)w.ni * w.nj;
+            std::vector<double> hv(nv + 7 * nc, 0.0);
+            for (int jj = 0; jj < w.nj; ++jj)
+                for (int ii = 0; ii < w.ni; ++ii) {
+                    const int64_t h = (int64_t)(ilo + ii - 1) + (int64_t)nx * (jlo + jj - 1);
+                    for (int q = 0; q < 7; ++q) hv[nv + q * nc + (size_t)(ii + w.ni * jj)] = co[q][(size_t)h];
+                }
+            void *p = nullptr;
+            if (hipMalloc(&p, hv.size() * sizeof(double)) != hipSuccess) return SDT_ERR_HIP;
+            w.v = (double *)p;
+            w.co = w.v + nv;
+            if (hipMemcpy(p, hv.data(), hv.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+                return SDT_ERR_HIP;
+            // initial E in FP64 (Hy starts at zero)
+            if (in[SDT_IC_X] && (rc = src_window_load(d, SDT_F_EX, in[SDT_IC_X])) != SDT_OK) return rc;
+            if (in[SDT_IC_Z] && (rc = src_window_load(d, SDT_F_EZ, in[SDT_IC_Z])) != SDT_OK) return rc;
+        }
+    }
 
     // adjoint layout: checkpoint = fields + memory variables; 7 coefficients;
     // per step 4 forward values (Ex, Ez, fz, fx) and 2 adjoint values

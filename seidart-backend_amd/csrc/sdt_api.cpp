@@ -181,6 +181,8 @@ static void release(sdt_handle *h)
     drop(d.rec_out);
     drop(d.adj.src_lam);      // the other adjoint arrays are grid allocations
     drop(d.adj.trace_grad);
+    drop(d.adj.win_ckpt);
+    drop(d.tail.win.v);   // holds the window's coefficients too
     drop(d.snap);
     for (auto ev : h->tev) (void)hipEventDestroy(ev);
     for (int q = 0; q < 2; ++q) {
@@ -285,6 +287,10 @@ static int prepare(sdt_handle *h)
         return fail(h, SDT_ERR_ARG, (d.physics == SDT_EM2 || d.physics == SDT_EM3)
                                         ? "plane-wave source: the permittivity needs a positive smallest eigenvalue"
                                         : "plane-wave source: the model needs positive density and stiffness");
+    if (h->pw.kind && d.tail.win.v) {   // the source window serves point sources only
+        (void)hipFree(d.tail.win.v);
+        d.tail.win = SrcWindow{};
+    }
     if (d.raw_source)
         for (int q = 0; q < d.tail.nterms; ++q) {
             d.tail.terms[q].mul = 1.0;
@@ -297,6 +303,8 @@ static int prepare(sdt_handle *h)
         void *p = nullptr;
         if (alloc_grid(d, &p, bytes) != SDT_OK) return fail(h, SDT_ERR_HIP, "checkpoint allocation failed");
         a.ckpt = (char *)p;
+        if (d.tail.win.v)
+            HIPCHK(h, hipMalloc((void **)&a.win_ckpt, (size_t)a.nckpt * src_window_bytes(d.tail.win)));
     }
     // source series, norms, step counter, receivers
     const size_t ns = (size_t)SDT_NUM_SRC * d.time_steps;
@@ -388,6 +396,12 @@ static void checkpoint_copy(Device &d, int k, bool save, hipStream_t s)
         char *slot = a.ckpt + ((size_t)k * a.nstate + q) * bytes;
         if (save) (void)hipMemcpyAsync(slot, a.state[q], bytes, hipMemcpyDeviceToDevice, s);
         else (void)hipMemcpyAsync(a.state[q], slot, bytes, hipMemcpyDeviceToDevice, s);
+    }
+    if (a.win_ckpt) {   // the FP64 source window is part of the state
+        const size_t wb = src_window_bytes(d.tail.win);
+        char *slot = (char *)a.win_ckpt + (size_t)k * wb;
+        if (save) (void)hipMemcpyAsync(slot, d.tail.win.v, wb, hipMemcpyDeviceToDevice, s);
+        else (void)hipMemcpyAsync(d.tail.win.v, slot, wb, hipMemcpyDeviceToDevice, s);
     }
 }
 
@@ -522,6 +536,7 @@ int32_t sdt_set_field(sdt_handle *h, int32_t f, const double *host, int64_t coun
     HIPCHK(h, hipSetDevice(h->p.device));
     HIPCHK(h, hipStreamSynchronize(d.stream));
     if (store_grid(d, d.field[f], host, d.ncomp) != SDT_OK) return fail(h, SDT_ERR_HIP, "field upload failed");
+    if (src_window_load(d, f, host) != SDT_OK) return fail(h, SDT_ERR_HIP, "field upload failed");
     return SDT_OK;
 }
 
@@ -736,6 +751,11 @@ int32_t sdt_set_option(sdt_handle *h, int32_t key, int64_t value)
     case SDT_OPT_RAW_SOURCE:
         if (h->prepared) return fail(h, SDT_ERR_STATE, "source scaling is fixed once the first step has run");
         h->d.raw_source = value != 0;
+        return SDT_OK;
+    case SDT_OPT_SOURCE_WINDOW:
+        if (h->prepared) return fail(h, SDT_ERR_STATE, "the source window is fixed once the first step has run");
+        if (value != 0 && value != 1) return fail(h, SDT_ERR_ARG, "source window option must be 0 or 1");
+        h->d.src_window_opt = value != 0;
         return SDT_OK;
     }
     return fail(h, SDT_ERR_ARG, "unknown option");

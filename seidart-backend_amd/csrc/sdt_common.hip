@@ -23,6 +23,81 @@ struct FieldPtrs {
     void *f[SDT_NUM_FIELDS];
 };
 
+// Index of cell c in the source window's value arrays, -1 outside the window.
+__device__ __forceinline__ int win_index(const SrcWindow &w, int64_t c)
+{
+    if (!w.v || c < w.base) return -1;
+    const int64_t o = c - w.base, jj = o / w.pitch, ii = o - jj * w.pitch;
+    return ii < w.ni && jj < w.nj ? (int)(ii + (int64_t)(w.ni + 1) * jj) : -1;
+}
+
+__device__ __forceinline__ const double *win_values(const SrcWindow &w, int f)
+{
+    const int m = (w.ni + 1) * (w.nj + 1);
+    return f == SDT_F_EX ? w.v : f == SDT_F_EZ ? w.v + m : f == SDT_F_HY ? w.v + 2 * m : nullptr;
+}
+
+// The FP64 source window of a 2D EM shot (whole workgroup, after the step's
+// FP32 H and E kernels): the Hy update of cpmlfdtd.f08:2576-2593 and the Ex,
+// Ez update of 2605-2620 for the window's cells, without CPML terms (the
+// window lies in the CPML-free box).  The H update reads the window's FP64 E
+// and, past its high edges, the ring (the FP32 E the previous step left
+// there); the E update reads the window's new FP64 Hy and, below its low
+// edges, the FP32 Hy the H kernel just wrote.  The point-source terms on a
+// window cell are added in FP64 (2678-2681); the rounded values overwrite
+// the FP32 fields, and the ring is refreshed from the FP32 E of this step.
+template <typename T>
+__device__ void src_window_step(const FieldPtrs &fp, const TailArgs &a, int it)
+{
+    const SrcWindow &w = a.win;
+    const int ni = w.ni, nj = w.nj, m = ni + 1, t = threadIdx.x;
+    double *vx = w.v, *vz = w.v + m * (nj + 1), *vh = w.v + 2 * m * (nj + 1);
+    T *Ex = fptr<T>(fp.f, SDT_F_EX), *Ez = fptr<T>(fp.f, SDT_F_EZ), *Hy = fptr<T>(fp.f, SDT_F_HY);
+    const bool on = t < ni * nj;
+    const int ii = on ? t % ni : 0, jj = on ? t / ni : 0, q = ii + m * jj;
+    const int64_t c = w.base + ii + (int64_t)w.pitch * jj;
+    double h = 0.0;
+    if (on) {
+        const double dEx_dz = (vx[q + m] - vx[q]) * w.rdz;
+        const double dEz_dx = (vz[q + 1] - vz[q]) * w.rdx;
+        h = vh[q] + w.dbHy * (dEz_dx - dEx_dz);
+    }
+    __syncthreads();
+    if (on) vh[q] = h;
+    __syncthreads();
+    if (on) {
+        const double hd = jj > 0 ? vh[q - m] : (double)Hy[c - w.pitch];
+        const double hl = ii > 0 ? vh[q - 1] : (double)Hy[c - 1];
+        const double exo = vx[q], ezo = vz[q];
+        const int nc = ni * nj;
+        const double aEx = w.co[t], bEx = w.co[nc + t], bEz = w.co[2 * nc + t], dEx = w.co[3 * nc + t],
+                     eEx = w.co[4 * nc + t], dEz = w.co[5 * nc + t], eEz = w.co[6 * nc + t];
+        const double dHy_dz = (h - hd) * w.rdz;
+        const double dHy_dx = (h - hl) * w.rdx;
+        double ez = ezo + (-bEx * dHy_dz + bEz * dHy_dx - dEz * exo - eEz * ezo) * w.dt;
+        double ex = exo + (-aEx * dHy_dz + bEx * dHy_dx - dEx * exo - eEx * ezo) * w.dt;
+        for (int k = 0; k < a.nterms; ++k) {
+            const SrcTerm &s = a.terms[k];
+            if (s.cell != c || s.on_face) continue;
+            const double v = (a.series[(int64_t)s.component * a.time_steps + (it - 1)] * s.mul) / s.div;
+            if (s.field == SDT_F_EX) ex += v;
+            else if (s.field == SDT_F_EZ) ez += v;
+        }
+        vx[q] = ex;
+        vz[q] = ez;
+        Ex[c] = (T)ex;
+        Ez[c] = (T)ez;
+        Hy[c] = (T)h;
+    }
+    if (t < ni + nj) {   // ring column (t < nj), ring row
+        const int ri = t < nj ? ni : t - nj, rj = t < nj ? t : nj;
+        const int64_t rc = w.base + ri + (int64_t)w.pitch * rj;
+        vx[ri + m * rj] = (double)Ex[rc];
+        vz[ri + m * rj] = (double)Ez[rc];
+    }
+    __syncthreads();
+}
+
 // One workgroup.  The point sources are injected in parallel, one thread per
 // term: every term of a solver adds to a different (field, cell) pair, so the
 // reference's sequential order (e.g. cpmlfdtd.f08:519-523, 2208-2219,
@@ -34,9 +109,10 @@ __device__ __forceinline__ void tail_body(const FieldPtrs &fp, const TailArgs &a
 {
     const int it = *a.d_it;
     const int t = threadIdx.x;
+    if (a.win.v) src_window_step<T>(fp, a, it);   // uniform per workgroup
     if (t < a.nterms) {
         const SrcTerm &s = a.terms[t];
-        if (s.cell >= 0 && !s.on_face) {
+        if (s.cell >= 0 && !s.on_face && win_index(a.win, s.cell) < 0) {
             const double v = a.series[(int64_t)s.component * a.time_steps + (it - 1)];
             T *f = reinterpret_cast<T *>(fp.f[s.field]);
             const int64_t e = s.cell << a.cplx;   // complex: the real part
@@ -72,11 +148,12 @@ __device__ __forceinline__ void tail_body(const FieldPtrs &fp, const TailArgs &a
         const int64_t c = a.rec_cell[q];
         const int nv = 1 << a.cplx;
         double *o = a.rec_out + ((int64_t)(it - 1) * a.nrec + q) * 3 * nv;
+        const int wq = win_index(a.win, c);   // inside the source window: its FP64 values
         for (int m = 0; m < 3; ++m)
             for (int p = 0; p < nv; ++p)
-                o[m * nv + p] = a.rec_field[m] >= 0
-                                    ? (double)reinterpret_cast<T *>(fp.f[a.rec_field[m]])[(c << a.cplx) + p]
-                                    : 0.0;
+                o[m * nv + p] = a.rec_field[m] < 0 ? 0.0
+                                : wq >= 0          ? win_values(a.win, a.rec_field[m])[wq]
+                                                   : (double)reinterpret_cast<T *>(fp.f[a.rec_field[m]])[(c << a.cplx) + p];
     }
     __syncthreads();
     if (t == 0) *a.d_it = it + 1;

@@ -32,6 +32,25 @@ struct SrcTerm {
 // Cells whose norm contribution the tail kernel recomputes after injection.
 constexpr int kMaxNormFix = 9;
 
+// FP64 window around the point source of a 2D EM shot run in FP32: its
+// cells' H and E updates are done again in FP64 every step (FP64 values,
+// FP64 coefficients), the source is added there in FP64, the FP32 fields
+// receive the rounded values, and receivers inside record the FP64 values.
+// Near the source the fields are large and FP32 rounding of them swamps the
+// small differences between two models that an FWI residual is made of.
+// The window lies in the CPML-free box, off the Dirichlet faces.
+constexpr int kSrcWinHalf = 7;                     // cells on each side of the source
+constexpr int kSrcWinMax = 2 * kSrcWinHalf + 1;    // window side at most (one thread per cell)
+struct SrcWindow {
+    double *v;           // null: no window; else [3][(ni + 1) * (nj + 1)] Ex, Ez, Hy of the
+                         // window, its ring column i0 + ni and ring row j0 + nj (E only)
+    const double *co;    // [7][ni * nj] update coefficients aEx bEx bEz dEx eEx dEz eEz
+    int i0, j0, ni, nj;  // first cell (1-based) and size
+    int pitch;
+    int64_t base;        // padded cell index of (i0, j0)
+    double rdx, rdz, dt, dbHy;
+};
+
 struct TailArgs {
     int *d_it;                       // device step counter (1-based step being run)
     const double *series;            // [SDT_NUM_SRC][time_steps]
@@ -52,6 +71,7 @@ struct TailArgs {
     double *rec_out;                 // [time_steps][nrec][3]
     int rec_field[3];                // field ids recorded (-1 -> 0.0)
     int cplx;                        // 1: complex fields (interleaved re, im)
+    SrcWindow win;                   // FP64 source window (win.v null: none)
 };
 
 // Adjoint-state gradients (2D EM, 2D elastic).  The forward run saves its
@@ -79,6 +99,7 @@ struct Adjoint {
     double *grad = nullptr;            // [ngrad][ncell_dev]
     double *src_lam = nullptr;         // [time_steps][kAdjSrcSlots] adjoint values at the source cells
     double *trace_grad = nullptr;      // [time_steps][nrec][3]
+    double *win_ckpt = nullptr;        // [nckpt] copies of the source window's values
     bool store = false;                // forward kernels write seg_fwd
     int store_first = 0;               // 0-based first step of the recomputed segment
 };
@@ -230,6 +251,7 @@ struct Device {
 
     const double *coef_in = nullptr;   // update coefficients given directly (during setup only)
     bool raw_source = false;           // SDT_OPT_RAW_SOURCE: series injected as given
+    bool src_window_opt = true;        // SDT_OPT_SOURCE_WINDOW (2D EM in FP32)
     Adjoint adj;
     PlaneWave pw;                      // plane-wave source (elastic solvers)
 };
@@ -372,6 +394,10 @@ int upload_cpml_dict(Device &d, const double *const *host_in);
 // choose pitch / xoff (call after compute_tile_flags): column `first` starts
 // a 128-byte line; 0 = the interior box's first column (d.box)
 void set_layout(Device &d, int first = 0);
+// source window values (bytes of win.v) and their refresh from an unpadded
+// FP64 host field (Fortran order) after the field is set
+size_t src_window_bytes(const SrcWindow &w);
+int src_window_load(Device &d, int f, const double *host);
 inline int64_t cell_index(const Device &d, int i, int j, int k)
 {
     return d.xoff + (int64_t)(i - 1) + (int64_t)d.pitch * ((int64_t)(j - 1) + (int64_t)d.ny * (k - 1));

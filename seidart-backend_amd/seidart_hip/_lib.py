@@ -71,7 +71,7 @@ EXPORTS = [
 
 OPT_ZCHUNK, OPT_GRAPH, OPT_STAGGER, OPT_TILE_ROWS, OPT_MATERIAL_TABLE, OPT_CONCURRENT, OPT_XCD_REMAP = (
     1, 2, 3, 4, 5, 6, 7)
-OPT_CHECKPOINT, OPT_RAW_SOURCE, OPT_ISOTROPIC, OPT_FUSED = 8, 9, 10, 11
+OPT_CHECKPOINT, OPT_RAW_SOURCE, OPT_ISOTROPIC, OPT_FUSED, OPT_SOURCE_WINDOW = 8, 9, 10, 11, 12
 (INFO_MATERIALS, INFO_SHELL_TILES, INFO_DEVICE_BYTES, INFO_COEFFICIENTS, INFO_ISOTROPIC, INFO_FUSED,
  INFO_CPML_TUPLES) = 1, 2, 3, 4, 5, 6, 7
 NUM_SRC = 9

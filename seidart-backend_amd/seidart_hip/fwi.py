@@ -204,20 +204,17 @@ def l2_misfit(pred, obs, weight=None):
 
 def near_offset_weights(shots, radius=5):
     """Per-shot receiver weights of a near-offset mute: 0 for receivers within
-    `radius` cells of the shot's source cell (in x and in z), 1 otherwise.
+    `radius` cells of the shot's source cell (in x and in z), 1 otherwise, a
+    standard FWI data mute (the `weight` of l2_misfit).
 
-    A receiver on or next to the source cell records the injected source
-    signature itself; its predicted and observed traces share that large
-    pulse, so their difference -- the adjoint source -- is a small
-    difference of large numbers, and with FP32 wavefields it carries the
-    rounding of the pulse (relative 1e-7 of it), which the adjoint then
-    injects back at the source, where the zero-lag correlations of the
-    gradient cancel to a small sum.  Muting those traces (standard in FWI)
-    removes the term: on a 300 x 150, 1500-step EM problem with a receiver
-    on a source cell the FP32 and FP64 permittivity / conductivity gradients
-    differ by up to 5e-3 without a mute, 1.2e-4 with a 2-cell one and 6e-6
-    with the 5-cell default (0.25 m at 5 cm cells, a fifth of the 100 MHz
-    wavelength)."""
+    FP32 accuracy does not need it: a receiver on or next to the source
+    records the injected pulse itself, and its residual is a small
+    difference of large traces; 2D EM shots run in FP32 compute a window
+    around the source in FP64 (solver option 'source_window', default on),
+    so those traces carry no FP32 rounding of the pulse.  On a 300 x 150,
+    1500-step EM problem with a receiver on a source cell the FP32 and FP64
+    gradients agree to 5e-6 with or without this mute (5e-3 without the
+    window and without a mute)."""
     out = []
     for sh in shots:
         rec = np.asarray(sh["receivers"]).reshape(-1, 3)

@@ -277,12 +277,14 @@ class Solver:
         (3D elastic without a material table: 1 = hold only c11, c12, c44 when
         every cell's stiffness has the isotropic form, 0 = all 21 arrays) or
         'fused' (3D EM: 1 = one kernel per step, 0 or -1 = one per half step,
-        the default)."""
+        the default) or 'source_window' (2D EM in FP32: 1 = the cells within
+        7 of the point source also updated in FP64, receivers there record
+        the FP64 values, the default; 0 = FP32 throughout)."""
         k = {"zchunk": L.OPT_ZCHUNK, "graph": L.OPT_GRAPH, "stagger": L.OPT_STAGGER,
              "tile_rows": L.OPT_TILE_ROWS, "material_table": L.OPT_MATERIAL_TABLE,
              "concurrent": L.OPT_CONCURRENT, "xcd_remap": L.OPT_XCD_REMAP,
              "checkpoint": L.OPT_CHECKPOINT, "raw_source": L.OPT_RAW_SOURCE,
-             "isotropic": L.OPT_ISOTROPIC, "fused": L.OPT_FUSED}[key]
+             "isotropic": L.OPT_ISOTROPIC, "fused": L.OPT_FUSED, "source_window": L.OPT_SOURCE_WINDOW}[key]
         L.check(L.lib.sdt_set_option(self._h, k, int(value)), self._h)
 
     def info(self, key):

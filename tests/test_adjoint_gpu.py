@@ -313,12 +313,13 @@ def _em2_fwi_case(nshots=3, steps=1500):
 
 def test_em2_fp32_batch_gradient_within_1e4_of_fp64():
     """Batched FP32 forward + adjoint against serial FP64 forward + adjoint,
-    L2 misfit with the 5-cell near-offset mute: every permittivity and
-    conductivity gradient within 1e-4 relative L2 (observed <= 6e-6).
-    Without the mute the receiver on shot 0's source cell adds the FP32
-    rounding of the recorded source pulse to that shot's adjoint source
-    (observed up to 5e-3; a 2-cell mute leaves 1e-4 from the receivers a few
-    cells away; both printed for reference)."""
+    L2 misfit over every trace (no mute), a receiver on shot 0's source cell:
+    every permittivity and conductivity gradient within 1e-4 relative L2
+    (observed <= 5.1e-6), and likewise with 2- and 5-cell near-offset mutes.
+    The FP64 window around each FP32 shot's source (SDT_OPT_SOURCE_WINDOW)
+    keeps that receiver's trace free of FP32 rounding of the source pulse,
+    which otherwise enters the adjoint source (up to 5e-3 here without the
+    window and without a mute)."""
     from seidart_hip import fwi
     prob, src, start, obs = _em2_fwi_case()
     shots = prob["shots"]
@@ -342,4 +343,4 @@ def test_em2_fp32_batch_gradient_within_1e4_of_fp64():
             g[prec] = np.stack([x.grad.numpy() for x in p])
         out[mute] = [rel_l2(g["fp32"][k], g["fp64"][k]) for k in range(6)]
         print(f"near-offset mute {mute}: fp32 vs fp64 gradient rel L2", [f"{e:.2e}" for e in out[mute]], flush=True)
-    assert max(out[5]) <= 1e-4
+    assert max(max(e) for e in out.values()) <= 1e-4
