@@ -748,7 +748,8 @@ def bench_fwi2d(args):
         "dtype": "f32", "data": "synthetic (seeded 6-layer anisotropic model + 10 Gaussian anomalies)",
         "config": {"workload": f"fwi2d: 600x300 + 10-cell CPML, {args.fwi_steps} steps, {args.fwi_shots} shots "
                                f"per solver kind, checkpoints every {args.checkpoint} steps, L2 trace misfit "
-                               f"with a {args.near_offset_mute}-cell near-offset mute", "kinds": list(out)},
+                               + (f"with a {args.near_offset_mute}-cell near-offset mute" if args.near_offset_mute >= 0
+                                  else "over every trace (no near-offset mute)"), "kinds": list(out)},
         "roofline": None, "cpu_baseline": None, "fwi": out,
     }
 
