@@ -63,6 +63,34 @@ def test_isotropic_option_range_and_state():
         s.close()
 
 
+def test_source_window_option_range_and_state():
+    """SDT_OPT_SOURCE_WINDOW takes 0 or 1 before the first step and is fixed
+    after it; a 2D EM FP32 run with a plane-wave source (no point source, so
+    no window) still runs."""
+    from seidart_hip import models
+    from seidart_hip._lib import SdtError
+    from seidart_hip.solver import make_solver
+    print("[api errors] source window option", flush=True)
+    cfg = models.em2(nx=30, nz=24, cpml=4, steps=4, src=(15, 10), nrec=2)
+    s = make_solver(cfg, precision="fp32")
+    try:
+        with pytest.raises(SdtError, match="0 or 1"):
+            s.set_option("source_window", 2)
+        s.set_option("source_window", 0)
+        s.run(1)
+        with pytest.raises(SdtError, match="fixed"):
+            s.set_option("source_window", 1)
+    finally:
+        s.close()
+    s = make_solver(cfg, precision="fp32")
+    try:
+        s.set_plane_wave("s", (0.0, 0.0, 0.0), 1e8, 1.0)
+        s.run()
+        assert np.all(np.isfinite(s.receivers()))
+    finally:
+        s.close()
+
+
 def test_adjoint_takes_point_sources():
     from seidart_hip._lib import SdtError
     print("[api errors] adjoint with a plane wave", flush=True)
